@@ -1,0 +1,307 @@
+/*
+ * rive_hip.h -- C ABI of the MI355X (gfx950) backend for Rive's GPU renderer.
+ *
+ * The backend plugs in where the reference's per-API backends do: behind
+ * rive::gpu::RenderContextImpl (renderer/include/rive/renderer/
+ * render_context_impl.hpp:26-243), which rive::gpu::RenderContext owns and
+ * drives once per frame (render_context.cpp:770-1044).  Every entry point
+ * below names the RenderContextImpl virtual it replaces.  Nothing in these
+ * signatures is a C++ or torch type: plain pointers, sizes and PODs only.
+ *
+ * Two flush modes share one device pipeline:
+ *   RIVE_HIP_FLUSH_SPANS     the host already wrote TessVertexSpan/ContourData
+ *                            records, exactly as the reference RenderContext
+ *                            does on the CPU (render_context.cpp:1424-2408).
+ *   RIVE_HIP_FLUSH_GEOMETRY  the host wrote raw path geometry plus one
+ *                            rive_hip_draw per PathDraw; the device counts
+ *                            curves (Wang's formula, polar segments, joins,
+ *                            caps), scans offsets and writes the same
+ *                            TessVertexSpan/ContourData records itself.
+ * Both then tessellate, bin, rasterise and resolve to RGBA8 on the device.
+ *
+ * Status: every function returns RIVE_HIP_OK (0) or a positive error code;
+ * rive_hip_last_error() gives a message.  No function throws.
+ */
+#ifndef RIVE_HIP_H
+#define RIVE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum rive_hip_status
+{
+    RIVE_HIP_OK = 0,
+    RIVE_HIP_ERROR_INVALID_ARGUMENT = 1,
+    RIVE_HIP_ERROR_DEVICE = 2,
+    RIVE_HIP_ERROR_OUT_OF_MEMORY = 3,
+    RIVE_HIP_ERROR_UNSUPPORTED = 4,
+    RIVE_HIP_ERROR_NOT_MAPPED = 5,
+};
+
+/* ------------------------------------------------------------------------ */
+/* Record formats (byte-compatible with the reference's GPU records).       */
+/* ------------------------------------------------------------------------ */
+
+/* gpu::TessVertexSpan, 64 bytes (gpu.hpp:291-375). */
+typedef struct rive_hip_tess_span
+{
+    float pts[8];
+    float join_tangent[2];
+    float y;
+    float reflection_y;
+    int32_t x0x1;
+    int32_t reflection_x0x1;
+    uint32_t segment_counts; /* join<<20 | polar<<10 | parametric */
+    uint32_t contour_id_with_flags;
+} rive_hip_tess_span;
+
+/* gpu::ContourData, 16 bytes (gpu.hpp:1698-1714). */
+typedef struct rive_hip_contour
+{
+    float midpoint[2];
+    uint32_t path_id;
+    uint32_t vertex_index0;
+} rive_hip_contour;
+
+/* gpu::PathData, 64 bytes (gpu.hpp, PathData::set at gpu.cpp:859-877). */
+typedef struct rive_hip_path_data
+{
+    float matrix[6];
+    float stroke_radius; /* 0 => fill */
+    float feather_radius;
+    uint32_t z_index;
+    float atlas_transform[3];
+    uint32_t coverage_range[4];
+} rive_hip_path_data;
+
+/* gpu::PaintData, 8 bytes (PaintData::set at gpu.cpp:879-939). */
+typedef struct rive_hip_paint_data
+{
+    uint32_t params; /* clipID<<16 | blend<<4 | flags | paintType */
+    uint32_t value;  /* RGBA color, gradient row (float bits) or clip ID<<16 */
+} rive_hip_paint_data;
+
+/* gpu::PaintAuxData, 128 bytes (PaintAuxData::set at gpu.cpp:941-1063). */
+typedef struct rive_hip_paint_aux
+{
+    float paint_matrix[6];
+    float grad_span[2]; /* horizontal span, normalized */
+    float clip_rect_inverse_matrix[6];
+    float inverse_fwidth[2];
+    float image_matrix[6];
+    float image_lod;
+    float pad[9];
+} rive_hip_paint_aux;
+
+/* gpu::GradientSpan, 16 bytes (gpu.hpp:248-270). */
+typedef struct rive_hip_grad_span
+{
+    uint32_t horizontal_span; /* x1Fixed<<16 | x0Fixed */
+    uint32_t y_with_flags;
+    uint32_t color0; /* ColorInt ARGB */
+    uint32_t color1;
+} rive_hip_grad_span;
+
+/* One PathDraw for device-side counting (RIVE_HIP_FLUSH_GEOMETRY).  The host
+ * computes what PathDraw::PathDraw/initForMidpointFan compute once per draw
+ * (draw.cpp:561-825): the matrix max scale, polar segments per radian and the
+ * contour directions.  The device does the per-curve work. */
+typedef struct rive_hip_draw
+{
+    uint32_t path_id;          /* 1-based PathData/PaintData index */
+    uint32_t first_verb;       /* into the VERB buffer */
+    uint32_t verb_count;
+    uint32_t first_point;      /* into the POINT buffer (float pairs) */
+    uint32_t point_count;
+    uint32_t contour_directions; /* 0 fwd, 1 rev, 2 fwdThenRev, 3 revThenFwd */
+    uint32_t contour_flags;    /* e.g. NEGATE_PATH_FILL_COVERAGE_FLAG */
+    uint32_t join;             /* 0 miter, 1 round, 2 bevel */
+    uint32_t cap;              /* 0 butt, 1 round, 2 square */
+    uint32_t is_stroke;
+    float matrix[6];
+    float stroke_radius;
+    float matrix_max_scale;
+    float polar_segments_per_radian;
+    float parametric_precision;
+    uint32_t pad[6];
+} rive_hip_draw;
+
+/* ------------------------------------------------------------------------ */
+/* Backend context.                                                         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct rive_hip_context rive_hip_context;
+typedef struct rive_hip_target rive_hip_target;
+
+/* Message for the last error on this thread (never NULL). */
+const char* rive_hip_last_error(void);
+
+/* Creates a backend on HIP device `device` with its own stream.
+ * Replaces: the backend's MakeContext factory, e.g.
+ * RenderContextVulkanImpl::MakeContext (render_context_vulkan_impl.cpp:4259). */
+int rive_hip_create(int device, rive_hip_context** out);
+void rive_hip_destroy(rive_hip_context* ctx);
+
+/* Buffer kinds.  The first nine mirror RenderContextImpl::resize*Buffer
+ * (render_context_impl.hpp:108-120); VERB/POINT/DRAW carry the
+ * RIVE_HIP_FLUSH_GEOMETRY input. */
+enum rive_hip_buffer_kind
+{
+    RIVE_HIP_BUFFER_FLUSH_UNIFORM = 0,
+    RIVE_HIP_BUFFER_PATH = 1,
+    RIVE_HIP_BUFFER_PAINT = 2,
+    RIVE_HIP_BUFFER_PAINT_AUX = 3,
+    RIVE_HIP_BUFFER_CONTOUR = 4,
+    RIVE_HIP_BUFFER_GRAD_SPAN = 5,
+    RIVE_HIP_BUFFER_TESS_SPAN = 6,
+    RIVE_HIP_BUFFER_TRIANGLE_VERTEX = 7,
+    RIVE_HIP_BUFFER_IMAGE_DRAW = 8,
+    RIVE_HIP_BUFFER_VERB = 9,
+    RIVE_HIP_BUFFER_POINT = 10,
+    RIVE_HIP_BUFFER_DRAW = 11,
+    RIVE_HIP_BUFFER_KIND_COUNT = 12
+};
+
+/* Replaces RenderContextImpl::resize{FlushUniform,Path,Paint,PaintAux,Contour,
+ * GradSpan,TessVertexSpan,TriangleVertex,ImageDrawInstance}Buffer
+ * (render_context_impl.hpp:108-120).  Allocates exactly `bytes` of device
+ * memory plus pinned host staging. */
+int rive_hip_buffer_resize(rive_hip_context* ctx, int kind, size_t bytes);
+
+/* Replaces RenderContextImpl::map*Buffer (render_context_impl.hpp:155-165).
+ * Returns a write-only pinned host pointer, or NULL on failure (the caller
+ * then drops the frame, render_context.cpp:1012-1016). */
+void* rive_hip_buffer_map(rive_hip_context* ctx, int kind, size_t bytes);
+
+/* Replaces RenderContextImpl::unmap*Buffer (render_context_impl.hpp:166-174).
+ * Enqueues the host-to-device copy of `bytes` on the context stream. */
+int rive_hip_buffer_unmap(rive_hip_context* ctx, int kind, size_t bytes);
+
+/* Replaces RenderContextImpl::makeRenderTarget (render_target.hpp:15-39 /
+ * the backend's RenderTarget factory): an RGBA8 image in HBM. */
+int rive_hip_target_create(rive_hip_context* ctx, uint32_t width, uint32_t height,
+                           rive_hip_target** out);
+void rive_hip_target_destroy(rive_hip_target* target);
+
+enum rive_hip_flush_mode
+{
+    RIVE_HIP_FLUSH_SPANS = 0,
+    RIVE_HIP_FLUSH_GEOMETRY = 1
+};
+
+/* POD mirror of the gpu::FlushDescriptor fields the HIP pipeline reads
+ * (gpu.hpp:1320-1438). */
+typedef struct rive_hip_flush_desc
+{
+    rive_hip_target* target;
+    uint32_t mode;             /* rive_hip_flush_mode */
+    uint32_t load_action;      /* 0 clear, 1 preserve */
+    uint32_t clear_color;      /* ColorInt ARGB, unpremultiplied */
+    uint32_t dither;           /* 0 none, 1 interleaved gradient noise */
+    uint32_t path_count;       /* PathData records 1..path_count are live */
+    uint32_t contour_count;    /* SPANS mode: ContourData records */
+    uint32_t tess_span_count;  /* SPANS mode: TessVertexSpan records */
+    uint32_t tess_vertex_count;/* SPANS mode: texels used (incl. padding) */
+    uint32_t draw_count;       /* GEOMETRY mode: rive_hip_draw records */
+    uint32_t grad_span_count;
+    uint32_t grad_simple_rows;   /* rows of simple (2-texel) ramps */
+    uint32_t grad_complex_rows;  /* rows of complex ramps, after the simple rows */
+    uint32_t first_path_tess_location; /* GEOMETRY mode: usually 8 */
+} rive_hip_flush_desc;
+
+/* Replaces RenderContextImpl::flush (render_context_impl.hpp:217): ramps,
+ * (counting), tessellation, binning, rasterisation and RGBA8 resolve, all
+ * enqueued on the context stream. */
+int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* desc);
+
+/* Waits for the context stream (the reference's postFlush/fence wait). */
+int rive_hip_sync(rive_hip_context* ctx);
+
+/* Copies the target's RGBA8 pixels (row-major, premultiplied) to host. */
+int rive_hip_read_pixels(rive_hip_context* ctx, rive_hip_target* target, uint8_t* rgba);
+
+/* Debug/parity readback of device intermediates from the last flush:
+ *  0 TessVertexSpans, 1 ContourData, 2 tessellation texture (16 B/texel),
+ *  3 per-draw tessellation vertex counts (u32), 4 gradient ramp texture
+ *  (RGBA8, 512 wide).  Writes min(capacity, size) bytes; *size gets the
+ *  full size. */
+int rive_hip_debug_readback(rive_hip_context* ctx, int what, void* dst,
+                            size_t capacity, size_t* size);
+
+/* Per-pass device timings of the last flush in milliseconds (hipEvents on the
+ * context stream): [0] ramps, [1] counting+spans, [2] tessellation,
+ * [3] binning, [4] raster+resolve, [5] total.  n <= 6. */
+int rive_hip_pass_times(rive_hip_context* ctx, float* ms, int n);
+
+/* ------------------------------------------------------------------------ */
+/* Draw streams: a recorded sequence of rive::Renderer calls.               */
+/* ------------------------------------------------------------------------ */
+
+/* For draw i: save(); transform(matrix); clipPath() for each clip path;
+ * drawPath(path, paint); restore() -- rive::Renderer (renderer.hpp:209-236). */
+typedef struct rive_stream_path
+{
+    uint32_t first_verb, verb_count;   /* rive::PathVerb values (move=0, line=1,
+                                          quad=2, cubic=4, close=5) */
+    uint32_t first_point, point_count; /* float pairs */
+    uint32_t fill_rule;                /* 0 nonZero, 1 evenOdd, 2 clockwise */
+} rive_stream_path;
+
+typedef struct rive_stream_draw
+{
+    uint32_t path;          /* index into paths */
+    float matrix[6];
+    uint32_t style;         /* 0 fill, 1 stroke */
+    float thickness;
+    uint32_t join, cap;
+    float feather;
+    uint32_t blend_mode;    /* rive::BlendMode value (srcOver = 3) */
+    uint32_t paint_type;    /* 0 solid, 1 linear gradient, 2 radial gradient */
+    uint32_t color;         /* ColorInt ARGB for solid paints */
+    uint32_t first_stop, stop_count; /* gradients: into stop arrays */
+    float grad[4];          /* linear: sx, sy, ex, ey; radial: cx, cy, r, 0 */
+    uint32_t clip_count;    /* 0..2 */
+    uint32_t clip_paths[2];
+} rive_stream_draw;
+
+typedef struct rive_stream
+{
+    uint32_t width, height;
+    uint32_t clear_color;   /* ColorInt ARGB */
+    uint32_t dither;        /* 0 none, 1 interleaved gradient noise */
+    const uint8_t* verbs;
+    uint32_t verb_count;
+    const float* points;    /* x,y pairs */
+    uint32_t point_count;
+    const rive_stream_path* paths;
+    uint32_t path_count;
+    const rive_stream_draw* draws;
+    uint32_t draw_count;
+    const uint32_t* stop_colors;
+    const float* stop_offsets;
+    uint32_t stop_count;
+} rive_stream;
+
+/* Seeded synthetic streams (std::mt19937_64 with the float/int mappings of
+ * tests/common/rand.hpp:13-54).  config: 1 = 1024x1024, 200 closed cubic
+ * fills; 2 = 1920x1080, 5000 paths of fills/strokes/gradients/clips.
+ * path_scale < 1 keeps only that fraction of the draws (parity-test sizes).
+ * The stream owns its arrays until rive_stream_free. */
+int rive_stream_generate(int config, uint64_t seed, uint32_t width, uint32_t height,
+                         uint32_t draw_count, rive_stream** out);
+void rive_stream_free(rive_stream* stream);
+
+/* Renders a stream through the C++ front end (RiveRenderer -> RenderContext
+ * -> RenderContextHIPImpl) into `target`, GEOMETRY mode. */
+int rive_hip_render_stream(rive_hip_context* ctx, const rive_stream* stream,
+                           rive_hip_target* target);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RIVE_HIP_H */

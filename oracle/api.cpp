@@ -1,0 +1,172 @@
+// CPU ORACLE (test infrastructure only): extern "C" entry points used by the
+// tests and by bench.py's cpu_baseline leg.  Never linked into the product.
+#include "oracle.hpp"
+#include "oracle_flush.hpp"
+
+#include <chrono>
+#include <cstring>
+
+using namespace oracle;
+
+struct oracle_flush
+{
+    Flush f;
+};
+
+namespace
+{
+template <typename T> size_t copy_out(const std::vector<T>& v, void* dst, size_t cap)
+{
+    size_t bytes = v.size() * sizeof(T);
+    if (dst != nullptr)
+        memcpy(dst, v.data(), bytes < cap ? bytes : cap);
+    return bytes;
+}
+} // namespace
+
+extern "C" {
+
+oracle_flush* oracle_build(const rive_stream* s)
+{
+    auto* of = new oracle_flush;
+    build_flush(*s, &of->f);
+    return of;
+}
+
+void oracle_free(oracle_flush* of) { delete of; }
+
+// what: 0 spans, 1 contours, 2 PathData, 3 PaintData, 4 PaintAux, 5 grad spans,
+// 6 records (CubicRecord), 7 draws (DrawInfo as 5 u32), 8 info (u32 x 10)
+size_t oracle_get(oracle_flush* of, int what, void* dst, size_t cap)
+{
+    const Flush& f = of->f;
+    switch (what)
+    {
+        case 0: return copy_out(f.spans, dst, cap);
+        case 1: return copy_out(f.contours, dst, cap);
+        case 2: return copy_out(f.pathData, dst, cap);
+        case 3: return copy_out(f.paintData, dst, cap);
+        case 4: return copy_out(f.paintAux, dst, cap);
+        case 5: return copy_out(f.gradSpans, dst, cap);
+        case 6: return copy_out(f.records, dst, cap);
+        case 7:
+        {
+            std::vector<uint32_t> v;
+            for (const DrawInfo& d : f.draws)
+            {
+                v.push_back(d.pathID);
+                v.push_back(d.tessLocation);
+                v.push_back(d.tessVertexCount);
+                v.push_back((uint32_t)d.dirs);
+                v.push_back(d.isStroke ? 1u : 0u);
+            }
+            return copy_out(v, dst, cap);
+        }
+        case 8:
+        {
+            std::vector<uint32_t> v = {f.pathCount,
+                                       (uint32_t)f.contours.size(),
+                                       (uint32_t)f.spans.size(),
+                                       f.tessVertexCount,
+                                       f.gradSimpleRows,
+                                       f.gradComplexRows,
+                                       f.gradHeight,
+                                       (uint32_t)f.draws.size(),
+                                       f.width,
+                                       f.height};
+            return copy_out(v, dst, cap);
+        }
+    }
+    return 0;
+}
+
+size_t oracle_tessellate(oracle_flush* of, void* dst, size_t cap)
+{
+    std::vector<TessVertex> tex;
+    tessellate_flush(of->f, &tex);
+    return copy_out(tex, dst, cap);
+}
+
+size_t oracle_ramps(oracle_flush* of, void* dst, size_t cap)
+{
+    std::vector<uint32_t> tex;
+    render_ramps(of->f, &tex);
+    return copy_out(tex, dst, cap);
+}
+
+// Renders the flush into rgba (width*height*4 bytes).  Returns seconds spent.
+double oracle_render(oracle_flush* of, uint8_t* rgba)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<TessVertex> tex;
+    tessellate_flush(of->f, &tex);
+    render_flush(of->f, tex, reinterpret_cast<uint32_t*>(rgba));
+    auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+// ---- maths entry points for golden-vector tests ----
+static void load_pts(const float* p, V2 out[4])
+{
+    for (int i = 0; i < 4; ++i)
+        out[i] = V2{p[2 * i], p[2 * i + 1]};
+}
+static Mat load_mat(const float* m)
+{
+    Mat r;
+    if (m != nullptr)
+        memcpy(r.m, m, sizeof(r.m));
+    return r;
+}
+float oracle_wang_cubic(const float* pts8, float precision, const float* m6)
+{
+    V2 p[4];
+    load_pts(pts8, p);
+    return wang_cubic(p, precision, load_mat(m6));
+}
+float oracle_wang_cubic_pow4(const float* pts8, float precision, const float* m6)
+{
+    V2 p[4];
+    load_pts(pts8, p);
+    return wang_cubic_pow4(p, precision, load_mat(m6));
+}
+float oracle_fast_acos(float x) { return fast_acos(x); }
+float oracle_polar_segments_per_radian(float r) { return calc_polar_segments_per_radian(r); }
+float oracle_find_max_scale(const float* m6) { return find_max_scale(load_mat(m6)); }
+void oracle_eval_cubic(const float* pts8, float t, float* out2)
+{
+    V2 p[4];
+    load_pts(pts8, p);
+    V2 r = eval_cubic_at(p, t);
+    out2[0] = r.x;
+    out2[1] = r.y;
+}
+void oracle_chop_cubic_at(const float* pts8, float t, float* out14)
+{
+    V2 p[4];
+    load_pts(pts8, p);
+    V2 d[7];
+    chop_cubic_at(p, d, t);
+    memcpy(out14, d, sizeof(d));
+}
+void oracle_chop_cubic_at2(const float* pts8, float t0, float t1, float* out20)
+{
+    V2 p[4];
+    load_pts(pts8, p);
+    V2 d[10];
+    chop_cubic_at2(p, d, t0, t1);
+    memcpy(out20, d, sizeof(d));
+}
+int oracle_convex_180_chops(const float* pts8, float* T2, int* areCusps)
+{
+    V2 p[4];
+    load_pts(pts8, p);
+    bool c = false;
+    float T[2] = {0, 0};
+    int n = find_cubic_convex_180_chops(p, T, &c);
+    T2[0] = T[0];
+    T2[1] = T[1];
+    *areCusps = c ? 1 : 0;
+    return n;
+}
+}

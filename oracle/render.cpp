@@ -1,0 +1,1178 @@
+// CPU ORACLE (test infrastructure only): the front end (RiveRenderer clip
+// logic, PathDraw records, gradients, one logical flush) and the raster
+// (midpoint-fan patch vertex shader, triangle rasterisation, raster-ordered
+// pixel-local-storage resolve, color ramps, dither).
+//
+// Rasterisation is exact on a 1/256-pixel fixed-point grid with a symbolic
+// perturbation tie rule (sample point nudged by (+e, +e^2)), so every sample
+// lands strictly inside exactly one of two triangles sharing an edge.
+#include "oracle.hpp"
+#include "oracle_flush.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cassert>
+#include <cmath>
+#include <cstring>
+
+namespace oracle
+{
+namespace
+{
+constexpr float PI_ = 3.14159265359f;
+constexpr float _2PI_ = 6.28318530718f;
+constexpr float AA_RADIUS = .5f;
+constexpr uint32_t SOLID_COLOR_PAINT_TYPE = 1, LINEAR_GRADIENT_PAINT_TYPE = 2,
+                   RADIAL_GRADIENT_PAINT_TYPE = 3, CLIP_UPDATE_PAINT_TYPE = 0;
+constexpr uint32_t PAINT_FLAG_NON_ZERO_FILL = 0x100u, PAINT_FLAG_EVEN_ODD_FILL = 0x200u;
+constexpr uint32_t GRAD_SPAN_FLAG_LEFT_BORDER = 0x80000000u;
+constexpr uint32_t GRAD_SPAN_FLAG_RIGHT_BORDER = 0x40000000u;
+constexpr uint32_t GRAD_SPAN_FLAG_COMPLEX_BORDER = 0x20000000u;
+constexpr uint32_t GRAD_SPAN_FLAGS_MASK = 0xe0000000u;
+
+inline uint32_t fbits(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return u;
+}
+inline float ubits(uint32_t u)
+{
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+// IEEE fp32 -> fp16 -> fp32 with round-to-nearest-even (packHalf2x16 /
+// unpackHalf2x16 of the pixel-local-storage coverage plane).
+float round_to_f16(float x)
+{
+    if (!std::isfinite(x))
+        return x;
+    float ax = fabsf(x);
+    if (ax >= 65520.f)
+        return copysignf(INFINITY, x);
+    if (ax < 6.103515625e-05f)
+    {
+        // subnormal half: quantum 2^-24
+        float q = rintf(ax * 16777216.f) / 16777216.f;
+        return copysignf(q, x);
+    }
+    int e;
+    frexpf(ax, &e); // ax = m * 2^e, m in [0.5,1)
+    float quantum = ldexpf(1.f, e - 11);
+    return copysignf(rintf(ax / quantum) * quantum, x);
+}
+
+// ---------------- Mat2D helpers (include/rive/math/mat2d.hpp) -------------
+Mat mul(const Mat& a, const Mat& b) // a * b: apply b first
+{
+    Mat r;
+    r.m[0] = a.m[0] * b.m[0] + a.m[2] * b.m[1];
+    r.m[1] = a.m[1] * b.m[0] + a.m[3] * b.m[1];
+    r.m[2] = a.m[0] * b.m[2] + a.m[2] * b.m[3];
+    r.m[3] = a.m[1] * b.m[2] + a.m[3] * b.m[3];
+    r.m[4] = a.m[0] * b.m[4] + a.m[2] * b.m[5] + a.m[4];
+    r.m[5] = a.m[1] * b.m[4] + a.m[3] * b.m[5] + a.m[5];
+    return r;
+}
+bool invert(const Mat& a, Mat* out)
+{
+    double det = (double)a.m[0] * a.m[3] - (double)a.m[1] * a.m[2];
+    if (det == 0 || !std::isfinite(det))
+        return false;
+    double inv = 1 / det;
+    out->m[0] = (float)(a.m[3] * inv);
+    out->m[1] = (float)(-a.m[1] * inv);
+    out->m[2] = (float)(-a.m[2] * inv);
+    out->m[3] = (float)(a.m[0] * inv);
+    out->m[4] = (float)((a.m[2] * (double)a.m[5] - a.m[3] * (double)a.m[4]) * inv);
+    out->m[5] = (float)((a.m[1] * (double)a.m[4] - a.m[0] * (double)a.m[5]) * inv);
+    return true;
+}
+V2 map(const Mat& m, V2 p)
+{
+    return {m.m[0] * p.x + m.m[2] * p.y + m.m[4], m.m[1] * p.x + m.m[3] * p.y + m.m[5]};
+}
+
+struct IAABB
+{
+    int l, t, r, b;
+    bool empty() const { return l >= r || t >= b; }
+    IAABB intersect(const IAABB& o) const
+    {
+        return {std::max(l, o.l), std::max(t, o.t), std::min(r, o.r), std::min(b, o.b)};
+    }
+};
+// Mat2D::mapBoundingBox + AABB::roundOut (include/rive/math/aabb.hpp).
+IAABB map_bounds_round_out(const Mat& m, const V2* pts, uint32_t n, float outsetX,
+                           float outsetY)
+{
+    float l = INFINITY, t = INFINITY, r = -INFINITY, b = -INFINITY;
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        V2 p = map(m, pts[i]);
+        l = std::min(l, p.x);
+        t = std::min(t, p.y);
+        r = std::max(r, p.x);
+        b = std::max(b, p.y);
+    }
+    l -= outsetX;
+    r += outsetX;
+    t -= outsetY;
+    b += outsetY;
+    return {(int)floorf(l), (int)floorf(t), (int)ceilf(r), (int)ceilf(b)};
+}
+} // namespace
+
+// ======================= front end ========================================
+
+namespace
+{
+struct Gradient
+{
+    uint32_t paintType; // LINEAR/RADIAL
+    std::vector<uint32_t> colors;
+    std::vector<float> stops;
+    float coeffs[3];
+};
+
+// Gradient::MakeLinear / MakeRadial (renderer/src/gradient.cpp:38-175).
+bool make_gradient(const rive_stream_draw& d, const rive_stream& s, Gradient* g)
+{
+    size_t count = d.stop_count;
+    if (count == 0)
+        return false;
+    const uint32_t* colors = s.stop_colors + d.first_stop;
+    const float* stops = s.stop_offsets + d.first_stop;
+    for (size_t i = 0; i < count; ++i)
+    {
+        if (!(stops[i] >= 0 && stops[i] <= 1))
+            return false;
+        if (i > 0 && !(stops[i - 1] <= stops[i]))
+            return false;
+    }
+    g->colors.assign(colors, colors + count);
+    g->stops.assign(stops, stops + count);
+    if (d.paint_type == 1)
+    {
+        g->paintType = LINEAR_GRADIENT_PAINT_TYPE;
+        float sx = d.grad[0], sy = d.grad[1], ex = d.grad[2], ey = d.grad[3];
+        float firstStop = stops[0], lastStop = stops[count - 1];
+        if ((firstStop != 0 || lastStop != 1) && lastStop - firstStop > (1.f / (1 << 12)))
+        {
+            // precise_mix(start, end, t) = start*(1-t) + end*t
+            float nsx = sx * (1.f - firstStop) + ex * firstStop;
+            float nsy = sy * (1.f - firstStop) + ey * firstStop;
+            float nex = sx * (1.f - lastStop) + ex * lastStop;
+            float ney = sy * (1.f - lastStop) + ey * lastStop;
+            sx = nsx, sy = nsy, ex = nex, ey = ney;
+            g->stops[0] = 0;
+            g->stops[count - 1] = 1;
+            if (count > 2)
+            {
+                float m = 1.f / (lastStop - firstStop);
+                float a = -firstStop * m;
+                for (size_t i = 1; i < count - 1; ++i)
+                    g->stops[i] = stops[i] * m + a;
+                for (size_t i = 1; i < count - 1; ++i)
+                    g->stops[i] = fmaxf(g->stops[i - 1], g->stops[i]);
+                for (size_t i = count - 2; i != 0; --i)
+                    g->stops[i] = fminf(g->stops[i], g->stops[i + 1]);
+            }
+        }
+        float vx = ex - sx, vy = ey - sy;
+        float inv = 1.f / (vx * vx + vy * vy);
+        vx *= inv;
+        vy *= inv;
+        g->coeffs[0] = vx;
+        g->coeffs[1] = vy;
+        g->coeffs[2] = -(vx * sx + vy * sy);
+    }
+    else
+    {
+        g->paintType = RADIAL_GRADIENT_PAINT_TYPE;
+        float cx = d.grad[0], cy = d.grad[1], radius = d.grad[2];
+        float lastStop = stops[count - 1];
+        if (lastStop != 1 && lastStop > (1.f / (1 << 12)))
+        {
+            g->stops[count - 1] = 1;
+            radius *= lastStop;
+            float inverseLastStop = 1.f / lastStop;
+            for (size_t i = 0; i < count - 1; ++i)
+                g->stops[i] = stops[i] * inverseLastStop;
+            if (count > 1)
+            {
+                g->stops[0] = fmaxf(0, g->stops[0]);
+                for (size_t i = 1; i < count - 1; ++i)
+                    g->stops[i] = fmaxf(g->stops[i - 1], g->stops[i]);
+                for (size_t i = count - 2; i != (size_t)-1; --i)
+                    g->stops[i] = fminf(g->stops[i], g->stops[i + 1]);
+            }
+        }
+        g->coeffs[0] = cx;
+        g->coeffs[1] = cy;
+        g->coeffs[2] = radius;
+    }
+    return true;
+}
+
+struct ClipElement
+{
+    uint32_t path;
+    Mat matrix;
+    FillRule fillRule;
+    IAABB pixelBounds;
+    uint32_t clipID;
+};
+
+uint32_t swizzle_rgba(uint32_t argb) // SwizzleRiveColorToRGBA
+{
+    uint32_t a = argb >> 24, r = (argb >> 16) & 0xff, g = (argb >> 8) & 0xff, b = argb & 0xff;
+    return r | (g << 8) | (b << 16) | (a << 24);
+}
+} // namespace
+
+struct Builder
+{
+    const rive_stream& s;
+    Flush& f;
+    std::vector<ClipElement> clipStack;
+    uint32_t clipContentID = 0;
+    uint32_t nextClipID = 1;
+    uint32_t nextContourID = 1;
+    uint32_t tessLocation = kMidpointFanPatchSegmentSpan; // pre-padding
+    // gradient bookkeeping (render_context.cpp:588-674)
+    std::vector<std::pair<uint32_t, uint32_t>> simpleRamps;
+    std::vector<Gradient> complexRamps;
+    struct PendingPaint
+    {
+        uint32_t pathID;
+        bool complex;
+        uint32_t row, col;
+    };
+    std::vector<PendingPaint> pendingGradPaints;
+
+    Builder(const rive_stream& s_, Flush& f_) : s(s_), f(f_) {}
+
+    const V2* pathPts(uint32_t path) const
+    {
+        return reinterpret_cast<const V2*>(s.points) + s.paths[path].first_point;
+    }
+
+    // Pushes one PathDraw: records + tessellation (LogicalFlush::pushDraws).
+    void pushPathDraw(uint32_t pathIdx, const Mat& matrix, float strokeRadius, Join join,
+                      Cap cap, FillRule fillRule, uint32_t paintType, uint32_t paintValue,
+                      uint32_t clipID, uint32_t outerClipID, const Gradient* grad)
+    {
+        const rive_stream_path& sp = s.paths[pathIdx];
+        PathDrawDesc d;
+        d.verbs = s.verbs + sp.first_verb;
+        d.verbCount = sp.verb_count;
+        d.pts = pathPts(pathIdx);
+        d.ptCount = sp.point_count;
+        d.matrix = matrix;
+        d.strokeRadius = strokeRadius;
+        d.join = join;
+        d.cap = cap;
+        d.fillRule = fillRule;
+        derive_directions(&d);
+
+        uint32_t pathID = (uint32_t)f.pathData.size(); // record 0 is reserved
+        d.pathID = pathID;
+        PathTess pt = tessellate_path_records(d, nextContourID, tessLocation);
+
+        // PathData (gpu.cpp:859-877)
+        rive_hip_path_data pd{};
+        memcpy(pd.matrix, matrix.m, sizeof(pd.matrix));
+        pd.stroke_radius = strokeRadius;
+        pd.feather_radius = 0;
+        pd.z_index = 0;
+        f.pathData.push_back(pd);
+
+        // PaintData (gpu.cpp:879-939)
+        rive_hip_paint_data pp{};
+        uint32_t params = paintType;
+        if (paintType == CLIP_UPDATE_PAINT_TYPE)
+        {
+            pp.value = clipID << 16;
+            params |= outerClipID << 16;
+        }
+        else
+        {
+            params |= clipID << 16; // srcOver blend mode = 0
+            pp.value = paintValue;
+        }
+        if (strokeRadius == 0)
+        {
+            if (fillRule == FillRule::nonZero)
+                params |= PAINT_FLAG_NON_ZERO_FILL;
+            else if (fillRule == FillRule::evenOdd)
+                params |= PAINT_FLAG_EVEN_ODD_FILL;
+        }
+        pp.params = params;
+        f.paintData.push_back(pp);
+
+        rive_hip_paint_aux aux{};
+        // clip rect: WideOpen (no clip rect) -> identity-ish, inverseFwidth 0
+        if (grad != nullptr)
+        {
+            Mat inv;
+            if (!invert(matrix, &inv))
+                inv = Mat();
+            Mat pm;
+            if (grad->paintType == LINEAR_GRADIENT_PAINT_TYPE)
+            {
+                Mat g;
+                g.m[0] = grad->coeffs[0];
+                g.m[1] = 0;
+                g.m[2] = grad->coeffs[1];
+                g.m[3] = 0;
+                g.m[4] = grad->coeffs[2];
+                g.m[5] = 0;
+                pm = mul(g, inv);
+            }
+            else
+            {
+                float w = 1 / grad->coeffs[2];
+                Mat g;
+                g.m[0] = w;
+                g.m[1] = 0;
+                g.m[2] = 0;
+                g.m[3] = w;
+                g.m[4] = -grad->coeffs[0] * w;
+                g.m[5] = -grad->coeffs[1] * w;
+                pm = mul(g, inv);
+            }
+            memcpy(aux.paint_matrix, pm.m, sizeof(aux.paint_matrix));
+            allocateGradient(*grad, pathID);
+        }
+        f.paintAux.push_back(aux);
+
+        // Tessellation bookkeeping
+        DrawInfo di;
+        di.pathID = pathID;
+        di.tessLocation = tessLocation;
+        di.tessVertexCount = pt.midpointFanTessVertexCount;
+        di.dirs = d.dirs;
+        di.isStroke = strokeRadius > 0;
+        f.draws.push_back(di);
+        for (size_t c = 0; c < pt.contours.size(); ++c)
+            f.contours.push_back(pt.contours[c]);
+        nextContourID += (uint32_t)pt.contours.size();
+        for (const CubicRecord& r : pt.records)
+        {
+            f.records.push_back(r);
+            write_spans(r, d.dirs, &f.spans);
+        }
+        f.pathTessVertexCounts.push_back(pt.midpointFanTessVertexCount);
+        tessLocation += pt.midpointFanTessVertexCount;
+    }
+
+    void allocateGradient(const Gradient& g, uint32_t pathID)
+    {
+        size_t n = g.stops.size();
+        PendingPaint pp;
+        pp.pathID = pathID;
+        if (n == 1 || (n == 2 && g.stops[0] == 0 && g.stops[1] == 1))
+        {
+            std::pair<uint32_t, uint32_t> key{g.colors[0], g.colors[std::min<size_t>(1, n - 1)]};
+            uint32_t idx = ~0u;
+            for (size_t i = 0; i < simpleRamps.size(); ++i)
+                if (simpleRamps[i] == key)
+                    idx = (uint32_t)i * 2;
+            if (idx == ~0u)
+            {
+                idx = (uint32_t)simpleRamps.size() * 2;
+                simpleRamps.push_back(key);
+            }
+            pp.complex = false;
+            pp.row = idx / kGradTextureWidth;
+            pp.col = idx % kGradTextureWidth;
+        }
+        else
+        {
+            uint32_t row = ~0u;
+            for (size_t i = 0; i < complexRamps.size(); ++i)
+                if (complexRamps[i].colors == g.colors && complexRamps[i].stops == g.stops)
+                    row = (uint32_t)i;
+            if (row == ~0u)
+            {
+                row = (uint32_t)complexRamps.size();
+                complexRamps.push_back(g);
+            }
+            pp.complex = true;
+            pp.row = row;
+            pp.col = 0;
+        }
+        pendingGradPaints.push_back(pp);
+    }
+
+    // Writes gradient spans and patches the gradient paint records
+    // (render_context.cpp:1462-1533, gpu.cpp:900-914, 959-985).
+    void finishGradients()
+    {
+        const uint32_t simpleRows =
+            (uint32_t)((simpleRamps.size() * 2 + kGradTextureWidth - 1) / kGradTextureWidth);
+        const uint32_t complexRows = (uint32_t)complexRamps.size();
+        f.gradSimpleRows = simpleRows;
+        f.gradComplexRows = complexRows;
+        const uint32_t height = std::max<uint32_t>(simpleRows + complexRows, 1);
+        const uint32_t ONE_TEXEL_FIXED = 65536 / kGradTextureWidth;
+        for (size_t i = 0; i < simpleRamps.size(); ++i)
+        {
+            uint32_t y = (uint32_t)(i / (kGradTextureWidth / 2));
+            uint32_t centerX = (uint32_t)((i % (kGradTextureWidth / 2)) * 2 + 1);
+            uint32_t cx = centerX * ONE_TEXEL_FIXED;
+            rive_hip_grad_span gs;
+            gs.horizontal_span = (cx << 16) | cx;
+            gs.y_with_flags = GRAD_SPAN_FLAG_LEFT_BORDER | GRAD_SPAN_FLAG_RIGHT_BORDER | y;
+            gs.color0 = simpleRamps[i].first;
+            gs.color1 = simpleRamps[i].second;
+            f.gradSpans.push_back(gs);
+        }
+        for (uint32_t i = 0; i < complexRows; ++i)
+        {
+            const Gradient& g = complexRamps[i];
+            uint32_t y = i + simpleRows;
+            float m = (kGradTextureWidth - 1.f) * ONE_TEXEL_FIXED;
+            float a = .5f * ONE_TEXEL_FIXED;
+            uint32_t lastXFixed = (uint32_t)(g.stops[0] * m + a);
+            uint32_t lastColor = g.colors[0];
+            for (size_t k = 1; k < g.stops.size(); ++k)
+            {
+                uint32_t xFixed = (uint32_t)(g.stops[k] * m + a);
+                uint32_t flags = GRAD_SPAN_FLAG_COMPLEX_BORDER;
+                if (k == 1)
+                    flags |= GRAD_SPAN_FLAG_LEFT_BORDER;
+                if (k == g.stops.size() - 1)
+                    flags |= GRAD_SPAN_FLAG_RIGHT_BORDER;
+                rive_hip_grad_span gs;
+                gs.horizontal_span = (xFixed << 16) | lastXFixed;
+                gs.y_with_flags = flags | y;
+                gs.color0 = lastColor;
+                gs.color1 = g.colors[k];
+                f.gradSpans.push_back(gs);
+                lastColor = g.colors[k];
+                lastXFixed = xFixed;
+            }
+        }
+        const float inverseHeight = 1.f / height;
+        for (const PendingPaint& pp : pendingGradPaints)
+        {
+            uint32_t row = pp.row + (pp.complex ? simpleRows : 0);
+            f.paintData[pp.pathID].value = fbits((static_cast<float>(row) + .5f) * inverseHeight);
+            float left, right;
+            if (pp.complex)
+            {
+                left = 0;
+                right = (float)kGradTextureWidth;
+            }
+            else
+            {
+                left = (float)pp.col;
+                right = left + 2;
+            }
+            f.paintAux[pp.pathID].grad_span[0] = (right - left - 1) * (1.f / kGradTextureWidth);
+            f.paintAux[pp.pathID].grad_span[1] = (left + .5f) * (1.f / kGradTextureWidth);
+        }
+        f.gradHeight = height;
+    }
+
+    void build()
+    {
+        f.width = s.width;
+        f.height = s.height;
+        f.clearColor = s.clear_color;
+        f.dither = s.dither;
+        // Path record 0 is reserved for the clear color (render_context.cpp:1535-1548).
+        f.pathData.push_back(rive_hip_path_data{});
+        rive_hip_paint_data clearPaint{};
+        clearPaint.params = SOLID_COLOR_PAINT_TYPE;
+        clearPaint.value = swizzle_rgba(s.clear_color);
+        f.paintData.push_back(clearPaint);
+        f.paintAux.push_back(rive_hip_paint_aux{});
+
+        const IAABB frame{0, 0, (int)s.width, (int)s.height};
+        for (uint32_t di = 0; di < s.draw_count; ++di)
+        {
+            const rive_stream_draw& d = s.draws[di];
+            Mat m;
+            memcpy(m.m, d.matrix, sizeof(m.m));
+            // save(); transform(matrix); clipPath()... (rive_renderer.cpp:93-112,
+            // 199-228, 323-380)
+            size_t height = 0;
+            IAABB overall = frame;
+            bool clippedOut = false;
+            for (uint32_t c = 0; c < d.clip_count && !clippedOut; ++c)
+            {
+                uint32_t cp = d.clip_paths[c];
+                const rive_stream_path& sp = s.paths[cp];
+                if (sp.verb_count == 0)
+                {
+                    clippedOut = true;
+                    break;
+                }
+                bool equivalent = clipStack.size() > height && clipStack[height].path == cp &&
+                                  memcmp(clipStack[height].matrix.m, m.m, sizeof(m.m)) == 0;
+                if (!equivalent)
+                {
+                    IAABB pb = map_bounds_round_out(m, pathPts(cp), sp.point_count, 0, 0);
+                    overall = overall.intersect(pb);
+                    if (overall.empty())
+                    {
+                        clippedOut = true;
+                        break;
+                    }
+                    clipStack.resize(height);
+                    clipStack.push_back({cp, m, (FillRule)sp.fill_rule, pb, 0});
+                }
+                else
+                {
+                    overall = overall.intersect(clipStack[height].pixelBounds);
+                }
+                ++height;
+            }
+            if (clippedOut || overall.empty())
+                continue;
+
+            // drawPath (rive_renderer.cpp:121-197)
+            const rive_stream_path& sp = s.paths[d.path];
+            if (sp.verb_count == 0)
+                continue;
+            bool isStroke = d.style == 1;
+            if (isStroke && !(d.thickness > 0))
+                continue;
+            if (!(d.feather >= 0))
+                continue;
+            if (d.feather != 0)
+                continue; // feathering is not restated yet
+            // PathDraw::Make bounds (draw.cpp:452-494)
+            float ox = 0, oy = 0;
+            if (isStroke)
+            {
+                float outset = d.thickness * .5f;
+                if ((Join)d.join == Join::miter)
+                    outset *= 4.f;
+                else if ((Cap)d.cap == Cap::square)
+                    outset *= 1.41421356f;
+                // strokePixelOutset = paintMatrix.mapBoundingBox({0,0,outset,outset})
+                V2 corners[4] = {{0, 0}, {outset, 0}, {0, outset}, {outset, outset}};
+                Mat lin = m;
+                lin.m[4] = lin.m[5] = 0;
+                float l = INFINITY, t = INFINITY, r = -INFINITY, b = -INFINITY;
+                for (V2 c : corners)
+                {
+                    V2 p = map(lin, c);
+                    l = std::min(l, p.x), t = std::min(t, p.y);
+                    r = std::max(r, p.x), b = std::max(b, p.y);
+                }
+                ox = (r - l) + 1;
+                oy = (b - t) + 1;
+            }
+            IAABB pixelBounds = map_bounds_round_out(m, pathPts(d.path), sp.point_count, ox, oy);
+            if (pixelBounds.intersect(frame).empty())
+                continue;
+            // applyClip (rive_renderer.cpp:631-824)
+            if (pixelBounds.intersect(overall).empty())
+                continue;
+            uint32_t drawClipID = 0;
+            if (height > 0)
+            {
+                size_t inBuffer = (size_t)-1;
+                if (clipContentID != 0)
+                    for (size_t i = height - 1; i != (size_t)-1; --i)
+                        if (clipStack[i].clipID == clipContentID)
+                        {
+                            inBuffer = i;
+                            break;
+                        }
+                uint32_t parentClipID = inBuffer == (size_t)-1 ? 0 : clipStack[inBuffer].clipID;
+                for (size_t i = inBuffer + 1; i < height; ++i)
+                {
+                    ClipElement& ce = clipStack[i];
+                    ce.clipID = nextClipID++;
+                    IAABB cb = ce.pixelBounds;
+                    if (!cb.intersect(frame).empty())
+                        pushPathDraw(ce.path, ce.matrix, 0, Join::miter, Cap::butt, ce.fillRule,
+                                     CLIP_UPDATE_PAINT_TYPE, 0, ce.clipID, parentClipID, nullptr);
+                    parentClipID = ce.clipID;
+                }
+                drawClipID = parentClipID;
+                clipContentID = parentClipID;
+            }
+            Gradient grad;
+            const Gradient* gp = nullptr;
+            uint32_t paintType = SOLID_COLOR_PAINT_TYPE;
+            uint32_t paintValue = swizzle_rgba(d.color);
+            if (d.paint_type != 0)
+            {
+                if (make_gradient(d, s, &grad))
+                {
+                    gp = &grad;
+                    paintType = grad.paintType;
+                }
+                else
+                {
+                    paintValue = swizzle_rgba(0); // invalid gradients draw nothing visible
+                }
+            }
+            float strokeRadius = 0;
+            if (isStroke)
+                strokeRadius = fmaxf(d.thickness * .5f, 1.17549435e-38f);
+            pushPathDraw(d.path, m, strokeRadius, (Join)d.join, (Cap)d.cap,
+                         (FillRule)sp.fill_rule, paintType, paintValue, drawClipID, 0, gp);
+        }
+        finishGradients();
+        f.pathCount = (uint32_t)f.pathData.size() - 1;
+        f.tessEnd = tessLocation;
+        // Padding spans (render_context.cpp:1550-1568): 8 at 0 and 1 at the end.
+        std::vector<TessVertexSpan> padding;
+        write_padding_span(kMidpointFanPatchSegmentSpan, 0, &padding);
+        write_padding_span(1, tessLocation, &padding);
+        f.spans.insert(f.spans.begin(), padding.begin(), padding.end());
+        f.tessVertexCount = tessLocation + 1;
+    }
+};
+
+void build_flush(const rive_stream& s, Flush* f) { Builder(s, *f).build(); }
+
+// ======================= color ramps ======================================
+
+void render_ramps(const Flush& f, std::vector<uint32_t>* texture)
+{
+    // color_ramp.glsl:40-106: each GradientSpan is a 1px-tall strip; color0 at
+    // x0, color1 at x1, linearly interpolated, with solid borders.
+    const uint32_t W = kGradTextureWidth;
+    texture->assign((size_t)W * f.gradHeight, 0);
+    for (const rive_hip_grad_span& gs : f.gradSpans)
+    {
+        float x0 = float(gs.horizontal_span & 0xffffu) / 65536.f;
+        float x1 = float(gs.horizontal_span >> 16) / 65536.f;
+        uint32_t yflags = gs.y_with_flags;
+        uint32_t y = yflags & ~GRAD_SPAN_FLAGS_MASK;
+        float left = x0, right = x1;
+        if (yflags & GRAD_SPAN_FLAG_LEFT_BORDER)
+            left = (yflags & GRAD_SPAN_FLAG_COMPLEX_BORDER) ? 0.f : x0 - 1.f / W;
+        if (yflags & GRAD_SPAN_FLAG_RIGHT_BORDER)
+            right = (yflags & GRAD_SPAN_FLAG_COMPLEX_BORDER) ? 1.f : x1 + 1.f / W;
+        float c0[4], c1[4];
+        for (int k = 0; k < 4; ++k)
+        {
+            int shift = k < 3 ? 16 - 8 * k : 24; // r, g, b, a
+            c0[k] = float((gs.color0 >> shift) & 0xff) / 255.f;
+            c1[k] = float((gs.color1 >> shift) & 0xff) / 255.f;
+        }
+        for (uint32_t x = 0; x < W; ++x)
+        {
+            float u = (float(x) + .5f) / W;
+            if (!(u >= left && u < right))
+                continue;
+            float c[4];
+            if (u < x0)
+                memcpy(c, c0, sizeof(c));
+            else if (u >= x1)
+                memcpy(c, c1, sizeof(c));
+            else
+            {
+                float t = (u - x0) / (x1 - x0);
+                for (int k = 0; k < 4; ++k)
+                    c[k] = (c1[k] - c0[k]) * t + c0[k];
+            }
+            uint32_t px = 0;
+            for (int k = 0; k < 4; ++k)
+            {
+                float v = std::min(std::max(c[k], 0.f), 1.f);
+                px |= (uint32_t)floorf(v * 255.f + .5f) << (8 * k);
+            }
+            (*texture)[(size_t)y * W + x] = px; // RGBA8 little endian
+        }
+    }
+}
+
+// ======================= raster ===========================================
+
+namespace
+{
+struct FxP
+{
+    int64_t x, y;
+};
+inline int64_t snap(float v)
+{
+    float c = std::min(std::max(v * 256.f, -8388608.f), 8388607.f);
+    if (std::isnan(c))
+        c = 0;
+    return (int64_t)rintf(c);
+}
+// Orientation with symbolic perturbation of q by (+e, +e^2).
+inline int sos_sign(FxP a, FxP b, FxP q)
+{
+    int64_t dx = b.x - a.x, dy = b.y - a.y;
+    int64_t D = dx * (q.y - a.y) - dy * (q.x - a.x);
+    if (D != 0)
+        return D > 0 ? 1 : -1;
+    if (dy != 0)
+        return dy > 0 ? -1 : 1;
+    return dx > 0 ? 1 : (dx < 0 ? -1 : 0);
+}
+inline int64_t edgeD(FxP a, FxP b, FxP q)
+{
+    return (b.x - a.x) * (q.y - a.y) - (b.y - a.y) * (q.x - a.x);
+}
+
+struct PatchVertexDef
+{
+    float id, outset, cov;
+    float mid, mout, mcov;
+    int type;
+};
+
+std::array<PatchVertexDef, 42> make_midpoint_fan_patch(std::array<uint16_t, 72>* idx)
+{
+    // gpu.cpp:446-687 (PatchType::midpointFan)
+    std::array<PatchVertexDef, 42> v{};
+    int n = 0;
+    for (int i = 0; i < 8; ++i)
+    {
+        float l = (float)i, r = l + 1;
+        v[n + 0] = {l, -1, 1, r - 1, -1, 1, 0};
+        v[n + 1] = {l, +1, 0, l - 1, -1, 1, 0};
+        v[n + 2] = {r, -1, 1, r - 1, +1, 0, 0};
+        v[n + 3] = {r, +1, 0, l - 1, +1, 0, 0};
+        n += 4;
+    }
+    for (int i = 0; i <= 8; ++i)
+        v[n++] = {(float)i, -1, 1, (float)i - 1, -1, 1, 1};
+    v[n++] = {0, 0, 1, 0, 0, 1, 2};
+    int k = 0;
+    const uint16_t pat[6] = {0, 1, 2, 2, 1, 3};
+    for (int s = 0; s < 8; ++s)
+        for (int j = 0; j < 6; ++j)
+            (*idx)[k++] = (uint16_t)(s * 4 + pat[j]);
+    const int fan = 32;
+    for (int step = 1; step < 8; step <<= 1)
+        for (int i = 0; i < 8; i += step * 2)
+        {
+            (*idx)[k++] = (uint16_t)(fan + i);
+            (*idx)[k++] = (uint16_t)(fan + i + step);
+            (*idx)[k++] = (uint16_t)(fan + i + step * 2);
+        }
+    (*idx)[k++] = (uint16_t)(fan);
+    (*idx)[k++] = (uint16_t)(fan + 8);
+    (*idx)[k++] = (uint16_t)(41);
+    return v;
+}
+
+struct OutVertex
+{
+    bool discard;
+    float x, y;   // pixel coordinates
+    float cx, cy; // coverages.xy
+    uint32_t pathID;
+};
+
+inline float signf_(float x) { return x > 0 ? 1.f : (x < 0 ? -1.f : 0.f); }
+inline float manhattan_pixel_width(const float M[4], float nx, float ny)
+{
+    // draw_path_common.glsl:268-273
+    float vx = M[0] * nx + M[2] * ny;
+    float vy = M[1] * nx + M[3] * ny;
+    return (fabsf(vx) + fabsf(vy)) * (1.f / (vx * vx + vy * vy));
+}
+
+// unpack_tessellated_path_vertex (draw_path_common.glsl:275-789), midpoint-fan
+// patches, no feather, no MSAA.
+OutVertex draw_vertex(const Flush& f, const std::vector<TessVertex>& tex,
+                      const PatchVertexDef& pv, int instanceID)
+{
+    OutVertex o{};
+    int localVertexID = (int)pv.id;
+    float outset = pv.outset;
+    float fillCoverage = pv.cov;
+    const int patchSegmentSpan = 8;
+    int vertexIDOnContour = std::min(localVertexID, patchSegmentSpan - 1);
+    int tessVertexIdx = instanceID * patchSegmentSpan + vertexIDOnContour;
+    TessVertex tv = tex[(size_t)tessVertexIdx];
+    uint32_t contourIDWithFlags = tv.contourIDWithFlags;
+    uint32_t contourID = std::max(contourIDWithFlags & CONTOUR_ID_MASK, 1u);
+    const ContourData& cd = f.contours[contourID - 1];
+    V2 midpoint = cd.midpoint;
+    uint32_t pathID = cd.pathID & 0xffffu;
+    uint32_t vertexIndex0 = cd.vertexIndex0;
+    const rive_hip_path_data& pd = f.pathData[pathID];
+    const float M[4] = {pd.matrix[0], pd.matrix[1], pd.matrix[2], pd.matrix[3]};
+    float tx = pd.matrix[4], ty = pd.matrix[5];
+    float strokeRadius = pd.stroke_radius;
+    uint32_t mirroredContourFlag = contourIDWithFlags & MIRRORED_CONTOUR_CONTOUR_FLAG;
+    if (mirroredContourFlag != 0u)
+    {
+        localVertexID = (int)pv.mid;
+        outset = pv.mout;
+        fillCoverage = pv.mcov;
+    }
+    if (localVertexID != vertexIDOnContour)
+    {
+        int replacement = tessVertexIdx + localVertexID - vertexIDOnContour;
+        TessVertex rv = tex[(size_t)replacement];
+        if ((rv.contourIDWithFlags & (MIRRORED_CONTOUR_CONTOUR_FLAG | 0xffffu)) !=
+            (contourIDWithFlags & (MIRRORED_CONTOUR_CONTOUR_FLAG | 0xffffu)))
+        {
+            bool isClosed = strokeRadius == 0.f || midpoint.x != 0.f;
+            if (isClosed)
+            {
+                tessVertexIdx = (int)vertexIndex0;
+                tv = tex[(size_t)tessVertexIdx];
+            }
+        }
+        else
+        {
+            tessVertexIdx = replacement;
+            tv = rv;
+        }
+        contourIDWithFlags = (tv.contourIDWithFlags & ~MIRRORED_CONTOUR_CONTOUR_FLAG) |
+                             mirroredContourFlag;
+    }
+    float theta = ubits(tv.z);
+    float nx = sinf(theta), ny = -cosf(theta);
+    float ox = tv.x, oy = tv.y;
+    float offx = 0, offy = 0;
+    float covx, covy;
+    if (strokeRadius != 0.f)
+    {
+        float det = M[0] * M[3] - M[2] * M[1];
+        outset *= signf_(det);
+        if (contourIDWithFlags & LEFT_JOIN_CONTOUR_FLAG)
+            outset = std::min(outset, 0.f);
+        if (contourIDWithFlags & RIGHT_JOIN_CONTOUR_FLAG)
+            outset = std::max(outset, 0.f);
+        float aaRadius = manhattan_pixel_width(M, nx, ny) * AA_RADIUS;
+        float globalCoverage = 1.f;
+        if (aaRadius > strokeRadius)
+        {
+            globalCoverage = strokeRadius / aaRadius;
+            strokeRadius = aaRadius;
+        }
+        float vox = nx * (strokeRadius + aaRadius), voy = ny * (strokeRadius + aaRadius);
+        float x = outset * (strokeRadius + aaRadius);
+        covx = (1.f / (aaRadius * 2.f)) * (x + strokeRadius) + .5f;
+        covy = (1.f / (aaRadius * 2.f)) * (-x + strokeRadius) + .5f;
+        uint32_t joinType = contourIDWithFlags & JOIN_TYPE_MASK;
+        if (joinType > ROUND_JOIN_CONTOUR_FLAG)
+        {
+            int peekDir = 2;
+            if ((contourIDWithFlags & JOIN_TANGENT_0_CONTOUR_FLAG) == 0u)
+                peekDir = -peekDir;
+            if ((contourIDWithFlags & MIRRORED_CONTOUR_CONTOUR_FLAG) != 0u)
+                peekDir = -peekDir;
+            TessVertex other = tex[(size_t)(tessVertexIdx + peekDir)];
+            float otherJoinTheta = ubits(other.z);
+            float joinAngle = fabsf(otherJoinTheta - theta);
+            if (joinAngle > PI_)
+                joinAngle = _2PI_ - joinAngle;
+            bool isTan0 = (contourIDWithFlags & JOIN_TANGENT_0_CONTOUR_FLAG) != 0u;
+            bool isLeftJoin = (contourIDWithFlags & LEFT_JOIN_CONTOUR_FLAG) != 0u;
+            float bisectTheta = joinAngle * (isTan0 == isLeftJoin ? -.5f : .5f) + theta;
+            float bx = sinf(bisectTheta), by = -cosf(bisectTheta);
+            float bisectPixelWidth = manhattan_pixel_width(M, bx, by);
+            float miterRatio = cosf(joinAngle * .5f);
+            float clipRadius;
+            if ((joinType == MITER_CLIP_JOIN_CONTOUR_FLAG) ||
+                (joinType == MITER_REVERT_JOIN_CONTOUR_FLAG && miterRatio >= .25f))
+            {
+                float miterInverseLimit =
+                    (contourIDWithFlags & EMULATED_STROKE_CAP_CONTOUR_FLAG) != 0u ? 1.f : .25f;
+                clipRadius = strokeRadius * (1.f / std::max(miterRatio, miterInverseLimit));
+            }
+            else
+            {
+                clipRadius = strokeRadius * miterRatio + bisectPixelWidth * .5f;
+            }
+            float clipAARadius = clipRadius + bisectPixelWidth * AA_RADIUS;
+            if ((contourIDWithFlags & JOIN_TANGENT_INNER_CONTOUR_FLAG) != 0u)
+            {
+                float strokeAARaidus = strokeRadius + aaRadius;
+                float slop = aaRadius * .125f;
+                if (strokeAARaidus <= clipAARadius * miterRatio + slop)
+                {
+                    float miterAARadius = strokeAARaidus * (1.f / miterRatio);
+                    vox = bx * miterAARadius;
+                    voy = by * miterAARadius;
+                }
+                else
+                {
+                    float bax = bx * clipAARadius, bay = by * clipAARadius;
+                    float k0 = vox * vox + voy * voy, k1 = bax * bax + bay * bay;
+                    // k * inverse(float2x2(vertexOffset, bisectAAOffset))
+                    float a = vox, b = voy, c = bax, d = bay;
+                    float idet = 1.f / (a * d - c * b);
+                    float nvx = (k0 * d - k1 * b) * idet;
+                    float nvy = (-k0 * c + k1 * a) * idet;
+                    vox = nvx;
+                    voy = nvy;
+                }
+            }
+            float ptx = fabsf(outset) * vox, pty = fabsf(outset) * voy;
+            float clipDistance =
+                (clipAARadius - (ptx * bx + pty * by)) / (bisectPixelWidth * (AA_RADIUS * 2.f));
+            if ((contourIDWithFlags & LEFT_JOIN_CONTOUR_FLAG) != 0u)
+                covy = clipDistance;
+            else
+                covx = clipDistance;
+        }
+        covx *= globalCoverage;
+        covy *= globalCoverage;
+        covy = std::max(covy, 1e-4f);
+        float px_ = outset * vox, py_ = outset * voy;
+        offx = M[0] * px_ + M[2] * py_;
+        offy = M[1] * px_ + M[3] * py_;
+        if (pv.type != 0)
+        {
+            o.discard = true;
+            return o;
+        }
+    }
+    else
+    {
+        covx = fillCoverage;
+        covy = -1.f;
+        // sign(MUL(outset * norm, inverse(M))) * AA_RADIUS
+        float det = M[0] * M[3] - M[2] * M[1];
+        float idet = 1.f / det;
+        float vx = outset * nx, vy = outset * ny;
+        // row vector * inverse(M); inverse(M) columns: (M3, -M1)/det, (-M2, M0)/det
+        float rx = (vx * M[3] + vy * -M[1]) * idet;
+        float ry = (vx * -M[2] + vy * M[0]) * idet;
+        offx = signf_(rx) * AA_RADIUS;
+        offy = signf_(ry) * AA_RADIUS;
+        if (bool(contourIDWithFlags & MIRRORED_CONTOUR_CONTOUR_FLAG) !=
+            bool(contourIDWithFlags & NEGATE_PATH_FILL_COVERAGE_FLAG))
+            covx = -covx;
+        if (pv.type == 2)
+        {
+            ox = midpoint.x;
+            oy = midpoint.y;
+        }
+    }
+    o.x = (M[0] * ox + M[2] * oy) + offx + tx;
+    o.y = (M[1] * ox + M[3] * oy) + offy + ty;
+    o.cx = covx;
+    o.cy = covy;
+    o.pathID = pathID;
+    o.discard = false;
+    return o;
+}
+
+float dither_noise(int x, int y)
+{
+    // common.glsl:269-274 with ditherScale 1/256, ditherBias -1/512
+    // (gpu.cpp:822-824); fragCoord is the pixel center.
+    float fx = float(x) + .5f, fy = float(y) + .5f;
+    float v = 0.06711056f * fx + 0.00583715f * fy;
+    float v1 = v - floorf(v);
+    float w = 52.9829189f * v1;
+    float v2 = w - floorf(w);
+    return v2 * (1.f / 256.f) + (-.5f / 256.f);
+}
+inline float unorm8(uint32_t packed, int k) { return float((packed >> (8 * k)) & 0xff) / 255.f; }
+inline uint32_t to_unorm8(float v)
+{
+    v = std::min(std::max(v, 0.f), 1.f);
+    return (uint32_t)floorf(v * 255.f + .5f);
+}
+} // namespace
+
+void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* rgba)
+{
+    const uint32_t W = f.width, H = f.height;
+    std::vector<uint32_t> ramps;
+    render_ramps(f, &ramps);
+
+    // Initial color: clear (premultiplied, gpu.cpp SwizzleRiveColorToRGBAPremul).
+    {
+        uint32_t c = f.clearColor;
+        float a = float(c >> 24) / 255.f;
+        float r = float((c >> 16) & 0xff) / 255.f * a;
+        float g = float((c >> 8) & 0xff) / 255.f * a;
+        float b = float(c & 0xff) / 255.f * a;
+        uint32_t px = to_unorm8(r) | (to_unorm8(g) << 8) | (to_unorm8(b) << 16) | (to_unorm8(a) << 24);
+        std::fill(rgba, rgba + (size_t)W * H, px);
+    }
+    std::vector<float> clipCov((size_t)W * H, 0.f);
+    std::vector<uint32_t> clipIDs((size_t)W * H, 0u);
+    std::vector<float> cov((size_t)W * H, 0.f);
+    std::vector<uint32_t> stamp((size_t)W * H, 0u);
+    std::vector<uint32_t> touched;
+    touched.reserve((size_t)W * H);
+
+    std::array<uint16_t, 72> idx;
+    auto patch = make_midpoint_fan_patch(&idx);
+
+    for (const DrawInfo& di : f.draws)
+    {
+        if (di.tessVertexCount == 0)
+            continue;
+        const uint32_t pathID = di.pathID;
+        const bool isStroke = di.isStroke;
+        touched.clear();
+        const int firstInstance = (int)(di.tessLocation / 8);
+        const int endInstance = (int)((di.tessLocation + di.tessVertexCount) / 8);
+        for (int inst = firstInstance; inst < endInstance; ++inst)
+        {
+            OutVertex ov[42];
+            for (int v = 0; v < 42; ++v)
+                ov[v] = draw_vertex(f, tex, patch[v], inst);
+            for (int t = 0; t < 24; ++t)
+            {
+                const OutVertex& A = ov[idx[t * 3 + 0]];
+                const OutVertex& B = ov[idx[t * 3 + 1]];
+                const OutVertex& C = ov[idx[t * 3 + 2]];
+                if (A.discard || B.discard || C.discard)
+                    continue;
+                FxP a{snap(A.x), snap(A.y)}, b{snap(B.x), snap(B.y)}, c{snap(C.x), snap(C.y)};
+                int64_t area = edgeD(a, b, c);
+                if (area <= 0)
+                    continue; // counterclockwise or degenerate: culled
+                int64_t minx = std::min({a.x, b.x, c.x}), maxx = std::max({a.x, b.x, c.x});
+                int64_t miny = std::min({a.y, b.y, c.y}), maxy = std::max({a.y, b.y, c.y});
+                int x0 = (int)std::max<int64_t>(0, (minx - 128) >> 8);
+                int x1 = (int)std::min<int64_t>((int64_t)W - 1, (maxx - 128 + 255) >> 8);
+                int y0 = (int)std::max<int64_t>(0, (miny - 128) >> 8);
+                int y1 = (int)std::min<int64_t>((int64_t)H - 1, (maxy - 128 + 255) >> 8);
+                const double inv = 1.0 / (double)area;
+                for (int y = y0; y <= y1; ++y)
+                {
+                    for (int x = x0; x <= x1; ++x)
+                    {
+                        FxP q{(int64_t)x * 256 + 128, (int64_t)y * 256 + 128};
+                        if (sos_sign(a, b, q) <= 0 || sos_sign(b, c, q) <= 0 ||
+                            sos_sign(c, a, q) <= 0)
+                            continue;
+                        double la = (double)edgeD(b, c, q) * inv;
+                        double lb = (double)edgeD(c, a, q) * inv;
+                        double lc = (double)edgeD(a, b, q) * inv;
+                        size_t p = (size_t)y * W + x;
+                        float fcx = (float)(la * A.cx + lb * B.cx + lc * C.cx);
+                        float count = stamp[p] == pathID ? cov[p] : 0.f;
+                        if (stamp[p] != pathID)
+                        {
+                            stamp[p] = pathID;
+                            touched.push_back((uint32_t)p);
+                        }
+                        if (isStroke)
+                        {
+                            float fcy = (float)(la * A.cy + lb * B.cy + lc * C.cy);
+                            count = std::max(std::min(fcx, fcy), count);
+                        }
+                        else
+                        {
+                            count = count + fcx;
+                        }
+                        cov[p] = round_to_f16(count);
+                    }
+                }
+            }
+        }
+
+        // Resolve every pixel this path touched (draw_raster_order_path.frag:14-238).
+        const rive_hip_paint_data& paint = f.paintData[pathID];
+        const rive_hip_paint_aux& aux = f.paintAux[pathID];
+        const uint32_t paintType = paint.params & 0xfu;
+        const bool evenOdd = (paint.params & PAINT_FLAG_EVEN_ODD_FILL) != 0u;
+        const bool clockwise = !isStroke && !evenOdd && !(paint.params & PAINT_FLAG_NON_ZERO_FILL);
+        for (uint32_t p : touched)
+        {
+            int x = (int)(p % W), y = (int)(p / W);
+            float count = cov[p];
+            float coverage;
+            if (clockwise)
+                coverage = std::min(std::max(count, 0.f), 1.f);
+            else
+            {
+                coverage = fabsf(count);
+                if (evenOdd)
+                {
+                    float h = coverage * .5f;
+                    coverage = 1.f - fabsf((h - floorf(h)) * 2.f + -1.f);
+                }
+                coverage = std::min(coverage, 1.f);
+            }
+            if (paintType == CLIP_UPDATE_PAINT_TYPE)
+            {
+                uint32_t clipID = paint.value >> 16;
+                uint32_t outerClipID = paint.params >> 16;
+                if (outerClipID != 0)
+                {
+                    float outer = clipIDs[p] == outerClipID ? clipCov[p] : 0.f;
+                    coverage = std::min(coverage, outer);
+                }
+                clipCov[p] = round_to_f16(coverage);
+                clipIDs[p] = clipID;
+                continue;
+            }
+            uint32_t clipID = paint.params >> 16;
+            if (clipID != 0)
+                coverage = clipIDs[p] == clipID ? std::min(clipCov[p], coverage) : 0.f;
+            float color[4];
+            if (paintType == SOLID_COLOR_PAINT_TYPE)
+            {
+                float a = unorm8(paint.value, 3);
+                color[0] = unorm8(paint.value, 0) * a;
+                color[1] = unorm8(paint.value, 1) * a;
+                color[2] = unorm8(paint.value, 2) * a;
+                color[3] = a;
+                for (int k = 0; k < 4; ++k)
+                    color[k] *= coverage;
+            }
+            else
+            {
+                float fx = float(x) + .5f, fy = float(y) + .5f;
+                float pcx = aux.paint_matrix[0] * fx + aux.paint_matrix[2] * fy + aux.paint_matrix[4];
+                float pcy = aux.paint_matrix[1] * fx + aux.paint_matrix[3] * fy + aux.paint_matrix[5];
+                float t = paintType == LINEAR_GRADIENT_PAINT_TYPE ? pcx : sqrtf(pcx * pcx + pcy * pcy);
+                t = std::min(std::max(t, 0.f), 1.f);
+                float span = aux.grad_span[0] > .9f ? 2.f : aux.grad_span[1];
+                float u = span > 1.f ? (1.f - 1.f / 512.f) * t + (.5f / 512.f)
+                                     : (1.f / 512.f) * t + span;
+                float rowf = ubits(paint.value);
+                // Linear filtering at texel centers; rows are sampled at centers.
+                uint32_t row = (uint32_t)(rowf * f.gradHeight);
+                float tc = u * 512.f - .5f;
+                int i0 = (int)floorf(tc);
+                float fr = tc - (float)i0;
+                int ia = std::min(std::max(i0, 0), 511), ib = std::min(std::max(i0 + 1, 0), 511);
+                uint32_t ta = ramps[(size_t)row * 512 + ia], tb = ramps[(size_t)row * 512 + ib];
+                for (int k = 0; k < 4; ++k)
+                    color[k] = (unorm8(tb, k) - unorm8(ta, k)) * fr + unorm8(ta, k);
+                color[3] *= coverage;
+                color[0] *= color[3];
+                color[1] *= color[3];
+                color[2] *= color[3];
+            }
+            uint32_t dst = rgba[p];
+            float paintAlpha = color[3];
+            float out[4];
+            for (int k = 0; k < 4; ++k)
+                out[k] = color[k] + unorm8(dst, k) * (1.f - paintAlpha);
+            if (f.dither && paintAlpha != 0.f)
+            {
+                float n = dither_noise(x, y);
+                out[0] += n;
+                out[1] += n;
+                out[2] += n;
+            }
+            rgba[p] = to_unorm8(out[0]) | (to_unorm8(out[1]) << 8) | (to_unorm8(out[2]) << 16) |
+                      (to_unorm8(out[3]) << 24);
+        }
+    }
+}
+
+void tessellate_flush(const Flush& f, std::vector<TessVertex>* tex)
+{
+    uint32_t rows = (f.tessVertexCount + kTessTextureWidth - 1) / kTessTextureWidth;
+    rows = std::max(rows, 1u);
+    tex->assign((size_t)rows * kTessTextureWidth, TessVertex{0, 0, 0, 0});
+    for (const TessVertexSpan& s : f.spans)
+        tessellate_span(s, f.contours.data(), nullptr, tex->data(), rows);
+}
+} // namespace oracle
