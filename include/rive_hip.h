@@ -127,7 +127,12 @@ typedef struct rive_hip_draw
     float matrix_max_scale;
     float polar_segments_per_radian;
     float parametric_precision;
-    uint32_t pad[6];
+    /* SPANS mode: the draw's tessellation allocation, as in the reference's
+     * DrawBatch (gpu.hpp) -- first texel and texel count (both directions).
+     * GEOMETRY mode: written by the device. */
+    uint32_t tess_location;
+    uint32_t tess_vertex_count;
+    uint32_t pad[2];
 } rive_hip_draw;
 
 /* ------------------------------------------------------------------------ */

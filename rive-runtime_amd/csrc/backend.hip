@@ -1,0 +1,655 @@
+// Host side of the gfx950 backend: the C ABI of include/rive_hip.h.
+//
+// One rive_hip_context owns one HIP stream.  Buffers follow the reference's
+// BufferRing contract (renderer/include/rive/renderer/buffer_ring.hpp:22-80):
+// map returns write-only pinned host memory, unmap enqueues the upload on the
+// context stream.  rive_hip_flush enqueues the whole pipeline on that stream.
+#include "../../include/rive_hip.h"
+#include "device_common.hpp"
+#include "raster_kernels.hip"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace rivehip;
+
+namespace
+{
+thread_local std::string g_lastError;
+
+int fail(int code, const std::string& msg)
+{
+    g_lastError = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do                                                                                       \
+    {                                                                                        \
+        hipError_t _e = (expr);                                                              \
+        if (_e != hipSuccess)                                                                \
+            return fail(RIVE_HIP_ERROR_DEVICE,                                               \
+                        std::string(#expr) + ": " + hipGetErrorString(_e));                  \
+    } while (0)
+
+struct DevBuf
+{
+    void* ptr = nullptr;
+    size_t cap = 0;
+};
+
+hipError_t ensure(DevBuf& b, size_t bytes)
+{
+    if (bytes <= b.cap)
+        return hipSuccess;
+    if (b.ptr != nullptr)
+    {
+        hipError_t e = hipFree(b.ptr);
+        if (e != hipSuccess)
+            return e;
+        b.ptr = nullptr;
+        b.cap = 0;
+    }
+    size_t cap = std::max<size_t>(bytes + bytes / 4, 256);
+    hipError_t e = hipMalloc(&b.ptr, cap);
+    if (e == hipSuccess)
+        b.cap = cap;
+    return e;
+}
+
+template <typename T> T* P(DevBuf& b) { return reinterpret_cast<T*>(b.ptr); }
+
+struct MappedBuffer
+{
+    void* dev = nullptr;
+    void* host = nullptr;
+    size_t size = 0; // requested size (exact, as the reference requires)
+    bool mapped = false;
+};
+
+constexpr int kEvents = 7;
+} // namespace
+
+struct rive_hip_target
+{
+    rive_hip_context* ctx;
+    uint32_t width, height;
+    uint32_t* pixels;
+};
+
+struct rive_hip_context
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    MappedBuffer bufs[RIVE_HIP_BUFFER_KIND_COUNT];
+    // pipeline scratch (grow-only)
+    DevBuf spanTexelCounts, spanTexelOffsets, scanPartials, scalars;
+    DevBuf tess, prims, cells, cellCount, cellOffset, cellCursor, cellBackdrop, binned;
+    DevBuf pathRaster, ramps, drawTess;
+    DevBuf geoCounts, geoRecords, geoSpanCounts, geoSpanOffsets;
+    uint32_t* hostScalars = nullptr; // pinned
+    hipEvent_t ev[kEvents] = {};
+    float passMs[6] = {};
+    bool timed = false;
+    // last flush sizes, for debug readback
+    uint32_t lastSpanCount = 0, lastContourCount = 0, lastTexels = 0, lastDrawCount = 0;
+    uint32_t lastGradRows = 0;
+};
+
+extern "C" {
+
+const char* rive_hip_last_error(void) { return g_lastError.c_str(); }
+
+int rive_hip_create(int device, rive_hip_context** out)
+{
+    if (out == nullptr)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "out is null");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return fail(RIVE_HIP_ERROR_DEVICE, "no HIP device available");
+    if (device < 0 || device >= n)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "device ordinal out of range");
+    HIP_TRY(hipSetDevice(device));
+    auto* ctx = new rive_hip_context;
+    ctx->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&ctx->hostScalars, 64, hipHostMallocDefault);
+    for (int i = 0; i < kEvents && e == hipSuccess; ++i)
+        e = hipEventCreate(&ctx->ev[i]);
+    if (e != hipSuccess)
+    {
+        delete ctx;
+        return fail(RIVE_HIP_ERROR_DEVICE, std::string("context setup: ") + hipGetErrorString(e));
+    }
+    *out = ctx;
+    return RIVE_HIP_OK;
+}
+
+void rive_hip_destroy(rive_hip_context* ctx)
+{
+    if (ctx == nullptr)
+        return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& b : ctx->bufs)
+    {
+        if (b.dev)
+            (void)hipFree(b.dev);
+        if (b.host)
+            (void)hipHostFree(b.host);
+    }
+    DevBuf* all[] = {&ctx->spanTexelCounts, &ctx->spanTexelOffsets, &ctx->scanPartials, &ctx->scalars,
+                     &ctx->tess, &ctx->prims, &ctx->cells, &ctx->cellCount, &ctx->cellOffset,
+                     &ctx->cellCursor, &ctx->cellBackdrop, &ctx->binned, &ctx->pathRaster,
+                     &ctx->ramps, &ctx->drawTess, &ctx->geoCounts, &ctx->geoRecords,
+                     &ctx->geoSpanCounts, &ctx->geoSpanOffsets};
+    for (DevBuf* b : all)
+        if (b->ptr)
+            (void)hipFree(b->ptr);
+    if (ctx->hostScalars)
+        (void)hipHostFree(ctx->hostScalars);
+    for (auto& e : ctx->ev)
+        if (e)
+            (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int rive_hip_buffer_resize(rive_hip_context* ctx, int kind, size_t bytes)
+{
+    if (ctx == nullptr || kind < 0 || kind >= RIVE_HIP_BUFFER_KIND_COUNT)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "bad buffer kind");
+    HIP_TRY(hipSetDevice(ctx->device));
+    MappedBuffer& b = ctx->bufs[kind];
+    if (b.mapped)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "resize while mapped");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (b.dev)
+        HIP_TRY(hipFree(b.dev));
+    if (b.host)
+        HIP_TRY(hipHostFree(b.host));
+    b.dev = b.host = nullptr;
+    b.size = bytes;
+    if (bytes > 0)
+    {
+        HIP_TRY(hipMalloc(&b.dev, bytes));
+        HIP_TRY(hipHostMalloc(&b.host, bytes, hipHostMallocDefault));
+    }
+    return RIVE_HIP_OK;
+}
+
+void* rive_hip_buffer_map(rive_hip_context* ctx, int kind, size_t bytes)
+{
+    if (ctx == nullptr || kind < 0 || kind >= RIVE_HIP_BUFFER_KIND_COUNT)
+    {
+        fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "bad buffer kind");
+        return nullptr;
+    }
+    MappedBuffer& b = ctx->bufs[kind];
+    if (bytes > b.size || b.host == nullptr)
+    {
+        fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "map exceeds buffer size");
+        return nullptr;
+    }
+    // The staging copy of the previous flush must have landed before the host
+    // overwrites it (the reference keeps 3 ring slots; one slot and a stream
+    // wait keep the same guarantee).
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess)
+    {
+        fail(RIVE_HIP_ERROR_DEVICE, "stream sync failed");
+        return nullptr;
+    }
+    b.mapped = true;
+    return b.host;
+}
+
+int rive_hip_buffer_unmap(rive_hip_context* ctx, int kind, size_t bytes)
+{
+    if (ctx == nullptr || kind < 0 || kind >= RIVE_HIP_BUFFER_KIND_COUNT)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "bad buffer kind");
+    MappedBuffer& b = ctx->bufs[kind];
+    if (!b.mapped)
+        return fail(RIVE_HIP_ERROR_NOT_MAPPED, "buffer is not mapped");
+    b.mapped = false;
+    if (bytes > b.size)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "unmap exceeds buffer size");
+    if (bytes > 0)
+        HIP_TRY(hipMemcpyAsync(b.dev, b.host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return RIVE_HIP_OK;
+}
+
+int rive_hip_target_create(rive_hip_context* ctx, uint32_t width, uint32_t height,
+                           rive_hip_target** out)
+{
+    if (ctx == nullptr || out == nullptr || width == 0 || height == 0 || width > 16384 ||
+        height > 16384)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "bad target size");
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto* t = new rive_hip_target{ctx, width, height, nullptr};
+    hipError_t e = hipMalloc(&t->pixels, (size_t)width * height * 4);
+    if (e != hipSuccess)
+    {
+        delete t;
+        return fail(RIVE_HIP_ERROR_OUT_OF_MEMORY, "target allocation failed");
+    }
+    HIP_TRY(hipMemsetAsync(t->pixels, 0, (size_t)width * height * 4, ctx->stream));
+    *out = t;
+    return RIVE_HIP_OK;
+}
+
+void rive_hip_target_destroy(rive_hip_target* t)
+{
+    if (t == nullptr)
+        return;
+    (void)hipSetDevice(t->ctx->device);
+    (void)hipStreamSynchronize(t->ctx->stream);
+    (void)hipFree(t->pixels);
+    delete t;
+}
+
+int rive_hip_sync(rive_hip_context* ctx)
+{
+    if (ctx == nullptr)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "ctx is null");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return RIVE_HIP_OK;
+}
+
+int rive_hip_read_pixels(rive_hip_context* ctx, rive_hip_target* t, uint8_t* rgba)
+{
+    if (ctx == nullptr || t == nullptr || rgba == nullptr)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMemcpyAsync(rgba, t->pixels, (size_t)t->width * t->height * 4, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return RIVE_HIP_OK;
+}
+
+} // extern "C"
+
+namespace
+{
+uint32_t premul_clear(uint32_t argb)
+{
+    float a = float(argb >> 24) / 255.f;
+    auto q = [](float v) { return (uint32_t)floorf(std::min(std::max(v, 0.f), 1.f) * 255.f + .5f); };
+    float r = float((argb >> 16) & 0xff) / 255.f * a;
+    float g = float((argb >> 8) & 0xff) / 255.f * a;
+    float b = float(argb & 0xff) / 255.f * a;
+    return q(r) | (q(g) << 8) | (q(b) << 16) | (q(a) << 24);
+}
+
+// Exclusive scan of n u32 values into out[0..n], out[n] = total.  partials and
+// total live in ctx scratch.
+int device_scan(rive_hip_context* ctx, const uint32_t* in, uint32_t n, uint32_t* out,
+                uint32_t* totalSlot)
+{
+    uint32_t blocks = std::max<uint32_t>((n + kScanTile - 1) / kScanTile, 1);
+    HIP_TRY(ensure(ctx->scanPartials, (size_t)blocks * 4));
+    scan_reduce_kernel<<<blocks, kScanBlock, 0, ctx->stream>>>(in, n, P<uint32_t>(ctx->scanPartials));
+    scan_partials_kernel<<<1, 1024, 0, ctx->stream>>>(P<uint32_t>(ctx->scanPartials), blocks, totalSlot);
+    scan_apply_kernel<<<blocks, kScanBlock, 0, ctx->stream>>>(in, n, P<uint32_t>(ctx->scanPartials),
+                                                               totalSlot, out);
+    HIP_TRY(hipGetLastError());
+    return RIVE_HIP_OK;
+}
+
+int read_scalar(rive_hip_context* ctx, const uint32_t* devSlot, uint32_t* value)
+{
+    HIP_TRY(hipMemcpyAsync(ctx->hostScalars, devSlot, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    *value = ctx->hostScalars[0];
+    return RIVE_HIP_OK;
+}
+
+// Raster half of the pipeline, shared by both flush modes.  Inputs: the
+// tessellation texture, contours, paths, paints and per-draw ranges.
+int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const SpanV* spans,
+                    uint32_t spanCount, const ContourV* contours, uint32_t contourCount, uint32_t tessRows,
+                    DrawTess* drawTess, uint32_t drawCount, uint32_t totalPrims)
+{
+    rive_hip_target* tgt = d->target;
+    const int W = (int)tgt->width, H = (int)tgt->height;
+    const int tilesX = (W + kTile - 1) / kTile, tilesY = (H + kTile - 1) / kTile;
+    uint32_t* scal = P<uint32_t>(ctx->scalars);
+    const uint32_t pathCount = d->path_count;
+    const PathDataV* paths = reinterpret_cast<const PathDataV*>(ctx->bufs[RIVE_HIP_BUFFER_PATH].dev);
+    const uint2* paints = reinterpret_cast<const uint2*>(ctx->bufs[RIVE_HIP_BUFFER_PAINT].dev);
+    const float* aux = reinterpret_cast<const float*>(ctx->bufs[RIVE_HIP_BUFFER_PAINT_AUX].dev);
+    const uint32_t gradRows = std::max<uint32_t>(d->grad_simple_rows + d->grad_complex_rows, 1);
+
+    // ---- color ramps
+    HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
+    HIP_TRY(ensure(ctx->ramps, (size_t)gradRows * kGradWidth * 4));
+    if (d->grad_span_count > 0)
+    {
+        HIP_TRY(hipMemsetAsync(ctx->ramps.ptr, 0, (size_t)gradRows * kGradWidth * 4, ctx->stream));
+        ramp_kernel<<<d->grad_span_count, 64, 0, ctx->stream>>>(
+            reinterpret_cast<const uint4*>(ctx->bufs[RIVE_HIP_BUFFER_GRAD_SPAN].dev),
+            d->grad_span_count, gradRows, P<uint32_t>(ctx->ramps));
+    }
+    ctx->lastGradRows = gradRows;
+    HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
+
+    // ---- tessellation: texel counts -> scan -> one thread per texel
+    const uint32_t texels = tessRows * kTessTextureWidth;
+    HIP_TRY(ensure(ctx->tess, (size_t)texels * sizeof(TessTexel)));
+    HIP_TRY(hipMemsetAsync(ctx->tess.ptr, 0, (size_t)texels * sizeof(TessTexel), ctx->stream));
+    if (spanCount > 0)
+    {
+        HIP_TRY(ensure(ctx->spanTexelCounts, (size_t)spanCount * 4));
+        HIP_TRY(ensure(ctx->spanTexelOffsets, (size_t)(spanCount + 1) * 4));
+        span_texel_count_kernel<<<(spanCount + 255) / 256, 256, 0, ctx->stream>>>(
+            spans, spanCount, tessRows, P<uint32_t>(ctx->spanTexelCounts));
+        int rc = device_scan(ctx, P<uint32_t>(ctx->spanTexelCounts), spanCount,
+                             P<uint32_t>(ctx->spanTexelOffsets), scal + 0);
+        if (rc)
+            return rc;
+        uint32_t used = 0;
+        rc = read_scalar(ctx, scal + 0, &used);
+        if (rc)
+            return rc;
+        if (used > 0)
+            tessellate_kernel<<<(used + 255) / 256, 256, 0, ctx->stream>>>(
+                spans, spanCount, P<uint32_t>(ctx->spanTexelOffsets), used, P<TessTexel>(ctx->tess));
+        ctx->lastTexels = texels;
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
+
+    // ---- expansion + binning
+    HIP_TRY(ensure(ctx->cells, (size_t)(pathCount + 1) * sizeof(PathCells)));
+    cells_init_kernel<<<(pathCount + 1 + 255) / 256, 256, 0, ctx->stream>>>(P<PathCells>(ctx->cells),
+                                                                             pathCount + 1);
+    HIP_TRY(ensure(ctx->prims, (size_t)std::max<uint32_t>(totalPrims, 1) * sizeof(Prim)));
+    if (totalPrims > 0)
+        expand_kernel<<<(totalPrims + 255) / 256, 256, 0, ctx->stream>>>(
+            P<TessTexel>(ctx->tess), (int)texels, contours, contourCount, paths, drawTess, drawCount,
+            totalPrims, tilesX,
+            tilesY, P<Prim>(ctx->prims), P<PathCells>(ctx->cells));
+    cells_layout_kernel<<<1, 1024, 0, ctx->stream>>>(P<PathCells>(ctx->cells), pathCount + 1, tilesX,
+                                                     tilesY, scal + 1);
+    HIP_TRY(hipGetLastError());
+    uint32_t totalCells = 0;
+    int rc = read_scalar(ctx, scal + 1, &totalCells);
+    if (rc)
+        return rc;
+    HIP_TRY(ensure(ctx->cellCount, (size_t)(totalCells + 1) * 4));
+    HIP_TRY(ensure(ctx->cellOffset, (size_t)(totalCells + 1) * 4));
+    HIP_TRY(ensure(ctx->cellCursor, (size_t)(totalCells + 1) * 4));
+    HIP_TRY(ensure(ctx->cellBackdrop, (size_t)(totalCells + 1) * 4));
+    HIP_TRY(hipMemsetAsync(ctx->cellCount.ptr, 0, (size_t)(totalCells + 1) * 4, ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->cellCursor.ptr, 0, (size_t)(totalCells + 1) * 4, ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->cellBackdrop.ptr, 0, (size_t)(totalCells + 1) * 4, ctx->stream));
+    if (totalPrims > 0)
+        bin_count_kernel<<<(totalPrims + 255) / 256, 256, 0, ctx->stream>>>(
+            P<Prim>(ctx->prims), totalPrims, P<PathCells>(ctx->cells), P<uint32_t>(ctx->cellCount),
+            P<int32_t>(ctx->cellBackdrop));
+    rc = device_scan(ctx, P<uint32_t>(ctx->cellCount), totalCells, P<uint32_t>(ctx->cellOffset), scal + 2);
+    if (rc)
+        return rc;
+    {
+        int maxRows = tilesY;
+        uint32_t threads = (pathCount + 1) * (uint32_t)maxRows;
+        backdrop_prefix_kernel<<<(threads + 255) / 256, 256, 0, ctx->stream>>>(
+            P<PathCells>(ctx->cells), pathCount + 1, maxRows, P<int32_t>(ctx->cellBackdrop));
+    }
+    uint32_t totalPairs = 0;
+    rc = read_scalar(ctx, scal + 2, &totalPairs);
+    if (rc)
+        return rc;
+    HIP_TRY(ensure(ctx->binned, (size_t)std::max<uint32_t>(totalPairs, 1) * 4));
+    if (totalPrims > 0)
+        bin_scatter_kernel<<<(totalPrims + 255) / 256, 256, 0, ctx->stream>>>(
+            P<Prim>(ctx->prims), totalPrims, P<PathCells>(ctx->cells), P<uint32_t>(ctx->cellOffset),
+            P<uint32_t>(ctx->cellCursor), P<uint32_t>(ctx->binned));
+    HIP_TRY(ensure(ctx->pathRaster, (size_t)(pathCount + 1) * sizeof(PathRaster)));
+    path_raster_kernel<<<(pathCount + 1 + 255) / 256, 256, 0, ctx->stream>>>(
+        paths, paints, aux, pathCount, gradRows, P<PathRaster>(ctx->pathRaster));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
+
+    // ---- raster + resolve
+    raster_kernel<<<tilesX * tilesY, 256, 0, ctx->stream>>>(
+        P<PathCells>(ctx->cells), P<PathRaster>(ctx->pathRaster), pathCount,
+        P<uint32_t>(ctx->cellOffset), P<int32_t>(ctx->cellBackdrop), P<uint32_t>(ctx->binned),
+        P<Prim>(ctx->prims), P<uint32_t>(ctx->ramps), gradRows, W, H, tilesX,
+        d->load_action == 1 ? 1u : 0u, premul_clear(d->clear_color), d->dither, tgt->pixels);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
+    return RIVE_HIP_OK;
+}
+
+// Single-block prefix over draws: forward ranges and primitive bases.
+__global__ void draw_tess_kernel(const rive_hip_draw* __restrict__ draws, uint32_t n,
+                                 uint32_t pathCount, uint32_t texels, DrawTess* __restrict__ out,
+                                 uint32_t* __restrict__ totalPrims, uint32_t* __restrict__ unsupported)
+{
+    __shared__ uint32_t buf[1024];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0)
+        carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += blockDim.x)
+    {
+        uint32_t i = base + threadIdx.x;
+        DrawTess t{};
+        uint32_t cnt = 0;
+        if (i < n)
+        {
+            const rive_hip_draw d = draws[i];
+            t.pathID = d.path_id;
+            t.tessLocation = d.tess_location;
+            t.tessVertexCount = d.tess_vertex_count;
+            t.dirs = d.contour_directions;
+            t.isStroke = d.is_stroke;
+            switch (d.contour_directions)
+            {
+                case 0: // forward
+                    t.forwardBase = d.tess_location;
+                    t.forwardCount = d.tess_vertex_count;
+                    break;
+                case 2: // forwardThenReverse
+                    t.forwardBase = d.tess_location;
+                    t.forwardCount = d.tess_vertex_count / 2;
+                    break;
+                case 3: // reverseThenForward
+                    t.forwardBase = d.tess_location + d.tess_vertex_count / 2;
+                    t.forwardCount = d.tess_vertex_count / 2;
+                    break;
+                default: // reverse-only draws exist only in MSAA/atlas modes
+                    t.forwardBase = d.tess_location;
+                    t.forwardCount = 0;
+                    if (d.tess_vertex_count != 0)
+                        atomicAdd(unsupported, 1u);
+                    break;
+            }
+            // A fill drawn forward only would need positive-winding-only
+            // coverage; rasterOrdering fills are always double sided.
+            if (!d.is_stroke && d.contour_directions == 0 && d.tess_vertex_count != 0)
+                atomicAdd(unsupported, 1u);
+            // Reject records that would index outside the path or texel arrays.
+            if (d.path_id == 0 || d.path_id > pathCount ||
+                (uint64_t)t.forwardBase + t.forwardCount + 1 > texels)
+            {
+                atomicAdd(unsupported, 1u);
+                t.forwardCount = 0;
+                t.pathID = 0;
+            }
+            cnt = t.forwardCount;
+        }
+        buf[threadIdx.x] = cnt;
+        __syncthreads();
+        for (uint32_t off = 1; off < blockDim.x; off <<= 1)
+        {
+            uint32_t v = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += v;
+            __syncthreads();
+        }
+        if (i < n)
+        {
+            t.primBase = carry + buf[threadIdx.x] - cnt;
+            out[i] = t;
+        }
+        __syncthreads();
+        if (threadIdx.x == blockDim.x - 1)
+            carry += buf[threadIdx.x];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        *totalPrims = carry;
+}
+} // namespace
+
+#include "count_kernels.hip"
+
+extern "C" int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* d)
+{
+    if (ctx == nullptr || d == nullptr || d->target == nullptr)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    for (int k = 0; k < RIVE_HIP_BUFFER_KIND_COUNT; ++k)
+        if (ctx->bufs[k].mapped)
+            return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "flush with a mapped buffer");
+    auto need = [&](int kind, size_t bytes) { return ctx->bufs[kind].size >= bytes; };
+    const uint32_t pathRecords = d->path_count + 1;
+    if (!need(RIVE_HIP_BUFFER_PATH, pathRecords * sizeof(rive_hip_path_data)) ||
+        !need(RIVE_HIP_BUFFER_PAINT, pathRecords * sizeof(rive_hip_paint_data)) ||
+        !need(RIVE_HIP_BUFFER_PAINT_AUX, pathRecords * sizeof(rive_hip_paint_aux)) ||
+        !need(RIVE_HIP_BUFFER_GRAD_SPAN, (size_t)d->grad_span_count * sizeof(rive_hip_grad_span)) ||
+        !need(RIVE_HIP_BUFFER_DRAW, (size_t)d->draw_count * sizeof(rive_hip_draw)))
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "buffers smaller than the flush descriptor");
+    HIP_TRY(ensure(ctx->scalars, 64));
+    HIP_TRY(ensure(ctx->drawTess, (size_t)std::max<uint32_t>(d->draw_count, 1) * sizeof(DrawTess)));
+    HIP_TRY(hipMemsetAsync(ctx->scalars.ptr, 0, 64, ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream)); // flush start
+
+    const SpanV* spans;
+    uint32_t spanCount;
+    const ContourV* contours;
+    uint32_t tessRows;
+    if (d->mode == RIVE_HIP_FLUSH_SPANS)
+    {
+        if (!need(RIVE_HIP_BUFFER_TESS_SPAN, (size_t)d->tess_span_count * sizeof(rive_hip_tess_span)) ||
+            !need(RIVE_HIP_BUFFER_CONTOUR, (size_t)d->contour_count * sizeof(rive_hip_contour)))
+            return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "span/contour buffers too small");
+        spans = reinterpret_cast<const SpanV*>(ctx->bufs[RIVE_HIP_BUFFER_TESS_SPAN].dev);
+        spanCount = d->tess_span_count;
+        contours = reinterpret_cast<const ContourV*>(ctx->bufs[RIVE_HIP_BUFFER_CONTOUR].dev);
+        tessRows = std::max<uint32_t>((d->tess_vertex_count + kTessTextureWidth - 1) / kTessTextureWidth, 1);
+        ctx->lastContourCount = d->contour_count;
+        HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream)); // no counting pass
+    }
+    else if (d->mode == RIVE_HIP_FLUSH_GEOMETRY)
+    {
+        int rc = geometry_counting(ctx, d, &spans, &spanCount, &contours, &tessRows);
+        if (rc)
+            return rc;
+    }
+    else
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "unknown flush mode");
+
+    uint32_t* scal = P<uint32_t>(ctx->scalars);
+    draw_tess_kernel<<<1, 1024, 0, ctx->stream>>>(
+        reinterpret_cast<const rive_hip_draw*>(ctx->bufs[RIVE_HIP_BUFFER_DRAW].dev), d->draw_count,
+        d->path_count, tessRows * kTessTextureWidth, P<DrawTess>(ctx->drawTess), scal + 3, scal + 4);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(ctx->hostScalars, scal + 3, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    uint32_t totalPrims = ctx->hostScalars[0];
+    if (ctx->hostScalars[1] != 0)
+        return fail(RIVE_HIP_ERROR_UNSUPPORTED,
+                    "draw record out of range, or contour directions the raster-ordering "
+                    "pipeline does not use");
+    ctx->lastSpanCount = spanCount;
+    ctx->lastDrawCount = d->draw_count;
+    int rc = raster_pipeline(ctx, d, spans, spanCount, contours, ctx->lastContourCount, tessRows,
+                             P<DrawTess>(ctx->drawTess),
+                             d->draw_count, totalPrims);
+    if (rc)
+        return rc;
+    ctx->timed = true;
+    return RIVE_HIP_OK;
+}
+
+extern "C" int rive_hip_pass_times(rive_hip_context* ctx, float* ms, int n)
+{
+    if (ctx == nullptr || ms == nullptr || n < 0 || n > 6)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "bad arguments");
+    if (!ctx->timed)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "no flush recorded");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipEventSynchronize(ctx->ev[5]));
+    float t[6];
+    // ev: 6 start, 1 counted, 0 ramps start, 2 ramps end, 3 tess end, 4 bin end, 5 raster end
+    HIP_TRY(hipEventElapsedTime(&t[0], ctx->ev[0], ctx->ev[2]));
+    HIP_TRY(hipEventElapsedTime(&t[1], ctx->ev[6], ctx->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&t[2], ctx->ev[2], ctx->ev[3]));
+    HIP_TRY(hipEventElapsedTime(&t[3], ctx->ev[3], ctx->ev[4]));
+    HIP_TRY(hipEventElapsedTime(&t[4], ctx->ev[4], ctx->ev[5]));
+    HIP_TRY(hipEventElapsedTime(&t[5], ctx->ev[6], ctx->ev[5]));
+    for (int i = 0; i < n; ++i)
+        ms[i] = t[i];
+    return RIVE_HIP_OK;
+}
+
+extern "C" int rive_hip_debug_readback(rive_hip_context* ctx, int what, void* dst, size_t capacity,
+                                       size_t* size)
+{
+    if (ctx == nullptr)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "ctx is null");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const void* src = nullptr;
+    size_t bytes = 0;
+    switch (what)
+    {
+        case 0:
+            src = ctx->geoRecords.ptr ? ctx->geoRecords.ptr : ctx->bufs[RIVE_HIP_BUFFER_TESS_SPAN].dev;
+            bytes = (size_t)ctx->lastSpanCount * sizeof(rive_hip_tess_span);
+            break;
+        case 1:
+            src = ctx->geoCounts.ptr ? ctx->geoCounts.ptr : ctx->bufs[RIVE_HIP_BUFFER_CONTOUR].dev;
+            bytes = (size_t)ctx->lastContourCount * sizeof(rive_hip_contour);
+            break;
+        case 2:
+            src = ctx->tess.ptr;
+            bytes = (size_t)ctx->lastTexels * sizeof(TessTexel);
+            break;
+        case 3:
+            src = ctx->drawTess.ptr;
+            bytes = (size_t)ctx->lastDrawCount * sizeof(DrawTess);
+            break;
+        case 4:
+            src = ctx->ramps.ptr;
+            bytes = (size_t)ctx->lastGradRows * kGradWidth * 4;
+            break;
+        default:
+            return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "unknown readback");
+    }
+    if (size)
+        *size = bytes;
+    if (dst != nullptr && src != nullptr && bytes > 0)
+    {
+        HIP_TRY(hipMemcpyAsync(dst, src, std::min(bytes, capacity), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    return RIVE_HIP_OK;
+}
+
+extern "C" int rive_hip_render_stream(rive_hip_context* ctx, const rive_stream* stream,
+                                      rive_hip_target* target)
+{
+    (void)ctx;
+    (void)stream;
+    (void)target;
+    return fail(RIVE_HIP_ERROR_UNSUPPORTED, "front end not built yet");
+}

@@ -1,0 +1,108 @@
+// Shared device-side definitions of the gfx950 backend.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rivehip
+{
+// ---- constants of the reference's GPU data contract
+//      (renderer/src/shaders/constants.glsl, renderer/include/rive/renderer/gpu.hpp)
+constexpr uint32_t kTessTextureWidth = 2048;
+constexpr uint32_t kTessTextureWidthLog2 = 11;
+constexpr uint32_t kPatchSpan = 8; // kMidpointFanPatchSegmentSpan
+constexpr uint32_t kGradWidth = 512;
+
+constexpr uint32_t CONTOUR_ID_MASK = 0xffffu;
+constexpr uint32_t RETROFIT_TRI_STRIP_CONTOUR_FLAG = 1u << 31;
+constexpr uint32_t JOIN_TYPE_MASK = 7u << 26;
+constexpr uint32_t MITER_CLIP_JOIN_CONTOUR_FLAG = 5u << 26;
+constexpr uint32_t MITER_REVERT_JOIN_CONTOUR_FLAG = 4u << 26;
+constexpr uint32_t BEVEL_JOIN_CONTOUR_FLAG = 3u << 26;
+constexpr uint32_t ROUND_JOIN_CONTOUR_FLAG = 2u << 26;
+constexpr uint32_t FEATHER_JOIN_CONTOUR_FLAG = 1u << 26;
+constexpr uint32_t EMULATED_STROKE_CAP_CONTOUR_FLAG = 1u << 25;
+constexpr uint32_t NEGATE_PATH_FILL_COVERAGE_FLAG = 1u << 24;
+constexpr uint32_t MIRRORED_CONTOUR_CONTOUR_FLAG = 1u << 23;
+constexpr uint32_t JOIN_TANGENT_0_CONTOUR_FLAG = 1u << 22;
+constexpr uint32_t JOIN_TANGENT_INNER_CONTOUR_FLAG = 1u << 21;
+constexpr uint32_t LEFT_JOIN_CONTOUR_FLAG = 1u << 20;
+constexpr uint32_t RIGHT_JOIN_CONTOUR_FLAG = 1u << 19;
+
+constexpr uint32_t CLIP_UPDATE_PAINT_TYPE = 0u;
+constexpr uint32_t SOLID_COLOR_PAINT_TYPE = 1u;
+constexpr uint32_t LINEAR_GRADIENT_PAINT_TYPE = 2u;
+constexpr uint32_t RADIAL_GRADIENT_PAINT_TYPE = 3u;
+constexpr uint32_t PAINT_FLAG_NON_ZERO_FILL = 0x100u;
+constexpr uint32_t PAINT_FLAG_EVEN_ODD_FILL = 0x200u;
+
+constexpr uint32_t GRAD_SPAN_FLAG_LEFT_BORDER = 0x80000000u;
+constexpr uint32_t GRAD_SPAN_FLAG_RIGHT_BORDER = 0x40000000u;
+constexpr uint32_t GRAD_SPAN_FLAG_COMPLEX_BORDER = 0x20000000u;
+constexpr uint32_t GRAD_SPAN_FLAGS_MASK = 0xe0000000u;
+
+// Raster tiles: 16x16 pixels, one pixel per lane, four wave64s per tile.
+constexpr int kTileLog2 = 4;
+constexpr int kTile = 1 << kTileLog2;
+// Screen positions are snapped to 1/256 pixel for the exact raster.
+constexpr int kSubpixelLog2 = 8;
+
+struct TessTexel
+{
+    float x, y;
+    uint32_t z; // theta bits (radians, [0, 2pi))
+    uint32_t w; // contourIDWithFlags
+};
+
+// One tessellated segment expanded to screen space, fixed point (1/256 px).
+//   fill:   p = [I0, O0, I1, O1] (inner/outer AA points), cov unused
+//   stroke: p = [P0-, P0+, P1-, P1+], cov = (x, y) per point
+struct Prim
+{
+    int32_t p[8];
+    float cov[8];
+    uint32_t path;  // 1-based path ID
+    uint32_t flags; // kPrimFill / kPrimStroke / kPrimNegate / kPrimEmpty
+    uint32_t pad[2];
+};
+static_assert(sizeof(Prim) == 80, "Prim is 80 bytes");
+constexpr uint32_t kPrimFill = 1u;
+constexpr uint32_t kPrimStroke = 2u;
+constexpr uint32_t kPrimNegate = 4u;
+constexpr uint32_t kPrimEmpty = 8u;
+
+// Per-path raster parameters, unpacked from PathData/PaintData/PaintAux.
+struct PathRaster
+{
+    uint32_t flags;      // bit0 stroke, bit1 evenOdd, bit2 clockwise, bit3 clipUpdate
+    uint32_t paintType;
+    uint32_t clipID;     // draws: clip to read (0 none); clip updates: ID to write
+    uint32_t outerClipID;
+    float color[4];      // premultiplied solid color
+    float paintMatrix[6];
+    float gradSpan;      // > 1: complex row; else simple-ramp texel offset
+    uint32_t gradRow;
+};
+
+// Per-path tile-cell grid (bounding box in tiles).
+struct PathCells
+{
+    int32_t tx0, ty0, tx1, ty1; // [tx0, tx1) x [ty0, ty1); empty if tx0 >= tx1
+    uint32_t base;              // first cell index
+    uint32_t width;             // tx1 - tx0
+    uint32_t pad[2];
+};
+
+// One draw's tessellation range (the reference's DrawBatch, gpu.hpp).
+struct DrawTess
+{
+    uint32_t pathID;
+    uint32_t tessLocation;    // first texel of the draw's allocation
+    uint32_t tessVertexCount; // both directions
+    uint32_t dirs;            // ContourDirections
+    uint32_t isStroke;
+    uint32_t forwardBase;     // first forward texel
+    uint32_t forwardCount;    // forward texels
+    uint32_t primBase;        // first primitive slot
+};
+} // namespace rivehip

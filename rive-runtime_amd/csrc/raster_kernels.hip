@@ -1,0 +1,1265 @@
+// gfx950 kernels of the RenderContext::flush hot path, after counting:
+// color ramps, tessellation, segment expansion, binning and the tile raster.
+//
+// Geometry semantics follow the reference's shaders:
+//   tessellation      renderer/src/shaders/tessellate.glsl:61-567
+//   patch expansion   renderer/src/shaders/draw_path_common.glsl:275-789
+//   pixel resolve     renderer/src/shaders/draw_raster_order_path.frag:14-238
+//   color ramps       renderer/src/shaders/color_ramp.glsl:40-106
+//
+// The raster is not a triangle pipeline.  A midpoint-fan fill drawn forward
+// and mirrored with counterclockwise culling adds +coverage for clockwise fan
+// triangles and -coverage for counterclockwise ones, so its interior term is
+// the winding number of the polygon through the inner AA points.  Each 16x16
+// tile gets that winding number as
+//     W(q) = C_T + (crossings of the tile's left edge above q)
+//              + (crossings of q's row between the left edge and q)
+// where C_T, the winding number at the tile's top-left corner, comes from a
+// prefix sum of per-path corner crossings.  The 1px AA border quads and stroke
+// quads are evaluated per pixel as triangles.  Every inside/outside decision is
+// an exact integer predicate on a 1/256-pixel grid with a symbolic
+// perturbation tie rule, so tile decompositions agree exactly.
+#include "device_common.hpp"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace rivehip
+{
+// ---------------------------------------------------------------------------
+// Small helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
+
+__device__ __forceinline__ int64_t floor_div128(int64_t a) { return a >> 7; }
+__device__ __forceinline__ int64_t ceil_div128(int64_t a) { return -((-a) >> 7); }
+__device__ __forceinline__ int32_t clamp_thr(int64_t t)
+{
+    const int64_t lim = (int64_t)1 << 30;
+    return (int32_t)(t < -lim ? -lim : (t > lim ? lim : t));
+}
+
+// ---------------------------------------------------------------------------
+// Color ramps (color_ramp.glsl:40-106).  One block per GradientSpan.
+// ---------------------------------------------------------------------------
+__global__ void ramp_kernel(const uint4* __restrict__ spans, uint32_t spanCount, uint32_t rows,
+                            uint32_t* __restrict__ ramps)
+{
+    const uint32_t s = blockIdx.x;
+    if (s >= spanCount)
+        return;
+    uint4 gs = spans[s];
+    float x0 = float(gs.x & 0xffffu) * (1.f / 65536.f);
+    float x1 = float(gs.x >> 16) * (1.f / 65536.f);
+    uint32_t yflags = gs.y;
+    uint32_t y = yflags & ~GRAD_SPAN_FLAGS_MASK;
+    if (y >= rows)
+        return;
+    float left = x0, right = x1;
+    if (yflags & GRAD_SPAN_FLAG_LEFT_BORDER)
+        left = (yflags & GRAD_SPAN_FLAG_COMPLEX_BORDER) ? 0.f : x0 - 1.f / kGradWidth;
+    if (yflags & GRAD_SPAN_FLAG_RIGHT_BORDER)
+        right = (yflags & GRAD_SPAN_FLAG_COMPLEX_BORDER) ? 1.f : x1 + 1.f / kGradWidth;
+    float c0[4], c1[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        int shift = k < 3 ? 16 - 8 * k : 24;
+        c0[k] = float((gs.z >> shift) & 0xffu) * (1.f / 255.f);
+        c1[k] = float((gs.w >> shift) & 0xffu) * (1.f / 255.f);
+    }
+    for (uint32_t x = threadIdx.x; x < kGradWidth; x += blockDim.x)
+    {
+        float u = (float(x) + .5f) / kGradWidth;
+        if (!(u >= left && u < right))
+            continue;
+        float c[4];
+        if (u < x0)
+        {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                c[k] = c0[k];
+        }
+        else if (u >= x1)
+        {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                c[k] = c1[k];
+        }
+        else
+        {
+            float t = (u - x0) / (x1 - x0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                c[k] = (c1[k] - c0[k]) * t + c0[k];
+        }
+        uint32_t px = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+        {
+            float v = fminf(fmaxf(c[k], 0.f), 1.f);
+            px |= (uint32_t)floorf(v * 255.f + .5f) << (8 * k);
+        }
+        ramps[(size_t)y * kGradWidth + x] = px;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tessellation (tessellate.glsl).  One thread per texel of each span's first
+// rectangle.  Double-sided spans also carry a mirrored rectangle; the raster
+// reads forward texels only, so mirrored copies are not written.
+// ---------------------------------------------------------------------------
+struct SpanV
+{
+    float p[8];
+    float jt[2];
+    float y, ry;
+    int32_t x0x1, rx0x1;
+    uint32_t counts, flags;
+};
+
+__device__ __forceinline__ void span_first_rect(const SpanV& s, int32_t* lo, int32_t* hi,
+                                                int32_t* x0, int32_t* x1)
+{
+    *x0 = (int32_t)((uint32_t)s.x0x1 << 16) >> 16;
+    *x1 = s.x0x1 >> 16;
+    *lo = max(min(*x0, *x1), 0);
+    *hi = min(max(*x0, *x1), (int32_t)kTessTextureWidth);
+}
+
+__global__ void span_texel_count_kernel(const SpanV* __restrict__ spans, uint32_t n,
+                                        uint32_t rows, uint32_t* __restrict__ counts)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    SpanV s = spans[i];
+    int32_t lo, hi, x0, x1;
+    span_first_rect(s, &lo, &hi, &x0, &x1);
+    uint32_t c = 0;
+    if (s.y >= 0.f && s.y < float(rows) && hi > lo)
+        c = (uint32_t)(hi - lo);
+    counts[i] = c;
+}
+
+struct F2
+{
+    float x, y;
+};
+__device__ __forceinline__ F2 f2sub(F2 a, F2 b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ float f2dot(F2 a, F2 b) { return a.x * b.x + a.y * b.y; }
+__device__ __forceinline__ bool f2neq(F2 a, F2 b) { return a.x != b.x || a.y != b.y; }
+__device__ __forceinline__ F2 f2norm(F2 v)
+{
+    float l = sqrtf(v.x * v.x + v.y * v.y);
+    return {v.x / l, v.y / l};
+}
+__device__ __forceinline__ F2 f2mix(F2 a, F2 b, float t)
+{
+    return {(b.x - a.x) * t + a.x, (b.y - a.y) * t + a.y};
+}
+__device__ __forceinline__ float det2(F2 c0, F2 c1) { return c0.x * c1.y - c1.x * c0.y; }
+__device__ __forceinline__ float clampf(float x, float lo, float hi)
+{
+    return fminf(fmaxf(x, lo), hi);
+}
+__device__ __forceinline__ float cosine_between(F2 a, F2 b) // bezier_utils.glsl:24-31
+{
+    float ab = f2dot(a, b);
+    float ab2 = f2dot(a, a) * f2dot(b, b);
+    return ab2 == 0.f ? 1.f : clampf(ab * (1.f / sqrtf(ab2)), -1.f, 1.f);
+}
+__device__ __forceinline__ float atan2v(F2 v) // common.glsl:197-202
+{
+    v = f2norm(v);
+    float th = acosf(clampf(v.x, -1.f, 1.f));
+    return v.y >= 0.f ? th : -th;
+}
+
+__global__ void tessellate_kernel(const SpanV* __restrict__ spans, uint32_t spanCount,
+                                  const uint32_t* __restrict__ texelOffsets, uint32_t totalTexels,
+                                  TessTexel* __restrict__ tex)
+{
+    constexpr float PI_ = 3.14159265359f;
+    constexpr float TWO_PI = 6.28318530718f;
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= totalTexels)
+        return;
+    // Find the span: last s with texelOffsets[s] <= g.
+    uint32_t lo = 0, hi = spanCount;
+    while (hi - lo > 1)
+    {
+        uint32_t mid = (lo + hi) >> 1;
+        if (texelOffsets[mid] <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const SpanV s = spans[lo];
+    int32_t rlo, rhi, x0, x1;
+    span_first_rect(s, &rlo, &rhi, &x0, &x1);
+    const int32_t x = rlo + (int32_t)(g - texelOffsets[lo]);
+    const uint32_t row = (uint32_t)s.y;
+
+    F2 P0{s.p[0], s.p[1]}, P1{s.p[2], s.p[3]}, P2{s.p[4], s.p[5]}, P3{s.p[6], s.p[7]};
+    const uint32_t parametricSegmentCount = s.counts & 0x3ffu;
+    const uint32_t polarSegmentCount = (s.counts >> 10) & 0x3ffu;
+    const uint32_t joinSegmentCount = s.counts >> 20;
+    const uint32_t totalVertexCount = parametricSegmentCount + polarSegmentCount + joinSegmentCount - 1u;
+    const F2 joinTangent{s.jt[0], s.jt[1]};
+
+    // Vertex stage.
+    F2 t0 = f2sub(f2neq(P0, P1) ? P1 : f2neq(P1, P2) ? P2 : P3, P0);
+    F2 t1 = f2sub(P3, f2neq(P3, P2) ? P2 : f2neq(P2, P1) ? P1 : P0);
+    float radsPerPolarSegment = acosf(cosine_between(t0, t1)) / float(polarSegmentCount);
+    float turn = det2(f2sub(P2, P0), f2sub(P3, P1));
+    if (turn == 0.f)
+        turn = det2(t0, t1);
+    if (turn < 0.f)
+        radsPerPolarSegment = -radsPerPolarSegment;
+    uint32_t cf = s.flags;
+    float radsPerJoinSegment = 0.f;
+    if (joinSegmentCount > 1u)
+    {
+        float joinTheta = acosf(cosine_between(t1, joinTangent));
+        float joinSpan = float(joinSegmentCount);
+        if ((cf & (JOIN_TYPE_MASK | EMULATED_STROKE_CAP_CONTOUR_FLAG)) ==
+            (ROUND_JOIN_CONTOUR_FLAG | EMULATED_STROKE_CAP_CONTOUR_FLAG))
+            joinSpan -= 2.f;
+        radsPerJoinSegment = joinTheta / joinSpan;
+        if (det2(t1, joinTangent) < 0.f)
+            radsPerJoinSegment = -radsPerJoinSegment;
+    }
+    if (x1 < x0)
+        cf |= MIRRORED_CONTOUR_CONTOUR_FLAG;
+
+    // Fragment stage.
+    float px = float(x) + .5f;
+    float vertexIdx = fmaxf(floorf(float(totalVertexCount) - fabsf(float(x1) - px)), 0.f);
+    float parametric = float(parametricSegmentCount);
+    float joinCount = float(joinSegmentCount);
+    float rads = radsPerPolarSegment;
+    F2 p0 = P0, p1 = P1, p2 = P2, p3 = P3;
+    F2 ta = t0, tb = t1;
+    float mergedSegmentCount = float(totalVertexCount) - joinCount;
+    float mergedVertexID = vertexIdx;
+    if (mergedVertexID <= mergedSegmentCount)
+    {
+        cf &= ~JOIN_TYPE_MASK;
+    }
+    else
+    {
+        p0 = p1 = p2 = p3;
+        ta = t1;
+        tb = joinTangent;
+        parametric = 1.f;
+        mergedVertexID -= mergedSegmentCount;
+        mergedSegmentCount = joinCount;
+        rads = radsPerJoinSegment;
+        if ((cf & JOIN_TYPE_MASK) > ROUND_JOIN_CONTOUR_FLAG)
+        {
+            if (mergedVertexID < 2.5f)
+                cf |= JOIN_TANGENT_0_CONTOUR_FLAG;
+            if (mergedVertexID > 1.5f && mergedVertexID < 3.5f)
+                cf |= JOIN_TANGENT_INNER_CONTOUR_FLAG;
+        }
+        else if ((cf & EMULATED_STROKE_CAP_CONTOUR_FLAG) != 0u ||
+                 (cf & JOIN_TYPE_MASK) == FEATHER_JOIN_CONTOUR_FLAG)
+        {
+            mergedSegmentCount -= 2.f;
+            mergedVertexID -= 1.f;
+        }
+        cf |= rads < 0.f ? LEFT_JOIN_CONTOUR_FLAG : RIGHT_JOIN_CONTOUR_FLAG;
+    }
+    F2 tessCoord;
+    float theta = 0.f;
+    if (mergedVertexID == 0.f || mergedVertexID == mergedSegmentCount ||
+        (cf & JOIN_TYPE_MASK) > ROUND_JOIN_CONTOUR_FLAG)
+    {
+        bool isTan0 = mergedVertexID < mergedSegmentCount * .5f;
+        tessCoord = isTan0 ? p0 : p3;
+        theta = atan2v(isTan0 ? ta : tb);
+    }
+    else if ((cf & RETROFIT_TRI_STRIP_CONTOUR_FLAG) != 0u)
+    {
+        tessCoord = p0;
+    }
+    else
+    {
+        float T, polarT;
+        if (parametric == mergedSegmentCount)
+        {
+            T = mergedVertexID / parametric;
+            polarT = 0.f;
+        }
+        else
+        {
+            F2 C = f2sub(p1, p0);
+            F2 D = f2sub(p3, p0);
+            F2 E = f2sub(p2, p1);
+            F2 B = f2sub(E, C);
+            F2 A{-3.f * E.x + D.x, -3.f * E.y + D.y};
+            F2 B_{B.x * (parametric * 2.f), B.y * (parametric * 2.f)};
+            float pp = parametric * parametric;
+            F2 C_{C.x * pp, C.y * pp};
+            float lastParametricVertexID = 0.f;
+            float maxParametricVertexID = fminf(parametric - 1.f, mergedVertexID);
+            F2 tan0norm = f2norm(ta);
+            float negAbsRads = -fabsf(rads);
+            float maxRotation0 = (1.f + mergedVertexID) * fabsf(rads);
+            for (int p = 9; p >= 0; --p)
+            {
+                float testID = lastParametricVertexID + float(1 << p);
+                if (testID <= maxParametricVertexID)
+                {
+                    F2 tt{testID * A.x + B_.x, testID * A.y + B_.y};
+                    tt = F2{testID * tt.x + C_.x, testID * tt.y + C_.y};
+                    float cosRotation = f2dot(f2norm(tt), tan0norm);
+                    float maxRotation = fminf(testID * negAbsRads + maxRotation0, PI_);
+                    if (cosRotation >= cosf(maxRotation))
+                        lastParametricVertexID = testID;
+                }
+            }
+            float parametricT = lastParametricVertexID / parametric;
+            float lastPolarVertexID = mergedVertexID - lastParametricVertexID;
+            float theta0 = acosf(clampf(tan0norm.x, -1.f, 1.f));
+            theta0 = tan0norm.y >= 0.f ? theta0 : -theta0;
+            theta = lastPolarVertexID * rads + theta0;
+            F2 nrm{sinf(theta), -cosf(theta)};
+            float a = f2dot(nrm, A), b_over_2 = f2dot(nrm, B), c = f2dot(nrm, C);
+            float discr_over_4 = fmaxf(b_over_2 * b_over_2 - a * c, 0.f);
+            float q = sqrtf(discr_over_4);
+            if (b_over_2 > 0.f)
+                q = -q;
+            q -= b_over_2;
+            float _5qa = -.5f * q * a;
+            bool useQA = fabsf(q * q + _5qa) < fabsf(a * c + _5qa);
+            float rs = useQA ? q : c, rt = useQA ? a : q;
+            polarT = rt != 0.f ? rs / rt : 0.f;
+            polarT = clampf(polarT, 0.f, 1.f);
+            if (lastPolarVertexID == 0.f)
+                polarT = 0.f;
+            T = fmaxf(parametricT, polarT);
+        }
+        F2 ab = f2mix(p0, p1, T);
+        F2 bc = f2mix(p1, p2, T);
+        F2 cd = f2mix(p2, p3, T);
+        F2 abc = f2mix(ab, bc, T);
+        F2 bcd = f2mix(bc, cd, T);
+        tessCoord = f2mix(abc, bcd, T);
+        if (T != polarT)
+            theta = atan2v(f2sub(bcd, abc));
+    }
+    TessTexel o;
+    o.x = tessCoord.x;
+    o.y = tessCoord.y;
+    if ((cf & JOIN_TYPE_MASK) == FEATHER_JOIN_CONTOUR_FLAG)
+        o.z = ((uint32_t)mergedSegmentCount << 16) | (uint32_t)mergedVertexID;
+    else
+        o.z = f2u(theta - TWO_PI * floorf(theta / TWO_PI));
+    o.w = cf;
+    tex[(size_t)row * kTessTextureWidth + (uint32_t)x] = o;
+}
+
+// ---------------------------------------------------------------------------
+// Segment expansion: the draw vertex shader (draw_path_common.glsl:275-789)
+// evaluated at both ends of every forward tessellation segment.
+// ---------------------------------------------------------------------------
+struct PathDataV
+{
+    float m[6];
+    float strokeRadius, featherRadius;
+    uint32_t z;
+    float at[3];
+    uint32_t cr[4];
+};
+struct ContourV
+{
+    float mx, my;
+    uint32_t pathID, vertexIndex0;
+};
+
+__device__ __forceinline__ float signf_(float x) { return x > 0.f ? 1.f : (x < 0.f ? -1.f : 0.f); }
+__device__ __forceinline__ float manhattan_pixel_width(const float M[4], float nx, float ny)
+{
+    float vx = M[0] * nx + M[2] * ny;
+    float vy = M[1] * nx + M[3] * ny;
+    return (fabsf(vx) + fabsf(vy)) * (1.f / (vx * vx + vy * vy));
+}
+__device__ __forceinline__ int32_t snap(float v)
+{
+    float c = fminf(fmaxf(v * 256.f, -8388608.f), 8388607.f);
+    if (isnan(c))
+        c = 0.f;
+    return __float2int_rn(c);
+}
+
+struct VsOut
+{
+    float x, y, cx, cy;
+};
+
+// Evaluates one patch vertex: tessellation texel `k` (with flags `cf`) and
+// outset `outset` (+1/-1).  Fill coverage is implied (1 inner, 0 outer).
+__device__ VsOut draw_vertex(const TessTexel* __restrict__ tex, int texels, int k, uint32_t cf,
+                             float outset, const PathDataV& pd)
+{
+    constexpr float PI_ = 3.14159265359f;
+    constexpr float TWO_PI = 6.28318530718f;
+    const float AA_RADIUS = .5f;
+    TessTexel tv = tex[k];
+    const float M[4] = {pd.m[0], pd.m[1], pd.m[2], pd.m[3]};
+    float theta = u2f(tv.z);
+    float nx = sinf(theta), ny = -cosf(theta);
+    float offx, offy, covx = 0.f, covy = 0.f;
+    float strokeRadius = pd.strokeRadius;
+    if (strokeRadius != 0.f)
+    {
+        float det = M[0] * M[3] - M[2] * M[1];
+        outset *= signf_(det);
+        if (cf & LEFT_JOIN_CONTOUR_FLAG)
+            outset = fminf(outset, 0.f);
+        if (cf & RIGHT_JOIN_CONTOUR_FLAG)
+            outset = fmaxf(outset, 0.f);
+        float aaRadius = manhattan_pixel_width(M, nx, ny) * AA_RADIUS;
+        float globalCoverage = 1.f;
+        if (aaRadius > strokeRadius)
+        {
+            globalCoverage = strokeRadius / aaRadius;
+            strokeRadius = aaRadius;
+        }
+        float vox = nx * (strokeRadius + aaRadius), voy = ny * (strokeRadius + aaRadius);
+        float x = outset * (strokeRadius + aaRadius);
+        covx = (1.f / (aaRadius * 2.f)) * (x + strokeRadius) + .5f;
+        covy = (1.f / (aaRadius * 2.f)) * (-x + strokeRadius) + .5f;
+        uint32_t joinType = cf & JOIN_TYPE_MASK;
+        if (joinType > ROUND_JOIN_CONTOUR_FLAG)
+        {
+            int peekDir = 2;
+            if ((cf & JOIN_TANGENT_0_CONTOUR_FLAG) == 0u)
+                peekDir = -peekDir;
+            if ((cf & MIRRORED_CONTOUR_CONTOUR_FLAG) != 0u)
+                peekDir = -peekDir;
+            float otherTheta = u2f(tex[min(max(k + peekDir, 0), texels - 1)].z);
+            float joinAngle = fabsf(otherTheta - theta);
+            if (joinAngle > PI_)
+                joinAngle = TWO_PI - joinAngle;
+            bool isTan0 = (cf & JOIN_TANGENT_0_CONTOUR_FLAG) != 0u;
+            bool isLeftJoin = (cf & LEFT_JOIN_CONTOUR_FLAG) != 0u;
+            float bisectTheta = joinAngle * (isTan0 == isLeftJoin ? -.5f : .5f) + theta;
+            float bx = sinf(bisectTheta), by = -cosf(bisectTheta);
+            float bisectPixelWidth = manhattan_pixel_width(M, bx, by);
+            float miterRatio = cosf(joinAngle * .5f);
+            float clipRadius;
+            if ((joinType == MITER_CLIP_JOIN_CONTOUR_FLAG) ||
+                (joinType == MITER_REVERT_JOIN_CONTOUR_FLAG && miterRatio >= .25f))
+            {
+                float miterInverseLimit = (cf & EMULATED_STROKE_CAP_CONTOUR_FLAG) != 0u ? 1.f : .25f;
+                clipRadius = strokeRadius * (1.f / fmaxf(miterRatio, miterInverseLimit));
+            }
+            else
+            {
+                clipRadius = strokeRadius * miterRatio + bisectPixelWidth * .5f;
+            }
+            float clipAARadius = clipRadius + bisectPixelWidth * AA_RADIUS;
+            if ((cf & JOIN_TANGENT_INNER_CONTOUR_FLAG) != 0u)
+            {
+                float strokeAARadius = strokeRadius + aaRadius;
+                float slop = aaRadius * .125f;
+                if (strokeAARadius <= clipAARadius * miterRatio + slop)
+                {
+                    float miterAARadius = strokeAARadius * (1.f / miterRatio);
+                    vox = bx * miterAARadius;
+                    voy = by * miterAARadius;
+                }
+                else
+                {
+                    float bax = bx * clipAARadius, bay = by * clipAARadius;
+                    float k0 = vox * vox + voy * voy, k1 = bax * bax + bay * bay;
+                    float a = vox, b = voy, c = bax, d = bay;
+                    float idet = 1.f / (a * d - c * b);
+                    float nvx = (k0 * d - k1 * b) * idet;
+                    float nvy = (-k0 * c + k1 * a) * idet;
+                    vox = nvx;
+                    voy = nvy;
+                }
+            }
+            float ptx = fabsf(outset) * vox, pty = fabsf(outset) * voy;
+            float clipDistance =
+                (clipAARadius - (ptx * bx + pty * by)) / (bisectPixelWidth * (AA_RADIUS * 2.f));
+            if ((cf & LEFT_JOIN_CONTOUR_FLAG) != 0u)
+                covy = clipDistance;
+            else
+                covx = clipDistance;
+        }
+        covx *= globalCoverage;
+        covy *= globalCoverage;
+        covy = fmaxf(covy, 1e-4f);
+        float px_ = outset * vox, py_ = outset * voy;
+        offx = M[0] * px_ + M[2] * py_;
+        offy = M[1] * px_ + M[3] * py_;
+    }
+    else
+    {
+        float det = M[0] * M[3] - M[2] * M[1];
+        float idet = 1.f / det;
+        float vx = outset * nx, vy = outset * ny;
+        float rx = (vx * M[3] + vy * -M[1]) * idet;
+        float ry = (vx * -M[2] + vy * M[0]) * idet;
+        offx = signf_(rx) * AA_RADIUS;
+        offy = signf_(ry) * AA_RADIUS;
+    }
+    VsOut o;
+    o.x = (M[0] * tv.x + M[2] * tv.y) + offx + pd.m[4];
+    o.y = (M[1] * tv.x + M[3] * tv.y) + offy + pd.m[5];
+    o.cx = covx;
+    o.cy = covy;
+    return o;
+}
+
+__device__ __forceinline__ int64_t orient64(int32_t ax, int32_t ay, int32_t bx, int32_t by, int32_t qx,
+                                            int32_t qy)
+{
+    return (int64_t)(bx - ax) * (int64_t)(qy - ay) - (int64_t)(by - ay) * (int64_t)(qx - ax);
+}
+
+__global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels,
+                              const ContourV* __restrict__ contours, uint32_t contourCount,
+                              const PathDataV* __restrict__ paths, const DrawTess* __restrict__ draws,
+                              uint32_t drawCount, uint32_t totalPrims, int canvasTilesX,
+                              int canvasTilesY, Prim* __restrict__ prims,
+                              PathCells* __restrict__ cells)
+{
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= totalPrims)
+        return;
+    uint32_t lo = 0, hi = drawCount;
+    while (hi - lo > 1)
+    {
+        uint32_t mid = (lo + hi) >> 1;
+        if (draws[mid].primBase <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const DrawTess d = draws[lo];
+    const int i = min((int)(d.forwardBase + (g - d.primBase)), texels - 2);
+    Prim out;
+    out.path = d.pathID;
+    out.pad[0] = out.pad[1] = 0;
+    const TessTexel ti = tex[i];
+    const uint32_t cfi = ti.w;
+    const uint32_t contourID = min(max(cfi & CONTOUR_ID_MASK, 1u), max(contourCount, 1u));
+    const ContourV cd = contours[contourID - 1];
+    const PathDataV pd = paths[d.pathID];
+    // next(i): the next vertex of the contour, wrapping closed contours to
+    // vertexIndex0 (draw_path_common.glsl:336-388).
+    int j = i + 1;
+    uint32_t cfj = tex[j].w;
+    if ((cfj & (MIRRORED_CONTOUR_CONTOUR_FLAG | 0xffffu)) !=
+        (cfi & (MIRRORED_CONTOUR_CONTOUR_FLAG | 0xffffu)))
+    {
+        bool isClosed = pd.strokeRadius == 0.f || cd.mx != 0.f;
+        j = isClosed ? min((int)cd.vertexIndex0, texels - 1) : i;
+        cfj = (tex[j].w & ~MIRRORED_CONTOUR_CONTOUR_FLAG) | (cfi & MIRRORED_CONTOUR_CONTOUR_FLAG);
+    }
+    const bool stroke = pd.strokeRadius != 0.f;
+    VsOut a0 = draw_vertex(tex, texels, i, cfi, -1.f, pd);
+    VsOut a1 = draw_vertex(tex, texels, i, cfi, +1.f, pd);
+    VsOut b0 = draw_vertex(tex, texels, j, cfj, -1.f, pd);
+    VsOut b1 = draw_vertex(tex, texels, j, cfj, +1.f, pd);
+    // fill: [I0, O0, I1, O1]; stroke: [P0-, P0+, P1-, P1+]
+    out.p[0] = snap(a0.x);
+    out.p[1] = snap(a0.y);
+    out.p[2] = snap(a1.x);
+    out.p[3] = snap(a1.y);
+    out.p[4] = snap(b0.x);
+    out.p[5] = snap(b0.y);
+    out.p[6] = snap(b1.x);
+    out.p[7] = snap(b1.y);
+    out.cov[0] = a0.cx;
+    out.cov[1] = a0.cy;
+    out.cov[2] = a1.cx;
+    out.cov[3] = a1.cy;
+    out.cov[4] = b0.cx;
+    out.cov[5] = b0.cy;
+    out.cov[6] = b1.cx;
+    out.cov[7] = b1.cy;
+    uint32_t flags = stroke ? kPrimStroke : kPrimFill;
+    if (!stroke && ((cfi & NEGATE_PATH_FILL_COVERAGE_FLAG) != 0u))
+        flags |= kPrimNegate;
+    bool empty;
+    if (stroke)
+    {
+        // Triangles (P0-, P0+, P1-) and (P1-, P0+, P1+); clockwise only.
+        int64_t A1 = orient64(out.p[0], out.p[1], out.p[2], out.p[3], out.p[4], out.p[5]);
+        int64_t A2 = orient64(out.p[4], out.p[5], out.p[2], out.p[3], out.p[6], out.p[7]);
+        empty = A1 <= 0 && A2 <= 0;
+    }
+    else
+    {
+        int64_t A1 = orient64(out.p[0], out.p[1], out.p[2], out.p[3], out.p[4], out.p[5]);
+        int64_t A2 = orient64(out.p[4], out.p[5], out.p[2], out.p[3], out.p[6], out.p[7]);
+        empty = out.p[0] == out.p[4] && out.p[1] == out.p[5] && A1 == 0 && A2 == 0;
+    }
+    if (empty)
+        flags |= kPrimEmpty;
+    out.flags = flags;
+    prims[g] = out;
+    if (!empty)
+    {
+        int32_t minx = min(min(out.p[0], out.p[2]), min(out.p[4], out.p[6]));
+        int32_t maxx = max(max(out.p[0], out.p[2]), max(out.p[4], out.p[6]));
+        int32_t miny = min(min(out.p[1], out.p[3]), min(out.p[5], out.p[7]));
+        int32_t maxy = max(max(out.p[1], out.p[3]), max(out.p[5], out.p[7]));
+        // Unclamped union; cells_layout_kernel clips it to the canvas, so
+        // geometry off the canvas still widens the grid toward it.
+        (void)canvasTilesX;
+        (void)canvasTilesY;
+        PathCells* pc = &cells[d.pathID];
+        atomicMin(&pc->tx0, minx >> (kSubpixelLog2 + kTileLog2));
+        atomicMin(&pc->ty0, miny >> (kSubpixelLog2 + kTileLog2));
+        atomicMax(&pc->tx1, (maxx >> (kSubpixelLog2 + kTileLog2)) + 1);
+        atomicMax(&pc->ty1, (maxy >> (kSubpixelLog2 + kTileLog2)) + 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-path cell grids.
+// ---------------------------------------------------------------------------
+__global__ void cells_init_kernel(PathCells* __restrict__ cells, uint32_t n)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    PathCells c;
+    c.tx0 = c.ty0 = 0x7fffffff;
+    c.tx1 = c.ty1 = -0x7fffffff;
+    c.base = 0;
+    c.width = 0;
+    c.pad[0] = c.pad[1] = 0;
+    cells[i] = c;
+}
+
+// Single block: sizes each path's grid and assigns bases (exclusive scan).
+__global__ void cells_layout_kernel(PathCells* __restrict__ cells, uint32_t n, int tilesX, int tilesY,
+                                    uint32_t* __restrict__ totalCells)
+{
+    __shared__ uint32_t partial[1024];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0)
+        carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += blockDim.x)
+    {
+        uint32_t i = base + threadIdx.x;
+        uint32_t area = 0;
+        PathCells c;
+        if (i < n)
+        {
+            c = cells[i];
+            c.tx0 = max(c.tx0, 0);
+            c.ty0 = max(c.ty0, 0);
+            c.tx1 = min(c.tx1, tilesX);
+            c.ty1 = min(c.ty1, tilesY);
+            if (c.tx0 < c.tx1 && c.ty0 < c.ty1)
+                area = (uint32_t)((c.tx1 - c.tx0) * (c.ty1 - c.ty0));
+            else
+            {
+                c.tx0 = c.ty0 = 0;
+                c.tx1 = c.ty1 = 0;
+            }
+        }
+        partial[threadIdx.x] = area;
+        __syncthreads();
+        for (uint32_t off = 1; off < blockDim.x; off <<= 1)
+        {
+            uint32_t v = threadIdx.x >= off ? partial[threadIdx.x - off] : 0;
+            __syncthreads();
+            partial[threadIdx.x] += v;
+            __syncthreads();
+        }
+        if (i < n)
+        {
+            c.base = carry + partial[threadIdx.x] - area;
+            c.width = (uint32_t)(c.tx1 - c.tx0);
+            cells[i] = c;
+        }
+        __syncthreads();
+        if (threadIdx.x == blockDim.x - 1)
+            carry += partial[threadIdx.x];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        *totalCells = carry;
+}
+
+// ---------------------------------------------------------------------------
+// Binning.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void prim_tile_range(const Prim& p, const PathCells& c, int* tx0, int* ty0,
+                                                int* tx1, int* ty1)
+{
+    int32_t minx = min(min(p.p[0], p.p[2]), min(p.p[4], p.p[6]));
+    int32_t maxx = max(max(p.p[0], p.p[2]), max(p.p[4], p.p[6]));
+    int32_t miny = min(min(p.p[1], p.p[3]), min(p.p[5], p.p[7]));
+    int32_t maxy = max(max(p.p[1], p.p[3]), max(p.p[5], p.p[7]));
+    *tx0 = max(minx >> (kSubpixelLog2 + kTileLog2), c.tx0);
+    *ty0 = max(miny >> (kSubpixelLog2 + kTileLog2), c.ty0);
+    *tx1 = min((maxx >> (kSubpixelLog2 + kTileLog2)) + 1, c.tx1);
+    *ty1 = min((maxy >> (kSubpixelLog2 + kTileLog2)) + 1, c.ty1);
+}
+
+// "q right of edge a->b" for the ray-left crossing rule, with the sample point
+// perturbed by (+e, +e^2): s*D(q) >= 0 where s = +1 for upward edges.
+__device__ __forceinline__ bool right_of(int32_t ax, int32_t ay, int32_t bx, int32_t by, int64_t qx,
+                                         int64_t qy)
+{
+    int64_t D = (int64_t)(bx - ax) * (qy - ay) - (int64_t)(by - ay) * (qx - ax);
+    return by < ay ? D >= 0 : D <= 0;
+}
+
+__global__ void bin_count_kernel(const Prim* __restrict__ prims, uint32_t totalPrims,
+                                 const PathCells* __restrict__ cells, uint32_t* __restrict__ cellCount,
+                                 int32_t* __restrict__ cellBackdrop)
+{
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= totalPrims)
+        return;
+    const Prim p = prims[g];
+    if (p.flags & kPrimEmpty)
+        return;
+    const PathCells c = cells[p.path];
+    int tx0, ty0, tx1, ty1;
+    prim_tile_range(p, c, &tx0, &ty0, &tx1, &ty1);
+    for (int ty = ty0; ty < ty1; ++ty)
+        for (int tx = tx0; tx < tx1; ++tx)
+            atomicAdd(&cellCount[c.base + (uint32_t)(ty - c.ty0) * c.width + (uint32_t)(tx - c.tx0)], 1u);
+    if (!(p.flags & kPrimFill))
+        return;
+    // Corner crossings of the inner edge I0->I1 on tile-row lines.
+    const int32_t ax = p.p[0], ay = p.p[1], bx = p.p[4], by = p.p[5];
+    if (ay == by)
+        return;
+    const int s = by < ay ? 1 : -1;
+    const int32_t ymin = min(ay, by), ymax = max(ay, by);
+    const int unit = 1 << (kSubpixelLog2 + kTileLog2);
+    int tyA = max(c.ty0, (ymin + unit - 1) >> (kSubpixelLog2 + kTileLog2));
+    for (int ty = tyA; ty < c.ty1; ++ty)
+    {
+        int64_t Y = (int64_t)ty * unit;
+        if (!(ymin <= Y && Y < ymax))
+            break;
+        // Smallest column c with corner (c*unit, Y) right of the edge.
+        double xe = ax + (double)(bx - ax) * (double)(Y - ay) / (double)(by - ay);
+        int64_t cc = (int64_t)floor(xe / unit) + 1;
+        while (right_of(ax, ay, bx, by, (cc - 1) * unit, Y))
+            --cc;
+        while (!right_of(ax, ay, bx, by, cc * unit, Y))
+            ++cc;
+        int col = (int)max<int64_t>(cc, (int64_t)c.tx0);
+        if (col < c.tx1)
+            atomicAdd(&cellBackdrop[c.base + (uint32_t)(ty - c.ty0) * c.width + (uint32_t)(col - c.tx0)], s);
+    }
+}
+
+// Prefix-sum of corner crossings along each row of each path's grid.
+__global__ void backdrop_prefix_kernel(const PathCells* __restrict__ cells, uint32_t pathCount,
+                                       int maxRows, int32_t* __restrict__ cellBackdrop)
+{
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t path = g / (uint32_t)maxRows;
+    int row = (int)(g % (uint32_t)maxRows);
+    if (path >= pathCount)
+        return;
+    const PathCells c = cells[path];
+    if (row >= c.ty1 - c.ty0)
+        return;
+    int32_t* r = cellBackdrop + c.base + (uint32_t)row * c.width;
+    int32_t acc = 0;
+    for (uint32_t x = 0; x < c.width; ++x)
+    {
+        acc += r[x];
+        r[x] = acc;
+    }
+}
+
+__global__ void bin_scatter_kernel(const Prim* __restrict__ prims, uint32_t totalPrims,
+                                   const PathCells* __restrict__ cells,
+                                   const uint32_t* __restrict__ cellOffset,
+                                   uint32_t* __restrict__ cellCursor, uint32_t* __restrict__ binned)
+{
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= totalPrims)
+        return;
+    const Prim p = prims[g];
+    if (p.flags & kPrimEmpty)
+        return;
+    const PathCells c = cells[p.path];
+    int tx0, ty0, tx1, ty1;
+    prim_tile_range(p, c, &tx0, &ty0, &tx1, &ty1);
+    for (int ty = ty0; ty < ty1; ++ty)
+        for (int tx = tx0; tx < tx1; ++tx)
+        {
+            uint32_t cell = c.base + (uint32_t)(ty - c.ty0) * c.width + (uint32_t)(tx - c.tx0);
+            uint32_t slot = atomicAdd(&cellCursor[cell], 1u);
+            binned[cellOffset[cell] + slot] = g;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Tile raster.
+// ---------------------------------------------------------------------------
+
+// Half-plane test of edge a->b against a tile's pixels, as an integer
+// threshold on  k(i, j) = -dy*(2i+1) + dx*(2j+1),  where the edge function at
+// pixel (i, j) is D = D0 + 128*k and D0 is the edge function at the tile's
+// top-left corner.
+struct EdgeT
+{
+    int32_t A, B; // -dy, dx
+    int64_t D0;
+};
+__device__ __forceinline__ EdgeT make_edge(int32_t ax, int32_t ay, int32_t bx, int32_t by, int32_t X0,
+                                           int32_t Y0)
+{
+    EdgeT e;
+    int32_t dx = bx - ax, dy = by - ay;
+    e.A = -dy;
+    e.B = dx;
+    e.D0 = (int64_t)dx * (int64_t)(Y0 - ay) - (int64_t)dy * (int64_t)(X0 - ax);
+    return e;
+}
+// Threshold T such that "o * D(q) has positive perturbed sign" <=> o*k >= T.
+__device__ __forceinline__ int32_t inside_threshold(const EdgeT& e, int o)
+{
+    // perturbation sign of D at q+(e,e^2): -dy if dy != 0 else dx
+    int pert = e.A != 0 ? (e.A > 0 ? 1 : -1) : (e.B > 0 ? 1 : (e.B < 0 ? -1 : 0));
+    int64_t oD0 = o > 0 ? e.D0 : -e.D0;
+    // o*D0 + 128*(o*k) >= 0 (non-strict) or > 0 (strict)
+    int64_t t = (o * pert > 0) ? ceil_div128(-oD0) : floor_div128(-oD0) + 1;
+    return clamp_thr(t);
+}
+
+__device__ __forceinline__ float dither_noise(int x, int y)
+{
+    // common.glsl:269-274, scale 1/256, bias -1/512 (gpu.cpp:822-824).
+    float fx = float(x) + .5f, fy = float(y) + .5f;
+    float v = 0.06711056f * fx + 0.00583715f * fy;
+    float v1 = v - floorf(v);
+    float w = 52.9829189f * v1;
+    float v2 = w - floorf(w);
+    return v2 * (1.f / 256.f) + (-.5f / 256.f);
+}
+__device__ __forceinline__ float quantize8(float v)
+{
+    v = fminf(fmaxf(v, 0.f), 1.f);
+    return floorf(v * 255.f + .5f) * (1.f / 255.f);
+}
+
+__global__ __launch_bounds__(256) void raster_kernel(
+    const PathCells* __restrict__ cells, const PathRaster* __restrict__ raster, uint32_t pathCount,
+    const uint32_t* __restrict__ cellOffset, const int32_t* __restrict__ cellBackdrop,
+    const uint32_t* __restrict__ binned, const Prim* __restrict__ prims,
+    const uint32_t* __restrict__ ramps, uint32_t gradHeight, int width, int height, int tilesX,
+    uint32_t loadPreserve, uint32_t clearPremul, uint32_t dither, uint32_t* __restrict__ target)
+{
+    const int tile = blockIdx.x;
+    const int tx = tile % tilesX, ty = tile / tilesX;
+    const int li = threadIdx.x & (kTile - 1), lj = threadIdx.x >> kTileLog2;
+    const int px = tx * kTile + li, py = ty * kTile + lj;
+    const bool inside = px < width && py < height;
+    const int32_t X0 = tx << (kSubpixelLog2 + kTileLog2);
+    const int32_t Y0 = ty << (kSubpixelLog2 + kTileLog2);
+    const int32_t i2 = 2 * li + 1, j2 = 2 * lj + 1;
+    const int32_t yq = Y0 + (lj << kSubpixelLog2) + 128; // sample y, fixed point
+
+    float col[4];
+    {
+        uint32_t c = clearPremul;
+        if (loadPreserve && inside)
+            c = target[(size_t)py * width + px];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            col[k] = float((c >> (8 * k)) & 0xffu) * (1.f / 255.f);
+    }
+    float clipCov = 0.f;
+    uint32_t clipID = 0;
+    const float noise = dither ? dither_noise(px, py) : 0.f;
+
+    for (uint32_t path = 1; path <= pathCount; ++path)
+    {
+        const PathCells c = cells[path];
+        if (tx < c.tx0 || tx >= c.tx1 || ty < c.ty0 || ty >= c.ty1)
+            continue;
+        const uint32_t cell = c.base + (uint32_t)(ty - c.ty0) * c.width + (uint32_t)(tx - c.tx0);
+        const uint32_t off = cellOffset[cell];
+        const uint32_t n = cellOffset[cell + 1] - off;
+        const int32_t C = cellBackdrop[cell];
+        if (n == 0 && C == 0)
+            continue;
+        const PathRaster pr = raster[path];
+        const bool isStroke = pr.flags & 1u;
+        float coverage;
+        if (!isStroke)
+        {
+            int32_t wind = C;
+            int32_t borderFx = 0;
+            for (uint32_t k = 0; k < n; ++k)
+            {
+                const Prim p = prims[binned[off + k]];
+                const int neg = (p.flags & kPrimNegate) ? -1 : 1;
+                // Inner edge I0 -> I1: winding.
+                const EdgeT e = make_edge(p.p[0], p.p[1], p.p[4], p.p[5], X0, Y0);
+                const int32_t ke = e.A * i2 + e.B * j2;
+                if (p.p[1] != p.p[5])
+                {
+                    const int s = p.p[5] < p.p[1] ? 1 : -1;
+                    // R(q) <=> s*D(q) >= 0 <=> s*k >= ceil(-s*D0/128)
+                    const int32_t T = clamp_thr(ceil_div128(s > 0 ? -e.D0 : e.D0));
+                    const int32_t krow = e.B * j2;
+                    const bool Rq = s * ke >= T;
+                    const bool Rrow = s * krow >= T;
+                    const bool Rcorner = (s > 0 ? e.D0 : -e.D0) >= 0;
+                    const int32_t ymin = min(p.p[1], p.p[5]), ymax = max(p.p[1], p.p[5]);
+                    const int32_t xmin = min(p.p[0], p.p[4]), xmax = max(p.p[0], p.p[4]);
+                    const bool cross = xmin <= X0 && X0 < xmax;
+                    const bool inRange = ymin <= yq && yq < ymax;
+                    int32_t d = 0;
+                    if (cross)
+                        d += (int32_t)Rrow - (int32_t)Rcorner;
+                    if (inRange)
+                        d += (int32_t)Rq - (int32_t)Rrow;
+                    wind += s * d;
+                }
+                else if (p.p[0] != p.p[4])
+                {
+                    // A horizontal edge never meets a row, but where it crosses
+                    // the tile's left edge the winding along that edge steps
+                    // by sign(dx) (its neighbours' range ends no longer cancel).
+                    const int32_t xmin = min(p.p[0], p.p[4]), xmax = max(p.p[0], p.p[4]);
+                    if (xmin <= X0 && X0 < xmax)
+                    {
+                        const int32_t h = p.p[1];
+                        const int sg = p.p[4] > p.p[0] ? 1 : -1;
+                        wind += sg * ((int32_t)(yq >= h) - (int32_t)(Y0 >= h));
+                    }
+                }
+                // Border triangle T1 = (I0, O0, I1), coverage 1 - lambda(O0).
+                const int64_t A1 = orient64(p.p[0], p.p[1], p.p[2], p.p[3], p.p[4], p.p[5]);
+                if (A1 != 0)
+                {
+                    const int o = A1 > 0 ? 1 : -1;
+                    const EdgeT e01 = make_edge(p.p[0], p.p[1], p.p[2], p.p[3], X0, Y0);
+                    const EdgeT e12 = make_edge(p.p[2], p.p[3], p.p[4], p.p[5], X0, Y0);
+                    const EdgeT e20 = make_edge(p.p[4], p.p[5], p.p[0], p.p[1], X0, Y0);
+                    const bool in = o * (e01.A * i2 + e01.B * j2) >= inside_threshold(e01, o) &&
+                                    o * (e12.A * i2 + e12.B * j2) >= inside_threshold(e12, o) &&
+                                    o * (e20.A * i2 + e20.B * j2) >= inside_threshold(e20, o);
+                    if (in)
+                    {
+                        // D of edge I1->I0 at q = -(D of I0->I1)
+                        float De = (float)e.D0 + 128.f * (float)ke;
+                        float cv = 1.f + De / (float)A1;
+                        borderFx += neg * o * __float2int_rn(cv * 65536.f);
+                    }
+                }
+                // Border triangle T2 = (I1, O0, O1), coverage lambda(I1).
+                const int64_t A2 = orient64(p.p[4], p.p[5], p.p[2], p.p[3], p.p[6], p.p[7]);
+                if (A2 != 0)
+                {
+                    const int o = A2 > 0 ? 1 : -1;
+                    const EdgeT f01 = make_edge(p.p[4], p.p[5], p.p[2], p.p[3], X0, Y0);
+                    const EdgeT f12 = make_edge(p.p[2], p.p[3], p.p[6], p.p[7], X0, Y0);
+                    const EdgeT f20 = make_edge(p.p[6], p.p[7], p.p[4], p.p[5], X0, Y0);
+                    const int32_t k12 = f12.A * i2 + f12.B * j2;
+                    const bool in = o * (f01.A * i2 + f01.B * j2) >= inside_threshold(f01, o) &&
+                                    o * k12 >= inside_threshold(f12, o) &&
+                                    o * (f20.A * i2 + f20.B * j2) >= inside_threshold(f20, o);
+                    if (in)
+                    {
+                        float D12 = (float)f12.D0 + 128.f * (float)k12;
+                        float cv = D12 / (float)A2;
+                        borderFx += neg * o * __float2int_rn(cv * 65536.f);
+                    }
+                }
+            }
+            float count = (float)wind + (float)borderFx * (1.f / 65536.f);
+            if (pr.flags & 4u) // clockwise
+                coverage = fminf(fmaxf(count, 0.f), 1.f);
+            else
+            {
+                coverage = fabsf(count);
+                if (pr.flags & 2u) // even-odd
+                {
+                    float h = coverage * .5f;
+                    coverage = 1.f - fabsf((h - floorf(h)) * 2.f + -1.f);
+                }
+                coverage = fminf(coverage, 1.f);
+            }
+        }
+        else
+        {
+            float m = 0.f;
+            for (uint32_t k = 0; k < n; ++k)
+            {
+                const Prim p = prims[binned[off + k]];
+                // T1 = (P0-, P0+, P1-): vertices 0, 1, 2; T2 = (P1-, P0+, P1+): 2, 1, 3.
+                const int64_t A1 = orient64(p.p[0], p.p[1], p.p[2], p.p[3], p.p[4], p.p[5]);
+                const int64_t A2 = orient64(p.p[4], p.p[5], p.p[2], p.p[3], p.p[6], p.p[7]);
+                const EdgeT e01 = make_edge(p.p[0], p.p[1], p.p[2], p.p[3], X0, Y0);
+                const EdgeT e12 = make_edge(p.p[2], p.p[3], p.p[4], p.p[5], X0, Y0);
+                const EdgeT e20 = make_edge(p.p[4], p.p[5], p.p[0], p.p[1], X0, Y0);
+                const EdgeT e13 = make_edge(p.p[2], p.p[3], p.p[6], p.p[7], X0, Y0);
+                const EdgeT e32 = make_edge(p.p[6], p.p[7], p.p[4], p.p[5], X0, Y0);
+                const int32_t k01 = e01.A * i2 + e01.B * j2;
+                const int32_t k12 = e12.A * i2 + e12.B * j2;
+                const int32_t k20 = e20.A * i2 + e20.B * j2;
+                const int32_t k13 = e13.A * i2 + e13.B * j2;
+                const int32_t k32 = e32.A * i2 + e32.B * j2;
+                if (A1 > 0 && k01 >= inside_threshold(e01, 1) && k12 >= inside_threshold(e12, 1) &&
+                    k20 >= inside_threshold(e20, 1))
+                {
+                    float inv = 1.f / (float)A1;
+                    float l0 = ((float)e12.D0 + 128.f * (float)k12) * inv;
+                    float l1 = ((float)e20.D0 + 128.f * (float)k20) * inv;
+                    float l2 = ((float)e01.D0 + 128.f * (float)k01) * inv;
+                    float cx = l0 * p.cov[0] + l1 * p.cov[2] + l2 * p.cov[4];
+                    float cy = l0 * p.cov[1] + l1 * p.cov[3] + l2 * p.cov[5];
+                    m = fmaxf(fminf(cx, cy), m);
+                }
+                // T2 edges: P1- -> P0+ is the reverse of e12.
+                if (A2 > 0 && -k12 >= inside_threshold(EdgeT{-e12.A, -e12.B, -e12.D0}, 1) &&
+                    k13 >= inside_threshold(e13, 1) && k32 >= inside_threshold(e32, 1))
+                {
+                    float inv = 1.f / (float)A2;
+                    float l0 = ((float)e13.D0 + 128.f * (float)k13) * inv;      // P1-
+                    float l1 = ((float)e32.D0 + 128.f * (float)k32) * inv;      // P0+
+                    float l2 = -((float)e12.D0 + 128.f * (float)k12) * inv;     // P1+
+                    float cx = l0 * p.cov[4] + l1 * p.cov[2] + l2 * p.cov[6];
+                    float cy = l0 * p.cov[5] + l1 * p.cov[3] + l2 * p.cov[7];
+                    m = fmaxf(fminf(cx, cy), m);
+                }
+            }
+            coverage = fminf(m, 1.f);
+        }
+
+        if (pr.flags & 8u) // clip update
+        {
+            if (pr.outerClipID != 0u)
+                coverage = fminf(coverage, clipID == pr.outerClipID ? clipCov : 0.f);
+            clipCov = coverage;
+            clipID = pr.clipID;
+            continue;
+        }
+        if (pr.clipID != 0u)
+            coverage = clipID == pr.clipID ? fminf(clipCov, coverage) : 0.f;
+        float src[4];
+        if (pr.paintType == SOLID_COLOR_PAINT_TYPE)
+        {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                src[k] = pr.color[k] * coverage;
+        }
+        else
+        {
+            float fx = float(px) + .5f, fy = float(py) + .5f;
+            float pcx = pr.paintMatrix[0] * fx + pr.paintMatrix[2] * fy + pr.paintMatrix[4];
+            float pcy = pr.paintMatrix[1] * fx + pr.paintMatrix[3] * fy + pr.paintMatrix[5];
+            float t = pr.paintType == LINEAR_GRADIENT_PAINT_TYPE ? pcx : sqrtf(pcx * pcx + pcy * pcy);
+            t = fminf(fmaxf(t, 0.f), 1.f);
+            float u = pr.gradSpan > 1.f ? (1.f - 1.f / 512.f) * t + (.5f / 512.f)
+                                        : (1.f / 512.f) * t + pr.gradSpan;
+            float tc = u * 512.f - .5f;
+            int i0 = (int)floorf(tc);
+            float fr = tc - (float)i0;
+            int ia = min(max(i0, 0), 511), ib = min(max(i0 + 1, 0), 511);
+            const uint32_t* rowp = ramps + (size_t)min(pr.gradRow, gradHeight - 1) * kGradWidth;
+            uint32_t ta = rowp[ia], tb = rowp[ib];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+            {
+                float va = float((ta >> (8 * k)) & 0xffu) * (1.f / 255.f);
+                float vb = float((tb >> (8 * k)) & 0xffu) * (1.f / 255.f);
+                src[k] = (vb - va) * fr + va;
+            }
+            src[3] *= coverage;
+            src[0] *= src[3];
+            src[1] *= src[3];
+            src[2] *= src[3];
+        }
+        const float paintAlpha = src[3];
+        float out[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            out[k] = src[k] + col[k] * (1.f - paintAlpha);
+        if (paintAlpha != 0.f)
+        {
+            out[0] += noise;
+            out[1] += noise;
+            out[2] += noise;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            col[k] = quantize8(out[k]);
+    }
+    if (inside)
+    {
+        uint32_t packed = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            packed |= (uint32_t)__float2int_rn(col[k] * 255.f) << (8 * k);
+        target[(size_t)py * width + px] = packed;
+    }
+}
+
+// Unpacks PathData/PaintData/PaintAux into PathRaster (gpu.cpp:859-1063,
+// draw_path.vert:431-545).
+__global__ void path_raster_kernel(const PathDataV* __restrict__ paths, const uint2* __restrict__ paints,
+                                   const float* __restrict__ aux, uint32_t pathCount,
+                                   uint32_t gradHeight, PathRaster* __restrict__ out)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > pathCount)
+        return;
+    PathRaster r;
+    const PathDataV pd = paths[i];
+    const uint2 paint = paints[i];
+    const float* ax = aux + (size_t)i * 32;
+    uint32_t paintType = paint.x & 0xfu;
+    uint32_t flags = 0;
+    if (pd.strokeRadius != 0.f)
+        flags |= 1u;
+    else if (paint.x & PAINT_FLAG_EVEN_ODD_FILL)
+        flags |= 2u;
+    else if (!(paint.x & PAINT_FLAG_NON_ZERO_FILL))
+        flags |= 4u;
+    r.paintType = paintType;
+    r.outerClipID = 0;
+    if (paintType == CLIP_UPDATE_PAINT_TYPE)
+    {
+        flags |= 8u;
+        r.clipID = paint.y >> 16;
+        r.outerClipID = paint.x >> 16;
+    }
+    else
+    {
+        r.clipID = paint.x >> 16;
+    }
+    r.flags = flags;
+    float a = float(paint.y >> 24) * (1.f / 255.f);
+    r.color[0] = float(paint.y & 0xffu) * (1.f / 255.f) * a;
+    r.color[1] = float((paint.y >> 8) & 0xffu) * (1.f / 255.f) * a;
+    r.color[2] = float((paint.y >> 16) & 0xffu) * (1.f / 255.f) * a;
+    r.color[3] = a;
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        r.paintMatrix[k] = ax[k];
+    float span0 = ax[6], span1 = ax[7];
+    r.gradSpan = span0 > .9f ? 2.f : span1;
+    r.gradRow = (uint32_t)(u2f(paint.y) * (float)gradHeight);
+    out[i] = r;
+}
+
+// ---------------------------------------------------------------------------
+// Device-wide exclusive scan of u32 (three phases, 4096 items per block).
+// ---------------------------------------------------------------------------
+constexpr int kScanItems = 16;
+constexpr int kScanBlock = 256;
+constexpr int kScanTile = kScanItems * kScanBlock;
+
+__global__ void scan_reduce_kernel(const uint32_t* __restrict__ in, uint32_t n,
+                                   uint32_t* __restrict__ partials)
+{
+    __shared__ uint32_t red[kScanBlock];
+    uint32_t base = blockIdx.x * kScanTile;
+    uint32_t s = 0;
+    for (int k = 0; k < kScanItems; ++k)
+    {
+        uint32_t i = base + k * kScanBlock + threadIdx.x;
+        if (i < n)
+            s += in[i];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = kScanBlock / 2; off > 0; off >>= 1)
+    {
+        if (threadIdx.x < off)
+            red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        partials[blockIdx.x] = red[0];
+}
+
+__global__ void scan_partials_kernel(uint32_t* __restrict__ partials, uint32_t n,
+                                     uint32_t* __restrict__ total)
+{
+    __shared__ uint32_t buf[1024];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0)
+        carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += blockDim.x)
+    {
+        uint32_t i = base + threadIdx.x;
+        uint32_t v = i < n ? partials[i] : 0;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (uint32_t off = 1; off < blockDim.x; off <<= 1)
+        {
+            uint32_t t = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < n)
+            partials[i] = carry + buf[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == blockDim.x - 1)
+            carry += buf[threadIdx.x];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        *total = carry;
+}
+
+__global__ void scan_apply_kernel(const uint32_t* __restrict__ in, uint32_t n,
+                                  const uint32_t* __restrict__ partials, const uint32_t* __restrict__ total,
+                                  uint32_t* __restrict__ out)
+{
+    __shared__ uint32_t buf[kScanTile];
+    __shared__ uint32_t wsum[kScanBlock];
+    uint32_t base = blockIdx.x * kScanTile;
+    // Each thread owns kScanItems consecutive items.
+    uint32_t local[kScanItems];
+    uint32_t s = 0;
+    for (int k = 0; k < kScanItems; ++k)
+    {
+        uint32_t i = base + threadIdx.x * kScanItems + k;
+        uint32_t v = i < n ? in[i] : 0;
+        local[k] = s;
+        s += v;
+    }
+    wsum[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < kScanBlock; off <<= 1)
+    {
+        uint32_t t = threadIdx.x >= off ? wsum[threadIdx.x - off] : 0;
+        __syncthreads();
+        wsum[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t prefix = partials[blockIdx.x] + wsum[threadIdx.x] - s;
+    for (int k = 0; k < kScanItems; ++k)
+    {
+        uint32_t i = base + threadIdx.x * kScanItems + k;
+        if (i < n)
+            out[i] = prefix + local[k];
+    }
+    (void)buf;
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+        out[n] = *total;
+}
+} // namespace rivehip

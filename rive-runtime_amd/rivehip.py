@@ -1,0 +1,195 @@
+"""ctypes binding of librive_hip.so (include/rive_hip.h).
+
+The library is built in-tree by __graft_entry__.build() (or `make` in this
+directory).  Loading fails loudly if it is missing: there is no fallback path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librive_hip.so")
+
+OK = 0
+BUFFER = {
+    "flush_uniform": 0, "path": 1, "paint": 2, "paint_aux": 3, "contour": 4,
+    "grad_span": 5, "tess_span": 6, "triangle_vertex": 7, "image_draw": 8,
+    "verb": 9, "point": 10, "draw": 11,
+}
+FLUSH_SPANS = 0
+FLUSH_GEOMETRY = 1
+
+# numpy mirrors of the record structs (byte-compatible with include/rive_hip.h)
+TESS_SPAN = np.dtype([("pts", "<f4", 8), ("join_tangent", "<f4", 2), ("y", "<f4"),
+                      ("reflection_y", "<f4"), ("x0x1", "<i4"), ("reflection_x0x1", "<i4"),
+                      ("segment_counts", "<u4"), ("contour_id_with_flags", "<u4")])
+CONTOUR = np.dtype([("midpoint", "<f4", 2), ("path_id", "<u4"), ("vertex_index0", "<u4")])
+PATH_DATA = np.dtype([("matrix", "<f4", 6), ("stroke_radius", "<f4"), ("feather_radius", "<f4"),
+                      ("z_index", "<u4"), ("atlas_transform", "<f4", 3), ("coverage_range", "<u4", 4)])
+PAINT_DATA = np.dtype([("params", "<u4"), ("value", "<u4")])
+PAINT_AUX = np.dtype([("paint_matrix", "<f4", 6), ("grad_span", "<f4", 2),
+                      ("clip_rect_inverse_matrix", "<f4", 6), ("inverse_fwidth", "<f4", 2),
+                      ("image_matrix", "<f4", 6), ("image_lod", "<f4"), ("pad", "<f4", 9)])
+GRAD_SPAN = np.dtype([("horizontal_span", "<u4"), ("y_with_flags", "<u4"), ("color0", "<u4"),
+                      ("color1", "<u4")])
+DRAW = np.dtype([("path_id", "<u4"), ("first_verb", "<u4"), ("verb_count", "<u4"),
+                 ("first_point", "<u4"), ("point_count", "<u4"), ("contour_directions", "<u4"),
+                 ("contour_flags", "<u4"), ("join", "<u4"), ("cap", "<u4"), ("is_stroke", "<u4"),
+                 ("matrix", "<f4", 6), ("stroke_radius", "<f4"), ("matrix_max_scale", "<f4"),
+                 ("polar_segments_per_radian", "<f4"), ("parametric_precision", "<f4"),
+                 ("tess_location", "<u4"), ("tess_vertex_count", "<u4"), ("pad", "<u4", 2)])
+TESS_TEXEL = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<u4"), ("w", "<u4")])
+assert TESS_SPAN.itemsize == 64 and CONTOUR.itemsize == 16 and PATH_DATA.itemsize == 64
+assert PAINT_DATA.itemsize == 8 and PAINT_AUX.itemsize == 128 and GRAD_SPAN.itemsize == 16
+assert DRAW.itemsize == 96
+
+
+class Stream(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("clear_color", ctypes.c_uint32), ("dither", ctypes.c_uint32),
+                ("verbs", ctypes.c_void_p), ("verb_count", ctypes.c_uint32),
+                ("points", ctypes.c_void_p), ("point_count", ctypes.c_uint32),
+                ("paths", ctypes.c_void_p), ("path_count", ctypes.c_uint32),
+                ("draws", ctypes.c_void_p), ("draw_count", ctypes.c_uint32),
+                ("stop_colors", ctypes.c_void_p), ("stop_offsets", ctypes.c_void_p),
+                ("stop_count", ctypes.c_uint32)]
+
+
+class FlushDesc(ctypes.Structure):
+    _fields_ = [("target", ctypes.c_void_p), ("mode", ctypes.c_uint32),
+                ("load_action", ctypes.c_uint32), ("clear_color", ctypes.c_uint32),
+                ("dither", ctypes.c_uint32), ("path_count", ctypes.c_uint32),
+                ("contour_count", ctypes.c_uint32), ("tess_span_count", ctypes.c_uint32),
+                ("tess_vertex_count", ctypes.c_uint32), ("draw_count", ctypes.c_uint32),
+                ("grad_span_count", ctypes.c_uint32), ("grad_simple_rows", ctypes.c_uint32),
+                ("grad_complex_rows", ctypes.c_uint32),
+                ("first_path_tess_location", ctypes.c_uint32)]
+
+
+EXPORTED = [
+    "rive_hip_last_error", "rive_hip_create", "rive_hip_destroy", "rive_hip_buffer_resize",
+    "rive_hip_buffer_map", "rive_hip_buffer_unmap", "rive_hip_target_create",
+    "rive_hip_target_destroy", "rive_hip_flush", "rive_hip_sync", "rive_hip_read_pixels",
+    "rive_hip_debug_readback", "rive_hip_pass_times", "rive_stream_generate", "rive_stream_free",
+    "rive_hip_render_stream",
+]
+
+_lib = None
+
+
+def lib():
+    """Loads librive_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u32, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int
+    L.rive_hip_last_error.restype = ctypes.c_char_p
+    L.rive_hip_create.argtypes = [i32, ctypes.POINTER(vp)]
+    L.rive_hip_destroy.argtypes = [vp]
+    L.rive_hip_destroy.restype = None
+    L.rive_hip_buffer_resize.argtypes = [vp, i32, ctypes.c_size_t]
+    L.rive_hip_buffer_map.argtypes = [vp, i32, ctypes.c_size_t]
+    L.rive_hip_buffer_map.restype = vp
+    L.rive_hip_buffer_unmap.argtypes = [vp, i32, ctypes.c_size_t]
+    L.rive_hip_target_create.argtypes = [vp, u32, u32, ctypes.POINTER(vp)]
+    L.rive_hip_target_destroy.argtypes = [vp]
+    L.rive_hip_target_destroy.restype = None
+    L.rive_hip_flush.argtypes = [vp, ctypes.POINTER(FlushDesc)]
+    L.rive_hip_sync.argtypes = [vp]
+    L.rive_hip_read_pixels.argtypes = [vp, vp, vp]
+    L.rive_hip_debug_readback.argtypes = [vp, i32, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    L.rive_hip_pass_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float), i32]
+    L.rive_stream_generate.argtypes = [i32, ctypes.c_uint64, u32, u32, u32,
+                                       ctypes.POINTER(ctypes.POINTER(Stream))]
+    L.rive_stream_free.argtypes = [ctypes.POINTER(Stream)]
+    L.rive_stream_free.restype = None
+    L.rive_hip_render_stream.argtypes = [vp, ctypes.POINTER(Stream), vp]
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != OK:
+        raise RuntimeError(f"rive_hip error {rc}: {lib().rive_hip_last_error().decode()}")
+
+
+CONFIG_SEEDS = {1: 0x5249564500000001, 2: 0x5249564500000002}
+
+
+def generate_stream(config, seed=None, width=0, height=0, draw_count=0):
+    """Seeded synthetic draw stream (see rive_stream_generate)."""
+    L = lib()
+    p = ctypes.POINTER(Stream)()
+    check(L.rive_stream_generate(config, CONFIG_SEEDS[config] if seed is None else seed,
+                                 width, height, draw_count, ctypes.byref(p)))
+    return p
+
+
+def free_stream(p):
+    lib().rive_stream_free(p)
+
+
+class Context:
+    """One backend context (RenderContextImpl) on one HIP device."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        self.h = ctypes.c_void_p()
+        check(self.L.rive_hip_create(device, ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            self.L.rive_hip_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def target(self, w, h):
+        t = ctypes.c_void_p()
+        check(self.L.rive_hip_target_create(self.h, w, h, ctypes.byref(t)))
+        return t
+
+    def free_target(self, t):
+        self.L.rive_hip_target_destroy(t)
+
+    def upload(self, kind, array):
+        """resize + map + memcpy + unmap of one record buffer."""
+        k = BUFFER[kind]
+        data = np.ascontiguousarray(array).view(np.uint8).ravel()
+        n = max(int(data.size), 16)
+        check(self.L.rive_hip_buffer_resize(self.h, k, n))
+        ptr = self.L.rive_hip_buffer_map(self.h, k, n)
+        if not ptr:
+            raise RuntimeError(self.L.rive_hip_last_error().decode())
+        if data.size:
+            ctypes.memmove(ptr, data.ctypes.data, data.size)
+        check(self.L.rive_hip_buffer_unmap(self.h, k, int(data.size)))
+
+    def flush(self, desc):
+        check(self.L.rive_hip_flush(self.h, ctypes.byref(desc)))
+
+    def sync(self):
+        check(self.L.rive_hip_sync(self.h))
+
+    def read_pixels(self, target, w, h):
+        out = np.empty((h, w, 4), np.uint8)
+        check(self.L.rive_hip_read_pixels(self.h, target, out.ctypes.data))
+        return out
+
+    def readback(self, what):
+        size = ctypes.c_size_t()
+        check(self.L.rive_hip_debug_readback(self.h, what, None, 0, ctypes.byref(size)))
+        buf = np.empty(size.value, np.uint8)
+        check(self.L.rive_hip_debug_readback(self.h, what, buf.ctypes.data, buf.size,
+                                             ctypes.byref(size)))
+        return buf
+
+    def pass_times(self):
+        ms = (ctypes.c_float * 6)()
+        check(self.L.rive_hip_pass_times(self.h, ms, 6))
+        return list(ms)
+
+    def render_stream(self, stream, target):
+        check(self.L.rive_hip_render_stream(self.h, stream, target))
