@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Frame benchmark of the RenderContext::flush hot path on one MI355X.
+
+Workload: one 1920x1080 RGBA8 frame of 5000 seeded draws (60% nonzero fills,
+20% even-odd fills, 20% strokes with random joins/caps, 30% gradient paints,
+10% of draws under depth-2 clip paths), the seeded mt19937_64 stream
+0x5249564500000002.  A step is one plain GEOMETRY-mode flush of that frame
+from records already resident in HBM: curve counting, span emission,
+tessellation, ramps, binning, rasterisation and the RGBA8 resolve.  The
+end-to-end figure (front end on the host + PCIe upload + flush) is reported
+beside it and is never `value`.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line.  The roofline block prices the raster pass (the
+dominant kernel) with its algorithmic bytes
+    B_raster = 4*W*H*(1 + preserve) + 16*V + 72*P + 128*P_aux
+               + 2048*G_complex + 8*G_simple
+against MI355X's 8 TB/s HBM3E; its duration comes from HIP events around
+the raster kernel on the context stream.  cpu_baseline times the CPU oracle
+(single-threaded C++ restatement of the reference pipeline) on the first
+draws of the same stream at full resolution.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "rive-runtime_amd"))
+
+import rivehip  # noqa: E402
+
+METRIC = "1080p 5k-path frames/s + HBM roofline % (plain vs batched); p50/p99 frame ms"
+HBM_PEAK_GBS = 8000.0
+W, H, DRAWS = 1920, 1080, 5000
+SEED = rivehip.CONFIG_SEEDS[2]
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of raster_kernel on this workload
+# (KiB per dispatch, scaled by the copy-kernel calibration factor); None
+# until measured.
+RASTER_TRAFFIC_BYTES = None
+
+
+def percentile(xs, q):
+    xs = sorted(xs)
+    k = min(len(xs) - 1, max(0, int(round(q / 100.0 * (len(xs) - 1)))))
+    return xs[k]
+
+
+def raster_bytes(st):
+    preserve = 1 if st.load_action == 1 else 0
+    return (4 * st.width * st.height * (1 + preserve) + 16 * st.tess_vertex_count
+            + 72 * st.path_count + 128 * st.aux_path_count + 2048 * st.grad_complex_rows
+            + 8 * st.grad_simple_ramps)
+
+
+def cpu_baseline(sample_draws):
+    """The oracle rasterising the first `sample_draws` draws of the stream at
+    1920x1080 on one host core; the frame rate is scaled by draw count."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_binding as ob
+
+    s = rivehip.generate_stream(2, width=W, height=H, draw_count=sample_draws)
+    f = ob.OracleFlush(s)
+    try:
+        _, secs = f.render()
+    finally:
+        f.close()
+        rivehip.free_stream(s)
+    frame_secs = secs * DRAWS / sample_draws
+    return {"value": round(1.0 / frame_secs, 6), "unit": "frames/s", "cores": 1, "kind": "port",
+            "sample": f"oracle render of draws 0..{sample_draws - 1} of the 5000-draw 1920x1080 "
+                      f"stream took {secs:.2f} s on one core; scaled x{DRAWS / sample_draws:g} "
+                      f"by draw count"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-sample-draws", type=int, default=100)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this backend renders a frame on one GPU; --gpus {args.gpus} is not "
+              "supported (multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+    if os.environ.get("WORLD_SIZE", "1") not in ("", "1"):
+        print("bench.py: run as a single process (WORLD_SIZE must be 1)", file=sys.stderr)
+        sys.exit(2)
+
+    ctx = rivehip.Context(0)
+    stream = rivehip.generate_stream(2)
+    target = ctx.target(W, H)
+    try:
+        # End-to-end frames: RiveRenderer front end + record upload + flush.
+        for _ in range(args.warmup):
+            ctx.render_stream(stream, target)
+            ctx.sync()
+        e2e = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            ctx.render_stream(stream, target)
+            ctx.sync()
+            e2e.append((time.perf_counter() - t0) * 1e3)
+        stats = ctx.last_flush_stats()
+        print(f"# end-to-end frame mean {statistics.mean(e2e):.3f} ms; stats {stats.as_dict()}",
+              file=sys.stderr, flush=True)
+
+        # Resident frames: the flush alone, from records already in HBM.
+        for _ in range(args.warmup):
+            ctx.reflush_geometry(target, stats)
+            ctx.sync()
+        step_ms, passes = [], []
+        ctx.sync()
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            ctx.reflush_geometry(target, stats)
+            ctx.sync()
+            step_ms.append((time.perf_counter() - t0) * 1e3)
+            passes.append(ctx.pass_times())
+        pixels = ctx.read_pixels(target, W, H)
+    finally:
+        ctx.free_target(target)
+        rivehip.free_stream(stream)
+        ctx.close()
+    if int(pixels[..., 3].max()) == 0:
+        raise RuntimeError("the frame rendered nothing")
+
+    total_s = sum(step_ms) / 1e3
+    fps = args.steps / total_s
+    names = ["ramps", "count_spans", "tessellate", "bin", "raster", "total"]
+    pass_ms = {n: round(statistics.mean(p[i] for p in passes), 4) for i, n in enumerate(names)}
+    raster_s = pass_ms["raster"] / 1e3
+    b_raster = raster_bytes(stats)
+    achieved = b_raster / raster_s / 1e9
+    segments = stats.tess_vertex_count - stats.contour_count
+    out = {
+        "metric": METRIC,
+        "value": round(fps, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total_s * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": f"synthetic: seeded mt19937_64 draw stream, seed {SEED:#x}",
+        "config": {"workload": "1920x1080 RGBA8 frame, 5000 seeded paths (60% nonzero, 20% even-odd, "
+                               "20% strokes; 30% gradients; 10% under depth-2 clips); plain "
+                               "per-frame flush from HBM-resident records",
+                   "width": W, "height": H, "draws": DRAWS, "flush_mode": "geometry",
+                   "paths": stats.path_count, "contours": stats.contour_count,
+                   "tess_vertices": stats.tess_vertex_count, "spans": stats.span_count},
+        "p50_ms": round(percentile(step_ms, 50), 4),
+        "p99_ms": round(percentile(step_ms, 99), 4),
+        "mpixels_per_s": round(W * H * fps / 1e6, 2),
+        "segments_per_s": round(segments * fps, 1),
+        "passes_ms": pass_ms,
+        "end_to_end_ms": {"mean": round(statistics.mean(e2e), 4), "p50": round(percentile(e2e, 50), 4),
+                          "p99": round(percentile(e2e, 99), 4),
+                          "note": "host front end + PCIe upload + flush"},
+        "roofline": {"kernel": "raster_kernel", "bound": "hbm", "achieved": round(achieved, 3),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
+                     "traffic": RASTER_TRAFFIC_BYTES, "algorithmic_bytes": b_raster,
+                     "kernel_ms": pass_ms["raster"]},
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample_draws),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

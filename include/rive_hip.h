@@ -231,9 +231,10 @@ int rive_hip_read_pixels(rive_hip_context* ctx, rive_hip_target* target, uint8_t
 
 /* Debug/parity readback of device intermediates from the last flush:
  *  0 TessVertexSpans, 1 ContourData, 2 tessellation texture (16 B/texel),
- *  3 per-draw tessellation vertex counts (u32), 4 gradient ramp texture
- *  (RGBA8, 512 wide).  Writes min(capacity, size) bytes; *size gets the
- *  full size. */
+ *  3 rive_hip_draw records (GEOMETRY mode: with the device-written
+ *  tess_location / tess_vertex_count), 4 gradient ramp texture (RGBA8,
+ *  512 wide), 5 GEOMETRY-mode curve records (64 B each, before row
+ *  wrapping).  Writes min(capacity, size) bytes; *size gets the full size. */
 int rive_hip_debug_readback(rive_hip_context* ctx, int what, void* dst,
                             size_t capacity, size_t* size);
 
@@ -241,6 +242,28 @@ int rive_hip_debug_readback(rive_hip_context* ctx, int what, void* dst,
  * context stream): [0] ramps, [1] counting+spans, [2] tessellation,
  * [3] binning, [4] raster+resolve, [5] total.  n <= 6. */
 int rive_hip_pass_times(rive_hip_context* ctx, float* ms, int n);
+
+/* Sizes of the last flush, as the device laid it out (the counters the
+ * reference keeps in ResourceCounters / FlushDescriptor,
+ * render_context.hpp:354-397, gpu.hpp:1320-1438). */
+typedef struct rive_hip_flush_stats
+{
+    uint32_t width, height, load_action, mode;
+    uint32_t path_count, draw_count, contour_count, span_count;
+    uint32_t tess_vertex_count;  /* texels laid out, incl. 8 + 1 padding */
+    uint32_t curve_record_count; /* GEOMETRY mode: pushCubic records */
+    uint32_t prim_count;         /* expanded fan/border/stroke primitives */
+    uint32_t bin_entries;        /* (tile, primitive) pairs */
+    uint32_t tile_count;
+    uint32_t grad_simple_rows, grad_complex_rows;
+    uint32_t aux_path_count;     /* paths whose paint reads PaintAuxData */
+    uint32_t grad_span_count;
+    uint32_t first_path_tess_location;
+    uint32_t grad_simple_ramps;  /* two-texel ramps (one GradientSpan each) */
+    uint32_t clear_color, dither;
+} rive_hip_flush_stats;
+
+int rive_hip_last_flush_stats(rive_hip_context* ctx, rive_hip_flush_stats* out);
 
 /* ------------------------------------------------------------------------ */
 /* Draw streams: a recorded sequence of rive::Renderer calls.               */

@@ -21,7 +21,9 @@ struct F2
 inline F2 sub(F2 a, F2 b) { return {a.x - b.x, a.y - b.y}; }
 inline float dot2(F2 a, F2 b) { return a.x * b.x + a.y * b.y; }
 inline bool neq(F2 a, F2 b) { return a.x != b.x || a.y != b.y; }
-inline float clampf(float x, float lo, float hi) { return std::min(std::max(x, lo), hi); }
+// GLSL clamp() = min(max(x, lo), hi) with GPU min/max, which return the
+// non-NaN operand (IEEE maxNum/minNum): a NaN input clamps to lo.
+inline float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 inline F2 normalize2(F2 v)
 {
     float l = sqrtf(v.x * v.x + v.y * v.y);

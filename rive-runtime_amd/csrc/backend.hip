@@ -91,7 +91,8 @@ struct rive_hip_context
     DevBuf spanTexelCounts, spanTexelOffsets, scanPartials, scalars;
     DevBuf tess, prims, cells, cellCount, cellOffset, cellCursor, cellBackdrop, binned;
     DevBuf pathRaster, ramps, drawTess;
-    DevBuf geoCounts, geoRecords, geoSpanCounts, geoSpanOffsets;
+    DevBuf geoCounts, geoRecords, geoContours, geoSpans, geoSpanCounts, geoSpanOffsets;
+    uint32_t lastMode = 0, lastRecordCount = 0, lastTessVertexCount = 0;
     uint32_t* hostScalars = nullptr; // pinned
     hipEvent_t ev[kEvents] = {};
     float passMs[6] = {};
@@ -99,6 +100,7 @@ struct rive_hip_context
     // last flush sizes, for debug readback
     uint32_t lastSpanCount = 0, lastContourCount = 0, lastTexels = 0, lastDrawCount = 0;
     uint32_t lastGradRows = 0;
+    rive_hip_flush_stats stats = {};
 };
 
 extern "C" {
@@ -148,7 +150,7 @@ void rive_hip_destroy(rive_hip_context* ctx)
                      &ctx->tess, &ctx->prims, &ctx->cells, &ctx->cellCount, &ctx->cellOffset,
                      &ctx->cellCursor, &ctx->cellBackdrop, &ctx->binned, &ctx->pathRaster,
                      &ctx->ramps, &ctx->drawTess, &ctx->geoCounts, &ctx->geoRecords,
-                     &ctx->geoSpanCounts, &ctx->geoSpanOffsets};
+                     &ctx->geoContours, &ctx->geoSpans, &ctx->geoSpanCounts, &ctx->geoSpanOffsets};
     for (DevBuf* b : all)
         if (b->ptr)
             (void)hipFree(b->ptr);
@@ -406,6 +408,9 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     rc = read_scalar(ctx, scal + 2, &totalPairs);
     if (rc)
         return rc;
+    ctx->stats.bin_entries = totalPairs;
+    ctx->stats.prim_count = totalPrims;
+    ctx->stats.tile_count = (uint32_t)(tilesX * tilesY);
     HIP_TRY(ensure(ctx->binned, (size_t)std::max<uint32_t>(totalPairs, 1) * 4));
     if (totalPrims > 0)
         bin_scatter_kernel<<<(totalPrims + 255) / 256, 256, 0, ctx->stream>>>(
@@ -572,6 +577,40 @@ extern "C" int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* 
                     "pipeline does not use");
     ctx->lastSpanCount = spanCount;
     ctx->lastDrawCount = d->draw_count;
+    ctx->lastMode = d->mode;
+    {
+        rive_hip_flush_stats& st = ctx->stats;
+        st = rive_hip_flush_stats{};
+        st.width = d->target->width;
+        st.height = d->target->height;
+        st.load_action = d->load_action;
+        st.mode = d->mode;
+        st.path_count = d->path_count;
+        st.draw_count = d->draw_count;
+        st.contour_count = ctx->lastContourCount;
+        st.span_count = spanCount;
+        st.tess_vertex_count = d->mode == RIVE_HIP_FLUSH_GEOMETRY ? ctx->lastTessVertexCount : d->tess_vertex_count;
+        st.curve_record_count = d->mode == RIVE_HIP_FLUSH_GEOMETRY ? ctx->lastRecordCount : 0;
+        st.grad_simple_rows = d->grad_simple_rows;
+        st.grad_complex_rows = d->grad_complex_rows;
+        st.grad_span_count = d->grad_span_count;
+        st.first_path_tess_location = d->first_path_tess_location;
+        st.clear_color = d->clear_color;
+        st.dither = d->dither;
+        const auto* gradHost = static_cast<const rive_hip_grad_span*>(ctx->bufs[RIVE_HIP_BUFFER_GRAD_SPAN].host);
+        if (gradHost != nullptr)
+            for (uint32_t i = 0; i < d->grad_span_count; ++i)
+                st.grad_simple_ramps += (gradHost[i].y_with_flags & 0x20000000u) == 0u ? 1u : 0u;
+        // Gradient paints read PaintAuxData; the host staging copy of the
+        // PAINT buffer holds this flush's records.
+        const auto* paintsHost = static_cast<const rive_hip_paint_data*>(ctx->bufs[RIVE_HIP_BUFFER_PAINT].host);
+        if (paintsHost != nullptr)
+            for (uint32_t i = 1; i <= d->path_count; ++i)
+            {
+                const uint32_t type = paintsHost[i].params & 0xfu;
+                st.aux_path_count += (type == 2u || type == 3u) ? 1u : 0u;
+            }
+    }
     int rc = raster_pipeline(ctx, d, spans, spanCount, contours, ctx->lastContourCount, tessRows,
                              P<DrawTess>(ctx->drawTess),
                              d->draw_count, totalPrims);
@@ -608,29 +647,34 @@ extern "C" int rive_hip_debug_readback(rive_hip_context* ctx, int what, void* ds
     if (ctx == nullptr)
         return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "ctx is null");
     HIP_TRY(hipSetDevice(ctx->device));
+    const bool geo = ctx->lastMode == RIVE_HIP_FLUSH_GEOMETRY;
     const void* src = nullptr;
     size_t bytes = 0;
     switch (what)
     {
-        case 0:
-            src = ctx->geoRecords.ptr ? ctx->geoRecords.ptr : ctx->bufs[RIVE_HIP_BUFFER_TESS_SPAN].dev;
+        case 0: // TessVertexSpans consumed by the last flush
+            src = geo ? ctx->geoSpans.ptr : ctx->bufs[RIVE_HIP_BUFFER_TESS_SPAN].dev;
             bytes = (size_t)ctx->lastSpanCount * sizeof(rive_hip_tess_span);
             break;
-        case 1:
-            src = ctx->geoCounts.ptr ? ctx->geoCounts.ptr : ctx->bufs[RIVE_HIP_BUFFER_CONTOUR].dev;
+        case 1: // ContourData
+            src = geo ? ctx->geoContours.ptr : ctx->bufs[RIVE_HIP_BUFFER_CONTOUR].dev;
             bytes = (size_t)ctx->lastContourCount * sizeof(rive_hip_contour);
             break;
-        case 2:
+        case 2: // tessellation texture
             src = ctx->tess.ptr;
             bytes = (size_t)ctx->lastTexels * sizeof(TessTexel);
             break;
-        case 3:
-            src = ctx->drawTess.ptr;
-            bytes = (size_t)ctx->lastDrawCount * sizeof(DrawTess);
+        case 3: // rive_hip_draw records (GEOMETRY mode: with device-written ranges)
+            src = ctx->bufs[RIVE_HIP_BUFFER_DRAW].dev;
+            bytes = (size_t)ctx->lastDrawCount * sizeof(rive_hip_draw);
             break;
-        case 4:
+        case 4: // gradient ramps
             src = ctx->ramps.ptr;
             bytes = (size_t)ctx->lastGradRows * kGradWidth * 4;
+            break;
+        case 5: // device cubic records (GEOMETRY mode)
+            src = geo ? ctx->geoRecords.ptr : nullptr;
+            bytes = geo ? (size_t)ctx->lastRecordCount * 64 : 0;
             break;
         default:
             return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "unknown readback");
@@ -645,11 +689,12 @@ extern "C" int rive_hip_debug_readback(rive_hip_context* ctx, int what, void* ds
     return RIVE_HIP_OK;
 }
 
-extern "C" int rive_hip_render_stream(rive_hip_context* ctx, const rive_stream* stream,
-                                      rive_hip_target* target)
+extern "C" int rive_hip_last_flush_stats(rive_hip_context* ctx, rive_hip_flush_stats* out)
 {
-    (void)ctx;
-    (void)stream;
-    (void)target;
-    return fail(RIVE_HIP_ERROR_UNSUPPORTED, "front end not built yet");
+    if (ctx == nullptr || out == nullptr)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->timed)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "no flush recorded");
+    *out = ctx->stats;
+    return RIVE_HIP_OK;
 }

@@ -1,17 +1,1157 @@
-// Device-side counting (RIVE_HIP_FLUSH_GEOMETRY).  Placeholder until the
-// counting kernels land.
+// Device-side counting for RIVE_HIP_FLUSH_GEOMETRY: the work the reference
+// does on the CPU in PathDraw::initForMidpointFan (draw.cpp:768-1392),
+// PathDraw::pushMidpointFanTessellationData (draw.cpp:1992-2375) and the
+// TessellationWriter (render_context.cpp:3079-3424), done on gfx950.
+//
+//   count_kernel    one thread per draw: per-curve Wang's formula, convex-180
+//                   chops, polar segments, joins, caps and per-contour padding
+//                   -> tessellation vertex, contour and record counts.
+//   (scan)          draw offsets: tessellation locations, contour IDs,
+//                   record slots.
+//   emit_kernel     one thread per draw: the same walk again, writing
+//                   cubic records (with their locations) and ContourData.
+//   span kernels    one thread per record: 2048-wide row wrapping into
+//                   TessVertexSpan records, byte-compatible with the
+//                   reference's.
+//
+// All arithmetic is plain IEEE fp32 with no contraction (the file is built
+// with -ffp-contract=off); division and sqrt are correctly rounded, so the
+// integer outputs match the reference's strict-IEEE CPU build bit for bit.
 #pragma once
+
+namespace rivehip
+{
+struct V2d
+{
+    float x, y;
+};
+__device__ __forceinline__ V2d operator+(V2d a, V2d b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ V2d operator-(V2d a, V2d b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ V2d operator*(V2d a, float s) { return {a.x * s, a.y * s}; }
+__device__ __forceinline__ V2d operator*(float s, V2d a) { return {s * a.x, s * a.y}; }
+__device__ __forceinline__ bool veq(V2d a, V2d b) { return a.x == b.x && a.y == b.y; }
+__device__ __forceinline__ bool vne(V2d a, V2d b) { return !veq(a, b); }
+__device__ __forceinline__ V2d vmix(V2d a, V2d b, float t) { return (b - a) * t + a; } // simd.hpp:746-753
+__device__ __forceinline__ float vcross(V2d a, V2d b) { return a.x * b.y - a.y * b.x; }
+__device__ __forceinline__ float vdot(V2d a, V2d b) { return a.x * b.x + a.y * b.y; }
+// simd::clamp ignores NaN (simd.hpp:211-254): fminf/fmaxf have the same rule.
+__device__ __forceinline__ float sclamp(float x, float lo, float hi) { return fminf(fmaxf(lo, x), hi); }
+
+constexpr float kTessEpsilon = 1.f / (1 << 10); // bezier_utils.cpp:178
+constexpr float kMathEpsilon = 1.f / (1 << 12); // math_types.hpp:20-21
+constexpr float kMathPI = 3.14159265f;          // math_types.hpp:18
+constexpr uint32_t kMiterBevelJoinSegments = 5;  // draw.cpp:40
+constexpr float kPolarPrecisionF = 8.f;
+
+__device__ __forceinline__ float fast_acos_d(float x) // simd.hpp:496-506
+{
+    const float a = -0.939115566365855f;
+    const float b = 0.9217841528914573f;
+    const float c = -1.2845906244690837f;
+    const float d = 0.295624144969963174f;
+    const float pi_over_2 = 1.5707963267948966f;
+    float xx = x * x;
+    float numer = b * xx + a;
+    float denom = xx * (d * xx + c) + 1.f;
+    return x * (numer / denom) + pi_over_2;
+}
+
+// wangs_formula.hpp:157-169 (cubic, raised to the 4th power).
+__device__ __forceinline__ float wang_pow4(const V2d p[4], float precision, const float m[6])
+{
+    float v0 = (-2.f * p[1].x + p[0].x) + p[2].x;
+    float v1 = (-2.f * p[1].y + p[0].y) + p[2].y;
+    float v2 = (-2.f * p[2].x + p[1].x) + p[3].x;
+    float v3 = (-2.f * p[2].y + p[1].y) + p[3].y;
+    float w0 = m[0] * v0 + m[2] * v1;
+    float w1 = m[3] * v1 + m[1] * v0;
+    float w2 = m[0] * v2 + m[2] * v3;
+    float w3 = m[3] * v3 + m[1] * v2;
+    const float k = ((3 * 3) * (2 * 2) / 64.f) * (precision * precision);
+    return fmaxf(w0 * w0 + w1 * w1, w2 * w2 + w3 * w3) * k;
+}
+__device__ __forceinline__ uint32_t parametric_count(float n4) // draw.cpp:1195-1203
+{
+    float n = ceilf(sqrtf(sqrtf(n4)));
+    n = sclamp(n, 1.f, 1023.f);
+    return (uint32_t)n;
+}
+__device__ __forceinline__ uint32_t polar_count(V2d t0, V2d t1, float spr) // draw.cpp:1219-1238
+{
+    float numer = t0.x * t1.x + t0.y * t1.y;
+    float denom_pow2 = (t0.x * t0.x + t0.y * t0.y) * (t1.x * t1.x + t1.y * t1.y);
+    float cosTheta = numer / sqrtf(denom_pow2);
+    cosTheta = sclamp(cosTheta, -1.f, 1.f);
+    float theta = fast_acos_d(cosTheta);
+    float n = ceilf(theta * spr);
+    n = sclamp(n, 1.f, 1023.f);
+    return (uint32_t)n;
+}
+__device__ __forceinline__ V2d tan0(const V2d p[4]) // bezier_utils.hpp:147-151
+{
+    return (vne(p[0], p[1]) ? p[1] : vne(p[1], p[2]) ? p[2] : p[3]) - p[0];
+}
+__device__ __forceinline__ V2d tan1(const V2d p[4])
+{
+    return p[3] - (vne(p[3], p[2]) ? p[2] : vne(p[2], p[1]) ? p[1] : p[0]);
+}
+
+__device__ void chop_at(const V2d* src, V2d* dst, float t) // bezier_utils.cpp:34-60
+{
+    if (t == 1.f)
+    {
+        V2d s0 = src[0], s1 = src[1], s2 = src[2], s3 = src[3];
+        dst[0] = s0;
+        dst[1] = s1;
+        dst[2] = s2;
+        dst[3] = s3;
+        dst[4] = dst[5] = dst[6] = src[3];
+        return;
+    }
+    V2d p0 = src[0], p1 = src[1], p2 = src[2], p3 = src[3];
+    V2d ab = vmix(p0, p1, t), bc = vmix(p1, p2, t), cd = vmix(p2, p3, t);
+    V2d abc = vmix(ab, bc, t), bcd = vmix(bc, cd, t);
+    V2d abcd = vmix(abc, bcd, t);
+    dst[0] = p0;
+    dst[1] = ab;
+    dst[2] = abc;
+    dst[3] = abcd;
+    dst[4] = bcd;
+    dst[5] = cd;
+    dst[6] = p3;
+}
+__device__ void chop_at2(const V2d* src, V2d* dst, float t0, float t1) // bezier_utils.cpp:62-101
+{
+    if (t1 == 1.f)
+    {
+        chop_at(src, dst, t0);
+        dst[7] = dst[8] = dst[9] = src[3]; // src may alias dst, as in the reference
+        return;
+    }
+    V2d p0 = src[0], p1 = src[1], p2 = src[2], p3 = src[3];
+    V2d ab0 = vmix(p0, p1, t0), ab1 = vmix(p0, p1, t1);
+    V2d bc0 = vmix(p1, p2, t0), bc1 = vmix(p1, p2, t1);
+    V2d cd0 = vmix(p2, p3, t0), cd1 = vmix(p2, p3, t1);
+    V2d abc0 = vmix(ab0, bc0, t0), abc1 = vmix(ab1, bc1, t1);
+    V2d bcd0 = vmix(bc0, cd0, t0), bcd1 = vmix(bc1, cd1, t1);
+    V2d abcd0 = vmix(abc0, bcd0, t0), abcd1 = vmix(abc1, bcd1, t1);
+    V2d mid0 = vmix(abc0, bcd0, t1), mid1 = vmix(abc1, bcd1, t0);
+    dst[0] = p0;
+    dst[1] = ab0;
+    dst[2] = abc0;
+    dst[3] = abcd0;
+    dst[4] = mid0;
+    dst[5] = mid1;
+    dst[6] = abcd1;
+    dst[7] = bcd1;
+    dst[8] = cd1;
+    dst[9] = p3;
+}
+__device__ void chop_at_n(const V2d* src, V2d* dst, const float* tv, int n) // bezier_utils.cpp:103-158
+{
+    if (n == 0)
+    {
+        V2d s0 = src[0], s1 = src[1], s2 = src[2], s3 = src[3];
+        dst[0] = s0;
+        dst[1] = s1;
+        dst[2] = s2;
+        dst[3] = s3;
+        return;
+    }
+    int i = 0;
+    float lastT = 0.f;
+    for (; i < n - 1; i += 2)
+    {
+        float tt0 = sclamp((tv[i] - lastT) / (1.f - lastT), 0.f, 1.f);
+        float tt1 = sclamp((tv[i + 1] - lastT) / (1.f - lastT), 0.f, 1.f);
+        lastT = tv[i + 1];
+        chop_at2(src, dst, tt0, tt1);
+        src = dst = dst + 6;
+    }
+    if (i < n)
+    {
+        float t = sclamp((tv[i] - lastT) / (1.f - lastT), 0.f, 1.f);
+        chop_at(src, dst, t);
+    }
+}
+__device__ V2d eval_cubic(const V2d p[4], float t) // bezier_utils.cpp:21-32
+{
+    V2d a = (p[3] + 3.f * (p[1] - p[2])) - p[0];
+    V2d b = 3.f * ((p[2] - 2.f * p[1]) + p[0]);
+    V2d c = 3.f * (p[1] - p[0]);
+    return ((a * t + b) * t + c) * t + p[0];
+}
+__device__ int convex_180_chops(const V2d pts[4], float T[2], bool* areCusps) // bezier_utils.cpp:180-330
+{
+    const uint32_t kOneMinus2Eps = (127u << 23) - 2u * (1u << (24 - 10));
+    V2d p0 = pts[0], p1 = pts[1], p2 = pts[2], p3 = pts[3];
+    V2d C = p1 - p0;
+    V2d D = p2 - p1;
+    V2d E = p3 - p0;
+    V2d B = D - C;
+    V2d A = -3.f * D + E;
+    float a = vcross(A, B);
+    float b = vcross(A, C);
+    float c = vcross(B, C);
+    float b_over_minus_2 = -.5f * b;
+    float discr_over_4 = b_over_minus_2 * b_over_minus_2 - a * c;
+    float cuspThreshold = a * (kTessEpsilon / 2);
+    cuspThreshold *= cuspThreshold;
+    if (discr_over_4 < -cuspThreshold)
+    {
+        *areCusps = false;
+        float root = c / b_over_minus_2;
+        if (__float_as_uint(root - kTessEpsilon) < kOneMinus2Eps)
+        {
+            T[0] = root;
+            return 1;
+        }
+        return 0;
+    }
+    *areCusps = discr_over_4 <= cuspThreshold;
+    if (*areCusps)
+    {
+        if (a != 0.f || b_over_minus_2 != 0.f || c != 0.f)
+        {
+            float root = b_over_minus_2 / a;
+            if (__float_as_uint(root - kTessEpsilon) < kOneMinus2Eps)
+            {
+                T[0] = root;
+                return 1;
+            }
+            *areCusps = false;
+            return 0;
+        }
+        V2d base = p3 - p0;
+        float dp0 = pts[0].x * base.x + pts[0].y * base.y;
+        float dp1 = pts[1].x * base.x + pts[1].y * base.y;
+        float dp2 = pts[2].x * base.x + pts[2].y * base.y;
+        float dp3 = pts[3].x * base.x + pts[3].y * base.y;
+        if (dp1 > dp0 && dp2 > dp1 && dp3 > dp2)
+        {
+            *areCusps = false;
+            return 0;
+        }
+        V2d t0v = (C.x != 0.f || C.y != 0.f) ? C : p2 - p0;
+        a = vdot(t0v, A);
+        b_over_minus_2 = -vdot(t0v, B);
+        c = vdot(t0v, C);
+        discr_over_4 = fmaxf(b_over_minus_2 * b_over_minus_2 - a * c, 0.f);
+    }
+    float q = sqrtf(discr_over_4);
+    q = copysignf(q, b_over_minus_2);
+    q = q + b_over_minus_2;
+    float r0 = q / a, r1 = c / q;
+    bool in0 = r0 > kTessEpsilon && r0 < (1.f - kTessEpsilon);
+    bool in1 = r1 > kTessEpsilon && r1 < (1.f - kTessEpsilon);
+    if (in0)
+    {
+        if (in1 && r0 != r1)
+        {
+            if (r0 > r1)
+            {
+                float t = r0;
+                r0 = r1;
+                r1 = t;
+            }
+            T[0] = r0;
+            T[1] = r1;
+            return 2;
+        }
+        T[0] = r0;
+        return 1;
+    }
+    if (in1)
+    {
+        T[0] = r1;
+        return 1;
+    }
+    return 0;
+}
+__device__ void chop_around_cusps(const V2d p[4], V2d* dst, const float* cuspT, int n,
+                                  float matrixMaxScale) // draw.cpp:139-174
+{
+    float t[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < n; ++i)
+    {
+        float minT = i == 0 ? 0.f : (cuspT[i - 1] + cuspT[i]) * .5f;
+        float maxT = i + 1 == n ? 1.f : (cuspT[i + 1] + cuspT[i]) * .5f;
+        t[i * 2 + 0] = fmaxf(cuspT[i] - kMathEpsilon, minT);
+        t[i * 2 + 1] = fminf(cuspT[i] + kMathEpsilon, maxT);
+    }
+    chop_at_n(p, dst, t, n * 2);
+    for (int i = 0; i < n; ++i)
+    {
+        V2d* chops = dst + i * 6;
+        V2d cusp = eval_cubic(p, cuspT[i]);
+        chops[3] = chops[6] = cusp;
+        V2d pivot = (chops[2] + chops[7]) * .5f;
+        V2d dd = cusp - pivot;
+        float len2 = dd.x * dd.x + dd.y * dd.y;
+        float scale = len2 > 0.f ? (1.f / sqrtf(len2)) : 1.f;
+        V2d nrm = dd * scale;
+        float denom = matrixMaxScale * kPolarPrecisionF * 2.f;
+        pivot = V2d{nrm.x / denom, nrm.y / denom} + cusp;
+        chops[4] = chops[5] = pivot;
+    }
+}
+
+// Chops a stroked cubic into convex pieces (draw.cpp:1062-1101).  Returns the
+// number of sub-cubics written to out[0..3n].
+__device__ int stroke_subcubics(const V2d* p, float maxScale, V2d* out /*[16]*/)
+{
+    float t[2] = {0.f, 0.f};
+    bool areCusps = false;
+    int numChops = convex_180_chops(p, t, &areCusps);
+    if (areCusps && numChops > 0)
+    {
+        chop_around_cusps(p, out, t, numChops, maxScale);
+        return numChops * 2 + 1;
+    }
+    if (numChops == 2)
+    {
+        chop_at2(p, out, t[0], t[1]);
+        return 3;
+    }
+    if (numChops == 1)
+    {
+        chop_at(p, out, t[0]);
+        return 2;
+    }
+    out[0] = p[0];
+    out[1] = p[1];
+    out[2] = p[2];
+    out[3] = p[3];
+    return 1;
+}
+
+// One cubic record (a TessellationWriter::pushCubic call) as written by the
+// device: the span fields before row wrapping.
+struct CubicRec
+{
+    float pts[8];
+    float jt[2];
+    uint32_t counts; // join<<20 | polar<<10 | parametric
+    uint32_t flags;  // contourIDWithFlags
+    uint32_t total;  // padding + parametric + polar + join - 1
+    uint32_t fwdLoc; // forward location or ~0
+    uint32_t mirLoc; // one past the mirrored run, or ~0
+    uint32_t dirs;
+};
+static_assert(sizeof(CubicRec) == 64, "CubicRec is 64 bytes");
+
+struct DrawCounts
+{
+    uint32_t tessVertexCount; // one direction
+    uint32_t contours;
+    uint32_t records;
+    uint32_t pad;
+};
+
+struct Emit
+{
+    CubicRec* recs;
+    ContourV* contours;
+    uint32_t* overflow; // set when the emit walk outgrows the counted ranges
+    uint32_t recCap, contourCap;
+    uint32_t rec, contourIndex, contourID;
+    uint32_t fwd, mir;
+    uint32_t dirs;
+    uint32_t nextPad;
+};
+
+__device__ __forceinline__ void push_cubic(Emit* e, const V2d* pts, V2d jt, uint32_t param, uint32_t polar,
+                                           uint32_t join, uint32_t flags)
+{
+    uint32_t total = e->nextPad + param + polar + join - 1u;
+    e->nextPad = 0;
+    CubicRec r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        r.pts[2 * k] = pts[k].x;
+        r.pts[2 * k + 1] = pts[k].y;
+    }
+    r.jt[0] = jt.x;
+    r.jt[1] = jt.y;
+    r.counts = (join << 20) | (polar << 10) | param;
+    r.flags = flags;
+    r.total = total;
+    r.fwdLoc = ~0u;
+    r.mirLoc = ~0u;
+    r.dirs = e->dirs;
+    if (e->dirs != 1u)
+    {
+        r.fwdLoc = e->fwd;
+        e->fwd += total;
+    }
+    if (e->dirs != 0u)
+    {
+        r.mirLoc = e->mir;
+        e->mir -= total;
+    }
+    // The counting walk sized this draw's range; never write past it.
+    if (e->rec < e->recCap)
+        e->recs[e->rec] = r;
+    else
+        atomicOr(e->overflow, 1u);
+    ++e->rec;
+}
+
+// Walks one draw.  EMIT = false counts; EMIT = true writes records.
+template <bool EMIT>
+__device__ DrawCounts walk_draw(const rive_hip_draw& d, const uint8_t* __restrict__ verbs,
+                                const V2d* __restrict__ P, Emit* e)
+{
+    DrawCounts out{0, 0, 0, 0};
+    const bool isStroke = d.is_stroke != 0;
+    const bool roundJoin = isStroke && d.join == 1;
+    const float spr = d.polar_segments_per_radian;
+    const float maxScale = d.matrix_max_scale;
+    const float prec = d.parametric_precision;
+    const uint32_t nv = d.verb_count;
+    uint32_t vi = 0, pi = 0;
+    while (vi < nv)
+    {
+        // ---- one contour: [cs, ce) verbs starting at point ps
+        uint32_t cs = vi, ps = pi;
+        // Contour end: the next move after cs.
+        uint32_t ce = cs + 1, pe = ps + (verbs[cs] == 0 ? 1u : 0u);
+        if (verbs[cs] != 0)
+        {
+            // A contour always starts with a move in a RawPath.
+            ce = cs;
+            pe = ps;
+        }
+        while (ce < nv && verbs[ce] != 0)
+        {
+            uint8_t v = verbs[ce];
+            pe += v == 1 ? 1u : (v == 2 ? 2u : (v == 4 ? 3u : 0u));
+            ++ce;
+        }
+        if (ce == cs)
+        {
+            ++vi; // malformed: skip a stray verb
+            continue;
+        }
+        bool closed = !isStroke;
+        for (uint32_t k = cs; k < ce; ++k)
+            if (verbs[k] == 5)
+                closed = true;
+        const V2d movePt = P[ps];
+        const V2d finalPt = P[pe - 1];
+        const bool implicitClose =
+            closed && (__float_as_uint(movePt.x) != __float_as_uint(finalPt.x) ||
+                       __float_as_uint(movePt.y) != __float_as_uint(finalPt.y));
+
+        // ---- walk 1: counts (draw.cpp:905-1376)
+        uint32_t lines = 0, curves = 0, paramSum = 0, polarSum = 0, preChop = 0;
+        V2d endpointsSum{0.f, 0.f};
+        V2d lastTan{0.f, 1.f}, firstTan{0.f, 1.f};
+        {
+            uint32_t q = ps + 1;
+            for (uint32_t k = cs + 1; k < ce; ++k)
+            {
+                uint8_t v = verbs[k];
+                if (v == 1)
+                {
+                    const V2d* lp = P + q - 1;
+                    if (roundJoin)
+                    {
+                        V2d tg = lp[1] - lp[0];
+                        if (preChop == 0)
+                            firstTan = tg;
+                        else
+                            polarSum += polar_count(lastTan, tg, spr);
+                        lastTan = tg;
+                    }
+                    ++preChop;
+                    endpointsSum = endpointsSum + lp[1];
+                    ++lines;
+                    q += 1;
+                }
+                else if (v == 4)
+                {
+                    const V2d* cp = P + q - 1;
+                    ++preChop;
+                    endpointsSum = endpointsSum + cp[3];
+                    if (!isStroke)
+                    {
+                        paramSum += parametric_count(wang_pow4(cp, prec, d.matrix));
+                        ++curves;
+                    }
+                    else
+                    {
+                        if (roundJoin)
+                        {
+                            V2d t0 = tan0(cp), t1 = tan1(cp);
+                            if (preChop == 1)
+                                firstTan = t0;
+                            else
+                                polarSum += polar_count(lastTan, t0, spr);
+                            lastTan = t1;
+                        }
+                        V2d sub[16];
+                        int n = stroke_subcubics(cp, maxScale, sub);
+                        for (int s = 0; s < n; ++s)
+                        {
+                            paramSum += parametric_count(wang_pow4(sub + 3 * s, prec, d.matrix));
+                            polarSum += polar_count(tan0(sub + 3 * s), tan1(sub + 3 * s), spr);
+                            ++curves;
+                        }
+                    }
+                    q += 3;
+                }
+                else if (v == 2)
+                {
+                    q += 2; // quads are converted to cubics by the path object
+                }
+            }
+        }
+        // finishAndAppendContour (draw.cpp:916-972)
+        if (implicitClose)
+        {
+            if (roundJoin)
+            {
+                V2d tg = movePt - finalPt;
+                polarSum += polar_count(lastTan, tg, spr);
+                lastTan = tg;
+            }
+            ++lines;
+            ++preChop;
+            endpointsSum = endpointsSum + movePt;
+        }
+        if (closed && roundJoin && preChop != 0)
+            polarSum += polar_count(lastTan, firstTan, spr);
+        uint32_t strokeJoinCount = preChop;
+        if (!closed)
+            strokeJoinCount = (strokeJoinCount > 1u ? strokeJoinCount : 1u) - 1u;
+        const bool empty = lines == 0 && curves == 0;
+        uint32_t v = lines * 2 + paramSum;
+        uint32_t capSegs = 0;
+        uint32_t capFlags = 0;
+        if (isStroke)
+        {
+            v += polarSum;
+            if (roundJoin)
+                v -= strokeJoinCount;
+            else
+                v += strokeJoinCount * (kMiterBevelJoinSegments - 1);
+            uint32_t cap = d.cap;
+            bool needsCaps = false;
+            if (!empty)
+                needsCaps = !closed;
+            else
+            {
+                // empty_stroke_cap (draw.cpp:253-281)
+                if (closed)
+                    cap = d.join == 1 ? 1u : (d.join == 0 ? 2u : 0u);
+                needsCaps = cap != 0u;
+            }
+            if (needsCaps)
+            {
+                if (cap == 1u)
+                {
+                    float n = ceilf(spr * kMathPI);
+                    n += 2.f;
+                    n = fminf(n, 1023.f);
+                    capSegs = (uint32_t)n;
+                }
+                else
+                {
+                    capSegs = kMiterBevelJoinSegments;
+                }
+                v += (capSegs - 1u) * 2u;
+                capFlags = (cap == 0u ? BEVEL_JOIN_CONTOUR_FLAG
+                                      : (cap == 2u ? MITER_CLIP_JOIN_CONTOUR_FLAG : ROUND_JOIN_CONTOUR_FLAG)) |
+                           EMULATED_STROKE_CAP_CONTOUR_FLAG;
+            }
+        }
+        else
+        {
+            v += curves;
+        }
+        const uint32_t padding = (kPatchSpan - (v % kPatchSpan)) % kPatchSpan;
+        v += padding;
+        out.tessVertexCount += v;
+        out.contours += 1;
+        // records in this contour
+        uint32_t recs = (capSegs != 0 && !empty) ? 1u : 0u;
+        recs += (capSegs != 0 && empty) ? 2u : 0u;
+        recs += lines - (implicitClose ? 1u : 0u); // explicit lines
+        recs += curves;
+        recs += implicitClose ? 1u : 0u;
+        out.records += recs;
+
+        if (EMIT)
+        {
+            // ---- walk 2: records (draw.cpp:1992-2375)
+            e->nextPad = padding;
+            ContourV cdat;
+            cdat.mx = isStroke ? (closed ? 1.f : 0.f) : endpointsSum.x * (1.f / (float)preChop);
+            cdat.my = isStroke ? 0.f : endpointsSum.y * (1.f / (float)preChop);
+            cdat.pathID = d.path_id;
+            cdat.vertexIndex0 = e->dirs != 1u ? e->fwd : e->mir - 1u;
+            if (e->contourIndex < e->contourCap)
+                e->contours[e->contourIndex] = cdat;
+            else
+                atomicOr(e->overflow, 1u);
+            ++e->contourIndex;
+            const uint32_t cid = (d.contour_flags | e->contourID);
+            ++e->contourID;
+            uint32_t joinTypeFlags =
+                isStroke ? (d.join == 0 ? MITER_REVERT_JOIN_CONTOUR_FLAG
+                                        : (d.join == 1 ? ROUND_JOIN_CONTOUR_FLAG : BEVEL_JOIN_CONTOUR_FLAG))
+                         : 0u;
+            bool needsFirstCap = capSegs != 0;
+            V2d joinTangent{0.f, 1.f};
+            uint32_t joinSegs = 1;
+            const V2d* cpts = P + ps;
+            const V2d* cend = P + pe;
+            // Tangent of the first verb after `k` (for round joins), or the
+            // closing line's / contour start's tangent.
+            V2d lastT{0.f, 1.f};
+            auto capBefore = [&](const V2d* cubic) {
+                V2d rev[4] = {cubic[3], cubic[2], cubic[1], cubic[0]};
+                push_cubic(e, rev, tan0(cubic), 0, 0, capSegs, cid | capFlags);
+            };
+            // find_join_tangent (draw.cpp:210-251)
+            auto joinTangentAt = [&](const V2d* jp) -> V2d {
+                const V2d* np = jp + 1;
+                np = np != cend ? np : cpts;
+                V2d tg = *np - *jp;
+                if (tg.x != 0.f || tg.y != 0.f)
+                    return tg;
+                for (const V2d* s = jp + 1; s != cend; ++s)
+                    if (vne(*s, *jp))
+                        return *s - *jp;
+                if (closed)
+                    for (const V2d* s = cpts; s != jp; ++s)
+                        if (vne(*s, *jp))
+                            return *s - *jp;
+                return V2d{0.f, 0.f};
+            };
+            auto endingTangent = [&]() -> V2d { // draw.cpp:195-208
+                const V2d ep = cend[-1];
+                const V2d* s = cend;
+                while (--s > cpts)
+                {
+                    V2d pp = s[-1];
+                    if (vne(pp, ep))
+                        return ep - pp;
+                }
+                return V2d{-1.f, 0.f};
+            };
+            auto startingTangent = [&]() -> V2d { // draw.cpp:178-191
+                const V2d p0 = cpts[0];
+                for (const V2d* s = cpts + 1; s < cend; ++s)
+                    if (vne(*s, p0))
+                        return *s - p0;
+                return V2d{1.f, 0.f};
+            };
+            // First tangent of the verb at index k (k < ce), for round joins.
+            auto verbFirstTangent = [&](uint32_t k, uint32_t q) -> V2d {
+                const V2d* pp = P + q - 1;
+                if (verbs[k] == 1)
+                    return pp[1] - pp[0];
+                return tan0(pp);
+            };
+            uint32_t q = ps + 1;
+            for (uint32_t k = cs + 1; k < ce; ++k)
+            {
+                uint8_t vb = verbs[k];
+                if (vb != 1 && vb != 4)
+                {
+                    if (vb == 2)
+                        q += 2;
+                    continue;
+                }
+                const bool isFinal = k + 1 == ce || (k + 2 == ce && verbs[k + 1] == 5);
+                const uint32_t qn = q + (vb == 1 ? 1u : 3u);
+                // Flags of the verb's last record: the contour's join type, or
+                // the emulated cap at the end of an open contour.
+                uint32_t lastFlags = joinTypeFlags;
+                // The join that follows this verb.
+                if (isStroke)
+                {
+                    if (closed || !isFinal)
+                    {
+                        if (roundJoin)
+                        {
+                            V2d myLast = vb == 1 ? (P[q] - P[q - 1]) : tan1(P + q - 1);
+                            V2d nextFirst;
+                            // next line/cubic verb in the contour, else the closing line or firstTan
+                            uint32_t kk = k + 1, qq = qn;
+                            while (kk < ce && verbs[kk] != 1 && verbs[kk] != 4)
+                            {
+                                if (verbs[kk] == 2)
+                                    qq += 2;
+                                ++kk;
+                            }
+                            if (kk < ce)
+                                nextFirst = verbFirstTangent(kk, qq);
+                            else if (implicitClose)
+                                nextFirst = movePt - finalPt;
+                            else
+                                nextFirst = firstTan;
+                            joinTangent = nextFirst;
+                            joinSegs = polar_count(myLast, nextFirst, spr);
+                        }
+                        else
+                        {
+                            joinTangent = joinTangentAt(P + qn - 1);
+                            joinSegs = kMiterBevelJoinSegments;
+                        }
+                    }
+                    else
+                    {
+                        V2d et = endingTangent();
+                        joinTangent = V2d{-et.x, -et.y};
+                        lastFlags = capFlags;
+                        joinSegs = capSegs;
+                    }
+                }
+                if (vb == 1)
+                {
+                    const V2d* lp = P + q - 1;
+                    const float third = 1.f / 3.f;
+                    V2d cub[4] = {lp[0], (lp[1] - lp[0]) * third + lp[0], (lp[0] - lp[1]) * third + lp[1],
+                                  lp[1]};
+                    if (needsFirstCap)
+                    {
+                        capBefore(cub);
+                        needsFirstCap = false;
+                    }
+                    push_cubic(e, cub, joinTangent, 1, 1, joinSegs, cid | lastFlags);
+                }
+                else
+                {
+                    const V2d* cp = P + q - 1;
+                    if (!isStroke)
+                    {
+                        push_cubic(e, cp, V2d{0.f, 0.f}, parametric_count(wang_pow4(cp, prec, d.matrix)), 1, 1,
+                                   cid);
+                    }
+                    else
+                    {
+                        V2d sub[16];
+                        int n = stroke_subcubics(cp, maxScale, sub);
+                        if (needsFirstCap)
+                        {
+                            capBefore(sub);
+                            needsFirstCap = false;
+                        }
+                        // Chops before the last one keep the previous join
+                        // tangent and the contour's join type.
+                        for (int s = 0; s + 1 < n; ++s)
+                        {
+                            const V2d* sc = sub + 3 * s;
+                            push_cubic(e, sc, lastT, parametric_count(wang_pow4(sc, prec, d.matrix)),
+                                       polar_count(tan0(sc), tan1(sc), spr), 1, cid | joinTypeFlags);
+                        }
+                        const V2d* sc = sub + 3 * (n - 1);
+                        push_cubic(e, sc, joinTangent, parametric_count(wang_pow4(sc, prec, d.matrix)),
+                                   polar_count(tan0(sc), tan1(sc), spr), joinSegs, cid | lastFlags);
+                    }
+                }
+                lastT = joinTangent;
+                q = qn;
+            }
+            if (needsFirstCap)
+            {
+                V2d p0 = cpts[0], left{p0.x - 1.f, p0.y}, right{p0.x + 1.f, p0.y};
+                V2d a[4] = {p0, right, right, right};
+                V2d b[4] = {p0, left, left, left};
+                capBefore(a);
+                capBefore(b);
+            }
+            else if (closed && implicitClose)
+            {
+                V2d cub[4];
+                const float third = 1.f / 3.f;
+                cub[0] = finalPt;
+                cub[1] = (movePt - finalPt) * third + finalPt;
+                cub[2] = (finalPt - movePt) * third + movePt;
+                cub[3] = movePt;
+                if (roundJoin)
+                {
+                    joinTangent = firstTan;
+                    joinSegs = polar_count(movePt - finalPt, firstTan, spr);
+                }
+                else if (isStroke)
+                {
+                    joinTangent = startingTangent();
+                    joinSegs = kMiterBevelJoinSegments;
+                }
+                push_cubic(e, cub, joinTangent, 1, 1, joinSegs, cid | joinTypeFlags);
+            }
+        }
+        vi = ce;
+        pi = pe;
+    }
+    return out;
+}
+
+__global__ void count_kernel(const rive_hip_draw* __restrict__ draws, uint32_t n,
+                             const uint8_t* __restrict__ verbs, uint64_t verbCapacity,
+                             const V2d* __restrict__ points, uint64_t pointCapacity,
+                             uint32_t* __restrict__ invalid,
+                             uint32_t* __restrict__ tessCounts, uint32_t* __restrict__ contourCounts,
+                             uint32_t* __restrict__ recordCounts)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const rive_hip_draw d = draws[i];
+    // Validate the host-provided ranges before walking them: the verbs must
+    // stay inside the VERB buffer and consume exactly point_count points.
+    bool ok = (uint64_t)d.first_verb + d.verb_count <= verbCapacity &&
+              (uint64_t)d.first_point + d.point_count <= pointCapacity && d.verb_count > 0 &&
+              verbs[d.first_verb] == 0;
+    if (ok)
+    {
+        uint32_t pts = 0;
+        for (uint32_t k = 0; k < d.verb_count; ++k)
+        {
+            uint8_t v = verbs[d.first_verb + k];
+            pts += v == 0 ? 1u : v == 1 ? 1u : v == 2 ? 2u : v == 4 ? 3u : 0u;
+            if (v == 3 || v > 5)
+                ok = false;
+        }
+        ok = ok && pts == d.point_count;
+    }
+    if (!ok)
+    {
+        atomicAdd(invalid, 1u);
+        tessCounts[i] = 0;
+        contourCounts[i] = 0;
+        recordCounts[i] = 0;
+        return;
+    }
+    DrawCounts c = walk_draw<false>(d, verbs + d.first_verb, points + d.first_point, nullptr);
+    // midpointFanTessVertexCount: doubled for double-sided draws (draw.cpp:1384-1390)
+    const bool doubleSided = d.contour_directions == 2u || d.contour_directions == 3u;
+    tessCounts[i] = doubleSided ? c.tessVertexCount * 2u : c.tessVertexCount;
+    contourCounts[i] = c.contours;
+    recordCounts[i] = c.records;
+}
+
+__global__ void emit_kernel(rive_hip_draw* __restrict__ draws, uint32_t n, const uint8_t* __restrict__ verbs,
+                            const V2d* __restrict__ points, const uint32_t* __restrict__ tessOffsets,
+                            const uint32_t* __restrict__ contourOffsets,
+                            const uint32_t* __restrict__ recordOffsets, uint32_t firstLocation,
+                            CubicRec* __restrict__ recs, ContourV* __restrict__ contours,
+                            uint32_t* __restrict__ overflow)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    rive_hip_draw d = draws[i];
+    const uint32_t loc = firstLocation + tessOffsets[i];
+    const uint32_t count = tessOffsets[i + 1] - tessOffsets[i];
+    Emit e;
+    e.recs = recs + recordOffsets[i];
+    e.contours = contours + contourOffsets[i];
+    e.overflow = overflow;
+    e.recCap = recordOffsets[i + 1] - recordOffsets[i];
+    e.contourCap = contourOffsets[i + 1] - contourOffsets[i];
+    e.rec = 0;
+    e.contourIndex = 0;
+    e.contourID = contourOffsets[i] + 1u; // contour IDs are 1-based per flush
+    e.dirs = d.contour_directions;
+    e.nextPad = 0;
+    // pushTessellationData (draw.cpp:1899-1990)
+    switch (d.contour_directions)
+    {
+        case 0:
+            e.fwd = loc;
+            e.mir = 0;
+            break;
+        case 1:
+            e.fwd = 0;
+            e.mir = loc + count;
+            break;
+        case 3: // reverseThenForward
+            e.fwd = e.mir = loc + count / 2u;
+            break;
+        default: // forwardThenReverse
+            e.fwd = loc;
+            e.mir = loc + count;
+            break;
+    }
+    walk_draw<true>(d, verbs + d.first_verb, points + d.first_point, &e);
+    if (e.rec != e.recCap || e.contourIndex != e.contourCap)
+        atomicOr(overflow, 1u);
+    draws[i].tess_location = loc;
+    draws[i].tess_vertex_count = count;
+}
+
+// Row wrapping (render_context.cpp:3271-3402): number of spans of a record.
+__device__ __forceinline__ uint32_t record_span_count(const CubicRec& r)
+{
+    const int32_t W = (int32_t)kTessTextureWidth;
+    uint32_t n = 0;
+    if (r.dirs == 0u)
+    {
+        int32_t x1 = (int32_t)(r.fwdLoc % kTessTextureWidth) + (int32_t)r.total;
+        for (;;)
+        {
+            ++n;
+            if (x1 > W)
+            {
+                x1 -= W;
+                continue;
+            }
+            break;
+        }
+    }
+    else if (r.dirs == 1u)
+    {
+        int32_t rx1 = (int32_t)((r.mirLoc - 1u) % kTessTextureWidth + 1u) - (int32_t)r.total;
+        for (;;)
+        {
+            ++n;
+            if (rx1 < 0)
+            {
+                rx1 += W;
+                continue;
+            }
+            break;
+        }
+    }
+    else
+    {
+        int32_t x1 = (int32_t)(r.fwdLoc % kTessTextureWidth) + (int32_t)r.total;
+        int32_t rx1 = (int32_t)((r.mirLoc - 1u) % kTessTextureWidth + 1u) - (int32_t)r.total;
+        for (;;)
+        {
+            ++n;
+            if (x1 > W || rx1 < 0)
+            {
+                x1 -= W;
+                rx1 += W;
+                continue;
+            }
+            break;
+        }
+    }
+    return n;
+}
+
+__global__ void span_count_kernel(const CubicRec* __restrict__ recs, uint32_t n, uint32_t* __restrict__ counts)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    counts[i] = record_span_count(recs[i]);
+}
+
+__device__ __forceinline__ void write_span(SpanV* out, const float* pts, const float* jt, float y, int32_t x0,
+                                           int32_t x1, float ry, int32_t rx0, int32_t rx1, uint32_t counts,
+                                           uint32_t flags)
+{
+    SpanV s;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        s.p[k] = pts[k];
+    s.jt[0] = jt[0];
+    s.jt[1] = jt[1];
+    s.y = y;
+    s.ry = ry;
+    s.x0x1 = (int32_t)(((uint32_t)x1 << 16) | ((uint32_t)x0 & 0xffffu));
+    s.rx0x1 = (int32_t)(((uint32_t)rx1 << 16) | ((uint32_t)rx0 & 0xffffu));
+    s.counts = counts;
+    s.flags = flags;
+    *out = s;
+}
+
+__global__ void span_write_kernel(const CubicRec* __restrict__ recs, uint32_t n,
+                                  const uint32_t* __restrict__ spanOffsets, uint32_t firstSpan,
+                                  SpanV* __restrict__ spans)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const CubicRec r = recs[i];
+    SpanV* out = spans + firstSpan + spanOffsets[i];
+    const int32_t W = (int32_t)kTessTextureWidth;
+    const float qnan = __uint_as_float(0x7fc00000u);
+    if (r.dirs == 0u)
+    {
+        uint32_t y = r.fwdLoc / kTessTextureWidth;
+        int32_t x0 = (int32_t)(r.fwdLoc % kTessTextureWidth);
+        int32_t x1 = x0 + (int32_t)r.total;
+        for (;;)
+        {
+            write_span(out++, r.pts, r.jt, (float)y, x0, x1, qnan, -1, -1, r.counts, r.flags);
+            if (x1 > W)
+            {
+                ++y;
+                x0 -= W;
+                x1 -= W;
+                continue;
+            }
+            break;
+        }
+    }
+    else if (r.dirs == 1u)
+    {
+        uint32_t ry = (r.mirLoc - 1u) / kTessTextureWidth;
+        int32_t rx0 = (int32_t)((r.mirLoc - 1u) % kTessTextureWidth + 1u);
+        int32_t rx1 = rx0 - (int32_t)r.total;
+        for (;;)
+        {
+            write_span(out++, r.pts, r.jt, (float)ry, rx0, rx1, qnan, -1, -1, r.counts, r.flags);
+            if (rx1 < 0)
+            {
+                --ry;
+                rx0 += W;
+                rx1 += W;
+                continue;
+            }
+            break;
+        }
+    }
+    else
+    {
+        int32_t y = (int32_t)(r.fwdLoc / kTessTextureWidth);
+        int32_t x0 = (int32_t)(r.fwdLoc % kTessTextureWidth);
+        int32_t x1 = x0 + (int32_t)r.total;
+        uint32_t ry = (r.mirLoc - 1u) / kTessTextureWidth;
+        int32_t rx0 = (int32_t)((r.mirLoc - 1u) % kTessTextureWidth + 1u);
+        int32_t rx1 = rx0 - (int32_t)r.total;
+        for (;;)
+        {
+            write_span(out++, r.pts, r.jt, (float)y, x0, x1, (float)ry, rx0, rx1, r.counts, r.flags);
+            if (x1 > W || rx1 < 0)
+            {
+                ++y;
+                x0 -= W;
+                x1 -= W;
+                --ry;
+                rx0 += W;
+                rx1 += W;
+                continue;
+            }
+            break;
+        }
+    }
+}
+
+// Padding spans (render_context.cpp:1550-1568): 8 vertices at 0 and one after
+// the last path.
+__global__ void padding_span_kernel(SpanV* __restrict__ spans, const uint32_t* __restrict__ tessTotal,
+                                    uint32_t firstLocation)
+{
+    const float zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const float qnan = __uint_as_float(0x7fc00000u);
+    const uint32_t counts = (1u << 20); // join 1, polar 0, parametric 0
+    if (threadIdx.x == 0)
+        write_span(&spans[0], zero, zero, 0.f, 0, (int32_t)kPatchSpan, qnan, -1, -1, counts, 0u);
+    if (threadIdx.x == 1)
+    {
+        uint32_t endLoc = firstLocation + *tessTotal;
+        uint32_t y = endLoc / kTessTextureWidth;
+        int32_t x0 = (int32_t)(endLoc % kTessTextureWidth);
+        write_span(&spans[1], zero, zero, (float)y, x0, x0 + 1, qnan, -1, -1, counts, 0u);
+    }
+}
+} // namespace rivehip
+
 namespace
 {
+// GEOMETRY-mode front half: count -> scan -> emit -> spans.  Leaves spans and
+// contours in ctx scratch and the tessellation ranges in the DRAW buffer.
 int geometry_counting(rive_hip_context* ctx, const rive_hip_flush_desc* d, const rivehip::SpanV** spans,
                       uint32_t* spanCount, const rivehip::ContourV** contours, uint32_t* tessRows)
 {
-    (void)ctx;
-    (void)d;
-    (void)spans;
-    (void)spanCount;
-    (void)contours;
-    (void)tessRows;
-    return fail(RIVE_HIP_ERROR_UNSUPPORTED, "geometry mode not built yet");
+    using namespace rivehip;
+    const uint32_t n = d->draw_count;
+    auto* draws = reinterpret_cast<rive_hip_draw*>(ctx->bufs[RIVE_HIP_BUFFER_DRAW].dev);
+    const auto* verbs = reinterpret_cast<const uint8_t*>(ctx->bufs[RIVE_HIP_BUFFER_VERB].dev);
+    const auto* points = reinterpret_cast<const V2d*>(ctx->bufs[RIVE_HIP_BUFFER_POINT].dev);
+    if (n > 0 && (verbs == nullptr || points == nullptr))
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "GEOMETRY flush without VERB/POINT buffers");
+    uint32_t* scal = P<uint32_t>(ctx->scalars);
+    // Per-draw counts: tess, contours, records (3 arrays of n) + 3 offset arrays of n+1.
+    HIP_TRY(ensure(ctx->geoCounts, (size_t)(6 * (n + 1) + 16) * 4));
+    uint32_t* tessCounts = P<uint32_t>(ctx->geoCounts);
+    uint32_t* contourCounts = tessCounts + (n + 1);
+    uint32_t* recordCounts = contourCounts + (n + 1);
+    uint32_t* tessOffsets = recordCounts + (n + 1);
+    uint32_t* contourOffsets = tessOffsets + (n + 1);
+    uint32_t* recordOffsets = contourOffsets + (n + 1);
+    if (n > 0)
+        count_kernel<<<(n + 127) / 128, 128, 0, ctx->stream>>>(
+            draws, n, verbs, ctx->bufs[RIVE_HIP_BUFFER_VERB].size, points,
+            ctx->bufs[RIVE_HIP_BUFFER_POINT].size / sizeof(V2d), scal + 12, tessCounts, contourCounts,
+            recordCounts);
+    HIP_TRY(hipGetLastError());
+    int rc = device_scan(ctx, tessCounts, n, tessOffsets, scal + 8);
+    if (rc)
+        return rc;
+    rc = device_scan(ctx, contourCounts, n, contourOffsets, scal + 9);
+    if (rc)
+        return rc;
+    rc = device_scan(ctx, recordCounts, n, recordOffsets, scal + 10);
+    if (rc)
+        return rc;
+    HIP_TRY(hipMemcpyAsync(ctx->hostScalars, scal + 8, 20, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->hostScalars[4] != 0)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT,
+                    "draw record with verb/point ranges outside the VERB/POINT buffers");
+    const uint32_t tessTotal = ctx->hostScalars[0];
+    const uint32_t contourTotal = ctx->hostScalars[1];
+    const uint32_t recordTotal = ctx->hostScalars[2];
+    const uint32_t firstLocation = d->first_path_tess_location ? d->first_path_tess_location : kPatchSpan;
+    if (contourTotal > 0xffffu)
+        return fail(RIVE_HIP_ERROR_UNSUPPORTED, "more than 65535 contours in one flush");
+    HIP_TRY(ensure(ctx->geoRecords, (size_t)std::max<uint32_t>(recordTotal, 1) * sizeof(CubicRec)));
+    HIP_TRY(ensure(ctx->geoContours, (size_t)std::max<uint32_t>(contourTotal, 1) * sizeof(ContourV)));
+    if (n > 0)
+        emit_kernel<<<(n + 127) / 128, 128, 0, ctx->stream>>>(draws, n, verbs, points, tessOffsets,
+                                                              contourOffsets, recordOffsets, firstLocation,
+                                                              P<CubicRec>(ctx->geoRecords),
+                                                              P<ContourV>(ctx->geoContours), scal + 13);
+    HIP_TRY(hipGetLastError());
+    uint32_t overflow = 0;
+    rc = read_scalar(ctx, scal + 13, &overflow);
+    if (rc)
+        return rc;
+    if (overflow != 0)
+        return fail(RIVE_HIP_ERROR_DEVICE, "curve emission disagreed with the counting pass");
+    // spans
+    HIP_TRY(ensure(ctx->geoSpanCounts, (size_t)(recordTotal + 1) * 4));
+    HIP_TRY(ensure(ctx->geoSpanOffsets, (size_t)(recordTotal + 1) * 4));
+    if (recordTotal > 0)
+        span_count_kernel<<<(recordTotal + 255) / 256, 256, 0, ctx->stream>>>(
+            P<CubicRec>(ctx->geoRecords), recordTotal, P<uint32_t>(ctx->geoSpanCounts));
+    rc = device_scan(ctx, P<uint32_t>(ctx->geoSpanCounts), recordTotal, P<uint32_t>(ctx->geoSpanOffsets),
+                     scal + 11);
+    if (rc)
+        return rc;
+    uint32_t spanTotal = 0;
+    rc = read_scalar(ctx, scal + 11, &spanTotal);
+    if (rc)
+        return rc;
+    const uint32_t allSpans = spanTotal + 2; // + padding spans
+    HIP_TRY(ensure(ctx->geoSpans, (size_t)allSpans * sizeof(SpanV)));
+    padding_span_kernel<<<1, 64, 0, ctx->stream>>>(P<SpanV>(ctx->geoSpans), scal + 8, firstLocation);
+    if (recordTotal > 0)
+        span_write_kernel<<<(recordTotal + 255) / 256, 256, 0, ctx->stream>>>(
+            P<CubicRec>(ctx->geoRecords), recordTotal, P<uint32_t>(ctx->geoSpanOffsets), 2u,
+            P<SpanV>(ctx->geoSpans));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
+    const uint32_t texelsUsed = firstLocation + tessTotal + 1u;
+    *tessRows = std::max<uint32_t>((texelsUsed + kTessTextureWidth - 1) / kTessTextureWidth, 1);
+    *spans = P<SpanV>(ctx->geoSpans);
+    *spanCount = allSpans;
+    *contours = P<ContourV>(ctx->geoContours);
+    ctx->lastContourCount = contourTotal;
+    ctx->lastRecordCount = recordTotal;
+    ctx->lastTessVertexCount = texelsUsed;
+    return RIVE_HIP_OK;
 }
 } // namespace

@@ -1,0 +1,712 @@
+// Clean-room front end: RiveRenderer (clip stack, clip IDs), RenderContext
+// (per-flush records, gradients) and RenderContextHIPImpl.  See
+// rive_renderer.hpp for the reference interfaces these mirror.
+#include "rive_renderer.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <utility>
+
+namespace rive
+{
+// ---------------------------------------------------------------- Mat2D
+Mat2D Mat2D::operator*(const Mat2D& b) const
+{
+    const float* a = m;
+    const float* c = b.m;
+    return Mat2D(a[0] * c[0] + a[2] * c[1], a[1] * c[0] + a[3] * c[1], a[0] * c[2] + a[2] * c[3],
+                 a[1] * c[2] + a[3] * c[3], a[0] * c[4] + a[2] * c[5] + a[4],
+                 a[1] * c[4] + a[3] * c[5] + a[5]);
+}
+
+bool Mat2D::invert(Mat2D* out) const
+{
+    double det = (double)m[0] * m[3] - (double)m[1] * m[2];
+    if (det == 0 || !std::isfinite(det))
+        return false;
+    double inv = 1 / det;
+    *out = Mat2D((float)(m[3] * inv), (float)(-m[1] * inv), (float)(-m[2] * inv), (float)(m[0] * inv),
+                 (float)((m[2] * (double)m[5] - m[3] * (double)m[4]) * inv),
+                 (float)((m[1] * (double)m[4] - m[0] * (double)m[5]) * inv));
+    return true;
+}
+
+float Mat2D::findMaxScale() const
+{
+    // src/math/mat2d_find_max_scale.cpp:24-67
+    const float xx = m[0], xy = m[1], yx = m[2], yy = m[3];
+    if (xy == 0 && yx == 0)
+        return std::max(fabsf(xx), fabsf(yy));
+    float a = xx * xx + xy * xy;
+    float b = xx * yx + yy * xy;
+    float c = yx * yx + yy * yy;
+    float bSqd = b * b;
+    const float eps = 1.f / (1 << 12);
+    float result;
+    if (bSqd <= eps * eps)
+        result = std::max(a, c);
+    else
+    {
+        float aminusc = a - c;
+        float apluscdiv2 = (a + c) * .5f;
+        float x = sqrtf(aminusc * aminusc + 4 * bSqd) * .5f;
+        result = apluscdiv2 + x;
+    }
+    if (!std::isfinite(result))
+        result = 0;
+    result = std::max(result, 0.f);
+    return sqrtf(result);
+}
+
+// ---------------------------------------------------------------- RawPath
+void RawPath::injectImplicitMoveIfNeeded()
+{
+    if (!m_contourIsOpen)
+        move(m_points.empty() ? Vec2D{0, 0} : m_points[m_lastMoveIdx]);
+}
+void RawPath::move(Vec2D p)
+{
+    m_contourIsOpen = true;
+    m_lastMoveIdx = m_points.size();
+    m_points.push_back(p);
+    m_verbs.push_back(PathVerb::move);
+}
+void RawPath::line(Vec2D p)
+{
+    injectImplicitMoveIfNeeded();
+    m_points.push_back(p);
+    m_verbs.push_back(PathVerb::line);
+}
+void RawPath::cubic(Vec2D a, Vec2D b, Vec2D c)
+{
+    injectImplicitMoveIfNeeded();
+    m_points.push_back(a);
+    m_points.push_back(b);
+    m_points.push_back(c);
+    m_verbs.push_back(PathVerb::cubic);
+}
+void RawPath::quad(Vec2D a, Vec2D b)
+{
+    // RawPath::quadToCubic (src/math/raw_path.cpp:484-495)
+    injectImplicitMoveIfNeeded();
+    Vec2D p0 = m_points.back();
+    auto lerp = [](Vec2D u, Vec2D v, float t) { return Vec2D{u.x + (v.x - u.x) * t, u.y + (v.y - u.y) * t}; };
+    cubic(lerp(p0, a, 2 / 3.f), lerp(b, a, 2 / 3.f), b);
+}
+void RawPath::close()
+{
+    if (m_contourIsOpen)
+    {
+        m_verbs.push_back(PathVerb::close);
+        m_contourIsOpen = false;
+    }
+}
+
+// ---------------------------------------------------------------- Gradient
+namespace
+{
+bool valid_stops(const float stops[], size_t count)
+{
+    if (count == 0)
+        return false;
+    for (size_t i = 0; i < count; ++i)
+    {
+        if (!(stops[i] >= 0 && stops[i] <= 1))
+            return false;
+        if (i > 0 && !(stops[i - 1] <= stops[i]))
+            return false;
+    }
+    return true;
+}
+} // namespace
+
+std::shared_ptr<Gradient> Gradient::MakeLinear(float sx, float sy, float ex, float ey, const ColorInt colors[],
+                                               const float stops[], size_t count)
+{
+    // renderer/src/gradient.cpp:38-108
+    if (!valid_stops(stops, count))
+        return nullptr;
+    auto g = std::make_shared<Gradient>();
+    g->paintType = 2;
+    g->colors.assign(colors, colors + count);
+    g->stops.assign(stops, stops + count);
+    float firstStop = stops[0], lastStop = stops[count - 1];
+    if ((firstStop != 0 || lastStop != 1) && lastStop - firstStop > (1.f / (1 << 12)))
+    {
+        float nsx = sx * (1.f - firstStop) + ex * firstStop;
+        float nsy = sy * (1.f - firstStop) + ey * firstStop;
+        float nex = sx * (1.f - lastStop) + ex * lastStop;
+        float ney = sy * (1.f - lastStop) + ey * lastStop;
+        sx = nsx, sy = nsy, ex = nex, ey = ney;
+        g->stops[0] = 0;
+        g->stops[count - 1] = 1;
+        if (count > 2)
+        {
+            float m = 1.f / (lastStop - firstStop);
+            float a = -firstStop * m;
+            for (size_t i = 1; i < count - 1; ++i)
+                g->stops[i] = stops[i] * m + a;
+            for (size_t i = 1; i < count - 1; ++i)
+                g->stops[i] = fmaxf(g->stops[i - 1], g->stops[i]);
+            for (size_t i = count - 2; i != 0; --i)
+                g->stops[i] = fminf(g->stops[i], g->stops[i + 1]);
+        }
+    }
+    float vx = ex - sx, vy = ey - sy;
+    float inv = 1.f / (vx * vx + vy * vy);
+    vx *= inv;
+    vy *= inv;
+    g->coeffs[0] = vx;
+    g->coeffs[1] = vy;
+    g->coeffs[2] = -(vx * sx + vy * sy);
+    return g;
+}
+
+std::shared_ptr<Gradient> Gradient::MakeRadial(float cx, float cy, float radius, const ColorInt colors[],
+                                               const float stops[], size_t count)
+{
+    // renderer/src/gradient.cpp:110-175
+    if (!valid_stops(stops, count))
+        return nullptr;
+    auto g = std::make_shared<Gradient>();
+    g->paintType = 3;
+    g->colors.assign(colors, colors + count);
+    g->stops.assign(stops, stops + count);
+    float lastStop = stops[count - 1];
+    if (lastStop != 1 && lastStop > (1.f / (1 << 12)))
+    {
+        g->stops[count - 1] = 1;
+        radius *= lastStop;
+        float inverseLastStop = 1.f / lastStop;
+        for (size_t i = 0; i < count - 1; ++i)
+            g->stops[i] = stops[i] * inverseLastStop;
+        if (count > 1)
+        {
+            g->stops[0] = fmaxf(0, g->stops[0]);
+            for (size_t i = 1; i < count - 1; ++i)
+                g->stops[i] = fmaxf(g->stops[i - 1], g->stops[i]);
+            for (size_t i = count - 2; i != (size_t)-1; --i)
+                g->stops[i] = fminf(g->stops[i], g->stops[i + 1]);
+        }
+    }
+    g->coeffs[0] = cx;
+    g->coeffs[1] = cy;
+    g->coeffs[2] = radius;
+    return g;
+}
+
+// ---------------------------------------------------------------- RiveRenderer
+RiveRenderer::IAABB RiveRenderer::IAABB::intersect(const IAABB& o) const
+{
+    return {std::max(l, o.l), std::max(t, o.t), std::min(r, o.r), std::min(b, o.b)};
+}
+
+RiveRenderer::IAABB RiveRenderer::MapBounds(const Mat2D& m, const RawPath& raw, float ox, float oy)
+{
+    float l = INFINITY, t = INFINITY, r = -INFINITY, b = -INFINITY;
+    for (Vec2D p : raw.points())
+    {
+        Vec2D q = m.map(p);
+        l = std::min(l, q.x), t = std::min(t, q.y);
+        r = std::max(r, q.x), b = std::max(b, q.y);
+    }
+    l -= ox, r += ox, t -= oy, b += oy;
+    return {(int)floorf(l), (int)floorf(t), (int)ceilf(r), (int)ceilf(b)};
+}
+
+RiveRenderer::RiveRenderer(gpu::RenderContext* ctx) : m_context(ctx)
+{
+    RenderState s;
+    s.overallClipPixelBounds = {0, 0, (int)ctx->frameDescriptor().renderTargetWidth,
+                                (int)ctx->frameDescriptor().renderTargetHeight};
+    m_stack.push_back(s);
+}
+
+void RiveRenderer::save() { m_stack.push_back(m_stack.back()); }
+void RiveRenderer::restore()
+{
+    if (m_stack.size() > 1)
+        m_stack.pop_back();
+}
+void RiveRenderer::transform(const Mat2D& m) { m_stack.back().matrix = m_stack.back().matrix * m; }
+
+void RiveRenderer::clipPath(RenderPath* renderPath)
+{
+    // rive_renderer.cpp:199-228, 323-380 (clip rects are not advertised).
+    auto* path = static_cast<RiveRenderPath*>(renderPath);
+    RenderState& st = m_stack.back();
+    if (st.overallClipPixelBounds.empty())
+        return;
+    if (path->getRawPath().empty())
+    {
+        st.overallClipPixelBounds = {};
+        return;
+    }
+    const size_t h = st.clipStackHeight;
+    bool equivalent = m_clipStack.size() > h && m_clipStack[h].path == path && m_clipStack[h].matrix == st.matrix;
+    if (!equivalent)
+    {
+        IAABB pb = MapBounds(st.matrix, path->getRawPath(), 0, 0);
+        st.overallClipPixelBounds = st.overallClipPixelBounds.intersect(pb);
+        if (st.overallClipPixelBounds.empty())
+            return;
+        m_clipStack.resize(h);
+        m_clipStack.push_back({st.matrix, path, path->getFillRule(), pb, 0});
+    }
+    else
+    {
+        st.overallClipPixelBounds = st.overallClipPixelBounds.intersect(m_clipStack[h].pixelBounds);
+    }
+    st.clipStackHeight = h + 1;
+}
+
+void RiveRenderer::drawPath(RenderPath* renderPath, RenderPaint* renderPaint)
+{
+    // rive_renderer.cpp:121-197 and applyClip (:631-824), rasterOrdering.
+    auto* path = static_cast<RiveRenderPath*>(renderPath);
+    auto* paint = static_cast<RiveRenderPaint*>(renderPaint);
+    const RenderState& st = m_stack.back();
+    if (path->getRawPath().empty())
+        return;
+    if (paint->getIsStroked() && !(paint->getThickness() > 0))
+        return;
+    if (!(paint->getFeather() >= 0) || paint->getFeather() != 0)
+        return; // feathered paints are not supported by this backend yet
+    if (st.overallClipPixelBounds.empty())
+        return;
+    float ox = 0, oy = 0;
+    if (paint->getIsStroked())
+    {
+        // PathDraw::Make stroke outset (draw.cpp:452-494)
+        float outset = paint->getThickness() * .5f;
+        if (paint->getJoin() == StrokeJoin::miter)
+            outset *= 4.f;
+        else if (paint->getCap() == StrokeCap::square)
+            outset *= 1.41421356f;
+        Mat2D lin(st.matrix[0], st.matrix[1], st.matrix[2], st.matrix[3], 0, 0);
+        Vec2D c[4] = {{0, 0}, {outset, 0}, {0, outset}, {outset, outset}};
+        float l = INFINITY, t = INFINITY, r = -INFINITY, b = -INFINITY;
+        for (Vec2D p : c)
+        {
+            Vec2D q = lin.map(p);
+            l = std::min(l, q.x), t = std::min(t, q.y);
+            r = std::max(r, q.x), b = std::max(b, q.y);
+        }
+        ox = (r - l) + 1;
+        oy = (b - t) + 1;
+    }
+    const IAABB frame{0, 0, (int)m_context->frameDescriptor().renderTargetWidth,
+                      (int)m_context->frameDescriptor().renderTargetHeight};
+    IAABB pixelBounds = MapBounds(st.matrix, path->getRawPath(), ox, oy);
+    if (pixelBounds.intersect(frame).empty())
+        return;
+    if (pixelBounds.intersect(st.overallClipPixelBounds).empty())
+        return;
+    uint32_t clipID = 0;
+    const size_t h = st.clipStackHeight;
+    if (h > 0)
+    {
+        size_t inBuffer = (size_t)-1;
+        if (m_context->getClipContentID() != 0)
+            for (size_t i = h - 1; i != (size_t)-1; --i)
+                if (m_clipStack[i].clipID == m_context->getClipContentID())
+                {
+                    inBuffer = i;
+                    break;
+                }
+        uint32_t parent = inBuffer == (size_t)-1 ? 0 : m_clipStack[inBuffer].clipID;
+        for (size_t i = inBuffer + 1; i < h; ++i)
+        {
+            ClipElement& ce = m_clipStack[i];
+            ce.clipID = m_context->generateClipID();
+            if (!ce.pixelBounds.intersect(frame).empty())
+                m_context->pushPathDraw(ce.path, ce.matrix, nullptr, ce.clipID, true, parent, ce.fillRule);
+            parent = ce.clipID;
+        }
+        clipID = parent;
+        m_context->setClipContentID(parent);
+    }
+    m_context->pushPathDraw(path, st.matrix, paint, clipID, false, 0, path->getFillRule());
+}
+
+namespace gpu
+{
+namespace
+{
+uint32_t swizzle_rgba(uint32_t argb)
+{
+    uint32_t a = argb >> 24, r = (argb >> 16) & 0xff, g = (argb >> 8) & 0xff, b = argb & 0xff;
+    return r | (g << 8) | (b << 16) | (a << 24);
+}
+// calc_polar_segments_per_radian<kPolarPrecision> (bezier_utils.hpp:108-113)
+float polar_segments_per_radian(float r)
+{
+    float cosTheta = 1.f - (1.f / 8.f) / r;
+    return .5f / acosf(std::max(cosTheta, -1.f));
+}
+} // namespace
+
+void RenderContext::beginFrame(const FrameDescriptor& fd)
+{
+    m_frame = fd;
+    m_draws.clear();
+    m_lastClipID = 0;
+    m_clipContentID = 0;
+}
+
+void RenderContext::pushPathDraw(const RiveRenderPath* path, const Mat2D& m, const RiveRenderPaint* paint,
+                                 uint32_t clipID, bool clipUpdate, uint32_t outerClipID, FillRule fillRule)
+{
+    DrawRecord r{};
+    r.raw = &path->getRawPath();
+    const bool stroke = paint != nullptr && paint->getIsStroked();
+    float strokeRadius = stroke ? fmaxf(paint->getThickness() * .5f, 1.17549435e-38f) : 0.f;
+    memcpy(r.path.matrix, m.values(), sizeof(r.path.matrix));
+    r.path.stroke_radius = strokeRadius;
+    // PaintData (gpu.cpp:879-939)
+    uint32_t params;
+    if (clipUpdate)
+    {
+        params = 0; // clipUpdate paint type
+        r.paint.value = clipID << 16;
+        params |= outerClipID << 16;
+    }
+    else
+    {
+        const Gradient* g = paint->getGradient();
+        if (g != nullptr)
+        {
+            params = g->paintType;
+            r.gradient = g;
+        }
+        else
+        {
+            params = 1; // solid color
+            r.paint.value = swizzle_rgba(paint->getColor());
+        }
+        params |= clipID << 16;
+    }
+    if (!stroke)
+    {
+        if (fillRule == FillRule::nonZero)
+            params |= 0x100u;
+        else if (fillRule == FillRule::evenOdd)
+            params |= 0x200u;
+    }
+    r.paint.params = params;
+    // PathDraw's per-draw constants (draw.cpp:561-825)
+    rive_hip_draw& d = r.draw;
+    d.is_stroke = stroke ? 1 : 0;
+    d.join = stroke ? (uint32_t)paint->getJoin() : 0;
+    d.cap = stroke ? (uint32_t)paint->getCap() : 0;
+    memcpy(d.matrix, m.values(), sizeof(d.matrix));
+    d.stroke_radius = strokeRadius;
+    d.parametric_precision = 4.f;
+    if (stroke)
+    {
+        d.matrix_max_scale = m.findMaxScale();
+        d.polar_segments_per_radian = polar_segments_per_radian(strokeRadius * d.matrix_max_scale);
+        d.contour_directions = 0; // forward
+    }
+    else if (fillRule == FillRule::clockwise)
+    {
+        float det = m[0] * m[3] - m[2] * m[1];
+        d.contour_directions = det < 0 ? 2 : 3;
+        if (det < 0)
+            d.contour_flags |= 1u << 24; // NEGATE_PATH_FILL_COVERAGE_FLAG
+    }
+    else
+    {
+        d.contour_directions = 3; // reverseThenForward
+    }
+    m_draws.push_back(r);
+}
+
+bool RenderContext::flush(rive_hip_target* target)
+{
+    // LogicalFlush::layoutResources + writeResources (render_context.cpp:1116-2408)
+    FlushDescriptor fd;
+    fd.renderTarget = target;
+    fd.loadAction = m_frame.loadAction;
+    fd.clearColor = m_frame.clearColor;
+    fd.dither = m_frame.ditherMode;
+    fd.paths.push_back(rive_hip_path_data{});
+    rive_hip_paint_data clearPaint{};
+    clearPaint.params = 1;
+    clearPaint.value = swizzle_rgba(m_frame.clearColor);
+    fd.paints.push_back(clearPaint);
+    fd.paintAux.push_back(rive_hip_paint_aux{});
+
+    struct PendingGrad
+    {
+        uint32_t pathID;
+        bool complex;
+        uint32_t row, col;
+    };
+    std::vector<std::pair<uint32_t, uint32_t>> simple;
+    std::vector<const Gradient*> complex;
+    std::vector<PendingGrad> pending;
+    for (DrawRecord& r : m_draws)
+    {
+        const uint32_t pathID = (uint32_t)fd.paths.size();
+        r.draw.path_id = pathID;
+        r.draw.first_verb = (uint32_t)fd.verbs.size();
+        r.draw.verb_count = (uint32_t)r.raw->verbs().size();
+        r.draw.first_point = (uint32_t)(fd.points.size() / 2);
+        r.draw.point_count = (uint32_t)r.raw->points().size();
+        for (PathVerb v : r.raw->verbs())
+            fd.verbs.push_back((uint8_t)v);
+        for (Vec2D p : r.raw->points())
+        {
+            fd.points.push_back(p.x);
+            fd.points.push_back(p.y);
+        }
+        if (r.gradient != nullptr)
+        {
+            // PaintAux paint matrix (gpu.cpp:950-985) and ramp allocation
+            // (render_context.cpp:588-674).
+            const Gradient* g = r.gradient;
+            Mat2D view(r.path.matrix[0], r.path.matrix[1], r.path.matrix[2], r.path.matrix[3], r.path.matrix[4],
+                       r.path.matrix[5]);
+            Mat2D inv;
+            if (!view.invert(&inv))
+                inv = Mat2D();
+            Mat2D pm;
+            if (g->paintType == 2)
+                pm = Mat2D(g->coeffs[0], 0, g->coeffs[1], 0, g->coeffs[2], 0) * inv;
+            else
+            {
+                float w = 1 / g->coeffs[2];
+                pm = Mat2D(w, 0, 0, w, -g->coeffs[0] * w, -g->coeffs[1] * w) * inv;
+            }
+            memcpy(r.aux.paint_matrix, pm.values(), sizeof(r.aux.paint_matrix));
+            size_t n = g->stops.size();
+            PendingGrad pg{pathID, false, 0, 0};
+            if (n == 1 || (n == 2 && g->stops[0] == 0 && g->stops[1] == 1))
+            {
+                std::pair<uint32_t, uint32_t> key{g->colors[0], g->colors[std::min<size_t>(1, n - 1)]};
+                auto it = std::find(simple.begin(), simple.end(), key);
+                uint32_t idx;
+                if (it != simple.end())
+                    idx = (uint32_t)(it - simple.begin()) * 2;
+                else
+                {
+                    idx = (uint32_t)simple.size() * 2;
+                    simple.push_back(key);
+                }
+                pg.row = idx / 512;
+                pg.col = idx % 512;
+            }
+            else
+            {
+                uint32_t row = ~0u;
+                for (size_t i = 0; i < complex.size(); ++i)
+                    if (complex[i]->colors == g->colors && complex[i]->stops == g->stops)
+                        row = (uint32_t)i;
+                if (row == ~0u)
+                {
+                    row = (uint32_t)complex.size();
+                    complex.push_back(g);
+                }
+                pg.complex = true;
+                pg.row = row;
+            }
+            pending.push_back(pg);
+        }
+        fd.paths.push_back(r.path);
+        fd.paints.push_back(r.paint);
+        fd.paintAux.push_back(r.aux);
+        fd.draws.push_back(r.draw);
+    }
+    // Gradient spans (render_context.cpp:1462-1533).
+    const uint32_t simpleRows = (uint32_t)((simple.size() * 2 + 511) / 512);
+    const uint32_t complexRows = (uint32_t)complex.size();
+    const uint32_t height = std::max<uint32_t>(simpleRows + complexRows, 1);
+    const uint32_t ONE_TEXEL_FIXED = 65536 / 512;
+    for (size_t i = 0; i < simple.size(); ++i)
+    {
+        uint32_t y = (uint32_t)(i / 256);
+        uint32_t cx = (uint32_t)((i % 256) * 2 + 1) * ONE_TEXEL_FIXED;
+        fd.gradSpans.push_back({(cx << 16) | cx, 0x80000000u | 0x40000000u | y, simple[i].first, simple[i].second});
+    }
+    for (uint32_t i = 0; i < complexRows; ++i)
+    {
+        const Gradient* g = complex[i];
+        uint32_t y = i + simpleRows;
+        float m = (512 - 1.f) * ONE_TEXEL_FIXED;
+        float a = .5f * ONE_TEXEL_FIXED;
+        uint32_t lastX = (uint32_t)(g->stops[0] * m + a);
+        uint32_t lastColor = g->colors[0];
+        for (size_t k = 1; k < g->stops.size(); ++k)
+        {
+            uint32_t x = (uint32_t)(g->stops[k] * m + a);
+            uint32_t flags = 0x20000000u;
+            if (k == 1)
+                flags |= 0x80000000u;
+            if (k == g->stops.size() - 1)
+                flags |= 0x40000000u;
+            fd.gradSpans.push_back({(x << 16) | lastX, flags | y, lastColor, g->colors[k]});
+            lastColor = g->colors[k];
+            lastX = x;
+        }
+    }
+    const float invH = 1.f / height;
+    for (const PendingGrad& pg : pending)
+    {
+        uint32_t row = pg.row + (pg.complex ? simpleRows : 0);
+        float v = (static_cast<float>(row) + .5f) * invH;
+        memcpy(&fd.paints[pg.pathID].value, &v, 4);
+        float left = pg.complex ? 0.f : (float)pg.col;
+        float right = pg.complex ? 512.f : left + 2;
+        fd.paintAux[pg.pathID].grad_span[0] = (right - left - 1) * (1.f / 512);
+        fd.paintAux[pg.pathID].grad_span[1] = (left + .5f) * (1.f / 512);
+    }
+    fd.gradSimpleRows = simpleRows;
+    fd.gradComplexRows = complexRows;
+    bool ok = m_impl->flush(fd);
+    m_draws.clear();
+    return ok;
+}
+} // namespace gpu
+
+// ---------------------------------------------------------------- HIP impl
+bool RenderContextHIPImpl::upload(int kind, const void* data, size_t bytes)
+{
+    size_t need = std::max<size_t>(bytes, 16);
+    if (m_sizes[kind] < need)
+    {
+        // Grow like the reference's resource allocator: round up by 1.25x.
+        size_t grown = need + need / 4;
+        if (rive_hip_buffer_resize(m_ctx, kind, grown) != RIVE_HIP_OK)
+            return false;
+        m_sizes[kind] = grown;
+    }
+    void* p = rive_hip_buffer_map(m_ctx, kind, bytes);
+    if (p == nullptr && bytes > 0)
+        return false;
+    if (bytes > 0)
+        memcpy(p, data, bytes);
+    return rive_hip_buffer_unmap(m_ctx, kind, bytes) == RIVE_HIP_OK;
+}
+
+bool RenderContextHIPImpl::flush(const gpu::FlushDescriptor& fd)
+{
+    if (!upload(RIVE_HIP_BUFFER_PATH, fd.paths.data(), fd.paths.size() * sizeof(rive_hip_path_data)) ||
+        !upload(RIVE_HIP_BUFFER_PAINT, fd.paints.data(), fd.paints.size() * sizeof(rive_hip_paint_data)) ||
+        !upload(RIVE_HIP_BUFFER_PAINT_AUX, fd.paintAux.data(), fd.paintAux.size() * sizeof(rive_hip_paint_aux)) ||
+        !upload(RIVE_HIP_BUFFER_DRAW, fd.draws.data(), fd.draws.size() * sizeof(rive_hip_draw)) ||
+        !upload(RIVE_HIP_BUFFER_VERB, fd.verbs.data(), fd.verbs.size()) ||
+        !upload(RIVE_HIP_BUFFER_POINT, fd.points.data(), fd.points.size() * sizeof(float)) ||
+        !upload(RIVE_HIP_BUFFER_GRAD_SPAN, fd.gradSpans.data(), fd.gradSpans.size() * sizeof(rive_hip_grad_span)))
+        return false;
+    rive_hip_flush_desc d{};
+    d.target = fd.renderTarget;
+    d.mode = RIVE_HIP_FLUSH_GEOMETRY;
+    d.load_action = fd.loadAction;
+    d.clear_color = fd.clearColor;
+    d.dither = fd.dither;
+    d.path_count = (uint32_t)fd.paths.size() - 1;
+    d.draw_count = (uint32_t)fd.draws.size();
+    d.grad_span_count = (uint32_t)fd.gradSpans.size();
+    d.grad_simple_rows = fd.gradSimpleRows;
+    d.grad_complex_rows = fd.gradComplexRows;
+    d.first_path_tess_location = 8;
+    return rive_hip_flush(m_ctx, &d) == RIVE_HIP_OK;
+}
+} // namespace rive
+
+// Replays a rive_stream through RiveRenderer (the draw-stream contract in
+// include/rive_hip.h).
+namespace
+{
+struct StreamReplay
+{
+    std::vector<rive::RiveRenderPath> paths;
+    std::vector<rive::RiveRenderPaint> paints;
+};
+} // namespace
+
+extern "C" int rive_hip_render_stream(rive_hip_context* ctx, const rive_stream* s, rive_hip_target* target)
+{
+    if (ctx == nullptr || s == nullptr || target == nullptr)
+        return RIVE_HIP_ERROR_INVALID_ARGUMENT;
+    StreamReplay r;
+    r.paths.reserve(s->path_count);
+    for (uint32_t i = 0; i < s->path_count; ++i)
+    {
+        const rive_stream_path& sp = s->paths[i];
+        rive::RiveRenderPath p((rive::FillRule)sp.fill_rule);
+        const float* pts = s->points + 2 * (size_t)sp.first_point;
+        uint32_t pc = 0;
+        for (uint32_t k = 0; k < sp.verb_count; ++k)
+        {
+            switch (s->verbs[sp.first_verb + k])
+            {
+                case 0:
+                    p.moveTo(pts[2 * pc], pts[2 * pc + 1]);
+                    pc += 1;
+                    break;
+                case 1:
+                    p.lineTo(pts[2 * pc], pts[2 * pc + 1]);
+                    pc += 1;
+                    break;
+                case 2:
+                    p.mutableRawPath().quad({pts[2 * pc], pts[2 * pc + 1]}, {pts[2 * pc + 2], pts[2 * pc + 3]});
+                    pc += 2;
+                    break;
+                case 4:
+                    p.cubicTo(pts[2 * pc], pts[2 * pc + 1], pts[2 * pc + 2], pts[2 * pc + 3], pts[2 * pc + 4],
+                              pts[2 * pc + 5]);
+                    pc += 3;
+                    break;
+                case 5:
+                    p.close();
+                    break;
+                default:
+                    return RIVE_HIP_ERROR_INVALID_ARGUMENT;
+            }
+        }
+        r.paths.push_back(std::move(p));
+    }
+    rive::gpu::RenderContext rc(std::make_unique<rive::RenderContextHIPImpl>(ctx));
+    rive::gpu::RenderContext::FrameDescriptor fd;
+    fd.renderTargetWidth = s->width;
+    fd.renderTargetHeight = s->height;
+    fd.clearColor = s->clear_color;
+    fd.ditherMode = s->dither;
+    rc.beginFrame(fd);
+    rive::RiveRenderer renderer(&rc);
+    r.paints.resize(s->draw_count);
+    for (uint32_t i = 0; i < s->draw_count; ++i)
+    {
+        const rive_stream_draw& d = s->draws[i];
+        rive::RiveRenderPaint& paint = r.paints[i];
+        paint.style(d.style == 1 ? rive::RenderPaintStyle::stroke : rive::RenderPaintStyle::fill);
+        paint.thickness(d.thickness);
+        paint.join((rive::StrokeJoin)d.join);
+        paint.cap((rive::StrokeCap)d.cap);
+        paint.feather(d.feather);
+        if (d.paint_type == 0)
+            paint.color(d.color);
+        else
+        {
+            const uint32_t* colors = s->stop_colors + d.first_stop;
+            const float* stops = s->stop_offsets + d.first_stop;
+            auto g = d.paint_type == 1
+                         ? rive::Gradient::MakeLinear(d.grad[0], d.grad[1], d.grad[2], d.grad[3], colors, stops,
+                                                      d.stop_count)
+                         : rive::Gradient::MakeRadial(d.grad[0], d.grad[1], d.grad[2], colors, stops, d.stop_count);
+            if (g)
+                paint.shader(g);
+            else
+                paint.color(0);
+        }
+        renderer.save();
+        renderer.transform(rive::Mat2D(d.matrix[0], d.matrix[1], d.matrix[2], d.matrix[3], d.matrix[4], d.matrix[5]));
+        for (uint32_t c = 0; c < d.clip_count && c < 2; ++c)
+            renderer.clipPath(&r.paths[d.clip_paths[c]]);
+        renderer.drawPath(&r.paths[d.path], &paint);
+        renderer.restore();
+    }
+    return rc.flush(target) ? RIVE_HIP_OK : RIVE_HIP_ERROR_DEVICE;
+}

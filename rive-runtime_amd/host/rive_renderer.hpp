@@ -1,0 +1,300 @@
+// Clean-room C++ front end with the reference's API names:
+//   rive::Renderer / RiveRenderer          include/rive/renderer.hpp:209-236,
+//                                          renderer/include/rive/renderer/rive_renderer.hpp:26-144
+//   rive::RiveRenderPath / RiveRenderPaint  renderer/src/rive_render_path.hpp:18-120,
+//                                          renderer/src/rive_render_paint.hpp:19-86
+//   rive::gpu::RenderContext               renderer/include/rive/renderer/render_context.hpp:87-1057
+//   rive::gpu::RenderContextImpl           renderer/include/rive/renderer/render_context_impl.hpp:26-243
+// Draw streams written against these names run unchanged; RenderContext hands
+// each flush to a RenderContextImpl, and RenderContextHIPImpl sends it to the
+// gfx950 backend through include/rive_hip.h.
+#pragma once
+
+#include "../../include/rive_hip.h"
+
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+namespace rive
+{
+struct Vec2D
+{
+    float x = 0, y = 0;
+};
+inline Vec2D operator-(Vec2D a, Vec2D b) { return {a.x - b.x, a.y - b.y}; }
+
+// [xx, xy, yx, yy, tx, ty]: x' = xx*x + yx*y + tx, y' = xy*x + yy*y + ty.
+class Mat2D
+{
+public:
+    Mat2D() = default;
+    Mat2D(float xx, float xy, float yx, float yy, float tx, float ty) : m{xx, xy, yx, yy, tx, ty} {}
+    const float* values() const { return m; }
+    float operator[](int i) const { return m[i]; }
+    Vec2D map(Vec2D p) const { return {m[0] * p.x + m[2] * p.y + m[4], m[1] * p.x + m[3] * p.y + m[5]}; }
+    Mat2D operator*(const Mat2D& b) const;
+    bool invert(Mat2D* out) const;
+    float findMaxScale() const; // src/math/mat2d_find_max_scale.cpp:24-67
+    bool operator==(const Mat2D& o) const
+    {
+        for (int i = 0; i < 6; ++i)
+            if (m[i] != o.m[i])
+                return false;
+        return true;
+    }
+
+private:
+    float m[6] = {1, 0, 0, 1, 0, 0};
+};
+
+enum class FillRule : uint32_t { nonZero, evenOdd, clockwise };
+enum class StrokeJoin : uint32_t { miter = 0, round = 1, bevel = 2 };
+enum class StrokeCap : uint32_t { butt = 0, round = 1, square = 2 };
+enum class RenderPaintStyle : uint32_t { stroke, fill };
+enum class BlendMode : uint8_t { srcOver = 3 };
+using ColorInt = uint32_t;
+
+enum class PathVerb : uint8_t { move = 0, line = 1, quad = 2, cubic = 4, close = 5 };
+
+// Verb + point storage (include/rive/math/raw_path.hpp:25-297).
+class RawPath
+{
+public:
+    void move(Vec2D p);
+    void line(Vec2D p);
+    void cubic(Vec2D a, Vec2D b, Vec2D c);
+    void quad(Vec2D a, Vec2D b); // stored as the equivalent cubic
+    void close();
+    bool empty() const { return m_verbs.empty(); }
+    const std::vector<PathVerb>& verbs() const { return m_verbs; }
+    const std::vector<Vec2D>& points() const { return m_points; }
+
+private:
+    void injectImplicitMoveIfNeeded();
+    std::vector<PathVerb> m_verbs;
+    std::vector<Vec2D> m_points;
+    bool m_contourIsOpen = false;
+    size_t m_lastMoveIdx = 0;
+};
+
+class RenderShader
+{
+public:
+    virtual ~RenderShader() = default;
+};
+
+// Linear and radial gradients (renderer/src/gradient.cpp:38-175).
+class Gradient : public RenderShader
+{
+public:
+    static std::shared_ptr<Gradient> MakeLinear(float sx, float sy, float ex, float ey, const ColorInt colors[],
+                                                const float stops[], size_t count);
+    static std::shared_ptr<Gradient> MakeRadial(float cx, float cy, float radius, const ColorInt colors[],
+                                                const float stops[], size_t count);
+    uint32_t paintType = 0; // 2 linear, 3 radial
+    std::vector<ColorInt> colors;
+    std::vector<float> stops;
+    float coeffs[3] = {0, 0, 0};
+};
+
+class RenderPath
+{
+public:
+    virtual ~RenderPath() = default;
+};
+class RenderPaint
+{
+public:
+    virtual ~RenderPaint() = default;
+};
+
+class RiveRenderPath : public RenderPath
+{
+public:
+    explicit RiveRenderPath(FillRule rule = FillRule::nonZero) : m_fillRule(rule) {}
+    void moveTo(float x, float y) { m_raw.move({x, y}); }
+    void lineTo(float x, float y) { m_raw.line({x, y}); }
+    void cubicTo(float ox, float oy, float ix, float iy, float x, float y)
+    {
+        m_raw.cubic({ox, oy}, {ix, iy}, {x, y});
+    }
+    void close() { m_raw.close(); }
+    void fillRule(FillRule r) { m_fillRule = r; }
+    FillRule getFillRule() const { return m_fillRule; }
+    const RawPath& getRawPath() const { return m_raw; }
+    RawPath& mutableRawPath() { return m_raw; }
+
+private:
+    RawPath m_raw;
+    FillRule m_fillRule;
+};
+
+class RiveRenderPaint : public RenderPaint
+{
+public:
+    void style(RenderPaintStyle s) { m_stroked = s == RenderPaintStyle::stroke; }
+    void color(ColorInt c)
+    {
+        m_color = c;
+        m_shader.reset();
+    }
+    void thickness(float t) { m_thickness = t; }
+    void join(StrokeJoin j) { m_join = j; }
+    void cap(StrokeCap c) { m_cap = c; }
+    void feather(float f) { m_feather = f; }
+    void blendMode(BlendMode b) { m_blend = b; }
+    void shader(std::shared_ptr<Gradient> g) { m_shader = std::move(g); }
+    bool getIsStroked() const { return m_stroked; }
+    ColorInt getColor() const { return m_color; }
+    float getThickness() const { return m_thickness; }
+    StrokeJoin getJoin() const { return m_join; }
+    StrokeCap getCap() const { return m_cap; }
+    float getFeather() const { return m_feather; }
+    BlendMode getBlendMode() const { return m_blend; }
+    const Gradient* getGradient() const { return m_shader.get(); }
+
+private:
+    bool m_stroked = false;
+    ColorInt m_color = 0xff000000;
+    float m_thickness = 1;
+    StrokeJoin m_join = StrokeJoin::miter;
+    StrokeCap m_cap = StrokeCap::butt;
+    float m_feather = 0;
+    BlendMode m_blend = BlendMode::srcOver;
+    std::shared_ptr<Gradient> m_shader;
+};
+
+class Renderer
+{
+public:
+    virtual ~Renderer() = default;
+    virtual void save() = 0;
+    virtual void restore() = 0;
+    virtual void transform(const Mat2D& m) = 0;
+    virtual void drawPath(RenderPath* path, RenderPaint* paint) = 0;
+    virtual void clipPath(RenderPath* path) = 0;
+};
+
+namespace gpu
+{
+// One PathDraw's records (PathData/PaintData/PaintAux + counting inputs).
+struct DrawRecord
+{
+    rive_hip_path_data path;
+    rive_hip_paint_data paint;
+    rive_hip_paint_aux aux;
+    rive_hip_draw draw;
+    const RawPath* raw;
+    const Gradient* gradient; // null for solid/clip paints
+};
+
+// The flush descriptor handed to the backend (gpu.hpp:1320-1438, reduced).
+struct FlushDescriptor
+{
+    rive_hip_target* renderTarget = nullptr;
+    uint32_t loadAction = 0;
+    ColorInt clearColor = 0;
+    uint32_t dither = 1;
+    std::vector<rive_hip_path_data> paths; // [0] reserved
+    std::vector<rive_hip_paint_data> paints;
+    std::vector<rive_hip_paint_aux> paintAux;
+    std::vector<rive_hip_draw> draws;
+    std::vector<uint8_t> verbs;
+    std::vector<float> points;
+    std::vector<rive_hip_grad_span> gradSpans;
+    uint32_t gradSimpleRows = 0, gradComplexRows = 0;
+};
+
+// Backend plugin interface (render_context_impl.hpp:26-243, reduced to what
+// the GEOMETRY flush needs).
+class RenderContextImpl
+{
+public:
+    virtual ~RenderContextImpl() = default;
+    virtual bool flush(const FlushDescriptor& desc) = 0;
+};
+
+class RenderContext
+{
+public:
+    struct FrameDescriptor // render_context.hpp:102-144
+    {
+        uint32_t renderTargetWidth = 0, renderTargetHeight = 0;
+        uint32_t loadAction = 0; // 0 clear, 1 preserve
+        ColorInt clearColor = 0;
+        uint32_t ditherMode = 1;
+    };
+    explicit RenderContext(std::unique_ptr<RenderContextImpl> impl) : m_impl(std::move(impl)) {}
+    void beginFrame(const FrameDescriptor& fd);
+    bool flush(rive_hip_target* target);
+    const FrameDescriptor& frameDescriptor() const { return m_frame; }
+    RenderContextImpl* impl() { return m_impl.get(); }
+
+    // Used by RiveRenderer.
+    uint32_t generateClipID() { return ++m_lastClipID; }
+    uint32_t getClipContentID() const { return m_clipContentID; }
+    void setClipContentID(uint32_t id) { m_clipContentID = id; }
+    void pushPathDraw(const RiveRenderPath* path, const Mat2D& m, const RiveRenderPaint* paint,
+                      uint32_t clipID, bool clipUpdate, uint32_t outerClipID, FillRule fillRule);
+
+private:
+    std::unique_ptr<RenderContextImpl> m_impl;
+    FrameDescriptor m_frame;
+    std::vector<DrawRecord> m_draws;
+    uint32_t m_lastClipID = 0;
+    uint32_t m_clipContentID = 0;
+};
+} // namespace gpu
+
+class RiveRenderer : public Renderer
+{
+public:
+    explicit RiveRenderer(gpu::RenderContext* ctx);
+    void save() override;
+    void restore() override;
+    void transform(const Mat2D& m) override;
+    void drawPath(RenderPath* path, RenderPaint* paint) override;
+    void clipPath(RenderPath* path) override;
+
+private:
+    struct IAABB
+    {
+        int l = 0, t = 0, r = 0, b = 0;
+        bool empty() const { return l >= r || t >= b; }
+        IAABB intersect(const IAABB& o) const;
+    };
+    struct ClipElement
+    {
+        Mat2D matrix;
+        const RiveRenderPath* path;
+        FillRule fillRule;
+        IAABB pixelBounds;
+        uint32_t clipID;
+    };
+    struct RenderState
+    {
+        Mat2D matrix;
+        size_t clipStackHeight = 0;
+        IAABB overallClipPixelBounds;
+    };
+    static IAABB MapBounds(const Mat2D& m, const RawPath& raw, float outsetX, float outsetY);
+    gpu::RenderContext* m_context;
+    std::vector<RenderState> m_stack;
+    std::vector<ClipElement> m_clipStack;
+};
+
+// Backend that sends flushes to the gfx950 device (RIVE_HIP_FLUSH_GEOMETRY).
+class RenderContextHIPImpl : public gpu::RenderContextImpl
+{
+public:
+    explicit RenderContextHIPImpl(rive_hip_context* ctx) : m_ctx(ctx) {}
+    bool flush(const gpu::FlushDescriptor& desc) override;
+
+private:
+    bool upload(int kind, const void* data, size_t bytes);
+    rive_hip_context* m_ctx;
+    size_t m_sizes[RIVE_HIP_BUFFER_KIND_COUNT] = {};
+};
+} // namespace rive

@@ -67,12 +67,23 @@ class FlushDesc(ctypes.Structure):
                 ("first_path_tess_location", ctypes.c_uint32)]
 
 
+class FlushStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in (
+        "width", "height", "load_action", "mode", "path_count", "draw_count", "contour_count",
+        "span_count", "tess_vertex_count", "curve_record_count", "prim_count", "bin_entries",
+        "tile_count", "grad_simple_rows", "grad_complex_rows", "aux_path_count", "grad_span_count",
+        "first_path_tess_location", "grad_simple_ramps", "clear_color", "dither")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
 EXPORTED = [
     "rive_hip_last_error", "rive_hip_create", "rive_hip_destroy", "rive_hip_buffer_resize",
     "rive_hip_buffer_map", "rive_hip_buffer_unmap", "rive_hip_target_create",
     "rive_hip_target_destroy", "rive_hip_flush", "rive_hip_sync", "rive_hip_read_pixels",
     "rive_hip_debug_readback", "rive_hip_pass_times", "rive_stream_generate", "rive_stream_free",
-    "rive_hip_render_stream",
+    "rive_hip_render_stream", "rive_hip_last_flush_stats",
 ]
 
 _lib = None
@@ -108,6 +119,7 @@ def lib():
     L.rive_stream_free.argtypes = [ctypes.POINTER(Stream)]
     L.rive_stream_free.restype = None
     L.rive_hip_render_stream.argtypes = [vp, ctypes.POINTER(Stream), vp]
+    L.rive_hip_last_flush_stats.argtypes = [vp, ctypes.POINTER(FlushStats)]
     _lib = L
     return L
 
@@ -193,3 +205,26 @@ class Context:
 
     def render_stream(self, stream, target):
         check(self.L.rive_hip_render_stream(self.h, stream, target))
+
+    def last_flush_stats(self):
+        st = FlushStats()
+        check(self.L.rive_hip_last_flush_stats(self.h, ctypes.byref(st)))
+        return st
+
+    def reflush_geometry(self, target, stats):
+        """Flushes again from the records already resident in HBM (the last
+        GEOMETRY flush's PATH/PAINT/PAINT_AUX/DRAW/VERB/POINT/GRAD_SPAN
+        buffers), with no host writes."""
+        desc = FlushDesc()
+        desc.target = target
+        desc.mode = FLUSH_GEOMETRY
+        desc.load_action = stats.load_action
+        desc.clear_color = stats.clear_color
+        desc.dither = stats.dither
+        desc.path_count = stats.path_count
+        desc.draw_count = stats.draw_count
+        desc.grad_span_count = stats.grad_span_count
+        desc.grad_simple_rows = stats.grad_simple_rows
+        desc.grad_complex_rows = stats.grad_complex_rows
+        desc.first_path_tess_location = stats.first_path_tess_location
+        self.flush(desc)
