@@ -389,9 +389,10 @@ __device__ __forceinline__ float manhattan_pixel_width(const float M[4], float n
 }
 __device__ __forceinline__ int32_t snap(float v)
 {
-    float c = fminf(fmaxf(v * 256.f, -8388608.f), 8388607.f);
-    if (isnan(c))
-        c = 0.f;
+    // NaN snaps to 0 (before the clamp: fmin/fmax would turn it into a bound).
+    if (isnan(v))
+        return 0;
+    const float c = fminf(fmaxf(v * 256.f, -8388608.f), 8388607.f);
     return __float2int_rn(c);
 }
 
@@ -742,7 +743,8 @@ __global__ void bin_count_kernel(const Prim* __restrict__ prims, uint32_t totalP
     const int32_t ax = p.p[0], ay = p.p[1], bx = p.p[4], by = p.p[5];
     if (ay == by)
         return;
-    const int s = by < ay ? 1 : -1;
+    // NEGATE_PATH_FILL_COVERAGE negates the whole fan, backdrop included.
+    const int s = (by < ay ? 1 : -1) * ((p.flags & kPrimNegate) ? -1 : 1);
     const int32_t ymin = min(ay, by), ymax = max(ay, by);
     const int unit = 1 << (kSubpixelLog2 + kTileLog2);
     int tyA = max(c.ty0, (ymin + unit - 1) >> (kSubpixelLog2 + kTileLog2));
@@ -931,7 +933,7 @@ __global__ __launch_bounds__(256) void raster_kernel(
                         d += (int32_t)Rrow - (int32_t)Rcorner;
                     if (inRange)
                         d += (int32_t)Rq - (int32_t)Rrow;
-                    wind += s * d;
+                    wind += neg * s * d;
                 }
                 else if (p.p[0] != p.p[4])
                 {
@@ -943,7 +945,7 @@ __global__ __launch_bounds__(256) void raster_kernel(
                     {
                         const int32_t h = p.p[1];
                         const int sg = p.p[4] > p.p[0] ? 1 : -1;
-                        wind += sg * ((int32_t)(yq >= h) - (int32_t)(Y0 >= h));
+                        wind += neg * sg * ((int32_t)(yq >= h) - (int32_t)(Y0 >= h));
                     }
                 }
                 // Border triangle T1 = (I0, O0, I1), coverage 1 - lambda(O0).

@@ -1,0 +1,102 @@
+"""Hand-built draw streams (rive_stream, include/rive_hip.h) for edge-case
+and closed-form tests.
+
+    b = StreamBuilder(64, 64, dither=0)
+    sq = b.path([(0, 10, 10), (1, 30, 10), (1, 30, 30), (1, 10, 30), (5,)])
+    b.draw(sq, color=0xffff0000)
+    s = b.build()          # ctypes pointer, valid while `b` lives
+
+Verbs: (0, x, y) move, (1, x, y) line, (4, x1, y1, x2, y2, x3, y3) cubic,
+(5,) close.
+"""
+import ctypes
+
+import numpy as np
+
+import rivehip
+
+NONZERO, EVENODD, CLOCKWISE = 0, 1, 2
+FILL, STROKE = 0, 1
+MITER, ROUND, BEVEL = 0, 1, 2
+BUTT, ROUND_CAP, SQUARE = 0, 1, 2
+
+_PATH = np.dtype([("first_verb", "<u4"), ("verb_count", "<u4"), ("first_point", "<u4"),
+                  ("point_count", "<u4"), ("fill_rule", "<u4")])
+_DRAW = np.dtype([("path", "<u4"), ("matrix", "<f4", 6), ("style", "<u4"), ("thickness", "<f4"),
+                  ("join", "<u4"), ("cap", "<u4"), ("feather", "<f4"), ("blend_mode", "<u4"),
+                  ("paint_type", "<u4"), ("color", "<u4"), ("first_stop", "<u4"),
+                  ("stop_count", "<u4"), ("grad", "<f4", 4), ("clip_count", "<u4"),
+                  ("clip_paths", "<u4", 2)])
+assert _PATH.itemsize == 20 and _DRAW.itemsize == 96
+
+IDENTITY = (1.0, 0.0, 0.0, 1.0, 0.0, 0.0)
+
+
+def rect(x0, y0, x1, y1, clockwise=True):
+    """Closed rectangle; clockwise in y-down screen space by default."""
+    if clockwise:
+        return [(0, x0, y0), (1, x1, y0), (1, x1, y1), (1, x0, y1), (5,)]
+    return [(0, x0, y0), (1, x0, y1), (1, x1, y1), (1, x1, y0), (5,)]
+
+
+class StreamBuilder:
+    def __init__(self, width, height, clear_color=0, dither=0):
+        self.width, self.height = width, height
+        self.clear_color, self.dither = clear_color, dither
+        self.verbs, self.points, self.paths, self.draws = [], [], [], []
+        self.stop_colors, self.stop_offsets = [], []
+
+    def path(self, cmds, fill_rule=NONZERO):
+        first_verb, first_point = len(self.verbs), len(self.points) // 2
+        for c in cmds:
+            self.verbs.append(c[0])
+            self.points.extend(float(v) for v in c[1:])
+        self.paths.append((first_verb, len(self.verbs) - first_verb, first_point,
+                           len(self.points) // 2 - first_point, fill_rule))
+        return len(self.paths) - 1
+
+    def draw(self, path, color=0xff000000, matrix=IDENTITY, stroke=None, join=MITER, cap=BUTT,
+             gradient=None, clips=(), feather=0.0):
+        """stroke: width or None.  gradient: ("linear", sx, sy, ex, ey, stops)
+        or ("radial", cx, cy, r, stops) with stops [(offset, ARGB), ...]."""
+        d = np.zeros((), _DRAW)
+        d["path"] = path
+        d["matrix"] = matrix
+        d["style"] = STROKE if stroke is not None else FILL
+        d["thickness"] = stroke if stroke is not None else 0.0
+        d["join"], d["cap"], d["feather"], d["blend_mode"] = join, cap, feather, 3
+        d["color"] = color
+        if gradient is not None:
+            kind, *geo, stops = gradient
+            d["paint_type"] = 1 if kind == "linear" else 2
+            d["first_stop"], d["stop_count"] = len(self.stop_colors), len(stops)
+            for off, col in stops:
+                self.stop_offsets.append(off)
+                self.stop_colors.append(col)
+            g = list(geo) + [0.0] * (4 - len(geo))
+            d["grad"] = g
+        d["clip_count"] = len(clips)
+        for i, c in enumerate(clips):
+            d["clip_paths"][i] = c
+        self.draws.append(d)
+
+    def build(self):
+        self._verbs = np.asarray(self.verbs, np.uint8)
+        self._points = np.asarray(self.points, np.float32)
+        self._paths = np.array(self.paths, dtype=_PATH) if self.paths else np.zeros(0, _PATH)
+        self._draws = np.zeros(len(self.draws), _DRAW)
+        for i, d in enumerate(self.draws):
+            self._draws[i] = d
+        self._colors = np.asarray(self.stop_colors or [0], np.uint32)
+        self._offsets = np.asarray(self.stop_offsets or [0.0], np.float32)
+        s = rivehip.Stream()
+        s.width, s.height = self.width, self.height
+        s.clear_color, s.dither = self.clear_color, self.dither
+        s.verbs, s.verb_count = self._verbs.ctypes.data, len(self._verbs)
+        s.points, s.point_count = self._points.ctypes.data, len(self._points) // 2
+        s.paths, s.path_count = self._paths.ctypes.data, len(self._paths)
+        s.draws, s.draw_count = self._draws.ctypes.data, len(self._draws)
+        s.stop_colors, s.stop_offsets = self._colors.ctypes.data, self._offsets.ctypes.data
+        s.stop_count = len(self.stop_colors)
+        self._struct = s
+        return ctypes.pointer(s)

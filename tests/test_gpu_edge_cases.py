@@ -1,0 +1,150 @@
+"""Edge cases through the C ABI on the device, against the oracle: empty and
+culled draws, odd target sizes, off-canvas and huge coordinates, degenerate
+and cusped cubics, every join and cap, clockwise fills under a mirroring
+matrix, nested and reused clips, radial and many-stop gradients.
+
+Layout (spans, contours, per-draw vertex ranges) is compared bit for bit;
+pixels within 2/255 on all but 0.1% of pixels and 8/255 everywhere.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_binding as ob
+import rivehip
+from streams import (BEVEL, BUTT, CLOCKWISE, EVENODD, MITER, NONZERO, ROUND, ROUND_CAP, SQUARE,
+                     StreamBuilder, rect)
+
+pytestmark = pytest.mark.gpu
+
+
+def star(cx, cy, r0, r1, n=7):
+    cmds = []
+    for i in range(2 * n):
+        r = r1 if i % 2 == 0 else r0
+        a = math.pi * i / n
+        cmds.append((0 if i == 0 else 1, cx + r * math.cos(a), cy + r * math.sin(a)))
+    return cmds + [(5,)]
+
+
+def sc_empty_frame():
+    return StreamBuilder(40, 24, clear_color=0xff102030)
+
+
+def sc_culled_draws():
+    b = StreamBuilder(37, 23, clear_color=0x80ffffff)
+    b.draw(b.path([]), color=0xffff0000)
+    b.draw(b.path(rect(100, 100, 120, 130)), color=0xffff0000)
+    b.draw(b.path(rect(2, 2, 9, 9)), color=0xffff0000, stroke=0.0)
+    b.draw(b.path(rect(3, 3, 20, 20)), color=0xff00ff00)
+    return b
+
+
+def sc_one_pixel():
+    b = StreamBuilder(1, 1)
+    b.draw(b.path(rect(0.25, 0.25, 0.75, 0.75)), color=0xffffffff)
+    return b
+
+
+def sc_offcanvas_and_huge():
+    b = StreamBuilder(50, 30, dither=1)
+    b.draw(b.path(rect(-40, -40, 20, 15)), color=0xff3366ff)
+    b.draw(b.path([(0, -1e5, 10), (1, 1e5, 12), (1, 0, 25), (5,)]), color=0xc0ff0000)
+    b.draw(b.path(star(45, 25, 8, 30)), color=0xa000ff00)
+    return b
+
+
+def sc_degenerate_cubics():
+    b = StreamBuilder(48, 48)
+    b.draw(b.path([(0, 10, 10), (4, 10, 10, 10, 10, 10, 10), (1, 30, 12), (4, 30, 12, 30, 12, 20, 40),
+                   (5,)]), color=0xffffffff)
+    b.draw(b.path([(0, 5, 40), (4, 45, 2, 2, 2, 42, 40)]), color=0xffff8000, stroke=3.0, join=ROUND,
+           cap=ROUND_CAP)  # a cusp
+    b.draw(b.path([(0, 30, 30), (4, 30, 30, 30, 30, 30, 30)]), color=0xff0000ff, stroke=6.0,
+           cap=ROUND_CAP)  # an empty stroke with caps is a dot
+    b.draw(b.path([(0, 40, 8), (4, 40, 8, 40, 8, 40, 8), (5,)]), color=0xff00ffff, stroke=5.0,
+           join=MITER)
+    return b
+
+
+def sc_clockwise_mirrored():
+    b = StreamBuilder(48, 48)
+    p = b.path(star(24, 24, 8, 20, 9), fill_rule=CLOCKWISE)
+    b.draw(p, color=0xff00ff00)
+    b.draw(p, color=0x8000ffff, matrix=(-1, 0, 0, 1, 48, 0))  # det < 0
+    q = b.path(rect(4, 4, 20, 20, clockwise=False), fill_rule=CLOCKWISE)
+    b.draw(q, color=0xffff0000)
+    return b
+
+
+def sc_clips():
+    b = StreamBuilder(64, 64, dither=1)
+    c1 = b.path(star(32, 32, 12, 30, 6))
+    c2 = b.path(rect(10, 8, 50, 40), fill_rule=EVENODD)
+    c3 = b.path(rect(200, 200, 210, 210))  # off canvas: clips everything
+    body = b.path(rect(0, 0, 64, 64))
+    b.draw(body, color=0xffff0000, clips=(c1,))
+    b.draw(body, color=0x800000ff, clips=(c1, c2))
+    b.draw(body, color=0x8000ff00, clips=(c1, c2))  # the same clip stack again
+    b.draw(body, color=0xffffffff, clips=(c3,))
+    b.draw(b.path(star(20, 44, 4, 12)), color=0xffffff00, stroke=2.0, clips=(c2,))
+    return b
+
+
+def sc_gradients():
+    b = StreamBuilder(96, 64, dither=1)
+    stops8 = [(i / 7, 0xff000000 | (i * 0x243b51 & 0xffffff)) for i in range(8)]
+    b.draw(b.path(rect(0, 0, 96, 30)), gradient=("linear", 5, 0, 90, 20, stops8))
+    b.draw(b.path(rect(0, 30, 96, 64)), gradient=("radial", 48, 47, 30,
+                                                  [(0, 0xffff0000), (.4, 0x8000ff00), (1, 0x000000ff)]))
+    b.draw(b.path(star(48, 32, 10, 25)), gradient=("linear", 0, 0, 0, 64, [(0, 0xff00ff00), (1, 0xffff00ff)]),
+           stroke=3.0, join=ROUND)
+    b.draw(b.path(rect(70, 5, 90, 25)), gradient=("linear", 70, 0, 90, 0, [(.2, 0xffffffff), (.8, 0xff000000)]))
+    return b
+
+
+SCENARIOS = {
+    "empty_frame": sc_empty_frame, "culled_draws": sc_culled_draws, "one_pixel": sc_one_pixel,
+    "offcanvas_and_huge": sc_offcanvas_and_huge, "degenerate_cubics": sc_degenerate_cubics,
+    "clockwise_mirrored": sc_clockwise_mirrored, "clips": sc_clips, "gradients": sc_gradients,
+}
+def build_join_cap(join, cap):
+    b = StreamBuilder(64, 48, dither=1)
+    zig = [(0, 6, 40), (1, 16, 8), (1, 26, 40), (1, 30, 10), (4, 40, 0, 60, 50, 58, 20)]
+    b.draw(b.path(zig), color=0xe0204080, stroke=4.5, join=join, cap=cap)
+    b.draw(b.path(star(32, 24, 6, 14, 5)), color=0x80ff0000, stroke=1.5, join=join, cap=cap)
+    return b
+
+
+for _j in (MITER, ROUND, BEVEL):
+    for _c in (BUTT, ROUND_CAP, SQUARE):
+        SCENARIOS[f"join{_j}_cap{_c}"] = (lambda j=_j, c=_c: build_join_cap(j, c))
+
+
+@pytest.mark.parametrize("name", sorted(SCENARIOS))
+def test_edge_case_matches_oracle(hip_ctx, name):
+    print(f"[edge-case] {name}", flush=True)
+    b = SCENARIOS[name]()
+    s = b.build()
+    w, h = b.width, b.height
+    f = ob.OracleFlush(s)
+    t = hip_ctx.target(w, h)
+    try:
+        ref, _ = f.render()
+        hip_ctx.render_stream(s, t)
+        out = hip_ctx.read_pixels(t, w, h)
+        spans = hip_ctx.readback(0)
+        draws = hip_ctx.readback(3).view(rivehip.DRAW)
+        ospans = f.get(0)
+        odraws = f.draws()
+    finally:
+        hip_ctx.free_target(t)
+        f.close()
+    assert spans.tobytes() == ospans.view(np.uint8).tobytes()
+    assert len(draws) == len(odraws)
+    if len(draws):
+        assert np.array_equal(draws["tess_location"], odraws[:, 1])
+        assert np.array_equal(draws["tess_vertex_count"], odraws[:, 2])
+    d = np.abs(out.astype(np.int32) - ref.astype(np.int32)).max(axis=2)
+    assert d.max() <= 8 and (d > 2).mean() <= 1e-3, f"max {d.max()}, frac>2 {(d > 2).mean()}"
