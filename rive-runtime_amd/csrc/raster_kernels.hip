@@ -860,259 +860,501 @@ __device__ __forceinline__ float quantize8(float v)
     return floorf(v * 255.f + .5f) * (1.f / 255.f);
 }
 
-__global__ __launch_bounds__(256) void raster_kernel(
+// One wave64 per 16x16 tile; each lane owns 4 pixels of one row
+// (row = lane / 4, columns 4 * (lane % 4) .. +3).  Paths are visited in
+// submission order.  For a path with primitives in the tile, the wave splits
+// them into (primitive, row) work items, 16 rows per primitive and 4
+// primitives per pass, that deposit into LDS:
+//   sRowBase[row]        winding change along the tile's left edge down to the
+//                        row's sample line (edges crossing x = X0);
+//   sRowDelta[row][i]    +-1 where an inner edge crosses the row's sample line
+//                        between pixels i-1 and i (prefix-summed per pixel);
+//   sBorder[row][i]      fills: signed AA-border coverage, 1/65536 fixed point;
+//                        strokes: max coverage as float bits.
+// The integer predicates are the ones of the per-pixel formulation: a pixel's
+// row crossing column comes from solving the same threshold inequality, and a
+// triangle's pixels on a row are the candidates of a float span estimate
+// (widened by a pixel) that pass the exact three-edge test.
+constexpr int kRasterLanes = 64;
+
+struct TileGeom
+{
+    int32_t X0, Y0;
+};
+
+__device__ __forceinline__ void tri_row_candidates(int32_t ax, int32_t ay, int32_t bx, int32_t by, int32_t cx,
+                                                   int32_t cy, int32_t yq, int32_t X0, int* ilo, int* ihi)
+{
+    float xl = INFINITY, xr = -INFINITY;
+    const int32_t vx[3] = {ax, bx, cx}, vy[3] = {ay, by, cy};
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+    {
+        const int32_t x0 = vx[e], y0 = vy[e], x1 = vx[(e + 1) % 3], y1 = vy[(e + 1) % 3];
+        if ((y0 <= yq && yq <= y1) || (y1 <= yq && yq <= y0))
+        {
+            if (y0 == y1)
+            {
+                xl = fminf(xl, (float)min(x0, x1));
+                xr = fmaxf(xr, (float)max(x0, x1));
+            }
+            else
+            {
+                const float t = (float)(yq - y0) / (float)(y1 - y0);
+                const float x = (float)x0 + t * (float)(x1 - x0);
+                xl = fminf(xl, x);
+                xr = fmaxf(xr, x);
+            }
+        }
+    }
+    if (!(xl <= xr))
+    {
+        *ilo = 1;
+        *ihi = 0;
+        return;
+    }
+    // pixel i's sample x is X0 + 256 i + 128; one pixel of slack each side
+    const float lo = (xl - (float)X0 - 128.f) * (1.f / 256.f);
+    const float hi = (xr - (float)X0 - 128.f) * (1.f / 256.f);
+    *ilo = (int)fmaxf(floorf(lo) - 1.f, 0.f);
+    *ihi = (int)fminf(ceilf(hi) + 1.f, (float)(kTile - 1));
+}
+
+// Exact "inside" test of an edge against pixel (i2, j2) with orientation o.
+struct EdgeK
+{
+    int32_t A, B, thr; // o-premultiplied: inside iff A*i2 + B*j2 >= thr
+};
+__device__ __forceinline__ EdgeK edge_k(const EdgeT& e, int o)
+{
+    EdgeK k;
+    k.A = o * e.A;
+    k.B = o * e.B;
+    k.thr = inside_threshold(e, o);
+    return k;
+}
+__device__ __forceinline__ bool edge_in(const EdgeK& k, int32_t i2, int32_t j2)
+{
+    return k.A * i2 + k.B * j2 >= k.thr;
+}
+
+__device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* __restrict__ sRowDelta,
+                              int32_t* __restrict__ sRowBase, int32_t* __restrict__ sBorder)
+{
+    const int32_t X0 = g.X0, Y0 = g.Y0;
+    const int32_t j2 = 2 * j + 1;
+    const int32_t yq = Y0 + (j << kSubpixelLog2) + (1 << (kSubpixelLog2 - 1));
+    const int neg = (p.flags & kPrimNegate) ? -1 : 1;
+    // Inner edge I0 -> I1: winding.
+    const EdgeT e = make_edge(p.p[0], p.p[1], p.p[4], p.p[5], X0, Y0);
+    if (p.p[1] != p.p[5])
+    {
+        const int s = p.p[5] < p.p[1] ? 1 : -1;
+        // R(q) <=> s*D(q) >= 0 <=> s*k >= ceil(-s*D0/128)
+        const int32_t T = clamp_thr(ceil_div128(s > 0 ? -e.D0 : e.D0));
+        const int32_t krow = e.B * j2;
+        const bool Rrow = s * krow >= T;
+        const int32_t xmin = min(p.p[0], p.p[4]), xmax = max(p.p[0], p.p[4]);
+        if (xmin <= X0 && X0 < xmax)
+        {
+            const bool Rcorner = (s > 0 ? e.D0 : -e.D0) >= 0;
+            const int d = (int)Rrow - (int)Rcorner;
+            if (d != 0)
+                atomicAdd(&sRowBase[j], neg * s * d);
+        }
+        const int32_t ymin = min(p.p[1], p.p[5]), ymax = max(p.p[1], p.p[5]);
+        if (!Rrow && ymin <= yq && yq < ymax)
+        {
+            // Smallest i with s*(A*(2i+1) + krow) >= T, where s*A = |dy| > 0:
+            // 2|dy| i >= T - |dy| - s*krow.  The quotient is exact in double
+            // wherever it decides a column inside the tile.
+            const int64_t sa = (int64_t)(s * e.A);
+            const int64_t r = (int64_t)T - sa - (int64_t)s * krow;
+            const double q = ceil((double)r / (double)(2 * sa));
+            if (q < (double)kTile)
+            {
+                const int istar = q < 0.0 ? 0 : (int)q;
+                atomicAdd(&sRowDelta[j * kTile + istar], neg * s);
+            }
+        }
+    }
+    else if (p.p[0] != p.p[4])
+    {
+        // A horizontal edge never meets a sample line, but where it crosses
+        // the tile's left edge the winding along that edge steps by sign(dx).
+        const int32_t xmin = min(p.p[0], p.p[4]), xmax = max(p.p[0], p.p[4]);
+        if (xmin <= X0 && X0 < xmax)
+        {
+            const int32_t h = p.p[1];
+            const int sg = p.p[4] > p.p[0] ? 1 : -1;
+            const int d = (int)(yq >= h) - (int)(Y0 >= h);
+            if (d != 0)
+                atomicAdd(&sRowBase[j], neg * sg * d);
+        }
+    }
+    // Border triangle T1 = (I0, O0, I1), coverage 1 - lambda(O0).
+    const int64_t A1 = orient64(p.p[0], p.p[1], p.p[2], p.p[3], p.p[4], p.p[5]);
+    if (A1 != 0)
+    {
+        int ilo, ihi;
+        tri_row_candidates(p.p[0], p.p[1], p.p[2], p.p[3], p.p[4], p.p[5], yq, X0, &ilo, &ihi);
+        if (ilo <= ihi)
+        {
+            const int o = A1 > 0 ? 1 : -1;
+            const EdgeK k01 = edge_k(make_edge(p.p[0], p.p[1], p.p[2], p.p[3], X0, Y0), o);
+            const EdgeK k12 = edge_k(make_edge(p.p[2], p.p[3], p.p[4], p.p[5], X0, Y0), o);
+            const EdgeK k20 = edge_k(make_edge(p.p[4], p.p[5], p.p[0], p.p[1], X0, Y0), o);
+            for (int i = ilo; i <= ihi; ++i)
+            {
+                const int32_t i2 = 2 * i + 1;
+                if (edge_in(k01, i2, j2) && edge_in(k12, i2, j2) && edge_in(k20, i2, j2))
+                {
+                    const int32_t ke = e.A * i2 + e.B * j2;
+                    const float De = (float)e.D0 + 128.f * (float)ke;
+                    const float cv = 1.f + De / (float)A1;
+                    atomicAdd(&sBorder[j * kTile + i], neg * o * __float2int_rn(cv * 65536.f));
+                }
+            }
+        }
+    }
+    // Border triangle T2 = (I1, O0, O1), coverage lambda(I1).
+    const int64_t A2 = orient64(p.p[4], p.p[5], p.p[2], p.p[3], p.p[6], p.p[7]);
+    if (A2 != 0)
+    {
+        int ilo, ihi;
+        tri_row_candidates(p.p[4], p.p[5], p.p[2], p.p[3], p.p[6], p.p[7], yq, X0, &ilo, &ihi);
+        if (ilo <= ihi)
+        {
+            const int o = A2 > 0 ? 1 : -1;
+            const EdgeK f01 = edge_k(make_edge(p.p[4], p.p[5], p.p[2], p.p[3], X0, Y0), o);
+            const EdgeT f12e = make_edge(p.p[2], p.p[3], p.p[6], p.p[7], X0, Y0);
+            const EdgeK f12 = edge_k(f12e, o);
+            const EdgeK f20 = edge_k(make_edge(p.p[6], p.p[7], p.p[4], p.p[5], X0, Y0), o);
+            for (int i = ilo; i <= ihi; ++i)
+            {
+                const int32_t i2 = 2 * i + 1;
+                if (edge_in(f01, i2, j2) && edge_in(f12, i2, j2) && edge_in(f20, i2, j2))
+                {
+                    const int32_t k12 = f12e.A * i2 + f12e.B * j2;
+                    const float D12 = (float)f12e.D0 + 128.f * (float)k12;
+                    const float cv = D12 / (float)A2;
+                    atomicAdd(&sBorder[j * kTile + i], neg * o * __float2int_rn(cv * 65536.f));
+                }
+            }
+        }
+    }
+}
+
+__device__ void stroke_prim_row(const Prim& p, int j, const TileGeom& g, uint32_t* __restrict__ sMax)
+{
+    const int32_t X0 = g.X0, Y0 = g.Y0;
+    const int32_t j2 = 2 * j + 1;
+    const int32_t yq = Y0 + (j << kSubpixelLog2) + (1 << (kSubpixelLog2 - 1));
+    // T1 = (P0-, P0+, P1-): vertices 0, 1, 2; T2 = (P1-, P0+, P1+): 2, 1, 3.
+    // Only clockwise triangles draw.
+    const int64_t A1 = orient64(p.p[0], p.p[1], p.p[2], p.p[3], p.p[4], p.p[5]);
+    const int64_t A2 = orient64(p.p[4], p.p[5], p.p[2], p.p[3], p.p[6], p.p[7]);
+    const EdgeT e12 = make_edge(p.p[2], p.p[3], p.p[4], p.p[5], X0, Y0);
+    if (A1 > 0)
+    {
+        int ilo, ihi;
+        tri_row_candidates(p.p[0], p.p[1], p.p[2], p.p[3], p.p[4], p.p[5], yq, X0, &ilo, &ihi);
+        if (ilo <= ihi)
+        {
+            const EdgeT e01 = make_edge(p.p[0], p.p[1], p.p[2], p.p[3], X0, Y0);
+            const EdgeT e20 = make_edge(p.p[4], p.p[5], p.p[0], p.p[1], X0, Y0);
+            const int32_t t01 = inside_threshold(e01, 1), t12 = inside_threshold(e12, 1),
+                          t20 = inside_threshold(e20, 1);
+            const float inv = 1.f / (float)A1;
+            for (int i = ilo; i <= ihi; ++i)
+            {
+                const int32_t i2 = 2 * i + 1;
+                const int32_t k01 = e01.A * i2 + e01.B * j2;
+                const int32_t k12 = e12.A * i2 + e12.B * j2;
+                const int32_t k20 = e20.A * i2 + e20.B * j2;
+                if (k01 >= t01 && k12 >= t12 && k20 >= t20)
+                {
+                    const float l0 = ((float)e12.D0 + 128.f * (float)k12) * inv;
+                    const float l1 = ((float)e20.D0 + 128.f * (float)k20) * inv;
+                    const float l2 = ((float)e01.D0 + 128.f * (float)k01) * inv;
+                    const float cx = l0 * p.cov[0] + l1 * p.cov[2] + l2 * p.cov[4];
+                    const float cy = l0 * p.cov[1] + l1 * p.cov[3] + l2 * p.cov[5];
+                    const float v = fminf(cx, cy);
+                    if (v > 0.f)
+                        atomicMax(&sMax[j * kTile + i], __float_as_uint(v));
+                }
+            }
+        }
+    }
+    if (A2 > 0)
+    {
+        int ilo, ihi;
+        tri_row_candidates(p.p[4], p.p[5], p.p[2], p.p[3], p.p[6], p.p[7], yq, X0, &ilo, &ihi);
+        if (ilo <= ihi)
+        {
+            // T2 edges: P1- -> P0+ is the reverse of e12.
+            const EdgeT e13 = make_edge(p.p[2], p.p[3], p.p[6], p.p[7], X0, Y0);
+            const EdgeT e32 = make_edge(p.p[6], p.p[7], p.p[4], p.p[5], X0, Y0);
+            const int32_t t21 = inside_threshold(EdgeT{-e12.A, -e12.B, -e12.D0}, 1);
+            const int32_t t13 = inside_threshold(e13, 1), t32 = inside_threshold(e32, 1);
+            const float inv = 1.f / (float)A2;
+            for (int i = ilo; i <= ihi; ++i)
+            {
+                const int32_t i2 = 2 * i + 1;
+                const int32_t k12 = e12.A * i2 + e12.B * j2;
+                const int32_t k13 = e13.A * i2 + e13.B * j2;
+                const int32_t k32 = e32.A * i2 + e32.B * j2;
+                if (-k12 >= t21 && k13 >= t13 && k32 >= t32)
+                {
+                    const float l0 = ((float)e13.D0 + 128.f * (float)k13) * inv;  // P1-
+                    const float l1 = ((float)e32.D0 + 128.f * (float)k32) * inv;  // P0+
+                    const float l2 = -((float)e12.D0 + 128.f * (float)k12) * inv; // P1+
+                    const float cx = l0 * p.cov[4] + l1 * p.cov[2] + l2 * p.cov[6];
+                    const float cy = l0 * p.cov[5] + l1 * p.cov[3] + l2 * p.cov[7];
+                    const float v = fminf(cx, cy);
+                    if (v > 0.f)
+                        atomicMax(&sMax[j * kTile + i], __float_as_uint(v));
+                }
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ float fill_rule_coverage(float count, uint32_t flags)
+{
+    if (flags & 4u) // clockwise
+        return fminf(fmaxf(count, 0.f), 1.f);
+    float coverage = fabsf(count);
+    if (flags & 2u) // even-odd
+    {
+        const float h = coverage * .5f;
+        coverage = 1.f - fabsf((h - floorf(h)) * 2.f + -1.f);
+    }
+    return fminf(coverage, 1.f);
+}
+
+// Clip, paint and src-over of one path into one pixel (draw_path.vert:431-545,
+// draw_raster_order_path.frag:84-238).  A zero coverage leaves the pixel
+// unchanged, so callers skip it (except for clip updates).
+__device__ __forceinline__ void resolve_pixel(const PathRaster& pr, float coverage, int px, int py, float noise,
+                                              const uint32_t* __restrict__ ramps, uint32_t gradHeight,
+                                              float* col, float& clipCov, uint32_t& clipID)
+{
+    if (pr.flags & 8u) // clip update
+    {
+        if (pr.outerClipID != 0u)
+            coverage = fminf(coverage, clipID == pr.outerClipID ? clipCov : 0.f);
+        clipCov = coverage;
+        clipID = pr.clipID;
+        return;
+    }
+    if (pr.clipID != 0u)
+        coverage = clipID == pr.clipID ? fminf(clipCov, coverage) : 0.f;
+    float src[4];
+    if (pr.paintType == SOLID_COLOR_PAINT_TYPE)
+    {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            src[k] = pr.color[k] * coverage;
+    }
+    else
+    {
+        const float fx = float(px) + .5f, fy = float(py) + .5f;
+        const float pcx = pr.paintMatrix[0] * fx + pr.paintMatrix[2] * fy + pr.paintMatrix[4];
+        const float pcy = pr.paintMatrix[1] * fx + pr.paintMatrix[3] * fy + pr.paintMatrix[5];
+        float t = pr.paintType == LINEAR_GRADIENT_PAINT_TYPE ? pcx : sqrtf(pcx * pcx + pcy * pcy);
+        t = fminf(fmaxf(t, 0.f), 1.f);
+        const float u = pr.gradSpan > 1.f ? (1.f - 1.f / 512.f) * t + (.5f / 512.f)
+                                          : (1.f / 512.f) * t + pr.gradSpan;
+        const float tc = u * 512.f - .5f;
+        const int i0 = (int)floorf(tc);
+        const float fr = tc - (float)i0;
+        const int ia = min(max(i0, 0), 511), ib = min(max(i0 + 1, 0), 511);
+        const uint32_t* rowp = ramps + (size_t)min(pr.gradRow, gradHeight - 1) * kGradWidth;
+        const uint32_t ta = rowp[ia], tb = rowp[ib];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+        {
+            const float va = float((ta >> (8 * k)) & 0xffu) * (1.f / 255.f);
+            const float vb = float((tb >> (8 * k)) & 0xffu) * (1.f / 255.f);
+            src[k] = (vb - va) * fr + va;
+        }
+        src[3] *= coverage;
+        src[0] *= src[3];
+        src[1] *= src[3];
+        src[2] *= src[3];
+    }
+    const float paintAlpha = src[3];
+    float out[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        out[k] = src[k] + col[k] * (1.f - paintAlpha);
+    if (paintAlpha != 0.f)
+    {
+        out[0] += noise;
+        out[1] += noise;
+        out[2] += noise;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        col[k] = quantize8(out[k]);
+}
+
+__global__ __launch_bounds__(kRasterLanes) void raster_kernel(
     const PathCells* __restrict__ cells, const PathRaster* __restrict__ raster, uint32_t pathCount,
     const uint32_t* __restrict__ cellOffset, const int32_t* __restrict__ cellBackdrop,
     const uint32_t* __restrict__ binned, const Prim* __restrict__ prims,
     const uint32_t* __restrict__ ramps, uint32_t gradHeight, int width, int height, int tilesX,
     uint32_t loadPreserve, uint32_t clearPremul, uint32_t dither, uint32_t* __restrict__ target)
 {
+    __shared__ __align__(16) int32_t sRowDelta[kTile * kTile];
+    __shared__ __align__(16) int32_t sBorder[kTile * kTile];
+    __shared__ int32_t sRowBase[kTile];
+
     const int tile = blockIdx.x;
     const int tx = tile % tilesX, ty = tile / tilesX;
-    const int li = threadIdx.x & (kTile - 1), lj = threadIdx.x >> kTileLog2;
-    const int px = tx * kTile + li, py = ty * kTile + lj;
-    const bool inside = px < width && py < height;
-    const int32_t X0 = tx << (kSubpixelLog2 + kTileLog2);
-    const int32_t Y0 = ty << (kSubpixelLog2 + kTileLog2);
-    const int32_t i2 = 2 * li + 1, j2 = 2 * lj + 1;
-    const int32_t yq = Y0 + (lj << kSubpixelLog2) + 128; // sample y, fixed point
+    const int lane = threadIdx.x;
+    const int pj = lane >> 2, pi0 = (lane & 3) * 4;
+    const int px0 = tx * kTile + pi0, py = ty * kTile + pj;
+    TileGeom g;
+    g.X0 = tx << (kSubpixelLog2 + kTileLog2);
+    g.Y0 = ty << (kSubpixelLog2 + kTileLog2);
+    // work items: primitive k0 + lane / 16, row lane % 16
+    const int itemPrim = lane >> 4, itemRow = lane & (kTile - 1);
 
-    float col[4];
+    float col[4][4];
+    float clipCov[4] = {0.f, 0.f, 0.f, 0.f};
+    uint32_t clipID[4] = {0u, 0u, 0u, 0u};
+    float noise[4];
+    const bool rowInside = py < height;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
     {
         uint32_t c = clearPremul;
-        if (loadPreserve && inside)
-            c = target[(size_t)py * width + px];
+        if (loadPreserve && rowInside && px0 + q < width)
+            c = target[(size_t)py * width + px0 + q];
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            col[k] = float((c >> (8 * k)) & 0xffu) * (1.f / 255.f);
+            col[q][k] = float((c >> (8 * k)) & 0xffu) * (1.f / 255.f);
+        noise[q] = dither ? dither_noise(px0 + q, py) : 0.f;
     }
-    float clipCov = 0.f;
-    uint32_t clipID = 0;
-    const float noise = dither ? dither_noise(px, py) : 0.f;
 
-    for (uint32_t path = 1; path <= pathCount; ++path)
+    for (uint32_t base = 1; base <= pathCount; base += kRasterLanes)
     {
-        const PathCells c = cells[path];
-        if (tx < c.tx0 || tx >= c.tx1 || ty < c.ty0 || ty >= c.ty1)
-            continue;
-        const uint32_t cell = c.base + (uint32_t)(ty - c.ty0) * c.width + (uint32_t)(tx - c.tx0);
-        const uint32_t off = cellOffset[cell];
-        const uint32_t n = cellOffset[cell + 1] - off;
-        const int32_t C = cellBackdrop[cell];
-        if (n == 0 && C == 0)
-            continue;
-        const PathRaster pr = raster[path];
-        const bool isStroke = pr.flags & 1u;
-        float coverage;
-        if (!isStroke)
+        const uint32_t p = base + (uint32_t)lane;
+        uint32_t off = 0, n = 0;
+        int32_t C = 0;
+        bool relevant = false;
+        if (p <= pathCount)
         {
-            int32_t wind = C;
-            int32_t borderFx = 0;
-            for (uint32_t k = 0; k < n; ++k)
+            const PathCells c = cells[p];
+            if (tx >= c.tx0 && tx < c.tx1 && ty >= c.ty0 && ty < c.ty1)
             {
-                const Prim p = prims[binned[off + k]];
-                const int neg = (p.flags & kPrimNegate) ? -1 : 1;
-                // Inner edge I0 -> I1: winding.
-                const EdgeT e = make_edge(p.p[0], p.p[1], p.p[4], p.p[5], X0, Y0);
-                const int32_t ke = e.A * i2 + e.B * j2;
-                if (p.p[1] != p.p[5])
-                {
-                    const int s = p.p[5] < p.p[1] ? 1 : -1;
-                    // R(q) <=> s*D(q) >= 0 <=> s*k >= ceil(-s*D0/128)
-                    const int32_t T = clamp_thr(ceil_div128(s > 0 ? -e.D0 : e.D0));
-                    const int32_t krow = e.B * j2;
-                    const bool Rq = s * ke >= T;
-                    const bool Rrow = s * krow >= T;
-                    const bool Rcorner = (s > 0 ? e.D0 : -e.D0) >= 0;
-                    const int32_t ymin = min(p.p[1], p.p[5]), ymax = max(p.p[1], p.p[5]);
-                    const int32_t xmin = min(p.p[0], p.p[4]), xmax = max(p.p[0], p.p[4]);
-                    const bool cross = xmin <= X0 && X0 < xmax;
-                    const bool inRange = ymin <= yq && yq < ymax;
-                    int32_t d = 0;
-                    if (cross)
-                        d += (int32_t)Rrow - (int32_t)Rcorner;
-                    if (inRange)
-                        d += (int32_t)Rq - (int32_t)Rrow;
-                    wind += neg * s * d;
-                }
-                else if (p.p[0] != p.p[4])
-                {
-                    // A horizontal edge never meets a row, but where it crosses
-                    // the tile's left edge the winding along that edge steps
-                    // by sign(dx) (its neighbours' range ends no longer cancel).
-                    const int32_t xmin = min(p.p[0], p.p[4]), xmax = max(p.p[0], p.p[4]);
-                    if (xmin <= X0 && X0 < xmax)
-                    {
-                        const int32_t h = p.p[1];
-                        const int sg = p.p[4] > p.p[0] ? 1 : -1;
-                        wind += neg * sg * ((int32_t)(yq >= h) - (int32_t)(Y0 >= h));
-                    }
-                }
-                // Border triangle T1 = (I0, O0, I1), coverage 1 - lambda(O0).
-                const int64_t A1 = orient64(p.p[0], p.p[1], p.p[2], p.p[3], p.p[4], p.p[5]);
-                if (A1 != 0)
-                {
-                    const int o = A1 > 0 ? 1 : -1;
-                    const EdgeT e01 = make_edge(p.p[0], p.p[1], p.p[2], p.p[3], X0, Y0);
-                    const EdgeT e12 = make_edge(p.p[2], p.p[3], p.p[4], p.p[5], X0, Y0);
-                    const EdgeT e20 = make_edge(p.p[4], p.p[5], p.p[0], p.p[1], X0, Y0);
-                    const bool in = o * (e01.A * i2 + e01.B * j2) >= inside_threshold(e01, o) &&
-                                    o * (e12.A * i2 + e12.B * j2) >= inside_threshold(e12, o) &&
-                                    o * (e20.A * i2 + e20.B * j2) >= inside_threshold(e20, o);
-                    if (in)
-                    {
-                        // D of edge I1->I0 at q = -(D of I0->I1)
-                        float De = (float)e.D0 + 128.f * (float)ke;
-                        float cv = 1.f + De / (float)A1;
-                        borderFx += neg * o * __float2int_rn(cv * 65536.f);
-                    }
-                }
-                // Border triangle T2 = (I1, O0, O1), coverage lambda(I1).
-                const int64_t A2 = orient64(p.p[4], p.p[5], p.p[2], p.p[3], p.p[6], p.p[7]);
-                if (A2 != 0)
-                {
-                    const int o = A2 > 0 ? 1 : -1;
-                    const EdgeT f01 = make_edge(p.p[4], p.p[5], p.p[2], p.p[3], X0, Y0);
-                    const EdgeT f12 = make_edge(p.p[2], p.p[3], p.p[6], p.p[7], X0, Y0);
-                    const EdgeT f20 = make_edge(p.p[6], p.p[7], p.p[4], p.p[5], X0, Y0);
-                    const int32_t k12 = f12.A * i2 + f12.B * j2;
-                    const bool in = o * (f01.A * i2 + f01.B * j2) >= inside_threshold(f01, o) &&
-                                    o * k12 >= inside_threshold(f12, o) &&
-                                    o * (f20.A * i2 + f20.B * j2) >= inside_threshold(f20, o);
-                    if (in)
-                    {
-                        float D12 = (float)f12.D0 + 128.f * (float)k12;
-                        float cv = D12 / (float)A2;
-                        borderFx += neg * o * __float2int_rn(cv * 65536.f);
-                    }
-                }
+                const uint32_t cell = c.base + (uint32_t)(ty - c.ty0) * c.width + (uint32_t)(tx - c.tx0);
+                off = cellOffset[cell];
+                n = cellOffset[cell + 1] - off;
+                C = cellBackdrop[cell];
+                relevant = n != 0 || C != 0;
             }
-            float count = (float)wind + (float)borderFx * (1.f / 65536.f);
-            if (pr.flags & 4u) // clockwise
-                coverage = fminf(fmaxf(count, 0.f), 1.f);
+        }
+        uint64_t mask = __ballot(relevant);
+        while (mask != 0)
+        {
+            const int b = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            const uint32_t path = base + (uint32_t)b;
+            const uint32_t pOff = __shfl(off, b);
+            const uint32_t pN = __shfl(n, b);
+            const int32_t pC = __shfl(C, b);
+            const PathRaster pr = raster[path];
+            const bool isStroke = pr.flags & 1u;
+            float cov[4];
+            if (pN == 0)
+            {
+                // No edge in the tile: the coverage is the corner winding's.
+                const float u = isStroke ? 0.f : fill_rule_coverage((float)pC, pr.flags);
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    cov[q] = u;
+            }
             else
             {
-                coverage = fabsf(count);
-                if (pr.flags & 2u) // even-odd
+                for (int q = lane; q < kTile * kTile; q += kRasterLanes)
                 {
-                    float h = coverage * .5f;
-                    coverage = 1.f - fabsf((h - floorf(h)) * 2.f + -1.f);
+                    sRowDelta[q] = 0;
+                    sBorder[q] = 0;
                 }
-                coverage = fminf(coverage, 1.f);
-            }
-        }
-        else
-        {
-            float m = 0.f;
-            for (uint32_t k = 0; k < n; ++k)
-            {
-                const Prim p = prims[binned[off + k]];
-                // T1 = (P0-, P0+, P1-): vertices 0, 1, 2; T2 = (P1-, P0+, P1+): 2, 1, 3.
-                const int64_t A1 = orient64(p.p[0], p.p[1], p.p[2], p.p[3], p.p[4], p.p[5]);
-                const int64_t A2 = orient64(p.p[4], p.p[5], p.p[2], p.p[3], p.p[6], p.p[7]);
-                const EdgeT e01 = make_edge(p.p[0], p.p[1], p.p[2], p.p[3], X0, Y0);
-                const EdgeT e12 = make_edge(p.p[2], p.p[3], p.p[4], p.p[5], X0, Y0);
-                const EdgeT e20 = make_edge(p.p[4], p.p[5], p.p[0], p.p[1], X0, Y0);
-                const EdgeT e13 = make_edge(p.p[2], p.p[3], p.p[6], p.p[7], X0, Y0);
-                const EdgeT e32 = make_edge(p.p[6], p.p[7], p.p[4], p.p[5], X0, Y0);
-                const int32_t k01 = e01.A * i2 + e01.B * j2;
-                const int32_t k12 = e12.A * i2 + e12.B * j2;
-                const int32_t k20 = e20.A * i2 + e20.B * j2;
-                const int32_t k13 = e13.A * i2 + e13.B * j2;
-                const int32_t k32 = e32.A * i2 + e32.B * j2;
-                if (A1 > 0 && k01 >= inside_threshold(e01, 1) && k12 >= inside_threshold(e12, 1) &&
-                    k20 >= inside_threshold(e20, 1))
+                if (lane < kTile)
+                    sRowBase[lane] = 0;
+                __syncthreads();
+                for (uint32_t k0 = 0; k0 < pN; k0 += 4)
                 {
-                    float inv = 1.f / (float)A1;
-                    float l0 = ((float)e12.D0 + 128.f * (float)k12) * inv;
-                    float l1 = ((float)e20.D0 + 128.f * (float)k20) * inv;
-                    float l2 = ((float)e01.D0 + 128.f * (float)k01) * inv;
-                    float cx = l0 * p.cov[0] + l1 * p.cov[2] + l2 * p.cov[4];
-                    float cy = l0 * p.cov[1] + l1 * p.cov[3] + l2 * p.cov[5];
-                    m = fmaxf(fminf(cx, cy), m);
+                    const uint32_t k = k0 + (uint32_t)itemPrim;
+                    if (k < pN)
+                    {
+                        const Prim pm = prims[binned[pOff + k]];
+                        if (isStroke)
+                            stroke_prim_row(pm, itemRow, g, reinterpret_cast<uint32_t*>(sBorder));
+                        else
+                            fill_prim_row(pm, itemRow, g, sRowDelta, sRowBase, sBorder);
+                    }
                 }
-                // T2 edges: P1- -> P0+ is the reverse of e12.
-                if (A2 > 0 && -k12 >= inside_threshold(EdgeT{-e12.A, -e12.B, -e12.D0}, 1) &&
-                    k13 >= inside_threshold(e13, 1) && k32 >= inside_threshold(e32, 1))
+                __syncthreads();
+                const int4 bv = reinterpret_cast<const int4*>(sBorder)[lane];
+                if (isStroke)
                 {
-                    float inv = 1.f / (float)A2;
-                    float l0 = ((float)e13.D0 + 128.f * (float)k13) * inv;      // P1-
-                    float l1 = ((float)e32.D0 + 128.f * (float)k32) * inv;      // P0+
-                    float l2 = -((float)e12.D0 + 128.f * (float)k12) * inv;     // P1+
-                    float cx = l0 * p.cov[4] + l1 * p.cov[2] + l2 * p.cov[6];
-                    float cy = l0 * p.cov[5] + l1 * p.cov[3] + l2 * p.cov[7];
-                    m = fmaxf(fminf(cx, cy), m);
+                    cov[0] = fminf(__int_as_float(bv.x), 1.f);
+                    cov[1] = fminf(__int_as_float(bv.y), 1.f);
+                    cov[2] = fminf(__int_as_float(bv.z), 1.f);
+                    cov[3] = fminf(__int_as_float(bv.w), 1.f);
                 }
+                else
+                {
+                    const int4* rd = reinterpret_cast<const int4*>(sRowDelta + pj * kTile);
+                    int32_t acc = pC + sRowBase[pj];
+                    for (int gq = 0; gq < (lane & 3); ++gq)
+                    {
+                        const int4 v = rd[gq];
+                        acc += v.x + v.y + v.z + v.w;
+                    }
+                    const int4 v = rd[lane & 3];
+                    const int32_t w0 = acc + v.x, w1 = w0 + v.y, w2 = w1 + v.z, w3 = w2 + v.w;
+                    cov[0] = fill_rule_coverage((float)w0 + (float)bv.x * (1.f / 65536.f), pr.flags);
+                    cov[1] = fill_rule_coverage((float)w1 + (float)bv.y * (1.f / 65536.f), pr.flags);
+                    cov[2] = fill_rule_coverage((float)w2 + (float)bv.z * (1.f / 65536.f), pr.flags);
+                    cov[3] = fill_rule_coverage((float)w3 + (float)bv.w * (1.f / 65536.f), pr.flags);
+                }
+                __syncthreads();
             }
-            coverage = fminf(m, 1.f);
-        }
-
-        if (pr.flags & 8u) // clip update
-        {
-            if (pr.outerClipID != 0u)
-                coverage = fminf(coverage, clipID == pr.outerClipID ? clipCov : 0.f);
-            clipCov = coverage;
-            clipID = pr.clipID;
-            continue;
-        }
-        if (pr.clipID != 0u)
-            coverage = clipID == pr.clipID ? fminf(clipCov, coverage) : 0.f;
-        float src[4];
-        if (pr.paintType == SOLID_COLOR_PAINT_TYPE)
-        {
+            const bool clipUpdate = pr.flags & 8u;
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                src[k] = pr.color[k] * coverage;
+            for (int q = 0; q < 4; ++q)
+                if (cov[q] != 0.f || clipUpdate)
+                    resolve_pixel(pr, cov[q], px0 + q, py, noise[q], ramps, gradHeight, col[q], clipCov[q],
+                                  clipID[q]);
         }
-        else
-        {
-            float fx = float(px) + .5f, fy = float(py) + .5f;
-            float pcx = pr.paintMatrix[0] * fx + pr.paintMatrix[2] * fy + pr.paintMatrix[4];
-            float pcy = pr.paintMatrix[1] * fx + pr.paintMatrix[3] * fy + pr.paintMatrix[5];
-            float t = pr.paintType == LINEAR_GRADIENT_PAINT_TYPE ? pcx : sqrtf(pcx * pcx + pcy * pcy);
-            t = fminf(fmaxf(t, 0.f), 1.f);
-            float u = pr.gradSpan > 1.f ? (1.f - 1.f / 512.f) * t + (.5f / 512.f)
-                                        : (1.f / 512.f) * t + pr.gradSpan;
-            float tc = u * 512.f - .5f;
-            int i0 = (int)floorf(tc);
-            float fr = tc - (float)i0;
-            int ia = min(max(i0, 0), 511), ib = min(max(i0 + 1, 0), 511);
-            const uint32_t* rowp = ramps + (size_t)min(pr.gradRow, gradHeight - 1) * kGradWidth;
-            uint32_t ta = rowp[ia], tb = rowp[ib];
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-            {
-                float va = float((ta >> (8 * k)) & 0xffu) * (1.f / 255.f);
-                float vb = float((tb >> (8 * k)) & 0xffu) * (1.f / 255.f);
-                src[k] = (vb - va) * fr + va;
-            }
-            src[3] *= coverage;
-            src[0] *= src[3];
-            src[1] *= src[3];
-            src[2] *= src[3];
-        }
-        const float paintAlpha = src[3];
-        float out[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            out[k] = src[k] + col[k] * (1.f - paintAlpha);
-        if (paintAlpha != 0.f)
-        {
-            out[0] += noise;
-            out[1] += noise;
-            out[2] += noise;
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            col[k] = quantize8(out[k]);
     }
-    if (inside)
+    if (rowInside)
     {
-        uint32_t packed = 0;
+        uint32_t packed[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            packed |= (uint32_t)__float2int_rn(col[k] * 255.f) << (8 * k);
-        target[(size_t)py * width + px] = packed;
+        for (int q = 0; q < 4; ++q)
+        {
+            packed[q] = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                packed[q] |= (uint32_t)__float2int_rn(col[q][k] * 255.f) << (8 * k);
+        }
+        uint32_t* row = target + (size_t)py * width;
+        if (px0 + 3 < width && ((((size_t)py * width + px0) & 3) == 0))
+            *reinterpret_cast<uint4*>(row + px0) = make_uint4(packed[0], packed[1], packed[2], packed[3]);
+        else
+        {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (px0 + q < width)
+                    row[px0 + q] = packed[q];
+        }
     }
 }
 
