@@ -83,7 +83,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-sample-draws", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device-to-device once (the reference kernel for "
+                         "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE readings) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        src = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 1 << 30}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this backend renders a frame on one GPU; --gpus {args.gpus} is not "
               "supported (multi-GPU sharding is a design note only)", file=sys.stderr)
