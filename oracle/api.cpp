@@ -169,4 +169,9 @@ int oracle_convex_180_chops(const float* pts8, float* T2, int* areCusps)
     *areCusps = c ? 1 : 0;
     return n;
 }
+void oracle_blend_coeffs(const float* src3, const float* dstPremul4, uint32_t mode, float* out3)
+{
+    advanced_blend_coeffs(src3, dstPremul4, mode, out3);
+}
+uint32_t oracle_shader_blend_mode(uint32_t riveBlendMode) { return shader_blend_mode(riveBlendMode); }
 }

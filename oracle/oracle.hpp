@@ -74,6 +74,32 @@ constexpr uint32_t JOIN_TANGENT_INNER_CONTOUR_FLAG = 1u << 21;
 constexpr uint32_t LEFT_JOIN_CONTOUR_FLAG = 1u << 20;
 constexpr uint32_t RIGHT_JOIN_CONTOUR_FLAG = 1u << 19;
 
+// Shader blend modes (constants.glsl:237-252), PaintData params bits 4..7.
+enum : uint32_t
+{
+    BLEND_SRC_OVER = 0,
+    BLEND_MODE_SCREEN = 1,
+    BLEND_MODE_OVERLAY = 2,
+    BLEND_MODE_DARKEN = 3,
+    BLEND_MODE_LIGHTEN = 4,
+    BLEND_MODE_COLORDODGE = 5,
+    BLEND_MODE_COLORBURN = 6,
+    BLEND_MODE_HARDLIGHT = 7,
+    BLEND_MODE_SOFTLIGHT = 8,
+    BLEND_MODE_DIFFERENCE = 9,
+    BLEND_MODE_EXCLUSION = 10,
+    BLEND_MODE_MULTIPLY = 11,
+    BLEND_MODE_HUE = 12,
+    BLEND_MODE_SATURATION = 13,
+    BLEND_MODE_COLOR = 14,
+    BLEND_MODE_LUMINOSITY = 15,
+};
+// rive::BlendMode (include/rive/shapes/paint/blend_mode.hpp:5-23) to the
+// shader mode (ConvertBlendModeToPLSBlendMode, gpu.cpp:717-753).
+uint32_t shader_blend_mode(uint32_t riveBlendMode);
+// advanced_blend_coeffs (advanced_blend.glsl:108-280), evaluated in fp32.
+void advanced_blend_coeffs(const float src[3], const float dstPremul[4], uint32_t mode, float out[3]);
+
 enum class Join : uint32_t { miter = 0, round = 1, bevel = 2 };
 enum class Cap : uint32_t { butt = 0, round = 1, square = 2 };
 enum class FillRule : uint32_t { nonZero = 0, evenOdd = 1, clockwise = 2 };

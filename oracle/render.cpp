@@ -263,7 +263,8 @@ struct Builder
     // Pushes one PathDraw: records + tessellation (LogicalFlush::pushDraws).
     void pushPathDraw(uint32_t pathIdx, const Mat& matrix, float strokeRadius, Join join,
                       Cap cap, FillRule fillRule, uint32_t paintType, uint32_t paintValue,
-                      uint32_t clipID, uint32_t outerClipID, const Gradient* grad)
+                      uint32_t clipID, uint32_t outerClipID, const Gradient* grad,
+                      uint32_t blendMode = BLEND_SRC_OVER)
     {
         const rive_stream_path& sp = s.paths[pathIdx];
         PathDrawDesc d;
@@ -300,7 +301,7 @@ struct Builder
         }
         else
         {
-            params |= clipID << 16; // srcOver blend mode = 0
+            params |= clipID << 16 | blendMode << 4;
             pp.value = paintValue;
         }
         if (strokeRadius == 0)
@@ -621,7 +622,8 @@ struct Builder
             if (isStroke)
                 strokeRadius = fmaxf(d.thickness * .5f, 1.17549435e-38f);
             pushPathDraw(d.path, m, strokeRadius, (Join)d.join, (Cap)d.cap,
-                         (FillRule)sp.fill_rule, paintType, paintValue, drawClipID, 0, gp);
+                         (FillRule)sp.fill_rule, paintType, paintValue, drawClipID, 0, gp,
+                         shader_blend_mode(d.blend_mode));
         }
         finishGradients();
         f.pathCount = (uint32_t)f.pathData.size() - 1;
@@ -982,6 +984,31 @@ inline uint32_t to_unorm8(float v)
 }
 } // namespace
 
+// Gradient paint color, unpremultiplied (draw_path.vert:285-331,450-476): the
+// paint matrix maps the pixel to t, the ramp row is sampled with linear
+// filtering at texel centers.
+void sample_gradient(const Flush& f, const rive_hip_paint_aux& aux, const rive_hip_paint_data& paint,
+                     const std::vector<uint32_t>& ramps, int x, int y, float color[4])
+{
+    const uint32_t paintType = paint.params & 0xfu;
+    float fx = float(x) + .5f, fy = float(y) + .5f;
+    float pcx = aux.paint_matrix[0] * fx + aux.paint_matrix[2] * fy + aux.paint_matrix[4];
+    float pcy = aux.paint_matrix[1] * fx + aux.paint_matrix[3] * fy + aux.paint_matrix[5];
+    float t = paintType == LINEAR_GRADIENT_PAINT_TYPE ? pcx : sqrtf(pcx * pcx + pcy * pcy);
+    t = std::min(std::max(t, 0.f), 1.f);
+    float span = aux.grad_span[0] > .9f ? 2.f : aux.grad_span[1];
+    float u = span > 1.f ? (1.f - 1.f / 512.f) * t + (.5f / 512.f) : (1.f / 512.f) * t + span;
+    float rowf = ubits(paint.value);
+    uint32_t row = (uint32_t)(rowf * f.gradHeight);
+    float tc = u * 512.f - .5f;
+    int i0 = (int)floorf(tc);
+    float fr = tc - (float)i0;
+    int ia = std::min(std::max(i0, 0), 511), ib = std::min(std::max(i0 + 1, 0), 511);
+    uint32_t ta = ramps[(size_t)row * 512 + ia], tb = ramps[(size_t)row * 512 + ib];
+    for (int k = 0; k < 4; ++k)
+        color[k] = (unorm8(tb, k) - unorm8(ta, k)) * fr + unorm8(ta, k);
+}
+
 void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* rgba)
 {
     const uint32_t W = f.width, H = f.height;
@@ -1113,8 +1140,35 @@ void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* 
             uint32_t clipID = paint.params >> 16;
             if (clipID != 0)
                 coverage = clipIDs[p] == clipID ? std::min(clipCov[p], coverage) : 0.f;
+            const uint32_t blendMode = (paint.params >> 4) & 0xfu;
+            uint32_t dst = rgba[p];
             float color[4];
-            if (paintType == SOLID_COLOR_PAINT_TYPE)
+            if (blendMode != BLEND_SRC_OVER)
+            {
+                // draw_raster_order_path.frag:191-204: the paint stays
+                // unpremultiplied, the advanced blend mixes it with the blend
+                // coefficients by the destination alpha, then premultiplies.
+                float src[4];
+                if (paintType == SOLID_COLOR_PAINT_TYPE)
+                {
+                    for (int k = 0; k < 4; ++k)
+                        src[k] = unorm8(paint.value, k);
+                }
+                else
+                {
+                    sample_gradient(f, aux, paint, ramps, x, y, src);
+                }
+                src[3] *= coverage;
+                const float dp[4] = {unorm8(dst, 0), unorm8(dst, 1), unorm8(dst, 2), unorm8(dst, 3)};
+                float coeffs[3];
+                advanced_blend_coeffs(src, dp, blendMode, coeffs);
+                for (int k = 0; k < 3; ++k)
+                    color[k] = (coeffs[k] - src[k]) * dp[3] + src[k];
+                color[3] = src[3];
+                for (int k = 0; k < 3; ++k)
+                    color[k] *= color[3];
+            }
+            else if (paintType == SOLID_COLOR_PAINT_TYPE)
             {
                 float a = unorm8(paint.value, 3);
                 color[0] = unorm8(paint.value, 0) * a;
@@ -1126,30 +1180,12 @@ void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* 
             }
             else
             {
-                float fx = float(x) + .5f, fy = float(y) + .5f;
-                float pcx = aux.paint_matrix[0] * fx + aux.paint_matrix[2] * fy + aux.paint_matrix[4];
-                float pcy = aux.paint_matrix[1] * fx + aux.paint_matrix[3] * fy + aux.paint_matrix[5];
-                float t = paintType == LINEAR_GRADIENT_PAINT_TYPE ? pcx : sqrtf(pcx * pcx + pcy * pcy);
-                t = std::min(std::max(t, 0.f), 1.f);
-                float span = aux.grad_span[0] > .9f ? 2.f : aux.grad_span[1];
-                float u = span > 1.f ? (1.f - 1.f / 512.f) * t + (.5f / 512.f)
-                                     : (1.f / 512.f) * t + span;
-                float rowf = ubits(paint.value);
-                // Linear filtering at texel centers; rows are sampled at centers.
-                uint32_t row = (uint32_t)(rowf * f.gradHeight);
-                float tc = u * 512.f - .5f;
-                int i0 = (int)floorf(tc);
-                float fr = tc - (float)i0;
-                int ia = std::min(std::max(i0, 0), 511), ib = std::min(std::max(i0 + 1, 0), 511);
-                uint32_t ta = ramps[(size_t)row * 512 + ia], tb = ramps[(size_t)row * 512 + ib];
-                for (int k = 0; k < 4; ++k)
-                    color[k] = (unorm8(tb, k) - unorm8(ta, k)) * fr + unorm8(ta, k);
+                sample_gradient(f, aux, paint, ramps, x, y, color);
                 color[3] *= coverage;
                 color[0] *= color[3];
                 color[1] *= color[3];
                 color[2] *= color[3];
             }
-            uint32_t dst = rgba[p];
             float paintAlpha = color[3];
             float out[4];
             for (int k = 0; k < 4; ++k)

@@ -90,7 +90,7 @@ struct rive_hip_context
     // pipeline scratch (grow-only)
     DevBuf spanTexelCounts, spanTexelOffsets, scanPartials, scalars;
     DevBuf tess, prims, cells, cellCount, cellOffset, cellCursor, cellBackdrop, binned;
-    DevBuf pathRaster, ramps, drawTess;
+    DevBuf pathRaster, ramps, drawTess, verts;
     DevBuf geoCounts, geoRecords, geoContours, geoSpans, geoSpanCounts, geoSpanOffsets;
     uint32_t lastMode = 0, lastRecordCount = 0, lastTessVertexCount = 0;
     uint32_t* hostScalars = nullptr; // pinned
@@ -150,7 +150,8 @@ void rive_hip_destroy(rive_hip_context* ctx)
                      &ctx->tess, &ctx->prims, &ctx->cells, &ctx->cellCount, &ctx->cellOffset,
                      &ctx->cellCursor, &ctx->cellBackdrop, &ctx->binned, &ctx->pathRaster,
                      &ctx->ramps, &ctx->drawTess, &ctx->geoCounts, &ctx->geoRecords,
-                     &ctx->geoContours, &ctx->geoSpans, &ctx->geoSpanCounts, &ctx->geoSpanOffsets};
+                     &ctx->geoContours, &ctx->geoSpans, &ctx->geoSpanCounts, &ctx->geoSpanOffsets,
+                     &ctx->verts};
     for (DevBuf* b : all)
         if (b->ptr)
             (void)hipFree(b->ptr);
@@ -372,9 +373,14 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     cells_init_kernel<<<(pathCount + 1 + 255) / 256, 256, 0, ctx->stream>>>(P<PathCells>(ctx->cells),
                                                                              pathCount + 1);
     HIP_TRY(ensure(ctx->prims, (size_t)std::max<uint32_t>(totalPrims, 1) * sizeof(Prim)));
+    HIP_TRY(ensure(ctx->verts, (size_t)std::max<uint32_t>(totalPrims, 1) * sizeof(VtxPair)));
+    if (totalPrims > 0)
+        vertex_kernel<<<(totalPrims + 255) / 256, 256, 0, ctx->stream>>>(
+            P<TessTexel>(ctx->tess), (int)texels, paths, drawTess, drawCount, totalPrims, P<VtxPair>(ctx->verts));
     if (totalPrims > 0)
         expand_kernel<<<(totalPrims + 255) / 256, 256, 0, ctx->stream>>>(
-            P<TessTexel>(ctx->tess), (int)texels, contours, contourCount, paths, drawTess, drawCount,
+            P<TessTexel>(ctx->tess), (int)texels, P<VtxPair>(ctx->verts), contours, contourCount, paths,
+            drawTess, drawCount,
             totalPrims, tilesX,
             tilesY, P<Prim>(ctx->prims), P<PathCells>(ctx->cells));
     cells_layout_kernel<<<1, 1024, 0, ctx->stream>>>(P<PathCells>(ctx->cells), pathCount + 1, tilesX,

@@ -525,7 +525,55 @@ __device__ __forceinline__ int64_t orient64(int32_t ax, int32_t ay, int32_t bx, 
     return (int64_t)(bx - ax) * (int64_t)(qy - ay) - (int64_t)(by - ay) * (int64_t)(qx - ax);
 }
 
-__global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels,
+// A forward texel's two outset vertices, snapped, with stroke coverage.
+struct VtxPair
+{
+    int32_t p[4]; // outset -1 (x, y), outset +1 (x, y)
+    float cov[4];
+};
+
+__device__ __forceinline__ uint32_t find_draw(const DrawTess* __restrict__ draws, uint32_t drawCount, uint32_t g)
+{
+    uint32_t lo = 0, hi = drawCount;
+    while (hi - lo > 1)
+    {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (draws[mid].primBase <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Vertex stage once per forward texel (draw_path_common.glsl:275-789); the
+// segment between texel i and next(i) reuses both ends.
+__global__ void vertex_kernel(const TessTexel* __restrict__ tex, int texels, const PathDataV* __restrict__ paths,
+                              const DrawTess* __restrict__ draws, uint32_t drawCount, uint32_t totalPrims,
+                              VtxPair* __restrict__ verts)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= totalPrims)
+        return;
+    const DrawTess d = draws[find_draw(draws, drawCount, g)];
+    const int i = min((int)(d.forwardBase + (g - d.primBase)), texels - 2);
+    const PathDataV pd = paths[d.pathID];
+    const uint32_t cfi = tex[i].w;
+    const VsOut a0 = draw_vertex(tex, texels, i, cfi, -1.f, pd);
+    const VsOut a1 = draw_vertex(tex, texels, i, cfi, +1.f, pd);
+    VtxPair v;
+    v.p[0] = snap(a0.x);
+    v.p[1] = snap(a0.y);
+    v.p[2] = snap(a1.x);
+    v.p[3] = snap(a1.y);
+    v.cov[0] = a0.cx;
+    v.cov[1] = a0.cy;
+    v.cov[2] = a1.cx;
+    v.cov[3] = a1.cy;
+    verts[g] = v;
+}
+
+__global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels, const VtxPair* __restrict__ verts,
                               const ContourV* __restrict__ contours, uint32_t contourCount,
                               const PathDataV* __restrict__ paths, const DrawTess* __restrict__ draws,
                               uint32_t drawCount, uint32_t totalPrims, int canvasTilesX,
@@ -535,16 +583,7 @@ __global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels,
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= totalPrims)
         return;
-    uint32_t lo = 0, hi = drawCount;
-    while (hi - lo > 1)
-    {
-        uint32_t mid = (lo + hi) >> 1;
-        if (draws[mid].primBase <= g)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    const DrawTess d = draws[lo];
+    const DrawTess d = draws[find_draw(draws, drawCount, g)];
     const int i = min((int)(d.forwardBase + (g - d.primBase)), texels - 2);
     Prim out;
     out.path = d.pathID;
@@ -566,27 +605,34 @@ __global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels,
         cfj = (tex[j].w & ~MIRRORED_CONTOUR_CONTOUR_FLAG) | (cfi & MIRRORED_CONTOUR_CONTOUR_FLAG);
     }
     const bool stroke = pd.strokeRadius != 0.f;
-    VsOut a0 = draw_vertex(tex, texels, i, cfi, -1.f, pd);
-    VsOut a1 = draw_vertex(tex, texels, i, cfi, +1.f, pd);
-    VsOut b0 = draw_vertex(tex, texels, j, cfj, -1.f, pd);
-    VsOut b1 = draw_vertex(tex, texels, j, cfj, +1.f, pd);
     // fill: [I0, O0, I1, O1]; stroke: [P0-, P0+, P1-, P1+]
-    out.p[0] = snap(a0.x);
-    out.p[1] = snap(a0.y);
-    out.p[2] = snap(a1.x);
-    out.p[3] = snap(a1.y);
-    out.p[4] = snap(b0.x);
-    out.p[5] = snap(b0.y);
-    out.p[6] = snap(b1.x);
-    out.p[7] = snap(b1.y);
-    out.cov[0] = a0.cx;
-    out.cov[1] = a0.cy;
-    out.cov[2] = a1.cx;
-    out.cov[3] = a1.cy;
-    out.cov[4] = b0.cx;
-    out.cov[5] = b0.cy;
-    out.cov[6] = b1.cx;
-    out.cov[7] = b1.cy;
+    const VtxPair va = verts[g];
+    VtxPair vb;
+    const int64_t rel = (int64_t)j - (int64_t)d.forwardBase;
+    if (rel >= 0 && rel < (int64_t)d.forwardCount && cfj == tex[j].w)
+        vb = verts[d.primBase + (uint32_t)rel];
+    else
+    {
+        // next(i) outside this draw's forward texels, or with other flags
+        const VsOut b0 = draw_vertex(tex, texels, j, cfj, -1.f, pd);
+        const VsOut b1 = draw_vertex(tex, texels, j, cfj, +1.f, pd);
+        vb.p[0] = snap(b0.x);
+        vb.p[1] = snap(b0.y);
+        vb.p[2] = snap(b1.x);
+        vb.p[3] = snap(b1.y);
+        vb.cov[0] = b0.cx;
+        vb.cov[1] = b0.cy;
+        vb.cov[2] = b1.cx;
+        vb.cov[3] = b1.cy;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        out.p[k] = va.p[k];
+        out.p[4 + k] = vb.p[k];
+        out.cov[k] = va.cov[k];
+        out.cov[4 + k] = vb.cov[k];
+    }
     uint32_t flags = stroke ? kPrimStroke : kPrimFill;
     if (!stroke && ((cfi & NEGATE_PATH_FILL_COVERAGE_FLAG) != 0u))
         flags |= kPrimNegate;
@@ -862,20 +908,30 @@ __device__ __forceinline__ float quantize8(float v)
 
 // One wave64 per 16x16 tile; each lane owns 4 pixels of one row
 // (row = lane / 4, columns 4 * (lane % 4) .. +3).  Paths are visited in
-// submission order.  For a path with primitives in the tile, the wave splits
-// them into (primitive, row) work items, 16 rows per primitive and 4
-// primitives per pass, that deposit into LDS:
-//   sRowBase[row]        winding change along the tile's left edge down to the
-//                        row's sample line (edges crossing x = X0);
-//   sRowDelta[row][i]    +-1 where an inner edge crosses the row's sample line
-//                        between pixels i-1 and i (prefix-summed per pixel);
-//   sBorder[row][i]      fills: signed AA-border coverage, 1/65536 fixed point;
-//                        strokes: max coverage as float bits.
-// The integer predicates are the ones of the per-pixel formulation: a pixel's
-// row crossing column comes from solving the same threshold inequality, and a
+// submission order and resolved per pixel in that order.
+//
+// Paths with primitives in the tile are gathered into batches of up to
+// kSlots paths (and, where it fits, 64 primitives).  Each batched path owns an
+// LDS accumulator slot:
+//   sLeft[slot][row]      winding change along the tile's left edge, as a
+//                         suffix delta over rows (prefix-summed per row);
+//   sDelta[slot][row][i]  +-1 where an inner edge crosses the row's sample
+//                         line between pixels i-1 and i (prefix-summed per
+//                         pixel);
+//   sBorder[slot][row][i] fills: signed AA-border coverage, 1/65536 fixed
+//                         point; strokes: max coverage as float bits.
+// The wave first takes one primitive per lane (left-edge terms, and the
+// primitive's range of sample rows), then compacts the (primitive, row) work
+// items of the whole batch across the 64 lanes.  Paths without primitives in
+// the tile (uniform coverage from the corner winding) ride in the same batch
+// without a slot.  The integer predicates are those of a per-pixel
+// evaluation: crossing columns solve the same threshold inequality, and a
 // triangle's pixels on a row are the candidates of a float span estimate
 // (widened by a pixel) that pass the exact three-edge test.
 constexpr int kRasterLanes = 64;
+constexpr int kSlots = 4;
+constexpr int kMaxEntries = 32;
+constexpr int kMaxItems = kRasterLanes * kTile;
 
 struct TileGeom
 {
@@ -900,7 +956,8 @@ __device__ __forceinline__ void tri_row_candidates(int32_t ax, int32_t ay, int32
             }
             else
             {
-                const float t = (float)(yq - y0) / (float)(y1 - y0);
+                // an estimate: the candidates carry a pixel of slack
+                const float t = __fdividef((float)(yq - y0), (float)(y1 - y0));
                 const float x = (float)x0 + t * (float)(x1 - x0);
                 xl = fminf(xl, x);
                 xr = fmaxf(xr, x);
@@ -938,58 +995,91 @@ __device__ __forceinline__ bool edge_in(const EdgeK& k, int32_t i2, int32_t j2)
     return k.A * i2 + k.B * j2 >= k.thr;
 }
 
-__device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* __restrict__ sRowDelta,
-                              int32_t* __restrict__ sRowBase, int32_t* __restrict__ sBorder)
+// Sample rows [*r0, *r1] of the tile that a primitive's quad can touch.
+__device__ __forceinline__ void prim_rows(const Prim& p, int32_t Y0, int* r0, int* r1)
+{
+    const int32_t miny = min(min(p.p[1], p.p[3]), min(p.p[5], p.p[7]));
+    const int32_t maxy = max(max(p.p[1], p.p[3]), max(p.p[5], p.p[7]));
+    const int32_t lo = miny - Y0 - (1 << (kSubpixelLog2 - 1));
+    const int32_t hi = maxy - Y0 - (1 << (kSubpixelLog2 - 1));
+    *r0 = max(-((-lo) >> kSubpixelLog2), 0);          // ceil(lo / 256)
+    *r1 = min(hi >> kSubpixelLog2, kTile - 1);        // floor(hi / 256)
+}
+
+// Left-edge winding of a fill primitive's inner edge I0 -> I1: the change of
+// the winding number from the tile's corner down to each row's sample line,
+// which is 0 above some row and a constant +-1 from it on.
+__device__ void fill_prim_left(const Prim& p, const TileGeom& g, int32_t* __restrict__ sLeft)
 {
     const int32_t X0 = g.X0, Y0 = g.Y0;
-    const int32_t j2 = 2 * j + 1;
-    const int32_t yq = Y0 + (j << kSubpixelLog2) + (1 << (kSubpixelLog2 - 1));
+    const int32_t xmin = min(p.p[0], p.p[4]), xmax = max(p.p[0], p.p[4]);
+    if (!(xmin <= X0 && X0 < xmax))
+        return;
     const int neg = (p.flags & kPrimNegate) ? -1 : 1;
-    // Inner edge I0 -> I1: winding.
-    const EdgeT e = make_edge(p.p[0], p.p[1], p.p[4], p.p[5], X0, Y0);
     if (p.p[1] != p.p[5])
     {
+        const EdgeT e = make_edge(p.p[0], p.p[1], p.p[4], p.p[5], X0, Y0);
         const int s = p.p[5] < p.p[1] ? 1 : -1;
         // R(q) <=> s*D(q) >= 0 <=> s*k >= ceil(-s*D0/128)
         const int32_t T = clamp_thr(ceil_div128(s > 0 ? -e.D0 : e.D0));
-        const int32_t krow = e.B * j2;
-        const bool Rrow = s * krow >= T;
-        const int32_t xmin = min(p.p[0], p.p[4]), xmax = max(p.p[0], p.p[4]);
-        if (xmin <= X0 && X0 < xmax)
+        const bool Rcorner = (s > 0 ? e.D0 : -e.D0) >= 0;
+        for (int j = 0; j < kTile; ++j)
         {
-            const bool Rcorner = (s > 0 ? e.D0 : -e.D0) >= 0;
-            const int d = (int)Rrow - (int)Rcorner;
-            if (d != 0)
-                atomicAdd(&sRowBase[j], neg * s * d);
-        }
-        const int32_t ymin = min(p.p[1], p.p[5]), ymax = max(p.p[1], p.p[5]);
-        if (!Rrow && ymin <= yq && yq < ymax)
-        {
-            // Smallest i with s*(A*(2i+1) + krow) >= T, where s*A = |dy| > 0:
-            // 2|dy| i >= T - |dy| - s*krow.  The quotient is exact in double
-            // wherever it decides a column inside the tile.
-            const int64_t sa = (int64_t)(s * e.A);
-            const int64_t r = (int64_t)T - sa - (int64_t)s * krow;
-            const double q = ceil((double)r / (double)(2 * sa));
-            if (q < (double)kTile)
+            const bool Rrow = s * (e.B * (2 * j + 1)) >= T;
+            if (Rrow != Rcorner)
             {
-                const int istar = q < 0.0 ? 0 : (int)q;
-                atomicAdd(&sRowDelta[j * kTile + istar], neg * s);
+                atomicAdd(&sLeft[j], neg * s * ((int)Rrow - (int)Rcorner));
+                break;
             }
         }
     }
     else if (p.p[0] != p.p[4])
     {
         // A horizontal edge never meets a sample line, but where it crosses
-        // the tile's left edge the winding along that edge steps by sign(dx).
-        const int32_t xmin = min(p.p[0], p.p[4]), xmax = max(p.p[0], p.p[4]);
-        if (xmin <= X0 && X0 < xmax)
+        // the tile's left edge the winding below it steps by sign(dx).
+        const int32_t h = p.p[1];
+        if (Y0 < h)
         {
-            const int32_t h = p.p[1];
-            const int sg = p.p[4] > p.p[0] ? 1 : -1;
-            const int d = (int)(yq >= h) - (int)(Y0 >= h);
-            if (d != 0)
-                atomicAdd(&sRowBase[j], neg * sg * d);
+            const int32_t v = h - Y0 - (1 << (kSubpixelLog2 - 1));
+            const int jstar = max(-((-v) >> kSubpixelLog2), 0); // first row with yq >= h
+            if (jstar < kTile)
+                atomicAdd(&sLeft[jstar], neg * (p.p[4] > p.p[0] ? 1 : -1));
+        }
+    }
+}
+
+// One sample row of a fill primitive: the inner edge's crossing column and
+// the two AA border triangles.
+__device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* __restrict__ sDelta,
+                              int32_t* __restrict__ sBorder)
+{
+    const int32_t X0 = g.X0, Y0 = g.Y0;
+    const int32_t j2 = 2 * j + 1;
+    const int32_t yq = Y0 + (j << kSubpixelLog2) + (1 << (kSubpixelLog2 - 1));
+    const int neg = (p.flags & kPrimNegate) ? -1 : 1;
+    const EdgeT e = make_edge(p.p[0], p.p[1], p.p[4], p.p[5], X0, Y0);
+    if (p.p[1] != p.p[5])
+    {
+        const int32_t ymin = min(p.p[1], p.p[5]), ymax = max(p.p[1], p.p[5]);
+        if (ymin <= yq && yq < ymax)
+        {
+            const int s = p.p[5] < p.p[1] ? 1 : -1;
+            const int32_t T = clamp_thr(ceil_div128(s > 0 ? -e.D0 : e.D0));
+            const int32_t krow = e.B * j2;
+            if (!(s * krow >= T))
+            {
+                // Smallest i with s*(A*(2i+1) + krow) >= T, where s*A = |dy|:
+                // 2|dy| i >= T - |dy| - s*krow.  The quotient is exact in
+                // double wherever it decides a column inside the tile.
+                const int64_t sa = (int64_t)(s * e.A);
+                const int64_t r = (int64_t)T - sa - (int64_t)s * krow;
+                const double q = ceil((double)r / (double)(2 * sa));
+                if (q < (double)kTile)
+                {
+                    const int istar = q < 0.0 ? 0 : (int)q;
+                    atomicAdd(&sDelta[j * kTile + istar], neg * s);
+                }
+            }
         }
     }
     // Border triangle T1 = (I0, O0, I1), coverage 1 - lambda(O0).
@@ -1004,6 +1094,7 @@ __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* 
             const EdgeK k01 = edge_k(make_edge(p.p[0], p.p[1], p.p[2], p.p[3], X0, Y0), o);
             const EdgeK k12 = edge_k(make_edge(p.p[2], p.p[3], p.p[4], p.p[5], X0, Y0), o);
             const EdgeK k20 = edge_k(make_edge(p.p[4], p.p[5], p.p[0], p.p[1], X0, Y0), o);
+            const float fA1 = (float)A1;
             for (int i = ilo; i <= ihi; ++i)
             {
                 const int32_t i2 = 2 * i + 1;
@@ -1011,7 +1102,7 @@ __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* 
                 {
                     const int32_t ke = e.A * i2 + e.B * j2;
                     const float De = (float)e.D0 + 128.f * (float)ke;
-                    const float cv = 1.f + De / (float)A1;
+                    const float cv = 1.f + De / fA1;
                     atomicAdd(&sBorder[j * kTile + i], neg * o * __float2int_rn(cv * 65536.f));
                 }
             }
@@ -1030,6 +1121,7 @@ __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* 
             const EdgeT f12e = make_edge(p.p[2], p.p[3], p.p[6], p.p[7], X0, Y0);
             const EdgeK f12 = edge_k(f12e, o);
             const EdgeK f20 = edge_k(make_edge(p.p[6], p.p[7], p.p[4], p.p[5], X0, Y0), o);
+            const float fA2 = (float)A2;
             for (int i = ilo; i <= ihi; ++i)
             {
                 const int32_t i2 = 2 * i + 1;
@@ -1037,7 +1129,7 @@ __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* 
                 {
                     const int32_t k12 = f12e.A * i2 + f12e.B * j2;
                     const float D12 = (float)f12e.D0 + 128.f * (float)k12;
-                    const float cv = D12 / (float)A2;
+                    const float cv = D12 / fA2;
                     atomicAdd(&sBorder[j * kTile + i], neg * o * __float2int_rn(cv * 65536.f));
                 }
             }
@@ -1133,6 +1225,134 @@ __device__ __forceinline__ float fill_rule_coverage(float count, uint32_t flags)
     return fminf(coverage, 1.f);
 }
 
+// Advanced blend coefficients from unpremultiplied src and premultiplied dst
+// (advanced_blend.glsl:87-280; unmultiply_rgb, common.glsl:210-216), fp32.
+constexpr float kEpsFp16NonDenorm = 6.2e-5f;
+__device__ __forceinline__ float min3f(const float* c) { return fminf(c[0], fminf(c[1], c[2])); }
+__device__ __forceinline__ float max3f(const float* c) { return fmaxf(c[0], fmaxf(c[1], c[2])); }
+__device__ __forceinline__ float lum3(const float* c) { return c[0] * .30f + c[1] * .59f + c[2] * .11f; }
+__device__ __forceinline__ float sign0(float x) { return x > 0.f ? 1.f : (x < 0.f ? -1.f : 0.f); }
+__device__ void set_lum_d(const float* base, const float* lumColor, float* out)
+{
+    const float lumTarget = lum3(lumColor);
+    const float l = lum3(base);
+    float biased[3] = {base[0] - l, base[1] - l, base[2] - l};
+    const float sx = lumTarget / fmaxf(kEpsFp16NonDenorm, -min3f(biased));
+    const float sy = (1.f - lumTarget) / fmaxf(kEpsFp16NonDenorm, max3f(biased));
+    const float satScale = fminf(1.f, fminf(sx, sy));
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        out[i] = biased[i] * satScale + lumTarget;
+}
+__device__ void set_lum_sat_d(const float* hue, const float* sat, const float* lumColor, float* out)
+{
+    const float satTarget = max3f(sat) - min3f(sat);
+    const float mn = min3f(hue);
+    float h[3] = {hue[0] - mn, hue[1] - mn, hue[2] - mn};
+    const float scale = satTarget / fmaxf(kEpsFp16NonDenorm, max3f(h));
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        h[i] *= scale;
+    set_lum_d(h, lumColor, out);
+}
+__device__ void advanced_blend_coeffs_d(const float* srcIn, const float* dp, uint32_t mode, float* out)
+{
+    float src[3] = {srcIn[0], srcIn[1], srcIn[2]};
+    const float inva = dp[3] != 0.f ? 1.f / dp[3] : 0.f;
+    const float dst[3] = {dp[0] * inva, dp[1] * inva, dp[2] * inva};
+    switch (mode)
+    {
+        case 11u: // multiply
+            for (int i = 0; i < 3; ++i)
+                out[i] = src[i] * dst[i];
+            break;
+        case 1u: // screen
+            for (int i = 0; i < 3; ++i)
+                out[i] = src[i] + dst[i] - src[i] * dst[i];
+            break;
+        case 2u: // overlay
+            for (int i = 0; i < 3; ++i)
+            {
+                const float sd = src[i] * dst[i];
+                out[i] = 2.f * (dst[i] > .5f ? src[i] + dst[i] - sd - .5f : sd);
+            }
+            break;
+        case 3u: // darken
+            for (int i = 0; i < 3; ++i)
+                out[i] = fminf(src[i], dst[i]);
+            break;
+        case 4u: // lighten
+            for (int i = 0; i < 3; ++i)
+                out[i] = fmaxf(src[i], dst[i]);
+            break;
+        case 5u: // color dodge
+            for (int i = 0; i < 3; ++i)
+            {
+                const float d = fminf(fmaxf(dp[i], 0.f), dp[3]);
+                const float denom = fminf(fmaxf(1.f - src[i], 0.f), 1.f) * dp[3];
+                out[i] = denom == 0.f ? sign0(d) : fminf(1.f, d / denom);
+            }
+            break;
+        case 6u: // color burn
+            for (int i = 0; i < 3; ++i)
+            {
+                const float s = fminf(fmaxf(src[i], 0.f), 1.f);
+                const float d = fminf(fmaxf(dp[i], 0.f), dp[3]);
+                const float aa = dp[3] == 0.f ? 1.f : dp[3];
+                const float numer = aa - d;
+                out[i] = 1.f - (s == 0.f ? sign0(numer) : fminf(1.f, numer / (s * aa)));
+            }
+            break;
+        case 7u: // hard light
+            for (int i = 0; i < 3; ++i)
+            {
+                const float sd = src[i] * dst[i];
+                out[i] = 2.f * (src[i] > .5f ? src[i] + dst[i] - sd - .5f : sd);
+            }
+            break;
+        case 8u: // soft light
+            for (int i = 0; i < 3; ++i)
+            {
+                float c;
+                if (src[i] <= .5f)
+                    c = 1.f - dst[i];
+                else if (dst[i] <= .25f)
+                    c = (16.f * dst[i] - 12.f) * dst[i] + 3.f;
+                else
+                    c = 1.f / sqrtf(dst[i]) - 1.f;
+                out[i] = dst[i] + dst[i] * (2.f * src[i] - 1.f) * c;
+            }
+            break;
+        case 9u: // difference
+            for (int i = 0; i < 3; ++i)
+                out[i] = fabsf(dst[i] - src[i]);
+            break;
+        case 10u: // exclusion
+            for (int i = 0; i < 3; ++i)
+                out[i] = src[i] + dst[i] - 2.f * src[i] * dst[i];
+            break;
+        case 12u: // hue
+        case 13u: // saturation
+        case 14u: // color
+        case 15u: // luminosity
+            for (int i = 0; i < 3; ++i)
+                src[i] = fminf(fmaxf(src[i], 0.f), 1.f);
+            if (mode == 12u)
+                set_lum_sat_d(src, dst, dst, out);
+            else if (mode == 13u)
+                set_lum_sat_d(dst, src, dst, out);
+            else if (mode == 14u)
+                set_lum_d(src, dst, out);
+            else
+                set_lum_d(dst, src, out);
+            break;
+        default:
+            for (int i = 0; i < 3; ++i)
+                out[i] = src[i];
+            break;
+    }
+}
+
 // Clip, paint and src-over of one path into one pixel (draw_path.vert:431-545,
 // draw_raster_order_path.frag:84-238).  A zero coverage leaves the pixel
 // unchanged, so callers skip it (except for clip updates).
@@ -1151,11 +1371,19 @@ __device__ __forceinline__ void resolve_pixel(const PathRaster& pr, float covera
     if (pr.clipID != 0u)
         coverage = clipID == pr.clipID ? fminf(clipCov, coverage) : 0.f;
     float src[4];
-    if (pr.paintType == SOLID_COLOR_PAINT_TYPE)
+    const uint32_t blendMode = (pr.flags >> 8) & 0xfu;
+    if (pr.paintType == SOLID_COLOR_PAINT_TYPE && blendMode == 0u)
     {
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             src[k] = pr.color[k] * coverage;
+    }
+    else if (pr.paintType == SOLID_COLOR_PAINT_TYPE)
+    {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            src[k] = pr.paintMatrix[k];
+        src[3] *= coverage;
     }
     else
     {
@@ -1180,9 +1408,23 @@ __device__ __forceinline__ void resolve_pixel(const PathRaster& pr, float covera
             src[k] = (vb - va) * fr + va;
         }
         src[3] *= coverage;
-        src[0] *= src[3];
-        src[1] *= src[3];
-        src[2] *= src[3];
+        if (blendMode == 0u)
+        {
+            src[0] *= src[3];
+            src[1] *= src[3];
+            src[2] *= src[3];
+        }
+    }
+    if (blendMode != 0u)
+    {
+        // draw_raster_order_path.frag:191-204: mix the unpremultiplied paint
+        // with the blend coefficients by the destination alpha, then
+        // premultiply.
+        float coeffs[3];
+        advanced_blend_coeffs_d(src, col, blendMode, coeffs);
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            src[k] = ((coeffs[k] - src[k]) * col[3] + src[k]) * src[3];
     }
     const float paintAlpha = src[3];
     float out[4];
@@ -1200,6 +1442,11 @@ __device__ __forceinline__ void resolve_pixel(const PathRaster& pr, float covera
         col[k] = quantize8(out[k]);
 }
 
+__device__ __forceinline__ uint32_t lane_read(uint32_t v, int lane)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+
 __global__ __launch_bounds__(kRasterLanes) void raster_kernel(
     const PathCells* __restrict__ cells, const PathRaster* __restrict__ raster, uint32_t pathCount,
     const uint32_t* __restrict__ cellOffset, const int32_t* __restrict__ cellBackdrop,
@@ -1207,9 +1454,13 @@ __global__ __launch_bounds__(kRasterLanes) void raster_kernel(
     const uint32_t* __restrict__ ramps, uint32_t gradHeight, int width, int height, int tilesX,
     uint32_t loadPreserve, uint32_t clearPremul, uint32_t dither, uint32_t* __restrict__ target)
 {
-    __shared__ __align__(16) int32_t sRowDelta[kTile * kTile];
-    __shared__ __align__(16) int32_t sBorder[kTile * kTile];
-    __shared__ int32_t sRowBase[kTile];
+    __shared__ __align__(16) int32_t sDelta[kSlots][kTile * kTile];
+    __shared__ __align__(16) int32_t sBorder[kSlots][kTile * kTile];
+    __shared__ int32_t sLeft[kSlots][kTile];
+    __shared__ int32_t sLeftPre[kSlots][kTile];
+    __shared__ uint8_t sOwner[kMaxItems];
+    __shared__ uint32_t sItemPrim[kRasterLanes];
+    __shared__ uint32_t sItemInfo[kRasterLanes]; // slot | stroke << 2 | row0 << 8 | item0 << 16
 
     const int tile = blockIdx.x;
     const int tx = tile % tilesX, ty = tile / tilesX;
@@ -1219,8 +1470,6 @@ __global__ __launch_bounds__(kRasterLanes) void raster_kernel(
     TileGeom g;
     g.X0 = tx << (kSubpixelLog2 + kTileLog2);
     g.Y0 = ty << (kSubpixelLog2 + kTileLog2);
-    // work items: primitive k0 + lane / 16, row lane % 16
-    const int itemPrim = lane >> 4, itemRow = lane & (kTile - 1);
 
     float col[4][4];
     float clipCov[4] = {0.f, 0.f, 0.f, 0.f};
@@ -1260,78 +1509,175 @@ __global__ __launch_bounds__(kRasterLanes) void raster_kernel(
         uint64_t mask = __ballot(relevant);
         while (mask != 0)
         {
-            const int b = __builtin_ctzll(mask);
-            mask &= mask - 1;
-            const uint32_t path = base + (uint32_t)b;
-            const uint32_t pOff = __shfl(off, b);
-            const uint32_t pN = __shfl(n, b);
-            const int32_t pC = __shfl(C, b);
-            const PathRaster pr = raster[path];
-            const bool isStroke = pr.flags & 1u;
-            float cov[4];
-            if (pN == 0)
+            // ---- gather a batch.  Entry e lives in lane e's registers, slot s
+            //      in lane s's.
+            int nE = 0, nSlots = 0;
+            uint32_t nPrim = 0;
+            uint32_t ePath = 0, eSlot = 0xffu;
+            int32_t eC = 0;
+            uint32_t sOff = 0, sPrim0 = 0, sStroke = 0;
+            while (mask != 0 && nE < kMaxEntries)
             {
-                // No edge in the tile: the coverage is the corner winding's.
-                const float u = isStroke ? 0.f : fill_rule_coverage((float)pC, pr.flags);
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    cov[q] = u;
-            }
-            else
-            {
-                for (int q = lane; q < kTile * kTile; q += kRasterLanes)
+                const int b = __builtin_ctzll(mask);
+                const uint32_t nb = lane_read(n, b);
+                const uint32_t path = base + (uint32_t)b;
+                uint32_t slotOf = 0xffu;
+                if (nb != 0)
                 {
-                    sRowDelta[q] = 0;
-                    sBorder[q] = 0;
-                }
-                if (lane < kTile)
-                    sRowBase[lane] = 0;
-                __syncthreads();
-                for (uint32_t k0 = 0; k0 < pN; k0 += 4)
-                {
-                    const uint32_t k = k0 + (uint32_t)itemPrim;
-                    if (k < pN)
+                    if (nSlots == kSlots || (nSlots > 0 && nPrim + nb > (uint32_t)kRasterLanes))
+                        break;
+                    const uint32_t offb = lane_read(off, b);
+                    const uint32_t strokeb = raster[path].flags & 1u;
+                    if (lane == nSlots)
                     {
-                        const Prim pm = prims[binned[pOff + k]];
-                        if (isStroke)
-                            stroke_prim_row(pm, itemRow, g, reinterpret_cast<uint32_t*>(sBorder));
-                        else
-                            fill_prim_row(pm, itemRow, g, sRowDelta, sRowBase, sBorder);
+                        sOff = offb;
+                        sPrim0 = nPrim;
+                        sStroke = strokeb;
                     }
+                    slotOf = (uint32_t)nSlots;
+                    ++nSlots;
+                    nPrim += nb;
+                }
+                const int32_t Cb = (int32_t)lane_read((uint32_t)C, b);
+                if (lane == nE)
+                {
+                    ePath = path;
+                    eSlot = slotOf;
+                    eC = Cb;
+                }
+                ++nE;
+                mask &= mask - 1;
+            }
+
+            // ---- primitives of the batch's slotted paths into LDS
+            if (nSlots > 0)
+            {
+                for (int q = lane; q < kSlots * kTile * kTile; q += kRasterLanes)
+                {
+                    (&sDelta[0][0])[q] = 0;
+                    (&sBorder[0][0])[q] = 0;
+                }
+                (&sLeft[0][0])[lane] = 0;
+                __syncthreads();
+                for (uint32_t c0 = 0; c0 < nPrim; c0 += kRasterLanes)
+                {
+                    const uint32_t gp = c0 + (uint32_t)lane;
+                    int rows = 0, r0 = 0;
+                    uint32_t info = 0, primIdx = 0;
+                    // this lane's slot: the last one starting at or before gp
+                    int slot = 0;
+#pragma unroll
+                    for (int s = 1; s < kSlots; ++s)
+                        if (s < nSlots && lane_read(sPrim0, s) <= gp)
+                            slot = s;
+                    const uint32_t so = (uint32_t)__shfl((int)sOff, slot);
+                    const uint32_t sp0 = (uint32_t)__shfl((int)sPrim0, slot);
+                    const uint32_t stroke = (uint32_t)__shfl((int)sStroke, slot);
+                    if (gp < nPrim)
+                    {
+                        primIdx = binned[so + (gp - sp0)];
+                        const Prim pm = prims[primIdx];
+                        int r1;
+                        prim_rows(pm, g.Y0, &r0, &r1);
+                        rows = max(r1 - r0 + 1, 0);
+                        if (!stroke)
+                            fill_prim_left(pm, g, sLeft[slot]);
+                        info = (uint32_t)slot | (stroke << 2) | ((uint32_t)r0 << 8);
+                    }
+                    // exclusive prefix of the row counts across the wave
+                    int incl = rows;
+#pragma unroll
+                    for (int d = 1; d < kRasterLanes; d <<= 1)
+                    {
+                        const int v = __shfl_up(incl, d);
+                        if (lane >= d)
+                            incl += v;
+                    }
+                    const int item0 = incl - rows;
+                    const int total = __shfl(incl, kRasterLanes - 1);
+                    for (int k = 0; k < rows; ++k)
+                        sOwner[item0 + k] = (uint8_t)lane;
+                    sItemPrim[lane] = primIdx;
+                    sItemInfo[lane] = info | ((uint32_t)item0 << 16);
+                    __syncthreads();
+                    for (int t = lane; t < total; t += kRasterLanes)
+                    {
+                        const int owner = sOwner[t];
+                        const uint32_t inf = sItemInfo[owner];
+                        const int slot = (int)(inf & 3u);
+                        const int row = (int)((inf >> 8) & 0xffu) + (t - (int)(inf >> 16));
+                        const Prim pm = prims[sItemPrim[owner]];
+                        if (inf & 4u)
+                            stroke_prim_row(pm, row, g, reinterpret_cast<uint32_t*>(sBorder[slot]));
+                        else
+                            fill_prim_row(pm, row, g, sDelta[slot], sBorder[slot]);
+                    }
+                    __syncthreads();
+                }
+                // row prefix of the left-edge deltas
+                {
+                    const int slot = lane >> kTileLog2, row = lane & (kTile - 1);
+                    int32_t acc = 0;
+                    for (int r = 0; r <= row; ++r)
+                        acc += sLeft[slot][r];
+                    sLeftPre[slot][row] = acc;
                 }
                 __syncthreads();
-                const int4 bv = reinterpret_cast<const int4*>(sBorder)[lane];
-                if (isStroke)
+            }
+
+            // ---- resolve the batch in submission order
+            for (int e = 0; e < nE; ++e)
+            {
+                const uint32_t path = lane_read(ePath, e);
+                const uint32_t slot = lane_read(eSlot, e);
+                const int32_t pC = (int32_t)lane_read((uint32_t)eC, e);
+                const PathRaster pr = raster[path];
+                const bool isStroke = pr.flags & 1u;
+                float cov[4];
+                if (slot == 0xffu)
                 {
-                    cov[0] = fminf(__int_as_float(bv.x), 1.f);
-                    cov[1] = fminf(__int_as_float(bv.y), 1.f);
-                    cov[2] = fminf(__int_as_float(bv.z), 1.f);
-                    cov[3] = fminf(__int_as_float(bv.w), 1.f);
+                    // No edge in the tile: the coverage is the corner winding's.
+                    const float u = isStroke ? 0.f : fill_rule_coverage((float)pC, pr.flags);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        cov[q] = u;
                 }
                 else
                 {
-                    const int4* rd = reinterpret_cast<const int4*>(sRowDelta + pj * kTile);
-                    int32_t acc = pC + sRowBase[pj];
-                    for (int gq = 0; gq < (lane & 3); ++gq)
+                    const int4 bv = reinterpret_cast<const int4*>(sBorder[slot])[lane];
+                    if (isStroke)
                     {
-                        const int4 v = rd[gq];
-                        acc += v.x + v.y + v.z + v.w;
+                        cov[0] = fminf(__int_as_float(bv.x), 1.f);
+                        cov[1] = fminf(__int_as_float(bv.y), 1.f);
+                        cov[2] = fminf(__int_as_float(bv.z), 1.f);
+                        cov[3] = fminf(__int_as_float(bv.w), 1.f);
                     }
-                    const int4 v = rd[lane & 3];
-                    const int32_t w0 = acc + v.x, w1 = w0 + v.y, w2 = w1 + v.z, w3 = w2 + v.w;
-                    cov[0] = fill_rule_coverage((float)w0 + (float)bv.x * (1.f / 65536.f), pr.flags);
-                    cov[1] = fill_rule_coverage((float)w1 + (float)bv.y * (1.f / 65536.f), pr.flags);
-                    cov[2] = fill_rule_coverage((float)w2 + (float)bv.z * (1.f / 65536.f), pr.flags);
-                    cov[3] = fill_rule_coverage((float)w3 + (float)bv.w * (1.f / 65536.f), pr.flags);
+                    else
+                    {
+                        const int4* rd = reinterpret_cast<const int4*>(sDelta[slot] + pj * kTile);
+                        int32_t acc = pC + sLeftPre[slot][pj];
+                        for (int gq = 0; gq < (lane & 3); ++gq)
+                        {
+                            const int4 v = rd[gq];
+                            acc += v.x + v.y + v.z + v.w;
+                        }
+                        const int4 v = rd[lane & 3];
+                        const int32_t w0 = acc + v.x, w1 = w0 + v.y, w2 = w1 + v.z, w3 = w2 + v.w;
+                        cov[0] = fill_rule_coverage((float)w0 + (float)bv.x * (1.f / 65536.f), pr.flags);
+                        cov[1] = fill_rule_coverage((float)w1 + (float)bv.y * (1.f / 65536.f), pr.flags);
+                        cov[2] = fill_rule_coverage((float)w2 + (float)bv.z * (1.f / 65536.f), pr.flags);
+                        cov[3] = fill_rule_coverage((float)w3 + (float)bv.w * (1.f / 65536.f), pr.flags);
+                    }
                 }
-                __syncthreads();
-            }
-            const bool clipUpdate = pr.flags & 8u;
+                const bool clipUpdate = pr.flags & 8u;
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (cov[q] != 0.f || clipUpdate)
-                    resolve_pixel(pr, cov[q], px0 + q, py, noise[q], ramps, gradHeight, col[q], clipCov[q],
-                                  clipID[q]);
+                for (int q = 0; q < 4; ++q)
+                    if (cov[q] != 0.f || clipUpdate)
+                        resolve_pixel(pr, cov[q], px0 + q, py, noise[q], ramps, gradHeight, col[q], clipCov[q],
+                                      clipID[q]);
+            }
+            if (nSlots > 0)
+                __syncthreads(); // the next batch clears the slots
         }
     }
     if (rowInside)
@@ -1391,6 +1737,8 @@ __global__ void path_raster_kernel(const PathDataV* __restrict__ paths, const ui
     {
         r.clipID = paint.x >> 16;
     }
+    if (paintType != CLIP_UPDATE_PAINT_TYPE)
+        flags |= ((paint.x >> 4) & 0xfu) << 8; // shader blend mode
     r.flags = flags;
     float a = float(paint.y >> 24) * (1.f / 255.f);
     r.color[0] = float(paint.y & 0xffu) * (1.f / 255.f) * a;
@@ -1400,6 +1748,14 @@ __global__ void path_raster_kernel(const PathDataV* __restrict__ paths, const ui
 #pragma unroll
     for (int k = 0; k < 6; ++k)
         r.paintMatrix[k] = ax[k];
+    if (paintType == SOLID_COLOR_PAINT_TYPE)
+    {
+        // Solid paints carry no paint matrix; keep the unpremultiplied color
+        // there for advanced blend modes.
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            r.paintMatrix[k] = float((paint.y >> (8 * k)) & 0xffu) * (1.f / 255.f);
+    }
     float span0 = ax[6], span1 = ax[7];
     r.gradSpan = span0 > .9f ? 2.f : span1;
     r.gradRow = (uint32_t)(u2f(paint.y) * (float)gradHeight);

@@ -333,6 +333,29 @@ namespace gpu
 {
 namespace
 {
+// ConvertBlendModeToPLSBlendMode (gpu.cpp:717-753): the shader's 4-bit mode.
+uint32_t shader_blend_mode(BlendMode m)
+{
+    switch (m)
+    {
+        case BlendMode::screen: return 1;
+        case BlendMode::overlay: return 2;
+        case BlendMode::darken: return 3;
+        case BlendMode::lighten: return 4;
+        case BlendMode::colorDodge: return 5;
+        case BlendMode::colorBurn: return 6;
+        case BlendMode::hardLight: return 7;
+        case BlendMode::softLight: return 8;
+        case BlendMode::difference: return 9;
+        case BlendMode::exclusion: return 10;
+        case BlendMode::multiply: return 11;
+        case BlendMode::hue: return 12;
+        case BlendMode::saturation: return 13;
+        case BlendMode::color: return 14;
+        case BlendMode::luminosity: return 15;
+        default: return 0;
+    }
+}
 uint32_t swizzle_rgba(uint32_t argb)
 {
     uint32_t a = argb >> 24, r = (argb >> 16) & 0xff, g = (argb >> 8) & 0xff, b = argb & 0xff;
@@ -384,7 +407,7 @@ void RenderContext::pushPathDraw(const RiveRenderPath* path, const Mat2D& m, con
             params = 1; // solid color
             r.paint.value = swizzle_rgba(paint->getColor());
         }
-        params |= clipID << 16;
+        params |= clipID << 16 | shader_blend_mode(paint->getBlendMode()) << 4;
     }
     if (!stroke)
     {
@@ -686,6 +709,7 @@ extern "C" int rive_hip_render_stream(rive_hip_context* ctx, const rive_stream* 
         paint.join((rive::StrokeJoin)d.join);
         paint.cap((rive::StrokeCap)d.cap);
         paint.feather(d.feather);
+        paint.blendMode((rive::BlendMode)d.blend_mode);
         if (d.paint_type == 0)
             paint.color(d.color);
         else

@@ -53,7 +53,26 @@ enum class FillRule : uint32_t { nonZero, evenOdd, clockwise };
 enum class StrokeJoin : uint32_t { miter = 0, round = 1, bevel = 2 };
 enum class StrokeCap : uint32_t { butt = 0, round = 1, square = 2 };
 enum class RenderPaintStyle : uint32_t { stroke, fill };
-enum class BlendMode : uint8_t { srcOver = 3 };
+// include/rive/shapes/paint/blend_mode.hpp:5-23
+enum class BlendMode : uint8_t
+{
+    srcOver = 3,
+    screen = 14,
+    overlay = 15,
+    darken = 16,
+    lighten = 17,
+    colorDodge = 18,
+    colorBurn = 19,
+    hardLight = 20,
+    softLight = 21,
+    difference = 22,
+    exclusion = 23,
+    multiply = 24,
+    hue = 25,
+    saturation = 26,
+    color = 27,
+    luminosity = 28
+};
 using ColorInt = uint32_t;
 
 enum class PathVerb : uint8_t { move = 0, line = 1, quad = 2, cubic = 4, close = 5 };

@@ -56,7 +56,7 @@ class StreamBuilder:
         return len(self.paths) - 1
 
     def draw(self, path, color=0xff000000, matrix=IDENTITY, stroke=None, join=MITER, cap=BUTT,
-             gradient=None, clips=(), feather=0.0):
+             gradient=None, clips=(), feather=0.0, blend=3):
         """stroke: width or None.  gradient: ("linear", sx, sy, ex, ey, stops)
         or ("radial", cx, cy, r, stops) with stops [(offset, ARGB), ...]."""
         d = np.zeros((), _DRAW)
@@ -64,7 +64,7 @@ class StreamBuilder:
         d["matrix"] = matrix
         d["style"] = STROKE if stroke is not None else FILL
         d["thickness"] = stroke if stroke is not None else 0.0
-        d["join"], d["cap"], d["feather"], d["blend_mode"] = join, cap, feather, 3
+        d["join"], d["cap"], d["feather"], d["blend_mode"] = join, cap, feather, blend
         d["color"] = color
         if gradient is not None:
             kind, *geo, stops = gradient

@@ -104,10 +104,23 @@ def sc_gradients():
     return b
 
 
+def sc_blend_modes():
+    b = StreamBuilder(100, 66, dither=1)
+    b.draw(b.path(rect(0, 0, 100, 66)),
+           gradient=("linear", 0, 0, 100, 66, [(0, 0xff2040a0), (.5, 0xa0e08040), (1, 0x60f0c020)]))
+    for k, mode in enumerate(range(14, 29)):
+        x, y = (k % 5) * 20 + 10, (k // 5) * 22 + 11
+        b.draw(b.path(star(x, y, 4, 10, 6)), color=0xc0000000 | (k * 0x10a3c7 & 0xffffff), blend=mode)
+    b.draw(b.path(rect(5, 50, 95, 64)), blend=24,
+           gradient=("radial", 50, 57, 40, [(0, 0xffffffff), (1, 0x80ff0000)]))
+    return b
+
+
 SCENARIOS = {
     "empty_frame": sc_empty_frame, "culled_draws": sc_culled_draws, "one_pixel": sc_one_pixel,
     "offcanvas_and_huge": sc_offcanvas_and_huge, "degenerate_cubics": sc_degenerate_cubics,
     "clockwise_mirrored": sc_clockwise_mirrored, "clips": sc_clips, "gradients": sc_gradients,
+    "blend_modes": sc_blend_modes,
 }
 def build_join_cap(join, cap):
     b = StreamBuilder(64, 48, dither=1)
