@@ -132,7 +132,12 @@ typedef struct rive_hip_draw
      * GEOMETRY mode: written by the device. */
     uint32_t tess_location;
     uint32_t tess_vertex_count;
-    uint32_t pad[2];
+    /* Image paints (the DrawBatch's imageTexture and ImageSampler): 0 = none,
+     * k = rive_hip_flush_desc.images[k - 1].  The sampler is
+     * ImageSampler::asKey() (image_sampler.hpp:56-61): wrapX + 3*wrapY +
+     * 9*filter. */
+    uint32_t image_index;
+    uint32_t image_sampler;
 } rive_hip_draw;
 
 /* ------------------------------------------------------------------------ */
@@ -141,6 +146,7 @@ typedef struct rive_hip_draw
 
 typedef struct rive_hip_context rive_hip_context;
 typedef struct rive_hip_target rive_hip_target;
+typedef struct rive_hip_image rive_hip_image;
 
 /* Message for the last error on this thread (never NULL). */
 const char* rive_hip_last_error(void);
@@ -192,6 +198,15 @@ int rive_hip_target_create(rive_hip_context* ctx, uint32_t width, uint32_t heigh
                            rive_hip_target** out);
 void rive_hip_target_destroy(rive_hip_target* target);
 
+/* Replaces RenderContextImpl::makeImageTexture (render_context_impl.hpp:61-70):
+ * a premultiplied RGBA8 image in HBM.  mip_levels = 0 builds the full chain;
+ * levels after the first are built on the device with a 2x2 box filter (the
+ * reference generates them on the GPU too, render_context_impl.hpp:43-70). */
+int rive_hip_image_create(rive_hip_context* ctx, uint32_t width, uint32_t height,
+                          uint32_t mip_levels, const uint8_t* rgba_premul,
+                          rive_hip_image** out);
+void rive_hip_image_destroy(rive_hip_image* image);
+
 enum rive_hip_flush_mode
 {
     RIVE_HIP_FLUSH_SPANS = 0,
@@ -216,6 +231,8 @@ typedef struct rive_hip_flush_desc
     uint32_t grad_simple_rows;   /* rows of simple (2-texel) ramps */
     uint32_t grad_complex_rows;  /* rows of complex ramps, after the simple rows */
     uint32_t first_path_tess_location; /* GEOMETRY mode: usually 8 */
+    rive_hip_image* const* images;     /* referenced by rive_hip_draw.image_index */
+    uint32_t image_count;
 } rive_hip_flush_desc;
 
 /* Replaces RenderContextImpl::flush (render_context_impl.hpp:217): ramps,
@@ -294,7 +311,19 @@ typedef struct rive_stream_draw
     float grad[4];          /* linear: sx, sy, ex, ey; radial: cx, cy, r, 0 */
     uint32_t clip_count;    /* 0..2 */
     uint32_t clip_paths[2];
+    /* drawImage instead of drawPath when image != 0: images[image - 1] under
+     * `matrix`, with `blend_mode`, ImageSampler key `image_sampler` and
+     * `opacity` (renderer.hpp drawImage). */
+    uint32_t image;
+    uint32_t image_sampler;
+    float opacity;
 } rive_stream_draw;
+
+typedef struct rive_stream_image
+{
+    uint32_t width, height;
+    const uint8_t* rgba_premul; /* width * height * 4 bytes */
+} rive_stream_image;
 
 typedef struct rive_stream
 {
@@ -312,6 +341,8 @@ typedef struct rive_stream
     const uint32_t* stop_colors;
     const float* stop_offsets;
     uint32_t stop_count;
+    const rive_stream_image* images;
+    uint32_t image_count;
 } rive_stream;
 
 /* Seeded synthetic streams (std::mt19937_64 with the float/int mappings of

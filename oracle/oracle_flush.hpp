@@ -18,9 +18,24 @@ struct DrawInfo
     bool isStroke;
 };
 
+// An image with its mip chain (2x2 box filter with rounding), premultiplied
+// RGBA8 texels packed level after level.
+struct OracleImage
+{
+    uint32_t width = 0, height = 0, levels = 0;
+    std::vector<uint32_t> levelOffset;
+    std::vector<uint32_t> texels;
+};
+void build_image_mips(uint32_t width, uint32_t height, const uint8_t* rgba, OracleImage* out);
+// Vulkan-style sampling with the reference's sampler state (sampler key =
+// wrapX + 3*wrapY + 9*filter; mip mode nearest).
+void sample_image(const OracleImage& img, float u, float v, float lod, uint32_t sampler, float out[4]);
+
 struct Flush
 {
     uint32_t width = 0, height = 0, clearColor = 0, dither = 0;
+    std::vector<OracleImage> images;
+    std::vector<uint32_t> pathImage; // per path: image index (1-based, 0 none) | sampler << 16
     std::vector<rive_hip_path_data> pathData; // [0] reserved
     std::vector<rive_hip_paint_data> paintData;
     std::vector<rive_hip_paint_aux> paintAux;

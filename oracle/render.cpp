@@ -267,11 +267,21 @@ struct Builder
                       uint32_t blendMode = BLEND_SRC_OVER)
     {
         const rive_stream_path& sp = s.paths[pathIdx];
+        pushGeometryDraw(s.verbs + sp.first_verb, sp.verb_count, pathPts(pathIdx), sp.point_count, matrix,
+                         strokeRadius, join, cap, fillRule, paintType, paintValue, clipID, outerClipID, grad,
+                         blendMode, 0, 0);
+    }
+
+    void pushGeometryDraw(const uint8_t* verbs, uint32_t verbCount, const V2* pts, uint32_t ptCount,
+                          const Mat& matrix, float strokeRadius, Join join, Cap cap, FillRule fillRule,
+                          uint32_t paintType, uint32_t paintValue, uint32_t clipID, uint32_t outerClipID,
+                          const Gradient* grad, uint32_t blendMode, uint32_t image, uint32_t sampler)
+    {
         PathDrawDesc d;
-        d.verbs = s.verbs + sp.first_verb;
-        d.verbCount = sp.verb_count;
-        d.pts = pathPts(pathIdx);
-        d.ptCount = sp.point_count;
+        d.verbs = verbs;
+        d.verbCount = verbCount;
+        d.pts = pts;
+        d.ptCount = ptCount;
         d.matrix = matrix;
         d.strokeRadius = strokeRadius;
         d.join = join;
@@ -311,10 +321,26 @@ struct Builder
             else if (fillRule == FillRule::evenOdd)
                 params |= PAINT_FLAG_EVEN_ODD_FILL;
         }
+        rive_hip_paint_aux aux{};
+        f.pathImage.resize(pathID + 1, 0);
+        if (image != 0)
+        {
+            // PAINT_FLAG_HAS_IMAGE; PaintAuxData image branch (gpu.cpp:1006-1034).
+            params |= 0x800u;
+            const OracleImage& img = f.images[image - 1];
+            Mat inv;
+            if (!invert(matrix, &inv))
+                inv = Mat();
+            memcpy(aux.image_matrix, inv.m, sizeof(aux.image_matrix));
+            float dudx = inv.m[0] * (float)img.width, dudy = inv.m[2] * (float)img.height;
+            float dvdx = inv.m[1] * (float)img.width, dvdy = inv.m[3] * (float)img.height;
+            float maxScaleFactorPow2 = std::max(dudx * dudx + dvdx * dvdx, dudy * dudy + dvdy * dvdy);
+            aux.image_lod = log2f(std::max(maxScaleFactorPow2, 1.f)) * .5f + -.5f; // MIP_MAP_LOD_BIAS
+            f.pathImage[pathID] = image | sampler << 16;
+        }
         pp.params = params;
         f.paintData.push_back(pp);
 
-        rive_hip_paint_aux aux{};
         // clip rect: WideOpen (no clip rect) -> identity-ish, inverseFwidth 0
         if (grad != nullptr)
         {
@@ -480,8 +506,40 @@ struct Builder
         f.gradHeight = height;
     }
 
+    // applyClip (rive_renderer.cpp:631-824): pushes the clip-update draws the
+    // stack still needs and returns the clip ID the draw reads.
+    uint32_t applyClip(size_t height, const IAABB& frame)
+    {
+        if (height == 0)
+            return 0;
+        size_t inBuffer = (size_t)-1;
+        if (clipContentID != 0)
+            for (size_t i = height - 1; i != (size_t)-1; --i)
+                if (clipStack[i].clipID == clipContentID)
+                {
+                    inBuffer = i;
+                    break;
+                }
+        uint32_t parentClipID = inBuffer == (size_t)-1 ? 0 : clipStack[inBuffer].clipID;
+        for (size_t i = inBuffer + 1; i < height; ++i)
+        {
+            ClipElement& ce = clipStack[i];
+            ce.clipID = nextClipID++;
+            IAABB cb = ce.pixelBounds;
+            if (!cb.intersect(frame).empty())
+                pushPathDraw(ce.path, ce.matrix, 0, Join::miter, Cap::butt, ce.fillRule,
+                             CLIP_UPDATE_PAINT_TYPE, 0, ce.clipID, parentClipID, nullptr);
+            parentClipID = ce.clipID;
+        }
+        clipContentID = parentClipID;
+        return parentClipID;
+    }
+
     void build()
     {
+        f.images.resize(s.image_count);
+        for (uint32_t i = 0; i < s.image_count; ++i)
+            build_image_mips(s.images[i].width, s.images[i].height, s.images[i].rgba_premul, &f.images[i]);
         f.width = s.width;
         f.height = s.height;
         f.clearColor = s.clear_color;
@@ -537,6 +595,30 @@ struct Builder
             if (clippedOut || overall.empty())
                 continue;
 
+            // drawImage (rive_renderer.cpp:383-446): the unit rect under
+            // scale(width, height) with an image paint.
+            static const uint8_t kUnitVerbs[4] = {kMove, kLine, kLine, kLine};
+            static const V2 kUnitPts[4] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+            if (d.image != 0)
+            {
+                if (d.image > f.images.size())
+                    continue;
+                const OracleImage& img = f.images[d.image - 1];
+                Mat scale;
+                scale.m[0] = (float)img.width;
+                scale.m[3] = (float)img.height;
+                Mat im = mul(m, scale);
+                IAABB pixelBounds = map_bounds_round_out(im, kUnitPts, 4, 0, 0);
+                if (pixelBounds.intersect(frame).empty() || pixelBounds.intersect(overall).empty())
+                    continue;
+                uint32_t drawClipID = applyClip(height, frame);
+                float op = std::max(0.f, std::min(1.f, std::max(0.f, d.opacity)));
+                uint32_t color = ((uint32_t)std::lround(255.f * op) << 24) | 0x00ffffffu;
+                pushGeometryDraw(kUnitVerbs, 4, kUnitPts, 4, im, 0, Join::miter, Cap::butt, FillRule::nonZero,
+                                 SOLID_COLOR_PAINT_TYPE, swizzle_rgba(color), drawClipID, 0, nullptr,
+                                 shader_blend_mode(d.blend_mode), d.image, d.image_sampler);
+                continue;
+            }
             // drawPath (rive_renderer.cpp:121-197)
             const rive_stream_path& sp = s.paths[d.path];
             if (sp.verb_count == 0)
@@ -577,31 +659,7 @@ struct Builder
             // applyClip (rive_renderer.cpp:631-824)
             if (pixelBounds.intersect(overall).empty())
                 continue;
-            uint32_t drawClipID = 0;
-            if (height > 0)
-            {
-                size_t inBuffer = (size_t)-1;
-                if (clipContentID != 0)
-                    for (size_t i = height - 1; i != (size_t)-1; --i)
-                        if (clipStack[i].clipID == clipContentID)
-                        {
-                            inBuffer = i;
-                            break;
-                        }
-                uint32_t parentClipID = inBuffer == (size_t)-1 ? 0 : clipStack[inBuffer].clipID;
-                for (size_t i = inBuffer + 1; i < height; ++i)
-                {
-                    ClipElement& ce = clipStack[i];
-                    ce.clipID = nextClipID++;
-                    IAABB cb = ce.pixelBounds;
-                    if (!cb.intersect(frame).empty())
-                        pushPathDraw(ce.path, ce.matrix, 0, Join::miter, Cap::butt, ce.fillRule,
-                                     CLIP_UPDATE_PAINT_TYPE, 0, ce.clipID, parentClipID, nullptr);
-                    parentClipID = ce.clipID;
-                }
-                drawClipID = parentClipID;
-                clipContentID = parentClipID;
-            }
+            uint32_t drawClipID = applyClip(height, frame);
             Gradient grad;
             const Gradient* gp = nullptr;
             uint32_t paintType = SOLID_COLOR_PAINT_TYPE;
@@ -627,6 +685,7 @@ struct Builder
         }
         finishGradients();
         f.pathCount = (uint32_t)f.pathData.size() - 1;
+        f.pathImage.resize(f.pathCount + 1, 0);
         f.tessEnd = tessLocation;
         // Padding spans (render_context.cpp:1550-1568): 8 at 0 and 1 at the end.
         std::vector<TessVertexSpan> padding;
@@ -984,6 +1043,108 @@ inline uint32_t to_unorm8(float v)
 }
 } // namespace
 
+void build_image_mips(uint32_t width, uint32_t height, const uint8_t* rgba, OracleImage* out)
+{
+    // Full mip chain (the reference builds it on the GPU, render_context_impl.hpp:43-70),
+    // each texel the rounded mean of its 2x2 footprint, clamped at odd edges.
+    uint32_t levels = 1;
+    while ((std::max(width, height) >> levels) != 0)
+        ++levels;
+    levels = std::min<uint32_t>(levels, 15);
+    out->width = width;
+    out->height = height;
+    out->levels = levels;
+    out->levelOffset.assign(levels + 1, 0);
+    uint32_t total = 0;
+    for (uint32_t l = 0; l < levels; ++l)
+    {
+        out->levelOffset[l] = total;
+        total += std::max(width >> l, 1u) * std::max(height >> l, 1u);
+    }
+    out->levelOffset[levels] = total;
+    out->texels.assign(total, 0);
+    memcpy(out->texels.data(), rgba, (size_t)width * height * 4);
+    for (uint32_t l = 0; l + 1 < levels; ++l)
+    {
+        const int sw = (int)std::max(width >> l, 1u), sh = (int)std::max(height >> l, 1u);
+        const int dw = (int)std::max(width >> (l + 1), 1u), dh = (int)std::max(height >> (l + 1), 1u);
+        const uint32_t* src = out->texels.data() + out->levelOffset[l];
+        uint32_t* dst = out->texels.data() + out->levelOffset[l + 1];
+        for (int y = 0; y < dh; ++y)
+            for (int x = 0; x < dw; ++x)
+            {
+                const int x0 = std::min(2 * x, sw - 1), x1 = std::min(2 * x + 1, sw - 1);
+                const int y0 = std::min(2 * y, sh - 1), y1 = std::min(2 * y + 1, sh - 1);
+                const uint32_t t[4] = {src[y0 * sw + x0], src[y0 * sw + x1], src[y1 * sw + x0], src[y1 * sw + x1]};
+                uint32_t px = 0;
+                for (int k = 0; k < 4; ++k)
+                {
+                    uint32_t sum = 0;
+                    for (uint32_t v : t)
+                        sum += (v >> (8 * k)) & 0xffu;
+                    px |= ((sum + 2u) >> 2) << (8 * k);
+                }
+                dst[y * dw + x] = px;
+            }
+    }
+}
+
+namespace
+{
+int wrap_coord(int i, int n, uint32_t mode)
+{
+    if (mode == 1) // repeat
+    {
+        int m = i % n;
+        return m < 0 ? m + n : m;
+    }
+    if (mode == 2) // mirrored repeat
+    {
+        int m = i % (2 * n);
+        if (m < 0)
+            m += 2 * n;
+        return m < n ? m : 2 * n - 1 - m;
+    }
+    return std::min(std::max(i, 0), n - 1);
+}
+} // namespace
+
+void sample_image(const OracleImage& img, float u, float v, float lod, uint32_t sampler, float out[4])
+{
+    // VkSamplerCreateInfo of pipeline_manager_vulkan.cpp:95-110: min/mag filter
+    // per ImageFilter, mipmapMode NEAREST (level = ceil(lod + .5) - 1), address
+    // mode per ImageWrap.
+    const uint32_t wrapX = sampler % 3, wrapY = (sampler / 3) % 3, filter = sampler / 9;
+    int level = 0;
+    if (lod > 0.f)
+        level = std::min((int)ceilf(lod + .5f) - 1, (int)img.levels - 1);
+    level = std::max(level, 0);
+    const int w = std::max((int)img.width >> level, 1), h = std::max((int)img.height >> level, 1);
+    const uint32_t* tex = img.texels.data() + img.levelOffset[level];
+    auto unorm = [](uint32_t t, int k) { return float((t >> (8 * k)) & 0xffu) / 255.f; };
+    if (filter == 1)
+    {
+        const int i = wrap_coord((int)floorf(u * (float)w), w, wrapX);
+        const int j = wrap_coord((int)floorf(v * (float)h), h, wrapY);
+        for (int k = 0; k < 4; ++k)
+            out[k] = unorm(tex[j * w + i], k);
+        return;
+    }
+    const float x = u * (float)w - .5f, y = v * (float)h - .5f;
+    const float fx0 = floorf(x), fy0 = floorf(y);
+    const float ax = x - fx0, ay = y - fy0;
+    const int i0 = wrap_coord((int)fx0, w, wrapX), i1 = wrap_coord((int)fx0 + 1, w, wrapX);
+    const int j0 = wrap_coord((int)fy0, h, wrapY), j1 = wrap_coord((int)fy0 + 1, h, wrapY);
+    for (int k = 0; k < 4; ++k)
+    {
+        const float c00 = unorm(tex[j0 * w + i0], k), c10 = unorm(tex[j0 * w + i1], k);
+        const float c01 = unorm(tex[j1 * w + i0], k), c11 = unorm(tex[j1 * w + i1], k);
+        const float top = (c10 - c00) * ax + c00;
+        const float bot = (c11 - c01) * ax + c01;
+        out[k] = (bot - top) * ay + top;
+    }
+}
+
 // Gradient paint color, unpremultiplied (draw_path.vert:285-331,450-476): the
 // paint matrix maps the pixel to t, the ramp row is sampled with linear
 // filtering at texel centers.
@@ -1143,6 +1304,23 @@ void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* 
             const uint32_t blendMode = (paint.params >> 4) & 0xfu;
             uint32_t dst = rgba[p];
             float color[4];
+            // Image paints (draw_path.vert:479-493): the paint color times the
+            // image sample; unmultiplied under advanced blend.
+            float imageColor[4] = {1.f, 1.f, 1.f, 1.f};
+            const uint32_t pimg = f.pathImage[pathID];
+            if ((pimg & 0xffffu) != 0)
+            {
+                const float fx = float(x) + .5f, fy = float(y) + .5f;
+                const float u = aux.image_matrix[0] * fx + aux.image_matrix[2] * fy + aux.image_matrix[4];
+                const float v = aux.image_matrix[1] * fx + aux.image_matrix[3] * fy + aux.image_matrix[5];
+                sample_image(f.images[(pimg & 0xffffu) - 1], u, v, aux.image_lod, pimg >> 16, imageColor);
+                if (blendMode != BLEND_SRC_OVER)
+                {
+                    const float inv = imageColor[3] != 0.f ? 1.f / imageColor[3] : 0.f;
+                    for (int k = 0; k < 3; ++k)
+                        imageColor[k] *= inv;
+                }
+            }
             if (blendMode != BLEND_SRC_OVER)
             {
                 // draw_raster_order_path.frag:191-204: the paint stays
@@ -1159,6 +1337,8 @@ void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* 
                     sample_gradient(f, aux, paint, ramps, x, y, src);
                 }
                 src[3] *= coverage;
+                for (int k = 0; k < 4; ++k)
+                    src[k] *= imageColor[k];
                 const float dp[4] = {unorm8(dst, 0), unorm8(dst, 1), unorm8(dst, 2), unorm8(dst, 3)};
                 float coeffs[3];
                 advanced_blend_coeffs(src, dp, blendMode, coeffs);
@@ -1177,6 +1357,8 @@ void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* 
                 color[3] = a;
                 for (int k = 0; k < 4; ++k)
                     color[k] *= coverage;
+                for (int k = 0; k < 4; ++k)
+                    color[k] *= imageColor[k];
             }
             else
             {

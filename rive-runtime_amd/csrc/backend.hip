@@ -94,6 +94,9 @@ struct rive_hip_context
     DevBuf geoCounts, geoRecords, geoContours, geoSpans, geoSpanCounts, geoSpanOffsets;
     uint32_t lastMode = 0, lastRecordCount = 0, lastTessVertexCount = 0;
     uint32_t* hostScalars = nullptr; // pinned
+    DevBuf imageTable, pathImage;
+    ImageDesc* hostImages = nullptr; // pinned staging of the image table
+    uint32_t hostImagesCap = 0;
     hipEvent_t ev[kEvents] = {};
     float passMs[6] = {};
     bool timed = false;
@@ -151,12 +154,14 @@ void rive_hip_destroy(rive_hip_context* ctx)
                      &ctx->cellCursor, &ctx->cellBackdrop, &ctx->binned, &ctx->pathRaster,
                      &ctx->ramps, &ctx->drawTess, &ctx->geoCounts, &ctx->geoRecords,
                      &ctx->geoContours, &ctx->geoSpans, &ctx->geoSpanCounts, &ctx->geoSpanOffsets,
-                     &ctx->verts};
+                     &ctx->verts, &ctx->imageTable, &ctx->pathImage};
     for (DevBuf* b : all)
         if (b->ptr)
             (void)hipFree(b->ptr);
     if (ctx->hostScalars)
         (void)hipHostFree(ctx->hostScalars);
+    if (ctx->hostImages)
+        (void)hipHostFree(ctx->hostImages);
     for (auto& e : ctx->ev)
         if (e)
             (void)hipEventDestroy(e);
@@ -254,6 +259,72 @@ void rive_hip_target_destroy(rive_hip_target* t)
     (void)hipStreamSynchronize(t->ctx->stream);
     (void)hipFree(t->pixels);
     delete t;
+}
+
+struct rive_hip_image
+{
+    rive_hip_context* ctx;
+    rivehip::ImageDesc desc;
+};
+
+int rive_hip_image_create(rive_hip_context* ctx, uint32_t width, uint32_t height, uint32_t mipLevels,
+                          const uint8_t* rgba, rive_hip_image** out)
+{
+    using namespace rivehip;
+    if (ctx == nullptr || out == nullptr || rgba == nullptr || width == 0 || height == 0 || width > 16384 ||
+        height > 16384)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "bad image arguments");
+    uint32_t full = 1;
+    while ((std::max(width, height) >> full) != 0)
+        ++full;
+    uint32_t levels = mipLevels == 0 ? full : std::min(mipLevels, full);
+    levels = std::min<uint32_t>(levels, kMaxImageLevels);
+    ImageDesc desc{};
+    desc.width = width;
+    desc.height = height;
+    desc.levels = levels;
+    uint32_t total = 0;
+    for (uint32_t l = 0; l < levels; ++l)
+    {
+        desc.levelOffset[l] = total;
+        total += std::max(width >> l, 1u) * std::max(height >> l, 1u);
+    }
+    desc.levelOffset[levels] = total;
+    HIP_TRY(hipSetDevice(ctx->device));
+    uint32_t* texels = nullptr;
+    if (hipMalloc(&texels, (size_t)total * 4) != hipSuccess)
+        return fail(RIVE_HIP_ERROR_OUT_OF_MEMORY, "image allocation failed");
+    desc.texels = texels;
+    auto* img = new rive_hip_image{ctx, desc};
+    hipError_t e = hipMemcpyAsync(texels, rgba, (size_t)width * height * 4, hipMemcpyHostToDevice, ctx->stream);
+    for (uint32_t l = 0; l + 1 < levels && e == hipSuccess; ++l)
+    {
+        const int sw = (int)std::max(width >> l, 1u), sh = (int)std::max(height >> l, 1u);
+        const int dw = (int)std::max(width >> (l + 1), 1u), dh = (int)std::max(height >> (l + 1), 1u);
+        mip_kernel<<<dim3((dw + 63) / 64, dh), 64, 0, ctx->stream>>>(texels, desc.levelOffset[l], sw, sh,
+                                                                      desc.levelOffset[l + 1], dw, dh);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess)
+    {
+        (void)hipFree(texels);
+        delete img;
+        return fail(RIVE_HIP_ERROR_DEVICE, hipGetErrorString(e));
+    }
+    *out = img;
+    return RIVE_HIP_OK;
+}
+
+void rive_hip_image_destroy(rive_hip_image* img)
+{
+    if (img == nullptr)
+        return;
+    (void)hipSetDevice(img->ctx->device);
+    (void)hipStreamSynchronize(img->ctx->stream);
+    (void)hipFree(const_cast<uint32_t*>(img->desc.texels));
+    delete img;
 }
 
 int rive_hip_sync(rive_hip_context* ctx)
@@ -424,7 +495,7 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
             P<uint32_t>(ctx->cellCursor), P<uint32_t>(ctx->binned));
     HIP_TRY(ensure(ctx->pathRaster, (size_t)(pathCount + 1) * sizeof(PathRaster)));
     path_raster_kernel<<<(pathCount + 1 + 255) / 256, 256, 0, ctx->stream>>>(
-        paths, paints, aux, pathCount, gradRows, P<PathRaster>(ctx->pathRaster));
+        paths, paints, aux, P<uint32_t>(ctx->pathImage), pathCount, gradRows, P<PathRaster>(ctx->pathRaster));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
 
@@ -432,7 +503,7 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     raster_kernel<<<tilesX * tilesY, kRasterLanes, 0, ctx->stream>>>(
         P<PathCells>(ctx->cells), P<PathRaster>(ctx->pathRaster), pathCount,
         P<uint32_t>(ctx->cellOffset), P<int32_t>(ctx->cellBackdrop), P<uint32_t>(ctx->binned),
-        P<Prim>(ctx->prims), P<uint32_t>(ctx->ramps), gradRows, W, H, tilesX,
+        P<Prim>(ctx->prims), P<uint32_t>(ctx->ramps), gradRows, P<ImageDesc>(ctx->imageTable), W, H, tilesX,
         d->load_action == 1 ? 1u : 0u, premul_clear(d->clear_color), d->dither, tgt->pixels);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
@@ -441,7 +512,8 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
 
 // Single-block prefix over draws: forward ranges and primitive bases.
 __global__ void draw_tess_kernel(const rive_hip_draw* __restrict__ draws, uint32_t n,
-                                 uint32_t pathCount, uint32_t texels, DrawTess* __restrict__ out,
+                                 uint32_t pathCount, uint32_t texels, uint32_t imageCount,
+                                 DrawTess* __restrict__ out, uint32_t* __restrict__ pathImage,
                                  uint32_t* __restrict__ totalPrims, uint32_t* __restrict__ unsupported)
 {
     __shared__ uint32_t buf[1024];
@@ -494,6 +566,13 @@ __global__ void draw_tess_kernel(const rive_hip_draw* __restrict__ draws, uint32
                 atomicAdd(unsupported, 1u);
                 t.forwardCount = 0;
                 t.pathID = 0;
+            }
+            else if (d.image_index != 0u)
+            {
+                if (d.image_index > imageCount || d.image_sampler >= 18u)
+                    atomicAdd(unsupported, 1u);
+                else
+                    pathImage[d.path_id] = d.image_index | (d.image_sampler << 16);
             }
             cnt = t.forwardCount;
         }
@@ -570,9 +649,36 @@ extern "C" int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* 
         return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "unknown flush mode");
 
     uint32_t* scal = P<uint32_t>(ctx->scalars);
+    // The flush's image table (the DrawBatch image textures).
+    if (d->image_count > 0)
+    {
+        if (d->images == nullptr)
+            return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "image_count without images");
+        if (ctx->hostImagesCap < d->image_count)
+        {
+            if (ctx->hostImages)
+                HIP_TRY(hipHostFree(ctx->hostImages));
+            ctx->hostImages = nullptr;
+            HIP_TRY(hipHostMalloc((void**)&ctx->hostImages, sizeof(ImageDesc) * d->image_count,
+                                  hipHostMallocDefault));
+            ctx->hostImagesCap = d->image_count;
+        }
+        for (uint32_t k = 0; k < d->image_count; ++k)
+        {
+            if (d->images[k] == nullptr || d->images[k]->ctx != ctx)
+                return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null or foreign image in the image table");
+            ctx->hostImages[k] = d->images[k]->desc;
+        }
+        HIP_TRY(ensure(ctx->imageTable, sizeof(ImageDesc) * d->image_count));
+        HIP_TRY(hipMemcpyAsync(ctx->imageTable.ptr, ctx->hostImages, sizeof(ImageDesc) * d->image_count,
+                               hipMemcpyHostToDevice, ctx->stream));
+    }
+    HIP_TRY(ensure(ctx->pathImage, (size_t)(d->path_count + 1) * 4));
+    HIP_TRY(hipMemsetAsync(ctx->pathImage.ptr, 0, (size_t)(d->path_count + 1) * 4, ctx->stream));
     draw_tess_kernel<<<1, 1024, 0, ctx->stream>>>(
         reinterpret_cast<const rive_hip_draw*>(ctx->bufs[RIVE_HIP_BUFFER_DRAW].dev), d->draw_count,
-        d->path_count, tessRows * kTessTextureWidth, P<DrawTess>(ctx->drawTess), scal + 3, scal + 4);
+        d->path_count, tessRows * kTessTextureWidth, d->image_count, P<DrawTess>(ctx->drawTess),
+        P<uint32_t>(ctx->pathImage), scal + 3, scal + 4);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(ctx->hostScalars, scal + 3, 8, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));

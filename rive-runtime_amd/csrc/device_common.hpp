@@ -82,6 +82,22 @@ struct PathRaster
     float paintMatrix[6];
     float gradSpan;      // > 1: complex row; else simple-ramp texel offset
     uint32_t gradRow;
+    uint32_t image;      // 0 none, else 1-based index into the flush's image table
+    uint32_t imageSampler; // ImageSampler key: wrapX + 3*wrapY + 9*filter
+    float imageMatrix[6];  // pixel -> unit uv (PaintAuxData image matrix)
+    float imageLod;
+    uint32_t pad;
+};
+static_assert(sizeof(PathRaster) == 104, "PathRaster layout");
+
+// One image of the flush's image table: RGBA8 premultiplied, mip levels
+// packed one after another.
+constexpr int kMaxImageLevels = 15;
+struct ImageDesc
+{
+    const uint32_t* texels;
+    uint32_t width, height, levels, pad;
+    uint32_t levelOffset[kMaxImageLevels + 1];
 };
 
 // Per-path tile-cell grid (bounding box in tiles).

@@ -1225,6 +1225,90 @@ __device__ __forceinline__ float fill_rule_coverage(float count, uint32_t flags)
     return fminf(coverage, 1.f);
 }
 
+// Image sampling with the reference's sampler state (pipeline_manager_vulkan.cpp:
+// 11-46, 95-110): min/mag filter nearest or linear, mip mode nearest (level
+// ceil(lod + 0.5) - 1, clamped), wrap clamp-to-edge / repeat / mirrored
+// repeat per axis.  Texels are premultiplied RGBA8.
+__device__ __forceinline__ int wrap_coord(int i, int n, uint32_t mode)
+{
+    if (mode == 1u) // repeat
+    {
+        const int m = i % n;
+        return m < 0 ? m + n : m;
+    }
+    if (mode == 2u) // mirrored repeat
+    {
+        int m = i % (2 * n);
+        if (m < 0)
+            m += 2 * n;
+        return m < n ? m : 2 * n - 1 - m;
+    }
+    return min(max(i, 0), n - 1);
+}
+__device__ void sample_image(const ImageDesc& img, float u, float v, float lod, uint32_t sampler, float* out)
+{
+    const uint32_t wrapX = sampler % 3u, wrapY = (sampler / 3u) % 3u, filter = sampler / 9u;
+    int level = 0;
+    if (lod > 0.f)
+        level = min((int)ceilf(lod + .5f) - 1, (int)img.levels - 1);
+    level = max(level, 0);
+    const int w = max((int)img.width >> level, 1), h = max((int)img.height >> level, 1);
+    const uint32_t* tex = img.texels + img.levelOffset[level];
+    if (filter == 1u) // nearest
+    {
+        const int i = wrap_coord((int)floorf(u * (float)w), w, wrapX);
+        const int j = wrap_coord((int)floorf(v * (float)h), h, wrapY);
+        const uint32_t t = tex[(size_t)j * w + i];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            out[k] = float((t >> (8 * k)) & 0xffu) * (1.f / 255.f);
+        return;
+    }
+    const float x = u * (float)w - .5f, y = v * (float)h - .5f;
+    const float fx0 = floorf(x), fy0 = floorf(y);
+    const float ax = x - fx0, ay = y - fy0;
+    const int i0 = wrap_coord((int)fx0, w, wrapX), i1 = wrap_coord((int)fx0 + 1, w, wrapX);
+    const int j0 = wrap_coord((int)fy0, h, wrapY), j1 = wrap_coord((int)fy0 + 1, h, wrapY);
+    const uint32_t t00 = tex[(size_t)j0 * w + i0], t10 = tex[(size_t)j0 * w + i1];
+    const uint32_t t01 = tex[(size_t)j1 * w + i0], t11 = tex[(size_t)j1 * w + i1];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        const float c00 = float((t00 >> (8 * k)) & 0xffu) * (1.f / 255.f);
+        const float c10 = float((t10 >> (8 * k)) & 0xffu) * (1.f / 255.f);
+        const float c01 = float((t01 >> (8 * k)) & 0xffu) * (1.f / 255.f);
+        const float c11 = float((t11 >> (8 * k)) & 0xffu) * (1.f / 255.f);
+        const float top = (c10 - c00) * ax + c00;
+        const float bot = (c11 - c01) * ax + c01;
+        out[k] = (bot - top) * ay + top;
+    }
+}
+
+// One texel of mip level L+1 from level L: the rounded mean of its 2x2
+// footprint, clamped at odd edges.
+__global__ void mip_kernel(uint32_t* __restrict__ texels, uint32_t srcOffset, int sw, int sh, uint32_t dstOffset,
+                           int dw, int dh)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= dw || y >= dh)
+        return;
+    const uint32_t* src = texels + srcOffset;
+    const int x0 = min(2 * x, sw - 1), x1 = min(2 * x + 1, sw - 1);
+    const int y0 = min(2 * y, sh - 1), y1 = min(2 * y + 1, sh - 1);
+    const uint32_t a = src[(size_t)y0 * sw + x0], b = src[(size_t)y0 * sw + x1];
+    const uint32_t c = src[(size_t)y1 * sw + x0], d = src[(size_t)y1 * sw + x1];
+    uint32_t out = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        const uint32_t sum = ((a >> (8 * k)) & 0xffu) + ((b >> (8 * k)) & 0xffu) + ((c >> (8 * k)) & 0xffu) +
+                             ((d >> (8 * k)) & 0xffu);
+        out |= ((sum + 2u) >> 2) << (8 * k);
+    }
+    texels[dstOffset + (size_t)y * dw + x] = out;
+}
+
 // Advanced blend coefficients from unpremultiplied src and premultiplied dst
 // (advanced_blend.glsl:87-280; unmultiply_rgb, common.glsl:210-216), fp32.
 constexpr float kEpsFp16NonDenorm = 6.2e-5f;
@@ -1358,7 +1442,8 @@ __device__ void advanced_blend_coeffs_d(const float* srcIn, const float* dp, uin
 // unchanged, so callers skip it (except for clip updates).
 __device__ __forceinline__ void resolve_pixel(const PathRaster& pr, float coverage, int px, int py, float noise,
                                               const uint32_t* __restrict__ ramps, uint32_t gradHeight,
-                                              float* col, float& clipCov, uint32_t& clipID)
+                                              const ImageDesc* __restrict__ images, float* col, float& clipCov,
+                                              uint32_t& clipID)
 {
     if (pr.flags & 8u) // clip update
     {
@@ -1415,6 +1500,26 @@ __device__ __forceinline__ void resolve_pixel(const PathRaster& pr, float covera
             src[2] *= src[3];
         }
     }
+    if (pr.image != 0u)
+    {
+        // find_paint_color (draw_path.vert:479-493): modulate by the image
+        // sample; with advanced blend the sample is unmultiplied first.
+        const float fx = float(px) + .5f, fy = float(py) + .5f;
+        const float u = pr.imageMatrix[0] * fx + pr.imageMatrix[2] * fy + pr.imageMatrix[4];
+        const float v = pr.imageMatrix[1] * fx + pr.imageMatrix[3] * fy + pr.imageMatrix[5];
+        float ic[4];
+        sample_image(images[pr.image - 1u], u, v, pr.imageLod, pr.imageSampler, ic);
+        if (blendMode != 0u)
+        {
+            const float inv = ic[3] != 0.f ? 1.f / ic[3] : 0.f;
+            ic[0] *= inv;
+            ic[1] *= inv;
+            ic[2] *= inv;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            src[k] *= ic[k];
+    }
     if (blendMode != 0u)
     {
         // draw_raster_order_path.frag:191-204: mix the unpremultiplied paint
@@ -1451,8 +1556,9 @@ __global__ __launch_bounds__(kRasterLanes) void raster_kernel(
     const PathCells* __restrict__ cells, const PathRaster* __restrict__ raster, uint32_t pathCount,
     const uint32_t* __restrict__ cellOffset, const int32_t* __restrict__ cellBackdrop,
     const uint32_t* __restrict__ binned, const Prim* __restrict__ prims,
-    const uint32_t* __restrict__ ramps, uint32_t gradHeight, int width, int height, int tilesX,
-    uint32_t loadPreserve, uint32_t clearPremul, uint32_t dither, uint32_t* __restrict__ target)
+    const uint32_t* __restrict__ ramps, uint32_t gradHeight, const ImageDesc* __restrict__ images, int width,
+    int height, int tilesX, uint32_t loadPreserve, uint32_t clearPremul, uint32_t dither,
+    uint32_t* __restrict__ target)
 {
     __shared__ __align__(16) int32_t sDelta[kSlots][kTile * kTile];
     __shared__ __align__(16) int32_t sBorder[kSlots][kTile * kTile];
@@ -1673,8 +1779,8 @@ __global__ __launch_bounds__(kRasterLanes) void raster_kernel(
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
                     if (cov[q] != 0.f || clipUpdate)
-                        resolve_pixel(pr, cov[q], px0 + q, py, noise[q], ramps, gradHeight, col[q], clipCov[q],
-                                      clipID[q]);
+                        resolve_pixel(pr, cov[q], px0 + q, py, noise[q], ramps, gradHeight, images, col[q],
+                                      clipCov[q], clipID[q]);
             }
             if (nSlots > 0)
                 __syncthreads(); // the next batch clears the slots
@@ -1707,8 +1813,8 @@ __global__ __launch_bounds__(kRasterLanes) void raster_kernel(
 // Unpacks PathData/PaintData/PaintAux into PathRaster (gpu.cpp:859-1063,
 // draw_path.vert:431-545).
 __global__ void path_raster_kernel(const PathDataV* __restrict__ paths, const uint2* __restrict__ paints,
-                                   const float* __restrict__ aux, uint32_t pathCount,
-                                   uint32_t gradHeight, PathRaster* __restrict__ out)
+                                   const float* __restrict__ aux, const uint32_t* __restrict__ pathImage,
+                                   uint32_t pathCount, uint32_t gradHeight, PathRaster* __restrict__ out)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > pathCount)
@@ -1759,6 +1865,15 @@ __global__ void path_raster_kernel(const PathDataV* __restrict__ paths, const ui
     float span0 = ax[6], span1 = ax[7];
     r.gradSpan = span0 > .9f ? 2.f : span1;
     r.gradRow = (uint32_t)(u2f(paint.y) * (float)gradHeight);
+    // Image paints (gpu.cpp:1006-1034): image matrix and a constant LOD.
+    const uint32_t img = pathImage[i];
+    r.image = img & 0xffffu;
+    r.imageSampler = img >> 16;
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        r.imageMatrix[k] = ax[16 + k];
+    r.imageLod = ax[22];
+    r.pad = 0;
     out[i] = r;
 }
 

@@ -216,6 +216,10 @@ RiveRenderer::IAABB RiveRenderer::MapBounds(const Mat2D& m, const RawPath& raw, 
 
 RiveRenderer::RiveRenderer(gpu::RenderContext* ctx) : m_context(ctx)
 {
+    // The unit rect drawImage draws through (rive_renderer.cpp:428-434).
+    m_unitRectPath.lineTo(1, 0);
+    m_unitRectPath.lineTo(1, 1);
+    m_unitRectPath.lineTo(0, 1);
     RenderState s;
     s.overallClipPixelBounds = {0, 0, (int)ctx->frameDescriptor().renderTargetWidth,
                                 (int)ctx->frameDescriptor().renderTargetHeight};
@@ -229,6 +233,24 @@ void RiveRenderer::restore()
         m_stack.pop_back();
 }
 void RiveRenderer::transform(const Mat2D& m) { m_stack.back().matrix = m_stack.back().matrix * m; }
+
+void RiveRenderer::drawImage(const RenderImage* renderImage, ImageSampler sampler, BlendMode blendMode,
+                             float opacity)
+{
+    // rive_renderer.cpp:383-446: an image is the unit rect under
+    // scale(width, height) with an image paint.
+    auto* image = static_cast<const RiveRenderImage*>(renderImage);
+    if (image == nullptr || image->handle() == nullptr)
+        return;
+    save();
+    transform(Mat2D((float)image->width(), 0, 0, (float)image->height(), 0, 0));
+    RiveRenderPaint paint;
+    paint.image(image, std::max(0.f, opacity));
+    paint.blendMode(blendMode);
+    paint.imageSampler(sampler);
+    drawPath(&m_unitRectPath, &paint);
+    restore();
+}
 
 void RiveRenderer::clipPath(RenderPath* renderPath)
 {
@@ -408,6 +430,23 @@ void RenderContext::pushPathDraw(const RiveRenderPath* path, const Mat2D& m, con
             r.paint.value = swizzle_rgba(paint->getColor());
         }
         params |= clipID << 16 | shader_blend_mode(paint->getBlendMode()) << 4;
+        if (paint->getImage() != nullptr)
+        {
+            // PAINT_FLAG_HAS_IMAGE; PaintAuxData image branch (gpu.cpp:1006-1034):
+            // the inverse image matrix and a constant mip LOD.
+            params |= 0x800u;
+            r.image = paint->getImage();
+            r.draw.image_sampler = paint->getImageSampler().asKey();
+            Mat2D inv;
+            if (!m.invert(&inv))
+                inv = Mat2D();
+            memcpy(r.aux.image_matrix, inv.values(), sizeof(r.aux.image_matrix));
+            const float w = (float)r.image->width(), h = (float)r.image->height();
+            const float dudx = inv[0] * w, dudy = inv[2] * h;
+            const float dvdx = inv[1] * w, dvdy = inv[3] * h;
+            const float maxScaleFactorPow2 = std::max(dudx * dudx + dvdx * dvdx, dudy * dudy + dvdy * dvdy);
+            r.aux.image_lod = log2f(std::max(maxScaleFactorPow2, 1.f)) * .5f + -.5f; // MIP_MAP_LOD_BIAS
+        }
     }
     if (!stroke)
     {
@@ -536,6 +575,16 @@ bool RenderContext::flush(rive_hip_target* target)
             }
             pending.push_back(pg);
         }
+        if (r.image != nullptr)
+        {
+            auto it = std::find(fd.images.begin(), fd.images.end(), r.image->handle());
+            if (it == fd.images.end())
+            {
+                fd.images.push_back(r.image->handle());
+                it = fd.images.end() - 1;
+            }
+            r.draw.image_index = (uint32_t)(it - fd.images.begin()) + 1;
+        }
         fd.paths.push_back(r.path);
         fd.paints.push_back(r.paint);
         fd.paintAux.push_back(r.aux);
@@ -634,6 +683,8 @@ bool RenderContextHIPImpl::flush(const gpu::FlushDescriptor& fd)
     d.grad_simple_rows = fd.gradSimpleRows;
     d.grad_complex_rows = fd.gradComplexRows;
     d.first_path_tess_location = 8;
+    d.images = fd.images.data();
+    d.image_count = (uint32_t)fd.images.size();
     return rive_hip_flush(m_ctx, &d) == RIVE_HIP_OK;
 }
 } // namespace rive
@@ -646,6 +697,13 @@ struct StreamReplay
 {
     std::vector<rive::RiveRenderPath> paths;
     std::vector<rive::RiveRenderPaint> paints;
+    std::vector<rive_hip_image*> handles;
+    std::vector<rive::RiveRenderImage> images;
+    ~StreamReplay()
+    {
+        for (rive_hip_image* h : handles)
+            rive_hip_image_destroy(h);
+    }
 };
 } // namespace
 
@@ -691,6 +749,18 @@ extern "C" int rive_hip_render_stream(rive_hip_context* ctx, const rive_stream* 
         }
         r.paths.push_back(std::move(p));
     }
+    r.handles.reserve(s->image_count);
+    r.images.reserve(s->image_count);
+    for (uint32_t i = 0; i < s->image_count; ++i)
+    {
+        const rive_stream_image& si = s->images[i];
+        rive_hip_image* h = nullptr;
+        int rc = rive_hip_image_create(ctx, si.width, si.height, 0, si.rgba_premul, &h);
+        if (rc != RIVE_HIP_OK)
+            return rc;
+        r.handles.push_back(h);
+        r.images.emplace_back(h, si.width, si.height);
+    }
     rive::gpu::RenderContext rc(std::make_unique<rive::RenderContextHIPImpl>(ctx));
     rive::gpu::RenderContext::FrameDescriptor fd;
     fd.renderTargetWidth = s->width;
@@ -729,7 +799,15 @@ extern "C" int rive_hip_render_stream(rive_hip_context* ctx, const rive_stream* 
         renderer.transform(rive::Mat2D(d.matrix[0], d.matrix[1], d.matrix[2], d.matrix[3], d.matrix[4], d.matrix[5]));
         for (uint32_t c = 0; c < d.clip_count && c < 2; ++c)
             renderer.clipPath(&r.paths[d.clip_paths[c]]);
-        renderer.drawPath(&r.paths[d.path], &paint);
+        if (d.image != 0)
+        {
+            if (d.image > r.images.size())
+                return RIVE_HIP_ERROR_INVALID_ARGUMENT;
+            renderer.drawImage(&r.images[d.image - 1], rive::ImageSampler::FromKey((uint8_t)d.image_sampler),
+                               (rive::BlendMode)d.blend_mode, d.opacity);
+        }
+        else
+            renderer.drawPath(&r.paths[d.path], &paint);
         renderer.restore();
     }
     return rc.flush(target) ? RIVE_HIP_OK : RIVE_HIP_ERROR_DEVICE;

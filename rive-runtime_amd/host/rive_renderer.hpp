@@ -12,6 +12,7 @@
 
 #include "../../include/rive_hip.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <memory>
@@ -123,6 +124,46 @@ class RenderPath
 public:
     virtual ~RenderPath() = default;
 };
+
+// include/rive/shapes/paint/image_sampler.hpp:7-61
+enum class ImageFilter : uint8_t { bilinear = 0, nearest = 1 };
+enum class ImageWrap : uint8_t { clamp = 0, repeat = 1, mirror = 2 };
+struct ImageSampler
+{
+    ImageWrap wrapX = ImageWrap::clamp;
+    ImageWrap wrapY = ImageWrap::clamp;
+    ImageFilter filter = ImageFilter::bilinear;
+    uint8_t asKey() const { return (uint8_t)((int)wrapX + (int)wrapY * 3 + (int)filter * 9); }
+    static ImageSampler FromKey(uint8_t key)
+    {
+        ImageSampler s;
+        s.wrapX = (ImageWrap)(key % 3);
+        s.wrapY = (ImageWrap)((key / 3) % 3);
+        s.filter = (ImageFilter)((key / 9) % 2);
+        return s;
+    }
+};
+
+class RenderImage
+{
+public:
+    virtual ~RenderImage() = default;
+};
+
+// A decoded, premultiplied RGBA8 image resident in HBM
+// (renderer/src/rive_render_image.hpp, gpu::Texture).
+class RiveRenderImage : public RenderImage
+{
+public:
+    RiveRenderImage(rive_hip_image* image, uint32_t w, uint32_t h) : m_image(image), m_w(w), m_h(h) {}
+    rive_hip_image* handle() const { return m_image; }
+    uint32_t width() const { return m_w; }
+    uint32_t height() const { return m_h; }
+
+private:
+    rive_hip_image* m_image;
+    uint32_t m_w, m_h;
+};
 class RenderPaint
 {
 public:
@@ -165,6 +206,18 @@ public:
     void feather(float f) { m_feather = f; }
     void blendMode(BlendMode b) { m_blend = b; }
     void shader(std::shared_ptr<Gradient> g) { m_shader = std::move(g); }
+    // RiveRenderPaint::image (rive_render_paint.cpp:58-64): white at `opacity`
+    // modulated by the image.
+    void image(const RiveRenderImage* img, float opacity)
+    {
+        m_shader.reset();
+        float a = std::max(0.f, std::min(1.f, opacity));
+        m_color = ((uint32_t)std::lround(255.f * a) << 24) | 0x00ffffffu;
+        m_image = img;
+    }
+    void imageSampler(ImageSampler s) { m_sampler = s; }
+    const RiveRenderImage* getImage() const { return m_image; }
+    ImageSampler getImageSampler() const { return m_sampler; }
     bool getIsStroked() const { return m_stroked; }
     ColorInt getColor() const { return m_color; }
     float getThickness() const { return m_thickness; }
@@ -183,6 +236,8 @@ private:
     float m_feather = 0;
     BlendMode m_blend = BlendMode::srcOver;
     std::shared_ptr<Gradient> m_shader;
+    const RiveRenderImage* m_image = nullptr;
+    ImageSampler m_sampler;
 };
 
 class Renderer
@@ -194,6 +249,7 @@ public:
     virtual void transform(const Mat2D& m) = 0;
     virtual void drawPath(RenderPath* path, RenderPaint* paint) = 0;
     virtual void clipPath(RenderPath* path) = 0;
+    virtual void drawImage(const RenderImage* image, ImageSampler sampler, BlendMode blendMode, float opacity) = 0;
 };
 
 namespace gpu
@@ -207,6 +263,7 @@ struct DrawRecord
     rive_hip_draw draw;
     const RawPath* raw;
     const Gradient* gradient; // null for solid/clip paints
+    const RiveRenderImage* image;
 };
 
 // The flush descriptor handed to the backend (gpu.hpp:1320-1438, reduced).
@@ -224,6 +281,7 @@ struct FlushDescriptor
     std::vector<float> points;
     std::vector<rive_hip_grad_span> gradSpans;
     uint32_t gradSimpleRows = 0, gradComplexRows = 0;
+    std::vector<rive_hip_image*> images;
 };
 
 // Backend plugin interface (render_context_impl.hpp:26-243, reduced to what
@@ -276,8 +334,10 @@ public:
     void transform(const Mat2D& m) override;
     void drawPath(RenderPath* path, RenderPaint* paint) override;
     void clipPath(RenderPath* path) override;
+    void drawImage(const RenderImage* image, ImageSampler sampler, BlendMode blendMode, float opacity) override;
 
 private:
+    RiveRenderPath m_unitRectPath;
     struct IAABB
     {
         int l = 0, t = 0, r = 0, b = 0;

@@ -38,7 +38,8 @@ DRAW = np.dtype([("path_id", "<u4"), ("first_verb", "<u4"), ("verb_count", "<u4"
                  ("contour_flags", "<u4"), ("join", "<u4"), ("cap", "<u4"), ("is_stroke", "<u4"),
                  ("matrix", "<f4", 6), ("stroke_radius", "<f4"), ("matrix_max_scale", "<f4"),
                  ("polar_segments_per_radian", "<f4"), ("parametric_precision", "<f4"),
-                 ("tess_location", "<u4"), ("tess_vertex_count", "<u4"), ("pad", "<u4", 2)])
+                 ("tess_location", "<u4"), ("tess_vertex_count", "<u4"), ("image_index", "<u4"),
+                 ("image_sampler", "<u4")])
 TESS_TEXEL = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<u4"), ("w", "<u4")])
 assert TESS_SPAN.itemsize == 64 and CONTOUR.itemsize == 16 and PATH_DATA.itemsize == 64
 assert PAINT_DATA.itemsize == 8 and PAINT_AUX.itemsize == 128 and GRAD_SPAN.itemsize == 16
@@ -53,7 +54,8 @@ class Stream(ctypes.Structure):
                 ("paths", ctypes.c_void_p), ("path_count", ctypes.c_uint32),
                 ("draws", ctypes.c_void_p), ("draw_count", ctypes.c_uint32),
                 ("stop_colors", ctypes.c_void_p), ("stop_offsets", ctypes.c_void_p),
-                ("stop_count", ctypes.c_uint32)]
+                ("stop_count", ctypes.c_uint32), ("images", ctypes.c_void_p),
+                ("image_count", ctypes.c_uint32)]
 
 
 class FlushDesc(ctypes.Structure):
@@ -64,7 +66,8 @@ class FlushDesc(ctypes.Structure):
                 ("tess_vertex_count", ctypes.c_uint32), ("draw_count", ctypes.c_uint32),
                 ("grad_span_count", ctypes.c_uint32), ("grad_simple_rows", ctypes.c_uint32),
                 ("grad_complex_rows", ctypes.c_uint32),
-                ("first_path_tess_location", ctypes.c_uint32)]
+                ("first_path_tess_location", ctypes.c_uint32), ("images", ctypes.c_void_p),
+                ("image_count", ctypes.c_uint32)]
 
 
 class FlushStats(ctypes.Structure):
@@ -83,7 +86,8 @@ EXPORTED = [
     "rive_hip_buffer_map", "rive_hip_buffer_unmap", "rive_hip_target_create",
     "rive_hip_target_destroy", "rive_hip_flush", "rive_hip_sync", "rive_hip_read_pixels",
     "rive_hip_debug_readback", "rive_hip_pass_times", "rive_stream_generate", "rive_stream_free",
-    "rive_hip_render_stream", "rive_hip_last_flush_stats",
+    "rive_hip_render_stream", "rive_hip_last_flush_stats", "rive_hip_image_create",
+    "rive_hip_image_destroy",
 ]
 
 _lib = None
@@ -120,6 +124,9 @@ def lib():
     L.rive_stream_free.restype = None
     L.rive_hip_render_stream.argtypes = [vp, ctypes.POINTER(Stream), vp]
     L.rive_hip_last_flush_stats.argtypes = [vp, ctypes.POINTER(FlushStats)]
+    L.rive_hip_image_create.argtypes = [vp, u32, u32, u32, vp, ctypes.POINTER(vp)]
+    L.rive_hip_image_destroy.argtypes = [vp]
+    L.rive_hip_image_destroy.restype = None
     _lib = L
     return L
 
@@ -228,3 +235,14 @@ class Context:
         desc.grad_complex_rows = stats.grad_complex_rows
         desc.first_path_tess_location = stats.first_path_tess_location
         self.flush(desc)
+
+    def image(self, rgba_premul, mip_levels=0):
+        """Uploads a premultiplied RGBA8 image (h, w, 4) and builds its mips."""
+        a = np.ascontiguousarray(rgba_premul, dtype=np.uint8)
+        h, w = a.shape[:2]
+        out = ctypes.c_void_p()
+        check(self.L.rive_hip_image_create(self.h, w, h, mip_levels, a.ctypes.data, ctypes.byref(out)))
+        return out
+
+    def free_image(self, img):
+        self.L.rive_hip_image_destroy(img)

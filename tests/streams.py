@@ -26,8 +26,10 @@ _DRAW = np.dtype([("path", "<u4"), ("matrix", "<f4", 6), ("style", "<u4"), ("thi
                   ("join", "<u4"), ("cap", "<u4"), ("feather", "<f4"), ("blend_mode", "<u4"),
                   ("paint_type", "<u4"), ("color", "<u4"), ("first_stop", "<u4"),
                   ("stop_count", "<u4"), ("grad", "<f4", 4), ("clip_count", "<u4"),
-                  ("clip_paths", "<u4", 2)])
-assert _PATH.itemsize == 20 and _DRAW.itemsize == 96
+                  ("clip_paths", "<u4", 2), ("image", "<u4"), ("image_sampler", "<u4"),
+                  ("opacity", "<f4")])
+_IMAGE = np.dtype([("width", "<u4"), ("height", "<u4"), ("rgba", "<u8")])
+assert _PATH.itemsize == 20 and _DRAW.itemsize == 108 and _IMAGE.itemsize == 16
 
 IDENTITY = (1.0, 0.0, 0.0, 1.0, 0.0, 0.0)
 
@@ -45,6 +47,24 @@ class StreamBuilder:
         self.clear_color, self.dither = clear_color, dither
         self.verbs, self.points, self.paths, self.draws = [], [], [], []
         self.stop_colors, self.stop_offsets = [], []
+        self.images = []
+
+    def image(self, rgba_premul):
+        """Adds a premultiplied RGBA8 image (h, w, 4); returns its 1-based index."""
+        self.images.append(np.ascontiguousarray(rgba_premul, dtype=np.uint8))
+        return len(self.images)
+
+    def draw_image(self, image, matrix=IDENTITY, blend=3, sampler=0, opacity=1.0, clips=()):
+        d = np.zeros((), _DRAW)
+        d["matrix"] = matrix
+        d["blend_mode"] = blend
+        d["image"] = image
+        d["image_sampler"] = sampler
+        d["opacity"] = opacity
+        d["clip_count"] = len(clips)
+        for i, c in enumerate(clips):
+            d["clip_paths"][i] = c
+        self.draws.append(d)
 
     def path(self, cmds, fill_rule=NONZERO):
         first_verb, first_point = len(self.verbs), len(self.points) // 2
@@ -98,5 +118,12 @@ class StreamBuilder:
         s.draws, s.draw_count = self._draws.ctypes.data, len(self._draws)
         s.stop_colors, s.stop_offsets = self._colors.ctypes.data, self._offsets.ctypes.data
         s.stop_count = len(self.stop_colors)
+        self._images = np.zeros(len(self.images), _IMAGE)
+        for i, img in enumerate(self.images):
+            self._images[i]["width"] = img.shape[1]
+            self._images[i]["height"] = img.shape[0]
+            self._images[i]["rgba"] = img.ctypes.data
+        s.images = self._images.ctypes.data if len(self.images) else None
+        s.image_count = len(self.images)
         self._struct = s
         return ctypes.pointer(s)

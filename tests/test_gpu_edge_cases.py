@@ -116,11 +116,32 @@ def sc_blend_modes():
     return b
 
 
+def random_premul(rng, h, w, opaque):
+    rgb = rng.integers(0, 256, (h, w, 3))
+    a = np.full((h, w, 1), 255) if opaque else rng.integers(0, 256, (h, w, 1))
+    return np.concatenate([np.minimum(rgb, a), a], axis=2).astype(np.uint8)
+
+
+def sc_images():
+    rng = np.random.default_rng(5)
+    b = StreamBuilder(96, 80, dither=1)
+    t1 = b.image(random_premul(rng, 24, 32, True))
+    t2 = b.image(random_premul(rng, 64, 64, False))
+    b.draw(b.path(rect(0, 0, 96, 80)), color=0xff305070)
+    b.draw_image(t1, matrix=(1, 0, 0, 1, 4, 4))
+    b.draw_image(t1, matrix=(.9, .3, -.3, .9, 60, 2), sampler=9, opacity=.7)
+    b.draw_image(t2, matrix=(.25, 0, 0, .25, 10, 50), blend=24)
+    b.draw_image(t2, matrix=(.6, 0, 0, .45, 40, 40), sampler=4)
+    # the clip is drawn under the image's matrix too
+    b.draw_image(t1, matrix=(1.5, 0, 0, 1.5, 50, 30), clips=(b.path(star(8, 13, 4, 11)),), opacity=.9)
+    return b
+
+
 SCENARIOS = {
     "empty_frame": sc_empty_frame, "culled_draws": sc_culled_draws, "one_pixel": sc_one_pixel,
     "offcanvas_and_huge": sc_offcanvas_and_huge, "degenerate_cubics": sc_degenerate_cubics,
     "clockwise_mirrored": sc_clockwise_mirrored, "clips": sc_clips, "gradients": sc_gradients,
-    "blend_modes": sc_blend_modes,
+    "blend_modes": sc_blend_modes, "images": sc_images,
 }
 def build_join_cap(join, cap):
     b = StreamBuilder(64, 48, dither=1)
