@@ -477,8 +477,8 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
         return rc;
     {
         int maxRows = tilesY;
-        uint32_t threads = (pathCount + 1) * (uint32_t)maxRows;
-        backdrop_prefix_kernel<<<(threads + 255) / 256, 256, 0, ctx->stream>>>(
+        const uint64_t threads = (uint64_t)(pathCount + 1) * (uint64_t)maxRows * 64u;
+        backdrop_prefix_kernel<<<(uint32_t)((threads + 255) / 256), 256, 0, ctx->stream>>>(
             P<PathCells>(ctx->cells), pathCount + 1, maxRows, P<int32_t>(ctx->cellBackdrop));
     }
     uint32_t totalPairs = 0;

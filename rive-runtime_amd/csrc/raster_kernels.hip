@@ -654,21 +654,55 @@ __global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels, con
         flags |= kPrimEmpty;
     out.flags = flags;
     prims[g] = out;
+    // Unclamped union of the path's tile bounds; cells_layout_kernel clips it
+    // to the canvas, so geometry off the canvas still widens the grid toward
+    // it.  Lanes of one path (prims of a path are contiguous) reduce first so
+    // a wave issues one set of atomics per path it touches.
+    (void)canvasTilesX;
+    (void)canvasTilesY;
+    int32_t tx0 = 0x7fffffff, ty0 = 0x7fffffff, tx1 = -0x7fffffff, ty1 = -0x7fffffff;
     if (!empty)
     {
-        int32_t minx = min(min(out.p[0], out.p[2]), min(out.p[4], out.p[6]));
-        int32_t maxx = max(max(out.p[0], out.p[2]), max(out.p[4], out.p[6]));
-        int32_t miny = min(min(out.p[1], out.p[3]), min(out.p[5], out.p[7]));
-        int32_t maxy = max(max(out.p[1], out.p[3]), max(out.p[5], out.p[7]));
-        // Unclamped union; cells_layout_kernel clips it to the canvas, so
-        // geometry off the canvas still widens the grid toward it.
-        (void)canvasTilesX;
-        (void)canvasTilesY;
+        const int32_t minx = min(min(out.p[0], out.p[2]), min(out.p[4], out.p[6]));
+        const int32_t maxx = max(max(out.p[0], out.p[2]), max(out.p[4], out.p[6]));
+        const int32_t miny = min(min(out.p[1], out.p[3]), min(out.p[5], out.p[7]));
+        const int32_t maxy = max(max(out.p[1], out.p[3]), max(out.p[5], out.p[7]));
+        tx0 = minx >> (kSubpixelLog2 + kTileLog2);
+        ty0 = miny >> (kSubpixelLog2 + kTileLog2);
+        tx1 = (maxx >> (kSubpixelLog2 + kTileLog2)) + 1;
+        ty1 = (maxy >> (kSubpixelLog2 + kTileLog2)) + 1;
+    }
+    const int lane = threadIdx.x & 63;
+    const uint32_t first = (uint32_t)__shfl((int)d.pathID, 0);
+    const bool same = d.pathID == first;
+    const uint64_t sameMask = __ballot(same);
+    if (sameMask == ~0ull)
+    {
+        // the whole wave (all 64 lanes active) belongs to one path
+#pragma unroll
+        for (int k = 32; k > 0; k >>= 1)
+        {
+            tx0 = min(tx0, __shfl_xor(tx0, k));
+            ty0 = min(ty0, __shfl_xor(ty0, k));
+            tx1 = max(tx1, __shfl_xor(tx1, k));
+            ty1 = max(ty1, __shfl_xor(ty1, k));
+        }
+        if (lane == __builtin_ctzll(sameMask) && tx0 < tx1)
+        {
+            PathCells* pc = &cells[d.pathID];
+            atomicMin(&pc->tx0, tx0);
+            atomicMin(&pc->ty0, ty0);
+            atomicMax(&pc->tx1, tx1);
+            atomicMax(&pc->ty1, ty1);
+        }
+    }
+    else if (!empty)
+    {
         PathCells* pc = &cells[d.pathID];
-        atomicMin(&pc->tx0, minx >> (kSubpixelLog2 + kTileLog2));
-        atomicMin(&pc->ty0, miny >> (kSubpixelLog2 + kTileLog2));
-        atomicMax(&pc->tx1, (maxx >> (kSubpixelLog2 + kTileLog2)) + 1);
-        atomicMax(&pc->ty1, (maxy >> (kSubpixelLog2 + kTileLog2)) + 1);
+        atomicMin(&pc->tx0, tx0);
+        atomicMin(&pc->ty0, ty0);
+        atomicMax(&pc->tx1, tx1);
+        atomicMax(&pc->ty1, ty1);
     }
 }
 
@@ -812,24 +846,37 @@ __global__ void bin_count_kernel(const Prim* __restrict__ prims, uint32_t totalP
     }
 }
 
-// Prefix-sum of corner crossings along each row of each path's grid.
-__global__ void backdrop_prefix_kernel(const PathCells* __restrict__ cells, uint32_t pathCount,
-                                       int maxRows, int32_t* __restrict__ cellBackdrop)
+// Prefix-sum of corner crossings along each row of each path's grid: one
+// wave64 per (path, row), 64 cells per step with a carried running sum.
+__global__ __launch_bounds__(256) void backdrop_prefix_kernel(const PathCells* __restrict__ cells,
+                                                              uint32_t pathCount, int maxRows,
+                                                              int32_t* __restrict__ cellBackdrop)
 {
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t path = g / (uint32_t)maxRows;
-    int row = (int)(g % (uint32_t)maxRows);
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    const uint32_t path = wave / (uint32_t)maxRows;
+    const int row = (int)(wave % (uint32_t)maxRows);
     if (path >= pathCount)
         return;
     const PathCells c = cells[path];
     if (row >= c.ty1 - c.ty0)
         return;
     int32_t* r = cellBackdrop + c.base + (uint32_t)row * c.width;
-    int32_t acc = 0;
-    for (uint32_t x = 0; x < c.width; ++x)
+    int32_t carry = 0;
+    for (uint32_t x0 = 0; x0 < c.width; x0 += 64)
     {
-        acc += r[x];
-        r[x] = acc;
+        const uint32_t x = x0 + (uint32_t)lane;
+        int32_t v = x < c.width ? r[x] : 0;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1)
+        {
+            const int32_t t = __shfl_up(v, d);
+            if (lane >= d)
+                v += t;
+        }
+        if (x < c.width)
+            r[x] = carry + v;
+        carry += __shfl(v, 63);
     }
 }
 
@@ -929,7 +976,7 @@ __device__ __forceinline__ float quantize8(float v)
 // triangle's pixels on a row are the candidates of a float span estimate
 // (widened by a pixel) that pass the exact three-edge test.
 constexpr int kRasterLanes = 64;
-constexpr int kSlots = 4;
+constexpr int kSlots = 4; // at most 16 (4 bits of sItemInfo)
 constexpr int kMaxEntries = 32;
 constexpr int kMaxItems = kRasterLanes * kTile;
 
@@ -941,8 +988,11 @@ struct TileGeom
 __device__ __forceinline__ void tri_row_candidates(int32_t ax, int32_t ay, int32_t bx, int32_t by, int32_t cx,
                                                    int32_t cy, int32_t yq, int32_t X0, int* ilo, int* ihi)
 {
-    float xl = INFINITY, xr = -INFINITY;
-    const int32_t vx[3] = {ax, bx, cx}, vy[3] = {ay, by, cy};
+    // Tile-relative x-range of the triangle on the row's sample line, as a
+    // float estimate widened by its error bound (an fdividef of relative
+    // error < 2^-21 scaled by the edge's width, plus rounding of the ends).
+    float xl = INFINITY, xr = -INFINITY, eps = 8.f;
+    const int32_t vx[3] = {ax - X0, bx - X0, cx - X0}, vy[3] = {ay, by, cy};
 #pragma unroll
     for (int e = 0; e < 3; ++e)
     {
@@ -956,11 +1006,12 @@ __device__ __forceinline__ void tri_row_candidates(int32_t ax, int32_t ay, int32
             }
             else
             {
-                // an estimate: the candidates carry a pixel of slack
+                const float dx = (float)(x1 - x0);
                 const float t = __fdividef((float)(yq - y0), (float)(y1 - y0));
-                const float x = (float)x0 + t * (float)(x1 - x0);
+                const float x = (float)x0 + t * dx;
                 xl = fminf(xl, x);
                 xr = fmaxf(xr, x);
+                eps = fmaxf(eps, fabsf(dx) * (1.f / (1 << 18)) + fabsf(x) * (1.f / (1 << 20)));
             }
         }
     }
@@ -970,11 +1021,12 @@ __device__ __forceinline__ void tri_row_candidates(int32_t ax, int32_t ay, int32
         *ihi = 0;
         return;
     }
-    // pixel i's sample x is X0 + 256 i + 128; one pixel of slack each side
-    const float lo = (xl - (float)X0 - 128.f) * (1.f / 256.f);
-    const float hi = (xr - (float)X0 - 128.f) * (1.f / 256.f);
-    *ilo = (int)fmaxf(floorf(lo) - 1.f, 0.f);
-    *ihi = (int)fminf(ceilf(hi) + 1.f, (float)(kTile - 1));
+    // pixel i's sample x is 256 i + 128 (tile-relative): the pixels whose
+    // samples fall in [xl - eps, xr + eps]
+    const float lo = (xl - eps - 128.f) * (1.f / 256.f);
+    const float hi = (xr + eps - 128.f) * (1.f / 256.f);
+    *ilo = (int)fmaxf(ceilf(lo), 0.f);
+    *ihi = (int)fminf(floorf(hi), (float)(kTile - 1));
 }
 
 // Exact "inside" test of an edge against pixel (i2, j2) with orientation o.
@@ -1071,13 +1123,18 @@ __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* 
                 // Smallest i with s*(A*(2i+1) + krow) >= T, where s*A = |dy|:
                 // 2|dy| i >= T - |dy| - s*krow.  The quotient is exact in
                 // double wherever it decides a column inside the tile.
-                const int64_t sa = (int64_t)(s * e.A);
-                const int64_t r = (int64_t)T - sa - (int64_t)s * krow;
-                const double q = ceil((double)r / (double)(2 * sa));
-                if (q < (double)kTile)
+                const int64_t sa2 = 2 * (int64_t)(s * e.A);
+                const int64_t r = (int64_t)T - sa2 / 2 - (int64_t)s * krow;
+                if (r <= sa2 * (kTile - 1))
                 {
-                    const int istar = q < 0.0 ? 0 : (int)q;
-                    atomicAdd(&sDelta[j * kTile + istar], neg * s);
+                    // ceil(r / sa2) in [0, kTile): a float estimate, then exact
+                    int q = r <= 0 ? 0 : (int)ceilf(__fdividef((float)r, (float)sa2));
+                    q = min(max(q, 0), kTile - 1);
+                    while (q > 0 && sa2 * (q - 1) >= r)
+                        --q;
+                    while (sa2 * q < r)
+                        ++q;
+                    atomicAdd(&sDelta[j * kTile + q], neg * s);
                 }
             }
         }
@@ -1566,7 +1623,7 @@ __global__ __launch_bounds__(kRasterLanes) void raster_kernel(
     __shared__ int32_t sLeftPre[kSlots][kTile];
     __shared__ uint8_t sOwner[kMaxItems];
     __shared__ uint32_t sItemPrim[kRasterLanes];
-    __shared__ uint32_t sItemInfo[kRasterLanes]; // slot | stroke << 2 | row0 << 8 | item0 << 16
+    __shared__ uint32_t sItemInfo[kRasterLanes]; // slot | stroke << 4 | row0 << 8 | item0 << 16
 
     const int tile = blockIdx.x;
     const int tx = tile % tilesX, ty = tile / tilesX;
@@ -1663,7 +1720,8 @@ __global__ __launch_bounds__(kRasterLanes) void raster_kernel(
                     (&sDelta[0][0])[q] = 0;
                     (&sBorder[0][0])[q] = 0;
                 }
-                (&sLeft[0][0])[lane] = 0;
+                for (int q = lane; q < kSlots * kTile; q += kRasterLanes)
+                    (&sLeft[0][0])[q] = 0;
                 __syncthreads();
                 for (uint32_t c0 = 0; c0 < nPrim; c0 += kRasterLanes)
                 {
@@ -1688,7 +1746,7 @@ __global__ __launch_bounds__(kRasterLanes) void raster_kernel(
                         rows = max(r1 - r0 + 1, 0);
                         if (!stroke)
                             fill_prim_left(pm, g, sLeft[slot]);
-                        info = (uint32_t)slot | (stroke << 2) | ((uint32_t)r0 << 8);
+                        info = (uint32_t)slot | (stroke << 4) | ((uint32_t)r0 << 8);
                     }
                     // exclusive prefix of the row counts across the wave
                     int incl = rows;
@@ -1710,10 +1768,10 @@ __global__ __launch_bounds__(kRasterLanes) void raster_kernel(
                     {
                         const int owner = sOwner[t];
                         const uint32_t inf = sItemInfo[owner];
-                        const int slot = (int)(inf & 3u);
+                        const int slot = (int)(inf & 15u);
                         const int row = (int)((inf >> 8) & 0xffu) + (t - (int)(inf >> 16));
                         const Prim pm = prims[sItemPrim[owner]];
-                        if (inf & 4u)
+                        if (inf & 16u)
                             stroke_prim_row(pm, row, g, reinterpret_cast<uint32_t*>(sBorder[slot]));
                         else
                             fill_prim_row(pm, row, g, sDelta[slot], sBorder[slot]);
@@ -1721,8 +1779,9 @@ __global__ __launch_bounds__(kRasterLanes) void raster_kernel(
                     __syncthreads();
                 }
                 // row prefix of the left-edge deltas
+                for (int q = lane; q < nSlots * kTile; q += kRasterLanes)
                 {
-                    const int slot = lane >> kTileLog2, row = lane & (kTile - 1);
+                    const int slot = q >> kTileLog2, row = q & (kTile - 1);
                     int32_t acc = 0;
                     for (int r = 0; r <= row; ++r)
                         acc += sLeft[slot][r];
