@@ -21,6 +21,7 @@
 // perturbation tie rule, so tile decompositions agree exactly.
 #include "device_common.hpp"
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -937,6 +938,43 @@ __device__ __forceinline__ int32_t inside_threshold(const EdgeT& e, int o)
     return clamp_thr(t);
 }
 
+// Width-generic forms of the edge helpers for the per-row work (instantiated
+// for int64).
+template <typename I> struct EdgeTI
+{
+    int32_t A, B; // -dy, dx
+    I D0;
+};
+template <typename I>
+__device__ __forceinline__ EdgeTI<I> make_edge_t(int32_t ax, int32_t ay, int32_t bx, int32_t by, int32_t X0,
+                                                 int32_t Y0)
+{
+    EdgeTI<I> e;
+    const int32_t dx = bx - ax, dy = by - ay;
+    e.A = -dy;
+    e.B = dx;
+    e.D0 = (I)dx * (I)(Y0 - ay) - (I)dy * (I)(X0 - ax);
+    return e;
+}
+template <typename I> __device__ __forceinline__ int32_t clamp_thr_t(I t)
+{
+    const I lim = (I)1 << 30;
+    return (int32_t)(t < -lim ? -lim : (t > lim ? lim : t));
+}
+template <typename I> __device__ __forceinline__ I ceil_div128_t(I a) { return -((-a) >> 7); }
+template <typename I> __device__ __forceinline__ I floor_div128_t(I a) { return a >> 7; }
+template <typename I> __device__ __forceinline__ int32_t inside_threshold_t(const EdgeTI<I>& e, int o)
+{
+    const int pert = e.A != 0 ? (e.A > 0 ? 1 : -1) : (e.B > 0 ? 1 : (e.B < 0 ? -1 : 0));
+    const I oD0 = o > 0 ? e.D0 : -e.D0;
+    const I t = (o * pert > 0) ? ceil_div128_t<I>(-oD0) : floor_div128_t<I>(-oD0) + 1;
+    return clamp_thr_t<I>(t);
+}
+template <typename I>
+__device__ __forceinline__ I orient_t(int32_t ax, int32_t ay, int32_t bx, int32_t by, int32_t qx, int32_t qy)
+{
+    return (I)(bx - ax) * (I)(qy - ay) - (I)(by - ay) * (I)(qx - ax);
+}
 __device__ __forceinline__ float dither_noise(int x, int y)
 {
     // common.glsl:269-274, scale 1/256, bias -1/512 (gpu.cpp:822-824).
@@ -1034,12 +1072,12 @@ struct EdgeK
 {
     int32_t A, B, thr; // o-premultiplied: inside iff A*i2 + B*j2 >= thr
 };
-__device__ __forceinline__ EdgeK edge_k(const EdgeT& e, int o)
+template <typename I> __device__ __forceinline__ EdgeK edge_k(const EdgeTI<I>& e, int o)
 {
     EdgeK k;
     k.A = o * e.A;
     k.B = o * e.B;
-    k.thr = inside_threshold(e, o);
+    k.thr = inside_threshold_t<I>(e, o);
     return k;
 }
 __device__ __forceinline__ bool edge_in(const EdgeK& k, int32_t i2, int32_t j2)
@@ -1102,6 +1140,7 @@ __device__ void fill_prim_left(const Prim& p, const TileGeom& g, int32_t* __rest
 
 // One sample row of a fill primitive: the inner edge's crossing column and
 // the two AA border triangles.
+template <typename I>
 __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* __restrict__ sDelta,
                               int32_t* __restrict__ sBorder)
 {
@@ -1109,14 +1148,14 @@ __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* 
     const int32_t j2 = 2 * j + 1;
     const int32_t yq = Y0 + (j << kSubpixelLog2) + (1 << (kSubpixelLog2 - 1));
     const int neg = (p.flags & kPrimNegate) ? -1 : 1;
-    const EdgeT e = make_edge(p.p[0], p.p[1], p.p[4], p.p[5], X0, Y0);
+    const EdgeTI<I> e = make_edge_t<I>(p.p[0], p.p[1], p.p[4], p.p[5], X0, Y0);
     if (p.p[1] != p.p[5])
     {
         const int32_t ymin = min(p.p[1], p.p[5]), ymax = max(p.p[1], p.p[5]);
         if (ymin <= yq && yq < ymax)
         {
             const int s = p.p[5] < p.p[1] ? 1 : -1;
-            const int32_t T = clamp_thr(ceil_div128(s > 0 ? -e.D0 : e.D0));
+            const int32_t T = clamp_thr_t<I>(ceil_div128_t<I>(s > 0 ? -e.D0 : e.D0));
             const int32_t krow = e.B * j2;
             if (!(s * krow >= T))
             {
@@ -1140,7 +1179,7 @@ __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* 
         }
     }
     // Border triangle T1 = (I0, O0, I1), coverage 1 - lambda(O0).
-    const int64_t A1 = orient64(p.p[0], p.p[1], p.p[2], p.p[3], p.p[4], p.p[5]);
+    const I A1 = orient_t<I>(p.p[0], p.p[1], p.p[2], p.p[3], p.p[4], p.p[5]);
     if (A1 != 0)
     {
         int ilo, ihi;
@@ -1148,9 +1187,9 @@ __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* 
         if (ilo <= ihi)
         {
             const int o = A1 > 0 ? 1 : -1;
-            const EdgeK k01 = edge_k(make_edge(p.p[0], p.p[1], p.p[2], p.p[3], X0, Y0), o);
-            const EdgeK k12 = edge_k(make_edge(p.p[2], p.p[3], p.p[4], p.p[5], X0, Y0), o);
-            const EdgeK k20 = edge_k(make_edge(p.p[4], p.p[5], p.p[0], p.p[1], X0, Y0), o);
+            const EdgeK k01 = edge_k<I>(make_edge_t<I>(p.p[0], p.p[1], p.p[2], p.p[3], X0, Y0), o);
+            const EdgeK k12 = edge_k<I>(make_edge_t<I>(p.p[2], p.p[3], p.p[4], p.p[5], X0, Y0), o);
+            const EdgeK k20 = edge_k<I>(make_edge_t<I>(p.p[4], p.p[5], p.p[0], p.p[1], X0, Y0), o);
             const float fA1 = (float)A1;
             for (int i = ilo; i <= ihi; ++i)
             {
@@ -1166,7 +1205,7 @@ __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* 
         }
     }
     // Border triangle T2 = (I1, O0, O1), coverage lambda(I1).
-    const int64_t A2 = orient64(p.p[4], p.p[5], p.p[2], p.p[3], p.p[6], p.p[7]);
+    const I A2 = orient_t<I>(p.p[4], p.p[5], p.p[2], p.p[3], p.p[6], p.p[7]);
     if (A2 != 0)
     {
         int ilo, ihi;
@@ -1174,10 +1213,10 @@ __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* 
         if (ilo <= ihi)
         {
             const int o = A2 > 0 ? 1 : -1;
-            const EdgeK f01 = edge_k(make_edge(p.p[4], p.p[5], p.p[2], p.p[3], X0, Y0), o);
-            const EdgeT f12e = make_edge(p.p[2], p.p[3], p.p[6], p.p[7], X0, Y0);
+            const EdgeK f01 = edge_k<I>(make_edge_t<I>(p.p[4], p.p[5], p.p[2], p.p[3], X0, Y0), o);
+            const EdgeTI<I> f12e = make_edge_t<I>(p.p[2], p.p[3], p.p[6], p.p[7], X0, Y0);
             const EdgeK f12 = edge_k(f12e, o);
-            const EdgeK f20 = edge_k(make_edge(p.p[6], p.p[7], p.p[4], p.p[5], X0, Y0), o);
+            const EdgeK f20 = edge_k<I>(make_edge_t<I>(p.p[6], p.p[7], p.p[4], p.p[5], X0, Y0), o);
             const float fA2 = (float)A2;
             for (int i = ilo; i <= ihi; ++i)
             {
@@ -1194,6 +1233,7 @@ __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* 
     }
 }
 
+template <typename I>
 __device__ void stroke_prim_row(const Prim& p, int j, const TileGeom& g, uint32_t* __restrict__ sMax)
 {
     const int32_t X0 = g.X0, Y0 = g.Y0;
@@ -1201,19 +1241,19 @@ __device__ void stroke_prim_row(const Prim& p, int j, const TileGeom& g, uint32_
     const int32_t yq = Y0 + (j << kSubpixelLog2) + (1 << (kSubpixelLog2 - 1));
     // T1 = (P0-, P0+, P1-): vertices 0, 1, 2; T2 = (P1-, P0+, P1+): 2, 1, 3.
     // Only clockwise triangles draw.
-    const int64_t A1 = orient64(p.p[0], p.p[1], p.p[2], p.p[3], p.p[4], p.p[5]);
-    const int64_t A2 = orient64(p.p[4], p.p[5], p.p[2], p.p[3], p.p[6], p.p[7]);
-    const EdgeT e12 = make_edge(p.p[2], p.p[3], p.p[4], p.p[5], X0, Y0);
+    const I A1 = orient_t<I>(p.p[0], p.p[1], p.p[2], p.p[3], p.p[4], p.p[5]);
+    const I A2 = orient_t<I>(p.p[4], p.p[5], p.p[2], p.p[3], p.p[6], p.p[7]);
+    const EdgeTI<I> e12 = make_edge_t<I>(p.p[2], p.p[3], p.p[4], p.p[5], X0, Y0);
     if (A1 > 0)
     {
         int ilo, ihi;
         tri_row_candidates(p.p[0], p.p[1], p.p[2], p.p[3], p.p[4], p.p[5], yq, X0, &ilo, &ihi);
         if (ilo <= ihi)
         {
-            const EdgeT e01 = make_edge(p.p[0], p.p[1], p.p[2], p.p[3], X0, Y0);
-            const EdgeT e20 = make_edge(p.p[4], p.p[5], p.p[0], p.p[1], X0, Y0);
-            const int32_t t01 = inside_threshold(e01, 1), t12 = inside_threshold(e12, 1),
-                          t20 = inside_threshold(e20, 1);
+            const EdgeTI<I> e01 = make_edge_t<I>(p.p[0], p.p[1], p.p[2], p.p[3], X0, Y0);
+            const EdgeTI<I> e20 = make_edge_t<I>(p.p[4], p.p[5], p.p[0], p.p[1], X0, Y0);
+            const int32_t t01 = inside_threshold_t<I>(e01, 1), t12 = inside_threshold_t<I>(e12, 1),
+                          t20 = inside_threshold_t<I>(e20, 1);
             const float inv = 1.f / (float)A1;
             for (int i = ilo; i <= ihi; ++i)
             {
@@ -1242,10 +1282,10 @@ __device__ void stroke_prim_row(const Prim& p, int j, const TileGeom& g, uint32_
         if (ilo <= ihi)
         {
             // T2 edges: P1- -> P0+ is the reverse of e12.
-            const EdgeT e13 = make_edge(p.p[2], p.p[3], p.p[6], p.p[7], X0, Y0);
-            const EdgeT e32 = make_edge(p.p[6], p.p[7], p.p[4], p.p[5], X0, Y0);
-            const int32_t t21 = inside_threshold(EdgeT{-e12.A, -e12.B, -e12.D0}, 1);
-            const int32_t t13 = inside_threshold(e13, 1), t32 = inside_threshold(e32, 1);
+            const EdgeTI<I> e13 = make_edge_t<I>(p.p[2], p.p[3], p.p[6], p.p[7], X0, Y0);
+            const EdgeTI<I> e32 = make_edge_t<I>(p.p[6], p.p[7], p.p[4], p.p[5], X0, Y0);
+            const int32_t t21 = inside_threshold_t<I>(EdgeTI<I>{-e12.A, -e12.B, -e12.D0}, 1);
+            const int32_t t13 = inside_threshold_t<I>(e13, 1), t32 = inside_threshold_t<I>(e32, 1);
             const float inv = 1.f / (float)A2;
             for (int i = ilo; i <= ihi; ++i)
             {
@@ -1497,21 +1537,72 @@ __device__ void advanced_blend_coeffs_d(const float* srcIn, const float* dp, uin
 // Clip, paint and src-over of one path into one pixel (draw_path.vert:431-545,
 // draw_raster_order_path.frag:84-238).  A zero coverage leaves the pixel
 // unchanged, so callers skip it (except for clip updates).
-__device__ __forceinline__ void resolve_pixel(const PathRaster& pr, float coverage, int px, int py, float noise,
-                                              const uint32_t* __restrict__ ramps, uint32_t gradHeight,
-                                              const ImageDesc* __restrict__ images, float* col, float& clipCov,
-                                              uint32_t& clipID)
+__device__ __forceinline__ float clip_coverage(uint32_t clip) { return __half2float(__ushort_as_half((unsigned short)(clip >> 16))); }
+
+// Paint, blend and store of one path's color into one pixel's RGBA8, after
+// clipping: src-over with solid or gradient paints inline; advanced blend
+// modes and image paints out of line (rare, and large).
+__device__ __forceinline__ void blend_store(const float* src, const float* col, float noise, uint32_t& colPacked)
 {
-    if (pr.flags & 8u) // clip update
+    const float paintAlpha = src[3];
+    float out[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        out[k] = src[k] + col[k] * (1.f - paintAlpha);
+    if (paintAlpha != 0.f)
     {
-        if (pr.outerClipID != 0u)
-            coverage = fminf(coverage, clipID == pr.outerClipID ? clipCov : 0.f);
-        clipCov = coverage;
-        clipID = pr.clipID;
-        return;
+        out[0] += noise;
+        out[1] += noise;
+        out[2] += noise;
     }
-    if (pr.clipID != 0u)
-        coverage = clipID == pr.clipID ? fminf(clipCov, coverage) : 0.f;
+    uint32_t packed = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        packed |= (uint32_t)floorf(fminf(fmaxf(out[k], 0.f), 1.f) * 255.f + .5f) << (8 * k);
+    colPacked = packed;
+}
+
+__device__ __forceinline__ void unpack_color(uint32_t c, float* col)
+{
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        col[k] = float((c >> (8 * k)) & 0xffu) * (1.f / 255.f);
+}
+
+// Gradient paint color, unpremultiplied (draw_path.vert:285-331,450-476).
+__device__ __forceinline__ void sample_gradient(const PathRaster& pr, int px, int py,
+                                                const uint32_t* __restrict__ ramps, uint32_t gradHeight, float* src)
+{
+    const float fx = float(px) + .5f, fy = float(py) + .5f;
+    const float pcx = pr.paintMatrix[0] * fx + pr.paintMatrix[2] * fy + pr.paintMatrix[4];
+    const float pcy = pr.paintMatrix[1] * fx + pr.paintMatrix[3] * fy + pr.paintMatrix[5];
+    float t = pr.paintType == LINEAR_GRADIENT_PAINT_TYPE ? pcx : sqrtf(pcx * pcx + pcy * pcy);
+    t = fminf(fmaxf(t, 0.f), 1.f);
+    const float u = pr.gradSpan > 1.f ? (1.f - 1.f / 512.f) * t + (.5f / 512.f) : (1.f / 512.f) * t + pr.gradSpan;
+    const float tc = u * 512.f - .5f;
+    const int i0 = (int)floorf(tc);
+    const float fr = tc - (float)i0;
+    const int ia = min(max(i0, 0), 511), ib = min(max(i0 + 1, 0), 511);
+    const uint32_t* rowp = ramps + (size_t)min(pr.gradRow, gradHeight - 1) * kGradWidth;
+    const uint32_t ta = rowp[ia], tb = rowp[ib];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        const float va = float((ta >> (8 * k)) & 0xffu) * (1.f / 255.f);
+        const float vb = float((tb >> (8 * k)) & 0xffu) * (1.f / 255.f);
+        src[k] = (vb - va) * fr + va;
+    }
+}
+
+// Advanced blend modes and image paints (draw_path.vert:431-545,
+// draw_raster_order_path.frag:191-204).
+__device__ __noinline__ void resolve_general(const PathRaster* __restrict__ prp, float coverage, int px, int py,
+                                             float noise, const uint32_t* __restrict__ ramps, uint32_t gradHeight,
+                                             const ImageDesc* __restrict__ images, uint32_t* colPacked)
+{
+    const PathRaster pr = *prp;
+    float col[4];
+    unpack_color(*colPacked, col);
     float src[4];
     const uint32_t blendMode = (pr.flags >> 8) & 0xfu;
     if (pr.paintType == SOLID_COLOR_PAINT_TYPE && blendMode == 0u)
@@ -1529,26 +1620,7 @@ __device__ __forceinline__ void resolve_pixel(const PathRaster& pr, float covera
     }
     else
     {
-        const float fx = float(px) + .5f, fy = float(py) + .5f;
-        const float pcx = pr.paintMatrix[0] * fx + pr.paintMatrix[2] * fy + pr.paintMatrix[4];
-        const float pcy = pr.paintMatrix[1] * fx + pr.paintMatrix[3] * fy + pr.paintMatrix[5];
-        float t = pr.paintType == LINEAR_GRADIENT_PAINT_TYPE ? pcx : sqrtf(pcx * pcx + pcy * pcy);
-        t = fminf(fmaxf(t, 0.f), 1.f);
-        const float u = pr.gradSpan > 1.f ? (1.f - 1.f / 512.f) * t + (.5f / 512.f)
-                                          : (1.f / 512.f) * t + pr.gradSpan;
-        const float tc = u * 512.f - .5f;
-        const int i0 = (int)floorf(tc);
-        const float fr = tc - (float)i0;
-        const int ia = min(max(i0, 0), 511), ib = min(max(i0 + 1, 0), 511);
-        const uint32_t* rowp = ramps + (size_t)min(pr.gradRow, gradHeight - 1) * kGradWidth;
-        const uint32_t ta = rowp[ia], tb = rowp[ib];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-        {
-            const float va = float((ta >> (8 * k)) & 0xffu) * (1.f / 255.f);
-            const float vb = float((tb >> (8 * k)) & 0xffu) * (1.f / 255.f);
-            src[k] = (vb - va) * fr + va;
-        }
+        sample_gradient(pr, px, py, ramps, gradHeight, src);
         src[3] *= coverage;
         if (blendMode == 0u)
         {
@@ -1579,29 +1651,61 @@ __device__ __forceinline__ void resolve_pixel(const PathRaster& pr, float covera
     }
     if (blendMode != 0u)
     {
-        // draw_raster_order_path.frag:191-204: mix the unpremultiplied paint
-        // with the blend coefficients by the destination alpha, then
-        // premultiply.
+        // mix the unpremultiplied paint with the blend coefficients by the
+        // destination alpha, then premultiply
         float coeffs[3];
         advanced_blend_coeffs_d(src, col, blendMode, coeffs);
 #pragma unroll
         for (int k = 0; k < 3; ++k)
             src[k] = ((coeffs[k] - src[k]) * col[3] + src[k]) * src[3];
     }
-    const float paintAlpha = src[3];
-    float out[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        out[k] = src[k] + col[k] * (1.f - paintAlpha);
-    if (paintAlpha != 0.f)
+    uint32_t packed;
+    blend_store(src, col, noise, packed);
+    *colPacked = packed;
+}
+
+// Clip, paint and src-over of one path into one pixel (draw_path.vert:431-545,
+// draw_raster_order_path.frag:84-238).  A zero coverage leaves the pixel
+// unchanged, so callers skip it (except for clip updates).
+__device__ __forceinline__ void resolve_pixel(const PathRaster& pr, const PathRaster* __restrict__ prp,
+                                              float coverage, int px, int py, float noise,
+                                              const uint32_t* __restrict__ ramps, uint32_t gradHeight,
+                                              const ImageDesc* __restrict__ images, uint32_t& colPacked,
+                                              uint32_t& clip)
+{
+    const uint32_t clipID = clip & 0xffffu;
+    if (pr.flags & 8u) // clip update
     {
-        out[0] += noise;
-        out[1] += noise;
-        out[2] += noise;
+        if (pr.outerClipID != 0u)
+            coverage = fminf(coverage, clipID == pr.outerClipID ? clip_coverage(clip) : 0.f);
+        clip = ((uint32_t)__half_as_ushort(__float2half_rn(coverage)) << 16) | (pr.clipID & 0xffffu);
+        return;
     }
+    if (pr.clipID != 0u)
+        coverage = clipID == pr.clipID ? fminf(clip_coverage(clip), coverage) : 0.f;
+    if ((pr.flags & 0xf00u) != 0u || pr.image != 0u)
+    {
+        resolve_general(prp, coverage, px, py, noise, ramps, gradHeight, images, &colPacked);
+        return;
+    }
+    float col[4];
+    unpack_color(colPacked, col);
+    float src[4];
+    if (pr.paintType == SOLID_COLOR_PAINT_TYPE)
+    {
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-        col[k] = quantize8(out[k]);
+        for (int k = 0; k < 4; ++k)
+            src[k] = pr.color[k] * coverage;
+    }
+    else
+    {
+        sample_gradient(pr, px, py, ramps, gradHeight, src);
+        src[3] *= coverage;
+        src[0] *= src[3];
+        src[1] *= src[3];
+        src[2] *= src[3];
+    }
+    blend_store(src, col, noise, colPacked);
 }
 
 __device__ __forceinline__ uint32_t lane_read(uint32_t v, int lane)
@@ -1634,9 +1738,12 @@ __global__ __launch_bounds__(kRasterLanes) void raster_kernel(
     g.X0 = tx << (kSubpixelLog2 + kTileLog2);
     g.Y0 = ty << (kSubpixelLog2 + kTileLog2);
 
-    float col[4][4];
-    float clipCov[4] = {0.f, 0.f, 0.f, 0.f};
-    uint32_t clipID[4] = {0u, 0u, 0u, 0u};
+    // Per pixel: the RGBA8 color (quantized after every path, as the
+    // reference's color buffer is), and the clip buffer as clip ID (low 16
+    // bits) with fp16 coverage (high 16 bits), like the reference's
+    // clipBuffer.
+    uint32_t col[4];
+    uint32_t clip[4] = {0u, 0u, 0u, 0u};
     float noise[4];
     const bool rowInside = py < height;
 #pragma unroll
@@ -1645,9 +1752,7 @@ __global__ __launch_bounds__(kRasterLanes) void raster_kernel(
         uint32_t c = clearPremul;
         if (loadPreserve && rowInside && px0 + q < width)
             c = target[(size_t)py * width + px0 + q];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            col[q][k] = float((c >> (8 * k)) & 0xffu) * (1.f / 255.f);
+        col[q] = c;
         noise[q] = dither ? dither_noise(px0 + q, py) : 0.f;
     }
 
@@ -1772,9 +1877,9 @@ __global__ __launch_bounds__(kRasterLanes) void raster_kernel(
                         const int row = (int)((inf >> 8) & 0xffu) + (t - (int)(inf >> 16));
                         const Prim pm = prims[sItemPrim[owner]];
                         if (inf & 16u)
-                            stroke_prim_row(pm, row, g, reinterpret_cast<uint32_t*>(sBorder[slot]));
+                            stroke_prim_row<int64_t>(pm, row, g, reinterpret_cast<uint32_t*>(sBorder[slot]));
                         else
-                            fill_prim_row(pm, row, g, sDelta[slot], sBorder[slot]);
+                            fill_prim_row<int64_t>(pm, row, g, sDelta[slot], sBorder[slot]);
                     }
                     __syncthreads();
                 }
@@ -1838,8 +1943,8 @@ __global__ __launch_bounds__(kRasterLanes) void raster_kernel(
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
                     if (cov[q] != 0.f || clipUpdate)
-                        resolve_pixel(pr, cov[q], px0 + q, py, noise[q], ramps, gradHeight, images, col[q],
-                                      clipCov[q], clipID[q]);
+                        resolve_pixel(pr, raster + path, cov[q], px0 + q, py, noise[q], ramps, gradHeight, images,
+                                      col[q], clip[q]);
             }
             if (nSlots > 0)
                 __syncthreads(); // the next batch clears the slots
@@ -1847,15 +1952,7 @@ __global__ __launch_bounds__(kRasterLanes) void raster_kernel(
     }
     if (rowInside)
     {
-        uint32_t packed[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-        {
-            packed[q] = 0;
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                packed[q] |= (uint32_t)__float2int_rn(col[q][k] * 255.f) << (8 * k);
-        }
+        const uint32_t* packed = col;
         uint32_t* row = target + (size_t)py * width;
         if (px0 + 3 < width && ((((size_t)py * width + px0) & 3) == 0))
             *reinterpret_cast<uint4*>(row + px0) = make_uint4(packed[0], packed[1], packed[2], packed[3]);
