@@ -500,7 +500,7 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
 
     // ---- raster + resolve
-    raster_kernel<<<tilesX * tilesY, kRasterLanes, 0, ctx->stream>>>(
+    raster_kernel<<<tilesX * tilesY, kWgThreads, 0, ctx->stream>>>(
         P<PathCells>(ctx->cells), P<PathRaster>(ctx->pathRaster), pathCount,
         P<uint32_t>(ctx->cellOffset), P<int32_t>(ctx->cellBackdrop), P<uint32_t>(ctx->binned),
         P<Prim>(ctx->prims), P<uint32_t>(ctx->ramps), gradRows, P<ImageDesc>(ctx->imageTable), W, H, tilesX,

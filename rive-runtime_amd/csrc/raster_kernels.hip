@@ -991,32 +991,41 @@ __device__ __forceinline__ float quantize8(float v)
     return floorf(v * 255.f + .5f) * (1.f / 255.f);
 }
 
-// One wave64 per 16x16 tile; each lane owns 4 pixels of one row
-// (row = lane / 4, columns 4 * (lane % 4) .. +3).  Paths are visited in
-// submission order and resolved per pixel in that order.
+// One 256-thread workgroup (four waves) per 16x16 tile; thread t owns pixel
+// (t % 16, t / 16).  Paths are visited in submission order and resolved per
+// pixel in that order.
 //
-// Paths with primitives in the tile are gathered into batches of up to
-// kSlots paths (and, where it fits, 64 primitives).  Each batched path owns an
-// LDS accumulator slot:
-//   sLeft[slot][row]      winding change along the tile's left edge, as a
-//                         suffix delta over rows (prefix-summed per row);
-//   sDelta[slot][row][i]  +-1 where an inner edge crosses the row's sample
-//                         line between pixels i-1 and i (prefix-summed per
-//                         pixel);
-//   sBorder[slot][row][i] fills: signed AA-border coverage, 1/65536 fixed
-//                         point; strokes: max coverage as float bits.
-// The wave first takes one primitive per lane (left-edge terms, and the
-// primitive's range of sample rows), then compacts the (primitive, row) work
-// items of the whole batch across the 64 lanes.  Paths without primitives in
-// the tile (uniform coverage from the corner winding) ride in the same batch
-// without a slot.  The integer predicates are those of a per-pixel
-// evaluation: crossing columns solve the same threshold inequality, and a
-// triangle's pixels on a row are the candidates of a float span estimate
-// (widened by a pixel) that pass the exact three-edge test.
-constexpr int kRasterLanes = 64;
-constexpr int kSlots = 4; // at most 16 (4 bits of sItemInfo)
-constexpr int kMaxEntries = 32;
-constexpr int kMaxItems = kRasterLanes * kTile;
+// The paths that touch the tile (primitives binned to it, or a nonzero
+// corner winding) are listed 256 at a time and cut into batches of at most
+// kBatchEntries paths, kAccArrays accumulator arrays and, unless the batch
+// has a single path with primitives, kBatchPrims primitives.  Each path with
+// primitives owns one LDS accumulator array, two when it has more than
+// kPackedPrims primitives in the tile:
+//   sLeft[a][row]     winding change along the tile's left edge, as a suffix
+//                     delta over rows;
+//   sAcc[a][row][i]   fills: +-1 where an inner edge crosses the row's sample
+//                     line between pixels i-1 and i, times 2^24, plus the
+//                     signed AA-border coverage in 1/65536 fixed point (a
+//                     two-array path keeps the crossings in sAcc[a] and the
+//                     border in sAcc[a + 1]); strokes: max coverage as float
+//                     bits.
+// With at most kPackedPrims primitives a pixel's crossing count stays within
+// +-127 and its border sum within +-2^23, so the packed form is exact.
+// Threads first take one primitive each (left-edge terms and the range of
+// sample rows), then the (primitive, row) work items are compacted across the
+// workgroup, fills before strokes.  A pass per (array, row) turns the
+// accumulators into coverage in place (prefix of the crossings, fill rule),
+// and every thread then resolves its pixel for each path of the batch.  The
+// integer predicates are those of a per-pixel evaluation: crossing columns
+// solve the same threshold inequality, and a triangle's pixels on a row are
+// the candidates of a float span estimate (widened by its error bound) that
+// pass the exact three-edge test.
+constexpr int kWgThreads = 256;
+constexpr int kAccArrays = 16;
+constexpr int kBatchEntries = 64; // one wave's ballot
+constexpr uint32_t kBatchPrims = kWgThreads;
+constexpr uint32_t kPackedPrims = 127;
+constexpr int kMaxItems = kBatchPrims * kTile;
 
 struct TileGeom
 {
@@ -1065,6 +1074,14 @@ __device__ __forceinline__ void tri_row_candidates(int32_t ax, int32_t ay, int32
     const float hi = (xr + eps - 128.f) * (1.f / 256.f);
     *ilo = (int)fmaxf(ceilf(lo), 0.f);
     *ihi = (int)fminf(floorf(hi), (float)(kTile - 1));
+}
+
+// A border triangle's barycentric coverage at a pixel inside it, in
+// 1/65536 fixed point, clamped to [0, 1] (it leaves that range only by
+// rounding), so that a sum over 127 triangles stays below 2^23.
+__device__ __forceinline__ int32_t border_fixed(float cv)
+{
+    return __float2int_rn(fminf(fmaxf(cv, 0.f), 1.f) * 65536.f);
 }
 
 // Exact "inside" test of an edge against pixel (i2, j2) with orientation o.
@@ -1142,7 +1159,7 @@ __device__ void fill_prim_left(const Prim& p, const TileGeom& g, int32_t* __rest
 // the two AA border triangles.
 template <typename I>
 __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* __restrict__ sDelta,
-                              int32_t* __restrict__ sBorder)
+                              int32_t* __restrict__ sBorder, int32_t deltaUnit)
 {
     const int32_t X0 = g.X0, Y0 = g.Y0;
     const int32_t j2 = 2 * j + 1;
@@ -1173,7 +1190,7 @@ __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* 
                         --q;
                     while (sa2 * q < r)
                         ++q;
-                    atomicAdd(&sDelta[j * kTile + q], neg * s);
+                    atomicAdd(&sDelta[j * kTile + q], neg * s * deltaUnit);
                 }
             }
         }
@@ -1190,16 +1207,16 @@ __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* 
             const EdgeK k01 = edge_k<I>(make_edge_t<I>(p.p[0], p.p[1], p.p[2], p.p[3], X0, Y0), o);
             const EdgeK k12 = edge_k<I>(make_edge_t<I>(p.p[2], p.p[3], p.p[4], p.p[5], X0, Y0), o);
             const EdgeK k20 = edge_k<I>(make_edge_t<I>(p.p[4], p.p[5], p.p[0], p.p[1], X0, Y0), o);
-            const float fA1 = (float)A1;
+            const float inv = __builtin_amdgcn_rcpf((float)A1);
+            const float D0 = (float)e.D0;
             for (int i = ilo; i <= ihi; ++i)
             {
                 const int32_t i2 = 2 * i + 1;
                 if (edge_in(k01, i2, j2) && edge_in(k12, i2, j2) && edge_in(k20, i2, j2))
                 {
                     const int32_t ke = e.A * i2 + e.B * j2;
-                    const float De = (float)e.D0 + 128.f * (float)ke;
-                    const float cv = 1.f + De / fA1;
-                    atomicAdd(&sBorder[j * kTile + i], neg * o * __float2int_rn(cv * 65536.f));
+                    const float De = D0 + 128.f * (float)ke;
+                    atomicAdd(&sBorder[j * kTile + i], neg * o * border_fixed(1.f + De * inv));
                 }
             }
         }
@@ -1217,16 +1234,16 @@ __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* 
             const EdgeTI<I> f12e = make_edge_t<I>(p.p[2], p.p[3], p.p[6], p.p[7], X0, Y0);
             const EdgeK f12 = edge_k(f12e, o);
             const EdgeK f20 = edge_k<I>(make_edge_t<I>(p.p[6], p.p[7], p.p[4], p.p[5], X0, Y0), o);
-            const float fA2 = (float)A2;
+            const float inv = __builtin_amdgcn_rcpf((float)A2);
+            const float D0 = (float)f12e.D0;
             for (int i = ilo; i <= ihi; ++i)
             {
                 const int32_t i2 = 2 * i + 1;
                 if (edge_in(f01, i2, j2) && edge_in(f12, i2, j2) && edge_in(f20, i2, j2))
                 {
                     const int32_t k12 = f12e.A * i2 + f12e.B * j2;
-                    const float D12 = (float)f12e.D0 + 128.f * (float)k12;
-                    const float cv = D12 / fA2;
-                    atomicAdd(&sBorder[j * kTile + i], neg * o * __float2int_rn(cv * 65536.f));
+                    const float D12 = D0 + 128.f * (float)k12;
+                    atomicAdd(&sBorder[j * kTile + i], neg * o * border_fixed(D12 * inv));
                 }
             }
         }
@@ -1254,7 +1271,7 @@ __device__ void stroke_prim_row(const Prim& p, int j, const TileGeom& g, uint32_
             const EdgeTI<I> e20 = make_edge_t<I>(p.p[4], p.p[5], p.p[0], p.p[1], X0, Y0);
             const int32_t t01 = inside_threshold_t<I>(e01, 1), t12 = inside_threshold_t<I>(e12, 1),
                           t20 = inside_threshold_t<I>(e20, 1);
-            const float inv = 1.f / (float)A1;
+            const float inv = __builtin_amdgcn_rcpf((float)A1);
             for (int i = ilo; i <= ihi; ++i)
             {
                 const int32_t i2 = 2 * i + 1;
@@ -1286,7 +1303,7 @@ __device__ void stroke_prim_row(const Prim& p, int j, const TileGeom& g, uint32_
             const EdgeTI<I> e32 = make_edge_t<I>(p.p[6], p.p[7], p.p[4], p.p[5], X0, Y0);
             const int32_t t21 = inside_threshold_t<I>(EdgeTI<I>{-e12.A, -e12.B, -e12.D0}, 1);
             const int32_t t13 = inside_threshold_t<I>(e13, 1), t32 = inside_threshold_t<I>(e32, 1);
-            const float inv = 1.f / (float)A2;
+            const float inv = __builtin_amdgcn_rcpf((float)A2);
             for (int i = ilo; i <= ihi; ++i)
             {
                 const int32_t i2 = 2 * i + 1;
@@ -1708,12 +1725,19 @@ __device__ __forceinline__ void resolve_pixel(const PathRaster& pr, const PathRa
     blend_store(src, col, noise, colPacked);
 }
 
-__device__ __forceinline__ uint32_t lane_read(uint32_t v, int lane)
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t v)
 {
-    return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
 
-__global__ __launch_bounds__(kRasterLanes) void raster_kernel(
+// Accumulator arrays a path with n primitives in the tile needs, packed with
+// a slot count of one: slots | arrays << 16.
+__device__ __forceinline__ uint32_t slot_arrays(uint32_t n)
+{
+    return n == 0 ? 0u : (1u | ((n > kPackedPrims ? 2u : 1u) << 16));
+}
+
+__global__ __launch_bounds__(kWgThreads) void raster_kernel(
     const PathCells* __restrict__ cells, const PathRaster* __restrict__ raster, uint32_t pathCount,
     const uint32_t* __restrict__ cellOffset, const int32_t* __restrict__ cellBackdrop,
     const uint32_t* __restrict__ binned, const Prim* __restrict__ prims,
@@ -1721,47 +1745,48 @@ __global__ __launch_bounds__(kRasterLanes) void raster_kernel(
     int height, int tilesX, uint32_t loadPreserve, uint32_t clearPremul, uint32_t dither,
     uint32_t* __restrict__ target)
 {
-    __shared__ __align__(16) int32_t sDelta[kSlots][kTile * kTile];
-    __shared__ __align__(16) int32_t sBorder[kSlots][kTile * kTile];
-    __shared__ int32_t sLeft[kSlots][kTile];
-    __shared__ int32_t sLeftPre[kSlots][kTile];
+    __shared__ __align__(16) int32_t sAcc[kAccArrays][kTile * kTile];
+    __shared__ int32_t sLeft[kAccArrays][kTile];
+    // The listed paths of the current window of 256 paths, in order, with
+    // exclusive prefixes of their primitive counts and of slot_arrays().
+    __shared__ uint32_t sRelPath[kWgThreads];
+    __shared__ uint32_t sRelOff[kWgThreads];
+    __shared__ uint32_t sRelN[kWgThreads];
+    __shared__ int32_t sRelC[kWgThreads];
+    __shared__ uint32_t sRelPrims[kWgThreads];
+    __shared__ uint32_t sRelSA[kWgThreads];
+    __shared__ uint32_t sWave[3][kWgThreads / 64];
+    __shared__ uint8_t sArrEntry[kAccArrays]; // batch entry whose first array is a, else 0xff
     __shared__ uint8_t sOwner[kMaxItems];
-    __shared__ uint32_t sItemPrim[kRasterLanes];
-    __shared__ uint32_t sItemInfo[kRasterLanes]; // slot | stroke << 4 | row0 << 8 | item0 << 16
+    __shared__ uint32_t sItemPrim[kWgThreads];
+    __shared__ uint32_t sItemInfo[kWgThreads]; // array | wide << 4 | stroke << 5 | row0 << 8 | item0 << 16
 
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
     const int tile = blockIdx.x;
     const int tx = tile % tilesX, ty = tile / tilesX;
-    const int lane = threadIdx.x;
-    const int pj = lane >> 2, pi0 = (lane & 3) * 4;
-    const int px0 = tx * kTile + pi0, py = ty * kTile + pj;
+    const int px = tx * kTile + (t & (kTile - 1)), py = ty * kTile + (t >> kTileLog2);
     TileGeom g;
     g.X0 = tx << (kSubpixelLog2 + kTileLog2);
     g.Y0 = ty << (kSubpixelLog2 + kTileLog2);
 
-    // Per pixel: the RGBA8 color (quantized after every path, as the
-    // reference's color buffer is), and the clip buffer as clip ID (low 16
-    // bits) with fp16 coverage (high 16 bits), like the reference's
-    // clipBuffer.
-    uint32_t col[4];
-    uint32_t clip[4] = {0u, 0u, 0u, 0u};
-    float noise[4];
-    const bool rowInside = py < height;
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-    {
-        uint32_t c = clearPremul;
-        if (loadPreserve && rowInside && px0 + q < width)
-            c = target[(size_t)py * width + px0 + q];
-        col[q] = c;
-        noise[q] = dither ? dither_noise(px0 + q, py) : 0.f;
-    }
+    // The pixel's RGBA8 color (quantized after every path, as the reference's
+    // color buffer is), and its clip buffer entry as clip ID (low 16 bits)
+    // with fp16 coverage (high 16 bits), like the reference's clipBuffer.
+    const bool inImage = px < width && py < height;
+    uint32_t col = clearPremul;
+    if (loadPreserve && inImage)
+        col = target[(size_t)py * width + px];
+    uint32_t clip = 0u;
+    const float noise = dither ? dither_noise(px, py) : 0.f;
 
-    for (uint32_t base = 1; base <= pathCount; base += kRasterLanes)
+    for (uint32_t base = 1; base <= pathCount; base += kWgThreads)
     {
-        const uint32_t p = base + (uint32_t)lane;
+        // ---- list the window's paths that touch the tile
+        const uint32_t p = base + (uint32_t)t;
         uint32_t off = 0, n = 0;
         int32_t C = 0;
-        bool relevant = false;
+        bool listed = false;
         if (p <= pathCount)
         {
             const PathCells c = cells[p];
@@ -1771,199 +1796,240 @@ __global__ __launch_bounds__(kRasterLanes) void raster_kernel(
                 off = cellOffset[cell];
                 n = cellOffset[cell + 1] - off;
                 C = cellBackdrop[cell];
-                relevant = n != 0 || C != 0;
+                listed = n != 0 || C != 0;
             }
         }
-        uint64_t mask = __ballot(relevant);
-        while (mask != 0)
+        const uint32_t vSA = slot_arrays(n);
+        uint32_t iSA = vSA, iP = n;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1)
         {
-            // ---- gather a batch.  Entry e lives in lane e's registers, slot s
-            //      in lane s's.
-            int nE = 0, nSlots = 0;
-            uint32_t nPrim = 0;
-            uint32_t ePath = 0, eSlot = 0xffu;
-            int32_t eC = 0;
-            uint32_t sOff = 0, sPrim0 = 0, sStroke = 0;
-            while (mask != 0 && nE < kMaxEntries)
+            const uint32_t a0 = (uint32_t)__shfl_up((int)iSA, d), a1 = (uint32_t)__shfl_up((int)iP, d);
+            if (lane >= d)
             {
-                const int b = __builtin_ctzll(mask);
-                const uint32_t nb = lane_read(n, b);
-                const uint32_t path = base + (uint32_t)b;
-                uint32_t slotOf = 0xffu;
-                if (nb != 0)
-                {
-                    if (nSlots == kSlots || (nSlots > 0 && nPrim + nb > (uint32_t)kRasterLanes))
-                        break;
-                    const uint32_t offb = lane_read(off, b);
-                    const uint32_t strokeb = raster[path].flags & 1u;
-                    if (lane == nSlots)
-                    {
-                        sOff = offb;
-                        sPrim0 = nPrim;
-                        sStroke = strokeb;
-                    }
-                    slotOf = (uint32_t)nSlots;
-                    ++nSlots;
-                    nPrim += nb;
-                }
-                const int32_t Cb = (int32_t)lane_read((uint32_t)C, b);
-                if (lane == nE)
-                {
-                    ePath = path;
-                    eSlot = slotOf;
-                    eC = Cb;
-                }
-                ++nE;
-                mask &= mask - 1;
+                iSA += a0;
+                iP += a1;
             }
-
-            // ---- primitives of the batch's slotted paths into LDS
-            if (nSlots > 0)
+        }
+        const uint64_t bal = __ballot(listed);
+        const uint32_t posInWave = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 63)
+        {
+            sWave[0][wave] = iSA;
+            sWave[1][wave] = iP;
+            sWave[2][wave] = (uint32_t)__popcll(bal);
+        }
+        __syncthreads();
+        uint32_t preSA = 0, preP = 0, preL = 0, nRel = 0;
+#pragma unroll
+        for (int w = 0; w < kWgThreads / 64; ++w)
+        {
+            if (w < wave)
             {
-                for (int q = lane; q < kSlots * kTile * kTile; q += kRasterLanes)
+                preSA += sWave[0][w];
+                preP += sWave[1][w];
+                preL += sWave[2][w];
+            }
+            nRel += sWave[2][w];
+        }
+        nRel = uniform_u32(nRel);
+        if (listed)
+        {
+            const uint32_t q = preL + posInWave;
+            sRelPath[q] = p;
+            sRelOff[q] = off;
+            sRelN[q] = n;
+            sRelC[q] = C;
+            sRelPrims[q] = preP + iP - n;
+            sRelSA[q] = preSA + iSA - vSA;
+        }
+        __syncthreads();
+
+        for (uint32_t s = 0; s < nRel;)
+        {
+            // ---- the batch: entries s .. s+len-1, cut where a limit breaks
+            const uint32_t baseP = uniform_u32(sRelPrims[s]), baseSA = uniform_u32(sRelSA[s]);
+            uint32_t len;
+            {
+                const uint32_t i = s + (uint32_t)lane;
+                bool stop = true;
+                if (i < nRel)
                 {
-                    (&sDelta[0][0])[q] = 0;
-                    (&sBorder[0][0])[q] = 0;
+                    const uint32_t ni = sRelN[i];
+                    const uint32_t sa = sRelSA[i] + slot_arrays(ni) - baseSA;
+                    const uint32_t primsIncl = sRelPrims[i] + ni - baseP;
+                    stop = (sa >> 16) > (uint32_t)kAccArrays || (primsIncl > kBatchPrims && (sa & 0xffffu) > 1u);
                 }
-                for (int q = lane; q < kSlots * kTile; q += kRasterLanes)
-                    (&sLeft[0][0])[q] = 0;
+                const uint64_t m = __ballot(stop);
+                len = uniform_u32(m ? (uint32_t)__builtin_ctzll(m) : (uint32_t)kBatchEntries);
+            }
+            const uint32_t last = s + len - 1;
+            const uint32_t nPrim = uniform_u32(sRelPrims[last] + sRelN[last] - baseP);
+            const uint32_t arraysUsed = uniform_u32((sRelSA[last] + slot_arrays(sRelN[last]) - baseSA) >> 16);
+
+            if (arraysUsed != 0)
+            {
+                for (int q = t; q < (int)arraysUsed * (kTile * kTile / 4); q += kWgThreads)
+                    reinterpret_cast<int4*>(&sAcc[0][0])[q] = make_int4(0, 0, 0, 0);
+                if (t < (int)arraysUsed * kTile)
+                    (&sLeft[0][0])[t] = 0;
+                if (t < kAccArrays)
+                    sArrEntry[t] = 0xffu;
                 __syncthreads();
-                for (uint32_t c0 = 0; c0 < nPrim; c0 += kRasterLanes)
+                if (t < (int)len && sRelN[s + t] != 0)
+                    sArrEntry[(sRelSA[s + t] - baseSA) >> 16] = (uint8_t)t;
+
+                for (uint32_t c0 = 0; c0 < nPrim; c0 += kBatchPrims)
                 {
-                    const uint32_t gp = c0 + (uint32_t)lane;
-                    int rows = 0, r0 = 0;
-                    uint32_t info = 0, primIdx = 0;
-                    // this lane's slot: the last one starting at or before gp
-                    int slot = 0;
-#pragma unroll
-                    for (int s = 1; s < kSlots; ++s)
-                        if (s < nSlots && lane_read(sPrim0, s) <= gp)
-                            slot = s;
-                    const uint32_t so = (uint32_t)__shfl((int)sOff, slot);
-                    const uint32_t sp0 = (uint32_t)__shfl((int)sPrim0, slot);
-                    const uint32_t stroke = (uint32_t)__shfl((int)sStroke, slot);
-                    if (gp < nPrim)
+                    // ---- one primitive per thread
+                    const uint32_t f = c0 + (uint32_t)t;
+                    uint32_t rowsF = 0, rowsS = 0, info = 0, primIdx = 0;
+                    if (f < nPrim)
                     {
-                        primIdx = binned[so + (gp - sp0)];
+                        // owning entry: the last one whose first primitive is at or before f
+                        uint32_t lo = s, hi = last;
+                        while (lo < hi)
+                        {
+                            const uint32_t mid = (lo + hi + 1) >> 1;
+                            if (sRelPrims[mid] - baseP <= f)
+                                lo = mid;
+                            else
+                                hi = mid - 1;
+                        }
+                        const uint32_t a = (sRelSA[lo] - baseSA) >> 16;
+                        const bool wide = sRelN[lo] > kPackedPrims;
+                        const bool stroke = raster[sRelPath[lo]].flags & 1u;
+                        primIdx = binned[sRelOff[lo] + (f - (sRelPrims[lo] - baseP))];
                         const Prim pm = prims[primIdx];
-                        int r1;
+                        int r0, r1;
                         prim_rows(pm, g.Y0, &r0, &r1);
-                        rows = max(r1 - r0 + 1, 0);
-                        if (!stroke)
-                            fill_prim_left(pm, g, sLeft[slot]);
-                        info = (uint32_t)slot | (stroke << 4) | ((uint32_t)r0 << 8);
-                    }
-                    // exclusive prefix of the row counts across the wave
-                    int incl = rows;
-#pragma unroll
-                    for (int d = 1; d < kRasterLanes; d <<= 1)
-                    {
-                        const int v = __shfl_up(incl, d);
-                        if (lane >= d)
-                            incl += v;
-                    }
-                    const int item0 = incl - rows;
-                    const int total = __shfl(incl, kRasterLanes - 1);
-                    for (int k = 0; k < rows; ++k)
-                        sOwner[item0 + k] = (uint8_t)lane;
-                    sItemPrim[lane] = primIdx;
-                    sItemInfo[lane] = info | ((uint32_t)item0 << 16);
-                    __syncthreads();
-                    for (int t = lane; t < total; t += kRasterLanes)
-                    {
-                        const int owner = sOwner[t];
-                        const uint32_t inf = sItemInfo[owner];
-                        const int slot = (int)(inf & 15u);
-                        const int row = (int)((inf >> 8) & 0xffu) + (t - (int)(inf >> 16));
-                        const Prim pm = prims[sItemPrim[owner]];
-                        if (inf & 16u)
-                            stroke_prim_row<int64_t>(pm, row, g, reinterpret_cast<uint32_t*>(sBorder[slot]));
+                        const uint32_t rows = (uint32_t)max(r1 - r0 + 1, 0);
+                        if (stroke)
+                            rowsS = rows;
                         else
-                            fill_prim_row<int64_t>(pm, row, g, sDelta[slot], sBorder[slot]);
+                        {
+                            rowsF = rows;
+                            fill_prim_left(pm, g, sLeft[a]);
+                        }
+                        info = a | (wide ? 16u : 0u) | (stroke ? 32u : 0u) | ((uint32_t)min(r0, kTile - 1) << 8);
+                    }
+                    // exclusive prefix of the row counts, fills then strokes
+                    const uint32_t v = rowsF | (rowsS << 16);
+                    uint32_t inc = v;
+#pragma unroll
+                    for (int d = 1; d < 64; d <<= 1)
+                    {
+                        const uint32_t a0 = (uint32_t)__shfl_up((int)inc, d);
+                        if (lane >= d)
+                            inc += a0;
+                    }
+                    if (lane == 63)
+                        sWave[0][wave] = inc;
+                    __syncthreads();
+                    uint32_t pre = 0, tot = 0;
+#pragma unroll
+                    for (int w = 0; w < kWgThreads / 64; ++w)
+                    {
+                        const uint32_t x = sWave[0][w];
+                        if (w < wave)
+                            pre += x;
+                        tot += x;
+                    }
+                    const uint32_t ex = pre + inc - v;
+                    const uint32_t totF = tot & 0xffffu, total = uniform_u32(totF + (tot >> 16));
+                    const uint32_t item0 = rowsS != 0 ? totF + (ex >> 16) : (ex & 0xffffu);
+                    for (uint32_t k = 0; k < rowsF + rowsS; ++k)
+                        sOwner[item0 + k] = (uint8_t)t;
+                    sItemPrim[t] = primIdx;
+                    sItemInfo[t] = info | (item0 << 16);
+                    __syncthreads();
+
+                    // ---- (primitive, row) items
+                    for (uint32_t it = (uint32_t)t; it < total; it += kWgThreads)
+                    {
+                        const uint32_t owner = sOwner[it];
+                        const uint32_t inf = sItemInfo[owner];
+                        const int a = (int)(inf & 15u);
+                        const int row = (int)((inf >> 8) & 15u) + (int)(it - (inf >> 16));
+                        const Prim pm = prims[sItemPrim[owner]];
+                        if (inf & 32u)
+                            stroke_prim_row<int64_t>(pm, row, g, reinterpret_cast<uint32_t*>(sAcc[a]));
+                        else if (inf & 16u)
+                            fill_prim_row<int64_t>(pm, row, g, sAcc[a], sAcc[a + 1], 1);
+                        else
+                            fill_prim_row<int64_t>(pm, row, g, sAcc[a], sAcc[a], 1 << 24);
                     }
                     __syncthreads();
                 }
-                // row prefix of the left-edge deltas
-                for (int q = lane; q < nSlots * kTile; q += kRasterLanes)
+
+                // ---- accumulators -> coverage, one thread per (array, row)
                 {
-                    const int slot = q >> kTileLog2, row = q & (kTile - 1);
-                    int32_t acc = 0;
-                    for (int r = 0; r <= row; ++r)
-                        acc += sLeft[slot][r];
-                    sLeftPre[slot][row] = acc;
+                    const int a = t >> kTileLog2, row = t & (kTile - 1);
+                    const uint32_t e = sArrEntry[a];
+                    if (e != 0xffu)
+                    {
+                        const uint32_t i = s + e;
+                        const uint32_t flags = raster[sRelPath[i]].flags;
+                        int32_t* accRow = sAcc[a] + row * kTile;
+                        if (flags & 1u)
+                        {
+#pragma unroll
+                            for (int k = 0; k < kTile; ++k)
+                                accRow[k] = __float_as_int(fminf(__int_as_float(accRow[k]), 1.f));
+                        }
+                        else
+                        {
+                            const bool wide = sRelN[i] > kPackedPrims;
+                            int32_t w = sRelC[i];
+                            for (int r = 0; r <= row; ++r)
+                                w += sLeft[a][r];
+                            const int32_t* bRow = wide ? sAcc[a + 1] + row * kTile : accRow;
+#pragma unroll
+                            for (int k = 0; k < kTile; ++k)
+                            {
+                                const int32_t v = accRow[k];
+                                int32_t d, b;
+                                if (wide)
+                                {
+                                    d = v;
+                                    b = bRow[k];
+                                }
+                                else
+                                {
+                                    d = (v + (1 << 23)) >> 24;
+                                    b = v - (int32_t)((uint32_t)d << 24);
+                                }
+                                w += d;
+                                accRow[k] = __float_as_int(
+                                    fill_rule_coverage((float)w + (float)b * (1.f / 65536.f), flags));
+                            }
+                        }
+                    }
                 }
                 __syncthreads();
             }
 
             // ---- resolve the batch in submission order
-            for (int e = 0; e < nE; ++e)
+            for (uint32_t e = 0; e < len; ++e)
             {
-                const uint32_t path = lane_read(ePath, e);
-                const uint32_t slot = lane_read(eSlot, e);
-                const int32_t pC = (int32_t)lane_read((uint32_t)eC, e);
+                const uint32_t i = s + e;
+                const uint32_t path = uniform_u32(sRelPath[i]);
+                const uint32_t ni = uniform_u32(sRelN[i]);
                 const PathRaster pr = raster[path];
-                const bool isStroke = pr.flags & 1u;
-                float cov[4];
-                if (slot == 0xffu)
-                {
-                    // No edge in the tile: the coverage is the corner winding's.
-                    const float u = isStroke ? 0.f : fill_rule_coverage((float)pC, pr.flags);
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        cov[q] = u;
-                }
+                float cov;
+                if (ni != 0)
+                    cov = __int_as_float(sAcc[uniform_u32((sRelSA[i] - baseSA) >> 16)][t]);
                 else
-                {
-                    const int4 bv = reinterpret_cast<const int4*>(sBorder[slot])[lane];
-                    if (isStroke)
-                    {
-                        cov[0] = fminf(__int_as_float(bv.x), 1.f);
-                        cov[1] = fminf(__int_as_float(bv.y), 1.f);
-                        cov[2] = fminf(__int_as_float(bv.z), 1.f);
-                        cov[3] = fminf(__int_as_float(bv.w), 1.f);
-                    }
-                    else
-                    {
-                        const int4* rd = reinterpret_cast<const int4*>(sDelta[slot] + pj * kTile);
-                        int32_t acc = pC + sLeftPre[slot][pj];
-                        for (int gq = 0; gq < (lane & 3); ++gq)
-                        {
-                            const int4 v = rd[gq];
-                            acc += v.x + v.y + v.z + v.w;
-                        }
-                        const int4 v = rd[lane & 3];
-                        const int32_t w0 = acc + v.x, w1 = w0 + v.y, w2 = w1 + v.z, w3 = w2 + v.w;
-                        cov[0] = fill_rule_coverage((float)w0 + (float)bv.x * (1.f / 65536.f), pr.flags);
-                        cov[1] = fill_rule_coverage((float)w1 + (float)bv.y * (1.f / 65536.f), pr.flags);
-                        cov[2] = fill_rule_coverage((float)w2 + (float)bv.z * (1.f / 65536.f), pr.flags);
-                        cov[3] = fill_rule_coverage((float)w3 + (float)bv.w * (1.f / 65536.f), pr.flags);
-                    }
-                }
-                const bool clipUpdate = pr.flags & 8u;
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (cov[q] != 0.f || clipUpdate)
-                        resolve_pixel(pr, raster + path, cov[q], px0 + q, py, noise[q], ramps, gradHeight, images,
-                                      col[q], clip[q]);
+                    cov = (pr.flags & 1u) ? 0.f : fill_rule_coverage((float)sRelC[i], pr.flags);
+                if (cov != 0.f || (pr.flags & 8u))
+                    resolve_pixel(pr, raster + path, cov, px, py, noise, ramps, gradHeight, images, col, clip);
             }
-            if (nSlots > 0)
-                __syncthreads(); // the next batch clears the slots
+            __syncthreads(); // the next batch clears the arrays and rewrites sArrEntry
+            s += len;
         }
     }
-    if (rowInside)
-    {
-        const uint32_t* packed = col;
-        uint32_t* row = target + (size_t)py * width;
-        if (px0 + 3 < width && ((((size_t)py * width + px0) & 3) == 0))
-            *reinterpret_cast<uint4*>(row + px0) = make_uint4(packed[0], packed[1], packed[2], packed[3]);
-        else
-        {
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (px0 + q < width)
-                    row[px0 + q] = packed[q];
-        }
-    }
+    if (inImage)
+        target[(size_t)py * width + px] = col;
 }
 
 // Unpacks PathData/PaintData/PaintAux into PathRaster (gpu.cpp:859-1063,
