@@ -72,21 +72,22 @@ constexpr uint32_t kPrimNegate = 4u;
 constexpr uint32_t kPrimEmpty = 8u;
 
 // Per-path raster parameters, unpacked from PathData/PaintData/PaintAux.
+// The fields the src-over resolve reads come first.
 struct PathRaster
 {
-    uint32_t flags;      // bit0 stroke, bit1 evenOdd, bit2 clockwise, bit3 clipUpdate
+    uint32_t flags;      // bit0 stroke, bit1 evenOdd, bit2 clockwise, bit3 clipUpdate, bits 8-11 blend mode
     uint32_t paintType;
     uint32_t clipID;     // draws: clip to read (0 none); clip updates: ID to write
     uint32_t outerClipID;
-    float color[4];      // premultiplied solid color
-    float paintMatrix[6];
+    float color255[4];   // premultiplied solid color, 0..255 scale
+    float alpha;         // solid color alpha, 0..1
     float gradSpan;      // > 1: complex row; else simple-ramp texel offset
     uint32_t gradRow;
     uint32_t image;      // 0 none, else 1-based index into the flush's image table
+    float paintMatrix[6];
     uint32_t imageSampler; // ImageSampler key: wrapX + 3*wrapY + 9*filter
     float imageMatrix[6];  // pixel -> unit uv (PaintAuxData image matrix)
     float imageLod;
-    uint32_t pad;
 };
 static_assert(sizeof(PathRaster) == 104, "PathRaster layout");
 

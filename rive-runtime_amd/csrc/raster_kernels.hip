@@ -985,12 +985,6 @@ __device__ __forceinline__ float dither_noise(int x, int y)
     float v2 = w - floorf(w);
     return v2 * (1.f / 256.f) + (-.5f / 256.f);
 }
-__device__ __forceinline__ float quantize8(float v)
-{
-    v = fminf(fmaxf(v, 0.f), 1.f);
-    return floorf(v * 255.f + .5f) * (1.f / 255.f);
-}
-
 // One 256-thread workgroup (four waves) per 16x16 tile; thread t owns pixel
 // (t % 16, t / 16).  Paths are visited in submission order and resolved per
 // pixel in that order.
@@ -1551,44 +1545,19 @@ __device__ void advanced_blend_coeffs_d(const float* srcIn, const float* dp, uin
     }
 }
 
-// Clip, paint and src-over of one path into one pixel (draw_path.vert:431-545,
-// draw_raster_order_path.frag:84-238).  A zero coverage leaves the pixel
-// unchanged, so callers skip it (except for clip updates).
-__device__ __forceinline__ float clip_coverage(uint32_t clip) { return __half2float(__ushort_as_half((unsigned short)(clip >> 16))); }
-
-// Paint, blend and store of one path's color into one pixel's RGBA8, after
-// clipping: src-over with solid or gradient paints inline; advanced blend
-// modes and image paints out of line (rare, and large).
-__device__ __forceinline__ void blend_store(const float* src, const float* col, float noise, uint32_t& colPacked)
+// Four 0..255 channel values to RGBA8: round to nearest even, saturating.
+__device__ __forceinline__ uint32_t pack_unorm8x4(const float* v)
 {
-    const float paintAlpha = src[3];
-    float out[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        out[k] = src[k] + col[k] * (1.f - paintAlpha);
-    if (paintAlpha != 0.f)
-    {
-        out[0] += noise;
-        out[1] += noise;
-        out[2] += noise;
-    }
-    uint32_t packed = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        packed |= (uint32_t)floorf(fminf(fmaxf(out[k], 0.f), 1.f) * 255.f + .5f) << (8 * k);
-    colPacked = packed;
+    uint32_t p = __builtin_amdgcn_cvt_pk_u8_f32(v[0], 0u, 0u);
+    p = __builtin_amdgcn_cvt_pk_u8_f32(v[1], 1u, p);
+    p = __builtin_amdgcn_cvt_pk_u8_f32(v[2], 2u, p);
+    return __builtin_amdgcn_cvt_pk_u8_f32(v[3], 3u, p);
 }
 
-__device__ __forceinline__ void unpack_color(uint32_t c, float* col)
-{
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        col[k] = float((c >> (8 * k)) & 0xffu) * (1.f / 255.f);
-}
-
-// Gradient paint color, unpremultiplied (draw_path.vert:285-331,450-476).
-__device__ __forceinline__ void sample_gradient(const PathRaster& pr, int px, int py,
-                                                const uint32_t* __restrict__ ramps, uint32_t gradHeight, float* src)
+// Gradient paint color, unpremultiplied, 0..255 scale (draw_path.vert:285-331,450-476).
+__device__ __forceinline__ void sample_gradient255(const PathRaster& pr, int px, int py,
+                                                   const uint32_t* __restrict__ ramps, uint32_t gradHeight,
+                                                   float* src)
 {
     const float fx = float(px) + .5f, fy = float(py) + .5f;
     const float pcx = pr.paintMatrix[0] * fx + pr.paintMatrix[2] * fy + pr.paintMatrix[4];
@@ -1605,28 +1574,31 @@ __device__ __forceinline__ void sample_gradient(const PathRaster& pr, int px, in
 #pragma unroll
     for (int k = 0; k < 4; ++k)
     {
-        const float va = float((ta >> (8 * k)) & 0xffu) * (1.f / 255.f);
-        const float vb = float((tb >> (8 * k)) & 0xffu) * (1.f / 255.f);
+        const float va = float((ta >> (8 * k)) & 0xffu);
+        const float vb = float((tb >> (8 * k)) & 0xffu);
         src[k] = (vb - va) * fr + va;
     }
 }
 
 // Advanced blend modes and image paints (draw_path.vert:431-545,
-// draw_raster_order_path.frag:191-204).
+// draw_raster_order_path.frag:191-204), in 0..1 scale.
 __device__ __noinline__ void resolve_general(const PathRaster* __restrict__ prp, float coverage, int px, int py,
-                                             float noise, const uint32_t* __restrict__ ramps, uint32_t gradHeight,
-                                             const ImageDesc* __restrict__ images, uint32_t* colPacked)
+                                             float noise255, const uint32_t* __restrict__ ramps,
+                                             uint32_t gradHeight, const ImageDesc* __restrict__ images,
+                                             uint32_t* colPacked)
 {
     const PathRaster pr = *prp;
     float col[4];
-    unpack_color(*colPacked, col);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        col[k] = float((*colPacked >> (8 * k)) & 0xffu) * (1.f / 255.f);
     float src[4];
     const uint32_t blendMode = (pr.flags >> 8) & 0xfu;
     if (pr.paintType == SOLID_COLOR_PAINT_TYPE && blendMode == 0u)
     {
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            src[k] = pr.color[k] * coverage;
+            src[k] = pr.color255[k] * (1.f / 255.f) * coverage;
     }
     else if (pr.paintType == SOLID_COLOR_PAINT_TYPE)
     {
@@ -1637,7 +1609,10 @@ __device__ __noinline__ void resolve_general(const PathRaster* __restrict__ prp,
     }
     else
     {
-        sample_gradient(pr, px, py, ramps, gradHeight, src);
+        sample_gradient255(pr, px, py, ramps, gradHeight, src);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            src[k] *= 1.f / 255.f;
         src[3] *= coverage;
         if (blendMode == 0u)
         {
@@ -1676,16 +1651,30 @@ __device__ __noinline__ void resolve_general(const PathRaster* __restrict__ prp,
         for (int k = 0; k < 3; ++k)
             src[k] = ((coeffs[k] - src[k]) * col[3] + src[k]) * src[3];
     }
-    uint32_t packed;
-    blend_store(src, col, noise, packed);
-    *colPacked = packed;
+    const float paintAlpha = src[3];
+    float out[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        out[k] = (src[k] + col[k] * (1.f - paintAlpha)) * 255.f;
+    if (paintAlpha != 0.f)
+    {
+        out[0] += noise255;
+        out[1] += noise255;
+        out[2] += noise255;
+    }
+    *colPacked = pack_unorm8x4(out);
 }
 
 // Clip, paint and src-over of one path into one pixel (draw_path.vert:431-545,
-// draw_raster_order_path.frag:84-238).  A zero coverage leaves the pixel
-// unchanged, so callers skip it (except for clip updates).
+// draw_raster_order_path.frag:84-238), in 0..255 scale.  A zero coverage
+// leaves the pixel unchanged, so callers skip it (except for clip updates).
+__device__ __forceinline__ float clip_coverage(uint32_t clip)
+{
+    return __half2float(__ushort_as_half((unsigned short)(clip >> 16)));
+}
+
 __device__ __forceinline__ void resolve_pixel(const PathRaster& pr, const PathRaster* __restrict__ prp,
-                                              float coverage, int px, int py, float noise,
+                                              float coverage, int px, int py, float noise255,
                                               const uint32_t* __restrict__ ramps, uint32_t gradHeight,
                                               const ImageDesc* __restrict__ images, uint32_t& colPacked,
                                               uint32_t& clip)
@@ -1702,27 +1691,39 @@ __device__ __forceinline__ void resolve_pixel(const PathRaster& pr, const PathRa
         coverage = clipID == pr.clipID ? fminf(clip_coverage(clip), coverage) : 0.f;
     if ((pr.flags & 0xf00u) != 0u || pr.image != 0u)
     {
-        resolve_general(prp, coverage, px, py, noise, ramps, gradHeight, images, &colPacked);
+        resolve_general(prp, coverage, px, py, noise255, ramps, gradHeight, images, &colPacked);
         return;
     }
-    float col[4];
-    unpack_color(colPacked, col);
-    float src[4];
+    float src[4], a;
     if (pr.paintType == SOLID_COLOR_PAINT_TYPE)
     {
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            src[k] = pr.color[k] * coverage;
+            src[k] = pr.color255[k] * coverage;
+        a = pr.alpha * coverage;
     }
     else
     {
-        sample_gradient(pr, px, py, ramps, gradHeight, src);
+        sample_gradient255(pr, px, py, ramps, gradHeight, src);
         src[3] *= coverage;
-        src[0] *= src[3];
-        src[1] *= src[3];
-        src[2] *= src[3];
+        a = src[3] * (1.f / 255.f);
+        src[0] *= a;
+        src[1] *= a;
+        src[2] *= a;
     }
-    blend_store(src, col, noise, colPacked);
+    const float ia = 1.f - a;
+    const uint32_t c = colPacked;
+    float out[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        out[k] = src[k] + float((c >> (8 * k)) & 0xffu) * ia;
+    if (a != 0.f)
+    {
+        out[0] += noise255;
+        out[1] += noise255;
+        out[2] += noise255;
+    }
+    colPacked = pack_unorm8x4(out);
 }
 
 __device__ __forceinline__ uint32_t uniform_u32(uint32_t v)
@@ -1778,7 +1779,7 @@ __global__ __launch_bounds__(kWgThreads) void raster_kernel(
     if (loadPreserve && inImage)
         col = target[(size_t)py * width + px];
     uint32_t clip = 0u;
-    const float noise = dither ? dither_noise(px, py) : 0.f;
+    const float noise255 = dither ? dither_noise(px, py) * 255.f : 0.f;
 
     for (uint32_t base = 1; base <= pathCount; base += kWgThreads)
     {
@@ -2022,7 +2023,7 @@ __global__ __launch_bounds__(kWgThreads) void raster_kernel(
                 else
                     cov = (pr.flags & 1u) ? 0.f : fill_rule_coverage((float)sRelC[i], pr.flags);
                 if (cov != 0.f || (pr.flags & 8u))
-                    resolve_pixel(pr, raster + path, cov, px, py, noise, ramps, gradHeight, images, col, clip);
+                    resolve_pixel(pr, raster + path, cov, px, py, noise255, ramps, gradHeight, images, col, clip);
             }
             __syncthreads(); // the next batch clears the arrays and rewrites sArrEntry
             s += len;
@@ -2068,11 +2069,12 @@ __global__ void path_raster_kernel(const PathDataV* __restrict__ paths, const ui
     if (paintType != CLIP_UPDATE_PAINT_TYPE)
         flags |= ((paint.x >> 4) & 0xfu) << 8; // shader blend mode
     r.flags = flags;
-    float a = float(paint.y >> 24) * (1.f / 255.f);
-    r.color[0] = float(paint.y & 0xffu) * (1.f / 255.f) * a;
-    r.color[1] = float((paint.y >> 8) & 0xffu) * (1.f / 255.f) * a;
-    r.color[2] = float((paint.y >> 16) & 0xffu) * (1.f / 255.f) * a;
-    r.color[3] = a;
+    const float a = float(paint.y >> 24) * (1.f / 255.f);
+    r.color255[0] = float(paint.y & 0xffu) * a;
+    r.color255[1] = float((paint.y >> 8) & 0xffu) * a;
+    r.color255[2] = float((paint.y >> 16) & 0xffu) * a;
+    r.color255[3] = float(paint.y >> 24);
+    r.alpha = a;
 #pragma unroll
     for (int k = 0; k < 6; ++k)
         r.paintMatrix[k] = ax[k];
@@ -2095,7 +2097,6 @@ __global__ void path_raster_kernel(const PathDataV* __restrict__ paths, const ui
     for (int k = 0; k < 6; ++k)
         r.imageMatrix[k] = ax[16 + k];
     r.imageLod = ax[22];
-    r.pad = 0;
     out[i] = r;
 }
 
