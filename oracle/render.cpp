@@ -25,6 +25,7 @@ constexpr float AA_RADIUS = .5f;
 constexpr uint32_t SOLID_COLOR_PAINT_TYPE = 1, LINEAR_GRADIENT_PAINT_TYPE = 2,
                    RADIAL_GRADIENT_PAINT_TYPE = 3, CLIP_UPDATE_PAINT_TYPE = 0;
 constexpr uint32_t PAINT_FLAG_NON_ZERO_FILL = 0x100u, PAINT_FLAG_EVEN_ODD_FILL = 0x200u;
+constexpr uint32_t PAINT_FLAG_HAS_CLIP_RECT = 0x400u;
 constexpr uint32_t GRAD_SPAN_FLAG_LEFT_BORDER = 0x80000000u;
 constexpr uint32_t GRAD_SPAN_FLAG_RIGHT_BORDER = 0x40000000u;
 constexpr uint32_t GRAD_SPAN_FLAG_COMPLEX_BORDER = 0x20000000u;
@@ -76,18 +77,20 @@ Mat mul(const Mat& a, const Mat& b) // a * b: apply b first
     r.m[5] = a.m[1] * b.m[4] + a.m[3] * b.m[5] + a.m[5];
     return r;
 }
+// Mat2D::invert (src/math/mat2d.cpp:179-200), in float.
 bool invert(const Mat& a, Mat* out)
 {
-    double det = (double)a.m[0] * a.m[3] - (double)a.m[1] * a.m[2];
-    if (det == 0 || !std::isfinite(det))
+    const float aa = a.m[0], ab = a.m[1], ac = a.m[2], ad = a.m[3], atx = a.m[4], aty = a.m[5];
+    float det = aa * ad - ab * ac;
+    if (det == 0.0f)
         return false;
-    double inv = 1 / det;
-    out->m[0] = (float)(a.m[3] * inv);
-    out->m[1] = (float)(-a.m[1] * inv);
-    out->m[2] = (float)(-a.m[2] * inv);
-    out->m[3] = (float)(a.m[0] * inv);
-    out->m[4] = (float)((a.m[2] * (double)a.m[5] - a.m[3] * (double)a.m[4]) * inv);
-    out->m[5] = (float)((a.m[1] * (double)a.m[4] - a.m[0] * (double)a.m[5]) * inv);
+    det = 1.0f / det;
+    out->m[0] = ad * det;
+    out->m[1] = -ab * det;
+    out->m[2] = -ac * det;
+    out->m[3] = aa * det;
+    out->m[4] = (ac * aty - ad * atx) * det;
+    out->m[5] = (ab * atx - aa * aty) * det;
     return true;
 }
 V2 map(const Mat& m, V2 p)
@@ -122,6 +125,112 @@ IAABB map_bounds_round_out(const Mat& m, const V2* pts, uint32_t n, float outset
     t -= outsetY;
     b += outsetY;
     return {(int)floorf(l), (int)floorf(t), (int)ceilf(r), (int)ceilf(b)};
+}
+
+// RiveRenderer::IsAABB (rive_renderer.cpp:16-55): a move and three lines, any
+// further points equal to the first, opposite corners sharing x and y.
+bool is_aabb(const uint8_t* verbs, uint32_t verbCount, const V2* pts, uint32_t ptCount, float rect[4])
+{
+    if (verbCount < 4 || verbs[0] != kMove || verbs[1] != kLine || verbs[2] != kLine || verbs[3] != kLine)
+        return false;
+    for (uint32_t i = 4; i < ptCount; ++i)
+        if (!(pts[i].x == pts[0].x && pts[i].y == pts[0].y))
+            return false;
+    const float c[4] = {pts[0].x, pts[0].y, pts[2].x, pts[2].y};
+    const float o[4] = {pts[1].x, pts[1].y, pts[3].x, pts[3].y};
+    const bool zyxw = c[0] == o[2] && c[1] == o[1] && c[2] == o[0] && c[3] == o[3];
+    const bool xwzy = c[0] == o[0] && c[1] == o[3] && c[2] == o[2] && c[3] == o[1];
+    if (!zyxw && !xwzy)
+        return false;
+    rect[0] = std::min(c[0], c[2]);
+    rect[1] = std::min(c[1], c[3]);
+    rect[2] = std::max(c[0], c[2]);
+    rect[3] = std::max(c[1], c[3]);
+    return true;
+}
+
+// transform_rect_to_new_space (rive_renderer.cpp:235-265).
+bool transform_rect_to_new_space(float rect[4], const Mat& current, const Mat& newMatrix)
+{
+    if (memcmp(current.m, newMatrix.m, sizeof(current.m)) == 0)
+        return true;
+    Mat toNew;
+    if (!invert(newMatrix, &toNew))
+        return false;
+    toNew = mul(toNew, current);
+    const float maxSkew = fmaxf(fabsf(toNew.m[1]), fabsf(toNew.m[2]));
+    const float maxScale = fmaxf(fabsf(toNew.m[0]), fabsf(toNew.m[3]));
+    if (maxSkew > 1.0e-5f && maxScale > 1.0e-5f) // math::EPSILON
+        return false;
+    const V2 a = map(toNew, {rect[0], rect[1]}), b = map(toNew, {rect[2], rect[3]});
+    rect[0] = std::min(a.x, b.x);
+    rect[1] = std::min(a.y, b.y);
+    rect[2] = std::max(a.x, b.x);
+    rect[3] = std::max(a.y, b.y);
+    return true;
+}
+
+// A render state's clip rect (clipRectImpl, rive_renderer.cpp:267-321) and
+// its ClipRectInverseMatrix (gpu.cpp:689-706).
+struct ClipRectState
+{
+    bool has = false;
+    float rect[4] = {0, 0, 0, 0};
+    Mat matrix;
+    void add(const float r[4], const Mat& m)
+    {
+        if (!has)
+        {
+            memcpy(rect, r, sizeof(rect));
+            matrix = m;
+            has = true;
+            return;
+        }
+        rect[0] = std::max(rect[0], r[0]);
+        rect[1] = std::max(rect[1], r[1]);
+        rect[2] = std::min(rect[2], r[2]);
+        rect[3] = std::min(rect[3], r[3]);
+    }
+    IAABB pixelBounds() const
+    {
+        const V2 corners[4] = {{rect[0], rect[1]}, {rect[2], rect[1]}, {rect[2], rect[3]}, {rect[0], rect[3]}};
+        return map_bounds_round_out(matrix, corners, 4, 0, 0);
+    }
+    bool inverse(Mat* out) const
+    {
+        if (!has)
+            return false;
+        const float w = rect[2] - rect[0], h = rect[3] - rect[1];
+        Mat toRect;
+        toRect.m[0] = w * .5f;
+        toRect.m[1] = 0;
+        toRect.m[2] = 0;
+        toRect.m[3] = h * .5f;
+        toRect.m[4] = (rect[0] + rect[2]) * .5f;
+        toRect.m[5] = (rect[1] + rect[3]) * .5f;
+        if (w <= 0 || h <= 0 || !invert(mul(matrix, toRect), out))
+            *out = Mat{{0, 0, 0, 0, 0, 0}};
+        return true;
+    }
+};
+
+// find_clip_rect_coverage_distances (common.glsl:376-400) at the pixel
+// center, then min_component of the half4 (draw_raster_order_path.frag:152-159).
+float clip_rect_coverage(const float m[6], int x, int y)
+{
+    const float awx = fabsf(m[0]) + fabsf(m[2]), awy = fabsf(m[1]) + fabsf(m[3]);
+    float d;
+    if (awx != 0 && awy != 0)
+    {
+        const float rx = 1.f / awx, ry = 1.f / awy;
+        const float fx = float(x) + .5f, fy = float(y) + .5f;
+        const float cx = m[0] * fx + m[2] * fy + m[4], cy = m[1] * fx + m[3] * fy + m[5];
+        d = std::min(std::min(cx * rx + rx + .5f, cy * ry + ry + .5f),
+                     std::min(-cx * rx + rx + .5f, -cy * ry + ry + .5f));
+    }
+    else
+        d = std::min(m[4], m[5]);
+    return round_to_f16(d);
 }
 } // namespace
 
@@ -264,18 +373,19 @@ struct Builder
     void pushPathDraw(uint32_t pathIdx, const Mat& matrix, float strokeRadius, Join join,
                       Cap cap, FillRule fillRule, uint32_t paintType, uint32_t paintValue,
                       uint32_t clipID, uint32_t outerClipID, const Gradient* grad,
-                      uint32_t blendMode = BLEND_SRC_OVER)
+                      uint32_t blendMode = BLEND_SRC_OVER, const Mat* clipRectInverse = nullptr)
     {
         const rive_stream_path& sp = s.paths[pathIdx];
         pushGeometryDraw(s.verbs + sp.first_verb, sp.verb_count, pathPts(pathIdx), sp.point_count, matrix,
                          strokeRadius, join, cap, fillRule, paintType, paintValue, clipID, outerClipID, grad,
-                         blendMode, 0, 0);
+                         blendMode, 0, 0, clipRectInverse);
     }
 
     void pushGeometryDraw(const uint8_t* verbs, uint32_t verbCount, const V2* pts, uint32_t ptCount,
                           const Mat& matrix, float strokeRadius, Join join, Cap cap, FillRule fillRule,
                           uint32_t paintType, uint32_t paintValue, uint32_t clipID, uint32_t outerClipID,
-                          const Gradient* grad, uint32_t blendMode, uint32_t image, uint32_t sampler)
+                          const Gradient* grad, uint32_t blendMode, uint32_t image, uint32_t sampler,
+                          const Mat* clipRectInverse = nullptr)
     {
         PathDrawDesc d;
         d.verbs = verbs;
@@ -338,10 +448,24 @@ struct Builder
             aux.image_lod = log2f(std::max(maxScaleFactorPow2, 1.f)) * .5f + -.5f; // MIP_MAP_LOD_BIAS
             f.pathImage[pathID] = image | sampler << 16;
         }
+        // PaintAuxData clip rect (gpu.cpp:1044-1062): the inverse matrix and
+        // inverse fwidth, or WideOpen {0, 0, 0, 0, 1, 1} without a clip rect.
+        if (clipRectInverse != nullptr)
+        {
+            params |= PAINT_FLAG_HAS_CLIP_RECT;
+            memcpy(aux.clip_rect_inverse_matrix, clipRectInverse->m, sizeof(aux.clip_rect_inverse_matrix));
+            const float* cm = clipRectInverse->m;
+            aux.inverse_fwidth[0] = -1.f / (fabsf(cm[0]) + fabsf(cm[1]));
+            aux.inverse_fwidth[1] = -1.f / (fabsf(cm[2]) + fabsf(cm[3]));
+        }
+        else
+        {
+            const float wideOpen[6] = {0, 0, 0, 0, 1, 1};
+            memcpy(aux.clip_rect_inverse_matrix, wideOpen, sizeof(wideOpen));
+        }
         pp.params = params;
         f.paintData.push_back(pp);
 
-        // clip rect: WideOpen (no clip rect) -> identity-ish, inverseFwidth 0
         if (grad != nullptr)
         {
             Mat inv;
@@ -563,6 +687,7 @@ struct Builder
             size_t height = 0;
             IAABB overall = frame;
             bool clippedOut = false;
+            ClipRectState cr;
             for (uint32_t c = 0; c < d.clip_count && !clippedOut; ++c)
             {
                 uint32_t cp = d.clip_paths[c];
@@ -571,6 +696,25 @@ struct Builder
                 {
                     clippedOut = true;
                     break;
+                }
+                // clipPath (rive_renderer.cpp:199-228): axis-aligned rectangles
+                // become clip rects (clipRectImpl, :267-321).
+                float rect[4];
+                if (is_aabb(s.verbs + sp.first_verb, sp.verb_count, pathPts(cp), sp.point_count, rect))
+                {
+                    if (!(rect[2] - rect[0] > 0 && rect[3] - rect[1] > 0))
+                    {
+                        clippedOut = true;
+                        break;
+                    }
+                    if (!cr.has || transform_rect_to_new_space(rect, m, cr.matrix))
+                    {
+                        cr.add(rect, m);
+                        overall = overall.intersect(cr.pixelBounds());
+                        if (overall.empty())
+                            clippedOut = true;
+                        continue;
+                    }
                 }
                 bool equivalent = clipStack.size() > height && clipStack[height].path == cp &&
                                   memcmp(clipStack[height].matrix.m, m.m, sizeof(m.m)) == 0;
@@ -614,9 +758,12 @@ struct Builder
                 uint32_t drawClipID = applyClip(height, frame);
                 float op = std::max(0.f, std::min(1.f, std::max(0.f, d.opacity)));
                 uint32_t color = ((uint32_t)std::lround(255.f * op) << 24) | 0x00ffffffu;
+                Mat crInv;
+                const bool hasCR = cr.inverse(&crInv);
                 pushGeometryDraw(kUnitVerbs, 4, kUnitPts, 4, im, 0, Join::miter, Cap::butt, FillRule::nonZero,
                                  SOLID_COLOR_PAINT_TYPE, swizzle_rgba(color), drawClipID, 0, nullptr,
-                                 shader_blend_mode(d.blend_mode), d.image, d.image_sampler);
+                                 shader_blend_mode(d.blend_mode), d.image, d.image_sampler,
+                                 hasCR ? &crInv : nullptr);
                 continue;
             }
             // drawPath (rive_renderer.cpp:121-197)
@@ -679,9 +826,11 @@ struct Builder
             float strokeRadius = 0;
             if (isStroke)
                 strokeRadius = fmaxf(d.thickness * .5f, 1.17549435e-38f);
+            Mat crInv;
+            const bool hasCR = cr.inverse(&crInv);
             pushPathDraw(d.path, m, strokeRadius, (Join)d.join, (Cap)d.cap,
                          (FillRule)sp.fill_rule, paintType, paintValue, drawClipID, 0, gp,
-                         shader_blend_mode(d.blend_mode));
+                         shader_blend_mode(d.blend_mode), hasCR ? &crInv : nullptr);
         }
         finishGradients();
         f.pathCount = (uint32_t)f.pathData.size() - 1;
@@ -1301,6 +1450,8 @@ void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* 
             uint32_t clipID = paint.params >> 16;
             if (clipID != 0)
                 coverage = clipIDs[p] == clipID ? std::min(clipCov[p], coverage) : 0.f;
+            if (paint.params & PAINT_FLAG_HAS_CLIP_RECT)
+                coverage = std::min(std::max(clip_rect_coverage(aux.clip_rect_inverse_matrix, x, y), 0.f), coverage);
             const uint32_t blendMode = (paint.params >> 4) & 0xfu;
             uint32_t dst = rgba[p];
             float color[4];

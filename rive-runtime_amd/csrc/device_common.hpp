@@ -35,6 +35,7 @@ constexpr uint32_t LINEAR_GRADIENT_PAINT_TYPE = 2u;
 constexpr uint32_t RADIAL_GRADIENT_PAINT_TYPE = 3u;
 constexpr uint32_t PAINT_FLAG_NON_ZERO_FILL = 0x100u;
 constexpr uint32_t PAINT_FLAG_EVEN_ODD_FILL = 0x200u;
+constexpr uint32_t PAINT_FLAG_HAS_CLIP_RECT = 0x400u;
 
 constexpr uint32_t GRAD_SPAN_FLAG_LEFT_BORDER = 0x80000000u;
 constexpr uint32_t GRAD_SPAN_FLAG_RIGHT_BORDER = 0x40000000u;
@@ -75,7 +76,8 @@ constexpr uint32_t kPrimEmpty = 8u;
 // The fields the src-over resolve reads come first.
 struct PathRaster
 {
-    uint32_t flags;      // bit0 stroke, bit1 evenOdd, bit2 clockwise, bit3 clipUpdate, bits 8-11 blend mode
+    uint32_t flags;      // bit0 stroke, bit1 evenOdd, bit2 clockwise, bit3 clipUpdate, bit4 clip rect,
+                         // bits 8-11 blend mode
     uint32_t paintType;
     uint32_t clipID;     // draws: clip to read (0 none); clip updates: ID to write
     uint32_t outerClipID;
@@ -88,8 +90,10 @@ struct PathRaster
     uint32_t imageSampler; // ImageSampler key: wrapX + 3*wrapY + 9*filter
     float imageMatrix[6];  // pixel -> unit uv (PaintAuxData image matrix)
     float imageLod;
+    float clipRect[6];     // pixel -> the clip rect normalized to [-1, 1]^2 (ClipRectInverseMatrix)
+    uint32_t pad;
 };
-static_assert(sizeof(PathRaster) == 104, "PathRaster layout");
+static_assert(sizeof(PathRaster) == 132, "PathRaster layout");
 
 // One image of the flush's image table: RGBA8 premultiplied, mip levels
 // packed one after another.

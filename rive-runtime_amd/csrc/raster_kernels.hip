@@ -1673,6 +1673,26 @@ __device__ __forceinline__ float clip_coverage(uint32_t clip)
     return __half2float(__ushort_as_half((unsigned short)(clip >> 16)));
 }
 
+// Clip rect coverage at a pixel center (common.glsl:376-400,
+// draw_raster_order_path.frag:152-159): the least of the Manhattan distances,
+// in AA widths, to the rect's four edges plus one half, rounded to half
+// precision.  A singular inverse matrix gives its translation as the coverage.
+__device__ __forceinline__ float clip_rect_coverage(const float* m, int px, int py)
+{
+    const float awx = fabsf(m[0]) + fabsf(m[2]), awy = fabsf(m[1]) + fabsf(m[3]);
+    float d;
+    if (awx != 0.f && awy != 0.f)
+    {
+        const float rx = 1.f / awx, ry = 1.f / awy;
+        const float fx = float(px) + .5f, fy = float(py) + .5f;
+        const float cx = m[0] * fx + m[2] * fy + m[4], cy = m[1] * fx + m[3] * fy + m[5];
+        d = fminf(fminf(cx * rx + rx + .5f, cy * ry + ry + .5f), fminf(-cx * rx + rx + .5f, -cy * ry + ry + .5f));
+    }
+    else
+        d = fminf(m[4], m[5]);
+    return __half2float(__float2half_rn(d));
+}
+
 __device__ __forceinline__ void resolve_pixel(const PathRaster& pr, const PathRaster* __restrict__ prp,
                                               float coverage, int px, int py, float noise255,
                                               const uint32_t* __restrict__ ramps, uint32_t gradHeight,
@@ -1689,6 +1709,8 @@ __device__ __forceinline__ void resolve_pixel(const PathRaster& pr, const PathRa
     }
     if (pr.clipID != 0u)
         coverage = clipID == pr.clipID ? fminf(clip_coverage(clip), coverage) : 0.f;
+    if (pr.flags & 16u)
+        coverage = fminf(fmaxf(clip_rect_coverage(pr.clipRect, px, py), 0.f), coverage);
     if ((pr.flags & 0xf00u) != 0u || pr.image != 0u)
     {
         resolve_general(prp, coverage, px, py, noise255, ramps, gradHeight, images, &colPacked);
@@ -2067,7 +2089,11 @@ __global__ void path_raster_kernel(const PathDataV* __restrict__ paths, const ui
         r.clipID = paint.x >> 16;
     }
     if (paintType != CLIP_UPDATE_PAINT_TYPE)
+    {
         flags |= ((paint.x >> 4) & 0xfu) << 8; // shader blend mode
+        if (paint.x & PAINT_FLAG_HAS_CLIP_RECT)
+            flags |= 16u;
+    }
     r.flags = flags;
     const float a = float(paint.y >> 24) * (1.f / 255.f);
     r.color255[0] = float(paint.y & 0xffu) * a;
@@ -2097,6 +2123,10 @@ __global__ void path_raster_kernel(const PathDataV* __restrict__ paths, const ui
     for (int k = 0; k < 6; ++k)
         r.imageMatrix[k] = ax[16 + k];
     r.imageLod = ax[22];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        r.clipRect[k] = ax[8 + k];
+    r.pad = 0;
     out[i] = r;
 }
 

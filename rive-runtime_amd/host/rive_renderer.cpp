@@ -21,13 +21,14 @@ Mat2D Mat2D::operator*(const Mat2D& b) const
 
 bool Mat2D::invert(Mat2D* out) const
 {
-    double det = (double)m[0] * m[3] - (double)m[1] * m[2];
-    if (det == 0 || !std::isfinite(det))
+    // src/math/mat2d.cpp:179-200, in float.
+    const float aa = m[0], ab = m[1], ac = m[2], ad = m[3], atx = m[4], aty = m[5];
+    float det = aa * ad - ab * ac;
+    if (det == 0.f)
         return false;
-    double inv = 1 / det;
-    *out = Mat2D((float)(m[3] * inv), (float)(-m[1] * inv), (float)(-m[2] * inv), (float)(m[0] * inv),
-                 (float)((m[2] * (double)m[5] - m[3] * (double)m[4]) * inv),
-                 (float)((m[1] * (double)m[4] - m[0] * (double)m[5]) * inv));
+    det = 1.f / det;
+    *out = Mat2D(ad * det, -ab * det, -ac * det, aa * det, (ac * aty - ad * atx) * det,
+                 (ab * atx - aa * aty) * det);
     return true;
 }
 
@@ -252,9 +253,60 @@ void RiveRenderer::drawImage(const RenderImage* renderImage, ImageSampler sample
     restore();
 }
 
+bool RiveRenderer::IsAABB(const RawPath& raw, float rect[4])
+{
+    // rive_renderer.cpp:16-55: a move and three lines, any further points
+    // equal to the first, with opposite corners sharing x and y.
+    const auto& verbs = raw.verbs();
+    const auto& pts = raw.points();
+    if (verbs.size() < 4 || verbs[0] != PathVerb::move || verbs[1] != PathVerb::line ||
+        verbs[2] != PathVerb::line || verbs[3] != PathVerb::line)
+        return false;
+    for (size_t i = 4; i < pts.size(); ++i)
+        if (!(pts[i].x == pts[0].x && pts[i].y == pts[0].y))
+            return false;
+    const float c[4] = {pts[0].x, pts[0].y, pts[2].x, pts[2].y};
+    const float o[4] = {pts[1].x, pts[1].y, pts[3].x, pts[3].y};
+    const bool zyxw = c[0] == o[2] && c[1] == o[1] && c[2] == o[0] && c[3] == o[3];
+    const bool xwzy = c[0] == o[0] && c[1] == o[3] && c[2] == o[2] && c[3] == o[1];
+    if (!zyxw && !xwzy)
+        return false;
+    rect[0] = std::min(c[0], c[2]);
+    rect[1] = std::min(c[1], c[3]);
+    rect[2] = std::max(c[0], c[2]);
+    rect[3] = std::max(c[1], c[3]);
+    return true;
+}
+
+namespace
+{
+// rive_renderer.cpp:235-265: re-expresses `rect` under `newMatrix` so that
+// currentMatrix * rect == newMatrix * newRect, if it stays a rect.
+bool transform_rect_to_new_space(float rect[4], const Mat2D& currentMatrix, const Mat2D& newMatrix)
+{
+    if (currentMatrix == newMatrix)
+        return true;
+    Mat2D currentToNew;
+    if (!newMatrix.invert(&currentToNew))
+        return false;
+    currentToNew = currentToNew * currentMatrix;
+    const float maxSkew = fmaxf(fabsf(currentToNew[1]), fabsf(currentToNew[2]));
+    const float maxScale = fmaxf(fabsf(currentToNew[0]), fabsf(currentToNew[3]));
+    if (maxSkew > 1.0e-5f && maxScale > 1.0e-5f) // math::EPSILON
+        return false;
+    const Vec2D a = currentToNew.map({rect[0], rect[1]}), b = currentToNew.map({rect[2], rect[3]});
+    rect[0] = std::min(a.x, b.x);
+    rect[1] = std::min(a.y, b.y);
+    rect[2] = std::max(a.x, b.x);
+    rect[3] = std::max(a.y, b.y);
+    return true;
+}
+} // namespace
+
 void RiveRenderer::clipPath(RenderPath* renderPath)
 {
-    // rive_renderer.cpp:199-228, 323-380 (clip rects are not advertised).
+    // rive_renderer.cpp:199-228: axis-aligned rectangles become clip rects
+    // (raster ordering supports them, render_context.cpp:456-461).
     auto* path = static_cast<RiveRenderPath*>(renderPath);
     RenderState& st = m_stack.back();
     if (st.overallClipPixelBounds.empty())
@@ -264,6 +316,60 @@ void RiveRenderer::clipPath(RenderPath* renderPath)
         st.overallClipPixelBounds = {};
         return;
     }
+    float rect[4];
+    if (IsAABB(path->getRawPath(), rect))
+        clipRectImpl(rect, path);
+    else
+        clipPathImpl(path);
+}
+
+void RiveRenderer::clipRectImpl(float rect[4], RiveRenderPath* path)
+{
+    // rive_renderer.cpp:267-321
+    RenderState& st = m_stack.back();
+    if (!(rect[2] - rect[0] > 0 && rect[3] - rect[1] > 0)) // isEmptyOrNaN
+    {
+        st.overallClipPixelBounds = {};
+        return;
+    }
+    if (st.hasClipRect && !transform_rect_to_new_space(rect, st.matrix, st.clipRectMatrix))
+    {
+        clipPathImpl(path);
+        return;
+    }
+    if (!st.hasClipRect)
+    {
+        memcpy(st.clipRect, rect, sizeof(st.clipRect));
+        st.clipRectMatrix = st.matrix;
+    }
+    else
+    {
+        st.clipRect[0] = fmaxf(st.clipRect[0], rect[0]);
+        st.clipRect[1] = fmaxf(st.clipRect[1], rect[1]);
+        st.clipRect[2] = fminf(st.clipRect[2], rect[2]);
+        st.clipRect[3] = fminf(st.clipRect[3], rect[3]);
+    }
+    st.hasClipRect = true;
+    RawPath corners;
+    corners.move({st.clipRect[0], st.clipRect[1]});
+    corners.line({st.clipRect[2], st.clipRect[1]});
+    corners.line({st.clipRect[2], st.clipRect[3]});
+    corners.line({st.clipRect[0], st.clipRect[3]});
+    st.overallClipPixelBounds = st.overallClipPixelBounds.intersect(MapBounds(st.clipRectMatrix, corners, 0, 0));
+    // ClipRectInverseMatrix::reset (gpu.cpp:689-706): pixel space -> the
+    // normalized rect [-1, -1, +1, +1]; an empty rect or singular matrix clips
+    // everything away.
+    const float w = st.clipRect[2] - st.clipRect[0], h = st.clipRect[3] - st.clipRect[1];
+    const Mat2D toRect = st.clipRectMatrix * Mat2D(w * .5f, 0, 0, h * .5f, (st.clipRect[0] + st.clipRect[2]) * .5f,
+                                                   (st.clipRect[1] + st.clipRect[3]) * .5f);
+    if (w <= 0 || h <= 0 || !toRect.invert(&st.clipRectInverse))
+        st.clipRectInverse = Mat2D(0, 0, 0, 0, 0, 0);
+}
+
+void RiveRenderer::clipPathImpl(RiveRenderPath* path)
+{
+    // rive_renderer.cpp:323-380
+    RenderState& st = m_stack.back();
     const size_t h = st.clipStackHeight;
     bool equivalent = m_clipStack.size() > h && m_clipStack[h].path == path && m_clipStack[h].matrix == st.matrix;
     if (!equivalent)
@@ -348,7 +454,8 @@ void RiveRenderer::drawPath(RenderPath* renderPath, RenderPaint* renderPaint)
         clipID = parent;
         m_context->setClipContentID(parent);
     }
-    m_context->pushPathDraw(path, st.matrix, paint, clipID, false, 0, path->getFillRule());
+    m_context->pushPathDraw(path, st.matrix, paint, clipID, false, 0, path->getFillRule(),
+                            st.hasClipRect ? &st.clipRectInverse : nullptr);
 }
 
 namespace gpu
@@ -400,7 +507,8 @@ void RenderContext::beginFrame(const FrameDescriptor& fd)
 }
 
 void RenderContext::pushPathDraw(const RiveRenderPath* path, const Mat2D& m, const RiveRenderPaint* paint,
-                                 uint32_t clipID, bool clipUpdate, uint32_t outerClipID, FillRule fillRule)
+                                 uint32_t clipID, bool clipUpdate, uint32_t outerClipID, FillRule fillRule,
+                                 const Mat2D* clipRectInverse)
 {
     DrawRecord r{};
     r.raw = &path->getRawPath();
@@ -454,6 +562,21 @@ void RenderContext::pushPathDraw(const RiveRenderPath* path, const Mat2D& m, con
             params |= 0x100u;
         else if (fillRule == FillRule::evenOdd)
             params |= 0x200u;
+    }
+    // PaintAuxData clip rect (gpu.cpp:1044-1062): the inverse matrix and
+    // inverse fwidth, or WideOpen (scale 0, translate 1: uniform coverage 1).
+    if (clipRectInverse != nullptr)
+    {
+        params |= 0x400u; // PAINT_FLAG_HAS_CLIP_RECT
+        const Mat2D& cm = *clipRectInverse;
+        memcpy(r.aux.clip_rect_inverse_matrix, cm.values(), sizeof(r.aux.clip_rect_inverse_matrix));
+        r.aux.inverse_fwidth[0] = -1.f / (fabsf(cm[0]) + fabsf(cm[1]));
+        r.aux.inverse_fwidth[1] = -1.f / (fabsf(cm[2]) + fabsf(cm[3]));
+    }
+    else
+    {
+        const float wideOpen[6] = {0, 0, 0, 0, 1, 1};
+        memcpy(r.aux.clip_rect_inverse_matrix, wideOpen, sizeof(wideOpen));
     }
     r.paint.params = params;
     // PathDraw's per-draw constants (draw.cpp:561-825)

@@ -314,7 +314,8 @@ public:
     uint32_t getClipContentID() const { return m_clipContentID; }
     void setClipContentID(uint32_t id) { m_clipContentID = id; }
     void pushPathDraw(const RiveRenderPath* path, const Mat2D& m, const RiveRenderPaint* paint,
-                      uint32_t clipID, bool clipUpdate, uint32_t outerClipID, FillRule fillRule);
+                      uint32_t clipID, bool clipUpdate, uint32_t outerClipID, FillRule fillRule,
+                      const Mat2D* clipRectInverse = nullptr);
 
 private:
     std::unique_ptr<RenderContextImpl> m_impl;
@@ -357,7 +358,16 @@ private:
         Mat2D matrix;
         size_t clipStackHeight = 0;
         IAABB overallClipPixelBounds;
+        // rive_renderer.hpp RenderState clip rect: the rect (l, t, r, b) in
+        // the space of clipRectMatrix, and ClipRectInverseMatrix.
+        bool hasClipRect = false;
+        float clipRect[4] = {0, 0, 0, 0};
+        Mat2D clipRectMatrix;
+        Mat2D clipRectInverse;
     };
+    static bool IsAABB(const RawPath& raw, float rect[4]);
+    void clipRectImpl(float rect[4], RiveRenderPath* path);
+    void clipPathImpl(RiveRenderPath* path);
     static IAABB MapBounds(const Mat2D& m, const RawPath& raw, float outsetX, float outsetY);
     gpu::RenderContext* m_context;
     std::vector<RenderState> m_stack;

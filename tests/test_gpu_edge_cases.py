@@ -137,7 +137,32 @@ def sc_images():
     return b
 
 
+def sc_clip_rects():
+    """Axis-aligned rectangle clips become clip rects (ClipRectInverseMatrix):
+    offset edges, rotation, intersection, a clip rect with a clip path, a
+    counterclockwise rect, a rotated quad (a clip path), an empty rect and an
+    image under a clip rect."""
+    rng = np.random.default_rng(9)
+    b = StreamBuilder(90, 70, dither=1)
+    b.draw(b.path(rect(0, 0, 90, 70)), color=0xff203040)
+    full = b.path(rect(-10, -10, 100, 80))
+    b.draw(full, color=0xc0ff8000, clips=(b.path(rect(10.5, 8.25, 40.75, 30.5)),))
+    rot = (math.cos(.4), math.sin(.4), -math.sin(.4), math.cos(.4), 55, 5)
+    b.draw(b.path(star(12, 14, 5, 16)), color=0xe000c0ff, matrix=rot, clips=(b.path(rect(0, 0, 20.5, 25)),))
+    b.draw(full, color=0x9000ff40,
+           clips=(b.path(rect(5, 35, 60, 65)), b.path(rect(20.25, 30, 80, 55.5, clockwise=False))))
+    b.draw(b.path(star(70, 50, 6, 18)), color=0xffffffff,
+           clips=(b.path(rect(55, 38, 86, 68)), b.path(star(70, 52, 8, 16))))
+    quad = [(0, 45, 20), (1, 60, 28), (1, 52, 43), (1, 37, 35), (5,)]
+    b.draw(full, color=0x60ff00ff, clips=(b.path(quad),))
+    b.draw(full, color=0xffff0000, clips=(b.path(rect(30, 30, 30, 50)),))
+    img = b.image(random_premul(rng, 16, 16, True))
+    b.draw_image(img, matrix=(2, 0, 0, 2, 2, 40), clips=(b.path(rect(2.5, 1.5, 13.5, 12)),), opacity=.8)
+    return b
+
+
 SCENARIOS = {
+    "clip_rects": sc_clip_rects,
     "empty_frame": sc_empty_frame, "culled_draws": sc_culled_draws, "one_pixel": sc_one_pixel,
     "offcanvas_and_huge": sc_offcanvas_and_huge, "degenerate_cubics": sc_degenerate_cubics,
     "clockwise_mirrored": sc_clockwise_mirrored, "clips": sc_clips, "gradients": sc_gradients,

@@ -166,3 +166,38 @@ def test_empty_and_culled_draws_leave_clear():
     b.draw(offscreen, color=0xffffffff)
     b.draw(b.path(rect(2, 2, 8, 8)), color=0xffffffff, stroke=0.0)  # zero-width stroke
     assert render(b).max() == 0
+
+
+def test_clip_rect_half_coverage_on_edge():
+    """A clip rect's coverage is 1/2 where a pixel center lies on its edge
+    (common.glsl:376-400), 0 and 1 half a pixel either side."""
+    b = StreamBuilder(16, 16)
+    b.draw(b.path(rect(0, 0, 16, 16)), color=0xffffffff, clips=(b.path(rect(4.5, 0, 12, 16)),))
+    a = render(b)[..., 3].astype(int)
+    assert np.all(a[:, 5:12] == 255) and np.all(a[:, :4] == 0) and np.all(a[:, 12:] == 0)
+    assert np.all(np.abs(a[:, 4] - 128) <= 1)
+
+
+def test_clip_rect_rotated_quarter_turn_is_exact():
+    """Under a quarter turn a pixel-aligned clip rect stays pixel aligned."""
+    b = StreamBuilder(20, 20)
+    m = (0.0, 1.0, -1.0, 0.0, 20.0, 0.0)  # (x, y) -> (20 - y, x)
+    b.draw(b.path(rect(-5, -5, 30, 30)), color=0xffffffff, matrix=m, clips=(b.path(rect(2, 3, 9, 17)),))
+    a = render(b)[..., 3]
+    inside = np.zeros((20, 20), bool)
+    inside[2:9, 3:17] = True  # rows from x, columns from 20 - y
+    assert np.all(a[inside] == 255) and np.all(a[~inside] == 0)
+
+
+def test_clip_rects_intersect_like_one_rect():
+    """Two clip rects under one matrix intersect geometrically; a rect with
+    no area clips everything away."""
+    def frame(clips):
+        b = StreamBuilder(40, 30, dither=1)
+        cs = tuple(b.path(c) for c in clips)
+        b.draw(b.path(rect(0, 0, 40, 30)), color=0xc080ff20, clips=cs)
+        return render(b)
+    two = frame([rect(3.25, 2.5, 30.5, 20), rect(10.75, 5, 38, 27.5, clockwise=False)])
+    one = frame([rect(10.75, 5, 30.5, 20)])
+    assert np.array_equal(two, one)
+    assert np.all(frame([rect(5, 5, 5, 25)])[..., 3] == 0)
