@@ -76,6 +76,84 @@ def cpu_baseline(sample_draws):
                       f"by draw count"}
 
 
+def batched_leg(args):
+    """The batched workload: 256 seeded canvases (sides 64..1024, 10..2000
+    closed cubic fills each) rendered by batched flushes from records resident
+    in HBM.  Canvases share a flush while their contours fit the 16-bit
+    contour IDs; each flush group keeps its records resident in its own
+    context, and a step flushes every group once.  The end-to-end figure (front
+    end + upload + flushes) is reported beside it."""
+    streams = rivehip.generate_batch(args.batch_canvases)
+    sizes = [(s.contents.width, s.contents.height) for s in streams]
+    groups = rivehip.flush_groups(streams)
+    ctxs = [rivehip.Context(0) for _ in groups]
+    targets = [None] * len(streams)
+    try:
+        for ctx, grp in zip(ctxs, groups):
+            for i in grp:
+                targets[i] = ctx.target(*sizes[i])
+
+        def render_all():
+            for ctx, grp in zip(ctxs, groups):
+                ctx.render_streams([streams[i] for i in grp], [targets[i] for i in grp])
+            for ctx in ctxs:
+                ctx.sync()
+
+        render_all()
+        e2e = []
+        for _ in range(max(2, args.steps // 5)):
+            t0 = time.perf_counter()
+            render_all()
+            e2e.append((time.perf_counter() - t0) * 1e3)
+        resident = [(ctx, ctx.last_flush_stats(), ctx.last_flush_canvases()) for ctx in ctxs]
+        for _ in range(args.warmup):
+            for ctx, st, cv in resident:
+                ctx.reflush_geometry(None, st, canvases=cv)
+                ctx.sync()
+        step_ms, passes = [], []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            for ctx, st, cv in resident:
+                ctx.reflush_geometry(None, st, canvases=cv)
+                ctx.sync()
+            step_ms.append((time.perf_counter() - t0) * 1e3)
+            passes.append([sum(x) for x in zip(*(ctx.pass_times() for ctx in ctxs))])
+        drawn = 0
+        for grp, ctx in zip(groups, ctxs):
+            for i in grp:
+                drawn += int(ctx.read_pixels(targets[i], *sizes[i])[..., 3].max() > 0)
+        stats = [st for _, st, _ in resident]
+    finally:
+        for grp, ctx in zip(groups, ctxs):
+            for i in grp:
+                if targets[i] is not None:
+                    ctx.free_target(targets[i])
+            ctx.close()
+        for s in streams:
+            rivehip.free_stream(s)
+    names = ["ramps", "count_spans", "tessellate", "bin", "raster", "total"]
+    pass_ms = {n: round(statistics.mean(p[i] for p in passes), 4) for i, n in enumerate(names)}
+    pixels = sum(w * h for w, h in sizes)
+    tot = lambda f: sum(getattr(st, f) for st in stats)
+    b_raster = (4 * pixels + 16 * tot("tess_vertex_count") + 72 * tot("path_count")
+                + 128 * tot("aux_path_count") + 2048 * tot("grad_complex_rows") + 8 * tot("grad_simple_ramps"))
+    achieved = b_raster / (pass_ms["raster"] / 1e3) / 1e9
+    step_s = sum(step_ms) / 1e3 / args.steps
+    return {"workload": f"{len(sizes)} canvases, {pixels} px, {tot('path_count')} paths (seeded "
+                        f"mt19937_64 {rivehip.CONFIG_SEEDS[4]:#x}) in {len(groups)} batched flushes "
+                        f"from HBM-resident records",
+            "canvases": len(sizes), "canvases_drawn": drawn, "flushes": len(groups), "pixels": pixels,
+            "paths": tot("path_count"), "tess_vertices": tot("tess_vertex_count"), "tiles": tot("tile_count"),
+            "batches_per_s": round(1.0 / step_s, 3), "canvases_per_s": round(len(sizes) / step_s, 1),
+            "mpixels_per_s": round(pixels / step_s / 1e6, 1),
+            "ms_per_batch": round(step_s * 1e3, 4), "p50_ms": round(percentile(step_ms, 50), 4),
+            "p99_ms": round(percentile(step_ms, 99), 4), "passes_ms": pass_ms,
+            "end_to_end_ms": round(statistics.mean(e2e), 4),
+            "roofline": {"kernel": "raster_kernel", "bound": "hbm", "achieved": round(achieved, 3),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
+                         "algorithmic_bytes": b_raster, "kernel_ms": pass_ms["raster"]}}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +161,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-sample-draws", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched 256-canvas leg")
+    ap.add_argument("--batch-canvases", type=int, default=256)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device once (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE readings) and exit")
@@ -141,6 +221,7 @@ def main():
         ctx.close()
     if int(pixels[..., 3].max()) == 0:
         raise RuntimeError("the frame rendered nothing")
+    batched = None if args.no_batched else batched_leg(args)
 
     total_s = sum(step_ms) / 1e3
     fps = args.steps / total_s
@@ -182,6 +263,7 @@ def main():
                      "traffic": RASTER_TRAFFIC_BYTES, "algorithmic_bytes": b_raster,
                      "kernel_ms": pass_ms["raster"]},
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample_draws),
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

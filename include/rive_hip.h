@@ -233,7 +233,26 @@ typedef struct rive_hip_flush_desc
     uint32_t first_path_tess_location; /* GEOMETRY mode: usually 8 */
     rive_hip_image* const* images;     /* referenced by rive_hip_draw.image_index */
     uint32_t image_count;
+    /* Batched flush: canvas_count > 0 renders each canvas's paths into its own
+     * target (target, load_action and clear_color above are then unused).
+     * canvas_count == 0 is the reference's single-target flush. */
+    const struct rive_hip_canvas* canvases;
+    uint32_t canvas_count;
 } rive_hip_flush_desc;
+
+/* One canvas of a batched flush: paths first_path .. first_path+path_count-1
+ * render into `target`.  No counterpart in the reference, whose FlushDescriptor
+ * has one render target (gpu.hpp:1320-1438); it lets one flush render many
+ * small canvases (one launch per pass for all of them).  Canvases must be in
+ * ascending first_path order and must not overlap. */
+typedef struct rive_hip_canvas
+{
+    rive_hip_target* target;
+    uint32_t first_path;   /* >= 1 */
+    uint32_t path_count;
+    uint32_t clear_color;  /* ColorInt ARGB */
+    uint32_t load_action;  /* 0 clear, 1 preserve */
+} rive_hip_canvas;
 
 /* Replaces RenderContextImpl::flush (render_context_impl.hpp:217): ramps,
  * (counting), tessellation, binning, rasterisation and RGBA8 resolve, all
@@ -281,6 +300,11 @@ typedef struct rive_hip_flush_stats
 } rive_hip_flush_stats;
 
 int rive_hip_last_flush_stats(rive_hip_context* ctx, rive_hip_flush_stats* out);
+
+/* The canvas table of the last flush (one entry for a single-target flush).
+ * Writes min(capacity, count) entries; *count gets the full count. */
+int rive_hip_last_flush_canvases(rive_hip_context* ctx, rive_hip_canvas* out, uint32_t capacity,
+                                 uint32_t* count);
 
 /* ------------------------------------------------------------------------ */
 /* Draw streams: a recorded sequence of rive::Renderer calls.               */
@@ -358,6 +382,25 @@ void rive_stream_free(rive_stream* stream);
  * -> RenderContextHIPImpl) into `target`, GEOMETRY mode. */
 int rive_hip_render_stream(rive_hip_context* ctx, const rive_stream* stream,
                            rive_hip_target* target);
+
+/* Renders n streams into n targets with batched flushes: each stream is
+ * replayed through RiveRenderer as its own canvas, and consecutive canvases
+ * share one flush while their contours fit the 16-bit contour IDs
+ * (rive_stream_contour_bound sums to at most 65535); a flush then renders all
+ * of its canvases at once. */
+int rive_hip_render_streams(rive_hip_context* ctx, const rive_stream* const* streams,
+                            rive_hip_target* const* targets, uint32_t n);
+
+/* An upper bound of the contours a stream's draws and clips produce (its
+ * move verbs, plus one per image draw). */
+uint32_t rive_stream_contour_bound(const rive_stream* stream);
+
+/* Seeded batch of canvases (std::mt19937_64(seed)): per canvas, side
+ * s = exp U[ln 64, ln 1024], aspect a = exp U[ln .5, ln 2],
+ * w = round(s sqrt a), h = round(s / sqrt a), and round(exp U[ln 10, ln 2000])
+ * closed cubic fills drawn like config 1 over w x h.  out[0..n) each own their
+ * arrays until rive_stream_free. */
+int rive_stream_generate_batch(uint64_t seed, uint32_t n, rive_stream** out);
 
 #ifdef __cplusplus
 }

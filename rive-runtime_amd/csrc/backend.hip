@@ -97,6 +97,10 @@ struct rive_hip_context
     DevBuf imageTable, pathImage;
     ImageDesc* hostImages = nullptr; // pinned staging of the image table
     uint32_t hostImagesCap = 0;
+    DevBuf canvasTable, pathCanvas;
+    CanvasDev* hostCanvases = nullptr; // pinned staging of the canvas table
+    uint32_t hostCanvasesCap = 0;
+    std::vector<rive_hip_canvas> lastCanvases;
     hipEvent_t ev[kEvents] = {};
     float passMs[6] = {};
     bool timed = false;
@@ -162,6 +166,8 @@ void rive_hip_destroy(rive_hip_context* ctx)
         (void)hipHostFree(ctx->hostScalars);
     if (ctx->hostImages)
         (void)hipHostFree(ctx->hostImages);
+    if (ctx->hostCanvases)
+        (void)hipHostFree(ctx->hostCanvases);
     for (auto& e : ctx->ev)
         if (e)
             (void)hipEventDestroy(e);
@@ -390,9 +396,39 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
                     uint32_t spanCount, const ContourV* contours, uint32_t contourCount, uint32_t tessRows,
                     DrawTess* drawTess, uint32_t drawCount, uint32_t totalPrims)
 {
-    rive_hip_target* tgt = d->target;
-    const int W = (int)tgt->width, H = (int)tgt->height;
-    const int tilesX = (W + kTile - 1) / kTile, tilesY = (H + kTile - 1) / kTile;
+    // The canvas table: the flush's canvases, or its one target.
+    const uint32_t canvasCount = (uint32_t)ctx->lastCanvases.size();
+    if (ctx->hostCanvasesCap < canvasCount)
+    {
+        if (ctx->hostCanvases)
+            HIP_TRY(hipHostFree(ctx->hostCanvases));
+        ctx->hostCanvases = nullptr;
+        HIP_TRY(hipHostMalloc((void**)&ctx->hostCanvases, sizeof(CanvasDev) * canvasCount, hipHostMallocDefault));
+        ctx->hostCanvasesCap = canvasCount;
+    }
+    uint32_t tileTotal = 0;
+    int maxTilesY = 1;
+    for (uint32_t c = 0; c < canvasCount; ++c)
+    {
+        const rive_hip_canvas& cv = ctx->lastCanvases[c];
+        CanvasDev& cd = ctx->hostCanvases[c];
+        cd = CanvasDev{};
+        cd.pixels = cv.target->pixels;
+        cd.width = (int)cv.target->width;
+        cd.height = (int)cv.target->height;
+        cd.tilesX = (cd.width + kTile - 1) / kTile;
+        cd.tilesY = (cd.height + kTile - 1) / kTile;
+        cd.tileBase = tileTotal;
+        cd.firstPath = cv.first_path;
+        cd.pathEnd = cv.first_path + cv.path_count;
+        cd.clearPremul = premul_clear(cv.clear_color);
+        cd.loadPreserve = cv.load_action == 1 ? 1u : 0u;
+        tileTotal += (uint32_t)(cd.tilesX * cd.tilesY);
+        maxTilesY = std::max(maxTilesY, cd.tilesY);
+    }
+    HIP_TRY(ensure(ctx->canvasTable, sizeof(CanvasDev) * std::max<uint32_t>(canvasCount, 1)));
+    HIP_TRY(hipMemcpyAsync(ctx->canvasTable.ptr, ctx->hostCanvases, sizeof(CanvasDev) * canvasCount,
+                           hipMemcpyHostToDevice, ctx->stream));
     uint32_t* scal = P<uint32_t>(ctx->scalars);
     const uint32_t pathCount = d->path_count;
     const PathDataV* paths = reinterpret_cast<const PathDataV*>(ctx->bufs[RIVE_HIP_BUFFER_PATH].dev);
@@ -440,6 +476,10 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
 
     // ---- expansion + binning
+    HIP_TRY(ensure(ctx->pathCanvas, (size_t)(pathCount + 1) * 4));
+    HIP_TRY(hipMemsetAsync(ctx->pathCanvas.ptr, 0xff, (size_t)(pathCount + 1) * 4, ctx->stream));
+    path_canvas_kernel<<<canvasCount, 256, 0, ctx->stream>>>(P<CanvasDev>(ctx->canvasTable),
+                                                             P<uint32_t>(ctx->pathCanvas));
     HIP_TRY(ensure(ctx->cells, (size_t)(pathCount + 1) * sizeof(PathCells)));
     cells_init_kernel<<<(pathCount + 1 + 255) / 256, 256, 0, ctx->stream>>>(P<PathCells>(ctx->cells),
                                                                              pathCount + 1);
@@ -451,11 +491,10 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     if (totalPrims > 0)
         expand_kernel<<<(totalPrims + 255) / 256, 256, 0, ctx->stream>>>(
             P<TessTexel>(ctx->tess), (int)texels, P<VtxPair>(ctx->verts), contours, contourCount, paths,
-            drawTess, drawCount,
-            totalPrims, tilesX,
-            tilesY, P<Prim>(ctx->prims), P<PathCells>(ctx->cells));
-    cells_layout_kernel<<<1, 1024, 0, ctx->stream>>>(P<PathCells>(ctx->cells), pathCount + 1, tilesX,
-                                                     tilesY, scal + 1);
+            drawTess, drawCount, totalPrims, P<Prim>(ctx->prims), P<PathCells>(ctx->cells));
+    cells_layout_kernel<<<1, 1024, 0, ctx->stream>>>(P<PathCells>(ctx->cells), pathCount + 1,
+                                                     P<uint32_t>(ctx->pathCanvas), P<CanvasDev>(ctx->canvasTable),
+                                                     scal + 1);
     HIP_TRY(hipGetLastError());
     uint32_t totalCells = 0;
     int rc = read_scalar(ctx, scal + 1, &totalCells);
@@ -476,7 +515,7 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     if (rc)
         return rc;
     {
-        int maxRows = tilesY;
+        int maxRows = maxTilesY;
         const uint64_t threads = (uint64_t)(pathCount + 1) * (uint64_t)maxRows * 64u;
         backdrop_prefix_kernel<<<(uint32_t)((threads + 255) / 256), 256, 0, ctx->stream>>>(
             P<PathCells>(ctx->cells), pathCount + 1, maxRows, P<int32_t>(ctx->cellBackdrop));
@@ -487,7 +526,7 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
         return rc;
     ctx->stats.bin_entries = totalPairs;
     ctx->stats.prim_count = totalPrims;
-    ctx->stats.tile_count = (uint32_t)(tilesX * tilesY);
+    ctx->stats.tile_count = tileTotal;
     HIP_TRY(ensure(ctx->binned, (size_t)std::max<uint32_t>(totalPairs, 1) * 4));
     if (totalPrims > 0)
         bin_scatter_kernel<<<(totalPrims + 255) / 256, 256, 0, ctx->stream>>>(
@@ -500,11 +539,12 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
 
     // ---- raster + resolve
-    raster_kernel<<<tilesX * tilesY, kWgThreads, 0, ctx->stream>>>(
-        P<PathCells>(ctx->cells), P<PathRaster>(ctx->pathRaster), pathCount,
-        P<uint32_t>(ctx->cellOffset), P<int32_t>(ctx->cellBackdrop), P<uint32_t>(ctx->binned),
-        P<Prim>(ctx->prims), P<uint32_t>(ctx->ramps), gradRows, P<ImageDesc>(ctx->imageTable), W, H, tilesX,
-        d->load_action == 1 ? 1u : 0u, premul_clear(d->clear_color), d->dither, tgt->pixels);
+    if (tileTotal > 0)
+        raster_kernel<<<tileTotal, kWgThreads, 0, ctx->stream>>>(
+            P<PathCells>(ctx->cells), P<PathRaster>(ctx->pathRaster), P<uint32_t>(ctx->cellOffset),
+            P<int32_t>(ctx->cellBackdrop), P<uint32_t>(ctx->binned), P<Prim>(ctx->prims),
+            P<uint32_t>(ctx->ramps), gradRows, P<ImageDesc>(ctx->imageTable), P<CanvasDev>(ctx->canvasTable),
+            canvasCount, d->dither);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
     return RIVE_HIP_OK;
@@ -602,11 +642,47 @@ __global__ void draw_tess_kernel(const rive_hip_draw* __restrict__ draws, uint32
 
 #include "count_kernels.hip"
 
+extern "C" int rive_hip_last_flush_canvases(rive_hip_context* ctx, rive_hip_canvas* out, uint32_t capacity,
+                                            uint32_t* count)
+{
+    if (ctx == nullptr || count == nullptr || (out == nullptr && capacity != 0))
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+    *count = (uint32_t)ctx->lastCanvases.size();
+    for (uint32_t c = 0; c < std::min(capacity, *count); ++c)
+        out[c] = ctx->lastCanvases[c];
+    return RIVE_HIP_OK;
+}
+
 extern "C" int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* d)
 {
-    if (ctx == nullptr || d == nullptr || d->target == nullptr)
+    if (ctx == nullptr || d == nullptr || (d->target == nullptr && d->canvas_count == 0))
         return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null argument");
     HIP_TRY(hipSetDevice(ctx->device));
+    // The canvas table: validated before anything is enqueued.
+    std::vector<rive_hip_canvas> canvases;
+    if (d->canvas_count == 0)
+    {
+        if (d->target->ctx != ctx)
+            return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "target belongs to another context");
+        canvases.push_back({d->target, 1u, d->path_count, d->clear_color, d->load_action});
+    }
+    else
+    {
+        if (d->canvases == nullptr)
+            return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "canvas_count without canvases");
+        uint64_t nextFree = 1;
+        for (uint32_t c = 0; c < d->canvas_count; ++c)
+        {
+            const rive_hip_canvas& cv = d->canvases[c];
+            if (cv.target == nullptr || cv.target->ctx != ctx)
+                return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null or foreign canvas target");
+            if (cv.first_path < nextFree || (uint64_t)cv.first_path + cv.path_count > (uint64_t)d->path_count + 1)
+                return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT,
+                            "canvas path ranges must ascend, not overlap and lie in 1..path_count");
+            nextFree = (uint64_t)cv.first_path + cv.path_count;
+            canvases.push_back(cv);
+        }
+    }
     for (int k = 0; k < RIVE_HIP_BUFFER_KIND_COUNT; ++k)
         if (ctx->bufs[k].mapped)
             return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "flush with a mapped buffer");
@@ -687,15 +763,16 @@ extern "C" int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* 
         return fail(RIVE_HIP_ERROR_UNSUPPORTED,
                     "draw record out of range, or contour directions the raster-ordering "
                     "pipeline does not use");
+    ctx->lastCanvases = canvases;
     ctx->lastSpanCount = spanCount;
     ctx->lastDrawCount = d->draw_count;
     ctx->lastMode = d->mode;
     {
         rive_hip_flush_stats& st = ctx->stats;
         st = rive_hip_flush_stats{};
-        st.width = d->target->width;
-        st.height = d->target->height;
-        st.load_action = d->load_action;
+        st.width = canvases[0].target->width;
+        st.height = canvases[0].target->height;
+        st.load_action = canvases[0].load_action;
         st.mode = d->mode;
         st.path_count = d->path_count;
         st.draw_count = d->draw_count;
@@ -707,7 +784,7 @@ extern "C" int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* 
         st.grad_complex_rows = d->grad_complex_rows;
         st.grad_span_count = d->grad_span_count;
         st.first_path_tess_location = d->first_path_tess_location;
-        st.clear_color = d->clear_color;
+        st.clear_color = canvases[0].clear_color;
         st.dither = d->dither;
         const auto* gradHost = static_cast<const rive_hip_grad_span*>(ctx->bufs[RIVE_HIP_BUFFER_GRAD_SPAN].host);
         if (gradHost != nullptr)

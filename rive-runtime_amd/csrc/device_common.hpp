@@ -114,6 +114,19 @@ struct PathCells
     uint32_t pad[2];
 };
 
+// One canvas of a flush: its target, tile grid and path range.  Tiles of all
+// canvases form one launch; canvas c owns tiles [tileBase, tileBase + tilesX*tilesY).
+struct CanvasDev
+{
+    uint32_t* pixels;
+    int32_t width, height, tilesX, tilesY;
+    uint32_t tileBase;
+    uint32_t firstPath, pathEnd; // paths [firstPath, pathEnd)
+    uint32_t clearPremul, loadPreserve;
+    uint32_t pad;
+};
+static_assert(sizeof(CanvasDev) == 48, "CanvasDev layout");
+
 // One draw's tessellation range (the reference's DrawBatch, gpu.hpp).
 struct DrawTess
 {

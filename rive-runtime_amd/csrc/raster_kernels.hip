@@ -577,8 +577,7 @@ __global__ void vertex_kernel(const TessTexel* __restrict__ tex, int texels, con
 __global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels, const VtxPair* __restrict__ verts,
                               const ContourV* __restrict__ contours, uint32_t contourCount,
                               const PathDataV* __restrict__ paths, const DrawTess* __restrict__ draws,
-                              uint32_t drawCount, uint32_t totalPrims, int canvasTilesX,
-                              int canvasTilesY, Prim* __restrict__ prims,
+                              uint32_t drawCount, uint32_t totalPrims, Prim* __restrict__ prims,
                               PathCells* __restrict__ cells)
 {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -659,8 +658,6 @@ __global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels, con
     // to the canvas, so geometry off the canvas still widens the grid toward
     // it.  Lanes of one path (prims of a path are contiguous) reduce first so
     // a wave issues one set of atomics per path it touches.
-    (void)canvasTilesX;
-    (void)canvasTilesY;
     int32_t tx0 = 0x7fffffff, ty0 = 0x7fffffff, tx1 = -0x7fffffff, ty1 = -0x7fffffff;
     if (!empty)
     {
@@ -725,8 +722,18 @@ __global__ void cells_init_kernel(PathCells* __restrict__ cells, uint32_t n)
 }
 
 // Single block: sizes each path's grid and assigns bases (exclusive scan).
-__global__ void cells_layout_kernel(PathCells* __restrict__ cells, uint32_t n, int tilesX, int tilesY,
-                                    uint32_t* __restrict__ totalCells)
+// The canvas of each path (0xffffffff for paths outside every canvas): one
+// block per canvas.
+__global__ void path_canvas_kernel(const CanvasDev* __restrict__ canvases, uint32_t* __restrict__ pathCanvas)
+{
+    const CanvasDev c = canvases[blockIdx.x];
+    for (uint32_t p = c.firstPath + threadIdx.x; p < c.pathEnd; p += blockDim.x)
+        pathCanvas[p] = blockIdx.x;
+}
+
+__global__ void cells_layout_kernel(PathCells* __restrict__ cells, uint32_t n,
+                                    const uint32_t* __restrict__ pathCanvas,
+                                    const CanvasDev* __restrict__ canvases, uint32_t* __restrict__ totalCells)
 {
     __shared__ uint32_t partial[1024];
     __shared__ uint32_t carry;
@@ -741,6 +748,9 @@ __global__ void cells_layout_kernel(PathCells* __restrict__ cells, uint32_t n, i
         if (i < n)
         {
             c = cells[i];
+            const uint32_t cv = pathCanvas[i];
+            const int tilesX = cv != 0xffffffffu ? canvases[cv].tilesX : 0;
+            const int tilesY = cv != 0xffffffffu ? canvases[cv].tilesY : 0;
             c.tx0 = max(c.tx0, 0);
             c.ty0 = max(c.ty0, 0);
             c.tx1 = min(c.tx1, tilesX);
@@ -1761,12 +1771,11 @@ __device__ __forceinline__ uint32_t slot_arrays(uint32_t n)
 }
 
 __global__ __launch_bounds__(kWgThreads) void raster_kernel(
-    const PathCells* __restrict__ cells, const PathRaster* __restrict__ raster, uint32_t pathCount,
+    const PathCells* __restrict__ cells, const PathRaster* __restrict__ raster,
     const uint32_t* __restrict__ cellOffset, const int32_t* __restrict__ cellBackdrop,
     const uint32_t* __restrict__ binned, const Prim* __restrict__ prims,
-    const uint32_t* __restrict__ ramps, uint32_t gradHeight, const ImageDesc* __restrict__ images, int width,
-    int height, int tilesX, uint32_t loadPreserve, uint32_t clearPremul, uint32_t dither,
-    uint32_t* __restrict__ target)
+    const uint32_t* __restrict__ ramps, uint32_t gradHeight, const ImageDesc* __restrict__ images,
+    const CanvasDev* __restrict__ canvases, uint32_t canvasCount, uint32_t dither)
 {
     __shared__ __align__(16) int32_t sAcc[kAccArrays][kTile * kTile];
     __shared__ int32_t sLeft[kAccArrays][kTile];
@@ -1786,7 +1795,22 @@ __global__ __launch_bounds__(kWgThreads) void raster_kernel(
 
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
-    const int tile = blockIdx.x;
+    // this workgroup's canvas: the last one whose first tile is at or before it
+    uint32_t lo = 0, hi = canvasCount - 1;
+    while (lo < hi)
+    {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (canvases[mid].tileBase <= blockIdx.x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const CanvasDev cv = canvases[lo];
+    const int width = cv.width, height = cv.height, tilesX = cv.tilesX;
+    uint32_t* __restrict__ target = cv.pixels;
+    const uint32_t loadPreserve = cv.loadPreserve, clearPremul = cv.clearPremul;
+    const uint32_t pathBegin = cv.firstPath, pathEnd = cv.pathEnd;
+    const int tile = (int)(blockIdx.x - cv.tileBase);
     const int tx = tile % tilesX, ty = tile / tilesX;
     const int px = tx * kTile + (t & (kTile - 1)), py = ty * kTile + (t >> kTileLog2);
     TileGeom g;
@@ -1803,14 +1827,14 @@ __global__ __launch_bounds__(kWgThreads) void raster_kernel(
     uint32_t clip = 0u;
     const float noise255 = dither ? dither_noise(px, py) * 255.f : 0.f;
 
-    for (uint32_t base = 1; base <= pathCount; base += kWgThreads)
+    for (uint32_t base = pathBegin; base < pathEnd; base += kWgThreads)
     {
         // ---- list the window's paths that touch the tile
         const uint32_t p = base + (uint32_t)t;
         uint32_t off = 0, n = 0;
         int32_t C = 0;
         bool listed = false;
-        if (p <= pathCount)
+        if (p < pathEnd)
         {
             const PathCells c = cells[p];
             if (tx >= c.tx0 && tx < c.tx1 && ty >= c.ty0 && ty < c.ty1)

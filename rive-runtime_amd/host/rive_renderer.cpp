@@ -502,6 +502,15 @@ void RenderContext::beginFrame(const FrameDescriptor& fd)
 {
     m_frame = fd;
     m_draws.clear();
+    m_canvases.clear();
+    m_lastClipID = 0;
+    m_clipContentID = 0;
+}
+
+void RenderContext::beginCanvas(const FrameDescriptor& fd, rive_hip_target* target)
+{
+    m_frame = fd;
+    m_canvases.push_back({target, m_draws.size(), fd});
     m_lastClipID = 0;
     m_clipContentID = 0;
 }
@@ -758,8 +767,17 @@ bool RenderContext::flush(rive_hip_target* target)
     }
     fd.gradSimpleRows = simpleRows;
     fd.gradComplexRows = complexRows;
+    // Batched: each canvas owns the paths of its draws (path ID = draw index + 1).
+    for (size_t c = 0; c < m_canvases.size(); ++c)
+    {
+        const CanvasRecord& cr = m_canvases[c];
+        const size_t end = c + 1 < m_canvases.size() ? m_canvases[c + 1].firstDraw : m_draws.size();
+        fd.canvases.push_back({cr.target, (uint32_t)cr.firstDraw + 1, (uint32_t)(end - cr.firstDraw),
+                               cr.frame.clearColor, cr.frame.loadAction});
+    }
     bool ok = m_impl->flush(fd);
     m_draws.clear();
+    m_canvases.clear();
     return ok;
 }
 } // namespace gpu
@@ -796,6 +814,8 @@ bool RenderContextHIPImpl::flush(const gpu::FlushDescriptor& fd)
         return false;
     rive_hip_flush_desc d{};
     d.target = fd.renderTarget;
+    d.canvases = fd.canvases.data();
+    d.canvas_count = (uint32_t)fd.canvases.size();
     d.mode = RIVE_HIP_FLUSH_GEOMETRY;
     d.load_action = fd.loadAction;
     d.clear_color = fd.clearColor;
@@ -814,6 +834,7 @@ bool RenderContextHIPImpl::flush(const gpu::FlushDescriptor& fd)
 
 // Replays a rive_stream through RiveRenderer (the draw-stream contract in
 // include/rive_hip.h).
+constexpr uint64_t kMaxContoursPerFlush = 65535; // contour IDs are 16 bits
 namespace
 {
 struct StreamReplay
@@ -830,11 +851,11 @@ struct StreamReplay
 };
 } // namespace
 
-extern "C" int rive_hip_render_stream(rive_hip_context* ctx, const rive_stream* s, rive_hip_target* target)
+namespace
 {
-    if (ctx == nullptr || s == nullptr || target == nullptr)
-        return RIVE_HIP_ERROR_INVALID_ARGUMENT;
-    StreamReplay r;
+// The stream's paths and images as front-end objects.
+int load_stream(rive_hip_context* ctx, const rive_stream* s, StreamReplay& r)
+{
     r.paths.reserve(s->path_count);
     for (uint32_t i = 0; i < s->path_count; ++i)
     {
@@ -884,14 +905,14 @@ extern "C" int rive_hip_render_stream(rive_hip_context* ctx, const rive_stream* 
         r.handles.push_back(h);
         r.images.emplace_back(h, si.width, si.height);
     }
-    rive::gpu::RenderContext rc(std::make_unique<rive::RenderContextHIPImpl>(ctx));
-    rive::gpu::RenderContext::FrameDescriptor fd;
-    fd.renderTargetWidth = s->width;
-    fd.renderTargetHeight = s->height;
-    fd.clearColor = s->clear_color;
-    fd.ditherMode = s->dither;
-    rc.beginFrame(fd);
-    rive::RiveRenderer renderer(&rc);
+    return RIVE_HIP_OK;
+}
+
+// Replays the stream's draws through `renderer` (whose frame or canvas is
+// already begun).  The renderer, like the paths and paints, must outlive the
+// flush: image draws reference its unit rect.
+int draw_stream(rive::RiveRenderer& renderer, const rive_stream* s, StreamReplay& r)
+{
     r.paints.resize(s->draw_count);
     for (uint32_t i = 0; i < s->draw_count; ++i)
     {
@@ -933,5 +954,97 @@ extern "C" int rive_hip_render_stream(rive_hip_context* ctx, const rive_stream* 
             renderer.drawPath(&r.paths[d.path], &paint);
         renderer.restore();
     }
+    return RIVE_HIP_OK;
+}
+
+rive::gpu::RenderContext::FrameDescriptor stream_frame(const rive_stream* s)
+{
+    rive::gpu::RenderContext::FrameDescriptor fd;
+    fd.renderTargetWidth = s->width;
+    fd.renderTargetHeight = s->height;
+    fd.clearColor = s->clear_color;
+    fd.ditherMode = s->dither;
+    return fd;
+}
+} // namespace
+
+extern "C" int rive_hip_render_stream(rive_hip_context* ctx, const rive_stream* s, rive_hip_target* target)
+{
+    if (ctx == nullptr || s == nullptr || target == nullptr)
+        return RIVE_HIP_ERROR_INVALID_ARGUMENT;
+    StreamReplay r;
+    int st = load_stream(ctx, s, r);
+    if (st != RIVE_HIP_OK)
+        return st;
+    rive::gpu::RenderContext rc(std::make_unique<rive::RenderContextHIPImpl>(ctx));
+    rc.beginFrame(stream_frame(s));
+    rive::RiveRenderer renderer(&rc);
+    st = draw_stream(renderer, s, r);
+    if (st != RIVE_HIP_OK)
+        return st;
     return rc.flush(target) ? RIVE_HIP_OK : RIVE_HIP_ERROR_DEVICE;
+}
+
+extern "C" uint32_t rive_stream_contour_bound(const rive_stream* s)
+{
+    if (s == nullptr)
+        return 0;
+    auto moves = [s](uint32_t path) {
+        if (path >= s->path_count)
+            return 0u;
+        const rive_stream_path& sp = s->paths[path];
+        uint32_t m = 0;
+        for (uint32_t k = 0; k < sp.verb_count; ++k)
+            m += s->verbs[sp.first_verb + k] == 0 ? 1u : 0u;
+        return m;
+    };
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < s->draw_count; ++i)
+    {
+        const rive_stream_draw& d = s->draws[i];
+        total += d.image != 0 ? 1u : moves(d.path);
+        for (uint32_t c = 0; c < d.clip_count && c < 2; ++c)
+            total += moves(d.clip_paths[c]);
+    }
+    return (uint32_t)std::min<uint64_t>(total, 0xffffffffu);
+}
+
+extern "C" int rive_hip_render_streams(rive_hip_context* ctx, const rive_stream* const* streams,
+                                       rive_hip_target* const* targets, uint32_t n)
+{
+    if (ctx == nullptr || streams == nullptr || targets == nullptr || n == 0)
+        return RIVE_HIP_ERROR_INVALID_ARGUMENT;
+    for (uint32_t i = 0; i < n; ++i)
+        if (streams[i] == nullptr || targets[i] == nullptr)
+            return RIVE_HIP_ERROR_INVALID_ARGUMENT;
+    // Consecutive canvases share a flush while their contours fit the 16-bit
+    // contour IDs (as the reference starts a new logical flush when a
+    // resource limit is reached, render_context.cpp:1012-1016).
+    uint32_t begin = 0;
+    while (begin < n)
+    {
+        uint64_t contours = rive_stream_contour_bound(streams[begin]);
+        uint32_t end = begin + 1;
+        while (end < n && contours + rive_stream_contour_bound(streams[end]) <= kMaxContoursPerFlush)
+            contours += rive_stream_contour_bound(streams[end++]);
+        std::vector<StreamReplay> replays(end - begin);
+        std::vector<std::unique_ptr<rive::RiveRenderer>> renderers;
+        rive::gpu::RenderContext rc(std::make_unique<rive::RenderContextHIPImpl>(ctx));
+        rc.beginFrame(stream_frame(streams[begin]));
+        for (uint32_t i = begin; i < end; ++i)
+        {
+            int st = load_stream(ctx, streams[i], replays[i - begin]);
+            if (st != RIVE_HIP_OK)
+                return st;
+            rc.beginCanvas(stream_frame(streams[i]), targets[i]);
+            renderers.push_back(std::make_unique<rive::RiveRenderer>(&rc));
+            st = draw_stream(*renderers.back(), streams[i], replays[i - begin]);
+            if (st != RIVE_HIP_OK)
+                return st;
+        }
+        if (!rc.flush(targets[begin]))
+            return RIVE_HIP_ERROR_DEVICE;
+        begin = end;
+    }
+    return RIVE_HIP_OK;
 }

@@ -282,6 +282,7 @@ struct FlushDescriptor
     std::vector<rive_hip_grad_span> gradSpans;
     uint32_t gradSimpleRows = 0, gradComplexRows = 0;
     std::vector<rive_hip_image*> images;
+    std::vector<rive_hip_canvas> canvases; // batched flush; empty: renderTarget
 };
 
 // Backend plugin interface (render_context_impl.hpp:26-243, reduced to what
@@ -305,6 +306,10 @@ public:
     };
     explicit RenderContext(std::unique_ptr<RenderContextImpl> impl) : m_impl(std::move(impl)) {}
     void beginFrame(const FrameDescriptor& fd);
+    // Starts the next canvas of a batched flush: draws from here on render
+    // into `target` with this frame's size and clear color.  Each canvas
+    // starts with an empty clip buffer.
+    void beginCanvas(const FrameDescriptor& fd, rive_hip_target* target);
     bool flush(rive_hip_target* target);
     const FrameDescriptor& frameDescriptor() const { return m_frame; }
     RenderContextImpl* impl() { return m_impl.get(); }
@@ -321,6 +326,13 @@ private:
     std::unique_ptr<RenderContextImpl> m_impl;
     FrameDescriptor m_frame;
     std::vector<DrawRecord> m_draws;
+    struct CanvasRecord
+    {
+        rive_hip_target* target;
+        size_t firstDraw;
+        FrameDescriptor frame;
+    };
+    std::vector<CanvasRecord> m_canvases;
     uint32_t m_lastClipID = 0;
     uint32_t m_clipContentID = 0;
 };

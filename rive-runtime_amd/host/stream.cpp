@@ -253,6 +253,28 @@ extern "C" int rive_stream_generate(int config, uint64_t seed, uint32_t width, u
     return RIVE_HIP_OK;
 }
 
+extern "C" int rive_stream_generate_batch(uint64_t seed, uint32_t n, rive_stream** out)
+{
+    // The batched workload: canvases of log-uniform size and aspect, each
+    // with a log-uniform number of config-1 paths over its own area.
+    if (out == nullptr || n == 0)
+        return RIVE_HIP_ERROR_INVALID_ARGUMENT;
+    Rand r(seed);
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        const float side = expf(r.f32(logf(64.f), logf(1024.f)));
+        const float aspect = expf(r.f32(logf(.5f), logf(2.f)));
+        const uint32_t W = std::max<uint32_t>(1, (uint32_t)lroundf(side * sqrtf(aspect)));
+        const uint32_t H = std::max<uint32_t>(1, (uint32_t)lroundf(side / sqrtf(aspect)));
+        const uint32_t paths = (uint32_t)lroundf(expf(r.f32(logf(10.f), logf(2000.f))));
+        auto* o = new OwnedStream;
+        gen_config1(o, r, W, H, paths);
+        o->finish(W, H);
+        out[i] = &o->s;
+    }
+    return RIVE_HIP_OK;
+}
+
 extern "C" void rive_stream_free(rive_stream* stream)
 {
     if (stream == nullptr)

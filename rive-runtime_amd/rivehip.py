@@ -67,7 +67,15 @@ class FlushDesc(ctypes.Structure):
                 ("grad_span_count", ctypes.c_uint32), ("grad_simple_rows", ctypes.c_uint32),
                 ("grad_complex_rows", ctypes.c_uint32),
                 ("first_path_tess_location", ctypes.c_uint32), ("images", ctypes.c_void_p),
-                ("image_count", ctypes.c_uint32)]
+                ("image_count", ctypes.c_uint32), ("canvases", ctypes.c_void_p),
+                ("canvas_count", ctypes.c_uint32)]
+
+
+class Canvas(ctypes.Structure):
+    """rive_hip_canvas: one canvas of a batched flush."""
+    _fields_ = [("target", ctypes.c_void_p), ("first_path", ctypes.c_uint32),
+                ("path_count", ctypes.c_uint32), ("clear_color", ctypes.c_uint32),
+                ("load_action", ctypes.c_uint32)]
 
 
 class FlushStats(ctypes.Structure):
@@ -87,7 +95,8 @@ EXPORTED = [
     "rive_hip_target_destroy", "rive_hip_flush", "rive_hip_sync", "rive_hip_read_pixels",
     "rive_hip_debug_readback", "rive_hip_pass_times", "rive_stream_generate", "rive_stream_free",
     "rive_hip_render_stream", "rive_hip_last_flush_stats", "rive_hip_image_create",
-    "rive_hip_image_destroy",
+    "rive_hip_image_destroy", "rive_hip_render_streams", "rive_hip_last_flush_canvases",
+    "rive_stream_generate_batch", "rive_stream_contour_bound",
 ]
 
 _lib = None
@@ -123,6 +132,12 @@ def lib():
     L.rive_stream_free.argtypes = [ctypes.POINTER(Stream)]
     L.rive_stream_free.restype = None
     L.rive_hip_render_stream.argtypes = [vp, ctypes.POINTER(Stream), vp]
+    L.rive_hip_render_streams.argtypes = [vp, ctypes.POINTER(ctypes.POINTER(Stream)),
+                                          ctypes.POINTER(vp), u32]
+    L.rive_hip_last_flush_canvases.argtypes = [vp, ctypes.POINTER(Canvas), u32, ctypes.POINTER(u32)]
+    L.rive_stream_generate_batch.argtypes = [ctypes.c_uint64, u32, ctypes.POINTER(ctypes.POINTER(Stream))]
+    L.rive_stream_contour_bound.argtypes = [ctypes.POINTER(Stream)]
+    L.rive_stream_contour_bound.restype = u32
     L.rive_hip_last_flush_stats.argtypes = [vp, ctypes.POINTER(FlushStats)]
     L.rive_hip_image_create.argtypes = [vp, u32, u32, u32, vp, ctypes.POINTER(vp)]
     L.rive_hip_image_destroy.argtypes = [vp]
@@ -136,7 +151,7 @@ def check(rc):
         raise RuntimeError(f"rive_hip error {rc}: {lib().rive_hip_last_error().decode()}")
 
 
-CONFIG_SEEDS = {1: 0x5249564500000001, 2: 0x5249564500000002}
+CONFIG_SEEDS = {1: 0x5249564500000001, 2: 0x5249564500000002, 4: 0x5249564500000004}
 
 
 def generate_stream(config, seed=None, width=0, height=0, draw_count=0):
@@ -150,6 +165,35 @@ def generate_stream(config, seed=None, width=0, height=0, draw_count=0):
 
 def free_stream(p):
     lib().rive_stream_free(p)
+
+
+def contour_bound(stream):
+    """Upper bound of the contours a stream produces (rive_stream_contour_bound)."""
+    return int(lib().rive_stream_contour_bound(stream))
+
+
+def flush_groups(streams, limit=65535):
+    """Consecutive runs of streams whose contour bounds sum to at most `limit`
+    (the canvases rive_hip_render_streams puts in one flush)."""
+    groups, cur, total = [], [], 0
+    for i, s in enumerate(streams):
+        b = contour_bound(s)
+        if cur and total + b > limit:
+            groups.append(cur)
+            cur, total = [], 0
+        cur.append(i)
+        total += b
+    if cur:
+        groups.append(cur)
+    return groups
+
+
+def generate_batch(n=256, seed=None):
+    """The batched workload's canvases (see rive_stream_generate_batch)."""
+    L = lib()
+    arr = (ctypes.POINTER(Stream) * n)()
+    check(L.rive_stream_generate_batch(CONFIG_SEEDS[4] if seed is None else seed, n, arr))
+    return list(arr)
 
 
 class Context:
@@ -213,17 +257,35 @@ class Context:
     def render_stream(self, stream, target):
         check(self.L.rive_hip_render_stream(self.h, stream, target))
 
+    def render_streams(self, streams, targets):
+        """One batched flush: stream i renders into targets[i]."""
+        n = len(streams)
+        sa = (ctypes.POINTER(Stream) * n)(*streams)
+        ta = (ctypes.c_void_p * n)(*[t.value if isinstance(t, ctypes.c_void_p) else t for t in targets])
+        check(self.L.rive_hip_render_streams(self.h, sa, ta, n))
+
+    def last_flush_canvases(self):
+        count = ctypes.c_uint32()
+        check(self.L.rive_hip_last_flush_canvases(self.h, None, 0, ctypes.byref(count)))
+        arr = (Canvas * max(count.value, 1))()
+        check(self.L.rive_hip_last_flush_canvases(self.h, arr, count.value, ctypes.byref(count)))
+        return arr, count.value
+
     def last_flush_stats(self):
         st = FlushStats()
         check(self.L.rive_hip_last_flush_stats(self.h, ctypes.byref(st)))
         return st
 
-    def reflush_geometry(self, target, stats):
+    def reflush_geometry(self, target, stats, canvases=None):
         """Flushes again from the records already resident in HBM (the last
         GEOMETRY flush's PATH/PAINT/PAINT_AUX/DRAW/VERB/POINT/GRAD_SPAN
-        buffers), with no host writes."""
+        buffers), with no host writes.  canvases: a (Canvas array, count)
+        pair from last_flush_canvases() for a batched flush."""
         desc = FlushDesc()
         desc.target = target
+        if canvases is not None:
+            desc.canvases = ctypes.cast(canvases[0], ctypes.c_void_p)
+            desc.canvas_count = canvases[1]
         desc.mode = FLUSH_GEOMETRY
         desc.load_action = stats.load_action
         desc.clear_color = stats.clear_color

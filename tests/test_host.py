@@ -168,3 +168,29 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_batch_generator_is_seeded_and_in_range():
+    """The batched workload's canvases: log-uniform sides 64..1024 with aspect
+    .5..2 and 10..2000 closed cubic fills each, reproducible from the seed."""
+    import rivehip
+
+    a = rivehip.generate_batch(24, seed=123)
+    b = rivehip.generate_batch(24, seed=123)
+    try:
+        for sa, sb in zip(a, b):
+            x, y = sa.contents, sb.contents
+            assert (x.width, x.height, x.draw_count) == (y.width, y.height, y.draw_count)
+            assert 45 <= x.width <= 1449 and 45 <= x.height <= 1449
+            assert 0.5 * 0.98 <= x.width / x.height <= 2.0 * 1.02
+            assert 10 <= x.draw_count <= 2000
+            n = x.point_count * 2
+            fp = ctypes.POINTER(ctypes.c_float)
+            pa = np.ctypeslib.as_array(ctypes.cast(x.points, fp), (n,))
+            pb = np.ctypeslib.as_array(ctypes.cast(y.points, fp), (n,))
+            assert np.array_equal(pa, pb)
+            assert pa[0::2].min() >= 0 and pa[0::2].max() < x.width
+        assert len({(s.contents.width, s.contents.height) for s in a}) > 20
+    finally:
+        for s in a + b:
+            rivehip.free_stream(s)
