@@ -1592,16 +1592,16 @@ __device__ __forceinline__ void sample_gradient255(const PathRaster& pr, int px,
 
 // Advanced blend modes and image paints (draw_path.vert:431-545,
 // draw_raster_order_path.frag:191-204), in 0..1 scale.
-__device__ __noinline__ void resolve_general(const PathRaster* __restrict__ prp, float coverage, int px, int py,
-                                             float noise255, const uint32_t* __restrict__ ramps,
-                                             uint32_t gradHeight, const ImageDesc* __restrict__ images,
-                                             uint32_t* colPacked)
+__device__ __noinline__ uint32_t resolve_general(const PathRaster* __restrict__ prp, float coverage, int px,
+                                                 int py, float noise255, const uint32_t* __restrict__ ramps,
+                                                 uint32_t gradHeight, const ImageDesc* __restrict__ images,
+                                                 uint32_t colPacked)
 {
     const PathRaster pr = *prp;
     float col[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-        col[k] = float((*colPacked >> (8 * k)) & 0xffu) * (1.f / 255.f);
+        col[k] = float((colPacked >> (8 * k)) & 0xffu) * (1.f / 255.f);
     float src[4];
     const uint32_t blendMode = (pr.flags >> 8) & 0xfu;
     if (pr.paintType == SOLID_COLOR_PAINT_TYPE && blendMode == 0u)
@@ -1672,7 +1672,7 @@ __device__ __noinline__ void resolve_general(const PathRaster* __restrict__ prp,
         out[1] += noise255;
         out[2] += noise255;
     }
-    *colPacked = pack_unorm8x4(out);
+    return pack_unorm8x4(out);
 }
 
 // Clip, paint and src-over of one path into one pixel (draw_path.vert:431-545,
@@ -1723,7 +1723,7 @@ __device__ __forceinline__ void resolve_pixel(const PathRaster& pr, const PathRa
         coverage = fminf(fmaxf(clip_rect_coverage(pr.clipRect, px, py), 0.f), coverage);
     if ((pr.flags & 0xf00u) != 0u || pr.image != 0u)
     {
-        resolve_general(prp, coverage, px, py, noise255, ramps, gradHeight, images, &colPacked);
+        colPacked = resolve_general(prp, coverage, px, py, noise255, ramps, gradHeight, images, colPacked);
         return;
     }
     float src[4], a;
