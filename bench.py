@@ -37,10 +37,12 @@ METRIC = "1080p 5k-path frames/s + HBM roofline % (plain vs batched); p50/p99 fr
 HBM_PEAK_GBS = 8000.0
 W, H, DRAWS = 1920, 1080, 5000
 SEED = rivehip.CONFIG_SEEDS[2]
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of raster_kernel on this workload
-# (KiB per dispatch, scaled by the copy-kernel calibration factor); None
-# until measured.
-RASTER_TRAFFIC_BYTES = None
+# HBM traffic of raster_kernel on this workload from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by what the same
+# counter reads for a 1 GiB device copy (FETCH x2.0, WRITE x1.0); the runs are
+# summarised in profiles/raster_kernel_1080p_5000paths_traffic.md.
+RASTER_FETCH_KIB, RASTER_WRITE_KIB = 10062363, 8100
+RASTER_TRAFFIC_BYTES = RASTER_FETCH_KIB * 1024 * 2 + RASTER_WRITE_KIB * 1024
 
 
 def percentile(xs, q):
