@@ -1000,7 +1000,7 @@ __device__ __forceinline__ float dither_noise(int x, int y)
 // pixel in that order.
 //
 // The paths that touch the tile (primitives binned to it, or a nonzero
-// corner winding) are listed 256 at a time and cut into batches of at most
+// corner winding) are listed kWindow at a time and cut into batches of at most
 // kBatchEntries paths, kAccArrays accumulator arrays and, unless the batch
 // has a single path with primitives, kBatchPrims primitives.  Each path with
 // primitives owns one LDS accumulator array, two when it has more than
@@ -1027,6 +1027,7 @@ __device__ __forceinline__ float dither_noise(int x, int y)
 constexpr int kWgThreads = 256;
 constexpr int kAccArrays = 16;
 constexpr int kBatchEntries = 64; // one wave's ballot
+constexpr int kWindow = 128;      // paths listed per round (LDS: six workgroups per CU)
 constexpr uint32_t kBatchPrims = kWgThreads;
 constexpr uint32_t kPackedPrims = 127;
 constexpr int kMaxItems = kBatchPrims * kTile;
@@ -1770,7 +1771,7 @@ __device__ __forceinline__ uint32_t slot_arrays(uint32_t n)
     return n == 0 ? 0u : (1u | ((n > kPackedPrims ? 2u : 1u) << 16));
 }
 
-__global__ __launch_bounds__(kWgThreads) void raster_kernel(
+__global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
     const PathCells* __restrict__ cells, const PathRaster* __restrict__ raster,
     const uint32_t* __restrict__ cellOffset, const int32_t* __restrict__ cellBackdrop,
     const uint32_t* __restrict__ binned, const Prim* __restrict__ prims,
@@ -1781,12 +1782,12 @@ __global__ __launch_bounds__(kWgThreads) void raster_kernel(
     __shared__ int32_t sLeft[kAccArrays][kTile];
     // The listed paths of the current window of 256 paths, in order, with
     // exclusive prefixes of their primitive counts and of slot_arrays().
-    __shared__ uint32_t sRelPath[kWgThreads];
-    __shared__ uint32_t sRelOff[kWgThreads];
-    __shared__ uint32_t sRelN[kWgThreads];
-    __shared__ int32_t sRelC[kWgThreads];
-    __shared__ uint32_t sRelPrims[kWgThreads];
-    __shared__ uint32_t sRelSA[kWgThreads];
+    __shared__ uint32_t sRelPath[kWindow];
+    __shared__ uint32_t sRelOff[kWindow];
+    __shared__ uint32_t sRelN[kWindow];
+    __shared__ int32_t sRelC[kWindow];
+    __shared__ uint32_t sRelPrims[kWindow];
+    __shared__ uint32_t sRelSA[kWindow];
     __shared__ uint32_t sWave[3][kWgThreads / 64];
     __shared__ uint8_t sArrEntry[kAccArrays]; // batch entry whose first array is a, else 0xff
     __shared__ uint8_t sOwner[kMaxItems];
@@ -1827,14 +1828,14 @@ __global__ __launch_bounds__(kWgThreads) void raster_kernel(
     uint32_t clip = 0u;
     const float noise255 = dither ? dither_noise(px, py) * 255.f : 0.f;
 
-    for (uint32_t base = pathBegin; base < pathEnd; base += kWgThreads)
+    for (uint32_t base = pathBegin; base < pathEnd; base += kWindow)
     {
         // ---- list the window's paths that touch the tile
         const uint32_t p = base + (uint32_t)t;
         uint32_t off = 0, n = 0;
         int32_t C = 0;
         bool listed = false;
-        if (p < pathEnd)
+        if (t < kWindow && p < pathEnd)
         {
             const PathCells c = cells[p];
             if (tx >= c.tx0 && tx < c.tx1 && ty >= c.ty0 && ty < c.ty1)
