@@ -507,9 +507,9 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     HIP_TRY(hipMemsetAsync(ctx->cellCount.ptr, 0, (size_t)(totalCells + 1) * 4, ctx->stream));
     HIP_TRY(hipMemsetAsync(ctx->cellCursor.ptr, 0, (size_t)(totalCells + 1) * 4, ctx->stream));
     HIP_TRY(hipMemsetAsync(ctx->cellBackdrop.ptr, 0, (size_t)(totalCells + 1) * 4, ctx->stream));
-    if (totalPrims > 0)
-        bin_count_kernel<<<(totalPrims + 255) / 256, 256, 0, ctx->stream>>>(
-            P<Prim>(ctx->prims), totalPrims, P<PathCells>(ctx->cells), P<uint32_t>(ctx->cellCount),
+    if (totalPrims > 0 && drawCount > 0)
+        bin_count_draw_kernel<<<drawCount, 256, 0, ctx->stream>>>(
+            P<Prim>(ctx->prims), drawTess, P<PathCells>(ctx->cells), P<uint32_t>(ctx->cellCount),
             P<int32_t>(ctx->cellBackdrop));
     rc = device_scan(ctx, P<uint32_t>(ctx->cellCount), totalCells, P<uint32_t>(ctx->cellOffset), scal + 2);
     if (rc)
@@ -528,9 +528,9 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     ctx->stats.prim_count = totalPrims;
     ctx->stats.tile_count = tileTotal;
     HIP_TRY(ensure(ctx->binned, (size_t)std::max<uint32_t>(totalPairs, 1) * 4));
-    if (totalPrims > 0)
-        bin_scatter_kernel<<<(totalPrims + 255) / 256, 256, 0, ctx->stream>>>(
-            P<Prim>(ctx->prims), totalPrims, P<PathCells>(ctx->cells), P<uint32_t>(ctx->cellOffset),
+    if (totalPrims > 0 && drawCount > 0)
+        bin_scatter_draw_kernel<<<drawCount, 256, 0, ctx->stream>>>(
+            P<Prim>(ctx->prims), drawTess, P<PathCells>(ctx->cells), P<uint32_t>(ctx->cellOffset),
             P<uint32_t>(ctx->cellCursor), P<uint32_t>(ctx->binned));
     HIP_TRY(ensure(ctx->pathRaster, (size_t)(pathCount + 1) * sizeof(PathRaster)));
     path_raster_kernel<<<(pathCount + 1 + 255) / 256, 256, 0, ctx->stream>>>(

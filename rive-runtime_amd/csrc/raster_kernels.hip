@@ -812,25 +812,25 @@ __device__ __forceinline__ bool right_of(int32_t ax, int32_t ay, int32_t bx, int
     return by < ay ? D >= 0 : D <= 0;
 }
 
-__global__ void bin_count_kernel(const Prim* __restrict__ prims, uint32_t totalPrims,
-                                 const PathCells* __restrict__ cells, uint32_t* __restrict__ cellCount,
-                                 int32_t* __restrict__ cellBackdrop)
+// The cells of a path's grid that primitive p covers (row-major, grid-local
+// index), passed to visit(cell).
+template <typename F>
+__device__ __forceinline__ void for_prim_cells(const Prim& p, const PathCells& c, F visit)
 {
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= totalPrims)
-        return;
-    const Prim p = prims[g];
-    if (p.flags & kPrimEmpty)
-        return;
-    const PathCells c = cells[p.path];
     int tx0, ty0, tx1, ty1;
     prim_tile_range(p, c, &tx0, &ty0, &tx1, &ty1);
     for (int ty = ty0; ty < ty1; ++ty)
         for (int tx = tx0; tx < tx1; ++tx)
-            atomicAdd(&cellCount[c.base + (uint32_t)(ty - c.ty0) * c.width + (uint32_t)(tx - c.tx0)], 1u);
+            visit((uint32_t)(ty - c.ty0) * c.width + (uint32_t)(tx - c.tx0));
+}
+
+// Corner crossings of a fill primitive's inner edge I0->I1 on tile-row lines:
+// visit(cell, +-1) at the first corner of each crossed row right of the edge.
+template <typename F>
+__device__ __forceinline__ void for_prim_crossings(const Prim& p, const PathCells& c, F visit)
+{
     if (!(p.flags & kPrimFill))
         return;
-    // Corner crossings of the inner edge I0->I1 on tile-row lines.
     const int32_t ax = p.p[0], ay = p.p[1], bx = p.p[4], by = p.p[5];
     if (ay == by)
         return;
@@ -838,22 +838,143 @@ __global__ void bin_count_kernel(const Prim* __restrict__ prims, uint32_t totalP
     const int s = (by < ay ? 1 : -1) * ((p.flags & kPrimNegate) ? -1 : 1);
     const int32_t ymin = min(ay, by), ymax = max(ay, by);
     const int unit = 1 << (kSubpixelLog2 + kTileLog2);
-    int tyA = max(c.ty0, (ymin + unit - 1) >> (kSubpixelLog2 + kTileLog2));
+    const int tyA = max(c.ty0, (ymin + unit - 1) >> (kSubpixelLog2 + kTileLog2));
     for (int ty = tyA; ty < c.ty1; ++ty)
     {
-        int64_t Y = (int64_t)ty * unit;
+        const int64_t Y = (int64_t)ty * unit;
         if (!(ymin <= Y && Y < ymax))
             break;
         // Smallest column c with corner (c*unit, Y) right of the edge.
-        double xe = ax + (double)(bx - ax) * (double)(Y - ay) / (double)(by - ay);
+        const double xe = ax + (double)(bx - ax) * (double)(Y - ay) / (double)(by - ay);
         int64_t cc = (int64_t)floor(xe / unit) + 1;
         while (right_of(ax, ay, bx, by, (cc - 1) * unit, Y))
             --cc;
         while (!right_of(ax, ay, bx, by, cc * unit, Y))
             ++cc;
-        int col = (int)max<int64_t>(cc, (int64_t)c.tx0);
+        const int col = (int)max<int64_t>(cc, (int64_t)c.tx0);
         if (col < c.tx1)
-            atomicAdd(&cellBackdrop[c.base + (uint32_t)(ty - c.ty0) * c.width + (uint32_t)(col - c.tx0)], s);
+            visit((uint32_t)(ty - c.ty0) * c.width + (uint32_t)(col - c.tx0), s);
+    }
+}
+
+// Binning per draw: one 256-thread workgroup takes one draw's primitives and
+// accumulates its (primitive, tile) counts and corner crossings over its
+// path's cell grid in LDS, then adds them to the global grids once per cell.
+// Grids larger than kBinLdsCells use global atomics directly.
+constexpr uint32_t kBinLdsCells = 8192;
+
+__device__ __forceinline__ uint32_t grid_cells(const PathCells& c)
+{
+    return c.tx0 < c.tx1 && c.ty0 < c.ty1 ? c.width * (uint32_t)(c.ty1 - c.ty0) : 0u;
+}
+
+__global__ __launch_bounds__(256) void bin_count_draw_kernel(const Prim* __restrict__ prims,
+                                                             const DrawTess* __restrict__ draws,
+                                                             const PathCells* __restrict__ cells,
+                                                             uint32_t* __restrict__ cellCount,
+                                                             int32_t* __restrict__ cellBackdrop)
+{
+    __shared__ uint32_t sCount[kBinLdsCells];
+    __shared__ int32_t sBack[kBinLdsCells];
+    const DrawTess d = draws[blockIdx.x];
+    if (d.pathID == 0 || d.forwardCount == 0)
+        return;
+    const PathCells c = cells[d.pathID];
+    const uint32_t nCells = grid_cells(c);
+    if (nCells == 0)
+        return;
+    const bool local = nCells <= kBinLdsCells;
+    if (local)
+    {
+        for (uint32_t i = threadIdx.x; i < nCells; i += blockDim.x)
+        {
+            sCount[i] = 0;
+            sBack[i] = 0;
+        }
+        __syncthreads();
+    }
+    uint32_t* gCount = cellCount + c.base;
+    int32_t* gBack = cellBackdrop + c.base;
+    for (uint32_t k = threadIdx.x; k < d.forwardCount; k += blockDim.x)
+    {
+        const Prim p = prims[d.primBase + k];
+        if (p.flags & kPrimEmpty)
+            continue;
+        if (local)
+        {
+            for_prim_cells(p, c, [&](uint32_t cell) { atomicAdd(&sCount[cell], 1u); });
+            for_prim_crossings(p, c, [&](uint32_t cell, int v) { atomicAdd(&sBack[cell], v); });
+        }
+        else
+        {
+            for_prim_cells(p, c, [&](uint32_t cell) { atomicAdd(&gCount[cell], 1u); });
+            for_prim_crossings(p, c, [&](uint32_t cell, int v) { atomicAdd(&gBack[cell], v); });
+        }
+    }
+    if (local)
+    {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nCells; i += blockDim.x)
+        {
+            if (sCount[i] != 0)
+                atomicAdd(&gCount[i], sCount[i]);
+            if (sBack[i] != 0)
+                atomicAdd(&gBack[i], sBack[i]);
+        }
+    }
+}
+
+// Writes each primitive index into its cells' bins: counts in LDS, one global
+// reservation per cell, then LDS cursors hand out the slots.
+__global__ __launch_bounds__(256) void bin_scatter_draw_kernel(const Prim* __restrict__ prims,
+                                                               const DrawTess* __restrict__ draws,
+                                                               const PathCells* __restrict__ cells,
+                                                               const uint32_t* __restrict__ cellOffset,
+                                                               uint32_t* __restrict__ cellCursor,
+                                                               uint32_t* __restrict__ binned)
+{
+    __shared__ uint32_t sCur[kBinLdsCells];
+    const DrawTess d = draws[blockIdx.x];
+    if (d.pathID == 0 || d.forwardCount == 0)
+        return;
+    const PathCells c = cells[d.pathID];
+    const uint32_t nCells = grid_cells(c);
+    if (nCells == 0)
+        return;
+    const uint32_t* gOffset = cellOffset + c.base;
+    uint32_t* gCursor = cellCursor + c.base;
+    if (nCells > kBinLdsCells)
+    {
+        for (uint32_t k = threadIdx.x; k < d.forwardCount; k += blockDim.x)
+        {
+            const uint32_t g = d.primBase + k;
+            const Prim p = prims[g];
+            if (p.flags & kPrimEmpty)
+                continue;
+            for_prim_cells(p, c, [&](uint32_t cell) { binned[gOffset[cell] + atomicAdd(&gCursor[cell], 1u)] = g; });
+        }
+        return;
+    }
+    for (uint32_t i = threadIdx.x; i < nCells; i += blockDim.x)
+        sCur[i] = 0;
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < d.forwardCount; k += blockDim.x)
+    {
+        const Prim p = prims[d.primBase + k];
+        if (!(p.flags & kPrimEmpty))
+            for_prim_cells(p, c, [&](uint32_t cell) { atomicAdd(&sCur[cell], 1u); });
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nCells; i += blockDim.x)
+        if (sCur[i] != 0)
+            sCur[i] = gOffset[i] + atomicAdd(&gCursor[i], sCur[i]);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < d.forwardCount; k += blockDim.x)
+    {
+        const uint32_t g = d.primBase + k;
+        const Prim p = prims[g];
+        if (!(p.flags & kPrimEmpty))
+            for_prim_cells(p, c, [&](uint32_t cell) { binned[atomicAdd(&sCur[cell], 1u)] = g; });
     }
 }
 
@@ -889,29 +1010,6 @@ __global__ __launch_bounds__(256) void backdrop_prefix_kernel(const PathCells* _
             r[x] = carry + v;
         carry += __shfl(v, 63);
     }
-}
-
-__global__ void bin_scatter_kernel(const Prim* __restrict__ prims, uint32_t totalPrims,
-                                   const PathCells* __restrict__ cells,
-                                   const uint32_t* __restrict__ cellOffset,
-                                   uint32_t* __restrict__ cellCursor, uint32_t* __restrict__ binned)
-{
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= totalPrims)
-        return;
-    const Prim p = prims[g];
-    if (p.flags & kPrimEmpty)
-        return;
-    const PathCells c = cells[p.path];
-    int tx0, ty0, tx1, ty1;
-    prim_tile_range(p, c, &tx0, &ty0, &tx1, &ty1);
-    for (int ty = ty0; ty < ty1; ++ty)
-        for (int tx = tx0; tx < tx1; ++tx)
-        {
-            uint32_t cell = c.base + (uint32_t)(ty - c.ty0) * c.width + (uint32_t)(tx - c.tx0);
-            uint32_t slot = atomicAdd(&cellCursor[cell], 1u);
-            binned[cellOffset[cell] + slot] = g;
-        }
 }
 
 // ---------------------------------------------------------------------------
