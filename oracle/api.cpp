@@ -26,6 +26,11 @@ template <typename T> size_t copy_out(const std::vector<T>& v, void* dst, size_t
 
 extern "C" {
 
+int oracle_is_aabb(const uint8_t* verbs, uint32_t verbCount, const float* pts, uint32_t ptCount, float* rect)
+{
+    return oracle::is_aabb(verbs, verbCount, reinterpret_cast<const oracle::V2*>(pts), ptCount, rect) ? 1 : 0;
+}
+
 oracle_flush* oracle_build(const rive_stream* s)
 {
     auto* of = new oracle_flush;

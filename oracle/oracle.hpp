@@ -37,6 +37,9 @@ struct Mat
     float m[6] = {1, 0, 0, 1, 0, 0};
 };
 
+// RiveRenderer::IsAABB restated (render.cpp).
+bool is_aabb(const uint8_t* verbs, uint32_t verbCount, const struct V2* pts, uint32_t ptCount, float rect[4]);
+
 // Verb values of rive::PathVerb (include/rive/math/path_types.hpp:29-38).
 enum Verb : uint8_t
 {

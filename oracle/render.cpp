@@ -127,6 +127,8 @@ IAABB map_bounds_round_out(const Mat& m, const V2* pts, uint32_t n, float outset
     return {(int)floorf(l), (int)floorf(t), (int)ceilf(r), (int)ceilf(b)};
 }
 
+} // namespace
+
 // RiveRenderer::IsAABB (rive_renderer.cpp:16-55): a move and three lines, any
 // further points equal to the first, opposite corners sharing x and y.
 bool is_aabb(const uint8_t* verbs, uint32_t verbCount, const V2* pts, uint32_t ptCount, float rect[4])
@@ -149,6 +151,8 @@ bool is_aabb(const uint8_t* verbs, uint32_t verbCount, const V2* pts, uint32_t p
     return true;
 }
 
+namespace
+{
 // transform_rect_to_new_space (rive_renderer.cpp:235-265).
 bool transform_rect_to_new_space(float rect[4], const Mat& current, const Mat& newMatrix)
 {

@@ -38,6 +38,11 @@ public:
     Mat2D operator*(const Mat2D& b) const;
     bool invert(Mat2D* out) const;
     float findMaxScale() const; // src/math/mat2d_find_max_scale.cpp:24-67
+    static Mat2D fromRotation(float radians)
+    {
+        const float s = sinf(radians), c = cosf(radians);
+        return Mat2D(c, s, -s, c, 0, 0);
+    }
     bool operator==(const Mat2D& o) const
     {
         for (int i = 0; i < 6; ++i)
@@ -250,6 +255,10 @@ public:
     virtual void drawPath(RenderPath* path, RenderPaint* paint) = 0;
     virtual void clipPath(RenderPath* path) = 0;
     virtual void drawImage(const RenderImage* image, ImageSampler sampler, BlendMode blendMode, float opacity) = 0;
+    // renderer.cpp: conveniences over transform().
+    void translate(float x, float y) { transform(Mat2D(1, 0, 0, 1, x, y)); }
+    void scale(float sx, float sy) { transform(Mat2D(sx, 0, 0, sy, 0, 0)); }
+    void rotate(float radians) { transform(Mat2D::fromRotation(radians)); }
 };
 
 namespace gpu
@@ -349,6 +358,24 @@ public:
     void clipPath(RenderPath* path) override;
     void drawImage(const RenderImage* image, ImageSampler sampler, BlendMode blendMode, float opacity) override;
 
+    // The current clip rect (rive_renderer.hpp: hasClipRect, getClipRect,
+    // getClipRectMatrix): left, top, right, bottom in the space of its matrix.
+    struct AABB
+    {
+        float left, top, right, bottom;
+    };
+    // rive_renderer.cpp:16-55: is the path an axis-aligned rectangle (a move
+    // and three lines, any further points equal to the first)?  rect gets
+    // left, top, right, bottom.
+    static bool IsAABB(const RawPath& raw, float rect[4]);
+    bool hasClipRect() const { return m_stack.back().hasClipRect; }
+    AABB getClipRect() const
+    {
+        const float* r = m_stack.back().clipRect;
+        return {r[0], r[1], r[2], r[3]};
+    }
+    const Mat2D& getClipRectMatrix() const { return m_stack.back().clipRectMatrix; }
+
 private:
     RiveRenderPath m_unitRectPath;
     struct IAABB
@@ -377,7 +404,6 @@ private:
         Mat2D clipRectMatrix;
         Mat2D clipRectInverse;
     };
-    static bool IsAABB(const RawPath& raw, float rect[4]);
     void clipRectImpl(float rect[4], RiveRenderPath* path);
     void clipPathImpl(RiveRenderPath* path);
     static IAABB MapBounds(const Mat2D& m, const RawPath& raw, float outsetX, float outsetY);

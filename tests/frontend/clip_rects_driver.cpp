@@ -1,0 +1,347 @@
+// Front-end check of IsAABB, clip rects and clip IDs (no device work: the
+// flush goes to a recording backend).  Replays the call sequences of the
+// reference's "IsAABB" and "clip-rects" renderer tests
+// (tests/unit_tests/renderer/pls_renderer_test.cpp:553-738, 825-932) and
+// checks the same expectations.  Prints "OK" or the failures.
+#include "rive_renderer.hpp"
+
+#include <cmath>
+#include <cstdio>
+
+using namespace rive;
+
+namespace
+{
+int g_failures = 0;
+
+void expect(bool ok, const char* what, int line)
+{
+    if (!ok)
+    {
+        std::printf("FAIL line %d: %s\n", line, what);
+        ++g_failures;
+    }
+}
+#define EXPECT(c) expect((c), #c, __LINE__)
+
+bool near_rect(const RiveRenderer::AABB& r, float l, float t, float rr, float b)
+{
+    auto near = [](float a, float x) { return std::fabs(a - x) <= 1e-3f; };
+    return near(r.left, l) && near(r.top, t) && near(r.right, rr) && near(r.bottom, b);
+}
+#define EXPECT_RECT(r, l, t, rr, b) expect(near_rect((r), (l), (t), (rr), (b)), #r " ~ " #l "," #t "," #rr "," #b, __LINE__)
+
+class RecordingImpl : public gpu::RenderContextImpl
+{
+public:
+    bool flush(const gpu::FlushDescriptor&) override { return true; }
+};
+
+void rect_path(RiveRenderPath& p, float l, float t, float r, float b)
+{
+    p.moveTo(l, t);
+    p.lineTo(r, t);
+    p.lineTo(r, b);
+    p.lineTo(l, b);
+    p.close();
+}
+
+void oval_path(RiveRenderPath& p, float l, float t, float r, float b)
+{
+    // Four cubics through the axis extremes (circle constant 0.5523).
+    const float cx = (l + r) * .5f, cy = (t + b) * .5f, rx = (r - l) * .5f, ry = (b - t) * .5f;
+    const float k = 0.55228475f;
+    p.moveTo(cx + rx, cy);
+    p.cubicTo(cx + rx, cy + k * ry, cx + k * rx, cy + ry, cx, cy + ry);
+    p.cubicTo(cx - k * rx, cy + ry, cx - rx, cy + k * ry, cx - rx, cy);
+    p.cubicTo(cx - rx, cy - k * ry, cx - k * rx, cy - ry, cx, cy - ry);
+    p.cubicTo(cx + k * rx, cy - ry, cx + rx, cy - k * ry, cx + rx, cy);
+    p.close();
+}
+} // namespace
+
+bool aabb_is(const RawPath& p, float l, float t, float r, float b)
+{
+    float rect[4];
+    return RiveRenderer::IsAABB(p, rect) && rect[0] == l && rect[1] == t && rect[2] == r && rect[3] == b;
+}
+
+bool not_aabb(const RawPath& p)
+{
+    float rect[4];
+    return !RiveRenderer::IsAABB(p, rect);
+}
+
+void add_rect(RawPath& p, float l, float t, float r, float b, bool clockwise)
+{
+    // RawPath::addRect (src/math/raw_path.cpp:133-156)
+    p.move({l, t});
+    if (clockwise)
+    {
+        p.line({r, t});
+        p.line({r, b});
+        p.line({l, b});
+    }
+    else
+    {
+        p.line({l, b});
+        p.line({r, b});
+        p.line({r, t});
+    }
+    p.close();
+}
+
+// The reference's "IsAABB" renderer test (pls_renderer_test.cpp:825-932).
+void is_aabb_cases()
+{
+    for (bool doClose : {true, false})
+    {
+        RawPath a;
+        if (doClose)
+            add_rect(a, 0, 0, 10, 10, false);
+        else
+        {
+            a.line({10, 0});
+            a.line({10, 10});
+            a.line({0, 10});
+        }
+        EXPECT(aabb_is(a, 0, 0, 10, 10));
+        RawPath b;
+        if (doClose)
+            add_rect(b, 1, 2, -1, 5, true);
+        else
+        {
+            b.move({1, 2});
+            b.line({-1, 2});
+            b.line({-1, 5});
+            b.line({1, 5});
+        }
+        EXPECT(aabb_is(b, -1, 2, 1, 5));
+        RawPath c;
+        if (doClose)
+        {
+            add_rect(c, 0, 0, 10, 10, false);
+            add_rect(c, 1, 2, -1, 5, false);
+        }
+        else
+        {
+            c.line({10, 0});
+            c.line({10, 10});
+            c.line({0, 10});
+            c.line({0, 0});
+            c.line({10, 0});
+        }
+        EXPECT(not_aabb(c));
+    }
+    RawPath p;
+    p.move({1, 0});
+    EXPECT(not_aabb(p));
+    p.line({0, 0});
+    EXPECT(not_aabb(p));
+    p.line({0, 1});
+    EXPECT(not_aabb(p));
+    p.line({1, 1});
+    EXPECT(aabb_is(p, 0, 0, 1, 1));
+    p.close();
+    EXPECT(aabb_is(p, 0, 0, 1, 1));
+    RawPath q;
+    q.move({0, 0});
+    q.line({1, 0});
+    q.line({1, 1});
+    q.line({0, 1.1f});
+    q.close();
+    EXPECT(not_aabb(q));
+    RawPath u;
+    u.move({0, 0});
+    u.line({1, 0});
+    u.line({1, 1});
+    u.line({0, 1});
+    EXPECT(aabb_is(u, 0, 0, 1, 1));
+    u.close();
+    EXPECT(aabb_is(u, 0, 0, 1, 1));
+    RawPath v;
+    v.move({1, 1});
+    v.line({0, 1});
+    v.line({0, 0});
+    v.line({1, 0});
+    EXPECT(aabb_is(v, 0, 0, 1, 1));
+    v.line({1, 1});
+    EXPECT(aabb_is(v, 0, 0, 1, 1));
+    v.close();
+    EXPECT(aabb_is(v, 0, 0, 1, 1));
+    v.close();
+    EXPECT(aabb_is(v, 0, 0, 1, 1));
+    v.cubic({1, 1}, {1, 1}, {1, 1});
+    EXPECT(aabb_is(v, 0, 0, 1, 1));
+    v.quad({1, 1}, {1, 1});
+    EXPECT(aabb_is(v, 0, 0, 1, 1));
+}
+
+int main()
+{
+    is_aabb_cases();
+
+    gpu::RenderContext ctx(std::make_unique<RecordingImpl>());
+    gpu::RenderContext::FrameDescriptor fd;
+    fd.renderTargetWidth = 100;
+    fd.renderTargetHeight = 100;
+    ctx.beginFrame(fd);
+
+    RiveRenderPath rectA, rectB, rectC, ovalA, ovalB, ovalC, drawable;
+    rect_path(rectA, 0, 0, 100, 500);
+    rect_path(rectB, 1, -1, 20, 40);
+    rect_path(rectC, 0, 0, 300, 600);
+    oval_path(ovalA, 0, 0, 100, 400);
+    oval_path(ovalB, 0, 0, 200, 500);
+    oval_path(ovalC, 0, 0, 300, 600);
+    oval_path(drawable, -100, -100, 700, 800);
+    RiveRenderPaint paint;
+    paint.color(0xffffffff);
+
+    RiveRenderer r(&ctx);
+    // A lone rect clips with a clip rect: no clip content.
+    r.save();
+    r.clipPath(&rectA);
+    EXPECT(ctx.getClipContentID() == 0);
+    r.drawPath(&drawable, &paint);
+    EXPECT(ctx.getClipContentID() == 0);
+    r.restore();
+
+    r.save();
+    r.clipPath(&ovalA);
+    r.drawPath(&drawable, &paint);
+    uint32_t clipID = ctx.getClipContentID();
+    EXPECT(clipID != 0);
+    EXPECT(!r.hasClipRect());
+    r.clipPath(&rectA); // nested: still a clip rect
+    r.drawPath(&drawable, &paint);
+    EXPECT(ctx.getClipContentID() == clipID);
+    EXPECT(r.hasClipRect());
+    EXPECT_RECT(r.getClipRect(), 0, 0, 100, 500);
+    EXPECT(r.getClipRectMatrix() == Mat2D());
+
+    // Nested rects under one matrix intersect into one clip rect.
+    r.clipPath(&ovalB);
+    r.drawPath(&drawable, &paint);
+    EXPECT(ctx.getClipContentID() != clipID);
+    clipID = ctx.getClipContentID();
+    r.clipPath(&rectB);
+    r.drawPath(&drawable, &paint);
+    EXPECT(ctx.getClipContentID() == clipID);
+    EXPECT(r.hasClipRect());
+    EXPECT_RECT(r.getClipRect(), 1, 0, 20, 40);
+    EXPECT(r.getClipRectMatrix() == Mat2D());
+
+    // An incompatible (rotated) matrix turns the rect into a clip path.
+    r.save();
+    r.rotate(1);
+    r.clipPath(&ovalA);
+    r.drawPath(&drawable, &paint);
+    EXPECT(ctx.getClipContentID() != clipID);
+    clipID = ctx.getClipContentID();
+    r.clipPath(&rectB);
+    r.drawPath(&drawable, &paint);
+    EXPECT(ctx.getClipContentID() != clipID);
+    clipID = ctx.getClipContentID();
+    r.restore();
+    EXPECT(r.hasClipRect());
+    EXPECT_RECT(r.getClipRect(), 1, 0, 20, 40);
+    EXPECT(r.getClipRectMatrix() == Mat2D());
+
+    // A compatible (scale + translate) matrix still intersects.
+    r.save();
+    r.scale(1, 2);
+    r.translate(3, 4);
+    r.clipPath(&ovalC);
+    r.drawPath(&drawable, &paint);
+    EXPECT(ctx.getClipContentID() != clipID);
+    clipID = ctx.getClipContentID();
+    r.clipPath(&rectC);
+    r.drawPath(&drawable, &paint);
+    EXPECT(ctx.getClipContentID() == clipID);
+    EXPECT(r.hasClipRect());
+    EXPECT_RECT(r.getClipRect(), 3, 8, 20, 40);
+    EXPECT(r.getClipRectMatrix() == Mat2D());
+    r.restore();
+    r.restore();
+
+    // A new frame (the reference flushes here; the clip buffer starts empty).
+    ctx.flush(nullptr);
+    ctx.beginFrame(fd);
+    RiveRenderer r2(&ctx);
+    r2.save();
+    const Mat2D m = Mat2D::fromRotation(1) * Mat2D(.3f, .3f, .9f, -.7f, .4f, .2f);
+    r2.transform(m);
+    r2.clipPath(&rectA);
+    r2.drawPath(&drawable, &paint);
+    EXPECT(ctx.getClipContentID() == 0);
+    EXPECT(r2.hasClipRect());
+    EXPECT_RECT(r2.getClipRect(), 0, 0, 100, 500);
+    EXPECT(r2.getClipRectMatrix() == m);
+    r2.clipPath(&rectB);
+    r2.drawPath(&drawable, &paint);
+    EXPECT(ctx.getClipContentID() == 0);
+    r2.translate(.9f, -.1f);
+    r2.clipPath(&rectB);
+    r2.drawPath(&drawable, &paint);
+    EXPECT(ctx.getClipContentID() == 0);
+    EXPECT(r2.hasClipRect());
+    EXPECT_RECT(r2.getClipRect(), 1.9f, 0, 20, 39.9f);
+    EXPECT(r2.getClipRectMatrix() == m);
+    r2.scale(10, 2);
+    r2.clipPath(&rectB);
+    r2.drawPath(&drawable, &paint);
+    EXPECT(ctx.getClipContentID() == 0);
+    // Quarter turns keep the rect axis aligned.
+    const float pi = 3.14159265358979f;
+    r2.rotate(pi / 2);
+    r2.clipPath(&rectB);
+    r2.drawPath(&drawable, &paint);
+    EXPECT(r2.hasClipRect());
+    EXPECT_RECT(r2.getClipRect(), 10.9f, 1.9f, 10.9f, 39.9f);
+    EXPECT(r2.getClipRectMatrix() == m);
+    EXPECT(ctx.getClipContentID() == 0);
+    r2.rotate(pi);
+    r2.translate(-10, 0);
+    r2.clipPath(&rectB);
+    r2.drawPath(&drawable, &paint);
+    EXPECT(r2.hasClipRect());
+    EXPECT_RECT(r2.getClipRect(), 10.9f, 1.9f, 10.9f, 17.9f);
+    EXPECT(r2.getClipRectMatrix() == m);
+    EXPECT(ctx.getClipContentID() == 0);
+    r2.rotate(-pi / 2);
+    r2.clipPath(&rectB);
+    r2.drawPath(&drawable, &paint);
+    EXPECT(ctx.getClipContentID() == 0);
+    EXPECT(r2.hasClipRect());
+    EXPECT_RECT(r2.getClipRect(), 10.9f, 1.9f, -9.1f, 17.9f);
+    EXPECT(r2.getClipRectMatrix() == m);
+    // Flips too.
+    r2.scale(1, -1);
+    r2.clipPath(&rectB);
+    r2.drawPath(&drawable, &paint);
+    EXPECT(ctx.getClipContentID() == 0);
+    EXPECT(r2.hasClipRect());
+    EXPECT_RECT(r2.getClipRect(), 10.9f, 17.9f, -9.1f, 17.9f);
+    EXPECT(r2.getClipRectMatrix() == m);
+    r2.scale(-1, -1);
+    r2.translate(2, 0);
+    r2.clipPath(&rectB);
+    r2.drawPath(&drawable, &paint);
+    EXPECT(ctx.getClipContentID() == 0);
+    EXPECT(r2.hasClipRect());
+    EXPECT_RECT(r2.getClipRect(), 30.9f, 17.9f, -9.1f, 17.9f);
+    EXPECT(r2.getClipRectMatrix() == m);
+    // A skew makes them incompatible: a clip path, and the rect stays.
+    r2.transform(Mat2D(1, .1f, 0, 1, 0, 0));
+    r2.clipPath(&rectB);
+    r2.drawPath(&drawable, &paint);
+    EXPECT(ctx.getClipContentID() != 0);
+    EXPECT_RECT(r2.getClipRect(), 30.9f, 17.9f, -9.1f, 17.9f);
+    EXPECT(r2.getClipRectMatrix() == m);
+    r2.restore();
+
+    if (g_failures == 0)
+        std::printf("OK\n");
+    return g_failures == 0 ? 0 : 1;
+}
