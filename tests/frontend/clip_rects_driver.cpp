@@ -1,12 +1,16 @@
-// Front-end check of IsAABB, clip rects and clip IDs (no device work: the
-// flush goes to a recording backend).  Replays the call sequences of the
-// reference's "IsAABB" and "clip-rects" renderer tests
-// (tests/unit_tests/renderer/pls_renderer_test.cpp:553-738, 825-932) and
+// Front-end checks (no device work: the flush goes to a recording backend).
+// Replays the call sequences of the reference's RiveRenderer tests
+// (tests/unit_tests/renderer/pls_renderer_test.cpp): IsAABB (:825-932),
+// clip-rects (:553-738), nan-render-path, stroke-thickness, empty-paths,
+// empty-paths2 and invalid-gradient-stops (:740-823, :934-1045, :1113-1174), and
 // checks the same expectations.  Prints "OK" or the failures.
 #include "rive_renderer.hpp"
 
 #include <cmath>
 #include <cstdio>
+#include <cstring>
+#include <limits>
+#include <vector>
 
 using namespace rive;
 
@@ -177,9 +181,125 @@ void is_aabb_cases()
     EXPECT(aabb_is(v, 0, 0, 1, 1));
 }
 
+// "nan-render-path", "stroke-thickness", "empty-paths", "empty-paths2" and
+// "invalid-gradient-stops" (pls_renderer_test.cpp:740-823, 934-1045,
+// 1113-1174): NaN and infinite geometry, degenerate strokes and move-only paths
+// must not break the front end; a NaN clip empties the clip; invalid gradient
+// stops make no gradient.
+void robustness_cases()
+{
+    const float nan = std::numeric_limits<float>::quiet_NaN(), inf = std::numeric_limits<float>::infinity();
+    gpu::RenderContext ctx(std::make_unique<RecordingImpl>());
+    gpu::RenderContext::FrameDescriptor fd;
+    fd.renderTargetWidth = 100;
+    fd.renderTargetHeight = 100;
+    ctx.beginFrame(fd);
+    RiveRenderer r(&ctx);
+    RiveRenderPaint paint;
+    paint.color(0xffffffff);
+
+    RiveRenderPath finite, nanPath;
+    finite.moveTo(1, 2);
+    finite.lineTo(3, 4);
+    finite.cubicTo(5, 6, 7, 8, 9, 10);
+    finite.cubicTo(11, 12, 13, 14, 15, 16);
+    finite.lineTo(17, 18);
+    nanPath.moveTo(nan, nan);
+    nanPath.lineTo(nan, nan);
+    nanPath.cubicTo(nan, nan, nan, nan, nan, nan);
+    nanPath.cubicTo(nan, nan, nan, nan, nan, nan);
+    nanPath.lineTo(nan, nan);
+
+    r.drawPath(&nanPath, &paint);
+    r.save();
+    r.clipPath(&nanPath);
+    r.drawPath(&finite, &paint);
+    r.restore();
+    r.save();
+    r.transform(Mat2D(nan, nan, nan, nan, nan, nan));
+    r.drawPath(&finite, &paint);
+    r.clipPath(&finite);
+    r.drawPath(&nanPath, &paint);
+    r.restore();
+    EXPECT(ctx.getClipContentID() == 0);
+    r.save();
+    r.clipPath(&nanPath); // a NaN clip empties the clip: nothing is drawn
+    r.drawPath(&finite, &paint);
+    EXPECT(ctx.getClipContentID() == 0);
+    r.clipPath(&finite);
+    r.drawPath(&finite, &paint);
+    EXPECT(ctx.getClipContentID() == 0);
+    r.restore();
+    r.save();
+    r.clipPath(&finite);
+    r.drawPath(&finite, &paint);
+    EXPECT(ctx.getClipContentID() != 0);
+    r.restore();
+    RiveRenderPath moveNaN;
+    moveNaN.moveTo(nan, nan);
+    moveNaN.close();
+    r.drawPath(&moveNaN, &paint);
+
+    RiveRenderPath zig;
+    zig.lineTo(100, 100);
+    zig.lineTo(100, 0);
+    RiveRenderPaint stroke;
+    stroke.style(RenderPaintStyle::stroke);
+    uint32_t tiny = 1u;
+    float tinyF;
+    std::memcpy(&tinyF, &tiny, 4);
+    for (float t : {tinyF, nan, inf, -inf})
+    {
+        stroke.thickness(t);
+        r.drawPath(&zig, &stroke);
+    }
+
+    RiveRenderPath empty, moves, big, moves2;
+    r.drawPath(&empty, &paint);
+    moves.moveTo(1, 1);
+    moves.moveTo(2, 2);
+    moves.lineTo(2, 2);
+    r.drawPath(&moves, &paint);
+    unsigned seed = 1;
+    auto rand100 = [&seed]() {
+        seed = seed * 1103515245u + 12345u;
+        return (float)((seed >> 8) & 0xffffu) * (100.f / 65535.f);
+    };
+    for (int i = 0; i < 100000; ++i)
+        big.cubicTo(rand100(), rand100(), rand100(), rand100(), rand100(), rand100());
+    r.drawPath(&big, &paint);
+    for (float k : {0.f, 1000.f, 3000.f, 5000.f, 7000.f})
+        moves2.moveTo(k, k + 1000.f);
+    r.drawPath(&moves2, &paint);
+    EXPECT(ctx.flush(nullptr));
+
+    auto grad = [](std::vector<float> stops) {
+        std::vector<ColorInt> colors(std::max<size_t>(stops.size(), 1), 0xff00ff00u);
+        const bool lin = (bool)Gradient::MakeLinear(0, 0, 0, 0, colors.data(), stops.data(), stops.size());
+        const bool rad = (bool)Gradient::MakeRadial(0, 0, 0, colors.data(), stops.data(), stops.size());
+        return lin == rad ? (int)lin : -1;
+    };
+    EXPECT(grad({0, .5f, 1}) == 1);
+    EXPECT(grad({0}) == 1);
+    EXPECT(grad({1}) == 1);
+    EXPECT(grad({}) == 0);
+    EXPECT(grad({0, .5f, 1.1f}) == 0);
+    EXPECT(grad({0, .5f, inf}) == 0);
+    EXPECT(grad({0, .5f, nan}) == 0);
+    EXPECT(grad({-.1f, .5f, 1}) == 0);
+    EXPECT(grad({-inf, .5f, 1}) == 0);
+    EXPECT(grad({nan, .5f, 1}) == 0);
+    EXPECT(grad({0, .4f, .5f, 1}) == 1);
+    EXPECT(grad({0, .5f, .4f, 1}) == 0);
+    EXPECT(grad({.4f, 0, .5f, 1}) == 0);
+    EXPECT(grad({0, .4f, 1, .5f}) == 0);
+    EXPECT(grad({0, nan, 1}) == 0);
+}
+
 int main()
 {
     is_aabb_cases();
+    robustness_cases();
 
     gpu::RenderContext ctx(std::make_unique<RecordingImpl>());
     gpu::RenderContext::FrameDescriptor fd;
