@@ -89,7 +89,7 @@ struct rive_hip_context
     MappedBuffer bufs[RIVE_HIP_BUFFER_KIND_COUNT];
     // pipeline scratch (grow-only)
     DevBuf spanTexelCounts, spanTexelOffsets, scanPartials, scalars;
-    DevBuf tess, prims, cells, cellCount, cellOffset, cellCursor, cellBackdrop, binned;
+    DevBuf tess, prims, cells, cellCount, cellOffset, cellCursor, cellBackdrop, binned, cellPacked;
     DevBuf pathRaster, ramps, drawTess, verts;
     DevBuf geoCounts, geoRecords, geoContours, geoSpans, geoSpanCounts, geoSpanOffsets;
     uint32_t lastMode = 0, lastRecordCount = 0, lastTessVertexCount = 0;
@@ -158,7 +158,8 @@ void rive_hip_destroy(rive_hip_context* ctx)
                      &ctx->cellCursor, &ctx->cellBackdrop, &ctx->binned, &ctx->pathRaster,
                      &ctx->ramps, &ctx->drawTess, &ctx->geoCounts, &ctx->geoRecords,
                      &ctx->geoContours, &ctx->geoSpans, &ctx->geoSpanCounts, &ctx->geoSpanOffsets,
-                     &ctx->verts, &ctx->imageTable, &ctx->pathImage};
+                     &ctx->verts, &ctx->imageTable, &ctx->pathImage, &ctx->canvasTable,
+                     &ctx->pathCanvas, &ctx->cellPacked};
     for (DevBuf* b : all)
         if (b->ptr)
             (void)hipFree(b->ptr);
@@ -532,6 +533,9 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
         bin_scatter_draw_kernel<<<drawCount, 256, 0, ctx->stream>>>(
             P<Prim>(ctx->prims), drawTess, P<PathCells>(ctx->cells), P<uint32_t>(ctx->cellOffset),
             P<uint32_t>(ctx->cellCursor), P<uint32_t>(ctx->binned));
+    HIP_TRY(ensure(ctx->cellPacked, (size_t)(totalCells + 1) * sizeof(uint2)));
+    cell_pack_kernel<<<(totalCells + 1 + 255) / 256, 256, 0, ctx->stream>>>(
+        P<uint32_t>(ctx->cellOffset), P<int32_t>(ctx->cellBackdrop), totalCells, P<uint2>(ctx->cellPacked));
     HIP_TRY(ensure(ctx->pathRaster, (size_t)(pathCount + 1) * sizeof(PathRaster)));
     path_raster_kernel<<<(pathCount + 1 + 255) / 256, 256, 0, ctx->stream>>>(
         paths, paints, aux, P<uint32_t>(ctx->pathImage), pathCount, gradRows, P<PathRaster>(ctx->pathRaster));
@@ -541,8 +545,8 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     // ---- raster + resolve
     if (tileTotal > 0)
         raster_kernel<<<tileTotal, kWgThreads, 0, ctx->stream>>>(
-            P<PathCells>(ctx->cells), P<PathRaster>(ctx->pathRaster), P<uint32_t>(ctx->cellOffset),
-            P<int32_t>(ctx->cellBackdrop), P<uint32_t>(ctx->binned), P<Prim>(ctx->prims),
+            P<PathCells>(ctx->cells), P<PathRaster>(ctx->pathRaster), P<uint2>(ctx->cellPacked),
+            P<uint32_t>(ctx->binned), P<Prim>(ctx->prims),
             P<uint32_t>(ctx->ramps), gradRows, P<ImageDesc>(ctx->imageTable), P<CanvasDev>(ctx->canvasTable),
             canvasCount, d->dither);
     HIP_TRY(hipGetLastError());

@@ -978,6 +978,18 @@ __global__ __launch_bounds__(256) void bin_scatter_draw_kernel(const Prim* __res
     }
 }
 
+// Each cell's first bin and corner winding side by side, so the raster reads
+// one 8-byte entry (and the next entry's offset for the count) per
+// (path, tile) instead of two arrays.
+__global__ void cell_pack_kernel(const uint32_t* __restrict__ cellOffset, const int32_t* __restrict__ cellBackdrop,
+                                 uint32_t totalCells, uint2* __restrict__ packed)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > totalCells)
+        return;
+    packed[i] = make_uint2(cellOffset[i], i < totalCells ? (uint32_t)cellBackdrop[i] : 0u);
+}
+
 // Prefix-sum of corner crossings along each row of each path's grid: one
 // wave64 per (path, row), 64 cells per step with a carried running sum.
 __global__ __launch_bounds__(256) void backdrop_prefix_kernel(const PathCells* __restrict__ cells,
@@ -1871,8 +1883,7 @@ __device__ __forceinline__ uint32_t slot_arrays(uint32_t n)
 
 __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
     const PathCells* __restrict__ cells, const PathRaster* __restrict__ raster,
-    const uint32_t* __restrict__ cellOffset, const int32_t* __restrict__ cellBackdrop,
-    const uint32_t* __restrict__ binned, const Prim* __restrict__ prims,
+    const uint2* __restrict__ cellPacked, const uint32_t* __restrict__ binned, const Prim* __restrict__ prims,
     const uint32_t* __restrict__ ramps, uint32_t gradHeight, const ImageDesc* __restrict__ images,
     const CanvasDev* __restrict__ canvases, uint32_t canvasCount, uint32_t dither)
 {
@@ -1939,9 +1950,10 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
             if (tx >= c.tx0 && tx < c.tx1 && ty >= c.ty0 && ty < c.ty1)
             {
                 const uint32_t cell = c.base + (uint32_t)(ty - c.ty0) * c.width + (uint32_t)(tx - c.tx0);
-                off = cellOffset[cell];
-                n = cellOffset[cell + 1] - off;
-                C = cellBackdrop[cell];
+                const uint2 here = cellPacked[cell];
+                off = here.x;
+                n = cellPacked[cell + 1].x - off;
+                C = (int32_t)here.y;
                 listed = n != 0 || C != 0;
             }
         }
