@@ -1245,15 +1245,14 @@ __device__ void fill_prim_left(const Prim& p, const TileGeom& g, int32_t* __rest
         // R(q) <=> s*D(q) >= 0 <=> s*k >= ceil(-s*D0/128)
         const int32_t T = clamp_thr(ceil_div128(s > 0 ? -e.D0 : e.D0));
         const bool Rcorner = (s > 0 ? e.D0 : -e.D0) >= 0;
+        // the first row whose sample-line point is on the other side of the
+        // edge than the corner (rows as a bit mask: no divergent exits)
+        uint32_t flipped = 0;
+#pragma unroll
         for (int j = 0; j < kTile; ++j)
-        {
-            const bool Rrow = s * (e.B * (2 * j + 1)) >= T;
-            if (Rrow != Rcorner)
-            {
-                atomicAdd(&sLeft[j], neg * s * ((int)Rrow - (int)Rcorner));
-                break;
-            }
-        }
+            flipped |= (uint32_t)((s * (e.B * (2 * j + 1)) >= T) != Rcorner) << j;
+        if (flipped != 0)
+            atomicAdd(&sLeft[__builtin_ctz(flipped)], neg * s * (Rcorner ? -1 : 1));
     }
     else if (p.p[0] != p.p[4])
     {
@@ -2044,15 +2043,16 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                     uint32_t rowsF = 0, rowsS = 0, info = 0, primIdx = 0;
                     if (f < nPrim)
                     {
-                        // owning entry: the last one whose first primitive is at or before f
-                        uint32_t lo = s, hi = last;
-                        while (lo < hi)
+                        // owning entry: the last one whose first primitive is at or
+                        // before f (a batch has at most 64 entries: six fixed steps,
+                        // selects rather than branches)
+                        uint32_t lo = s;
+#pragma unroll
+                        for (uint32_t step = kBatchEntries / 2; step > 0; step >>= 1)
                         {
-                            const uint32_t mid = (lo + hi + 1) >> 1;
-                            if (sRelPrims[mid] - baseP <= f)
-                                lo = mid;
-                            else
-                                hi = mid - 1;
+                            const uint32_t cand = lo + step;
+                            const uint32_t probe = sRelPrims[min(cand, last)] - baseP;
+                            lo = (cand <= last && probe <= f) ? cand : lo;
                         }
                         const uint32_t a = (sRelSA[lo] - baseSA) >> 16;
                         const bool wide = sRelN[lo] > kPackedPrims;
