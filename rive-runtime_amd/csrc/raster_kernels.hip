@@ -1081,8 +1081,17 @@ template <typename I> __device__ __forceinline__ int32_t clamp_thr_t(I t)
     const I lim = (I)1 << 30;
     return (int32_t)(t < -lim ? -lim : (t > lim ? lim : t));
 }
+template <> __device__ __forceinline__ int32_t clamp_thr_t<double>(double t)
+{
+    return (int32_t)fmin(fmax(t, -1073741824.0), 1073741824.0);
+}
 template <typename I> __device__ __forceinline__ I ceil_div128_t(I a) { return -((-a) >> 7); }
 template <typename I> __device__ __forceinline__ I floor_div128_t(I a) { return a >> 7; }
+// Edge functions in double: coordinates lie in [-2^23, 2^23) (1/256 px), so
+// every product is below 2^49 and every sum below 2^50 -- exact, as are the
+// divisions by 128 and their floor and ceiling.
+template <> __device__ __forceinline__ double ceil_div128_t<double>(double a) { return ceil(a * (1.0 / 128.0)); }
+template <> __device__ __forceinline__ double floor_div128_t<double>(double a) { return floor(a * (1.0 / 128.0)); }
 template <typename I> __device__ __forceinline__ int32_t inside_threshold_t(const EdgeTI<I>& e, int o)
 {
     const int pert = e.A != 0 ? (e.A > 0 ? 1 : -1) : (e.B > 0 ? 1 : (e.B < 0 ? -1 : 0));
@@ -2111,11 +2120,11 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                         const int row = (int)((inf >> 8) & 15u) + (int)(it - (inf >> 16));
                         const Prim pm = prims[sItemPrim[owner]];
                         if (inf & 32u)
-                            stroke_prim_row<int64_t>(pm, row, g, reinterpret_cast<uint32_t*>(sAcc[a]));
+                            stroke_prim_row<double>(pm, row, g, reinterpret_cast<uint32_t*>(sAcc[a]));
                         else if (inf & 16u)
-                            fill_prim_row<int64_t>(pm, row, g, sAcc[a], sAcc[a + 1], 1);
+                            fill_prim_row<double>(pm, row, g, sAcc[a], sAcc[a + 1], 1);
                         else
-                            fill_prim_row<int64_t>(pm, row, g, sAcc[a], sAcc[a], 1 << 24);
+                            fill_prim_row<double>(pm, row, g, sAcc[a], sAcc[a], 1 << 24);
                     }
                     __syncthreads();
                 }
