@@ -1047,17 +1047,6 @@ __device__ __forceinline__ EdgeT make_edge(int32_t ax, int32_t ay, int32_t bx, i
     e.D0 = (int64_t)dx * (int64_t)(Y0 - ay) - (int64_t)dy * (int64_t)(X0 - ax);
     return e;
 }
-// Threshold T such that "o * D(q) has positive perturbed sign" <=> o*k >= T.
-__device__ __forceinline__ int32_t inside_threshold(const EdgeT& e, int o)
-{
-    // perturbation sign of D at q+(e,e^2): -dy if dy != 0 else dx
-    int pert = e.A != 0 ? (e.A > 0 ? 1 : -1) : (e.B > 0 ? 1 : (e.B < 0 ? -1 : 0));
-    int64_t oD0 = o > 0 ? e.D0 : -e.D0;
-    // o*D0 + 128*(o*k) >= 0 (non-strict) or > 0 (strict)
-    int64_t t = (o * pert > 0) ? ceil_div128(-oD0) : floor_div128(-oD0) + 1;
-    return clamp_thr(t);
-}
-
 // Width-generic forms of the edge helpers for the per-row work (instantiated
 // for int64).
 template <typename I> struct EdgeTI
