@@ -1215,15 +1215,76 @@ __device__ __forceinline__ bool edge_in(const EdgeK& k, int32_t i2, int32_t j2)
     return k.A * i2 + k.B * j2 >= k.thr;
 }
 
-// Sample rows [*r0, *r1] of the tile that a primitive's quad can touch.
-__device__ __forceinline__ void prim_rows(const Prim& p, int32_t Y0, int* r0, int* r1)
+// Sample rows [*r0, *r1] of the tile that a primitive can touch inside the
+// tile's columns.  The per-row work looks at sample points with x in
+// [X0, X0 + 16 px] only (inner-edge crossings left of X0 are the left-edge
+// winding's), so the rows are those where the primitive meets that slab: the
+// y-extent of its five edges (those of both triangles, the inner edge among
+// them) clipped to the slab widened by 64 subpixels, taken in float and
+// widened by the float error, and intersected with the quad's exact y-extent.
+// A part of a polygon inside a vertical slab always has boundary inside it, so
+// the clipped edges bound it.
+__device__ __forceinline__ void prim_rows(const Prim& p, const TileGeom& g, int* r0, int* r1)
 {
     const int32_t miny = min(min(p.p[1], p.p[3]), min(p.p[5], p.p[7]));
     const int32_t maxy = max(max(p.p[1], p.p[3]), max(p.p[5], p.p[7]));
-    const int32_t lo = miny - Y0 - (1 << (kSubpixelLog2 - 1));
-    const int32_t hi = maxy - Y0 - (1 << (kSubpixelLog2 - 1));
-    *r0 = max(-((-lo) >> kSubpixelLog2), 0);          // ceil(lo / 256)
-    *r1 = min(hi >> kSubpixelLog2, kTile - 1);        // floor(hi / 256)
+    const int32_t lo = miny - g.Y0 - (1 << (kSubpixelLog2 - 1));
+    const int32_t hi = maxy - g.Y0 - (1 << (kSubpixelLog2 - 1));
+    int a0 = max(-((-lo) >> kSubpixelLog2), 0);   // ceil(lo / 256)
+    int a1 = min(hi >> kSubpixelLog2, kTile - 1); // floor(hi / 256)
+
+    // tile-relative vertices: |coordinate| < 2^24, exact in float
+    float vx[4], vy[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        vx[k] = (float)(p.p[2 * k] - g.X0);
+        vy[k] = (float)(p.p[2 * k + 1] - g.Y0);
+    }
+    constexpr float xa = -64.f, xb = (float)(kTile << kSubpixelLog2) + 64.f;
+    constexpr int ea[5] = {0, 1, 2, 1, 3}, eb[5] = {1, 2, 0, 3, 2};
+    float ylo = INFINITY, yhi = -INFINITY, eps = 1.f;
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+    {
+        float x0 = vx[ea[k]], y0 = vy[ea[k]], x1 = vx[eb[k]], y1 = vy[eb[k]];
+        if (x0 > x1)
+        {
+            const float tx = x0, ty = y0;
+            x0 = x1;
+            y0 = y1;
+            x1 = tx;
+            y1 = ty;
+        }
+        if (x1 < xa || x0 > xb)
+            continue;
+        float ya = y0, yb = y1;
+        if (x0 < xa || x1 > xb)
+        {
+            // x1 > x0 here.  The slope is within 2^-22 relative, and each
+            // clipped y within |dy| 2^-20 + |y| 2^-22 of the true one.
+            const float dy = y1 - y0;
+            const float m = dy * __builtin_amdgcn_rcpf(x1 - x0);
+            if (x0 < xa)
+                ya = y0 + (xa - x0) * m;
+            if (x1 > xb)
+                yb = y0 + (xb - x0) * m;
+            eps = fmaxf(eps, fabsf(dy) * (1.f / (1 << 19)) + (fabsf(ya) + fabsf(yb)) * (1.f / (1 << 21)));
+        }
+        ylo = fminf(ylo, fminf(ya, yb));
+        yhi = fmaxf(yhi, fmaxf(ya, yb));
+    }
+    if (!(ylo <= yhi))
+    {
+        *r0 = 1;
+        *r1 = 0;
+        return;
+    }
+    const float half = (float)(1 << (kSubpixelLog2 - 1)), inv = 1.f / (float)(1 << kSubpixelLog2);
+    a0 = max(a0, (int)ceilf(fmaxf((ylo - eps - half) * inv, -1.f)));
+    a1 = min(a1, (int)floorf(fminf((yhi + eps - half) * inv, (float)kTile)));
+    *r0 = a0;
+    *r1 = a1;
 }
 
 // Left-edge winding of a fill primitive's inner edge I0 -> I1: the change of
@@ -1934,6 +1995,15 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
     uint32_t clip = 0u;
     const float noise255 = dither ? dither_noise(px, py) * 255.f : 0.f;
 
+    // clear accumulators for the first batch (the listing's barriers order
+    // this before their use)
+    static_assert(kAccArrays * kTile == kWgThreads, "one sLeft entry per thread");
+    for (int q = t; q < kAccArrays * kTile * kTile / 4; q += kWgThreads)
+        reinterpret_cast<int4*>(&sAcc[0][0])[q] = make_int4(0, 0, 0, 0);
+    (&sLeft[0][0])[t] = 0;
+    if (t < kAccArrays)
+        sArrEntry[t] = 0xffu;
+
     for (uint32_t base = pathBegin; base < pathEnd; base += kWindow)
     {
         // ---- list the window's paths that touch the tile
@@ -2024,13 +2094,8 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
 
             if (arraysUsed != 0)
             {
-                for (int q = t; q < (int)arraysUsed * (kTile * kTile / 4); q += kWgThreads)
-                    reinterpret_cast<int4*>(&sAcc[0][0])[q] = make_int4(0, 0, 0, 0);
-                if (t < (int)arraysUsed * kTile)
-                    (&sLeft[0][0])[t] = 0;
-                if (t < kAccArrays)
-                    sArrEntry[t] = 0xffu;
-                __syncthreads();
+                // (the arrays, sLeft and sArrEntry are clear: set at the start
+                // and cleared again by the previous batch's resolve)
                 if (t < (int)len && sRelN[s + t] != 0)
                     sArrEntry[(sRelSA[s + t] - baseSA) >> 16] = (uint8_t)t;
 
@@ -2058,7 +2123,7 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                         primIdx = binned[sRelOff[lo] + (f - (sRelPrims[lo] - baseP))];
                         const Prim pm = prims[primIdx];
                         int r0, r1;
-                        prim_rows(pm, g.Y0, &r0, &r1);
+                        prim_rows(pm, g, &r0, &r1);
                         const uint32_t rows = (uint32_t)max(r1 - r0 + 1, 0);
                         if (stroke)
                             rowsS = rows;
@@ -2165,7 +2230,12 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                 __syncthreads();
             }
 
-            // ---- resolve the batch in submission order
+            // ---- resolve the batch in submission order, clearing what the next
+            // batch accumulates into: each pixel's cell of every array read
+            // here (a two-array path's border array too), sLeft, sArrEntry
+            (&sLeft[0][0])[t] = 0;
+            if (t < kAccArrays)
+                sArrEntry[t] = 0xffu;
             for (uint32_t e = 0; e < len; ++e)
             {
                 const uint32_t i = s + e;
@@ -2174,7 +2244,13 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                 const PathRaster pr = raster[path];
                 float cov;
                 if (ni != 0)
-                    cov = __int_as_float(sAcc[uniform_u32((sRelSA[i] - baseSA) >> 16)][t]);
+                {
+                    const uint32_t a = uniform_u32((sRelSA[i] - baseSA) >> 16);
+                    cov = __int_as_float(sAcc[a][t]);
+                    sAcc[a][t] = 0;
+                    if (ni > kPackedPrims)
+                        sAcc[a + 1][t] = 0;
+                }
                 else
                     cov = (pr.flags & 1u) ? 0.f : fill_rule_coverage((float)sRelC[i], pr.flags);
                 if (cov != 0.f || (pr.flags & 8u))
