@@ -1125,13 +1125,13 @@ __device__ __forceinline__ float dither_noise(int x, int y)
 // +-127 and its border sum within +-2^23, so the packed form is exact.
 // Threads first take one primitive each (left-edge terms and the range of
 // sample rows), then the (primitive, row) work items are compacted across the
-// workgroup, fills before strokes.  A pass per (array, row) turns the
-// accumulators into coverage in place (prefix of the crossings, fill rule),
-// and every thread then resolves its pixel for each path of the batch.  The
-// integer predicates are those of a per-pixel evaluation: crossing columns
-// solve the same threshold inequality, and a triangle's pixels on a row are
-// the candidates of a float span estimate (widened by its error bound) that
-// pass the exact three-edge test.
+// workgroup, fills before strokes.  Every thread then resolves its pixel for
+// each path of the batch, turning the path's accumulators into coverage on
+// the way (prefix of the left-edge deltas and of the row's crossings, fill
+// rule).  The integer predicates are those of a per-pixel evaluation:
+// crossing columns solve the same threshold inequality, and a triangle's
+// pixels on a row are the candidates of a float span estimate (widened by its
+// error bound) that pass the exact three-edge test.
 constexpr int kWgThreads = 256;
 constexpr int kAccArrays = 16;
 constexpr int kBatchEntries = 64; // one wave's ballot
@@ -1931,6 +1931,18 @@ __device__ __forceinline__ uint32_t uniform_u32(uint32_t v)
 {
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
+__device__ __forceinline__ int32_t uniform_i32(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Inclusive prefix sum within each DPP row of 16 lanes (row_shr:1, 2, 4, 8;
+// lanes shifted in from outside the row read 0).  All 64 lanes must be active.
+__device__ __forceinline__ int32_t row_scan16(int32_t x)
+{
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);
+    return x;
+}
 
 // Accumulator arrays a path with n primitives in the tile needs, packed with
 // a slot count of one: slots | arrays << 16.
@@ -1956,7 +1968,6 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
     __shared__ uint32_t sRelPrims[kWindow];
     __shared__ uint32_t sRelSA[kWindow];
     __shared__ uint32_t sWave[3][kWgThreads / 64];
-    __shared__ uint8_t sArrEntry[kAccArrays]; // batch entry whose first array is a, else 0xff
     __shared__ uint8_t sOwner[kMaxItems];
     __shared__ uint32_t sItemPrim[kWgThreads];
     __shared__ uint32_t sItemInfo[kWgThreads]; // array | wide << 4 | stroke << 5 | row0 << 8 | item0 << 16
@@ -2001,8 +2012,6 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
     for (int q = t; q < kAccArrays * kTile * kTile / 4; q += kWgThreads)
         reinterpret_cast<int4*>(&sAcc[0][0])[q] = make_int4(0, 0, 0, 0);
     (&sLeft[0][0])[t] = 0;
-    if (t < kAccArrays)
-        sArrEntry[t] = 0xffu;
 
     for (uint32_t base = pathBegin; base < pathEnd; base += kWindow)
     {
@@ -2094,11 +2103,8 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
 
             if (arraysUsed != 0)
             {
-                // (the arrays, sLeft and sArrEntry are clear: set at the start
-                // and cleared again by the previous batch's resolve)
-                if (t < (int)len && sRelN[s + t] != 0)
-                    sArrEntry[(sRelSA[s + t] - baseSA) >> 16] = (uint8_t)t;
-
+                // (the arrays and sLeft are clear: set at the start and
+                // cleared again after the previous batch's resolve)
                 for (uint32_t c0 = 0; c0 < nPrim; c0 += kBatchPrims)
                 {
                     // ---- one primitive per thread
@@ -2182,60 +2188,15 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                     }
                     __syncthreads();
                 }
-
-                // ---- accumulators -> coverage, one thread per (array, row)
-                {
-                    const int a = t >> kTileLog2, row = t & (kTile - 1);
-                    const uint32_t e = sArrEntry[a];
-                    if (e != 0xffu)
-                    {
-                        const uint32_t i = s + e;
-                        const uint32_t flags = raster[sRelPath[i]].flags;
-                        int32_t* accRow = sAcc[a] + row * kTile;
-                        if (flags & 1u)
-                        {
-#pragma unroll
-                            for (int k = 0; k < kTile; ++k)
-                                accRow[k] = __float_as_int(fminf(__int_as_float(accRow[k]), 1.f));
-                        }
-                        else
-                        {
-                            const bool wide = sRelN[i] > kPackedPrims;
-                            int32_t w = sRelC[i];
-                            for (int r = 0; r <= row; ++r)
-                                w += sLeft[a][r];
-                            const int32_t* bRow = wide ? sAcc[a + 1] + row * kTile : accRow;
-#pragma unroll
-                            for (int k = 0; k < kTile; ++k)
-                            {
-                                const int32_t v = accRow[k];
-                                int32_t d, b;
-                                if (wide)
-                                {
-                                    d = v;
-                                    b = bRow[k];
-                                }
-                                else
-                                {
-                                    d = (v + (1 << 23)) >> 24;
-                                    b = v - (int32_t)((uint32_t)d << 24);
-                                }
-                                w += d;
-                                accRow[k] = __float_as_int(
-                                    fill_rule_coverage((float)w + (float)b * (1.f / 65536.f), flags));
-                            }
-                        }
-                    }
-                }
-                __syncthreads();
             }
 
-            // ---- resolve the batch in submission order, clearing what the next
-            // batch accumulates into: each pixel's cell of every array read
-            // here (a two-array path's border array too), sLeft, sArrEntry
-            (&sLeft[0][0])[t] = 0;
-            if (t < kAccArrays)
-                sArrEntry[t] = 0xffu;
+            // ---- resolve the batch in submission order.  Every thread takes
+            // the same entry, so a path's coverage comes from its accumulators
+            // here: the winding at a pixel is the corner winding, plus the
+            // left-edge deltas of the rows up to its row, plus the crossings
+            // of its row up to its column -- two 16-lane scans, a DPP row
+            // being a row of the tile.  Each pixel's cells are cleared as they
+            // are read (a two-array path's border array too).
             for (uint32_t e = 0; e < len; ++e)
             {
                 const uint32_t i = s + e;
@@ -2246,17 +2207,38 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                 if (ni != 0)
                 {
                     const uint32_t a = uniform_u32((sRelSA[i] - baseSA) >> 16);
-                    cov = __int_as_float(sAcc[a][t]);
+                    const int32_t v = sAcc[a][t];
                     sAcc[a][t] = 0;
-                    if (ni > kPackedPrims)
-                        sAcc[a + 1][t] = 0;
+                    if (pr.flags & 1u)
+                        cov = fminf(__int_as_float(v), 1.f);
+                    else
+                    {
+                        int32_t d, b;
+                        if (ni > kPackedPrims)
+                        {
+                            d = v;
+                            b = sAcc[a + 1][t];
+                            sAcc[a + 1][t] = 0;
+                        }
+                        else
+                        {
+                            d = (v + (1 << 23)) >> 24;
+                            b = v - (int32_t)((uint32_t)d << 24);
+                        }
+                        const int32_t left = __shfl(row_scan16(sLeft[a][lane & (kTile - 1)]),
+                                                    (lane & (64 - kTile)) | (t >> kTileLog2));
+                        const int32_t w = uniform_i32(sRelC[i]) + left + row_scan16(d);
+                        cov = fill_rule_coverage((float)w + (float)b * (1.f / 65536.f), pr.flags);
+                    }
                 }
                 else
                     cov = (pr.flags & 1u) ? 0.f : fill_rule_coverage((float)sRelC[i], pr.flags);
                 if (cov != 0.f || (pr.flags & 8u))
                     resolve_pixel(pr, raster + path, cov, px, py, noise255, ramps, gradHeight, images, col, clip);
             }
-            __syncthreads(); // the next batch clears the arrays and rewrites sArrEntry
+            __syncthreads(); // every wave has read sLeft
+            (&sLeft[0][0])[t] = 0;
+            __syncthreads(); // the next batch accumulates into the arrays and sLeft
             s += len;
         }
     }
