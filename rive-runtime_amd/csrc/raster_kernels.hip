@@ -1912,12 +1912,14 @@ __device__ __forceinline__ void resolve_pixel(const PathRaster& pr, const PathRa
         src[1] *= a;
         src[2] *= a;
     }
+    // src-over, dst * (1 - a) + src, fused (as the reference's shaders are
+    // compiled)
     const float ia = 1.f - a;
     const uint32_t c = colPacked;
     float out[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-        out[k] = src[k] + float((c >> (8 * k)) & 0xffu) * ia;
+        out[k] = fmaf(float((c >> (8 * k)) & 0xffu), ia, src[k]);
     if (a != 0.f)
     {
         out[0] += noise255;
@@ -2228,7 +2230,7 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                         const int32_t left = __shfl(row_scan16(sLeft[a][lane & (kTile - 1)]),
                                                     (lane & (64 - kTile)) | (t >> kTileLog2));
                         const int32_t w = uniform_i32(sRelC[i]) + left + row_scan16(d);
-                        cov = fill_rule_coverage((float)w + (float)b * (1.f / 65536.f), pr.flags);
+                        cov = fill_rule_coverage(fmaf((float)b, 1.f / 65536.f, (float)w), pr.flags); // exact scaling
                     }
                 }
                 else
