@@ -803,6 +803,52 @@ __device__ __forceinline__ void prim_tile_range(const Prim& p, const PathCells& 
     *ty1 = min((maxy >> (kSubpixelLog2 + kTileLog2)) + 1, c.ty1);
 }
 
+// The extent along v of the part of a primitive inside the slab lo <= u <= hi
+// (u, v: its four vertices' coordinates across and along the slab, each of
+// magnitude below 2^24 so exact in float), from its two triangles' five edges
+// clipped to the slab, widened by the float error; false when no edge meets
+// the slab.  The part of a polygon inside a slab always has boundary inside
+// it, so the clipped edges bound it.
+__device__ __forceinline__ bool slab_extent(const float (&u)[4], const float (&v)[4], float lo, float hi,
+                                            float* vlo, float* vhi)
+{
+    constexpr int ea[5] = {0, 1, 2, 1, 3}, eb[5] = {1, 2, 0, 3, 2};
+    float rlo = INFINITY, rhi = -INFINITY, eps = 1.f;
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+    {
+        float u0 = u[ea[k]], v0 = v[ea[k]], u1 = u[eb[k]], v1 = v[eb[k]];
+        if (u0 > u1)
+        {
+            const float tu = u0, tv = v0;
+            u0 = u1;
+            v0 = v1;
+            u1 = tu;
+            v1 = tv;
+        }
+        if (u1 < lo || u0 > hi)
+            continue;
+        float va = v0, vb = v1;
+        if (u0 < lo || u1 > hi)
+        {
+            // u1 > u0 here.  The slope is within 2^-22 relative, and each
+            // clipped v within |dv| 2^-20 + |v| 2^-22 of the true one.
+            const float dv = v1 - v0;
+            const float m = dv * __builtin_amdgcn_rcpf(u1 - u0);
+            if (u0 < lo)
+                va = v0 + (lo - u0) * m;
+            if (u1 > hi)
+                vb = v0 + (hi - u0) * m;
+            eps = fmaxf(eps, fabsf(dv) * (1.f / (1 << 19)) + (fabsf(va) + fabsf(vb)) * (1.f / (1 << 21)));
+        }
+        rlo = fminf(rlo, fminf(va, vb));
+        rhi = fmaxf(rhi, fmaxf(va, vb));
+    }
+    *vlo = rlo - eps;
+    *vhi = rhi + eps;
+    return rlo <= rhi;
+}
+
 // "q right of edge a->b" for the ray-left crossing rule, with the sample point
 // perturbed by (+e, +e^2): s*D(q) >= 0 where s = +1 for upward edges.
 __device__ __forceinline__ bool right_of(int32_t ax, int32_t ay, int32_t bx, int32_t by, int64_t qx,
@@ -812,16 +858,44 @@ __device__ __forceinline__ bool right_of(int32_t ax, int32_t ay, int32_t bx, int
     return by < ay ? D >= 0 : D <= 0;
 }
 
-// The cells of a path's grid that primitive p covers (row-major, grid-local
-// index), passed to visit(cell).
+// The cells of a path's grid that primitive p can touch (row-major,
+// grid-local index), passed to visit(cell).  The raster reads a tile's
+// primitives at the points of [X0, X0 + 3968] x [Y0, Y0 + 3968] (subpixels:
+// sample points, sample-line crossings right of X0, the left edge's
+// crossings), so a primitive spanning several tile rows and columns visits,
+// in each tile row, the tiles whose region widened by 64 meets the part of
+// the primitive in that row's widened slab.
 template <typename F>
 __device__ __forceinline__ void for_prim_cells(const Prim& p, const PathCells& c, F visit)
 {
+    constexpr int kTileSub = 1 << (kSubpixelLog2 + kTileLog2);
+    constexpr float kNear = 64.f, kFar = (float)(kTileSub - (1 << kSubpixelLog2) / 2) + 64.f;
     int tx0, ty0, tx1, ty1;
     prim_tile_range(p, c, &tx0, &ty0, &tx1, &ty1);
+    float vx[4], vy[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        vx[k] = (float)p.p[2 * k];
+        vy[k] = (float)p.p[2 * k + 1];
+    }
+    const bool trim = ty1 - ty0 > 1 && tx1 - tx0 > 1;
     for (int ty = ty0; ty < ty1; ++ty)
-        for (int tx = tx0; tx < tx1; ++tx)
+    {
+        int a = tx0, b = tx1 - 1;
+        if (trim)
+        {
+            const float Y0 = (float)(ty * kTileSub);
+            float xlo, xhi;
+            if (!slab_extent(vy, vx, Y0 - kNear, Y0 + kFar, &xlo, &xhi))
+                continue;
+            // tile tx's widened region spans [tx*4096 - 64, tx*4096 + 3968 + 64]
+            a = max(a, (int)ceilf(fmaxf((xlo - kFar) * (1.f / kTileSub), -1.f)));
+            b = min(b, (int)floorf(fminf((xhi + kNear) * (1.f / kTileSub), (float)c.tx1)));
+        }
+        for (int tx = a; tx <= b; ++tx)
             visit((uint32_t)(ty - c.ty0) * c.width + (uint32_t)(tx - c.tx0));
+    }
 }
 
 // Corner crossings of a fill primitive's inner edge I0->I1 on tile-row lines:
@@ -1241,48 +1315,16 @@ __device__ __forceinline__ void prim_rows(const Prim& p, const TileGeom& g, int*
         vx[k] = (float)(p.p[2 * k] - g.X0);
         vy[k] = (float)(p.p[2 * k + 1] - g.Y0);
     }
-    constexpr float xa = -64.f, xb = (float)(kTile << kSubpixelLog2) + 64.f;
-    constexpr int ea[5] = {0, 1, 2, 1, 3}, eb[5] = {1, 2, 0, 3, 2};
-    float ylo = INFINITY, yhi = -INFINITY, eps = 1.f;
-#pragma unroll
-    for (int k = 0; k < 5; ++k)
-    {
-        float x0 = vx[ea[k]], y0 = vy[ea[k]], x1 = vx[eb[k]], y1 = vy[eb[k]];
-        if (x0 > x1)
-        {
-            const float tx = x0, ty = y0;
-            x0 = x1;
-            y0 = y1;
-            x1 = tx;
-            y1 = ty;
-        }
-        if (x1 < xa || x0 > xb)
-            continue;
-        float ya = y0, yb = y1;
-        if (x0 < xa || x1 > xb)
-        {
-            // x1 > x0 here.  The slope is within 2^-22 relative, and each
-            // clipped y within |dy| 2^-20 + |y| 2^-22 of the true one.
-            const float dy = y1 - y0;
-            const float m = dy * __builtin_amdgcn_rcpf(x1 - x0);
-            if (x0 < xa)
-                ya = y0 + (xa - x0) * m;
-            if (x1 > xb)
-                yb = y0 + (xb - x0) * m;
-            eps = fmaxf(eps, fabsf(dy) * (1.f / (1 << 19)) + (fabsf(ya) + fabsf(yb)) * (1.f / (1 << 21)));
-        }
-        ylo = fminf(ylo, fminf(ya, yb));
-        yhi = fmaxf(yhi, fmaxf(ya, yb));
-    }
-    if (!(ylo <= yhi))
+    float ylo, yhi;
+    if (!slab_extent(vx, vy, -64.f, (float)(kTile << kSubpixelLog2) + 64.f, &ylo, &yhi))
     {
         *r0 = 1;
         *r1 = 0;
         return;
     }
     const float half = (float)(1 << (kSubpixelLog2 - 1)), inv = 1.f / (float)(1 << kSubpixelLog2);
-    a0 = max(a0, (int)ceilf(fmaxf((ylo - eps - half) * inv, -1.f)));
-    a1 = min(a1, (int)floorf(fminf((yhi + eps - half) * inv, (float)kTile)));
+    a0 = max(a0, (int)ceilf(fmaxf((ylo - half) * inv, -1.f)));
+    a1 = min(a1, (int)floorf(fminf((yhi - half) * inv, (float)kTile)));
     *r0 = a0;
     *r1 = a1;
 }
