@@ -2074,7 +2074,16 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                 off = here.x;
                 n = cellPacked[cell + 1].x - off;
                 C = (int32_t)here.y;
-                listed = n != 0 || C != 0;
+                // A path with no primitives here covers the tile with its corner
+                // winding's coverage; when that is zero it draws nothing, and a
+                // clip update's zero-coverage write reads the same as no write
+                // (every clip draw has a fresh ID), so it is not listed.
+                listed = n != 0;
+                if (n == 0 && C != 0)
+                {
+                    const uint32_t flags = raster[p].flags;
+                    listed = !(flags & 1u) && fill_rule_coverage((float)C, flags) != 0.f;
+                }
             }
         }
         const uint32_t vSA = slot_arrays(n);

@@ -161,8 +161,23 @@ def sc_clip_rects():
     return b
 
 
+def sc_zero_coverage_windings():
+    """Whole tiles where a path's winding is nonzero but its coverage is zero:
+    even-odd with winding 2 (as a fill and as a clip) and a counterclockwise
+    rect under the clockwise rule, so no primitives and a zero-coverage corner
+    winding in those tiles."""
+    b = StreamBuilder(80, 72, dither=1)
+    b.draw(b.path(rect(0, 0, 80, 72)), color=0xff204060)
+    b.draw(b.path(rect(1, 1, 60, 62) + rect(3, 2, 62, 66), fill_rule=EVENODD), color=0xc0ff8000)
+    b.draw(b.path(rect(36, 2, 79, 50, clockwise=False), fill_rule=CLOCKWISE), color=0xff00ff00)
+    b.draw(b.path(rect(36, 2, 79, 50), fill_rule=CLOCKWISE), color=0x6000ff00)
+    b.draw(b.path(rect(0, 0, 80, 72)), color=0x9000ffff,
+           clips=(b.path(rect(2, 3, 70, 60) + rect(4, 5, 74, 68), fill_rule=EVENODD),))
+    return b
+
+
 SCENARIOS = {
-    "clip_rects": sc_clip_rects,
+    "clip_rects": sc_clip_rects, "zero_coverage_windings": sc_zero_coverage_windings,
     "empty_frame": sc_empty_frame, "culled_draws": sc_culled_draws, "one_pixel": sc_one_pixel,
     "offcanvas_and_huge": sc_offcanvas_and_huge, "degenerate_cubics": sc_degenerate_cubics,
     "clockwise_mirrored": sc_clockwise_mirrored, "clips": sc_clips, "gradients": sc_gradients,
