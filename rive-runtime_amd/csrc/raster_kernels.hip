@@ -948,8 +948,13 @@ __global__ __launch_bounds__(256) void bin_count_draw_kernel(const Prim* __restr
                                                              uint32_t* __restrict__ cellCount,
                                                              int32_t* __restrict__ cellBackdrop)
 {
-    __shared__ uint32_t sCount[kBinLdsCells];
-    __shared__ int32_t sBack[kBinLdsCells];
+    // One word per cell: the primitive count in the low 16 bits and the
+    // corner-crossing sum, two's complement, in the high 16.  A draw with
+    // fewer than 32768 primitives keeps the count below 2^15 (no carry) and
+    // the sum within +-2^15 (each primitive adds at most +-1 to a cell), so
+    // both halves are exact; larger draws use global atomics.  32 KB of LDS
+    // keeps five workgroups per CU.
+    __shared__ uint32_t sCell[kBinLdsCells];
     const DrawTess d = draws[blockIdx.x];
     if (d.pathID == 0 || d.forwardCount == 0)
         return;
@@ -957,14 +962,11 @@ __global__ __launch_bounds__(256) void bin_count_draw_kernel(const Prim* __restr
     const uint32_t nCells = grid_cells(c);
     if (nCells == 0)
         return;
-    const bool local = nCells <= kBinLdsCells;
+    const bool local = nCells <= kBinLdsCells && d.forwardCount < 32768u;
     if (local)
     {
         for (uint32_t i = threadIdx.x; i < nCells; i += blockDim.x)
-        {
-            sCount[i] = 0;
-            sBack[i] = 0;
-        }
+            sCell[i] = 0;
         __syncthreads();
     }
     uint32_t* gCount = cellCount + c.base;
@@ -976,8 +978,8 @@ __global__ __launch_bounds__(256) void bin_count_draw_kernel(const Prim* __restr
             continue;
         if (local)
         {
-            for_prim_cells(p, c, [&](uint32_t cell) { atomicAdd(&sCount[cell], 1u); });
-            for_prim_crossings(p, c, [&](uint32_t cell, int v) { atomicAdd(&sBack[cell], v); });
+            for_prim_cells(p, c, [&](uint32_t cell) { atomicAdd(&sCell[cell], 1u); });
+            for_prim_crossings(p, c, [&](uint32_t cell, int v) { atomicAdd(&sCell[cell], (uint32_t)v << 16); });
         }
         else
         {
@@ -990,10 +992,11 @@ __global__ __launch_bounds__(256) void bin_count_draw_kernel(const Prim* __restr
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < nCells; i += blockDim.x)
         {
-            if (sCount[i] != 0)
-                atomicAdd(&gCount[i], sCount[i]);
-            if (sBack[i] != 0)
-                atomicAdd(&gBack[i], sBack[i]);
+            const uint32_t w = sCell[i];
+            if ((w & 0xffffu) != 0)
+                atomicAdd(&gCount[i], w & 0xffffu);
+            if ((w >> 16) != 0)
+                atomicAdd(&gBack[i], (int32_t)w >> 16);
         }
     }
 }
