@@ -89,7 +89,7 @@ struct rive_hip_context
     MappedBuffer bufs[RIVE_HIP_BUFFER_KIND_COUNT];
     // pipeline scratch (grow-only)
     DevBuf spanTexelCounts, spanTexelOffsets, scanPartials, scalars;
-    DevBuf tess, prims, cells, cellCount, cellOffset, cellCursor, cellBackdrop, binned, cellPacked;
+    DevBuf tess, prims, cells, cellCount, cellOffset, cellCursor, cellBackdrop, binned, cellPacked, pathDraws;
     DevBuf pathRaster, ramps, drawTess, verts;
     DevBuf geoCounts, geoRecords, geoContours, geoSpans, geoSpanCounts, geoSpanOffsets;
     uint32_t lastMode = 0, lastRecordCount = 0, lastTessVertexCount = 0;
@@ -159,7 +159,7 @@ void rive_hip_destroy(rive_hip_context* ctx)
                      &ctx->ramps, &ctx->drawTess, &ctx->geoCounts, &ctx->geoRecords,
                      &ctx->geoContours, &ctx->geoSpans, &ctx->geoSpanCounts, &ctx->geoSpanOffsets,
                      &ctx->verts, &ctx->imageTable, &ctx->pathImage, &ctx->canvasTable,
-                     &ctx->pathCanvas, &ctx->cellPacked};
+                     &ctx->pathCanvas, &ctx->cellPacked, &ctx->pathDraws};
     for (DevBuf* b : all)
         if (b->ptr)
             (void)hipFree(b->ptr);
@@ -508,10 +508,12 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     HIP_TRY(hipMemsetAsync(ctx->cellCount.ptr, 0, (size_t)(totalCells + 1) * 4, ctx->stream));
     HIP_TRY(hipMemsetAsync(ctx->cellCursor.ptr, 0, (size_t)(totalCells + 1) * 4, ctx->stream));
     HIP_TRY(hipMemsetAsync(ctx->cellBackdrop.ptr, 0, (size_t)(totalCells + 1) * 4, ctx->stream));
+    HIP_TRY(ensure(ctx->pathDraws, (size_t)(pathCount + 1) * 4));
+    HIP_TRY(hipMemsetAsync(ctx->pathDraws.ptr, 0, (size_t)(pathCount + 1) * 4, ctx->stream));
     if (totalPrims > 0 && drawCount > 0)
         bin_count_draw_kernel<<<drawCount, 256, 0, ctx->stream>>>(
             P<Prim>(ctx->prims), drawTess, P<PathCells>(ctx->cells), P<uint32_t>(ctx->cellCount),
-            P<int32_t>(ctx->cellBackdrop));
+            P<int32_t>(ctx->cellBackdrop), P<uint32_t>(ctx->pathDraws));
     rc = device_scan(ctx, P<uint32_t>(ctx->cellCount), totalCells, P<uint32_t>(ctx->cellOffset), scal + 2);
     if (rc)
         return rc;
@@ -532,7 +534,7 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     if (totalPrims > 0 && drawCount > 0)
         bin_scatter_draw_kernel<<<drawCount, 256, 0, ctx->stream>>>(
             P<Prim>(ctx->prims), drawTess, P<PathCells>(ctx->cells), P<uint32_t>(ctx->cellOffset),
-            P<uint32_t>(ctx->cellCursor), P<uint32_t>(ctx->binned));
+            P<uint32_t>(ctx->cellCursor), P<uint32_t>(ctx->pathDraws), P<uint32_t>(ctx->binned));
     HIP_TRY(ensure(ctx->cellPacked, (size_t)(totalCells + 1) * sizeof(uint2)));
     cell_pack_kernel<<<(totalCells + 1 + 255) / 256, 256, 0, ctx->stream>>>(
         P<uint32_t>(ctx->cellOffset), P<int32_t>(ctx->cellBackdrop), totalCells, P<uint2>(ctx->cellPacked));

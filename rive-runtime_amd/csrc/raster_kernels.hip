@@ -946,7 +946,8 @@ __global__ __launch_bounds__(256) void bin_count_draw_kernel(const Prim* __restr
                                                              const DrawTess* __restrict__ draws,
                                                              const PathCells* __restrict__ cells,
                                                              uint32_t* __restrict__ cellCount,
-                                                             int32_t* __restrict__ cellBackdrop)
+                                                             int32_t* __restrict__ cellBackdrop,
+                                                             uint32_t* __restrict__ pathDraws)
 {
     // One word per cell: the primitive count in the low 16 bits and the
     // corner-crossing sum, two's complement, in the high 16.  A draw with
@@ -962,6 +963,8 @@ __global__ __launch_bounds__(256) void bin_count_draw_kernel(const Prim* __restr
     const uint32_t nCells = grid_cells(c);
     if (nCells == 0)
         return;
+    if (threadIdx.x == 0)
+        atomicAdd(&pathDraws[d.pathID], 1u); // draws binning into this path's grid
     const bool local = nCells <= kBinLdsCells && d.forwardCount < 32768u;
     if (local)
     {
@@ -1001,13 +1004,16 @@ __global__ __launch_bounds__(256) void bin_count_draw_kernel(const Prim* __restr
     }
 }
 
-// Writes each primitive index into its cells' bins: counts in LDS, one global
-// reservation per cell, then LDS cursors hand out the slots.
+// Writes each primitive index into its cells' bins.  A path binned by this
+// draw alone owns its cells' ranges, so LDS cursors from 0 hand out slots
+// after the cell offsets; a path shared by several draws counts in LDS,
+// reserves once per cell on the global cursors, then hands out slots.
 __global__ __launch_bounds__(256) void bin_scatter_draw_kernel(const Prim* __restrict__ prims,
                                                                const DrawTess* __restrict__ draws,
                                                                const PathCells* __restrict__ cells,
                                                                const uint32_t* __restrict__ cellOffset,
                                                                uint32_t* __restrict__ cellCursor,
+                                                               const uint32_t* __restrict__ pathDraws,
                                                                uint32_t* __restrict__ binned)
 {
     __shared__ uint32_t sCur[kBinLdsCells];
@@ -1035,6 +1041,17 @@ __global__ __launch_bounds__(256) void bin_scatter_draw_kernel(const Prim* __res
     for (uint32_t i = threadIdx.x; i < nCells; i += blockDim.x)
         sCur[i] = 0;
     __syncthreads();
+    if (pathDraws[d.pathID] == 1u)
+    {
+        for (uint32_t k = threadIdx.x; k < d.forwardCount; k += blockDim.x)
+        {
+            const uint32_t g = d.primBase + k;
+            const Prim p = prims[g];
+            if (!(p.flags & kPrimEmpty))
+                for_prim_cells(p, c, [&](uint32_t cell) { binned[gOffset[cell] + atomicAdd(&sCur[cell], 1u)] = g; });
+        }
+        return;
+    }
     for (uint32_t k = threadIdx.x; k < d.forwardCount; k += blockDim.x)
     {
         const Prim p = prims[d.primBase + k];
