@@ -1995,7 +1995,6 @@ __device__ __forceinline__ uint32_t uniform_u32(uint32_t v)
 {
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
-__device__ __forceinline__ int32_t uniform_i32(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Inclusive prefix sum within each DPP row of 16 lanes (row_shr:1, 2, 4, 8;
 // lanes shifted in from outside the row read 0).  All 64 lanes must be active.
@@ -2025,12 +2024,11 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
     __shared__ int32_t sLeft[kAccArrays][kTile];
     // The listed paths of the current window of 256 paths, in order, with
     // exclusive prefixes of their primitive counts and of slot_arrays().
-    __shared__ uint32_t sRelPath[kWindow];
-    __shared__ uint32_t sRelOff[kWindow];
-    __shared__ uint32_t sRelN[kWindow];
-    __shared__ int32_t sRelC[kWindow];
-    __shared__ uint32_t sRelPrims[kWindow];
-    __shared__ uint32_t sRelSA[kWindow];
+    // sRel[q] = (path, primitive count n, corner winding C, slot/array
+    // prefix) -- one 16-byte read per entry in the resolve; sRelB[q] = (bin
+    // offset, primitive prefix).
+    __shared__ uint4 sRel[kWindow];
+    __shared__ uint2 sRelB[kWindow];
     __shared__ uint32_t sWave[3][kWgThreads / 64];
     __shared__ uint8_t sOwner[kMaxItems];
     __shared__ uint32_t sItemPrim[kWgThreads];
@@ -2143,36 +2141,32 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
         if (listed)
         {
             const uint32_t q = preL + posInWave;
-            sRelPath[q] = p;
-            sRelOff[q] = off;
-            sRelN[q] = n;
-            sRelC[q] = C;
-            sRelPrims[q] = preP + iP - n;
-            sRelSA[q] = preSA + iSA - vSA;
+            sRel[q] = make_uint4(p, n, (uint32_t)C, preSA + iSA - vSA);
+            sRelB[q] = make_uint2(off, preP + iP - n);
         }
         __syncthreads();
 
         for (uint32_t s = 0; s < nRel;)
         {
             // ---- the batch: entries s .. s+len-1, cut where a limit breaks
-            const uint32_t baseP = uniform_u32(sRelPrims[s]), baseSA = uniform_u32(sRelSA[s]);
+            const uint32_t baseP = uniform_u32(sRelB[s].y), baseSA = uniform_u32(sRel[s].w);
             uint32_t len;
             {
                 const uint32_t i = s + (uint32_t)lane;
                 bool stop = true;
                 if (i < nRel)
                 {
-                    const uint32_t ni = sRelN[i];
-                    const uint32_t sa = sRelSA[i] + slot_arrays(ni) - baseSA;
-                    const uint32_t primsIncl = sRelPrims[i] + ni - baseP;
+                    const uint32_t ni = sRel[i].y;
+                    const uint32_t sa = sRel[i].w + slot_arrays(ni) - baseSA;
+                    const uint32_t primsIncl = sRelB[i].y + ni - baseP;
                     stop = (sa >> 16) > (uint32_t)kAccArrays || (primsIncl > kBatchPrims && (sa & 0xffffu) > 1u);
                 }
                 const uint64_t m = __ballot(stop);
                 len = uniform_u32(m ? (uint32_t)__builtin_ctzll(m) : (uint32_t)kBatchEntries);
             }
             const uint32_t last = s + len - 1;
-            const uint32_t nPrim = uniform_u32(sRelPrims[last] + sRelN[last] - baseP);
-            const uint32_t arraysUsed = uniform_u32((sRelSA[last] + slot_arrays(sRelN[last]) - baseSA) >> 16);
+            const uint32_t nPrim = uniform_u32(sRelB[last].y + sRel[last].y - baseP);
+            const uint32_t arraysUsed = uniform_u32((sRel[last].w + slot_arrays(sRel[last].y) - baseSA) >> 16);
 
             if (arraysUsed != 0)
             {
@@ -2193,13 +2187,15 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                         for (uint32_t step = kBatchEntries / 2; step > 0; step >>= 1)
                         {
                             const uint32_t cand = lo + step;
-                            const uint32_t probe = sRelPrims[min(cand, last)] - baseP;
+                            const uint32_t probe = sRelB[min(cand, last)].y - baseP;
                             lo = (cand <= last && probe <= f) ? cand : lo;
                         }
-                        const uint32_t a = (sRelSA[lo] - baseSA) >> 16;
-                        const bool wide = sRelN[lo] > kPackedPrims;
-                        const bool stroke = raster[sRelPath[lo]].flags & 1u;
-                        primIdx = binned[sRelOff[lo] + (f - (sRelPrims[lo] - baseP))];
+                        const uint4 r = sRel[lo];
+                        const uint2 rb = sRelB[lo];
+                        const uint32_t a = (r.w - baseSA) >> 16;
+                        const bool wide = r.y > kPackedPrims;
+                        const bool stroke = raster[r.x].flags & 1u;
+                        primIdx = binned[rb.x + (f - (rb.y - baseP))];
                         const Prim pm = prims[primIdx];
                         int r0, r1;
                         prim_rows(pm, g, &r0, &r1);
@@ -2273,13 +2269,15 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
             for (uint32_t e = 0; e < len; ++e)
             {
                 const uint32_t i = s + e;
-                const uint32_t path = uniform_u32(sRelPath[i]);
-                const uint32_t ni = uniform_u32(sRelN[i]);
+                const uint4 rel = sRel[i];
+                const uint32_t path = uniform_u32(rel.x);
+                const uint32_t ni = uniform_u32(rel.y);
+                const int32_t C = (int32_t)uniform_u32(rel.z);
                 const PathRaster pr = raster[path];
                 float cov;
                 if (ni != 0)
                 {
-                    const uint32_t a = uniform_u32((sRelSA[i] - baseSA) >> 16);
+                    const uint32_t a = uniform_u32((rel.w - baseSA) >> 16);
                     const int32_t v = sAcc[a][t];
                     sAcc[a][t] = 0;
                     if (pr.flags & 1u)
@@ -2300,12 +2298,12 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                         }
                         const int32_t left = __shfl(row_scan16(sLeft[a][lane & (kTile - 1)]),
                                                     (lane & (64 - kTile)) | (t >> kTileLog2));
-                        const int32_t w = uniform_i32(sRelC[i]) + left + row_scan16(d);
+                        const int32_t w = C + left + row_scan16(d);
                         cov = fill_rule_coverage(fmaf((float)b, 1.f / 65536.f, (float)w), pr.flags); // exact scaling
                     }
                 }
                 else
-                    cov = (pr.flags & 1u) ? 0.f : fill_rule_coverage((float)sRelC[i], pr.flags);
+                    cov = (pr.flags & 1u) ? 0.f : fill_rule_coverage((float)C, pr.flags);
                 if (cov != 0.f || (pr.flags & 8u))
                     resolve_pixel(pr, raster + path, cov, px, py, noise255, ramps, gradHeight, images, col, clip);
             }
