@@ -90,7 +90,7 @@ struct rive_hip_context
     // pipeline scratch (grow-only)
     DevBuf spanTexelCounts, spanTexelOffsets, scanPartials, scalars;
     DevBuf tess, prims, cells, cellCount, cellOffset, cellCursor, cellBackdrop, binned, cellPacked, pathDraws;
-    DevBuf pathRaster, ramps, drawTess, verts;
+    DevBuf pathRaster, pathPaint, ramps, drawTess, verts;
     DevBuf geoCounts, geoRecords, geoContours, geoSpans, geoSpanCounts, geoSpanOffsets;
     uint32_t lastMode = 0, lastRecordCount = 0, lastTessVertexCount = 0;
     uint32_t* hostScalars = nullptr; // pinned
@@ -159,7 +159,7 @@ void rive_hip_destroy(rive_hip_context* ctx)
                      &ctx->ramps, &ctx->drawTess, &ctx->geoCounts, &ctx->geoRecords,
                      &ctx->geoContours, &ctx->geoSpans, &ctx->geoSpanCounts, &ctx->geoSpanOffsets,
                      &ctx->verts, &ctx->imageTable, &ctx->pathImage, &ctx->canvasTable,
-                     &ctx->pathCanvas, &ctx->cellPacked, &ctx->pathDraws};
+                     &ctx->pathCanvas, &ctx->cellPacked, &ctx->pathDraws, &ctx->pathPaint};
     for (DevBuf* b : all)
         if (b->ptr)
             (void)hipFree(b->ptr);
@@ -539,15 +539,18 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     cell_pack_kernel<<<(totalCells + 1 + 255) / 256, 256, 0, ctx->stream>>>(
         P<uint32_t>(ctx->cellOffset), P<int32_t>(ctx->cellBackdrop), totalCells, P<uint2>(ctx->cellPacked));
     HIP_TRY(ensure(ctx->pathRaster, (size_t)(pathCount + 1) * sizeof(PathRaster)));
+    HIP_TRY(ensure(ctx->pathPaint, (size_t)(pathCount + 1) * sizeof(PathPaint)));
     path_raster_kernel<<<(pathCount + 1 + 255) / 256, 256, 0, ctx->stream>>>(
-        paths, paints, aux, P<uint32_t>(ctx->pathImage), pathCount, gradRows, P<PathRaster>(ctx->pathRaster));
+        paths, paints, aux, P<uint32_t>(ctx->pathImage), pathCount, gradRows, P<PathRaster>(ctx->pathRaster),
+        P<PathPaint>(ctx->pathPaint));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
 
     // ---- raster + resolve
     if (tileTotal > 0)
         raster_kernel<<<tileTotal, kWgThreads, 0, ctx->stream>>>(
-            P<PathCells>(ctx->cells), P<PathRaster>(ctx->pathRaster), P<uint2>(ctx->cellPacked),
+            P<PathCells>(ctx->cells), P<PathRaster>(ctx->pathRaster), P<PathPaint>(ctx->pathPaint),
+            P<uint2>(ctx->cellPacked),
             P<uint32_t>(ctx->binned), P<Prim>(ctx->prims),
             P<uint32_t>(ctx->ramps), gradRows, P<ImageDesc>(ctx->imageTable), P<CanvasDev>(ctx->canvasTable),
             canvasCount, d->dither);

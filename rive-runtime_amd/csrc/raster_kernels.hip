@@ -1934,40 +1934,55 @@ __device__ __forceinline__ float clip_rect_coverage(const float* m, int px, int 
     return __half2float(__float2half_rn(d));
 }
 
-__device__ __forceinline__ void resolve_pixel(const PathRaster& pr, const PathRaster* __restrict__ prp,
+// The fields of a path's PathRaster that every resolve reads, 32 bytes (one
+// scalar load); the rest is read from PathRaster where it is used.
+struct PathPaint
+{
+    uint32_t kind;  // PathRaster flags | paintType << 16 | kPaintImage
+    uint32_t clips; // clipID | outerClipID << 16
+    float color255[4];
+    float alpha;
+    uint32_t pad;
+};
+static_assert(sizeof(PathPaint) == 32, "PathPaint layout");
+constexpr uint32_t kPaintImage = 1u << 20;
+
+__device__ __forceinline__ void resolve_pixel(const PathPaint& pp, const PathRaster* __restrict__ prp,
                                               float coverage, int px, int py, float noise255,
                                               const uint32_t* __restrict__ ramps, uint32_t gradHeight,
                                               const ImageDesc* __restrict__ images, uint32_t& colPacked,
                                               uint32_t& clip)
 {
     const uint32_t clipID = clip & 0xffffu;
-    if (pr.flags & 8u) // clip update
+    const uint32_t flags = pp.kind & 0xffffu;
+    const uint32_t drawClip = pp.clips & 0xffffu, outerClip = pp.clips >> 16;
+    if (flags & 8u) // clip update
     {
-        if (pr.outerClipID != 0u)
-            coverage = fminf(coverage, clipID == pr.outerClipID ? clip_coverage(clip) : 0.f);
-        clip = ((uint32_t)__half_as_ushort(__float2half_rn(coverage)) << 16) | (pr.clipID & 0xffffu);
+        if (outerClip != 0u)
+            coverage = fminf(coverage, clipID == outerClip ? clip_coverage(clip) : 0.f);
+        clip = ((uint32_t)__half_as_ushort(__float2half_rn(coverage)) << 16) | drawClip;
         return;
     }
-    if (pr.clipID != 0u)
-        coverage = clipID == pr.clipID ? fminf(clip_coverage(clip), coverage) : 0.f;
-    if (pr.flags & 16u)
-        coverage = fminf(fmaxf(clip_rect_coverage(pr.clipRect, px, py), 0.f), coverage);
-    if ((pr.flags & 0xf00u) != 0u || pr.image != 0u)
+    if (drawClip != 0u)
+        coverage = clipID == drawClip ? fminf(clip_coverage(clip), coverage) : 0.f;
+    if (flags & 16u)
+        coverage = fminf(fmaxf(clip_rect_coverage(prp->clipRect, px, py), 0.f), coverage);
+    if ((flags & 0xf00u) != 0u || (pp.kind & kPaintImage))
     {
         colPacked = resolve_general(prp, coverage, px, py, noise255, ramps, gradHeight, images, colPacked);
         return;
     }
     float src[4], a;
-    if (pr.paintType == SOLID_COLOR_PAINT_TYPE)
+    if ((pp.kind >> 16 & 0xfu) == SOLID_COLOR_PAINT_TYPE)
     {
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            src[k] = pr.color255[k] * coverage;
-        a = pr.alpha * coverage;
+            src[k] = pp.color255[k] * coverage;
+        a = pp.alpha * coverage;
     }
     else
     {
-        sample_gradient255(pr, px, py, ramps, gradHeight, src);
+        sample_gradient255(*prp, px, py, ramps, gradHeight, src);
         src[3] *= coverage;
         a = src[3] * (1.f / 255.f);
         src[0] *= a;
@@ -2016,7 +2031,7 @@ __device__ __forceinline__ uint32_t slot_arrays(uint32_t n)
 
 __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
     const PathCells* __restrict__ cells, const PathRaster* __restrict__ raster,
-    const uint2* __restrict__ cellPacked, const uint32_t* __restrict__ binned, const Prim* __restrict__ prims,
+    const PathPaint* __restrict__ pathPaint, const uint2* __restrict__ cellPacked, const uint32_t* __restrict__ binned, const Prim* __restrict__ prims,
     const uint32_t* __restrict__ ramps, uint32_t gradHeight, const ImageDesc* __restrict__ images,
     const CanvasDev* __restrict__ canvases, uint32_t canvasCount, uint32_t dither)
 {
@@ -2099,7 +2114,7 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                 listed = n != 0;
                 if (n == 0 && C != 0)
                 {
-                    const uint32_t flags = raster[p].flags;
+                    const uint32_t flags = pathPaint[p].kind & 0xffffu;
                     listed = !(flags & 1u) && fill_rule_coverage((float)C, flags) != 0.f;
                 }
             }
@@ -2194,7 +2209,7 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                         const uint2 rb = sRelB[lo];
                         const uint32_t a = (r.w - baseSA) >> 16;
                         const bool wide = r.y > kPackedPrims;
-                        const bool stroke = raster[r.x].flags & 1u;
+                        const bool stroke = pathPaint[r.x].kind & 1u;
                         primIdx = binned[rb.x + (f - (rb.y - baseP))];
                         const Prim pm = prims[primIdx];
                         int r0, r1;
@@ -2273,14 +2288,15 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                 const uint32_t path = uniform_u32(rel.x);
                 const uint32_t ni = uniform_u32(rel.y);
                 const int32_t C = (int32_t)uniform_u32(rel.z);
-                const PathRaster pr = raster[path];
+                const PathPaint pr = pathPaint[path];
+                const uint32_t prFlags = pr.kind & 0xffffu;
                 float cov;
                 if (ni != 0)
                 {
                     const uint32_t a = uniform_u32((rel.w - baseSA) >> 16);
                     const int32_t v = sAcc[a][t];
                     sAcc[a][t] = 0;
-                    if (pr.flags & 1u)
+                    if (prFlags & 1u)
                         cov = fminf(__int_as_float(v), 1.f);
                     else
                     {
@@ -2299,12 +2315,12 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                         const int32_t left = __shfl(row_scan16(sLeft[a][lane & (kTile - 1)]),
                                                     (lane & (64 - kTile)) | (t >> kTileLog2));
                         const int32_t w = C + left + row_scan16(d);
-                        cov = fill_rule_coverage(fmaf((float)b, 1.f / 65536.f, (float)w), pr.flags); // exact scaling
+                        cov = fill_rule_coverage(fmaf((float)b, 1.f / 65536.f, (float)w), prFlags); // exact scaling
                     }
                 }
                 else
-                    cov = (pr.flags & 1u) ? 0.f : fill_rule_coverage((float)C, pr.flags);
-                if (cov != 0.f || (pr.flags & 8u))
+                    cov = (prFlags & 1u) ? 0.f : fill_rule_coverage((float)C, prFlags);
+                if (cov != 0.f || (prFlags & 8u))
                     resolve_pixel(pr, raster + path, cov, px, py, noise255, ramps, gradHeight, images, col, clip);
             }
             __syncthreads(); // every wave has read sLeft
@@ -2321,7 +2337,8 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
 // draw_path.vert:431-545).
 __global__ void path_raster_kernel(const PathDataV* __restrict__ paths, const uint2* __restrict__ paints,
                                    const float* __restrict__ aux, const uint32_t* __restrict__ pathImage,
-                                   uint32_t pathCount, uint32_t gradHeight, PathRaster* __restrict__ out)
+                                   uint32_t pathCount, uint32_t gradHeight, PathRaster* __restrict__ out,
+                                   PathPaint* __restrict__ outPaint)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > pathCount)
@@ -2390,6 +2407,15 @@ __global__ void path_raster_kernel(const PathDataV* __restrict__ paths, const ui
         r.clipRect[k] = ax[8 + k];
     r.pad = 0;
     out[i] = r;
+    PathPaint pp;
+    pp.kind = r.flags | (r.paintType << 16) | (r.image != 0u ? kPaintImage : 0u);
+    pp.clips = (r.clipID & 0xffffu) | (r.outerClipID << 16);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        pp.color255[k] = r.color255[k];
+    pp.alpha = r.alpha;
+    pp.pad = 0;
+    outPaint[i] = pp;
 }
 
 // ---------------------------------------------------------------------------
