@@ -1507,23 +1507,33 @@ __device__ void stroke_prim_row(const Prim& p, int j, const TileGeom& g, uint32_
             const int32_t t01 = inside_threshold_t<I>(e01, 1), t12 = inside_threshold_t<I>(e12, 1),
                           t20 = inside_threshold_t<I>(e20, 1);
             const float inv = __builtin_amdgcn_rcpf((float)A1);
+            // Edge values at pixel ilo, stepped exactly by 2A per pixel; the
+            // interpolated coverages are linear in the column too and step by
+            // float adds (within 16 ulp of a direct evaluation).
+            const int32_t i2 = 2 * ilo + 1;
+            int32_t k01 = e01.A * i2 + e01.B * j2, k12 = e12.A * i2 + e12.B * j2, k20 = e20.A * i2 + e20.B * j2;
+            const float l0 = ((float)e12.D0 + 128.f * (float)k12) * inv;
+            const float l1 = ((float)e20.D0 + 128.f * (float)k20) * inv;
+            const float l2 = ((float)e01.D0 + 128.f * (float)k01) * inv;
+            const float s0 = 256.f * (float)e12.A * inv, s1 = 256.f * (float)e20.A * inv,
+                        s2 = 256.f * (float)e01.A * inv;
+            float cx = l0 * p.cov[0] + l1 * p.cov[2] + l2 * p.cov[4];
+            float cy = l0 * p.cov[1] + l1 * p.cov[3] + l2 * p.cov[5];
+            const float dcx = s0 * p.cov[0] + s1 * p.cov[2] + s2 * p.cov[4];
+            const float dcy = s0 * p.cov[1] + s1 * p.cov[3] + s2 * p.cov[5];
             for (int i = ilo; i <= ihi; ++i)
             {
-                const int32_t i2 = 2 * i + 1;
-                const int32_t k01 = e01.A * i2 + e01.B * j2;
-                const int32_t k12 = e12.A * i2 + e12.B * j2;
-                const int32_t k20 = e20.A * i2 + e20.B * j2;
                 if (k01 >= t01 && k12 >= t12 && k20 >= t20)
                 {
-                    const float l0 = ((float)e12.D0 + 128.f * (float)k12) * inv;
-                    const float l1 = ((float)e20.D0 + 128.f * (float)k20) * inv;
-                    const float l2 = ((float)e01.D0 + 128.f * (float)k01) * inv;
-                    const float cx = l0 * p.cov[0] + l1 * p.cov[2] + l2 * p.cov[4];
-                    const float cy = l0 * p.cov[1] + l1 * p.cov[3] + l2 * p.cov[5];
                     const float v = fminf(cx, cy);
                     if (v > 0.f)
                         atomicMax(&sMax[j * kTile + i], __float_as_uint(v));
                 }
+                k01 += 2 * e01.A;
+                k12 += 2 * e12.A;
+                k20 += 2 * e20.A;
+                cx += dcx;
+                cy += dcy;
             }
         }
     }
@@ -1539,23 +1549,30 @@ __device__ void stroke_prim_row(const Prim& p, int j, const TileGeom& g, uint32_
             const int32_t t21 = inside_threshold_t<I>(EdgeTI<I>{-e12.A, -e12.B, -e12.D0}, 1);
             const int32_t t13 = inside_threshold_t<I>(e13, 1), t32 = inside_threshold_t<I>(e32, 1);
             const float inv = __builtin_amdgcn_rcpf((float)A2);
+            const int32_t i2 = 2 * ilo + 1;
+            int32_t k12 = e12.A * i2 + e12.B * j2, k13 = e13.A * i2 + e13.B * j2, k32 = e32.A * i2 + e32.B * j2;
+            const float l0 = ((float)e13.D0 + 128.f * (float)k13) * inv;  // P1-
+            const float l1 = ((float)e32.D0 + 128.f * (float)k32) * inv;  // P0+
+            const float l2 = -((float)e12.D0 + 128.f * (float)k12) * inv; // P1+
+            const float s0 = 256.f * (float)e13.A * inv, s1 = 256.f * (float)e32.A * inv,
+                        s2 = -256.f * (float)e12.A * inv;
+            float cx = l0 * p.cov[4] + l1 * p.cov[2] + l2 * p.cov[6];
+            float cy = l0 * p.cov[5] + l1 * p.cov[3] + l2 * p.cov[7];
+            const float dcx = s0 * p.cov[4] + s1 * p.cov[2] + s2 * p.cov[6];
+            const float dcy = s0 * p.cov[5] + s1 * p.cov[3] + s2 * p.cov[7];
             for (int i = ilo; i <= ihi; ++i)
             {
-                const int32_t i2 = 2 * i + 1;
-                const int32_t k12 = e12.A * i2 + e12.B * j2;
-                const int32_t k13 = e13.A * i2 + e13.B * j2;
-                const int32_t k32 = e32.A * i2 + e32.B * j2;
                 if (-k12 >= t21 && k13 >= t13 && k32 >= t32)
                 {
-                    const float l0 = ((float)e13.D0 + 128.f * (float)k13) * inv;  // P1-
-                    const float l1 = ((float)e32.D0 + 128.f * (float)k32) * inv;  // P0+
-                    const float l2 = -((float)e12.D0 + 128.f * (float)k12) * inv; // P1+
-                    const float cx = l0 * p.cov[4] + l1 * p.cov[2] + l2 * p.cov[6];
-                    const float cy = l0 * p.cov[5] + l1 * p.cov[3] + l2 * p.cov[7];
                     const float v = fminf(cx, cy);
                     if (v > 0.f)
                         atomicMax(&sMax[j * kTile + i], __float_as_uint(v));
                 }
+                k12 += 2 * e12.A;
+                k13 += 2 * e13.A;
+                k32 += 2 * e32.A;
+                cx += dcx;
+                cy += dcy;
             }
         }
     }
