@@ -1304,10 +1304,6 @@ template <typename I> __device__ __forceinline__ EdgeK edge_k(const EdgeTI<I>& e
     k.thr = inside_threshold_t<I>(e, o);
     return k;
 }
-__device__ __forceinline__ bool edge_in(const EdgeK& k, int32_t i2, int32_t j2)
-{
-    return k.A * i2 + k.B * j2 >= k.thr;
-}
 
 // Sample rows [*r0, *r1] of the tile that a primitive can touch inside the
 // tile's columns.  The per-row work looks at sample points with x in
@@ -1443,16 +1439,22 @@ __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* 
             const EdgeK k12 = edge_k<I>(make_edge_t<I>(p.p[2], p.p[3], p.p[4], p.p[5], X0, Y0), o);
             const EdgeK k20 = edge_k<I>(make_edge_t<I>(p.p[4], p.p[5], p.p[0], p.p[1], X0, Y0), o);
             const float inv = __builtin_amdgcn_rcpf((float)A1);
+            // edge values at pixel ilo, stepped exactly by 2A per pixel
+            const int32_t i2 = 2 * ilo + 1;
+            int32_t v01 = k01.A * i2 + k01.B * j2, v12 = k12.A * i2 + k12.B * j2, v20 = k20.A * i2 + k20.B * j2;
+            int32_t ke = e.A * i2 + e.B * j2;
             const float D0 = (float)e.D0;
             for (int i = ilo; i <= ihi; ++i)
             {
-                const int32_t i2 = 2 * i + 1;
-                if (edge_in(k01, i2, j2) && edge_in(k12, i2, j2) && edge_in(k20, i2, j2))
+                if (v01 >= k01.thr && v12 >= k12.thr && v20 >= k20.thr)
                 {
-                    const int32_t ke = e.A * i2 + e.B * j2;
                     const float De = D0 + 128.f * (float)ke;
                     atomicAdd(&sBorder[j * kTile + i], neg * o * border_fixed(1.f + De * inv));
                 }
+                v01 += 2 * k01.A;
+                v12 += 2 * k12.A;
+                v20 += 2 * k20.A;
+                ke += 2 * e.A;
             }
         }
     }
@@ -1470,16 +1472,21 @@ __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* 
             const EdgeK f12 = edge_k(f12e, o);
             const EdgeK f20 = edge_k<I>(make_edge_t<I>(p.p[6], p.p[7], p.p[4], p.p[5], X0, Y0), o);
             const float inv = __builtin_amdgcn_rcpf((float)A2);
+            const int32_t i2 = 2 * ilo + 1;
+            int32_t v01 = f01.A * i2 + f01.B * j2, v12 = f12.A * i2 + f12.B * j2, v20 = f20.A * i2 + f20.B * j2;
+            int32_t k12 = f12e.A * i2 + f12e.B * j2;
             const float D0 = (float)f12e.D0;
             for (int i = ilo; i <= ihi; ++i)
             {
-                const int32_t i2 = 2 * i + 1;
-                if (edge_in(f01, i2, j2) && edge_in(f12, i2, j2) && edge_in(f20, i2, j2))
+                if (v01 >= f01.thr && v12 >= f12.thr && v20 >= f20.thr)
                 {
-                    const int32_t k12 = f12e.A * i2 + f12e.B * j2;
                     const float D12 = D0 + 128.f * (float)k12;
                     atomicAdd(&sBorder[j * kTile + i], neg * o * border_fixed(D12 * inv));
                 }
+                v01 += 2 * f01.A;
+                v12 += 2 * f12.A;
+                v20 += 2 * f20.A;
+                k12 += 2 * f12e.A;
             }
         }
     }
