@@ -2021,12 +2021,11 @@ __device__ __forceinline__ void resolve_pixel(const PathPaint& pp, const PathRas
 #pragma unroll
     for (int k = 0; k < 4; ++k)
         out[k] = fmaf(float((c >> (8 * k)) & 0xffu), ia, src[k]);
-    if (a != 0.f)
-    {
-        out[0] += noise255;
-        out[1] += noise255;
-        out[2] += noise255;
-    }
+    // dither where the paint lands (adding +0 elsewhere changes nothing)
+    const float n = a != 0.f ? noise255 : 0.f;
+    out[0] += n;
+    out[1] += n;
+    out[2] += n;
     colPacked = pack_unorm8x4(out);
 }
 
@@ -2296,15 +2295,19 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                     }
                     __syncthreads();
                 }
+                // left-edge deltas -> prefix over rows, in place: thread t
+                // holds array t / 16, row t % 16, and a DPP row is one array
+                (&sLeft[0][0])[t] = row_scan16((&sLeft[0][0])[t]);
+                __syncthreads();
             }
 
             // ---- resolve the batch in submission order.  Every thread takes
             // the same entry, so a path's coverage comes from its accumulators
             // here: the winding at a pixel is the corner winding, plus the
-            // left-edge deltas of the rows up to its row, plus the crossings
-            // of its row up to its column -- two 16-lane scans, a DPP row
-            // being a row of the tile.  Each pixel's cells are cleared as they
-            // are read (a two-array path's border array too).
+            // left-edge prefix at its row, plus the crossings of its row up to
+            // its column -- a 16-lane scan, a DPP row being a row of the tile.
+            // Each pixel's cells are cleared as they are read (a two-array
+            // path's border array too).
             for (uint32_t e = 0; e < len; ++e)
             {
                 const uint32_t i = s + e;
@@ -2336,8 +2339,7 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                             d = (v + (1 << 23)) >> 24;
                             b = v - (int32_t)((uint32_t)d << 24);
                         }
-                        const int32_t left = __shfl(row_scan16(sLeft[a][lane & (kTile - 1)]),
-                                                    (lane & (64 - kTile)) | (t >> kTileLog2));
+                        const int32_t left = sLeft[a][t >> kTileLog2];
                         const int32_t w = C + left + row_scan16(d);
                         cov = fill_rule_coverage(fmaf((float)b, 1.f / 65536.f, (float)w), prFlags); // exact scaling
                     }
