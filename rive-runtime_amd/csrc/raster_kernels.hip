@@ -547,6 +547,23 @@ __device__ __forceinline__ uint32_t find_draw(const DrawTess* __restrict__ draws
     return lo;
 }
 
+// The draw of primitive g (the last draw whose primBase <= g) for a block of
+// consecutive primitives: one binary search for the block's first primitive,
+// then a short forward walk per thread (a block spans few draws).  Every
+// thread of the block must call it.
+__device__ __forceinline__ uint32_t find_draw_block(const DrawTess* __restrict__ draws, uint32_t drawCount,
+                                                    uint32_t g)
+{
+    __shared__ uint32_t sFirst;
+    if (threadIdx.x == 0)
+        sFirst = find_draw(draws, drawCount, blockIdx.x * blockDim.x);
+    __syncthreads();
+    uint32_t k = sFirst;
+    while (k + 1 < drawCount && draws[k + 1].primBase <= g)
+        ++k;
+    return k;
+}
+
 // Vertex stage once per forward texel (draw_path_common.glsl:275-789); the
 // segment between texel i and next(i) reuses both ends.
 __global__ void vertex_kernel(const TessTexel* __restrict__ tex, int texels, const PathDataV* __restrict__ paths,
@@ -554,9 +571,10 @@ __global__ void vertex_kernel(const TessTexel* __restrict__ tex, int texels, con
                               VtxPair* __restrict__ verts)
 {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t di = find_draw_block(draws, drawCount, g);
     if (g >= totalPrims)
         return;
-    const DrawTess d = draws[find_draw(draws, drawCount, g)];
+    const DrawTess d = draws[di];
     const int i = min((int)(d.forwardBase + (g - d.primBase)), texels - 2);
     const PathDataV pd = paths[d.pathID];
     const uint32_t cfi = tex[i].w;
@@ -581,9 +599,10 @@ __global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels, con
                               PathCells* __restrict__ cells)
 {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t di = find_draw_block(draws, drawCount, g);
     if (g >= totalPrims)
         return;
-    const DrawTess d = draws[find_draw(draws, drawCount, g)];
+    const DrawTess d = draws[di];
     const int i = min((int)(d.forwardBase + (g - d.primBase)), texels - 2);
     Prim out;
     out.path = d.pathID;
