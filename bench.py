@@ -41,7 +41,7 @@ SEED = rivehip.CONFIG_SEEDS[2]
 # FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by what the same
 # counter reads for a 1 GiB device copy (FETCH x2.0, WRITE x1.0); the runs are
 # summarised in profiles/raster_kernel_1080p_5000paths_traffic.md.
-RASTER_FETCH_KIB, RASTER_WRITE_KIB = 6753083, 24420
+RASTER_FETCH_KIB, RASTER_WRITE_KIB = 6639241, 40740
 RASTER_TRAFFIC_BYTES = RASTER_FETCH_KIB * 1024 * 2 + RASTER_WRITE_KIB * 1024
 
 
