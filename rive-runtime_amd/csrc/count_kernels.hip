@@ -1082,7 +1082,9 @@ int geometry_counting(rive_hip_context* ctx, const rive_hip_flush_desc* d, const
     uint32_t* contourOffsets = tessOffsets + (n + 1);
     uint32_t* recordOffsets = contourOffsets + (n + 1);
     if (n > 0)
-        count_kernel<<<(n + 127) / 128, 128, 0, ctx->stream>>>(
+        // one thread walks one draw: half-wave blocks spread the draws over
+        // more CUs (the walks are serial and latency-bound)
+        count_kernel<<<(n + 31) / 32, 32, 0, ctx->stream>>>(
             draws, n, verbs, ctx->bufs[RIVE_HIP_BUFFER_VERB].size, points,
             ctx->bufs[RIVE_HIP_BUFFER_POINT].size / sizeof(V2d), scal + 12, tessCounts, contourCounts,
             recordCounts);
@@ -1110,7 +1112,7 @@ int geometry_counting(rive_hip_context* ctx, const rive_hip_flush_desc* d, const
     HIP_TRY(ensure(ctx->geoRecords, (size_t)std::max<uint32_t>(recordTotal, 1) * sizeof(CubicRec)));
     HIP_TRY(ensure(ctx->geoContours, (size_t)std::max<uint32_t>(contourTotal, 1) * sizeof(ContourV)));
     if (n > 0)
-        emit_kernel<<<(n + 127) / 128, 128, 0, ctx->stream>>>(draws, n, verbs, points, tessOffsets,
+        emit_kernel<<<(n + 31) / 32, 32, 0, ctx->stream>>>(draws, n, verbs, points, tessOffsets,
                                                               contourOffsets, recordOffsets, firstLocation,
                                                               P<CubicRec>(ctx->geoRecords),
                                                               P<ContourV>(ctx->geoContours), scal + 13);
