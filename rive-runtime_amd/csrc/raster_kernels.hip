@@ -2355,8 +2355,9 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                         }
                         else
                         {
+                            // |border| < 2^23: it is the low 24 bits, sign-extended
                             d = (v + (1 << 23)) >> 24;
-                            b = v - (int32_t)((uint32_t)d << 24);
+                            b = (int32_t)((uint32_t)v << 8) >> 8;
                         }
                         const int32_t left = sLeft[a][t >> kTileLog2];
                         const int32_t w = C + left + row_scan16(d);
