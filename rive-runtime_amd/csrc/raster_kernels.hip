@@ -1057,20 +1057,24 @@ __global__ __launch_bounds__(256) void bin_scatter_draw_kernel(const Prim* __res
         }
         return;
     }
-    for (uint32_t i = threadIdx.x; i < nCells; i += blockDim.x)
-        sCur[i] = 0;
-    __syncthreads();
     if (pathDraws[d.pathID] == 1u)
     {
+        // the cursors start at the cells' bin offsets (read once, coalesced)
+        for (uint32_t i = threadIdx.x; i < nCells; i += blockDim.x)
+            sCur[i] = gOffset[i];
+        __syncthreads();
         for (uint32_t k = threadIdx.x; k < d.forwardCount; k += blockDim.x)
         {
             const uint32_t g = d.primBase + k;
             const Prim p = prims[g];
             if (!(p.flags & kPrimEmpty))
-                for_prim_cells(p, c, [&](uint32_t cell) { binned[gOffset[cell] + atomicAdd(&sCur[cell], 1u)] = g; });
+                for_prim_cells(p, c, [&](uint32_t cell) { binned[atomicAdd(&sCur[cell], 1u)] = g; });
         }
         return;
     }
+    for (uint32_t i = threadIdx.x; i < nCells; i += blockDim.x)
+        sCur[i] = 0;
+    __syncthreads();
     for (uint32_t k = threadIdx.x; k < d.forwardCount; k += blockDim.x)
     {
         const Prim p = prims[d.primBase + k];
