@@ -97,3 +97,48 @@ def test_binning_limits_match_oracle(hip_ctx, name):
         assert st["prim_count"] > 2 * 32768
     d = np.abs(out.astype(np.int32) - ref.astype(np.int32)).max(axis=2)
     assert d.max() <= 8 and (d > 2).mean() <= 1e-3, f"max {d.max()}, frac>2 {(d > 2).mean()}"
+
+
+def _desc(stats, target):
+    d = rivehip.FlushDesc()
+    d.target = target
+    d.mode = rivehip.FLUSH_GEOMETRY
+    d.load_action = stats.load_action
+    d.clear_color = stats.clear_color
+    d.dither = stats.dither
+    d.path_count = stats.path_count
+    d.draw_count = stats.draw_count
+    d.grad_span_count = stats.grad_span_count
+    d.grad_simple_rows = stats.grad_simple_rows
+    d.grad_complex_rows = stats.grad_complex_rows
+    d.first_path_tess_location = stats.first_path_tess_location
+    return d
+
+
+def test_draws_sharing_a_path_bin_together(hip_ctx):
+    """Two draw records naming one path bin into that path's grid (the
+    shared-path branch of the scatter: one count in LDS, a global reservation
+    per cell): the frame equals, bit for bit, one draw of a path holding both
+    contours."""
+    print("[binning-limits] shared path", flush=True)
+    ra, rb = rect(3, 4, 40, 30), rect(20, 12, 61, 45)
+    one = StreamBuilder(64, 48, dither=1)
+    one.draw(one.path(ra + rb), color=0xc0ff4000)
+    two = StreamBuilder(64, 48, dither=1)
+    two.draw(two.path(ra), color=0xc0ff4000)
+    two.draw(two.path(rb), color=0x8000a0ff)
+    t = hip_ctx.target(64, 48)
+    try:
+        hip_ctx.render_stream(one.build(), t)
+        ref = hip_ctx.read_pixels(t, 64, 48)
+        hip_ctx.render_stream(two.build(), t)
+        stats = hip_ctx.last_flush_stats()
+        draws = hip_ctx.readback(3).view(rivehip.DRAW).copy()
+        assert len(draws) == 2 and draws[0]["path_id"] != draws[1]["path_id"]
+        draws[1]["path_id"] = draws[0]["path_id"]
+        hip_ctx.upload("draw", draws)
+        hip_ctx.flush(_desc(stats, t))
+        out = hip_ctx.read_pixels(t, 64, 48)
+    finally:
+        hip_ctx.free_target(t)
+    assert np.array_equal(out, ref)
