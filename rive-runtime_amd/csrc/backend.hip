@@ -517,11 +517,13 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     rc = device_scan(ctx, P<uint32_t>(ctx->cellCount), totalCells, P<uint32_t>(ctx->cellOffset), scal + 2);
     if (rc)
         return rc;
+    HIP_TRY(ensure(ctx->cellPacked, (size_t)(totalCells + 1) * sizeof(uint2)));
     {
         int maxRows = maxTilesY;
-        const uint64_t threads = (uint64_t)(pathCount + 1) * (uint64_t)maxRows * 64u;
+        const uint64_t threads = std::max<uint64_t>((uint64_t)(pathCount + 1) * (uint64_t)maxRows * 64u, 1);
         backdrop_prefix_kernel<<<(uint32_t)((threads + 255) / 256), 256, 0, ctx->stream>>>(
-            P<PathCells>(ctx->cells), pathCount + 1, maxRows, P<int32_t>(ctx->cellBackdrop));
+            P<PathCells>(ctx->cells), pathCount + 1, maxRows, P<int32_t>(ctx->cellBackdrop),
+            P<uint32_t>(ctx->cellOffset), totalCells, P<uint2>(ctx->cellPacked));
     }
     uint32_t totalPairs = 0;
     rc = read_scalar(ctx, scal + 2, &totalPairs);
@@ -535,9 +537,6 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
         bin_scatter_draw_kernel<<<drawCount, 256, 0, ctx->stream>>>(
             P<Prim>(ctx->prims), drawTess, P<PathCells>(ctx->cells), P<uint32_t>(ctx->cellOffset),
             P<uint32_t>(ctx->cellCursor), P<uint32_t>(ctx->pathDraws), P<uint32_t>(ctx->binned));
-    HIP_TRY(ensure(ctx->cellPacked, (size_t)(totalCells + 1) * sizeof(uint2)));
-    cell_pack_kernel<<<(totalCells + 1 + 255) / 256, 256, 0, ctx->stream>>>(
-        P<uint32_t>(ctx->cellOffset), P<int32_t>(ctx->cellBackdrop), totalCells, P<uint2>(ctx->cellPacked));
     HIP_TRY(ensure(ctx->pathRaster, (size_t)(pathCount + 1) * sizeof(PathRaster)));
     HIP_TRY(ensure(ctx->pathPaint, (size_t)(pathCount + 1) * sizeof(PathPaint)));
     path_raster_kernel<<<(pathCount + 1 + 255) / 256, 256, 0, ctx->stream>>>(

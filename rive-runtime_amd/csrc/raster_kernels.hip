@@ -1098,21 +1098,20 @@ __global__ __launch_bounds__(256) void bin_scatter_draw_kernel(const Prim* __res
 // Each cell's first bin and corner winding side by side, so the raster reads
 // one 8-byte entry (and the next entry's offset for the count) per
 // (path, tile) instead of two arrays.
-__global__ void cell_pack_kernel(const uint32_t* __restrict__ cellOffset, const int32_t* __restrict__ cellBackdrop,
-                                 uint32_t totalCells, uint2* __restrict__ packed)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > totalCells)
-        return;
-    packed[i] = make_uint2(cellOffset[i], i < totalCells ? (uint32_t)cellBackdrop[i] : 0u);
-}
-
 // Prefix-sum of corner crossings along each row of each path's grid: one
-// wave64 per (path, row), 64 cells per step with a carried running sum.
+// wave64 per (path, row), 64 cells per step with a carried running sum.  Each
+// cell's result goes out with its bin offset as the raster's packed
+// (offset, corner winding) entry; entry totalCells is the end sentinel.
 __global__ __launch_bounds__(256) void backdrop_prefix_kernel(const PathCells* __restrict__ cells,
                                                               uint32_t pathCount, int maxRows,
-                                                              int32_t* __restrict__ cellBackdrop)
+                                                              const int32_t* __restrict__ cellBackdrop,
+                                                              const uint32_t* __restrict__ cellOffset,
+                                                              uint32_t totalCells, uint2* __restrict__ packed)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        packed[totalCells] = make_uint2(cellOffset[totalCells], 0u);
+    if (maxRows <= 0)
+        return;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     const uint32_t path = wave / (uint32_t)maxRows;
@@ -1122,7 +1121,8 @@ __global__ __launch_bounds__(256) void backdrop_prefix_kernel(const PathCells* _
     const PathCells c = cells[path];
     if (row >= c.ty1 - c.ty0)
         return;
-    int32_t* r = cellBackdrop + c.base + (uint32_t)row * c.width;
+    const uint32_t rowBase = c.base + (uint32_t)row * c.width;
+    const int32_t* r = cellBackdrop + rowBase;
     int32_t carry = 0;
     for (uint32_t x0 = 0; x0 < c.width; x0 += 64)
     {
@@ -1136,7 +1136,7 @@ __global__ __launch_bounds__(256) void backdrop_prefix_kernel(const PathCells* _
                 v += t;
         }
         if (x < c.width)
-            r[x] = carry + v;
+            packed[rowBase + x] = make_uint2(cellOffset[rowBase + x], (uint32_t)(carry + v));
         carry += __shfl(v, 63);
     }
 }
