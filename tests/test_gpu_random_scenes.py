@@ -89,3 +89,46 @@ def test_random_scene_matches_oracle(hip_ctx, seed):
         f.close()
     d = np.abs(out.astype(np.int32) - ref.astype(np.int32)).max(axis=2)
     assert d.max() <= 8 and (d > 2).mean() <= 1e-3, f"seed {seed}: max {d.max()}, frac>2 {(d > 2).mean()}"
+
+
+def dense_scene(seed):
+    """Many overlapping long paths on a larger canvas: every tile lists more
+    paths than one batch holds, so coverage flows through several batches and
+    listing windows per tile."""
+    rng = np.random.default_rng(1000 + seed)
+    w, h = 160, 144
+    b = StreamBuilder(w, h, clear_color=0xff202020, dither=1)
+    for _ in range(120):
+        cmds = [(0, float(rng.uniform(0, w)), float(rng.uniform(0, h)))]
+        for _ in range(int(rng.integers(10, 40))):
+            if rng.random() < .5:
+                cmds.append((1, float(rng.uniform(0, w)), float(rng.uniform(0, h))))
+            else:
+                cmds.append((4, *(float(v) for v in rng.uniform(0, [w, h, w, h, w, h]))))
+        stroke = rng.random() < .25
+        p = b.path(cmds + ([] if stroke else [(5,)]), fill_rule=[NONZERO, EVENODD][int(rng.integers(0, 2))])
+        color = int(rng.integers(0, 1 << 24)) | (int(rng.integers(128, 256)) << 24)
+        if stroke:
+            b.draw(p, color=color, stroke=float(rng.uniform(1, 6)), join=ROUND, cap=ROUND_CAP)
+        else:
+            b.draw(p, color=color)
+    return b
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_dense_random_scene_matches_oracle(hip_ctx, seed):
+    print(f"[random-scene] dense seed {seed}", flush=True)
+    b = dense_scene(seed)
+    s = b.build()
+    w, h = b.width, b.height
+    f = ob.OracleFlush(s)
+    t = hip_ctx.target(w, h)
+    try:
+        ref, _ = f.render()
+        hip_ctx.render_stream(s, t)
+        out = hip_ctx.read_pixels(t, w, h)
+    finally:
+        hip_ctx.free_target(t)
+        f.close()
+    d = np.abs(out.astype(np.int32) - ref.astype(np.int32)).max(axis=2)
+    assert d.max() <= 8 and (d > 2).mean() <= 1e-3, f"dense seed {seed}: max {d.max()}, frac>2 {(d > 2).mean()}"
