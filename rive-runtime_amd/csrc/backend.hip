@@ -73,6 +73,9 @@ struct MappedBuffer
 };
 
 constexpr int kEvents = 7;
+// Bounds on one flush's records (checked before any size arithmetic).
+constexpr uint32_t kMaxFlushPaths = 1u << 24;
+constexpr uint32_t kMaxFlushTexels = 1u << 31;
 } // namespace
 
 struct rive_hip_target
@@ -391,6 +394,19 @@ int read_scalar(rive_hip_context* ctx, const uint32_t* devSlot, uint32_t* value)
     return RIVE_HIP_OK;
 }
 
+// A device total (scan or layout) as read back; totals that do not fit 32
+// bits saturate on the device and fail the flush here, before any kernel
+// indexes with them.
+int read_total(rive_hip_context* ctx, const uint32_t* devSlot, uint32_t* value, const char* what)
+{
+    int rc = read_scalar(ctx, devSlot, value);
+    if (rc)
+        return rc;
+    if (*value == kScanSaturated)
+        return fail(RIVE_HIP_ERROR_UNSUPPORTED, std::string("flush too large: ") + what + " exceed 2^32-1");
+    return RIVE_HIP_OK;
+}
+
 // Raster half of the pipeline, shared by both flush modes.  Inputs: the
 // tessellation texture, contours, paths, paints and per-draw ranges.
 int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const SpanV* spans,
@@ -465,7 +481,7 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
         if (rc)
             return rc;
         uint32_t used = 0;
-        rc = read_scalar(ctx, scal + 0, &used);
+        rc = read_total(ctx, scal + 0, &used, "tessellation texels");
         if (rc)
             return rc;
         if (used > 0)
@@ -492,15 +508,21 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     if (totalPrims > 0)
         expand_kernel<<<(totalPrims + 255) / 256, 256, 0, ctx->stream>>>(
             P<TessTexel>(ctx->tess), (int)texels, P<VtxPair>(ctx->verts), contours, contourCount, paths,
-            drawTess, drawCount, totalPrims, P<Prim>(ctx->prims), P<PathCells>(ctx->cells));
+            drawTess, drawCount, totalPrims, P<Prim>(ctx->prims), P<PathCells>(ctx->cells), scal + 5);
     cells_layout_kernel<<<1, 1024, 0, ctx->stream>>>(P<PathCells>(ctx->cells), pathCount + 1,
                                                      P<uint32_t>(ctx->pathCanvas), P<CanvasDev>(ctx->canvasTable),
                                                      scal + 1);
     HIP_TRY(hipGetLastError());
-    uint32_t totalCells = 0;
-    int rc = read_scalar(ctx, scal + 1, &totalCells);
-    if (rc)
-        return rc;
+    // Cell total and the count of texels naming no ContourData record.
+    HIP_TRY(hipMemcpyAsync(ctx->hostScalars, scal + 1, 20, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const uint32_t totalCells = ctx->hostScalars[0];
+    if (ctx->hostScalars[4] != 0)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT,
+                    "tessellated vertices name contour IDs outside 1..contour_count");
+    if (totalCells == 0xffffffffu)
+        return fail(RIVE_HIP_ERROR_UNSUPPORTED, "flush too large: path cell grids exceed 2^32-1 cells");
+    int rc = RIVE_HIP_OK;
     HIP_TRY(ensure(ctx->cellCount, (size_t)(totalCells + 1) * 4));
     HIP_TRY(ensure(ctx->cellOffset, (size_t)(totalCells + 1) * 4));
     HIP_TRY(ensure(ctx->cellCursor, (size_t)(totalCells + 1) * 4));
@@ -526,7 +548,7 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
             P<uint32_t>(ctx->cellOffset), totalCells, P<uint2>(ctx->cellPacked));
     }
     uint32_t totalPairs = 0;
-    rc = read_scalar(ctx, scal + 2, &totalPairs);
+    rc = read_total(ctx, scal + 2, &totalPairs, "binned (primitive, tile) pairs");
     if (rc)
         return rc;
     ctx->stats.bin_entries = totalPairs;
@@ -565,7 +587,7 @@ __global__ void draw_tess_kernel(const rive_hip_draw* __restrict__ draws, uint32
                                  uint32_t* __restrict__ totalPrims, uint32_t* __restrict__ unsupported)
 {
     __shared__ uint32_t buf[1024];
-    __shared__ uint32_t carry;
+    __shared__ uint64_t carry; // 64-bit: a total beyond 2^32-1 saturates below
     if (threadIdx.x == 0)
         carry = 0;
     __syncthreads();
@@ -635,7 +657,7 @@ __global__ void draw_tess_kernel(const rive_hip_draw* __restrict__ draws, uint32
         }
         if (i < n)
         {
-            t.primBase = carry + buf[threadIdx.x] - cnt;
+            t.primBase = (uint32_t)carry + buf[threadIdx.x] - cnt;
             out[i] = t;
         }
         __syncthreads();
@@ -644,7 +666,7 @@ __global__ void draw_tess_kernel(const rive_hip_draw* __restrict__ draws, uint32
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        *totalPrims = carry;
+        *totalPrims = (uint32_t)min(carry, (uint64_t)kScanSaturated);
 }
 } // namespace
 
@@ -695,7 +717,13 @@ extern "C" int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* 
         if (ctx->bufs[k].mapped)
             return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "flush with a mapped buffer");
     auto need = [&](int kind, size_t bytes) { return ctx->bufs[kind].size >= bytes; };
-    const uint32_t pathRecords = d->path_count + 1;
+    // Record counts are bounded before any size arithmetic: path IDs index
+    // per-path device arrays sized (path_count + 1) records.
+    if (d->path_count > kMaxFlushPaths)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "path_count above 2^24");
+    if (d->mode == RIVE_HIP_FLUSH_SPANS && d->tess_vertex_count > kMaxFlushTexels)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "tess_vertex_count above 2^31");
+    const size_t pathRecords = (size_t)d->path_count + 1;
     if (!need(RIVE_HIP_BUFFER_PATH, pathRecords * sizeof(rive_hip_path_data)) ||
         !need(RIVE_HIP_BUFFER_PAINT, pathRecords * sizeof(rive_hip_paint_data)) ||
         !need(RIVE_HIP_BUFFER_PAINT_AUX, pathRecords * sizeof(rive_hip_paint_aux)) ||
@@ -716,6 +744,10 @@ extern "C" int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* 
         if (!need(RIVE_HIP_BUFFER_TESS_SPAN, (size_t)d->tess_span_count * sizeof(rive_hip_tess_span)) ||
             !need(RIVE_HIP_BUFFER_CONTOUR, (size_t)d->contour_count * sizeof(rive_hip_contour)))
             return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "span/contour buffers too small");
+        // Draws with texels index ContourData records (every tessellated
+        // vertex names its contour), so a SPANS flush needs them.
+        if (d->draw_count > 0 && d->tess_span_count > 0 && d->contour_count == 0)
+            return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "SPANS flush with spans and draws but no contours");
         spans = reinterpret_cast<const SpanV*>(ctx->bufs[RIVE_HIP_BUFFER_TESS_SPAN].dev);
         spanCount = d->tess_span_count;
         contours = reinterpret_cast<const ContourV*>(ctx->bufs[RIVE_HIP_BUFFER_CONTOUR].dev);
@@ -771,6 +803,8 @@ extern "C" int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* 
         return fail(RIVE_HIP_ERROR_UNSUPPORTED,
                     "draw record out of range, or contour directions the raster-ordering "
                     "pipeline does not use");
+    if (totalPrims > kMaxFlushTexels)
+        return fail(RIVE_HIP_ERROR_UNSUPPORTED, "flush too large: draws cover more than 2^31 texels");
     ctx->lastCanvases = canvases;
     ctx->lastSpanCount = spanCount;
     ctx->lastDrawCount = d->draw_count;

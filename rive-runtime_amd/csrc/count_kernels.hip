@@ -1109,6 +1109,8 @@ int geometry_counting(rive_hip_context* ctx, const rive_hip_flush_desc* d, const
     const uint32_t firstLocation = d->first_path_tess_location ? d->first_path_tess_location : kPatchSpan;
     if (contourTotal > 0xffffu)
         return fail(RIVE_HIP_ERROR_UNSUPPORTED, "more than 65535 contours in one flush");
+    if (tessTotal > kMaxFlushTexels || recordTotal == kScanSaturated)
+        return fail(RIVE_HIP_ERROR_UNSUPPORTED, "flush too large: more than 2^31 tessellation vertices");
     HIP_TRY(ensure(ctx->geoRecords, (size_t)std::max<uint32_t>(recordTotal, 1) * sizeof(CubicRec)));
     HIP_TRY(ensure(ctx->geoContours, (size_t)std::max<uint32_t>(contourTotal, 1) * sizeof(ContourV)));
     if (n > 0)
@@ -1134,7 +1136,7 @@ int geometry_counting(rive_hip_context* ctx, const rive_hip_flush_desc* d, const
     if (rc)
         return rc;
     uint32_t spanTotal = 0;
-    rc = read_scalar(ctx, scal + 11, &spanTotal);
+    rc = read_total(ctx, scal + 11, &spanTotal, "tessellation spans");
     if (rc)
         return rc;
     const uint32_t allSpans = spanTotal + 2; // + padding spans

@@ -596,7 +596,7 @@ __global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels, con
                               const ContourV* __restrict__ contours, uint32_t contourCount,
                               const PathDataV* __restrict__ paths, const DrawTess* __restrict__ draws,
                               uint32_t drawCount, uint32_t totalPrims, Prim* __restrict__ prims,
-                              PathCells* __restrict__ cells)
+                              PathCells* __restrict__ cells, uint32_t* __restrict__ badContours)
 {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t di = find_draw_block(draws, drawCount, g);
@@ -609,7 +609,12 @@ __global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels, con
     out.pad[0] = out.pad[1] = 0;
     const TessTexel ti = tex[i];
     const uint32_t cfi = ti.w;
-    const uint32_t contourID = min(max(cfi & CONTOUR_ID_MASK, 1u), max(contourCount, 1u));
+    // A texel naming no ContourData record (ID 0 or beyond contourCount) is
+    // counted and the flush fails; the clamp only keeps this read in bounds.
+    const uint32_t rawContourID = cfi & CONTOUR_ID_MASK;
+    if (rawContourID == 0u || rawContourID > contourCount)
+        atomicAdd(badContours, 1u);
+    const uint32_t contourID = min(max(rawContourID, 1u), max(contourCount, 1u));
     const ContourV cd = contours[contourID - 1];
     const PathDataV pd = paths[d.pathID];
     // next(i): the next vertex of the contour, wrapping closed contours to
@@ -755,7 +760,7 @@ __global__ void cells_layout_kernel(PathCells* __restrict__ cells, uint32_t n,
                                     const CanvasDev* __restrict__ canvases, uint32_t* __restrict__ totalCells)
 {
     __shared__ uint32_t partial[1024];
-    __shared__ uint32_t carry;
+    __shared__ uint64_t carry; // 64-bit: a total beyond 2^32-1 saturates below
     if (threadIdx.x == 0)
         carry = 0;
     __syncthreads();
@@ -793,7 +798,7 @@ __global__ void cells_layout_kernel(PathCells* __restrict__ cells, uint32_t n,
         }
         if (i < n)
         {
-            c.base = carry + partial[threadIdx.x] - area;
+            c.base = (uint32_t)carry + partial[threadIdx.x] - area;
             c.width = (uint32_t)(c.tx1 - c.tx0);
             cells[i] = c;
         }
@@ -803,7 +808,7 @@ __global__ void cells_layout_kernel(PathCells* __restrict__ cells, uint32_t n,
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        *totalCells = carry;
+        *totalCells = (uint32_t)min(carry, (uint64_t)0xffffffffu);
 }
 
 // ---------------------------------------------------------------------------
@@ -2474,13 +2479,17 @@ __global__ void path_raster_kernel(const PathDataV* __restrict__ paths, const ui
 constexpr int kScanItems = 16;
 constexpr int kScanBlock = 256;
 constexpr int kScanTile = kScanItems * kScanBlock;
+// A scan total that does not fit 32 bits reads as this value.
+constexpr uint32_t kScanSaturated = 0xffffffffu;
 
 __global__ void scan_reduce_kernel(const uint32_t* __restrict__ in, uint32_t n,
                                    uint32_t* __restrict__ partials)
 {
-    __shared__ uint32_t red[kScanBlock];
+    // Sums in 64 bits; a block sum beyond 2^32-1 saturates, so the total
+    // saturates too and the host rejects the flush (kScanSaturated).
+    __shared__ uint64_t red[kScanBlock];
     uint32_t base = blockIdx.x * kScanTile;
-    uint32_t s = 0;
+    uint64_t s = 0;
     for (int k = 0; k < kScanItems; ++k)
     {
         uint32_t i = base + k * kScanBlock + threadIdx.x;
@@ -2496,14 +2505,14 @@ __global__ void scan_reduce_kernel(const uint32_t* __restrict__ in, uint32_t n,
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        partials[blockIdx.x] = red[0];
+        partials[blockIdx.x] = (uint32_t)min(red[0], (uint64_t)kScanSaturated);
 }
 
 __global__ void scan_partials_kernel(uint32_t* __restrict__ partials, uint32_t n,
                                      uint32_t* __restrict__ total)
 {
     __shared__ uint32_t buf[1024];
-    __shared__ uint32_t carry;
+    __shared__ uint64_t carry;
     if (threadIdx.x == 0)
         carry = 0;
     __syncthreads();
@@ -2511,6 +2520,8 @@ __global__ void scan_partials_kernel(uint32_t* __restrict__ partials, uint32_t n
     {
         uint32_t i = base + threadIdx.x;
         uint32_t v = i < n ? partials[i] : 0;
+        if (v == kScanSaturated)
+            carry = kScanSaturated; // benign race: every writer stores the same value
         buf[threadIdx.x] = v;
         __syncthreads();
         for (uint32_t off = 1; off < blockDim.x; off <<= 1)
@@ -2521,14 +2532,14 @@ __global__ void scan_partials_kernel(uint32_t* __restrict__ partials, uint32_t n
             __syncthreads();
         }
         if (i < n)
-            partials[i] = carry + buf[threadIdx.x] - v;
+            partials[i] = (uint32_t)carry + buf[threadIdx.x] - v;
         __syncthreads();
         if (threadIdx.x == blockDim.x - 1)
             carry += buf[threadIdx.x];
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        *total = carry;
+        *total = (uint32_t)min(carry, (uint64_t)kScanSaturated);
 }
 
 __global__ void scan_apply_kernel(const uint32_t* __restrict__ in, uint32_t n,

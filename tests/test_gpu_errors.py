@@ -115,3 +115,56 @@ def test_null_arguments_are_rejected(hip_ctx):
     assert L.rive_hip_flush(None, None) != 0
     assert L.rive_hip_sync(None) != 0
     assert L.rive_hip_read_pixels(hip_ctx.h, None, None) != 0
+
+
+def test_oversized_and_inconsistent_records_are_rejected(hip_ctx):
+    """Counts that would overflow the per-path arrays are refused on the host;
+    a SPANS flush whose draws have texels but no ContourData, or whose spans
+    name contour IDs beyond contour_count, fails with an error instead of
+    reading records that do not exist."""
+    import oracle_binding as ob
+
+    print("[errors] oversized path_count, missing and out-of-range contours", flush=True)
+    b = scene()
+    s = b.build()
+    f = ob.OracleFlush(s)
+    t = hip_ctx.target(b.width, b.height)
+    try:
+        ref, _ = f.render()
+        hip_ctx.render_stream(s, t)
+        stats = hip_ctx.last_flush_stats()
+        d = desc_from(stats, t)
+        d.path_count = (1 << 24) + 1
+        expect_error(hip_ctx, d, "above 2^24")
+        d.path_count = 0xFFFFFFFF
+        expect_error(hip_ctx, d, "above 2^24")
+
+        f.flush_spans_on(hip_ctx, t, clear_color=b.clear_color, dither=b.dither)  # a valid SPANS flush
+        spans_desc = rivehip.FlushDesc()
+        spans_desc.target = t
+        spans_desc.mode = rivehip.FLUSH_SPANS
+        spans_desc.dither = b.dither
+        spans_desc.clear_color = b.clear_color
+        spans_desc.path_count = f.path_count
+        spans_desc.contour_count = 0
+        spans_desc.tess_span_count = f.span_count
+        spans_desc.tess_vertex_count = f.tess_vertex_count
+        spans_desc.draw_count = f.draw_count
+        spans_desc.grad_span_count = len(f.get(5))
+        expect_error(hip_ctx, spans_desc, "no contours")
+
+        spans = f.get(0).copy()
+        live = (spans["contour_id_with_flags"] & 0xFFFF) != 0
+        spans["contour_id_with_flags"][live] = (spans["contour_id_with_flags"][live] & ~np.uint32(0xFFFF)) | (
+            f.contour_count + 5)
+        hip_ctx.upload("tess_span", spans)
+        spans_desc.contour_count = f.contour_count
+        expect_error(hip_ctx, spans_desc, "contour IDs outside")
+
+        f.flush_spans_on(hip_ctx, t, clear_color=b.clear_color, dither=b.dither)
+        out = hip_ctx.read_pixels(t, b.width, b.height)
+        d = np.abs(out.astype(np.int32) - ref.astype(np.int32)).max()
+        assert d <= 8
+    finally:
+        hip_ctx.free_target(t)
+        f.close()
