@@ -259,6 +259,20 @@ typedef struct rive_hip_canvas
  * enqueued on the context stream. */
 int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* desc);
 
+/* rive_hip_flush that allocates no memory: for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments, records and
+ * results as rive_hip_flush; the caller still provides the inputs (the
+ * context's mapped buffers) and the target.  Every intermediate buffer is
+ * carved from one 256 MiB scratch array per GPU, held until the process
+ * ends, instead of the context's grow-only buffers; a flush that needs more
+ * fails with RIVE_HIP_ERROR_OUT_OF_MEMORY, one that needs more than one
+ * canvas or than 16 images with RIVE_HIP_ERROR_UNSUPPORTED.  All calls on a
+ * GPU share its array, so no-alloc flushes that overlap on one GPU (several
+ * contexts, several threads) race.  rive_hip_debug_readback of intermediates
+ * is unavailable after a no-alloc flush. */
+int rive_hip_flush_noalloc(rive_hip_context* ctx, const rive_hip_flush_desc* desc);
+
 /* Waits for the context stream (the reference's postFlush/fence wait). */
 int rive_hip_sync(rive_hip_context* ctx);
 

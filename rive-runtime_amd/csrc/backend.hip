@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -41,28 +42,19 @@ struct DevBuf
 {
     void* ptr = nullptr;
     size_t cap = 0;
+    bool carved = false; // carved from the no-alloc scratch by the current flush
 };
 
-hipError_t ensure(DevBuf& b, size_t bytes)
-{
-    if (bytes <= b.cap)
-        return hipSuccess;
-    if (b.ptr != nullptr)
-    {
-        hipError_t e = hipFree(b.ptr);
-        if (e != hipSuccess)
-            return e;
-        b.ptr = nullptr;
-        b.cap = 0;
-    }
-    size_t cap = std::max<size_t>(bytes + bytes / 4, 256);
-    hipError_t e = hipMalloc(&b.ptr, cap);
-    if (e == hipSuccess)
-        b.cap = cap;
-    return e;
-}
-
 template <typename T> T* P(DevBuf& b) { return reinterpret_cast<T*>(b.ptr); }
+
+// Bump allocator over the per-device no-alloc scratch array, active only
+// inside rive_hip_flush_noalloc.
+struct Arena
+{
+    bool active = false;
+    unsigned char* base = nullptr;
+    size_t used = 0, cap = 0;
+};
 
 struct MappedBuffer
 {
@@ -76,6 +68,8 @@ constexpr int kEvents = 7;
 // Bounds on one flush's records (checked before any size arithmetic).
 constexpr uint32_t kMaxFlushPaths = 1u << 24;
 constexpr uint32_t kMaxFlushTexels = 1u << 31;
+// Image table entries staged at context creation (what a no-alloc flush may use).
+constexpr uint32_t kNoAllocImages = 16;
 } // namespace
 
 struct rive_hip_target
@@ -111,7 +105,118 @@ struct rive_hip_context
     uint32_t lastSpanCount = 0, lastContourCount = 0, lastTexels = 0, lastDrawCount = 0;
     uint32_t lastGradRows = 0;
     rive_hip_flush_stats stats = {};
+    std::vector<rive_hip_canvas> flushCanvases; // the current flush's canvas table (capacity kept)
+    // No-alloc flushes: this device's scratch array (device address) and the
+    // context's own scratch buffers, set aside while a flush carves its own.
+    unsigned char* scratchPtr = nullptr;
+    Arena arena;
+    bool lastFlushNoAlloc = false;
 };
+
+namespace
+{
+// The scratch buffers of a context (every DevBuf member), in one list.
+constexpr int kDevBufCount = 29;
+void devbufs(rive_hip_context* c, DevBuf** out)
+{
+    DevBuf* all[kDevBufCount] = {&c->spanTexelCounts, &c->spanTexelOffsets, &c->scanPartials, &c->scalars,
+                                 &c->tess, &c->prims, &c->cells, &c->cellCount, &c->cellOffset,
+                                 &c->cellCursor, &c->cellBackdrop, &c->binned, &c->pathRaster,
+                                 &c->ramps, &c->drawTess, &c->geoCounts, &c->geoRecords,
+                                 &c->geoContours, &c->geoSpans, &c->geoSpanCounts, &c->geoSpanOffsets,
+                                 &c->verts, &c->imageTable, &c->pathImage, &c->canvasTable,
+                                 &c->pathCanvas, &c->cellPacked, &c->pathDraws, &c->pathPaint};
+    for (int i = 0; i < kDevBufCount; ++i)
+        out[i] = all[i];
+}
+
+// Grows a scratch buffer to at least `bytes` (grow-only, 1.25x headroom).
+// Inside a no-alloc flush it carves the buffer from the device's scratch
+// array instead and never calls the allocator.
+int ensure(rive_hip_context* ctx, DevBuf& b, size_t bytes)
+{
+    Arena& a = ctx->arena;
+    if (a.active)
+    {
+        if (b.carved && bytes <= b.cap)
+            return RIVE_HIP_OK;
+        const size_t need = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+        if (need > a.cap - a.used)
+            return fail(RIVE_HIP_ERROR_OUT_OF_MEMORY,
+                        "rive_hip_flush_noalloc: the flush needs more than the " +
+                            std::to_string(a.cap >> 20) + " MiB no-alloc scratch (" +
+                            std::to_string((a.used + need) >> 20) + " MiB so far)");
+        b.ptr = a.base + a.used;
+        b.cap = need;
+        b.carved = true;
+        a.used += need;
+        return RIVE_HIP_OK;
+    }
+    if (bytes <= b.cap)
+        return RIVE_HIP_OK;
+    if (b.ptr != nullptr)
+    {
+        hipError_t e = hipFree(b.ptr);
+        if (e != hipSuccess)
+            return fail(RIVE_HIP_ERROR_DEVICE, std::string("hipFree: ") + hipGetErrorString(e));
+        b.ptr = nullptr;
+        b.cap = 0;
+    }
+    size_t cap = std::max<size_t>(bytes + bytes / 4, 256);
+    hipError_t e = hipMalloc(&b.ptr, cap);
+    if (e != hipSuccess)
+    {
+        b.ptr = nullptr;
+        return fail(RIVE_HIP_ERROR_OUT_OF_MEMORY,
+                    "scratch allocation of " + std::to_string(cap) + " bytes failed: " + hipGetErrorString(e));
+    }
+    b.cap = cap;
+    return RIVE_HIP_OK;
+}
+
+#define RIVE_TRY(expr)                                                                       \
+    do                                                                                       \
+    {                                                                                        \
+        int _rc = (expr);                                                                    \
+        if (_rc != RIVE_HIP_OK)                                                              \
+            return _rc;                                                                      \
+    } while (0)
+} // namespace
+
+namespace
+{
+// The no-alloc flush's scratch: one array per GPU, held until the process
+// ends.  Host code never passes `scratch` itself to a kernel or a copy (on
+// the host the name is a stand-in, not a device address); it uses the
+// address hipGetSymbolAddress returns on each device, kept in g_scratch.
+constexpr size_t kNoAllocScratchBytes = (size_t)256 << 20;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+unsigned char* g_scratch[kMaxDevices] = {};
+} // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace
+{
+// Looks up this device's scratch address once (the device must be current).
+hipError_t scratch_address(int device, unsigned char** out)
+{
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (device < 0 || device >= kMaxDevices)
+        return hipErrorInvalidDevice;
+    if (g_scratch[device] == nullptr)
+    {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        g_scratch[device] = static_cast<unsigned char*>(ptr);
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
+}
+} // namespace
 
 extern "C" {
 
@@ -132,6 +237,22 @@ int rive_hip_create(int device, rive_hip_context** out)
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess)
         e = hipHostMalloc((void**)&ctx->hostScalars, 64, hipHostMallocDefault);
+    // Staging a no-alloc flush may use without growing it: one canvas and
+    // kNoAllocImages images.
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&ctx->hostCanvases, sizeof(CanvasDev), hipHostMallocDefault);
+    if (e == hipSuccess)
+    {
+        ctx->hostCanvasesCap = 1;
+        e = hipHostMalloc((void**)&ctx->hostImages, sizeof(ImageDesc) * kNoAllocImages, hipHostMallocDefault);
+    }
+    if (e == hipSuccess)
+    {
+        ctx->hostImagesCap = kNoAllocImages;
+        ctx->flushCanvases.reserve(1);
+        ctx->lastCanvases.reserve(1);
+        e = scratch_address(device, &ctx->scratchPtr);
+    }
     for (int i = 0; i < kEvents && e == hipSuccess; ++i)
         e = hipEventCreate(&ctx->ev[i]);
     if (e != hipSuccess)
@@ -156,15 +277,10 @@ void rive_hip_destroy(rive_hip_context* ctx)
         if (b.host)
             (void)hipHostFree(b.host);
     }
-    DevBuf* all[] = {&ctx->spanTexelCounts, &ctx->spanTexelOffsets, &ctx->scanPartials, &ctx->scalars,
-                     &ctx->tess, &ctx->prims, &ctx->cells, &ctx->cellCount, &ctx->cellOffset,
-                     &ctx->cellCursor, &ctx->cellBackdrop, &ctx->binned, &ctx->pathRaster,
-                     &ctx->ramps, &ctx->drawTess, &ctx->geoCounts, &ctx->geoRecords,
-                     &ctx->geoContours, &ctx->geoSpans, &ctx->geoSpanCounts, &ctx->geoSpanOffsets,
-                     &ctx->verts, &ctx->imageTable, &ctx->pathImage, &ctx->canvasTable,
-                     &ctx->pathCanvas, &ctx->cellPacked, &ctx->pathDraws, &ctx->pathPaint};
+    DevBuf* all[kDevBufCount];
+    devbufs(ctx, all);
     for (DevBuf* b : all)
-        if (b->ptr)
+        if (b->ptr && !b->carved)
             (void)hipFree(b->ptr);
     if (ctx->hostScalars)
         (void)hipHostFree(ctx->hostScalars);
@@ -377,7 +493,7 @@ int device_scan(rive_hip_context* ctx, const uint32_t* in, uint32_t n, uint32_t*
                 uint32_t* totalSlot)
 {
     uint32_t blocks = std::max<uint32_t>((n + kScanTile - 1) / kScanTile, 1);
-    HIP_TRY(ensure(ctx->scanPartials, (size_t)blocks * 4));
+    RIVE_TRY(ensure(ctx, ctx->scanPartials, (size_t)blocks * 4));
     scan_reduce_kernel<<<blocks, kScanBlock, 0, ctx->stream>>>(in, n, P<uint32_t>(ctx->scanPartials));
     scan_partials_kernel<<<1, 1024, 0, ctx->stream>>>(P<uint32_t>(ctx->scanPartials), blocks, totalSlot);
     scan_apply_kernel<<<blocks, kScanBlock, 0, ctx->stream>>>(in, n, P<uint32_t>(ctx->scanPartials),
@@ -417,6 +533,8 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     const uint32_t canvasCount = (uint32_t)ctx->lastCanvases.size();
     if (ctx->hostCanvasesCap < canvasCount)
     {
+        if (ctx->arena.active)
+            return fail(RIVE_HIP_ERROR_UNSUPPORTED, "rive_hip_flush_noalloc: more canvases than the staged table holds");
         if (ctx->hostCanvases)
             HIP_TRY(hipHostFree(ctx->hostCanvases));
         ctx->hostCanvases = nullptr;
@@ -443,7 +561,7 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
         tileTotal += (uint32_t)(cd.tilesX * cd.tilesY);
         maxTilesY = std::max(maxTilesY, cd.tilesY);
     }
-    HIP_TRY(ensure(ctx->canvasTable, sizeof(CanvasDev) * std::max<uint32_t>(canvasCount, 1)));
+    RIVE_TRY(ensure(ctx, ctx->canvasTable, sizeof(CanvasDev) * std::max<uint32_t>(canvasCount, 1)));
     HIP_TRY(hipMemcpyAsync(ctx->canvasTable.ptr, ctx->hostCanvases, sizeof(CanvasDev) * canvasCount,
                            hipMemcpyHostToDevice, ctx->stream));
     uint32_t* scal = P<uint32_t>(ctx->scalars);
@@ -455,7 +573,7 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
 
     // ---- color ramps
     HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
-    HIP_TRY(ensure(ctx->ramps, (size_t)gradRows * kGradWidth * 4));
+    RIVE_TRY(ensure(ctx, ctx->ramps, (size_t)gradRows * kGradWidth * 4));
     if (d->grad_span_count > 0)
     {
         HIP_TRY(hipMemsetAsync(ctx->ramps.ptr, 0, (size_t)gradRows * kGradWidth * 4, ctx->stream));
@@ -468,12 +586,12 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
 
     // ---- tessellation: texel counts -> scan -> one thread per texel
     const uint32_t texels = tessRows * kTessTextureWidth;
-    HIP_TRY(ensure(ctx->tess, (size_t)texels * sizeof(TessTexel)));
+    RIVE_TRY(ensure(ctx, ctx->tess, (size_t)texels * sizeof(TessTexel)));
     HIP_TRY(hipMemsetAsync(ctx->tess.ptr, 0, (size_t)texels * sizeof(TessTexel), ctx->stream));
     if (spanCount > 0)
     {
-        HIP_TRY(ensure(ctx->spanTexelCounts, (size_t)spanCount * 4));
-        HIP_TRY(ensure(ctx->spanTexelOffsets, (size_t)(spanCount + 1) * 4));
+        RIVE_TRY(ensure(ctx, ctx->spanTexelCounts, (size_t)spanCount * 4));
+        RIVE_TRY(ensure(ctx, ctx->spanTexelOffsets, (size_t)(spanCount + 1) * 4));
         span_texel_count_kernel<<<(spanCount + 255) / 256, 256, 0, ctx->stream>>>(
             spans, spanCount, tessRows, P<uint32_t>(ctx->spanTexelCounts));
         int rc = device_scan(ctx, P<uint32_t>(ctx->spanTexelCounts), spanCount,
@@ -493,15 +611,15 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
 
     // ---- expansion + binning
-    HIP_TRY(ensure(ctx->pathCanvas, (size_t)(pathCount + 1) * 4));
+    RIVE_TRY(ensure(ctx, ctx->pathCanvas, (size_t)(pathCount + 1) * 4));
     HIP_TRY(hipMemsetAsync(ctx->pathCanvas.ptr, 0xff, (size_t)(pathCount + 1) * 4, ctx->stream));
     path_canvas_kernel<<<canvasCount, 256, 0, ctx->stream>>>(P<CanvasDev>(ctx->canvasTable),
                                                              P<uint32_t>(ctx->pathCanvas));
-    HIP_TRY(ensure(ctx->cells, (size_t)(pathCount + 1) * sizeof(PathCells)));
+    RIVE_TRY(ensure(ctx, ctx->cells, (size_t)(pathCount + 1) * sizeof(PathCells)));
     cells_init_kernel<<<(pathCount + 1 + 255) / 256, 256, 0, ctx->stream>>>(P<PathCells>(ctx->cells),
                                                                              pathCount + 1);
-    HIP_TRY(ensure(ctx->prims, (size_t)std::max<uint32_t>(totalPrims, 1) * sizeof(Prim)));
-    HIP_TRY(ensure(ctx->verts, (size_t)std::max<uint32_t>(totalPrims, 1) * sizeof(VtxPair)));
+    RIVE_TRY(ensure(ctx, ctx->prims, (size_t)std::max<uint32_t>(totalPrims, 1) * sizeof(Prim)));
+    RIVE_TRY(ensure(ctx, ctx->verts, (size_t)std::max<uint32_t>(totalPrims, 1) * sizeof(VtxPair)));
     if (totalPrims > 0)
         vertex_kernel<<<(totalPrims + 255) / 256, 256, 0, ctx->stream>>>(
             P<TessTexel>(ctx->tess), (int)texels, paths, drawTess, drawCount, totalPrims, P<VtxPair>(ctx->verts));
@@ -523,14 +641,14 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     if (totalCells == 0xffffffffu)
         return fail(RIVE_HIP_ERROR_UNSUPPORTED, "flush too large: path cell grids exceed 2^32-1 cells");
     int rc = RIVE_HIP_OK;
-    HIP_TRY(ensure(ctx->cellCount, (size_t)(totalCells + 1) * 4));
-    HIP_TRY(ensure(ctx->cellOffset, (size_t)(totalCells + 1) * 4));
-    HIP_TRY(ensure(ctx->cellCursor, (size_t)(totalCells + 1) * 4));
-    HIP_TRY(ensure(ctx->cellBackdrop, (size_t)(totalCells + 1) * 4));
+    RIVE_TRY(ensure(ctx, ctx->cellCount, (size_t)(totalCells + 1) * 4));
+    RIVE_TRY(ensure(ctx, ctx->cellOffset, (size_t)(totalCells + 1) * 4));
+    RIVE_TRY(ensure(ctx, ctx->cellCursor, (size_t)(totalCells + 1) * 4));
+    RIVE_TRY(ensure(ctx, ctx->cellBackdrop, (size_t)(totalCells + 1) * 4));
     HIP_TRY(hipMemsetAsync(ctx->cellCount.ptr, 0, (size_t)(totalCells + 1) * 4, ctx->stream));
     HIP_TRY(hipMemsetAsync(ctx->cellCursor.ptr, 0, (size_t)(totalCells + 1) * 4, ctx->stream));
     HIP_TRY(hipMemsetAsync(ctx->cellBackdrop.ptr, 0, (size_t)(totalCells + 1) * 4, ctx->stream));
-    HIP_TRY(ensure(ctx->pathDraws, (size_t)(pathCount + 1) * 4));
+    RIVE_TRY(ensure(ctx, ctx->pathDraws, (size_t)(pathCount + 1) * 4));
     HIP_TRY(hipMemsetAsync(ctx->pathDraws.ptr, 0, (size_t)(pathCount + 1) * 4, ctx->stream));
     if (totalPrims > 0 && drawCount > 0)
         bin_count_draw_kernel<<<drawCount, 256, 0, ctx->stream>>>(
@@ -539,7 +657,7 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     rc = device_scan(ctx, P<uint32_t>(ctx->cellCount), totalCells, P<uint32_t>(ctx->cellOffset), scal + 2);
     if (rc)
         return rc;
-    HIP_TRY(ensure(ctx->cellPacked, (size_t)(totalCells + 1) * sizeof(uint2)));
+    RIVE_TRY(ensure(ctx, ctx->cellPacked, (size_t)(totalCells + 1) * sizeof(uint2)));
     {
         int maxRows = maxTilesY;
         const uint64_t threads = std::max<uint64_t>((uint64_t)(pathCount + 1) * (uint64_t)maxRows * 64u, 1);
@@ -554,13 +672,13 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     ctx->stats.bin_entries = totalPairs;
     ctx->stats.prim_count = totalPrims;
     ctx->stats.tile_count = tileTotal;
-    HIP_TRY(ensure(ctx->binned, (size_t)std::max<uint32_t>(totalPairs, 1) * 4));
+    RIVE_TRY(ensure(ctx, ctx->binned, (size_t)std::max<uint32_t>(totalPairs, 1) * 4));
     if (totalPrims > 0 && drawCount > 0)
         bin_scatter_draw_kernel<<<drawCount, 256, 0, ctx->stream>>>(
             P<Prim>(ctx->prims), drawTess, P<PathCells>(ctx->cells), P<uint32_t>(ctx->cellOffset),
             P<uint32_t>(ctx->cellCursor), P<uint32_t>(ctx->pathDraws), P<uint32_t>(ctx->binned));
-    HIP_TRY(ensure(ctx->pathRaster, (size_t)(pathCount + 1) * sizeof(PathRaster)));
-    HIP_TRY(ensure(ctx->pathPaint, (size_t)(pathCount + 1) * sizeof(PathPaint)));
+    RIVE_TRY(ensure(ctx, ctx->pathRaster, (size_t)(pathCount + 1) * sizeof(PathRaster)));
+    RIVE_TRY(ensure(ctx, ctx->pathPaint, (size_t)(pathCount + 1) * sizeof(PathPaint)));
     path_raster_kernel<<<(pathCount + 1 + 255) / 256, 256, 0, ctx->stream>>>(
         paths, paints, aux, P<uint32_t>(ctx->pathImage), pathCount, gradRows, P<PathRaster>(ctx->pathRaster),
         P<PathPaint>(ctx->pathPaint));
@@ -683,13 +801,19 @@ extern "C" int rive_hip_last_flush_canvases(rive_hip_context* ctx, rive_hip_canv
     return RIVE_HIP_OK;
 }
 
-extern "C" int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* d)
+namespace
+{
+int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d)
 {
     if (ctx == nullptr || d == nullptr || (d->target == nullptr && d->canvas_count == 0))
         return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null argument");
     HIP_TRY(hipSetDevice(ctx->device));
-    // The canvas table: validated before anything is enqueued.
-    std::vector<rive_hip_canvas> canvases;
+    // The canvas table: validated before anything is enqueued.  The vector
+    // keeps its capacity across flushes (a no-alloc flush must not grow it).
+    std::vector<rive_hip_canvas>& canvases = ctx->flushCanvases;
+    canvases.clear();
+    if (ctx->arena.active && std::max<uint32_t>(d->canvas_count, 1) > canvases.capacity())
+        return fail(RIVE_HIP_ERROR_UNSUPPORTED, "rive_hip_flush_noalloc renders one canvas per flush");
     if (d->canvas_count == 0)
     {
         if (d->target->ctx != ctx)
@@ -730,8 +854,8 @@ extern "C" int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* 
         !need(RIVE_HIP_BUFFER_GRAD_SPAN, (size_t)d->grad_span_count * sizeof(rive_hip_grad_span)) ||
         !need(RIVE_HIP_BUFFER_DRAW, (size_t)d->draw_count * sizeof(rive_hip_draw)))
         return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "buffers smaller than the flush descriptor");
-    HIP_TRY(ensure(ctx->scalars, 64));
-    HIP_TRY(ensure(ctx->drawTess, (size_t)std::max<uint32_t>(d->draw_count, 1) * sizeof(DrawTess)));
+    RIVE_TRY(ensure(ctx, ctx->scalars, 64));
+    RIVE_TRY(ensure(ctx, ctx->drawTess, (size_t)std::max<uint32_t>(d->draw_count, 1) * sizeof(DrawTess)));
     HIP_TRY(hipMemsetAsync(ctx->scalars.ptr, 0, 64, ctx->stream));
     HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream)); // flush start
 
@@ -772,6 +896,9 @@ extern "C" int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* 
             return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "image_count without images");
         if (ctx->hostImagesCap < d->image_count)
         {
+            if (ctx->arena.active)
+                return fail(RIVE_HIP_ERROR_UNSUPPORTED,
+                            "rive_hip_flush_noalloc: more images than the staged table holds");
             if (ctx->hostImages)
                 HIP_TRY(hipHostFree(ctx->hostImages));
             ctx->hostImages = nullptr;
@@ -785,11 +912,11 @@ extern "C" int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* 
                 return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null or foreign image in the image table");
             ctx->hostImages[k] = d->images[k]->desc;
         }
-        HIP_TRY(ensure(ctx->imageTable, sizeof(ImageDesc) * d->image_count));
+        RIVE_TRY(ensure(ctx, ctx->imageTable, sizeof(ImageDesc) * d->image_count));
         HIP_TRY(hipMemcpyAsync(ctx->imageTable.ptr, ctx->hostImages, sizeof(ImageDesc) * d->image_count,
                                hipMemcpyHostToDevice, ctx->stream));
     }
-    HIP_TRY(ensure(ctx->pathImage, (size_t)(d->path_count + 1) * 4));
+    RIVE_TRY(ensure(ctx, ctx->pathImage, (size_t)(d->path_count + 1) * 4));
     HIP_TRY(hipMemsetAsync(ctx->pathImage.ptr, 0, (size_t)(d->path_count + 1) * 4, ctx->stream));
     draw_tess_kernel<<<1, 1024, 0, ctx->stream>>>(
         reinterpret_cast<const rive_hip_draw*>(ctx->bufs[RIVE_HIP_BUFFER_DRAW].dev), d->draw_count,
@@ -850,6 +977,44 @@ extern "C" int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* 
     ctx->timed = true;
     return RIVE_HIP_OK;
 }
+} // namespace
+
+extern "C" int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* d)
+{
+    if (ctx != nullptr)
+        ctx->lastFlushNoAlloc = false;
+    return flush_impl(ctx, d);
+}
+
+extern "C" int rive_hip_flush_noalloc(rive_hip_context* ctx, const rive_hip_flush_desc* d)
+{
+    if (ctx == nullptr || d == nullptr)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+    if (ctx->scratchPtr == nullptr)
+        return fail(RIVE_HIP_ERROR_DEVICE, "no-alloc scratch address unavailable");
+    // Set the context's own scratch buffers aside; every buffer this flush
+    // needs is carved from the device's scratch array, and the set-aside
+    // buffers come back untouched afterwards (the enqueued kernels keep the
+    // carved addresses they were given).
+    DevBuf* all[kDevBufCount];
+    devbufs(ctx, all);
+    DevBuf saved[kDevBufCount];
+    for (int i = 0; i < kDevBufCount; ++i)
+    {
+        saved[i] = *all[i];
+        *all[i] = DevBuf{};
+    }
+    ctx->arena.active = true;
+    ctx->arena.base = ctx->scratchPtr;
+    ctx->arena.used = 0;
+    ctx->arena.cap = kNoAllocScratchBytes;
+    const int rc = flush_impl(ctx, d);
+    ctx->arena.active = false;
+    for (int i = 0; i < kDevBufCount; ++i)
+        *all[i] = saved[i];
+    ctx->lastFlushNoAlloc = true;
+    return rc;
+}
 
 extern "C" int rive_hip_pass_times(rive_hip_context* ctx, float* ms, int n)
 {
@@ -878,6 +1043,9 @@ extern "C" int rive_hip_debug_readback(rive_hip_context* ctx, int what, void* ds
     if (ctx == nullptr)
         return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "ctx is null");
     HIP_TRY(hipSetDevice(ctx->device));
+    if (ctx->lastFlushNoAlloc)
+        return fail(RIVE_HIP_ERROR_UNSUPPORTED,
+                    "intermediates of a no-alloc flush are not kept (its scratch is shared per device)");
     const bool geo = ctx->lastMode == RIVE_HIP_FLUSH_GEOMETRY;
     const void* src = nullptr;
     size_t bytes = 0;

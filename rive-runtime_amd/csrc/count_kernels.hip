@@ -1074,7 +1074,7 @@ int geometry_counting(rive_hip_context* ctx, const rive_hip_flush_desc* d, const
         return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "GEOMETRY flush without VERB/POINT buffers");
     uint32_t* scal = P<uint32_t>(ctx->scalars);
     // Per-draw counts: tess, contours, records (3 arrays of n) + 3 offset arrays of n+1.
-    HIP_TRY(ensure(ctx->geoCounts, (size_t)(6 * (n + 1) + 16) * 4));
+    RIVE_TRY(ensure(ctx, ctx->geoCounts, (size_t)(6 * (n + 1) + 16) * 4));
     uint32_t* tessCounts = P<uint32_t>(ctx->geoCounts);
     uint32_t* contourCounts = tessCounts + (n + 1);
     uint32_t* recordCounts = contourCounts + (n + 1);
@@ -1111,8 +1111,8 @@ int geometry_counting(rive_hip_context* ctx, const rive_hip_flush_desc* d, const
         return fail(RIVE_HIP_ERROR_UNSUPPORTED, "more than 65535 contours in one flush");
     if (tessTotal > kMaxFlushTexels || recordTotal == kScanSaturated)
         return fail(RIVE_HIP_ERROR_UNSUPPORTED, "flush too large: more than 2^31 tessellation vertices");
-    HIP_TRY(ensure(ctx->geoRecords, (size_t)std::max<uint32_t>(recordTotal, 1) * sizeof(CubicRec)));
-    HIP_TRY(ensure(ctx->geoContours, (size_t)std::max<uint32_t>(contourTotal, 1) * sizeof(ContourV)));
+    RIVE_TRY(ensure(ctx, ctx->geoRecords, (size_t)std::max<uint32_t>(recordTotal, 1) * sizeof(CubicRec)));
+    RIVE_TRY(ensure(ctx, ctx->geoContours, (size_t)std::max<uint32_t>(contourTotal, 1) * sizeof(ContourV)));
     if (n > 0)
         emit_kernel<<<(n + 31) / 32, 32, 0, ctx->stream>>>(draws, n, verbs, points, tessOffsets,
                                                               contourOffsets, recordOffsets, firstLocation,
@@ -1126,8 +1126,8 @@ int geometry_counting(rive_hip_context* ctx, const rive_hip_flush_desc* d, const
     if (overflow != 0)
         return fail(RIVE_HIP_ERROR_DEVICE, "curve emission disagreed with the counting pass");
     // spans
-    HIP_TRY(ensure(ctx->geoSpanCounts, (size_t)(recordTotal + 1) * 4));
-    HIP_TRY(ensure(ctx->geoSpanOffsets, (size_t)(recordTotal + 1) * 4));
+    RIVE_TRY(ensure(ctx, ctx->geoSpanCounts, (size_t)(recordTotal + 1) * 4));
+    RIVE_TRY(ensure(ctx, ctx->geoSpanOffsets, (size_t)(recordTotal + 1) * 4));
     if (recordTotal > 0)
         span_count_kernel<<<(recordTotal + 255) / 256, 256, 0, ctx->stream>>>(
             P<CubicRec>(ctx->geoRecords), recordTotal, P<uint32_t>(ctx->geoSpanCounts));
@@ -1140,7 +1140,7 @@ int geometry_counting(rive_hip_context* ctx, const rive_hip_flush_desc* d, const
     if (rc)
         return rc;
     const uint32_t allSpans = spanTotal + 2; // + padding spans
-    HIP_TRY(ensure(ctx->geoSpans, (size_t)allSpans * sizeof(SpanV)));
+    RIVE_TRY(ensure(ctx, ctx->geoSpans, (size_t)allSpans * sizeof(SpanV)));
     padding_span_kernel<<<1, 64, 0, ctx->stream>>>(P<SpanV>(ctx->geoSpans), scal + 8, firstLocation);
     if (recordTotal > 0)
         span_write_kernel<<<(recordTotal + 255) / 256, 256, 0, ctx->stream>>>(

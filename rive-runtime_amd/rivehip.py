@@ -96,7 +96,7 @@ EXPORTED = [
     "rive_hip_debug_readback", "rive_hip_pass_times", "rive_stream_generate", "rive_stream_free",
     "rive_hip_render_stream", "rive_hip_last_flush_stats", "rive_hip_image_create",
     "rive_hip_image_destroy", "rive_hip_render_streams", "rive_hip_last_flush_canvases",
-    "rive_stream_generate_batch", "rive_stream_contour_bound",
+    "rive_stream_generate_batch", "rive_stream_contour_bound", "rive_hip_flush_noalloc",
 ]
 
 _lib = None
@@ -123,6 +123,7 @@ def lib():
     L.rive_hip_target_destroy.argtypes = [vp]
     L.rive_hip_target_destroy.restype = None
     L.rive_hip_flush.argtypes = [vp, ctypes.POINTER(FlushDesc)]
+    L.rive_hip_flush_noalloc.argtypes = [vp, ctypes.POINTER(FlushDesc)]
     L.rive_hip_sync.argtypes = [vp]
     L.rive_hip_read_pixels.argtypes = [vp, vp, vp]
     L.rive_hip_debug_readback.argtypes = [vp, i32, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
