@@ -238,6 +238,16 @@ typedef struct rive_hip_flush_desc
      * canvas_count == 0 is the reference's single-target flush. */
     const struct rive_hip_canvas* canvases;
     uint32_t canvas_count;
+    /* Where this logical flush's records start in their buffers, as
+     * FlushDescriptor's firstPath/firstPaint/firstPaintAux/firstContour/
+     * firstGradSpan/firstTessVertexSpan (gpu.hpp:1386-1395): one frame's
+     * logical flushes share the mapped buffers, each flush reading its own
+     * run.  Path ID k reads PATH record first_path + k (k = 0 is the
+     * flush's reserved record), contour ID k reads CONTOUR record
+     * first_contour + k - 1.  first_draw offsets the DRAW records the same
+     * way.  All zero for a frame of one logical flush. */
+    uint64_t first_path, first_paint, first_paint_aux, first_contour;
+    uint64_t first_grad_span, first_tess_span, first_draw;
 } rive_hip_flush_desc;
 
 /* One canvas of a batched flush: paths first_path .. first_path+path_count-1
@@ -272,6 +282,25 @@ int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* desc);
  * contexts, several threads) race.  rive_hip_debug_readback of intermediates
  * is unavailable after a no-alloc flush. */
 int rive_hip_flush_noalloc(rive_hip_context* ctx, const rive_hip_flush_desc* desc);
+
+/* Per-draw resource counts of the draws in the DRAW/VERB/POINT buffers,
+ * computed on the device: tess_vertex_counts[i] = draw i's tessellation
+ * vertices (both directions, with contour padding) and contour_counts[i] its
+ * contours, i.e. PathDraw::resourceCounts() (draw.cpp:1378-1391).  The front
+ * end uses them to split a frame into logical flushes the way
+ * LogicalFlush::pushDraws does (render_context.cpp:517-541) before laying
+ * each flush out with rive_hip_flush.  Synchronous. */
+int rive_hip_count_draws(rive_hip_context* ctx, uint32_t draw_count, uint32_t* tess_vertex_counts,
+                         uint32_t* contour_counts);
+
+/* The descriptors of the last frame's logical flushes, in order: the flushes
+ * since the last one whose records start at offset 0 (first_* all zero).
+ * Their image and canvas tables are copies the context keeps until the next
+ * frame.  Re-issuing them with rive_hip_flush re-renders the frame from the
+ * records still resident in HBM (what a serving loop or a benchmark of the
+ * flush alone does).  Writes min(capacity, count) descriptors; *count gets
+ * the full count. */
+int rive_hip_last_frame(rive_hip_context* ctx, rive_hip_flush_desc* out, uint32_t capacity, uint32_t* count);
 
 /* Waits for the context stream (the reference's postFlush/fence wait). */
 int rive_hip_sync(rive_hip_context* ctx);
@@ -393,9 +422,22 @@ int rive_stream_generate(int config, uint64_t seed, uint32_t width, uint32_t hei
 void rive_stream_free(rive_stream* stream);
 
 /* Renders a stream through the C++ front end (RiveRenderer -> RenderContext
- * -> RenderContextHIPImpl) into `target`, GEOMETRY mode. */
+ * -> RenderContextHIPImpl) into `target`, GEOMETRY mode.  Like the
+ * reference's RenderContext, the front end splits the frame into logical
+ * flushes where a flush would exceed MaxPathID(1) - 1 = 30719 paths or clip
+ * IDs, 65535 contours, 2048 * 2048 - 25 tessellation vertices or 2048
+ * gradient rows (render_context.cpp:43-53,484-492,517-541,617-654;
+ * rive_renderer.cpp:496-570); each logical flush is one rive_hip_flush, the
+ * later ones loading the target. */
 int rive_hip_render_stream(rive_hip_context* ctx, const rive_stream* stream,
                            rive_hip_target* target);
+
+/* rive_hip_render_stream that stops after the frame's first max_flushes
+ * logical flushes (all of them when max_flushes == 0) and reports the
+ * frame's logical flush count in *flush_count (when not NULL): lets a
+ * caller inspect each logical flush's records with rive_hip_debug_readback. */
+int rive_hip_render_stream_ex(rive_hip_context* ctx, const rive_stream* stream, rive_hip_target* target,
+                              uint32_t max_flushes, uint32_t* flush_count);
 
 /* Renders n streams into n targets with batched flushes: each stream is
  * replayed through RiveRenderer as its own canvas, and consecutive canvases

@@ -8,9 +8,12 @@
 
 using namespace oracle;
 
+// A frame: its logical flushes and the one the record getters read.
 struct oracle_flush
 {
-    Flush f;
+    std::vector<Flush> flushes;
+    size_t sel = 0;
+    const Flush& cur() const { return flushes[sel]; }
 };
 
 namespace
@@ -34,8 +37,19 @@ int oracle_is_aabb(const uint8_t* verbs, uint32_t verbCount, const float* pts, u
 oracle_flush* oracle_build(const rive_stream* s)
 {
     auto* of = new oracle_flush;
-    build_flush(*s, &of->f);
+    build_flushes(*s, &of->flushes);
     return of;
+}
+
+uint32_t oracle_flush_count(const oracle_flush* of) { return (uint32_t)of->flushes.size(); }
+
+// Selects the logical flush the record getters read; -1 if out of range.
+int oracle_select(oracle_flush* of, uint32_t index)
+{
+    if (index >= of->flushes.size())
+        return -1;
+    of->sel = index;
+    return 0;
 }
 
 void oracle_free(oracle_flush* of) { delete of; }
@@ -44,7 +58,7 @@ void oracle_free(oracle_flush* of) { delete of; }
 // 6 records (CubicRecord), 7 draws (DrawInfo as 5 u32), 8 info (u32 x 10)
 size_t oracle_get(oracle_flush* of, int what, void* dst, size_t cap)
 {
-    const Flush& f = of->f;
+    const Flush& f = of->cur();
     switch (what)
     {
         case 0: return copy_out(f.spans, dst, cap);
@@ -88,27 +102,34 @@ size_t oracle_get(oracle_flush* of, int what, void* dst, size_t cap)
 size_t oracle_tessellate(oracle_flush* of, void* dst, size_t cap)
 {
     std::vector<TessVertex> tex;
-    tessellate_flush(of->f, &tex);
+    tessellate_flush(of->cur(), &tex);
     return copy_out(tex, dst, cap);
 }
 
 size_t oracle_ramps(oracle_flush* of, void* dst, size_t cap)
 {
     std::vector<uint32_t> tex;
-    render_ramps(of->f, &tex);
+    render_ramps(of->cur(), &tex);
     return copy_out(tex, dst, cap);
 }
 
-// Renders the flush into rgba (width*height*4 bytes).  Returns seconds spent.
-double oracle_render(oracle_flush* of, uint8_t* rgba)
+// Renders the frame -- every logical flush in order -- into rgba
+// (width*height*4 bytes); edges (optional, width*height bytes, zeroed by the
+// caller) marks pixels some draw covers fractionally.  Returns seconds spent.
+double oracle_render_edges(oracle_flush* of, uint8_t* rgba, uint8_t* edges)
 {
     auto t0 = std::chrono::steady_clock::now();
-    std::vector<TessVertex> tex;
-    tessellate_flush(of->f, &tex);
-    render_flush(of->f, tex, reinterpret_cast<uint32_t*>(rgba));
+    for (const Flush& f : of->flushes)
+    {
+        std::vector<TessVertex> tex;
+        tessellate_flush(f, &tex);
+        render_flush(f, tex, reinterpret_cast<uint32_t*>(rgba), edges);
+    }
     auto t1 = std::chrono::steady_clock::now();
     return std::chrono::duration<double>(t1 - t0).count();
 }
+
+double oracle_render(oracle_flush* of, uint8_t* rgba) { return oracle_render_edges(of, rgba, nullptr); }
 
 // ---- maths entry points for golden-vector tests ----
 static void load_pts(const float* p, V2 out[4])

@@ -5,6 +5,7 @@
 #include "oracle.hpp"
 #include "../include/rive_hip.h"
 
+#include <memory>
 #include <vector>
 
 namespace oracle
@@ -31,10 +32,15 @@ void build_image_mips(uint32_t width, uint32_t height, const uint8_t* rgba, Orac
 // wrapX + 3*wrapY + 9*filter; mip mode nearest).
 void sample_image(const OracleImage& img, float u, float v, float lod, uint32_t sampler, float out[4]);
 
+// One logical flush (LogicalFlush, render_context.hpp): its own path,
+// contour and clip IDs, gradient texture and tessellation texture.  A frame
+// is one or more logical flushes; the first clears the target, later ones
+// load it (render_context.cpp:1218-1222).
 struct Flush
 {
     uint32_t width = 0, height = 0, clearColor = 0, dither = 0;
-    std::vector<OracleImage> images;
+    bool loadPreserve = false;
+    std::shared_ptr<const std::vector<OracleImage>> images; // shared by a frame's flushes
     std::vector<uint32_t> pathImage; // per path: image index (1-based, 0 none) | sampler << 16
     std::vector<rive_hip_path_data> pathData; // [0] reserved
     std::vector<rive_hip_paint_data> paintData;
@@ -51,8 +57,14 @@ struct Flush
     uint32_t tessVertexCount = 0;
 };
 
-void build_flush(const rive_stream& s, Flush* f);
+// Replays a stream through the front end, splitting it into logical flushes
+// where the reference's resource limits require (rive_renderer.cpp:496-570).
+void build_flushes(const rive_stream& s, std::vector<Flush>* out);
 void render_ramps(const Flush& f, std::vector<uint32_t>* texture);
 void tessellate_flush(const Flush& f, std::vector<TessVertex>* tex);
-void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* rgba);
+// Renders one logical flush into rgba (cleared first unless f.loadPreserve).
+// edgeMask (optional, one byte per pixel) gets 1 where some draw's final
+// coverage lies strictly between 0 and 1.  Rows are split over the host's
+// threads.
+void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* rgba, uint8_t* edgeMask);
 } // namespace oracle

@@ -9,6 +9,8 @@
 #include "oracle.hpp"
 #include "oracle_flush.hpp"
 
+#include <omp.h>
+
 #include <algorithm>
 #include <array>
 #include <cassert>
@@ -337,6 +339,7 @@ struct ClipElement
     FillRule fillRule;
     IAABB pixelBounds;
     uint32_t clipID;
+    uint32_t flushIndex; // the logical flush clipID belongs to
 };
 
 uint32_t swizzle_rgba(uint32_t argb) // SwizzleRiveColorToRGBA
@@ -346,15 +349,29 @@ uint32_t swizzle_rgba(uint32_t argb) // SwizzleRiveColorToRGBA
 }
 } // namespace
 
+// Resource limits of one logical flush (render_context.cpp:43-53,517-541,
+// 484-492, 617-654; gpu.hpp:85-103): path and clip IDs up to
+// MaxPathID(1) - 1, contour IDs up to 65535, tessellation vertices up to a
+// 2048 x 2048 texture less 25 padding vertices, gradient texture rows up to
+// 2048.
+constexpr uint64_t kMaxPathID = (0x1fu << 10) - 1 - 1023 - 1;
+constexpr uint64_t kMaxContourID = 65535;
+constexpr uint64_t kMaxTessVerticesBeforePadding = 2048ull * 2048ull - (8 + 16 + 1);
+constexpr uint64_t kMaxGradientRows = 2048;
+
 struct Builder
 {
     const rive_stream& s;
-    Flush& f;
+    std::vector<Flush>& flushes;
+    Flush* fp = nullptr;
+    std::shared_ptr<std::vector<OracleImage>> images; // shared by the frame's flushes
     std::vector<ClipElement> clipStack;
     uint32_t clipContentID = 0;
     uint32_t nextClipID = 1;
     uint32_t nextContourID = 1;
     uint32_t tessLocation = kMidpointFanPatchSegmentSpan; // pre-padding
+    // LogicalFlush::m_resourceCounts of the current logical flush
+    uint64_t resPaths = 0, resContours = 0, resTess = 0;
     // gradient bookkeeping (render_context.cpp:588-674)
     std::vector<std::pair<uint32_t, uint32_t>> simpleRamps;
     std::vector<Gradient> complexRamps;
@@ -366,7 +383,8 @@ struct Builder
     };
     std::vector<PendingPaint> pendingGradPaints;
 
-    Builder(const rive_stream& s_, Flush& f_) : s(s_), f(f_) {}
+    Builder(const rive_stream& s_, std::vector<Flush>& out) : s(s_), flushes(out) {}
+    Flush& cur() { return *fp; }
 
     const V2* pathPts(uint32_t path) const
     {
@@ -403,7 +421,7 @@ struct Builder
         d.fillRule = fillRule;
         derive_directions(&d);
 
-        uint32_t pathID = (uint32_t)f.pathData.size(); // record 0 is reserved
+        uint32_t pathID = (uint32_t)cur().pathData.size(); // record 0 is reserved
         d.pathID = pathID;
         PathTess pt = tessellate_path_records(d, nextContourID, tessLocation);
 
@@ -413,7 +431,7 @@ struct Builder
         pd.stroke_radius = strokeRadius;
         pd.feather_radius = 0;
         pd.z_index = 0;
-        f.pathData.push_back(pd);
+        cur().pathData.push_back(pd);
 
         // PaintData (gpu.cpp:879-939)
         rive_hip_paint_data pp{};
@@ -436,12 +454,12 @@ struct Builder
                 params |= PAINT_FLAG_EVEN_ODD_FILL;
         }
         rive_hip_paint_aux aux{};
-        f.pathImage.resize(pathID + 1, 0);
+        cur().pathImage.resize(pathID + 1, 0);
         if (image != 0)
         {
             // PAINT_FLAG_HAS_IMAGE; PaintAuxData image branch (gpu.cpp:1006-1034).
             params |= 0x800u;
-            const OracleImage& img = f.images[image - 1];
+            const OracleImage& img = (*images)[image - 1];
             Mat inv;
             if (!invert(matrix, &inv))
                 inv = Mat();
@@ -450,7 +468,7 @@ struct Builder
             float dvdx = inv.m[1] * (float)img.width, dvdy = inv.m[3] * (float)img.height;
             float maxScaleFactorPow2 = std::max(dudx * dudx + dvdx * dvdx, dudy * dudy + dvdy * dvdy);
             aux.image_lod = log2f(std::max(maxScaleFactorPow2, 1.f)) * .5f + -.5f; // MIP_MAP_LOD_BIAS
-            f.pathImage[pathID] = image | sampler << 16;
+            cur().pathImage[pathID] = image | sampler << 16;
         }
         // PaintAuxData clip rect (gpu.cpp:1044-1062): the inverse matrix and
         // inverse fwidth, or WideOpen {0, 0, 0, 0, 1, 1} without a clip rect.
@@ -468,7 +486,7 @@ struct Builder
             memcpy(aux.clip_rect_inverse_matrix, wideOpen, sizeof(wideOpen));
         }
         pp.params = params;
-        f.paintData.push_back(pp);
+        cur().paintData.push_back(pp);
 
         if (grad != nullptr)
         {
@@ -502,7 +520,7 @@ struct Builder
             memcpy(aux.paint_matrix, pm.m, sizeof(aux.paint_matrix));
             allocateGradient(*grad, pathID);
         }
-        f.paintAux.push_back(aux);
+        cur().paintAux.push_back(aux);
 
         // Tessellation bookkeeping
         DrawInfo di;
@@ -511,17 +529,23 @@ struct Builder
         di.tessVertexCount = pt.midpointFanTessVertexCount;
         di.dirs = d.dirs;
         di.isStroke = strokeRadius > 0;
-        f.draws.push_back(di);
+        cur().draws.push_back(di);
         for (size_t c = 0; c < pt.contours.size(); ++c)
-            f.contours.push_back(pt.contours[c]);
+            cur().contours.push_back(pt.contours[c]);
         nextContourID += (uint32_t)pt.contours.size();
         for (const CubicRecord& r : pt.records)
         {
-            f.records.push_back(r);
-            write_spans(r, d.dirs, &f.spans);
+            cur().records.push_back(r);
+            write_spans(r, d.dirs, &cur().spans);
         }
-        f.pathTessVertexCounts.push_back(pt.midpointFanTessVertexCount);
+        cur().pathTessVertexCounts.push_back(pt.midpointFanTessVertexCount);
         tessLocation += pt.midpointFanTessVertexCount;
+        if (pt.midpointFanTessVertexCount > 0) // draw.cpp:1378-1391
+        {
+            resPaths += 1;
+            resContours += pt.contours.size();
+            resTess += pt.midpointFanTessVertexCount;
+        }
     }
 
     void allocateGradient(const Gradient& g, uint32_t pathID)
@@ -570,8 +594,8 @@ struct Builder
         const uint32_t simpleRows =
             (uint32_t)((simpleRamps.size() * 2 + kGradTextureWidth - 1) / kGradTextureWidth);
         const uint32_t complexRows = (uint32_t)complexRamps.size();
-        f.gradSimpleRows = simpleRows;
-        f.gradComplexRows = complexRows;
+        cur().gradSimpleRows = simpleRows;
+        cur().gradComplexRows = complexRows;
         const uint32_t height = std::max<uint32_t>(simpleRows + complexRows, 1);
         const uint32_t ONE_TEXEL_FIXED = 65536 / kGradTextureWidth;
         for (size_t i = 0; i < simpleRamps.size(); ++i)
@@ -584,7 +608,7 @@ struct Builder
             gs.y_with_flags = GRAD_SPAN_FLAG_LEFT_BORDER | GRAD_SPAN_FLAG_RIGHT_BORDER | y;
             gs.color0 = simpleRamps[i].first;
             gs.color1 = simpleRamps[i].second;
-            f.gradSpans.push_back(gs);
+            cur().gradSpans.push_back(gs);
         }
         for (uint32_t i = 0; i < complexRows; ++i)
         {
@@ -607,7 +631,7 @@ struct Builder
                 gs.y_with_flags = flags | y;
                 gs.color0 = lastColor;
                 gs.color1 = g.colors[k];
-                f.gradSpans.push_back(gs);
+                cur().gradSpans.push_back(gs);
                 lastColor = g.colors[k];
                 lastXFixed = xFixed;
             }
@@ -616,7 +640,7 @@ struct Builder
         for (const PendingPaint& pp : pendingGradPaints)
         {
             uint32_t row = pp.row + (pp.complex ? simpleRows : 0);
-            f.paintData[pp.pathID].value = fbits((static_cast<float>(row) + .5f) * inverseHeight);
+            cur().paintData[pp.pathID].value = fbits((static_cast<float>(row) + .5f) * inverseHeight);
             float left, right;
             if (pp.complex)
             {
@@ -628,31 +652,38 @@ struct Builder
                 left = (float)pp.col;
                 right = left + 2;
             }
-            f.paintAux[pp.pathID].grad_span[0] = (right - left - 1) * (1.f / kGradTextureWidth);
-            f.paintAux[pp.pathID].grad_span[1] = (left + .5f) * (1.f / kGradTextureWidth);
+            cur().paintAux[pp.pathID].grad_span[0] = (right - left - 1) * (1.f / kGradTextureWidth);
+            cur().paintAux[pp.pathID].grad_span[1] = (left + .5f) * (1.f / kGradTextureWidth);
         }
-        f.gradHeight = height;
+        cur().gradHeight = height;
     }
 
-    // applyClip (rive_renderer.cpp:631-824): pushes the clip-update draws the
-    // stack still needs and returns the clip ID the draw reads.
+    // The clip elements a draw under `height` clip paths must (re)issue in
+    // the current logical flush: those above the deepest element whose
+    // content the clip buffer holds (rive_renderer.cpp:631-824).  A clip ID
+    // from an earlier logical flush never matches.
+    size_t clipInBuffer(size_t height) const
+    {
+        if (clipContentID != 0)
+            for (size_t i = height - 1; i != (size_t)-1; --i)
+                if (clipStack[i].clipID == clipContentID && clipStack[i].flushIndex == flushes.size() - 1)
+                    return i;
+        return (size_t)-1;
+    }
+
+    // applyClip: pushes the clip-update draws the stack still needs and
+    // returns the clip ID the draw reads.
     uint32_t applyClip(size_t height, const IAABB& frame)
     {
         if (height == 0)
             return 0;
-        size_t inBuffer = (size_t)-1;
-        if (clipContentID != 0)
-            for (size_t i = height - 1; i != (size_t)-1; --i)
-                if (clipStack[i].clipID == clipContentID)
-                {
-                    inBuffer = i;
-                    break;
-                }
+        const size_t inBuffer = clipInBuffer(height);
         uint32_t parentClipID = inBuffer == (size_t)-1 ? 0 : clipStack[inBuffer].clipID;
         for (size_t i = inBuffer + 1; i < height; ++i)
         {
             ClipElement& ce = clipStack[i];
             ce.clipID = nextClipID++;
+            ce.flushIndex = (uint32_t)flushes.size() - 1;
             IAABB cb = ce.pixelBounds;
             if (!cb.intersect(frame).empty())
                 pushPathDraw(ce.path, ce.matrix, 0, Join::miter, Cap::butt, ce.fillRule,
@@ -663,15 +694,107 @@ struct Builder
         return parentClipID;
     }
 
-    void build()
+    // Draw::resourceCounts of one PathDraw (draw.cpp:1378-1391): a path, its
+    // contours and its tessellation vertices, or nothing when it tessellates
+    // to no vertices.
+    struct Needs
     {
-        f.images.resize(s.image_count);
-        for (uint32_t i = 0; i < s.image_count; ++i)
-            build_image_mips(s.images[i].width, s.images[i].height, s.images[i].rgba_premul, &f.images[i]);
+        uint64_t paths = 0, contours = 0, tess = 0;
+        void add(const Needs& o)
+        {
+            paths += o.paths;
+            contours += o.contours;
+            tess += o.tess;
+        }
+    };
+    Needs measure(const uint8_t* verbs, uint32_t verbCount, const V2* pts, uint32_t ptCount, const Mat& matrix,
+                  float strokeRadius, Join join, Cap cap, FillRule fillRule) const
+    {
+        PathDrawDesc d;
+        d.verbs = verbs;
+        d.verbCount = verbCount;
+        d.pts = pts;
+        d.ptCount = ptCount;
+        d.matrix = matrix;
+        d.strokeRadius = strokeRadius;
+        d.join = join;
+        d.cap = cap;
+        d.fillRule = fillRule;
+        derive_directions(&d);
+        d.pathID = 1;
+        PathTess pt = tessellate_path_records(d, 1, 0);
+        Needs n;
+        if (pt.midpointFanTessVertexCount > 0)
+        {
+            n.paths = 1;
+            n.contours = pt.contours.size();
+            n.tess = pt.midpointFanTessVertexCount;
+        }
+        return n;
+    }
+    Needs measurePath(uint32_t pathIdx, const Mat& m, float strokeRadius, Join join, Cap cap, FillRule rule) const
+    {
+        const rive_stream_path& sp = s.paths[pathIdx];
+        return measure(s.verbs + sp.first_verb, sp.verb_count, pathPts(pathIdx), sp.point_count, m, strokeRadius,
+                       join, cap, rule);
+    }
+
+    // Whether a draw batch -- the clip draws applyClip would issue now plus
+    // the draw itself -- fits the current logical flush: the checks of
+    // generateClipID (render_context.cpp:484-492), LogicalFlush::pushDraws
+    // (:517-541) and allocateGradient (:617-654).
+    bool batchFits(size_t height, const IAABB& frame, const Needs& drawNeeds, const Gradient* grad) const
+    {
+        Needs n = drawNeeds;
+        uint64_t newClips = 0;
+        if (height > 0)
+            for (size_t i = clipInBuffer(height) + 1; i < height; ++i)
+            {
+                ++newClips;
+                const ClipElement& ce = clipStack[i];
+                IAABB cb = ce.pixelBounds;
+                if (!cb.intersect(frame).empty())
+                    n.add(measurePath(ce.path, ce.matrix, 0, Join::miter, Cap::butt, ce.fillRule));
+            }
+        if ((uint64_t)(nextClipID - 1) + newClips > kMaxPathID)
+            return false;
+        if (resPaths + n.paths > kMaxPathID || resContours + n.contours > kMaxContourID ||
+            resTess + n.tess > kMaxTessVerticesBeforePadding)
+            return false;
+        if (grad != nullptr)
+        {
+            uint64_t simple = simpleRamps.size(), complex = complexRamps.size();
+            const size_t k = grad->stops.size();
+            if (k == 1 || (k == 2 && grad->stops[0] == 0 && grad->stops[1] == 1))
+            {
+                std::pair<uint32_t, uint32_t> key{grad->colors[0], grad->colors[std::min<size_t>(1, k - 1)]};
+                if (std::find(simpleRamps.begin(), simpleRamps.end(), key) == simpleRamps.end())
+                    ++simple;
+            }
+            else
+            {
+                bool found = false;
+                for (const Gradient& g : complexRamps)
+                    found |= g.colors == grad->colors && g.stops == grad->stops;
+                complex += found ? 0 : 1;
+            }
+            if ((simple + kGradTextureWidth / 2 - 1) / (kGradTextureWidth / 2) + complex > kMaxGradientRows)
+                return false;
+        }
+        return true;
+    }
+
+    void beginFlush()
+    {
+        flushes.emplace_back();
+        fp = &flushes.back();
+        Flush& f = cur();
+        f.images = images;
         f.width = s.width;
         f.height = s.height;
         f.clearColor = s.clear_color;
         f.dither = s.dither;
+        f.loadPreserve = flushes.size() > 1; // later logical flushes load the target
         // Path record 0 is reserved for the clear color (render_context.cpp:1535-1548).
         f.pathData.push_back(rive_hip_path_data{});
         rive_hip_paint_data clearPaint{};
@@ -679,6 +802,57 @@ struct Builder
         clearPaint.value = swizzle_rgba(s.clear_color);
         f.paintData.push_back(clearPaint);
         f.paintAux.push_back(rive_hip_paint_aux{});
+        clipContentID = 0;
+        nextClipID = 1;
+        nextContourID = 1;
+        tessLocation = kMidpointFanPatchSegmentSpan;
+        resPaths = resContours = resTess = 0;
+        simpleRamps.clear();
+        complexRamps.clear();
+        pendingGradPaints.clear();
+    }
+
+    void endFlush()
+    {
+        Flush& f = cur();
+        finishGradients();
+        f.pathCount = (uint32_t)f.pathData.size() - 1;
+        f.pathImage.resize(f.pathCount + 1, 0);
+        f.tessEnd = tessLocation;
+        // Padding spans (render_context.cpp:1550-1568): 8 at 0 and 1 at the end.
+        std::vector<TessVertexSpan> padding;
+        write_padding_span(kMidpointFanPatchSegmentSpan, 0, &padding);
+        write_padding_span(1, tessLocation, &padding);
+        f.spans.insert(f.spans.begin(), padding.begin(), padding.end());
+        f.tessVertexCount = tessLocation + 1;
+    }
+
+    // RiveRenderer::clipAndPushDraw (rive_renderer.cpp:496-570): the batch
+    // goes into the current logical flush when it fits, else the flush ends
+    // and the batch is tried once more in a fresh one (which re-issues every
+    // clip draw); a batch that fits no flush is dropped.
+    template <typename Push>
+    void clipAndPush(size_t height, const IAABB& frame, const Needs& drawNeeds, const Gradient* grad, Push push)
+    {
+        for (int attempt = 0; attempt < 2; ++attempt)
+        {
+            if (batchFits(height, frame, drawNeeds, grad))
+            {
+                const uint32_t clipID = applyClip(height, frame);
+                push(clipID);
+                return;
+            }
+            endFlush();
+            beginFlush();
+        }
+    }
+
+    void build()
+    {
+        images = std::make_shared<std::vector<OracleImage>>(s.image_count);
+        for (uint32_t i = 0; i < s.image_count; ++i)
+            build_image_mips(s.images[i].width, s.images[i].height, s.images[i].rgba_premul, &(*images)[i]);
+        beginFlush();
 
         const IAABB frame{0, 0, (int)s.width, (int)s.height};
         for (uint32_t di = 0; di < s.draw_count; ++di)
@@ -732,7 +906,7 @@ struct Builder
                         break;
                     }
                     clipStack.resize(height);
-                    clipStack.push_back({cp, m, (FillRule)sp.fill_rule, pb, 0});
+                    clipStack.push_back({cp, m, (FillRule)sp.fill_rule, pb, 0, 0});
                 }
                 else
                 {
@@ -749,9 +923,9 @@ struct Builder
             static const V2 kUnitPts[4] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
             if (d.image != 0)
             {
-                if (d.image > f.images.size())
+                if (d.image > images->size())
                     continue;
-                const OracleImage& img = f.images[d.image - 1];
+                const OracleImage& img = (*images)[d.image - 1];
                 Mat scale;
                 scale.m[0] = (float)img.width;
                 scale.m[3] = (float)img.height;
@@ -759,15 +933,18 @@ struct Builder
                 IAABB pixelBounds = map_bounds_round_out(im, kUnitPts, 4, 0, 0);
                 if (pixelBounds.intersect(frame).empty() || pixelBounds.intersect(overall).empty())
                     continue;
-                uint32_t drawClipID = applyClip(height, frame);
                 float op = std::max(0.f, std::min(1.f, std::max(0.f, d.opacity)));
                 uint32_t color = ((uint32_t)std::lround(255.f * op) << 24) | 0x00ffffffu;
                 Mat crInv;
                 const bool hasCR = cr.inverse(&crInv);
-                pushGeometryDraw(kUnitVerbs, 4, kUnitPts, 4, im, 0, Join::miter, Cap::butt, FillRule::nonZero,
-                                 SOLID_COLOR_PAINT_TYPE, swizzle_rgba(color), drawClipID, 0, nullptr,
-                                 shader_blend_mode(d.blend_mode), d.image, d.image_sampler,
-                                 hasCR ? &crInv : nullptr);
+                const Needs needs = measure(kUnitVerbs, 4, kUnitPts, 4, im, 0, Join::miter, Cap::butt,
+                                            FillRule::nonZero);
+                clipAndPush(height, frame, needs, nullptr, [&](uint32_t drawClipID) {
+                    pushGeometryDraw(kUnitVerbs, 4, kUnitPts, 4, im, 0, Join::miter, Cap::butt, FillRule::nonZero,
+                                     SOLID_COLOR_PAINT_TYPE, swizzle_rgba(color), drawClipID, 0, nullptr,
+                                     shader_blend_mode(d.blend_mode), d.image, d.image_sampler,
+                                     hasCR ? &crInv : nullptr);
+                });
                 continue;
             }
             // drawPath (rive_renderer.cpp:121-197)
@@ -810,7 +987,6 @@ struct Builder
             // applyClip (rive_renderer.cpp:631-824)
             if (pixelBounds.intersect(overall).empty())
                 continue;
-            uint32_t drawClipID = applyClip(height, frame);
             Gradient grad;
             const Gradient* gp = nullptr;
             uint32_t paintType = SOLID_COLOR_PAINT_TYPE;
@@ -832,24 +1008,22 @@ struct Builder
                 strokeRadius = fmaxf(d.thickness * .5f, 1.17549435e-38f);
             Mat crInv;
             const bool hasCR = cr.inverse(&crInv);
-            pushPathDraw(d.path, m, strokeRadius, (Join)d.join, (Cap)d.cap,
-                         (FillRule)sp.fill_rule, paintType, paintValue, drawClipID, 0, gp,
-                         shader_blend_mode(d.blend_mode), hasCR ? &crInv : nullptr);
+            const Needs needs = measurePath(d.path, m, strokeRadius, (Join)d.join, (Cap)d.cap, (FillRule)sp.fill_rule);
+            clipAndPush(height, frame, needs, gp, [&](uint32_t drawClipID) {
+                pushPathDraw(d.path, m, strokeRadius, (Join)d.join, (Cap)d.cap, (FillRule)sp.fill_rule, paintType,
+                             paintValue, drawClipID, 0, gp, shader_blend_mode(d.blend_mode),
+                             hasCR ? &crInv : nullptr);
+            });
         }
-        finishGradients();
-        f.pathCount = (uint32_t)f.pathData.size() - 1;
-        f.pathImage.resize(f.pathCount + 1, 0);
-        f.tessEnd = tessLocation;
-        // Padding spans (render_context.cpp:1550-1568): 8 at 0 and 1 at the end.
-        std::vector<TessVertexSpan> padding;
-        write_padding_span(kMidpointFanPatchSegmentSpan, 0, &padding);
-        write_padding_span(1, tessLocation, &padding);
-        f.spans.insert(f.spans.begin(), padding.begin(), padding.end());
-        f.tessVertexCount = tessLocation + 1;
+        endFlush();
     }
 };
 
-void build_flush(const rive_stream& s, Flush* f) { Builder(s, *f).build(); }
+void build_flushes(const rive_stream& s, std::vector<Flush>* out)
+{
+    out->clear();
+    Builder(s, *out).build();
+}
 
 // ======================= color ramps ======================================
 
@@ -1323,13 +1497,148 @@ void sample_gradient(const Flush& f, const rive_hip_paint_aux& aux, const rive_h
         color[k] = (unorm8(tb, k) - unorm8(ta, k)) * fr + unorm8(ta, k);
 }
 
-void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* rgba)
+// Resolves every pixel a path touched, in one band of rows
+// (draw_raster_order_path.frag:14-238).
+void resolve_touched(const Flush& f, const std::vector<uint32_t>& ramps, uint32_t pathID, bool isStroke,
+                     const std::vector<uint32_t>& touched, const std::vector<float>& cov,
+                     std::vector<float>& clipCov, std::vector<uint32_t>& clipIDs, uint32_t* rgba,
+                     uint8_t* edgeMask)
+{
+    const uint32_t W = f.width;
+    const rive_hip_paint_data& paint = f.paintData[pathID];
+    const rive_hip_paint_aux& aux = f.paintAux[pathID];
+    const uint32_t paintType = paint.params & 0xfu;
+    const bool evenOdd = (paint.params & PAINT_FLAG_EVEN_ODD_FILL) != 0u;
+    const bool clockwise = !isStroke && !evenOdd && !(paint.params & PAINT_FLAG_NON_ZERO_FILL);
+    for (uint32_t p : touched)
+    {
+        int x = (int)(p % W), y = (int)(p / W);
+        float count = cov[p];
+        float coverage;
+        if (clockwise)
+            coverage = std::min(std::max(count, 0.f), 1.f);
+        else
+        {
+            coverage = fabsf(count);
+            if (evenOdd)
+            {
+                float h = coverage * .5f;
+                coverage = 1.f - fabsf((h - floorf(h)) * 2.f + -1.f);
+            }
+            coverage = std::min(coverage, 1.f);
+        }
+        if (paintType == CLIP_UPDATE_PAINT_TYPE)
+        {
+            uint32_t clipID = paint.value >> 16;
+            uint32_t outerClipID = paint.params >> 16;
+            if (outerClipID != 0)
+            {
+                float outer = clipIDs[p] == outerClipID ? clipCov[p] : 0.f;
+                coverage = std::min(coverage, outer);
+            }
+            clipCov[p] = round_to_f16(coverage);
+            clipIDs[p] = clipID;
+            continue;
+        }
+        uint32_t clipID = paint.params >> 16;
+        if (clipID != 0)
+            coverage = clipIDs[p] == clipID ? std::min(clipCov[p], coverage) : 0.f;
+        if (paint.params & PAINT_FLAG_HAS_CLIP_RECT)
+            coverage = std::min(std::max(clip_rect_coverage(aux.clip_rect_inverse_matrix, x, y), 0.f), coverage);
+        if (edgeMask != nullptr && coverage > 0.f && coverage < 1.f)
+            edgeMask[p] = 1;
+        const uint32_t blendMode = (paint.params >> 4) & 0xfu;
+        uint32_t dst = rgba[p];
+        float color[4];
+        // Image paints (draw_path.vert:479-493): the paint color times the
+        // image sample; unmultiplied under advanced blend.
+        float imageColor[4] = {1.f, 1.f, 1.f, 1.f};
+        const uint32_t pimg = f.pathImage[pathID];
+        if ((pimg & 0xffffu) != 0)
+        {
+            const float fx = float(x) + .5f, fy = float(y) + .5f;
+            const float u = aux.image_matrix[0] * fx + aux.image_matrix[2] * fy + aux.image_matrix[4];
+            const float v = aux.image_matrix[1] * fx + aux.image_matrix[3] * fy + aux.image_matrix[5];
+            sample_image((*f.images)[(pimg & 0xffffu) - 1], u, v, aux.image_lod, pimg >> 16, imageColor);
+            if (blendMode != BLEND_SRC_OVER)
+            {
+                const float inv = imageColor[3] != 0.f ? 1.f / imageColor[3] : 0.f;
+                for (int k = 0; k < 3; ++k)
+                    imageColor[k] *= inv;
+            }
+        }
+        if (blendMode != BLEND_SRC_OVER)
+        {
+            // draw_raster_order_path.frag:191-204: the paint stays
+            // unpremultiplied, the advanced blend mixes it with the blend
+            // coefficients by the destination alpha, then premultiplies.
+            float src[4];
+            if (paintType == SOLID_COLOR_PAINT_TYPE)
+            {
+                for (int k = 0; k < 4; ++k)
+                    src[k] = unorm8(paint.value, k);
+            }
+            else
+            {
+                sample_gradient(f, aux, paint, ramps, x, y, src);
+            }
+            src[3] *= coverage;
+            for (int k = 0; k < 4; ++k)
+                src[k] *= imageColor[k];
+            const float dp[4] = {unorm8(dst, 0), unorm8(dst, 1), unorm8(dst, 2), unorm8(dst, 3)};
+            float coeffs[3];
+            advanced_blend_coeffs(src, dp, blendMode, coeffs);
+            for (int k = 0; k < 3; ++k)
+                color[k] = (coeffs[k] - src[k]) * dp[3] + src[k];
+            color[3] = src[3];
+            for (int k = 0; k < 3; ++k)
+                color[k] *= color[3];
+        }
+        else if (paintType == SOLID_COLOR_PAINT_TYPE)
+        {
+            float a = unorm8(paint.value, 3);
+            color[0] = unorm8(paint.value, 0) * a;
+            color[1] = unorm8(paint.value, 1) * a;
+            color[2] = unorm8(paint.value, 2) * a;
+            color[3] = a;
+            for (int k = 0; k < 4; ++k)
+                color[k] *= coverage;
+            for (int k = 0; k < 4; ++k)
+                color[k] *= imageColor[k];
+        }
+        else
+        {
+            sample_gradient(f, aux, paint, ramps, x, y, color);
+            color[3] *= coverage;
+            color[0] *= color[3];
+            color[1] *= color[3];
+            color[2] *= color[3];
+        }
+        float paintAlpha = color[3];
+        float out[4];
+        for (int k = 0; k < 4; ++k)
+            out[k] = color[k] + unorm8(dst, k) * (1.f - paintAlpha);
+        if (f.dither && paintAlpha != 0.f)
+        {
+            float n = dither_noise(x, y);
+            out[0] += n;
+            out[1] += n;
+            out[2] += n;
+        }
+        rgba[p] = to_unorm8(out[0]) | (to_unorm8(out[1]) << 8) | (to_unorm8(out[2]) << 16) |
+                  (to_unorm8(out[3]) << 24);
+    }
+}
+
+void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* rgba, uint8_t* edgeMask)
 {
     const uint32_t W = f.width, H = f.height;
     std::vector<uint32_t> ramps;
     render_ramps(f, &ramps);
 
-    // Initial color: clear (premultiplied, gpu.cpp SwizzleRiveColorToRGBAPremul).
+    // Initial color: clear (premultiplied, gpu.cpp SwizzleRiveColorToRGBAPremul),
+    // unless this logical flush loads the target.
+    if (!f.loadPreserve)
     {
         uint32_t c = f.clearColor;
         float a = float(c >> 24) / 255.f;
@@ -1343,11 +1652,17 @@ void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* 
     std::vector<uint32_t> clipIDs((size_t)W * H, 0u);
     std::vector<float> cov((size_t)W * H, 0.f);
     std::vector<uint32_t> stamp((size_t)W * H, 0u);
-    std::vector<uint32_t> touched;
-    touched.reserve((size_t)W * H);
 
     std::array<uint16_t, 72> idx;
     auto patch = make_midpoint_fan_patch(&idx);
+
+    // Rows are split into bands, one thread per band at a time: a pixel's
+    // triangles still accumulate in submission order, so the result does not
+    // depend on the thread count.
+    const int bands = std::max(1, std::min((int)H, 2 * omp_get_max_threads()));
+    std::vector<std::vector<uint32_t>> touchedBands(bands);
+    std::vector<OutVertex> ov;
+    std::vector<int> instRows;
 
     for (const DrawInfo& di : f.draws)
     {
@@ -1355,187 +1670,120 @@ void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* 
             continue;
         const uint32_t pathID = di.pathID;
         const bool isStroke = di.isStroke;
-        touched.clear();
         const int firstInstance = (int)(di.tessLocation / 8);
         const int endInstance = (int)((di.tessLocation + di.tessVertexCount) / 8);
-        for (int inst = firstInstance; inst < endInstance; ++inst)
+        const int nInst = endInstance - firstInstance;
+        ov.resize((size_t)nInst * 42);
+        instRows.resize((size_t)nInst * 2);
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < nInst; ++k)
         {
-            OutVertex ov[42];
+            int64_t lo = INT64_MAX, hi = INT64_MIN;
             for (int v = 0; v < 42; ++v)
-                ov[v] = draw_vertex(f, tex, patch[v], inst);
-            for (int t = 0; t < 24; ++t)
             {
-                const OutVertex& A = ov[idx[t * 3 + 0]];
-                const OutVertex& B = ov[idx[t * 3 + 1]];
-                const OutVertex& C = ov[idx[t * 3 + 2]];
-                if (A.discard || B.discard || C.discard)
-                    continue;
-                FxP a{snap(A.x), snap(A.y)}, b{snap(B.x), snap(B.y)}, c{snap(C.x), snap(C.y)};
-                int64_t area = edgeD(a, b, c);
-                if (area <= 0)
-                    continue; // counterclockwise or degenerate: culled
-                int64_t minx = std::min({a.x, b.x, c.x}), maxx = std::max({a.x, b.x, c.x});
-                int64_t miny = std::min({a.y, b.y, c.y}), maxy = std::max({a.y, b.y, c.y});
-                int x0 = (int)std::max<int64_t>(0, (minx - 128) >> 8);
-                int x1 = (int)std::min<int64_t>((int64_t)W - 1, (maxx - 128 + 255) >> 8);
-                int y0 = (int)std::max<int64_t>(0, (miny - 128) >> 8);
-                int y1 = (int)std::min<int64_t>((int64_t)H - 1, (maxy - 128 + 255) >> 8);
-                const double inv = 1.0 / (double)area;
-                for (int y = y0; y <= y1; ++y)
+                const OutVertex o = draw_vertex(f, tex, patch[v], firstInstance + k);
+                ov[(size_t)k * 42 + v] = o;
+                if (!o.discard)
                 {
-                    for (int x = x0; x <= x1; ++x)
+                    const int64_t y = snap(o.y);
+                    lo = std::min(lo, y);
+                    hi = std::max(hi, y);
+                }
+            }
+            instRows[2 * k] = lo <= hi ? (int)std::max<int64_t>(0, (lo - 128) >> 8) : 1;
+            instRows[2 * k + 1] = lo <= hi ? (int)std::min<int64_t>((int64_t)H - 1, (hi - 128 + 255) >> 8) : 0;
+        }
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int band = 0; band < bands; ++band)
+        {
+            const int yb0 = (int)((int64_t)band * H / bands), yb1 = (int)((int64_t)(band + 1) * H / bands) - 1;
+            std::vector<uint32_t>& touched = touchedBands[band];
+            touched.clear();
+            for (int k = 0; k < nInst; ++k)
+            {
+                if (instRows[2 * k] > yb1 || instRows[2 * k + 1] < yb0)
+                    continue;
+                const OutVertex* iv = &ov[(size_t)k * 42];
+                for (int t = 0; t < 24; ++t)
+                {
+                    const OutVertex& A = iv[idx[t * 3 + 0]];
+                    const OutVertex& B = iv[idx[t * 3 + 1]];
+                    const OutVertex& C = iv[idx[t * 3 + 2]];
+                    if (A.discard || B.discard || C.discard)
+                        continue;
+                    FxP a{snap(A.x), snap(A.y)}, b{snap(B.x), snap(B.y)}, c{snap(C.x), snap(C.y)};
+                    int64_t area = edgeD(a, b, c);
+                    if (area <= 0)
+                        continue; // counterclockwise or degenerate: culled
+                    int64_t minx = std::min({a.x, b.x, c.x}), maxx = std::max({a.x, b.x, c.x});
+                    int64_t miny = std::min({a.y, b.y, c.y}), maxy = std::max({a.y, b.y, c.y});
+                    int x0 = (int)std::max<int64_t>(0, (minx - 128) >> 8);
+                    int x1 = (int)std::min<int64_t>((int64_t)W - 1, (maxx - 128 + 255) >> 8);
+                    int y0 = (int)std::max<int64_t>(yb0, (miny - 128) >> 8);
+                    int y1 = (int)std::min<int64_t>(yb1, (maxy - 128 + 255) >> 8);
+                    const double inv = 1.0 / (double)area;
+                    for (int y = y0; y <= y1; ++y)
                     {
-                        FxP q{(int64_t)x * 256 + 128, (int64_t)y * 256 + 128};
-                        if (sos_sign(a, b, q) <= 0 || sos_sign(b, c, q) <= 0 ||
-                            sos_sign(c, a, q) <= 0)
-                            continue;
-                        double la = (double)edgeD(b, c, q) * inv;
-                        double lb = (double)edgeD(c, a, q) * inv;
-                        double lc = (double)edgeD(a, b, q) * inv;
-                        size_t p = (size_t)y * W + x;
-                        float fcx = (float)(la * A.cx + lb * B.cx + lc * C.cx);
-                        float count = stamp[p] == pathID ? cov[p] : 0.f;
-                        if (stamp[p] != pathID)
+                        // Candidate columns: the row's crossings of the
+                        // triangle's edges, widened by 2 px (the exact
+                        // tests below decide; this only skips pixels far
+                        // outside a thin triangle's bounding box).
+                        int cx0 = x0, cx1 = x1;
                         {
-                            stamp[p] = pathID;
-                            touched.push_back((uint32_t)p);
+                            const double yc = (double)y * 256 + 128;
+                            double lo = INFINITY, hi = -INFINITY;
+                            int crossings = 0;
+                            const FxP* vs[3] = {&a, &b, &c};
+                            for (int e = 0; e < 3; ++e)
+                            {
+                                const FxP& p0 = *vs[e];
+                                const FxP& p1 = *vs[(e + 1) % 3];
+                                if (p0.y == p1.y || yc < (double)std::min(p0.y, p1.y) ||
+                                    yc > (double)std::max(p0.y, p1.y))
+                                    continue;
+                                const double xe = (double)p0.x + (yc - (double)p0.y) *
+                                                                      (double)(p1.x - p0.x) / (double)(p1.y - p0.y);
+                                lo = std::min(lo, xe);
+                                hi = std::max(hi, xe);
+                                ++crossings;
+                            }
+                            if (crossings >= 2)
+                            {
+                                cx0 = std::max(x0, (int)std::floor((lo - 128) / 256) - 2);
+                                cx1 = std::min(x1, (int)std::ceil((hi - 128) / 256) + 2);
+                            }
                         }
-                        if (isStroke)
+                        for (int x = cx0; x <= cx1; ++x)
                         {
-                            float fcy = (float)(la * A.cy + lb * B.cy + lc * C.cy);
-                            count = std::max(std::min(fcx, fcy), count);
+                            FxP q{(int64_t)x * 256 + 128, (int64_t)y * 256 + 128};
+                            if (sos_sign(a, b, q) <= 0 || sos_sign(b, c, q) <= 0 || sos_sign(c, a, q) <= 0)
+                                continue;
+                            double la = (double)edgeD(b, c, q) * inv;
+                            double lb = (double)edgeD(c, a, q) * inv;
+                            double lc = (double)edgeD(a, b, q) * inv;
+                            size_t p = (size_t)y * W + x;
+                            float fcx = (float)(la * A.cx + lb * B.cx + lc * C.cx);
+                            float count = stamp[p] == pathID ? cov[p] : 0.f;
+                            if (stamp[p] != pathID)
+                            {
+                                stamp[p] = pathID;
+                                touched.push_back((uint32_t)p);
+                            }
+                            if (isStroke)
+                            {
+                                float fcy = (float)(la * A.cy + lb * B.cy + lc * C.cy);
+                                count = std::max(std::min(fcx, fcy), count);
+                            }
+                            else
+                            {
+                                count = count + fcx;
+                            }
+                            cov[p] = round_to_f16(count);
                         }
-                        else
-                        {
-                            count = count + fcx;
-                        }
-                        cov[p] = round_to_f16(count);
                     }
                 }
             }
-        }
-
-        // Resolve every pixel this path touched (draw_raster_order_path.frag:14-238).
-        const rive_hip_paint_data& paint = f.paintData[pathID];
-        const rive_hip_paint_aux& aux = f.paintAux[pathID];
-        const uint32_t paintType = paint.params & 0xfu;
-        const bool evenOdd = (paint.params & PAINT_FLAG_EVEN_ODD_FILL) != 0u;
-        const bool clockwise = !isStroke && !evenOdd && !(paint.params & PAINT_FLAG_NON_ZERO_FILL);
-        for (uint32_t p : touched)
-        {
-            int x = (int)(p % W), y = (int)(p / W);
-            float count = cov[p];
-            float coverage;
-            if (clockwise)
-                coverage = std::min(std::max(count, 0.f), 1.f);
-            else
-            {
-                coverage = fabsf(count);
-                if (evenOdd)
-                {
-                    float h = coverage * .5f;
-                    coverage = 1.f - fabsf((h - floorf(h)) * 2.f + -1.f);
-                }
-                coverage = std::min(coverage, 1.f);
-            }
-            if (paintType == CLIP_UPDATE_PAINT_TYPE)
-            {
-                uint32_t clipID = paint.value >> 16;
-                uint32_t outerClipID = paint.params >> 16;
-                if (outerClipID != 0)
-                {
-                    float outer = clipIDs[p] == outerClipID ? clipCov[p] : 0.f;
-                    coverage = std::min(coverage, outer);
-                }
-                clipCov[p] = round_to_f16(coverage);
-                clipIDs[p] = clipID;
-                continue;
-            }
-            uint32_t clipID = paint.params >> 16;
-            if (clipID != 0)
-                coverage = clipIDs[p] == clipID ? std::min(clipCov[p], coverage) : 0.f;
-            if (paint.params & PAINT_FLAG_HAS_CLIP_RECT)
-                coverage = std::min(std::max(clip_rect_coverage(aux.clip_rect_inverse_matrix, x, y), 0.f), coverage);
-            const uint32_t blendMode = (paint.params >> 4) & 0xfu;
-            uint32_t dst = rgba[p];
-            float color[4];
-            // Image paints (draw_path.vert:479-493): the paint color times the
-            // image sample; unmultiplied under advanced blend.
-            float imageColor[4] = {1.f, 1.f, 1.f, 1.f};
-            const uint32_t pimg = f.pathImage[pathID];
-            if ((pimg & 0xffffu) != 0)
-            {
-                const float fx = float(x) + .5f, fy = float(y) + .5f;
-                const float u = aux.image_matrix[0] * fx + aux.image_matrix[2] * fy + aux.image_matrix[4];
-                const float v = aux.image_matrix[1] * fx + aux.image_matrix[3] * fy + aux.image_matrix[5];
-                sample_image(f.images[(pimg & 0xffffu) - 1], u, v, aux.image_lod, pimg >> 16, imageColor);
-                if (blendMode != BLEND_SRC_OVER)
-                {
-                    const float inv = imageColor[3] != 0.f ? 1.f / imageColor[3] : 0.f;
-                    for (int k = 0; k < 3; ++k)
-                        imageColor[k] *= inv;
-                }
-            }
-            if (blendMode != BLEND_SRC_OVER)
-            {
-                // draw_raster_order_path.frag:191-204: the paint stays
-                // unpremultiplied, the advanced blend mixes it with the blend
-                // coefficients by the destination alpha, then premultiplies.
-                float src[4];
-                if (paintType == SOLID_COLOR_PAINT_TYPE)
-                {
-                    for (int k = 0; k < 4; ++k)
-                        src[k] = unorm8(paint.value, k);
-                }
-                else
-                {
-                    sample_gradient(f, aux, paint, ramps, x, y, src);
-                }
-                src[3] *= coverage;
-                for (int k = 0; k < 4; ++k)
-                    src[k] *= imageColor[k];
-                const float dp[4] = {unorm8(dst, 0), unorm8(dst, 1), unorm8(dst, 2), unorm8(dst, 3)};
-                float coeffs[3];
-                advanced_blend_coeffs(src, dp, blendMode, coeffs);
-                for (int k = 0; k < 3; ++k)
-                    color[k] = (coeffs[k] - src[k]) * dp[3] + src[k];
-                color[3] = src[3];
-                for (int k = 0; k < 3; ++k)
-                    color[k] *= color[3];
-            }
-            else if (paintType == SOLID_COLOR_PAINT_TYPE)
-            {
-                float a = unorm8(paint.value, 3);
-                color[0] = unorm8(paint.value, 0) * a;
-                color[1] = unorm8(paint.value, 1) * a;
-                color[2] = unorm8(paint.value, 2) * a;
-                color[3] = a;
-                for (int k = 0; k < 4; ++k)
-                    color[k] *= coverage;
-                for (int k = 0; k < 4; ++k)
-                    color[k] *= imageColor[k];
-            }
-            else
-            {
-                sample_gradient(f, aux, paint, ramps, x, y, color);
-                color[3] *= coverage;
-                color[0] *= color[3];
-                color[1] *= color[3];
-                color[2] *= color[3];
-            }
-            float paintAlpha = color[3];
-            float out[4];
-            for (int k = 0; k < 4; ++k)
-                out[k] = color[k] + unorm8(dst, k) * (1.f - paintAlpha);
-            if (f.dither && paintAlpha != 0.f)
-            {
-                float n = dither_noise(x, y);
-                out[0] += n;
-                out[1] += n;
-                out[2] += n;
-            }
-            rgba[p] = to_unorm8(out[0]) | (to_unorm8(out[1]) << 8) | (to_unorm8(out[2]) << 16) |
-                      (to_unorm8(out[3]) << 24);
+            resolve_touched(f, ramps, pathID, isStroke, touched, cov, clipCov, clipIDs, rgba, edgeMask);
         }
     }
 }

@@ -111,6 +111,26 @@ struct rive_hip_context
     unsigned char* scratchPtr = nullptr;
     Arena arena;
     bool lastFlushNoAlloc = false;
+    // The current (or last) flush's records: device pointers at the
+    // descriptor's first_* offsets, and the matching host staging.
+    struct FlushInputs
+    {
+        const rivehip::PathDataV* paths = nullptr;
+        const uint2* paints = nullptr;
+        const float* aux = nullptr;
+        const uint4* gradSpans = nullptr;
+        rive_hip_draw* draws = nullptr;
+        const rivehip::SpanV* spans = nullptr;
+        const rivehip::ContourV* contours = nullptr;
+        const rive_hip_paint_data* paintsHost = nullptr;
+        const rive_hip_grad_span* gradSpansHost = nullptr;
+    } in;
+    // The descriptors of the last frame's logical flushes (a frame starts at
+    // a flush whose records start at offset 0), with the image and canvas
+    // tables they point at copied here.
+    std::vector<rive_hip_flush_desc> frameDescs;
+    std::vector<std::vector<rive_hip_image*>> frameImages;
+    std::vector<std::vector<rive_hip_canvas>> frameCanvases;
 };
 
 namespace
@@ -566,9 +586,9 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
                            hipMemcpyHostToDevice, ctx->stream));
     uint32_t* scal = P<uint32_t>(ctx->scalars);
     const uint32_t pathCount = d->path_count;
-    const PathDataV* paths = reinterpret_cast<const PathDataV*>(ctx->bufs[RIVE_HIP_BUFFER_PATH].dev);
-    const uint2* paints = reinterpret_cast<const uint2*>(ctx->bufs[RIVE_HIP_BUFFER_PAINT].dev);
-    const float* aux = reinterpret_cast<const float*>(ctx->bufs[RIVE_HIP_BUFFER_PAINT_AUX].dev);
+    const PathDataV* paths = ctx->in.paths;
+    const uint2* paints = ctx->in.paints;
+    const float* aux = ctx->in.aux;
     const uint32_t gradRows = std::max<uint32_t>(d->grad_simple_rows + d->grad_complex_rows, 1);
 
     // ---- color ramps
@@ -578,7 +598,7 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     {
         HIP_TRY(hipMemsetAsync(ctx->ramps.ptr, 0, (size_t)gradRows * kGradWidth * 4, ctx->stream));
         ramp_kernel<<<d->grad_span_count, 64, 0, ctx->stream>>>(
-            reinterpret_cast<const uint4*>(ctx->bufs[RIVE_HIP_BUFFER_GRAD_SPAN].dev),
+            ctx->in.gradSpans,
             d->grad_span_count, gradRows, P<uint32_t>(ctx->ramps));
     }
     ctx->lastGradRows = gradRows;
@@ -848,12 +868,40 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d)
     if (d->mode == RIVE_HIP_FLUSH_SPANS && d->tess_vertex_count > kMaxFlushTexels)
         return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "tess_vertex_count above 2^31");
     const size_t pathRecords = (size_t)d->path_count + 1;
-    if (!need(RIVE_HIP_BUFFER_PATH, pathRecords * sizeof(rive_hip_path_data)) ||
-        !need(RIVE_HIP_BUFFER_PAINT, pathRecords * sizeof(rive_hip_paint_data)) ||
-        !need(RIVE_HIP_BUFFER_PAINT_AUX, pathRecords * sizeof(rive_hip_paint_aux)) ||
-        !need(RIVE_HIP_BUFFER_GRAD_SPAN, (size_t)d->grad_span_count * sizeof(rive_hip_grad_span)) ||
-        !need(RIVE_HIP_BUFFER_DRAW, (size_t)d->draw_count * sizeof(rive_hip_draw)))
+    // Each record run [first, first + count) must lie inside its buffer; the
+    // offsets are bounded first so the products cannot wrap.
+    const uint64_t kMaxOffset = (uint64_t)1 << 40;
+    if (d->first_path > kMaxOffset || d->first_paint > kMaxOffset || d->first_paint_aux > kMaxOffset ||
+        d->first_contour > kMaxOffset || d->first_grad_span > kMaxOffset || d->first_tess_span > kMaxOffset ||
+        d->first_draw > kMaxOffset)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "record offset above 2^40");
+    if (!need(RIVE_HIP_BUFFER_PATH, (d->first_path + pathRecords) * sizeof(rive_hip_path_data)) ||
+        !need(RIVE_HIP_BUFFER_PAINT, (d->first_paint + pathRecords) * sizeof(rive_hip_paint_data)) ||
+        !need(RIVE_HIP_BUFFER_PAINT_AUX, (d->first_paint_aux + pathRecords) * sizeof(rive_hip_paint_aux)) ||
+        !need(RIVE_HIP_BUFFER_GRAD_SPAN, (d->first_grad_span + d->grad_span_count) * sizeof(rive_hip_grad_span)) ||
+        !need(RIVE_HIP_BUFFER_DRAW, (d->first_draw + d->draw_count) * sizeof(rive_hip_draw)))
         return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "buffers smaller than the flush descriptor");
+    {
+        auto dev = [&](int kind, uint64_t first, size_t recordBytes) -> unsigned char* {
+            auto* base = static_cast<unsigned char*>(ctx->bufs[kind].dev);
+            return base == nullptr ? nullptr : base + first * recordBytes;
+        };
+        auto host = [&](int kind, uint64_t first, size_t recordBytes) -> const unsigned char* {
+            auto* base = static_cast<const unsigned char*>(ctx->bufs[kind].host);
+            return base == nullptr ? nullptr : base + first * recordBytes;
+        };
+        auto& in = ctx->in;
+        in.paths = reinterpret_cast<const PathDataV*>(dev(RIVE_HIP_BUFFER_PATH, d->first_path, 64));
+        in.paints = reinterpret_cast<const uint2*>(dev(RIVE_HIP_BUFFER_PAINT, d->first_paint, 8));
+        in.aux = reinterpret_cast<const float*>(dev(RIVE_HIP_BUFFER_PAINT_AUX, d->first_paint_aux, 128));
+        in.gradSpans = reinterpret_cast<const uint4*>(dev(RIVE_HIP_BUFFER_GRAD_SPAN, d->first_grad_span, 16));
+        in.draws = reinterpret_cast<rive_hip_draw*>(dev(RIVE_HIP_BUFFER_DRAW, d->first_draw, sizeof(rive_hip_draw)));
+        in.spans = reinterpret_cast<const SpanV*>(dev(RIVE_HIP_BUFFER_TESS_SPAN, d->first_tess_span, 64));
+        in.contours = reinterpret_cast<const ContourV*>(dev(RIVE_HIP_BUFFER_CONTOUR, d->first_contour, 16));
+        in.paintsHost = reinterpret_cast<const rive_hip_paint_data*>(host(RIVE_HIP_BUFFER_PAINT, d->first_paint, 8));
+        in.gradSpansHost =
+            reinterpret_cast<const rive_hip_grad_span*>(host(RIVE_HIP_BUFFER_GRAD_SPAN, d->first_grad_span, 16));
+    }
     RIVE_TRY(ensure(ctx, ctx->scalars, 64));
     RIVE_TRY(ensure(ctx, ctx->drawTess, (size_t)std::max<uint32_t>(d->draw_count, 1) * sizeof(DrawTess)));
     HIP_TRY(hipMemsetAsync(ctx->scalars.ptr, 0, 64, ctx->stream));
@@ -865,16 +913,16 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d)
     uint32_t tessRows;
     if (d->mode == RIVE_HIP_FLUSH_SPANS)
     {
-        if (!need(RIVE_HIP_BUFFER_TESS_SPAN, (size_t)d->tess_span_count * sizeof(rive_hip_tess_span)) ||
-            !need(RIVE_HIP_BUFFER_CONTOUR, (size_t)d->contour_count * sizeof(rive_hip_contour)))
+        if (!need(RIVE_HIP_BUFFER_TESS_SPAN, (d->first_tess_span + d->tess_span_count) * sizeof(rive_hip_tess_span)) ||
+            !need(RIVE_HIP_BUFFER_CONTOUR, (d->first_contour + d->contour_count) * sizeof(rive_hip_contour)))
             return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "span/contour buffers too small");
         // Draws with texels index ContourData records (every tessellated
         // vertex names its contour), so a SPANS flush needs them.
         if (d->draw_count > 0 && d->tess_span_count > 0 && d->contour_count == 0)
             return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "SPANS flush with spans and draws but no contours");
-        spans = reinterpret_cast<const SpanV*>(ctx->bufs[RIVE_HIP_BUFFER_TESS_SPAN].dev);
+        spans = ctx->in.spans;
         spanCount = d->tess_span_count;
-        contours = reinterpret_cast<const ContourV*>(ctx->bufs[RIVE_HIP_BUFFER_CONTOUR].dev);
+        contours = ctx->in.contours;
         tessRows = std::max<uint32_t>((d->tess_vertex_count + kTessTextureWidth - 1) / kTessTextureWidth, 1);
         ctx->lastContourCount = d->contour_count;
         HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream)); // no counting pass
@@ -919,7 +967,7 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d)
     RIVE_TRY(ensure(ctx, ctx->pathImage, (size_t)(d->path_count + 1) * 4));
     HIP_TRY(hipMemsetAsync(ctx->pathImage.ptr, 0, (size_t)(d->path_count + 1) * 4, ctx->stream));
     draw_tess_kernel<<<1, 1024, 0, ctx->stream>>>(
-        reinterpret_cast<const rive_hip_draw*>(ctx->bufs[RIVE_HIP_BUFFER_DRAW].dev), d->draw_count,
+        ctx->in.draws, d->draw_count,
         d->path_count, tessRows * kTessTextureWidth, d->image_count, P<DrawTess>(ctx->drawTess),
         P<uint32_t>(ctx->pathImage), scal + 3, scal + 4);
     HIP_TRY(hipGetLastError());
@@ -955,13 +1003,13 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d)
         st.first_path_tess_location = d->first_path_tess_location;
         st.clear_color = canvases[0].clear_color;
         st.dither = d->dither;
-        const auto* gradHost = static_cast<const rive_hip_grad_span*>(ctx->bufs[RIVE_HIP_BUFFER_GRAD_SPAN].host);
+        const rive_hip_grad_span* gradHost = ctx->in.gradSpansHost;
         if (gradHost != nullptr)
             for (uint32_t i = 0; i < d->grad_span_count; ++i)
                 st.grad_simple_ramps += (gradHost[i].y_with_flags & 0x20000000u) == 0u ? 1u : 0u;
         // Gradient paints read PaintAuxData; the host staging copy of the
         // PAINT buffer holds this flush's records.
-        const auto* paintsHost = static_cast<const rive_hip_paint_data*>(ctx->bufs[RIVE_HIP_BUFFER_PAINT].host);
+        const rive_hip_paint_data* paintsHost = ctx->in.paintsHost;
         if (paintsHost != nullptr)
             for (uint32_t i = 1; i <= d->path_count; ++i)
             {
@@ -983,7 +1031,41 @@ extern "C" int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* 
 {
     if (ctx != nullptr)
         ctx->lastFlushNoAlloc = false;
-    return flush_impl(ctx, d);
+    const int rc = flush_impl(ctx, d);
+    if (rc == RIVE_HIP_OK)
+    {
+        // Copy the tables first: d may point into the recorded frame itself
+        // (a frame re-issued from rive_hip_last_frame).
+        const rive_hip_flush_desc desc = *d;
+        std::vector<rive_hip_image*> images(d->images, d->images + (d->images ? d->image_count : 0));
+        std::vector<rive_hip_canvas> canvases(d->canvases, d->canvases + (d->canvases ? d->canvas_count : 0));
+        if (d->first_path == 0 && d->first_draw == 0 && d->first_tess_span == 0 && d->first_contour == 0)
+        {
+            ctx->frameDescs.clear();
+            ctx->frameImages.clear();
+            ctx->frameCanvases.clear();
+        }
+        ctx->frameDescs.push_back(desc);
+        ctx->frameImages.push_back(std::move(images));
+        ctx->frameCanvases.push_back(std::move(canvases));
+        for (size_t k = 0; k < ctx->frameDescs.size(); ++k)
+        {
+            ctx->frameDescs[k].images = ctx->frameImages[k].empty() ? nullptr : ctx->frameImages[k].data();
+            ctx->frameDescs[k].canvases = ctx->frameCanvases[k].empty() ? nullptr : ctx->frameCanvases[k].data();
+        }
+    }
+    return rc;
+}
+
+extern "C" int rive_hip_last_frame(rive_hip_context* ctx, rive_hip_flush_desc* out, uint32_t capacity,
+                                   uint32_t* count)
+{
+    if (ctx == nullptr || count == nullptr || (out == nullptr && capacity != 0))
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+    *count = (uint32_t)ctx->frameDescs.size();
+    for (uint32_t k = 0; k < std::min(capacity, *count); ++k)
+        out[k] = ctx->frameDescs[k];
+    return RIVE_HIP_OK;
 }
 
 extern "C" int rive_hip_flush_noalloc(rive_hip_context* ctx, const rive_hip_flush_desc* d)
@@ -1052,11 +1134,11 @@ extern "C" int rive_hip_debug_readback(rive_hip_context* ctx, int what, void* ds
     switch (what)
     {
         case 0: // TessVertexSpans consumed by the last flush
-            src = geo ? ctx->geoSpans.ptr : ctx->bufs[RIVE_HIP_BUFFER_TESS_SPAN].dev;
+            src = geo ? ctx->geoSpans.ptr : (const void*)ctx->in.spans;
             bytes = (size_t)ctx->lastSpanCount * sizeof(rive_hip_tess_span);
             break;
         case 1: // ContourData
-            src = geo ? ctx->geoContours.ptr : ctx->bufs[RIVE_HIP_BUFFER_CONTOUR].dev;
+            src = geo ? ctx->geoContours.ptr : (const void*)ctx->in.contours;
             bytes = (size_t)ctx->lastContourCount * sizeof(rive_hip_contour);
             break;
         case 2: // tessellation texture
@@ -1064,7 +1146,7 @@ extern "C" int rive_hip_debug_readback(rive_hip_context* ctx, int what, void* ds
             bytes = (size_t)ctx->lastTexels * sizeof(TessTexel);
             break;
         case 3: // rive_hip_draw records (GEOMETRY mode: with device-written ranges)
-            src = ctx->bufs[RIVE_HIP_BUFFER_DRAW].dev;
+            src = ctx->in.draws;
             bytes = (size_t)ctx->lastDrawCount * sizeof(rive_hip_draw);
             break;
         case 4: // gradient ramps

@@ -1067,7 +1067,7 @@ int geometry_counting(rive_hip_context* ctx, const rive_hip_flush_desc* d, const
 {
     using namespace rivehip;
     const uint32_t n = d->draw_count;
-    auto* draws = reinterpret_cast<rive_hip_draw*>(ctx->bufs[RIVE_HIP_BUFFER_DRAW].dev);
+    auto* draws = ctx->in.draws;
     const auto* verbs = reinterpret_cast<const uint8_t*>(ctx->bufs[RIVE_HIP_BUFFER_VERB].dev);
     const auto* points = reinterpret_cast<const V2d*>(ctx->bufs[RIVE_HIP_BUFFER_POINT].dev);
     if (n > 0 && (verbs == nullptr || points == nullptr))
@@ -1159,3 +1159,42 @@ int geometry_counting(rive_hip_context* ctx, const rive_hip_flush_desc* d, const
     return RIVE_HIP_OK;
 }
 } // namespace
+
+extern "C" int rive_hip_count_draws(rive_hip_context* ctx, uint32_t n, uint32_t* tessOut, uint32_t* contourOut)
+{
+    using namespace rivehip;
+    if (ctx == nullptr || (n > 0 && (tessOut == nullptr || contourOut == nullptr)))
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+    if (n == 0)
+        return RIVE_HIP_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    for (int k = 0; k < RIVE_HIP_BUFFER_KIND_COUNT; ++k)
+        if (ctx->bufs[k].mapped)
+            return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "count with a mapped buffer");
+    if (ctx->bufs[RIVE_HIP_BUFFER_DRAW].size < (size_t)n * sizeof(rive_hip_draw))
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "DRAW buffer smaller than draw_count records");
+    const auto* verbs = reinterpret_cast<const uint8_t*>(ctx->bufs[RIVE_HIP_BUFFER_VERB].dev);
+    const auto* points = reinterpret_cast<const V2d*>(ctx->bufs[RIVE_HIP_BUFFER_POINT].dev);
+    if (verbs == nullptr || points == nullptr)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "count without VERB/POINT buffers");
+    RIVE_TRY(ensure(ctx, ctx->scalars, 64));
+    RIVE_TRY(ensure(ctx, ctx->geoCounts, (size_t)(3 * (size_t)(n + 1) + 16) * 4));
+    uint32_t* tessCounts = P<uint32_t>(ctx->geoCounts);
+    uint32_t* contourCounts = tessCounts + (n + 1);
+    uint32_t* recordCounts = contourCounts + (n + 1);
+    uint32_t* scal = P<uint32_t>(ctx->scalars);
+    HIP_TRY(hipMemsetAsync(scal + 12, 0, 4, ctx->stream));
+    count_kernel<<<(n + 31) / 32, 32, 0, ctx->stream>>>(
+        reinterpret_cast<const rive_hip_draw*>(ctx->bufs[RIVE_HIP_BUFFER_DRAW].dev), n, verbs,
+        ctx->bufs[RIVE_HIP_BUFFER_VERB].size, points, ctx->bufs[RIVE_HIP_BUFFER_POINT].size / sizeof(V2d),
+        scal + 12, tessCounts, contourCounts, recordCounts);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(ctx->hostScalars, scal + 12, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(tessOut, tessCounts, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(contourOut, contourCounts, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->hostScalars[0] != 0)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT,
+                    "draw record with verb/point ranges outside the VERB/POINT buffers");
+    return RIVE_HIP_OK;
+}

@@ -4,6 +4,7 @@
 #include "rive_renderer.hpp"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <utility>
 
@@ -379,7 +380,7 @@ void RiveRenderer::clipPathImpl(RiveRenderPath* path)
         if (st.overallClipPixelBounds.empty())
             return;
         m_clipStack.resize(h);
-        m_clipStack.push_back({st.matrix, path, path->getFillRule(), pb, 0});
+        m_clipStack.push_back({st.matrix, path, path->getFillRule(), pb, m_context->newClipSerial()});
     }
     else
     {
@@ -430,32 +431,18 @@ void RiveRenderer::drawPath(RenderPath* renderPath, RenderPaint* renderPaint)
         return;
     if (pixelBounds.intersect(st.overallClipPixelBounds).empty())
         return;
-    uint32_t clipID = 0;
-    const size_t h = st.clipStackHeight;
-    if (h > 0)
+    // The clip stack the draw sits under; RenderContext issues its clip
+    // draws and IDs per logical flush (applyClip).
+    std::vector<gpu::ClipRef> clips;
+    clips.reserve(st.clipStackHeight);
+    for (size_t i = 0; i < st.clipStackHeight; ++i)
     {
-        size_t inBuffer = (size_t)-1;
-        if (m_context->getClipContentID() != 0)
-            for (size_t i = h - 1; i != (size_t)-1; --i)
-                if (m_clipStack[i].clipID == m_context->getClipContentID())
-                {
-                    inBuffer = i;
-                    break;
-                }
-        uint32_t parent = inBuffer == (size_t)-1 ? 0 : m_clipStack[inBuffer].clipID;
-        for (size_t i = inBuffer + 1; i < h; ++i)
-        {
-            ClipElement& ce = m_clipStack[i];
-            ce.clipID = m_context->generateClipID();
-            if (!ce.pixelBounds.intersect(frame).empty())
-                m_context->pushPathDraw(ce.path, ce.matrix, nullptr, ce.clipID, true, parent, ce.fillRule);
-            parent = ce.clipID;
-        }
-        clipID = parent;
-        m_context->setClipContentID(parent);
+        const ClipElement& ce = m_clipStack[i];
+        clips.push_back({ce.serial, ce.path, ce.matrix, ce.fillRule, !ce.pixelBounds.intersect(frame).empty()});
     }
-    m_context->pushPathDraw(path, st.matrix, paint, clipID, false, 0, path->getFillRule(),
-                            st.hasClipRect ? &st.clipRectInverse : nullptr);
+    m_context->pushDrawBatch(std::move(clips),
+                             m_context->makeRecord(path, st.matrix, paint, 0, false, 0, path->getFillRule(),
+                                                   st.hasClipRect ? &st.clipRectInverse : nullptr));
 }
 
 namespace gpu
@@ -501,23 +488,50 @@ float polar_segments_per_radian(float r)
 void RenderContext::beginFrame(const FrameDescriptor& fd)
 {
     m_frame = fd;
-    m_draws.clear();
+    m_batches.clear();
     m_canvases.clear();
-    m_lastClipID = 0;
-    m_clipContentID = 0;
+    m_provLastClipID = 0;
+    m_provContentSerial = 0;
+    m_provClipIDs.clear();
 }
 
 void RenderContext::beginCanvas(const FrameDescriptor& fd, rive_hip_target* target)
 {
     m_frame = fd;
-    m_canvases.push_back({target, m_draws.size(), fd});
-    m_lastClipID = 0;
-    m_clipContentID = 0;
+    m_canvases.push_back({target, m_batches.size(), fd});
+    m_provLastClipID = 0;
+    m_provContentSerial = 0;
+    m_provClipIDs.clear();
 }
 
-void RenderContext::pushPathDraw(const RiveRenderPath* path, const Mat2D& m, const RiveRenderPaint* paint,
-                                 uint32_t clipID, bool clipUpdate, uint32_t outerClipID, FillRule fillRule,
-                                 const Mat2D* clipRectInverse)
+void RenderContext::pushDrawBatch(std::vector<ClipRef> clips, const DrawRecord& draw)
+{
+    size_t first = 0;
+    if (m_provContentSerial != 0)
+        for (size_t i = clips.size(); i-- > 0;)
+            if (clips[i].serial == m_provContentSerial)
+            {
+                first = i + 1;
+                break;
+            }
+    for (size_t i = first; i < clips.size(); ++i)
+        m_provClipIDs.push_back({clips[i].serial, ++m_provLastClipID});
+    if (!clips.empty())
+        m_provContentSerial = clips.back().serial;
+    m_batches.push_back({std::move(clips), draw});
+}
+
+uint32_t RenderContext::getClipContentID() const
+{
+    for (auto it = m_provClipIDs.rbegin(); it != m_provClipIDs.rend(); ++it)
+        if (it->first == m_provContentSerial)
+            return it->second;
+    return 0;
+}
+
+DrawRecord RenderContext::makeRecord(const RiveRenderPath* path, const Mat2D& m, const RiveRenderPaint* paint,
+                                     uint32_t clipID, bool clipUpdate, uint32_t outerClipID, FillRule fillRule,
+                                     const Mat2D* clipRectInverse) const
 {
     DrawRecord r{};
     r.raw = &path->getRawPath();
@@ -613,15 +627,244 @@ void RenderContext::pushPathDraw(const RiveRenderPath* path, const Mat2D& m, con
     {
         d.contour_directions = 3; // reverseThenForward
     }
-    m_draws.push_back(r);
+    return r;
 }
 
-bool RenderContext::flush(rive_hip_target* target)
+namespace
+{
+// Resource limits of one logical flush (render_context.cpp:43-53,484-492,
+// 517-541,617-654; gpu.hpp:85-103): path and clip IDs up to MaxPathID(1) - 1,
+// contour IDs up to 65535, tessellation vertices up to a 2048 x 2048 texture
+// less 25 padding vertices, gradient texture rows up to 2048.
+constexpr uint64_t kMaxPathID = (0x1fu << 10) - 1 - 1023 - 1;
+constexpr uint64_t kMaxContourID = 65535;
+constexpr uint64_t kMaxTessVerticesBeforePadding = 2048ull * 2048ull - (8 + 16 + 1);
+constexpr uint64_t kMaxGradientRows = 2048;
+
+bool is_simple_ramp(const Gradient& g)
+{
+    const size_t n = g.stops.size();
+    return n == 1 || (n == 2 && g.stops[0] == 0 && g.stops[1] == 1);
+}
+std::pair<uint32_t, uint32_t> simple_key(const Gradient& g)
+{
+    return {g.colors[0], g.colors[std::min<size_t>(1, g.stops.size() - 1)]};
+}
+
+// One logical flush being filled: LogicalFlush's resource counters, its
+// clip IDs and clip content, and its gradient ramps.
+struct LogicalFlushState
+{
+    std::vector<DrawRecord> draws;
+    uint64_t paths = 0, contours = 0, tess = 0;
+    uint32_t lastClipID = 0;
+    uint64_t contentSerial = 0; // clip element whose content the clip buffer holds
+    std::vector<std::pair<uint64_t, uint32_t>> clipIDs; // serial -> clip ID in this flush
+    std::vector<std::pair<uint32_t, uint32_t>> simple;
+    std::vector<const Gradient*> complex;
+
+    uint32_t clipIDOf(uint64_t serial) const
+    {
+        for (auto it = clipIDs.rbegin(); it != clipIDs.rend(); ++it)
+            if (it->first == serial)
+                return it->second;
+        return 0;
+    }
+    // The first clip element (index) the clip buffer does not already hold.
+    size_t firstClipToIssue(const std::vector<ClipRef>& clips) const
+    {
+        if (contentSerial != 0)
+            for (size_t i = clips.size(); i-- > 0;)
+                if (clips[i].serial == contentSerial)
+                    return i + 1;
+        return 0;
+    }
+    bool newGradient(const Gradient* g, bool* simpleRamp) const
+    {
+        *simpleRamp = is_simple_ramp(*g);
+        if (*simpleRamp)
+            return std::find(simple.begin(), simple.end(), simple_key(*g)) == simple.end();
+        for (const Gradient* c : complex)
+            if (c->colors == g->colors && c->stops == g->stops)
+                return false;
+        return true;
+    }
+};
+} // namespace
+
+bool RenderContext::flush(rive_hip_target* target, uint32_t maxFlushes, uint32_t* flushCount)
+{
+    if (flushCount != nullptr)
+        *flushCount = 0;
+    if (!m_canvases.empty())
+    {
+        // Batched: every canvas in one flush, each with its own clip IDs; no
+        // logical-flush split (no counterpart in the reference).
+        std::vector<DrawRecord> draws;
+        std::vector<rive_hip_canvas> canvases;
+        for (size_t c = 0; c < m_canvases.size(); ++c)
+        {
+            const size_t end = c + 1 < m_canvases.size() ? m_canvases[c + 1].firstBatch : m_batches.size();
+            LogicalFlushState st;
+            const size_t first = draws.size();
+            for (size_t b = m_canvases[c].firstBatch; b < end; ++b)
+            {
+                Batch& batch = m_batches[b];
+                const size_t firstClip = st.firstClipToIssue(batch.clips);
+                uint32_t parent = firstClip > 0 ? st.clipIDOf(batch.clips[firstClip - 1].serial) : 0;
+                for (size_t i = firstClip; i < batch.clips.size(); ++i)
+                {
+                    const ClipRef& ce = batch.clips[i];
+                    const uint32_t id = ++st.lastClipID;
+                    st.clipIDs.push_back({ce.serial, id});
+                    if (ce.onFrame)
+                        draws.push_back(makeRecord(ce.path, ce.matrix, nullptr, id, true, parent, ce.fillRule));
+                    parent = id;
+                }
+                if (!batch.clips.empty())
+                    st.contentSerial = batch.clips.back().serial;
+                draws.push_back(batch.draw);
+                draws.back().paint.params = (draws.back().paint.params & 0xffffu) | (parent << 16);
+            }
+            canvases.push_back({m_canvases[c].target, (uint32_t)first + 1, (uint32_t)(draws.size() - first),
+                                m_canvases[c].frame.clearColor, m_canvases[c].frame.loadAction});
+        }
+        bool ok = m_impl->flushFrame({layoutFlush(draws, target, m_frame.loadAction, &canvases)});
+        if (flushCount != nullptr)
+            *flushCount = 1;
+        m_batches.clear();
+        m_canvases.clear();
+        return ok;
+    }
+
+    // 1. Resource counts of every draw and every clip element, on the
+    //    device: they decide where the frame splits.
+    std::vector<DrawRecord> items;
+    std::vector<size_t> drawItem(m_batches.size());
+    std::vector<std::pair<uint64_t, size_t>> clipItem; // serial -> item
+    auto clipItemOf = [&](uint64_t serial) -> size_t {
+        for (auto it = clipItem.rbegin(); it != clipItem.rend(); ++it)
+            if (it->first == serial)
+                return it->second;
+        return (size_t)-1;
+    };
+    for (size_t b = 0; b < m_batches.size(); ++b)
+    {
+        for (const ClipRef& ce : m_batches[b].clips)
+            if (ce.onFrame && clipItemOf(ce.serial) == (size_t)-1)
+            {
+                clipItem.push_back({ce.serial, items.size()});
+                items.push_back(makeRecord(ce.path, ce.matrix, nullptr, 0, true, 0, ce.fillRule));
+            }
+        drawItem[b] = items.size();
+        items.push_back(m_batches[b].draw);
+    }
+    std::vector<uint32_t> tess, contours;
+    if (!items.empty() && !m_impl->countDraws(items, &tess, &contours))
+        return false;
+
+    // 2. RiveRenderer::clipAndPushDraw over the batches (rive_renderer.cpp:
+    //    496-570): a batch -- the clip draws applyClip issues plus the draw --
+    //    joins the current logical flush when it fits, else the flush ends
+    //    and the batch is retried once in a fresh flush, which re-issues its
+    //    whole clip stack; a batch that fits no flush is dropped.
+    std::vector<LogicalFlushState> flushes(1);
+    for (size_t b = 0; b < m_batches.size(); ++b)
+    {
+        Batch& batch = m_batches[b];
+        bool pushed = false;
+        for (int attempt = 0; attempt < 2 && !pushed; ++attempt)
+        {
+            LogicalFlushState& st = flushes.back();
+            const size_t firstClip = st.firstClipToIssue(batch.clips);
+            uint64_t paths = 0, ctrs = 0, verts = 0;
+            auto add = [&](size_t item) {
+                if (tess[item] > 0) // draw.cpp:1378-1391
+                {
+                    paths += 1;
+                    ctrs += contours[item];
+                    verts += tess[item];
+                }
+            };
+            for (size_t i = firstClip; i < batch.clips.size(); ++i)
+                if (batch.clips[i].onFrame)
+                    add(clipItemOf(batch.clips[i].serial));
+            add(drawItem[b]);
+            bool fits = (uint64_t)st.lastClipID + (batch.clips.size() - firstClip) <= kMaxPathID &&
+                        st.paths + paths <= kMaxPathID && st.contours + ctrs <= kMaxContourID &&
+                        st.tess + verts <= kMaxTessVerticesBeforePadding;
+            bool simpleRamp = false, newRamp = false;
+            if (fits && batch.draw.gradient != nullptr)
+            {
+                newRamp = st.newGradient(batch.draw.gradient, &simpleRamp);
+                const uint64_t simple = st.simple.size() + (newRamp && simpleRamp ? 1 : 0);
+                const uint64_t complex = st.complex.size() + (newRamp && !simpleRamp ? 1 : 0);
+                fits = (simple + 255) / 256 + complex <= kMaxGradientRows;
+            }
+            if (!fits)
+            {
+                if (attempt == 0)
+                    flushes.emplace_back();
+                continue;
+            }
+            // The draw reads the innermost clip of its stack; elements below
+            // firstClip are already in the clip buffer.
+            uint32_t parent = firstClip > 0 ? st.clipIDOf(batch.clips[firstClip - 1].serial) : 0;
+            for (size_t i = firstClip; i < batch.clips.size(); ++i)
+            {
+                const ClipRef& ce = batch.clips[i];
+                const uint32_t id = ++st.lastClipID;
+                st.clipIDs.push_back({ce.serial, id});
+                if (ce.onFrame)
+                    st.draws.push_back(makeRecord(ce.path, ce.matrix, nullptr, id, true, parent, ce.fillRule));
+                parent = id;
+            }
+            if (!batch.clips.empty())
+                st.contentSerial = batch.clips.back().serial;
+            st.draws.push_back(batch.draw);
+            st.draws.back().paint.params = (st.draws.back().paint.params & 0xffffu) | (parent << 16);
+            st.paths += paths;
+            st.contours += ctrs;
+            st.tess += verts;
+            if (newRamp)
+            {
+                if (simpleRamp)
+                    st.simple.push_back(simple_key(*batch.draw.gradient));
+                else
+                    st.complex.push_back(batch.draw.gradient);
+            }
+            pushed = true;
+        }
+        if (!pushed)
+            fprintf(stderr, "RiveRenderer: a draw and its clip stack exceed one logical flush; dropped\n");
+    }
+
+    // 3. Each logical flush to the backend; later ones load the target
+    //    (render_context.cpp:1218-1222).
+    if (flushCount != nullptr)
+        *flushCount = (uint32_t)flushes.size();
+    std::vector<FlushDescriptor> descs;
+    for (size_t k = 0; k < flushes.size(); ++k)
+    {
+        if (maxFlushes != 0 && k >= maxFlushes)
+            break;
+        descs.push_back(layoutFlush(flushes[k].draws, target, k == 0 ? m_frame.loadAction : 1u, nullptr));
+    }
+    bool ok = m_impl->flushFrame(descs);
+    m_batches.clear();
+    m_provLastClipID = 0;
+    m_provContentSerial = 0;
+    m_provClipIDs.clear();
+    return ok;
+}
+
+FlushDescriptor RenderContext::layoutFlush(std::vector<DrawRecord>& draws, rive_hip_target* target,
+                                           uint32_t loadAction, const std::vector<rive_hip_canvas>* canvases)
 {
     // LogicalFlush::layoutResources + writeResources (render_context.cpp:1116-2408)
     FlushDescriptor fd;
     fd.renderTarget = target;
-    fd.loadAction = m_frame.loadAction;
+    fd.loadAction = loadAction;
     fd.clearColor = m_frame.clearColor;
     fd.dither = m_frame.ditherMode;
     fd.paths.push_back(rive_hip_path_data{});
@@ -640,7 +883,7 @@ bool RenderContext::flush(rive_hip_target* target)
     std::vector<std::pair<uint32_t, uint32_t>> simple;
     std::vector<const Gradient*> complex;
     std::vector<PendingGrad> pending;
-    for (DrawRecord& r : m_draws)
+    for (DrawRecord& r : draws)
     {
         const uint32_t pathID = (uint32_t)fd.paths.size();
         r.draw.path_id = pathID;
@@ -768,17 +1011,9 @@ bool RenderContext::flush(rive_hip_target* target)
     fd.gradSimpleRows = simpleRows;
     fd.gradComplexRows = complexRows;
     // Batched: each canvas owns the paths of its draws (path ID = draw index + 1).
-    for (size_t c = 0; c < m_canvases.size(); ++c)
-    {
-        const CanvasRecord& cr = m_canvases[c];
-        const size_t end = c + 1 < m_canvases.size() ? m_canvases[c + 1].firstDraw : m_draws.size();
-        fd.canvases.push_back({cr.target, (uint32_t)cr.firstDraw + 1, (uint32_t)(end - cr.firstDraw),
-                               cr.frame.clearColor, cr.frame.loadAction});
-    }
-    bool ok = m_impl->flush(fd);
-    m_draws.clear();
-    m_canvases.clear();
-    return ok;
+    if (canvases != nullptr)
+        fd.canvases = *canvases;
+    return fd;
 }
 } // namespace gpu
 
@@ -802,33 +1037,100 @@ bool RenderContextHIPImpl::upload(int kind, const void* data, size_t bytes)
     return rive_hip_buffer_unmap(m_ctx, kind, bytes) == RIVE_HIP_OK;
 }
 
-bool RenderContextHIPImpl::flush(const gpu::FlushDescriptor& fd)
+bool RenderContextHIPImpl::countDraws(const std::vector<gpu::DrawRecord>& draws, std::vector<uint32_t>* tess,
+                                      std::vector<uint32_t>* contours)
 {
-    if (!upload(RIVE_HIP_BUFFER_PATH, fd.paths.data(), fd.paths.size() * sizeof(rive_hip_path_data)) ||
-        !upload(RIVE_HIP_BUFFER_PAINT, fd.paints.data(), fd.paints.size() * sizeof(rive_hip_paint_data)) ||
-        !upload(RIVE_HIP_BUFFER_PAINT_AUX, fd.paintAux.data(), fd.paintAux.size() * sizeof(rive_hip_paint_aux)) ||
-        !upload(RIVE_HIP_BUFFER_DRAW, fd.draws.data(), fd.draws.size() * sizeof(rive_hip_draw)) ||
-        !upload(RIVE_HIP_BUFFER_VERB, fd.verbs.data(), fd.verbs.size()) ||
-        !upload(RIVE_HIP_BUFFER_POINT, fd.points.data(), fd.points.size() * sizeof(float)) ||
-        !upload(RIVE_HIP_BUFFER_GRAD_SPAN, fd.gradSpans.data(), fd.gradSpans.size() * sizeof(rive_hip_grad_span)))
+    std::vector<rive_hip_draw> recs(draws.size());
+    std::vector<uint8_t> verbs;
+    std::vector<float> points;
+    for (size_t i = 0; i < draws.size(); ++i)
+    {
+        const gpu::DrawRecord& r = draws[i];
+        recs[i] = r.draw;
+        recs[i].path_id = (uint32_t)i + 1;
+        recs[i].first_verb = (uint32_t)verbs.size();
+        recs[i].verb_count = (uint32_t)r.raw->verbs().size();
+        recs[i].first_point = (uint32_t)(points.size() / 2);
+        recs[i].point_count = (uint32_t)r.raw->points().size();
+        for (PathVerb v : r.raw->verbs())
+            verbs.push_back((uint8_t)v);
+        for (Vec2D p : r.raw->points())
+        {
+            points.push_back(p.x);
+            points.push_back(p.y);
+        }
+    }
+    if (!upload(RIVE_HIP_BUFFER_DRAW, recs.data(), recs.size() * sizeof(rive_hip_draw)) ||
+        !upload(RIVE_HIP_BUFFER_VERB, verbs.data(), verbs.size()) ||
+        !upload(RIVE_HIP_BUFFER_POINT, points.data(), points.size() * sizeof(float)))
         return false;
-    rive_hip_flush_desc d{};
-    d.target = fd.renderTarget;
-    d.canvases = fd.canvases.data();
-    d.canvas_count = (uint32_t)fd.canvases.size();
-    d.mode = RIVE_HIP_FLUSH_GEOMETRY;
-    d.load_action = fd.loadAction;
-    d.clear_color = fd.clearColor;
-    d.dither = fd.dither;
-    d.path_count = (uint32_t)fd.paths.size() - 1;
-    d.draw_count = (uint32_t)fd.draws.size();
-    d.grad_span_count = (uint32_t)fd.gradSpans.size();
-    d.grad_simple_rows = fd.gradSimpleRows;
-    d.grad_complex_rows = fd.gradComplexRows;
-    d.first_path_tess_location = 8;
-    d.images = fd.images.data();
-    d.image_count = (uint32_t)fd.images.size();
-    return rive_hip_flush(m_ctx, &d) == RIVE_HIP_OK;
+    tess->assign(draws.size(), 0);
+    contours->assign(draws.size(), 0);
+    return rive_hip_count_draws(m_ctx, (uint32_t)draws.size(), tess->data(), contours->data()) == RIVE_HIP_OK;
+}
+
+bool RenderContextHIPImpl::flushFrame(const std::vector<gpu::FlushDescriptor>& flushes)
+{
+    // One upload per buffer for the whole frame (the reference maps each
+    // buffer once per frame); each logical flush then reads its own run.
+    std::vector<rive_hip_path_data> paths;
+    std::vector<rive_hip_paint_data> paints;
+    std::vector<rive_hip_paint_aux> aux;
+    std::vector<rive_hip_draw> draws;
+    std::vector<uint8_t> verbs;
+    std::vector<float> points;
+    std::vector<rive_hip_grad_span> grads;
+    std::vector<rive_hip_flush_desc> descs;
+    for (const gpu::FlushDescriptor& fd : flushes)
+    {
+        rive_hip_flush_desc d{};
+        d.target = fd.renderTarget;
+        d.canvases = fd.canvases.empty() ? nullptr : fd.canvases.data();
+        d.canvas_count = (uint32_t)fd.canvases.size();
+        d.mode = RIVE_HIP_FLUSH_GEOMETRY;
+        d.load_action = fd.loadAction;
+        d.clear_color = fd.clearColor;
+        d.dither = fd.dither;
+        d.path_count = (uint32_t)fd.paths.size() - 1;
+        d.draw_count = (uint32_t)fd.draws.size();
+        d.grad_span_count = (uint32_t)fd.gradSpans.size();
+        d.grad_simple_rows = fd.gradSimpleRows;
+        d.grad_complex_rows = fd.gradComplexRows;
+        d.first_path_tess_location = 8;
+        d.images = fd.images.empty() ? nullptr : fd.images.data();
+        d.image_count = (uint32_t)fd.images.size();
+        d.first_path = paths.size();
+        d.first_paint = paints.size();
+        d.first_paint_aux = aux.size();
+        d.first_draw = draws.size();
+        d.first_grad_span = grads.size();
+        const uint32_t verbBase = (uint32_t)verbs.size(), pointBase = (uint32_t)(points.size() / 2);
+        paths.insert(paths.end(), fd.paths.begin(), fd.paths.end());
+        paints.insert(paints.end(), fd.paints.begin(), fd.paints.end());
+        aux.insert(aux.end(), fd.paintAux.begin(), fd.paintAux.end());
+        for (rive_hip_draw r : fd.draws)
+        {
+            r.first_verb += verbBase;
+            r.first_point += pointBase;
+            draws.push_back(r);
+        }
+        verbs.insert(verbs.end(), fd.verbs.begin(), fd.verbs.end());
+        points.insert(points.end(), fd.points.begin(), fd.points.end());
+        grads.insert(grads.end(), fd.gradSpans.begin(), fd.gradSpans.end());
+        descs.push_back(d);
+    }
+    if (!upload(RIVE_HIP_BUFFER_PATH, paths.data(), paths.size() * sizeof(rive_hip_path_data)) ||
+        !upload(RIVE_HIP_BUFFER_PAINT, paints.data(), paints.size() * sizeof(rive_hip_paint_data)) ||
+        !upload(RIVE_HIP_BUFFER_PAINT_AUX, aux.data(), aux.size() * sizeof(rive_hip_paint_aux)) ||
+        !upload(RIVE_HIP_BUFFER_DRAW, draws.data(), draws.size() * sizeof(rive_hip_draw)) ||
+        !upload(RIVE_HIP_BUFFER_VERB, verbs.data(), verbs.size()) ||
+        !upload(RIVE_HIP_BUFFER_POINT, points.data(), points.size() * sizeof(float)) ||
+        !upload(RIVE_HIP_BUFFER_GRAD_SPAN, grads.data(), grads.size() * sizeof(rive_hip_grad_span)))
+        return false;
+    for (const rive_hip_flush_desc& d : descs)
+        if (rive_hip_flush(m_ctx, &d) != RIVE_HIP_OK)
+            return false;
+    return true;
 }
 } // namespace rive
 
@@ -968,7 +1270,8 @@ rive::gpu::RenderContext::FrameDescriptor stream_frame(const rive_stream* s)
 }
 } // namespace
 
-extern "C" int rive_hip_render_stream(rive_hip_context* ctx, const rive_stream* s, rive_hip_target* target)
+extern "C" int rive_hip_render_stream_ex(rive_hip_context* ctx, const rive_stream* s, rive_hip_target* target,
+                                         uint32_t maxFlushes, uint32_t* flushCount)
 {
     if (ctx == nullptr || s == nullptr || target == nullptr)
         return RIVE_HIP_ERROR_INVALID_ARGUMENT;
@@ -982,7 +1285,12 @@ extern "C" int rive_hip_render_stream(rive_hip_context* ctx, const rive_stream* 
     st = draw_stream(renderer, s, r);
     if (st != RIVE_HIP_OK)
         return st;
-    return rc.flush(target) ? RIVE_HIP_OK : RIVE_HIP_ERROR_DEVICE;
+    return rc.flush(target, maxFlushes, flushCount) ? RIVE_HIP_OK : RIVE_HIP_ERROR_DEVICE;
+}
+
+extern "C" int rive_hip_render_stream(rive_hip_context* ctx, const rive_stream* s, rive_hip_target* target)
+{
+    return rive_hip_render_stream_ex(ctx, s, target, 0, nullptr);
 }
 
 extern "C" uint32_t rive_stream_contour_bound(const rive_stream* s)

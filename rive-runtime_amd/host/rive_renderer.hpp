@@ -264,6 +264,8 @@ public:
 namespace gpu
 {
 // One PathDraw's records (PathData/PaintData/PaintAux + counting inputs).
+// The clip ID a draw reads is patched into paint.params when its logical
+// flush is laid out.
 struct DrawRecord
 {
     rive_hip_path_data path;
@@ -273,6 +275,18 @@ struct DrawRecord
     const RawPath* raw;
     const Gradient* gradient; // null for solid/clip paints
     const RiveRenderImage* image;
+};
+
+// One element of the clip stack a draw sits under (rive_renderer.hpp
+// ClipElement): the serial names the element; equal serials are the same
+// clip content.
+struct ClipRef
+{
+    uint64_t serial;
+    const RiveRenderPath* path;
+    Mat2D matrix;
+    FillRule fillRule;
+    bool onFrame; // its pixel bounds meet the frame (else no draw, only an ID)
 };
 
 // The flush descriptor handed to the backend (gpu.hpp:1320-1438, reduced).
@@ -300,7 +314,16 @@ class RenderContextImpl
 {
 public:
     virtual ~RenderContextImpl() = default;
-    virtual bool flush(const FlushDescriptor& desc) = 0;
+    // One frame: its logical flushes in order.  The reference maps the
+    // frame's buffers once and calls flush(desc) per logical flush, each
+    // descriptor naming its records by first* offsets
+    // (render_context.cpp:971-1018); a backend sees the same here.
+    virtual bool flushFrame(const std::vector<FlushDescriptor>& flushes) = 0;
+    // The resource counts of each draw (Draw::resourceCounts, draw.cpp:
+    // 1378-1391): tessellation vertices and contours.  A backend that
+    // counts on the device answers here; false if it cannot.
+    virtual bool countDraws(const std::vector<DrawRecord>& draws, std::vector<uint32_t>* tess,
+                            std::vector<uint32_t>* contours) = 0;
 };
 
 class RenderContext
@@ -319,31 +342,54 @@ public:
     // into `target` with this frame's size and clear color.  Each canvas
     // starts with an empty clip buffer.
     void beginCanvas(const FrameDescriptor& fd, rive_hip_target* target);
-    bool flush(rive_hip_target* target);
+    // Lays the frame out in logical flushes and hands each to the backend.
+    // maxFlushes > 0 stops after that many (a debugging aid); flushCount, if
+    // given, receives the frame's number of logical flushes.
+    bool flush(rive_hip_target* target, uint32_t maxFlushes = 0, uint32_t* flushCount = nullptr);
     const FrameDescriptor& frameDescriptor() const { return m_frame; }
     RenderContextImpl* impl() { return m_impl.get(); }
 
+    // The clip ID whose content the clip buffer holds after the draws pushed
+    // so far (RenderContext::getClipContentID, render_context.hpp), as laid
+    // out while the frame fits one logical flush; 0 when none.
+    uint32_t getClipContentID() const;
+
     // Used by RiveRenderer.
-    uint32_t generateClipID() { return ++m_lastClipID; }
-    uint32_t getClipContentID() const { return m_clipContentID; }
-    void setClipContentID(uint32_t id) { m_clipContentID = id; }
-    void pushPathDraw(const RiveRenderPath* path, const Mat2D& m, const RiveRenderPaint* paint,
-                      uint32_t clipID, bool clipUpdate, uint32_t outerClipID, FillRule fillRule,
-                      const Mat2D* clipRectInverse = nullptr);
+    uint64_t newClipSerial() { return ++m_lastClipSerial; }
+    // The records of one draw (PathDraw::Make + the PathData/PaintData/
+    // PaintAuxData it writes, draw.cpp:420-825, gpu.cpp:859-1063).
+    DrawRecord makeRecord(const RiveRenderPath* path, const Mat2D& m, const RiveRenderPaint* paint, uint32_t clipID,
+                          bool clipUpdate, uint32_t outerClipID, FillRule fillRule,
+                          const Mat2D* clipRectInverse = nullptr) const;
+    // A draw and the clip stack it sits under; clip draws and clip IDs are
+    // assigned per logical flush when the frame is flushed (applyClip,
+    // rive_renderer.cpp:631-824).
+    void pushDrawBatch(std::vector<ClipRef> clips, const DrawRecord& draw);
 
 private:
+    struct Batch
+    {
+        std::vector<ClipRef> clips;
+        DrawRecord draw;
+    };
+    FlushDescriptor layoutFlush(std::vector<DrawRecord>& draws, rive_hip_target* target, uint32_t loadAction,
+                                const std::vector<rive_hip_canvas>* canvases);
     std::unique_ptr<RenderContextImpl> m_impl;
     FrameDescriptor m_frame;
-    std::vector<DrawRecord> m_draws;
+    std::vector<Batch> m_batches;
     struct CanvasRecord
     {
         rive_hip_target* target;
-        size_t firstDraw;
+        size_t firstBatch;
         FrameDescriptor frame;
     };
     std::vector<CanvasRecord> m_canvases;
-    uint32_t m_lastClipID = 0;
-    uint32_t m_clipContentID = 0;
+    uint64_t m_lastClipSerial = 0;
+    // applyClip's bookkeeping as draws arrive (for getClipContentID): clip
+    // IDs handed out so far and the element whose content is current.
+    uint32_t m_provLastClipID = 0;
+    uint64_t m_provContentSerial = 0;
+    std::vector<std::pair<uint64_t, uint32_t>> m_provClipIDs;
 };
 } // namespace gpu
 
@@ -390,7 +436,7 @@ private:
         const RiveRenderPath* path;
         FillRule fillRule;
         IAABB pixelBounds;
-        uint32_t clipID;
+        uint64_t serial;
     };
     struct RenderState
     {
@@ -417,7 +463,13 @@ class RenderContextHIPImpl : public gpu::RenderContextImpl
 {
 public:
     explicit RenderContextHIPImpl(rive_hip_context* ctx) : m_ctx(ctx) {}
-    bool flush(const gpu::FlushDescriptor& desc) override;
+    // Uploads every logical flush's records once, into shared buffers, then
+    // issues one rive_hip_flush per logical flush with its first_* offsets.
+    bool flushFrame(const std::vector<gpu::FlushDescriptor>& flushes) override;
+    // Per-draw (tessellation vertices, contours) of `draws`, counted on the
+    // device (rive_hip_count_draws).
+    bool countDraws(const std::vector<gpu::DrawRecord>& draws, std::vector<uint32_t>* tess,
+                    std::vector<uint32_t>* contours) override;
 
 private:
     bool upload(int kind, const void* data, size_t bytes);

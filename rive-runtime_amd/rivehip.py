@@ -68,7 +68,11 @@ class FlushDesc(ctypes.Structure):
                 ("grad_complex_rows", ctypes.c_uint32),
                 ("first_path_tess_location", ctypes.c_uint32), ("images", ctypes.c_void_p),
                 ("image_count", ctypes.c_uint32), ("canvases", ctypes.c_void_p),
-                ("canvas_count", ctypes.c_uint32)]
+                ("canvas_count", ctypes.c_uint32),
+                ("first_path", ctypes.c_uint64), ("first_paint", ctypes.c_uint64),
+                ("first_paint_aux", ctypes.c_uint64), ("first_contour", ctypes.c_uint64),
+                ("first_grad_span", ctypes.c_uint64), ("first_tess_span", ctypes.c_uint64),
+                ("first_draw", ctypes.c_uint64)]
 
 
 class Canvas(ctypes.Structure):
@@ -97,6 +101,7 @@ EXPORTED = [
     "rive_hip_render_stream", "rive_hip_last_flush_stats", "rive_hip_image_create",
     "rive_hip_image_destroy", "rive_hip_render_streams", "rive_hip_last_flush_canvases",
     "rive_stream_generate_batch", "rive_stream_contour_bound", "rive_hip_flush_noalloc",
+    "rive_hip_count_draws", "rive_hip_render_stream_ex", "rive_hip_last_frame",
 ]
 
 _lib = None
@@ -133,6 +138,9 @@ def lib():
     L.rive_stream_free.argtypes = [ctypes.POINTER(Stream)]
     L.rive_stream_free.restype = None
     L.rive_hip_render_stream.argtypes = [vp, ctypes.POINTER(Stream), vp]
+    L.rive_hip_render_stream_ex.argtypes = [vp, ctypes.POINTER(Stream), vp, u32, ctypes.POINTER(u32)]
+    L.rive_hip_count_draws.argtypes = [vp, u32, vp, vp]
+    L.rive_hip_last_frame.argtypes = [vp, ctypes.POINTER(FlushDesc), u32, ctypes.POINTER(u32)]
     L.rive_hip_render_streams.argtypes = [vp, ctypes.POINTER(ctypes.POINTER(Stream)),
                                           ctypes.POINTER(vp), u32]
     L.rive_hip_last_flush_canvases.argtypes = [vp, ctypes.POINTER(Canvas), u32, ctypes.POINTER(u32)]
@@ -255,8 +263,13 @@ class Context:
         check(self.L.rive_hip_pass_times(self.h, ms, 6))
         return list(ms)
 
-    def render_stream(self, stream, target):
-        check(self.L.rive_hip_render_stream(self.h, stream, target))
+    def render_stream(self, stream, target, max_flushes=0):
+        """Replays a stream through the front end and flushes it; the frame
+        may split into several logical flushes.  max_flushes > 0 stops after
+        that many.  Returns the frame's logical flush count."""
+        n = ctypes.c_uint32()
+        check(self.L.rive_hip_render_stream_ex(self.h, stream, target, max_flushes, ctypes.byref(n)))
+        return n.value
 
     def render_streams(self, streams, targets):
         """One batched flush: stream i renders into targets[i]."""
@@ -271,6 +284,25 @@ class Context:
         arr = (Canvas * max(count.value, 1))()
         check(self.L.rive_hip_last_flush_canvases(self.h, arr, count.value, ctypes.byref(count)))
         return arr, count.value
+
+    def last_frame(self):
+        """The last frame's logical flush descriptors (FlushDesc list)."""
+        n = ctypes.c_uint32()
+        check(self.L.rive_hip_last_frame(self.h, None, 0, ctypes.byref(n)))
+        arr = (FlushDesc * max(n.value, 1))()
+        check(self.L.rive_hip_last_frame(self.h, arr, n.value, ctypes.byref(n)))
+        return [arr[k] for k in range(n.value)]
+
+    def reflush_frame(self, descs, target=None):
+        """Re-issues a frame's logical flushes from the records resident in
+        HBM (optionally into another target); returns each flush's stats."""
+        out = []
+        for d in descs:
+            if target is not None:
+                d.target = target
+            self.flush(d)
+            out.append(self.last_flush_stats())
+        return out
 
     def last_flush_stats(self):
         st = FlushStats()

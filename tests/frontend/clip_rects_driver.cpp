@@ -38,7 +38,15 @@ bool near_rect(const RiveRenderer::AABB& r, float l, float t, float rr, float b)
 class RecordingImpl : public gpu::RenderContextImpl
 {
 public:
-    bool flush(const gpu::FlushDescriptor&) override { return true; }
+    bool flushFrame(const std::vector<gpu::FlushDescriptor>&) override { return true; }
+    // Every draw counts as one small path: these scenes fit one logical flush.
+    bool countDraws(const std::vector<gpu::DrawRecord>& draws, std::vector<uint32_t>* tess,
+                    std::vector<uint32_t>* contours) override
+    {
+        tess->assign(draws.size(), 8);
+        contours->assign(draws.size(), 1);
+        return true;
+    }
 };
 
 void rect_path(RiveRenderPath& p, float l, float t, float r, float b)

@@ -32,6 +32,12 @@ def lib():
         L.oracle_ramps.argtypes = [vp, vp, ctypes.c_size_t]
         L.oracle_render.restype = ctypes.c_double
         L.oracle_render.argtypes = [vp, vp]
+        L.oracle_render_edges.restype = ctypes.c_double
+        L.oracle_render_edges.argtypes = [vp, vp, vp]
+        L.oracle_flush_count.restype = ctypes.c_uint32
+        L.oracle_flush_count.argtypes = [vp]
+        L.oracle_select.restype = ctypes.c_int
+        L.oracle_select.argtypes = [vp, ctypes.c_uint32]
         for name in ["oracle_wang_cubic", "oracle_wang_cubic_pow4"]:
             getattr(L, name).restype = f
             getattr(L, name).argtypes = [fp, f, fp]
@@ -58,16 +64,28 @@ RECORD_DTYPES = {0: rivehip.TESS_SPAN, 1: rivehip.CONTOUR, 2: rivehip.PATH_DATA,
 
 
 class OracleFlush:
-    """One logical flush laid out by the oracle's restatement of RenderContext."""
+    """A frame laid out by the oracle's restatement of RiveRenderer and
+    RenderContext: one or more logical flushes (the reference splits a frame
+    where its resource limits require).  Record getters read the selected
+    logical flush (the first by default); render() renders all of them."""
 
     def __init__(self, stream_ptr):
         self.L = lib()
         self.h = self.L.oracle_build(stream_ptr)
+        self.flush_count = int(self.L.oracle_flush_count(self.h))
+        self.select(0)
+
+    def select(self, index):
+        """Points the record getters at logical flush `index`."""
+        if self.L.oracle_select(self.h, index) != 0:
+            raise IndexError(index)
+        self.index = index
         info = np.zeros(10, np.uint32)
         self.L.oracle_get(self.h, 8, info.ctypes.data, info.nbytes)
         (self.path_count, self.contour_count, self.span_count, self.tess_vertex_count,
          self.grad_simple_rows, self.grad_complex_rows, self.grad_height, self.draw_count,
          self.width, self.height) = [int(v) for v in info]
+        return self
 
     def close(self):
         if self.h:
@@ -99,9 +117,18 @@ class OracleFlush:
         return buf.view(np.uint32)
 
     def render(self):
+        """The whole frame (every logical flush): (RGBA8 image, seconds)."""
         img = np.zeros((self.height, self.width, 4), np.uint8)
         secs = self.L.oracle_render(self.h, img.ctypes.data)
         return img, secs
+
+    def render_edges(self):
+        """The frame plus a mask of pixels some draw covers fractionally
+        (its final coverage strictly between 0 and 1): (image, mask, secs)."""
+        img = np.zeros((self.height, self.width, 4), np.uint8)
+        mask = np.zeros((self.height, self.width), np.uint8)
+        secs = self.L.oracle_render_edges(self.h, img.ctypes.data, mask.ctypes.data)
+        return img, mask.astype(bool), secs
 
     def draw_records(self):
         """rive_hip_draw records for a SPANS-mode flush of this layout."""
@@ -138,3 +165,47 @@ class OracleFlush:
         desc.grad_simple_rows = self.grad_simple_rows
         desc.grad_complex_rows = self.grad_complex_rows
         ctx.flush(desc)
+
+    def flush_all_spans_on(self, ctx, target, clear_color=0, dither=1):
+        """The reference's multi-flush model in SPANS mode: every logical
+        flush's records concatenated into shared buffers (one upload per
+        buffer, as RenderContext maps them once per frame), then one
+        rive_hip_flush per logical flush with first_* offsets into those
+        buffers; flushes after the first load the target
+        (render_context.cpp:1218-1222; gpu.hpp:1386-1395)."""
+        keep = self.index
+        parts = {w: [] for w in ("path", "paint", "paint_aux", "contour", "tess_span", "grad_span", "draw")}
+        descs = []
+        offs = {w: 0 for w in parts}
+        for k in range(self.flush_count):
+            self.select(k)
+            recs = {"path": self.get(2), "paint": self.get(3), "paint_aux": self.get(4),
+                    "contour": self.get(1), "tess_span": self.get(0), "grad_span": self.get(5),
+                    "draw": self.draw_records()}
+            d = rivehip.FlushDesc()
+            d.target = target
+            d.mode = rivehip.FLUSH_SPANS
+            d.load_action = 0 if k == 0 else 1
+            d.clear_color = clear_color
+            d.dither = dither
+            d.path_count = self.path_count
+            d.contour_count = self.contour_count
+            d.tess_span_count = self.span_count
+            d.tess_vertex_count = self.tess_vertex_count
+            d.draw_count = self.draw_count
+            d.grad_span_count = len(recs["grad_span"])
+            d.grad_simple_rows = self.grad_simple_rows
+            d.grad_complex_rows = self.grad_complex_rows
+            d.first_path, d.first_paint, d.first_paint_aux = offs["path"], offs["paint"], offs["paint_aux"]
+            d.first_contour, d.first_tess_span = offs["contour"], offs["tess_span"]
+            d.first_grad_span, d.first_draw = offs["grad_span"], offs["draw"]
+            descs.append(d)
+            for w, r in recs.items():
+                parts[w].append(r)
+                offs[w] += len(r)
+        for w, rs in parts.items():
+            ctx.upload(w, np.concatenate(rs))
+        for d in descs:
+            ctx.flush(d)
+        self.select(keep)
+

@@ -92,28 +92,37 @@ def test_spans_mode_matches_oracle(hip_ctx, stream_case):
 
 @pytest.mark.parametrize("stream_case", LAYOUT_CASES, ids=case_id, indirect=True)
 def test_geometry_layout_bit_exact(hip_ctx, stream_case):
+    """Every logical flush of the frame -- the full 5000-draw 1080p stream
+    splits into six, as the reference's 2048x2048-texel tessellation limit
+    requires -- has the oracle's draw ranges, contours and spans bit for bit
+    (the device lays each flush out from its own counts)."""
     c, s, f = stream_case
     announce("geometry-layout", c)
     t = hip_ctx.target(c[2], c[3])
     try:
-        hip_ctx.render_stream(s, t)
-        hip_ctx.sync()
-        spans = hip_ctx.readback(0).view(rivehip.TESS_SPAN)
-        contours = hip_ctx.readback(1).view(rivehip.CONTOUR)
-        draws = hip_ctx.readback(3).view(rivehip.DRAW)
+        for k in range(f.flush_count):
+            n = hip_ctx.render_stream(s, t, max_flushes=k + 1)
+            assert n == f.flush_count, (n, f.flush_count)
+            hip_ctx.sync()
+            spans = hip_ctx.readback(0).view(rivehip.TESS_SPAN)
+            contours = hip_ctx.readback(1).view(rivehip.CONTOUR)
+            draws = hip_ctx.readback(3).view(rivehip.DRAW)
+            f.select(k)
+            od = f.draws()
+            assert len(draws) == len(od), k
+            assert np.array_equal(draws["path_id"], od[:, 0])
+            assert np.array_equal(draws["tess_location"], od[:, 1])
+            assert np.array_equal(draws["tess_vertex_count"], od[:, 2])
+            ocont = f.get(1)
+            assert len(contours) == len(ocont) and same_bits(contours, ocont)
+            ospans = f.get(0)
+            assert len(spans) == len(ospans)
+            mism = np.nonzero(spans.view(np.uint32).reshape(-1, 16) != ospans.view(np.uint32).reshape(-1, 16))
+            assert mism[0].size == 0, f"flush {k}: first mismatching span {mism[0][:4]}, fields {mism[1][:4]}"
     finally:
         hip_ctx.free_target(t)
-    od = f.draws()
-    assert len(draws) == len(od)
-    assert np.array_equal(draws["path_id"], od[:, 0])
-    assert np.array_equal(draws["tess_location"], od[:, 1])
-    assert np.array_equal(draws["tess_vertex_count"], od[:, 2])
-    ocont = f.get(1)
-    assert len(contours) == len(ocont) and same_bits(contours, ocont)
-    ospans = f.get(0)
-    assert len(spans) == len(ospans)
-    mism = np.nonzero(spans.view(np.uint32).reshape(-1, 16) != ospans.view(np.uint32).reshape(-1, 16))
-    assert mism[0].size == 0, f"first mismatching span {mism[0][:4]}, fields {mism[1][:4]}"
+    if c[1] == 5000:
+        assert f.flush_count == 6
 
 
 @pytest.mark.parametrize("stream_case", RASTER_CASES, ids=case_id, indirect=True)
@@ -132,7 +141,11 @@ def test_geometry_mode_pixels(hip_ctx, stream_case):
 
 def test_full_workload_modes_agree_and_repeat(hip_ctx):
     """1920x1080, 5000 draws: GEOMETRY and SPANS flushes of the same frame give
-    identical pixels, and a repeated flush reproduces them bit for bit."""
+    identical pixels, and a repeated flush reproduces them bit for bit.  The
+    SPANS frame is the reference's model of six logical flushes: every
+    flush's records uploaded once into shared buffers, then six rive_hip_flush
+    calls with FlushDescriptor-style first_* offsets, the later ones loading
+    the target."""
     c = (2, 5000, 1920, 1080)
     announce("full-modes", c)
     s = rivehip.generate_stream(2)
@@ -143,7 +156,7 @@ def test_full_workload_modes_agree_and_repeat(hip_ctx):
         a = hip_ctx.read_pixels(t, 1920, 1080)
         hip_ctx.render_stream(s, t)
         b = hip_ctx.read_pixels(t, 1920, 1080)
-        f.flush_spans_on(hip_ctx, t)
+        f.flush_all_spans_on(hip_ctx, t, clear_color=0, dither=1)
         c2 = hip_ctx.read_pixels(t, 1920, 1080)
     finally:
         hip_ctx.free_target(t)

@@ -3,9 +3,10 @@ bit for bit against rive_hip_flush on the same resident records.
 
 Two contexts hold two draw streams (the second smaller); their no-alloc
 flushes run back to back on the one GPU, so the second reuses the scratch
-array the first left dirty.  A frame too large for the scratch is refused
-with RIVE_HIP_ERROR_OUT_OF_MEMORY, and the allocating flush still works
-afterwards.
+array the first left dirty.  A flush too large for the scratch (the first
+logical flush of the 1920x1080 5000-draw frame) is refused with
+RIVE_HIP_ERROR_OUT_OF_MEMORY, and the allocating flushes still re-render the
+frame afterwards.
 """
 import ctypes
 
@@ -79,12 +80,12 @@ def test_noalloc_flush_matches_flush():
         targets.append((ctx, tb))
         ctx.render_stream(big, tb)
         ref = ctx.read_pixels(tb, 1920, 1080)
-        st = ctx.last_flush_stats()
-        d = desc_from(st, tb)
+        frame = ctx.last_frame()  # six logical flushes sharing the record buffers
+        d = frame[0]
         rc = ctx.L.rive_hip_flush_noalloc(ctx.h, ctypes.byref(d))
         assert rc == OUT_OF_MEMORY, rc
         assert b"no-alloc scratch" in ctx.L.rive_hip_last_error()
-        ctx.reflush_geometry(tb, st)
+        ctx.reflush_frame(ctx.last_frame(), tb)
         assert np.array_equal(ctx.read_pixels(tb, 1920, 1080), ref)
     finally:
         for ctx, t in targets:
