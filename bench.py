@@ -229,21 +229,18 @@ def main():
         ctx.sync()
         for _ in range(args.steps):
             t0 = time.perf_counter()
-            for d in frame:
-                ctx.flush(d)
+            ctx.flush_frame(frame, target)
             ctx.sync()
             step_ms.append((time.perf_counter() - t0) * 1e3)
-        # Per-pass times, summed over the logical flushes (separate frames,
-        # reading each flush's HIP events).
+        # Per-pass times summed over the frame's logical flushes (HIP events
+        # of each flush), from separate frames.
         passes = []
         for _ in range(max(3, args.steps // 5)):
-            per = []
-            stats = []
-            for d in frame:
-                ctx.flush(d)
-                per.append(ctx.pass_times())
-                stats.append(ctx.last_flush_stats())
-            passes.append([sum(x) for x in zip(*per)])
+            ctx.flush_frame(frame, target)
+            passes.append(ctx.pass_times())
+        # Each logical flush's sizes (for the algorithmic bytes).
+        stats = ctx.reflush_frame(frame, target)
+        ctx.sync()
         pixels = ctx.read_pixels(target, W, H)
     finally:
         ctx.free_target(target)

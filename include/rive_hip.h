@@ -302,6 +302,18 @@ int rive_hip_count_draws(rive_hip_context* ctx, uint32_t draw_count, uint32_t* t
  * the full count. */
 int rive_hip_last_frame(rive_hip_context* ctx, rive_hip_flush_desc* out, uint32_t capacity, uint32_t* count);
 
+/* Runs a frame's logical flushes in order, with the results of n
+ * rive_hip_flush calls.  When they are GEOMETRY flushes of one target whose
+ * DRAW records follow each other (descs[k].first_draw = descs[k-1].first_draw
+ * + descs[k-1].draw_count, at most 64 flushes), their curve counting, layout
+ * scans, record emission and span writing run once for the whole frame --
+ * each flush still laid out on its own from location 8 and contour ID 1 --
+ * and only the per-flush passes (tessellation, binning, raster) run per
+ * flush.  Replaces the per-logical-flush RenderContextImpl::flush calls of
+ * one RenderContext::flush (render_context.cpp:971-1018).
+ * rive_hip_pass_times then sums over the frame's flushes. */
+int rive_hip_flush_frame(rive_hip_context* ctx, const rive_hip_flush_desc* descs, uint32_t n);
+
 /* Waits for the context stream (the reference's postFlush/fence wait). */
 int rive_hip_sync(rive_hip_context* ctx);
 
@@ -318,7 +330,8 @@ int rive_hip_debug_readback(rive_hip_context* ctx, int what, void* dst,
                             size_t capacity, size_t* size);
 
 /* Per-pass device timings of the last flush in milliseconds (hipEvents on the
- * context stream): [0] ramps, [1] counting+spans, [2] tessellation,
+ * context stream), summed over the logical flushes of the last
+ * rive_hip_flush_frame call: [0] ramps, [1] counting+spans, [2] tessellation,
  * [3] binning, [4] raster+resolve, [5] total.  n <= 6. */
 int rive_hip_pass_times(rive_hip_context* ctx, float* ms, int n);
 

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -70,6 +71,8 @@ constexpr uint32_t kMaxFlushPaths = 1u << 24;
 constexpr uint32_t kMaxFlushTexels = 1u << 31;
 // Image table entries staged at context creation (what a no-alloc flush may use).
 constexpr uint32_t kNoAllocImages = 16;
+// Event sets kept for the logical flushes of one rive_hip_flush_frame call.
+constexpr uint32_t kMaxFrameEventSets = 64;
 } // namespace
 
 struct rive_hip_target
@@ -98,11 +101,17 @@ struct rive_hip_context
     CanvasDev* hostCanvases = nullptr; // pinned staging of the canvas table
     uint32_t hostCanvasesCap = 0;
     std::vector<rive_hip_canvas> lastCanvases;
-    hipEvent_t ev[kEvents] = {};
+    // HIP events per flush: the set of each logical flush of the last
+    // rive_hip_flush / rive_hip_flush_frame call (ev: the current flush's).
+    std::vector<std::array<hipEvent_t, kEvents>> evSets;
+    uint32_t evSetsUsed = 0;
+    hipEvent_t* ev = nullptr;
     float passMs[6] = {};
     bool timed = false;
     // last flush sizes, for debug readback
     uint32_t lastSpanCount = 0, lastContourCount = 0, lastTexels = 0, lastDrawCount = 0;
+    const void* lastSpans = nullptr;    // the last flush's TessVertexSpans (device)
+    const void* lastContours = nullptr; // and ContourData
     uint32_t lastGradRows = 0;
     rive_hip_flush_stats stats = {};
     std::vector<rive_hip_canvas> flushCanvases; // the current flush's canvas table (capacity kept)
@@ -273,8 +282,12 @@ int rive_hip_create(int device, rive_hip_context** out)
         ctx->lastCanvases.reserve(1);
         e = scratch_address(device, &ctx->scratchPtr);
     }
+    ctx->evSets.reserve(kMaxFrameEventSets);
+    ctx->evSets.emplace_back();
+    ctx->evSets[0].fill(nullptr);
     for (int i = 0; i < kEvents && e == hipSuccess; ++i)
-        e = hipEventCreate(&ctx->ev[i]);
+        e = hipEventCreate(&ctx->evSets[0][i]);
+    ctx->ev = ctx->evSets[0].data();
     if (e != hipSuccess)
     {
         delete ctx;
@@ -308,9 +321,10 @@ void rive_hip_destroy(rive_hip_context* ctx)
         (void)hipHostFree(ctx->hostImages);
     if (ctx->hostCanvases)
         (void)hipHostFree(ctx->hostCanvases);
-    for (auto& e : ctx->ev)
-        if (e)
-            (void)hipEventDestroy(e);
+    for (auto& set : ctx->evSets)
+        for (hipEvent_t e : set)
+            if (e)
+                (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -823,7 +837,7 @@ extern "C" int rive_hip_last_flush_canvases(rive_hip_context* ctx, rive_hip_canv
 
 namespace
 {
-int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d)
+int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d, const GeoLayout* pre = nullptr)
 {
     if (ctx == nullptr || d == nullptr || (d->target == nullptr && d->canvas_count == 0))
         return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null argument");
@@ -905,7 +919,8 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d)
     RIVE_TRY(ensure(ctx, ctx->scalars, 64));
     RIVE_TRY(ensure(ctx, ctx->drawTess, (size_t)std::max<uint32_t>(d->draw_count, 1) * sizeof(DrawTess)));
     HIP_TRY(hipMemsetAsync(ctx->scalars.ptr, 0, 64, ctx->stream));
-    HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream)); // flush start
+    if (pre == nullptr) // a frame's flushes have their start and counting events set already
+        HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream)); // flush start
 
     const SpanV* spans;
     uint32_t spanCount;
@@ -926,6 +941,17 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d)
         tessRows = std::max<uint32_t>((d->tess_vertex_count + kTessTextureWidth - 1) / kTessTextureWidth, 1);
         ctx->lastContourCount = d->contour_count;
         HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream)); // no counting pass
+    }
+    else if (d->mode == RIVE_HIP_FLUSH_GEOMETRY && pre != nullptr)
+    {
+        // laid out by the frame's counting pass
+        spans = pre->spans;
+        spanCount = pre->spanCount;
+        contours = pre->contours;
+        tessRows = pre->tessRows;
+        ctx->lastContourCount = pre->contourCount;
+        ctx->lastRecordCount = pre->recordCount;
+        ctx->lastTessVertexCount = pre->texelsUsed;
     }
     else if (d->mode == RIVE_HIP_FLUSH_GEOMETRY)
     {
@@ -982,6 +1008,8 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d)
         return fail(RIVE_HIP_ERROR_UNSUPPORTED, "flush too large: draws cover more than 2^31 texels");
     ctx->lastCanvases = canvases;
     ctx->lastSpanCount = spanCount;
+    ctx->lastSpans = spans;
+    ctx->lastContours = contours;
     ctx->lastDrawCount = d->draw_count;
     ctx->lastMode = d->mode;
     {
@@ -1027,12 +1055,94 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d)
 }
 } // namespace
 
+namespace
+{
+void record_frame_desc(rive_hip_context* ctx, const rive_hip_flush_desc* d);
+
+// Points ctx->ev at event set k (created on first use).
+int use_event_set(rive_hip_context* ctx, uint32_t k)
+{
+    while (ctx->evSets.size() <= k)
+    {
+        if (ctx->evSets.size() >= kMaxFrameEventSets)
+            return fail(RIVE_HIP_ERROR_UNSUPPORTED, "more than 64 logical flushes in one frame call");
+        ctx->evSets.emplace_back();
+        ctx->evSets.back().fill(nullptr);
+        for (hipEvent_t& e : ctx->evSets.back())
+            HIP_TRY(hipEventCreate(&e));
+    }
+    ctx->ev = ctx->evSets[k].data();
+    return RIVE_HIP_OK;
+}
+} // namespace
+
 extern "C" int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* d)
 {
     if (ctx != nullptr)
+    {
         ctx->lastFlushNoAlloc = false;
+        ctx->evSetsUsed = 1;
+        ctx->ev = ctx->evSets[0].data();
+    }
     const int rc = flush_impl(ctx, d);
     if (rc == RIVE_HIP_OK)
+        record_frame_desc(ctx, d);
+    return rc;
+}
+
+extern "C" int rive_hip_flush_frame(rive_hip_context* ctx, const rive_hip_flush_desc* descs, uint32_t n)
+{
+    if (ctx == nullptr || descs == nullptr || n == 0)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null argument or empty frame");
+    HIP_TRY(hipSetDevice(ctx->device));
+    ctx->lastFlushNoAlloc = false;
+    // One counting launch for the whole frame needs GEOMETRY flushes of one
+    // target whose draw records follow each other; anything else runs flush
+    // by flush.
+    bool fused = n <= kMaxFrameEventSets;
+    for (uint32_t k = 0; k < n && fused; ++k)
+        fused = descs[k].mode == RIVE_HIP_FLUSH_GEOMETRY && descs[k].canvas_count == 0 &&
+                descs[k].first_path_tess_location == descs[0].first_path_tess_location &&
+                (k == 0 || descs[k].first_draw == descs[k - 1].first_draw + descs[k - 1].draw_count) &&
+                descs[k].draw_count <= 0x7fffffffu;
+    if (fused)
+    {
+        uint64_t drawEnd = descs[n - 1].first_draw + descs[n - 1].draw_count;
+        fused = (uint64_t)ctx->bufs[RIVE_HIP_BUFFER_DRAW].size >= drawEnd * sizeof(rive_hip_draw) &&
+                drawEnd - descs[0].first_draw <= 0x7fffffffu;
+        for (int k = 0; k < RIVE_HIP_BUFFER_KIND_COUNT; ++k)
+            fused = fused && !ctx->bufs[k].mapped;
+    }
+    if (!fused)
+    {
+        for (uint32_t k = 0; k < n; ++k)
+            RIVE_TRY(rive_hip_flush(ctx, &descs[k]));
+        return RIVE_HIP_OK;
+    }
+    RIVE_TRY(use_event_set(ctx, 0));
+    HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream)); // frame start
+    std::vector<GeoLayout> layouts(n);
+    RIVE_TRY(geometry_counting_frame(ctx, descs, n, layouts.data()));
+    HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream)); // the frame's counting done
+    for (uint32_t k = 0; k < n; ++k)
+    {
+        RIVE_TRY(use_event_set(ctx, k));
+        if (k > 0)
+        {
+            HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream));
+            HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
+        }
+        ctx->evSetsUsed = k + 1;
+        RIVE_TRY(flush_impl(ctx, &descs[k], &layouts[k]));
+        record_frame_desc(ctx, &descs[k]);
+    }
+    return RIVE_HIP_OK;
+}
+
+namespace
+{
+void record_frame_desc(rive_hip_context* ctx, const rive_hip_flush_desc* d)
+{
     {
         // Copy the tables first: d may point into the recorded frame itself
         // (a frame re-issued from rive_hip_last_frame).
@@ -1054,8 +1164,8 @@ extern "C" int rive_hip_flush(rive_hip_context* ctx, const rive_hip_flush_desc* 
             ctx->frameDescs[k].canvases = ctx->frameCanvases[k].empty() ? nullptr : ctx->frameCanvases[k].data();
         }
     }
-    return rc;
 }
+} // namespace
 
 extern "C" int rive_hip_last_frame(rive_hip_context* ctx, rive_hip_flush_desc* out, uint32_t capacity,
                                    uint32_t* count)
@@ -1086,6 +1196,8 @@ extern "C" int rive_hip_flush_noalloc(rive_hip_context* ctx, const rive_hip_flus
         saved[i] = *all[i];
         *all[i] = DevBuf{};
     }
+    ctx->evSetsUsed = 1;
+    ctx->ev = ctx->evSets[0].data();
     ctx->arena.active = true;
     ctx->arena.base = ctx->scratchPtr;
     ctx->arena.used = 0;
@@ -1105,17 +1217,25 @@ extern "C" int rive_hip_pass_times(rive_hip_context* ctx, float* ms, int n)
     if (!ctx->timed)
         return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "no flush recorded");
     HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(hipEventSynchronize(ctx->ev[5]));
-    float t[6];
-    // ev: 6 start, 1 counted, 0 ramps start, 2 ramps end, 3 tess end, 4 bin end, 5 raster end
-    HIP_TRY(hipEventElapsedTime(&t[0], ctx->ev[0], ctx->ev[2]));
-    HIP_TRY(hipEventElapsedTime(&t[1], ctx->ev[6], ctx->ev[1]));
-    HIP_TRY(hipEventElapsedTime(&t[2], ctx->ev[2], ctx->ev[3]));
-    HIP_TRY(hipEventElapsedTime(&t[3], ctx->ev[3], ctx->ev[4]));
-    HIP_TRY(hipEventElapsedTime(&t[4], ctx->ev[4], ctx->ev[5]));
-    HIP_TRY(hipEventElapsedTime(&t[5], ctx->ev[6], ctx->ev[5]));
+    // Summed over the logical flushes of the last call (one for rive_hip_flush).
+    float sum[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t k = 0; k < std::max<uint32_t>(ctx->evSetsUsed, 1); ++k)
+    {
+        const hipEvent_t* ev = ctx->evSets[k].data();
+        HIP_TRY(hipEventSynchronize(ev[5]));
+        float t[6];
+        // ev: 6 start, 1 counted, 0 ramps start, 2 ramps end, 3 tess end, 4 bin end, 5 raster end
+        HIP_TRY(hipEventElapsedTime(&t[0], ev[0], ev[2]));
+        HIP_TRY(hipEventElapsedTime(&t[1], ev[6], ev[1]));
+        HIP_TRY(hipEventElapsedTime(&t[2], ev[2], ev[3]));
+        HIP_TRY(hipEventElapsedTime(&t[3], ev[3], ev[4]));
+        HIP_TRY(hipEventElapsedTime(&t[4], ev[4], ev[5]));
+        HIP_TRY(hipEventElapsedTime(&t[5], ev[6], ev[5]));
+        for (int i = 0; i < 6; ++i)
+            sum[i] += t[i];
+    }
     for (int i = 0; i < n; ++i)
-        ms[i] = t[i];
+        ms[i] = sum[i];
     return RIVE_HIP_OK;
 }
 
@@ -1134,11 +1254,11 @@ extern "C" int rive_hip_debug_readback(rive_hip_context* ctx, int what, void* ds
     switch (what)
     {
         case 0: // TessVertexSpans consumed by the last flush
-            src = geo ? ctx->geoSpans.ptr : (const void*)ctx->in.spans;
+            src = ctx->lastSpans;
             bytes = (size_t)ctx->lastSpanCount * sizeof(rive_hip_tess_span);
             break;
         case 1: // ContourData
-            src = geo ? ctx->geoContours.ptr : (const void*)ctx->in.contours;
+            src = ctx->lastContours;
             bytes = (size_t)ctx->lastContourCount * sizeof(rive_hip_contour);
             break;
         case 2: // tessellation texture

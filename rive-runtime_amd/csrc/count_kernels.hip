@@ -836,10 +836,15 @@ __global__ void count_kernel(const rive_hip_draw* __restrict__ draws, uint32_t n
     recordCounts[i] = c.records;
 }
 
+// tessOffsets/contourOffsets/recordOffsets: exclusive scans over the draws
+// (record and contour storage); tessLocal/contourLocal: each draw's
+// tessellation location and first contour ID within its own logical flush
+// (the same scans when the draws form one flush).
 __global__ void emit_kernel(rive_hip_draw* __restrict__ draws, uint32_t n, const uint8_t* __restrict__ verbs,
                             const V2d* __restrict__ points, const uint32_t* __restrict__ tessOffsets,
                             const uint32_t* __restrict__ contourOffsets,
-                            const uint32_t* __restrict__ recordOffsets, uint32_t firstLocation,
+                            const uint32_t* __restrict__ recordOffsets, const uint32_t* __restrict__ tessLocal,
+                            const uint32_t* __restrict__ contourLocal, uint32_t firstLocation,
                             CubicRec* __restrict__ recs, ContourV* __restrict__ contours,
                             uint32_t* __restrict__ overflow)
 {
@@ -847,7 +852,7 @@ __global__ void emit_kernel(rive_hip_draw* __restrict__ draws, uint32_t n, const
     if (i >= n)
         return;
     rive_hip_draw d = draws[i];
-    const uint32_t loc = firstLocation + tessOffsets[i];
+    const uint32_t loc = firstLocation + tessLocal[i];
     const uint32_t count = tessOffsets[i + 1] - tessOffsets[i];
     Emit e;
     e.recs = recs + recordOffsets[i];
@@ -857,7 +862,7 @@ __global__ void emit_kernel(rive_hip_draw* __restrict__ draws, uint32_t n, const
     e.contourCap = contourOffsets[i + 1] - contourOffsets[i];
     e.rec = 0;
     e.contourIndex = 0;
-    e.contourID = contourOffsets[i] + 1u; // contour IDs are 1-based per flush
+    e.contourID = contourLocal[i] + 1u; // contour IDs are 1-based per logical flush
     e.dirs = d.contour_directions;
     e.nextPad = 0;
     // pushTessellationData (draw.cpp:1899-1990)
@@ -965,15 +970,35 @@ __device__ __forceinline__ void write_span(SpanV* out, const float* pts, const f
     *out = s;
 }
 
+// Up to this many logical flushes are laid out by one frame launch.
+constexpr uint32_t kMaxFrameFlushes = 64;
+// Per-flush boundaries passed by value: v[k] is where flush k starts (in
+// draws, records, ...), v[n] the end.
+struct FrameBounds
+{
+    uint32_t n;
+    uint32_t v[kMaxFrameFlushes + 1];
+};
+__device__ __forceinline__ uint32_t frame_flush_of(const FrameBounds& b, uint32_t i)
+{
+    uint32_t k = 0;
+    while (k + 1 < b.n && b.v[k + 1] <= i)
+        ++k;
+    return k;
+}
+
+// Spans of the records of several logical flushes, each flush's spans after
+// its two padding spans: record i of flush k writes at
+// 2 * (k + 1) + spanOffsets[i].  One flush: recordBounds {1, {0, n}}.
 __global__ void span_write_kernel(const CubicRec* __restrict__ recs, uint32_t n,
-                                  const uint32_t* __restrict__ spanOffsets, uint32_t firstSpan,
+                                  const uint32_t* __restrict__ spanOffsets, const FrameBounds recordBounds,
                                   SpanV* __restrict__ spans)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     const CubicRec r = recs[i];
-    SpanV* out = spans + firstSpan + spanOffsets[i];
+    SpanV* out = spans + 2u * (frame_flush_of(recordBounds, i) + 1u) + spanOffsets[i];
     const int32_t W = (int32_t)kTessTextureWidth;
     const float qnan = __uint_as_float(0x7fc00000u);
     if (r.dirs == 0u)
@@ -1038,6 +1063,35 @@ __global__ void span_write_kernel(const CubicRec* __restrict__ recs, uint32_t n,
     }
 }
 
+// Per draw of a frame: its location and first contour ID within its own
+// logical flush (the frame-wide scans less their values at the flush start).
+__global__ void frame_local_kernel(uint32_t n, const FrameBounds drawBounds, const uint32_t* __restrict__ tessScan,
+                                   const uint32_t* __restrict__ contourScan, uint32_t* __restrict__ tessLocal,
+                                   uint32_t* __restrict__ contourLocal)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t first = drawBounds.v[frame_flush_of(drawBounds, i)];
+    tessLocal[i] = tessScan[i] - tessScan[first];
+    contourLocal[i] = contourScan[i] - contourScan[first];
+}
+
+// The scans' values at the flush boundaries: out[j * (n + 1) + k] = scan_j at
+// bounds.v[k] for the three scans.
+__global__ void frame_bounds_kernel(const FrameBounds bounds, const uint32_t* __restrict__ a,
+                                    const uint32_t* __restrict__ b, const uint32_t* __restrict__ c,
+                                    uint32_t* __restrict__ out)
+{
+    const uint32_t k = threadIdx.x;
+    if (k > bounds.n)
+        return;
+    const uint32_t m = bounds.n + 1;
+    out[k] = a[bounds.v[k]];
+    out[m + k] = b[bounds.v[k]];
+    out[2 * m + k] = c != nullptr ? c[bounds.v[k]] : 0u;
+}
+
 // Padding spans (render_context.cpp:1550-1568): 8 vertices at 0 and one after
 // the last path.
 __global__ void padding_span_kernel(SpanV* __restrict__ spans, const uint32_t* __restrict__ tessTotal,
@@ -1054,6 +1108,27 @@ __global__ void padding_span_kernel(SpanV* __restrict__ spans, const uint32_t* _
         uint32_t y = endLoc / kTessTextureWidth;
         int32_t x0 = (int32_t)(endLoc % kTessTextureWidth);
         write_span(&spans[1], zero, zero, (float)y, x0, x0 + 1, qnan, -1, -1, counts, 0u);
+    }
+}
+
+// The padding spans of every logical flush of a frame: block k writes flush
+// k's two at spanBase.v[k], its end padding after tessTotals.v[k] texels.
+__global__ void frame_padding_kernel(SpanV* __restrict__ spans, const FrameBounds spanBase,
+                                     const FrameBounds tessTotals, uint32_t firstLocation)
+{
+    const uint32_t k = blockIdx.x;
+    const float zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const float qnan = __uint_as_float(0x7fc00000u);
+    const uint32_t counts = (1u << 20);
+    SpanV* out = spans + spanBase.v[k];
+    if (threadIdx.x == 0)
+        write_span(&out[0], zero, zero, 0.f, 0, (int32_t)kPatchSpan, qnan, -1, -1, counts, 0u);
+    if (threadIdx.x == 1)
+    {
+        uint32_t endLoc = firstLocation + tessTotals.v[k];
+        uint32_t y = endLoc / kTessTextureWidth;
+        int32_t x0 = (int32_t)(endLoc % kTessTextureWidth);
+        write_span(&out[1], zero, zero, (float)y, x0, x0 + 1, qnan, -1, -1, counts, 0u);
     }
 }
 } // namespace rivehip
@@ -1115,7 +1190,8 @@ int geometry_counting(rive_hip_context* ctx, const rive_hip_flush_desc* d, const
     RIVE_TRY(ensure(ctx, ctx->geoContours, (size_t)std::max<uint32_t>(contourTotal, 1) * sizeof(ContourV)));
     if (n > 0)
         emit_kernel<<<(n + 31) / 32, 32, 0, ctx->stream>>>(draws, n, verbs, points, tessOffsets,
-                                                              contourOffsets, recordOffsets, firstLocation,
+                                                              contourOffsets, recordOffsets, tessOffsets,
+                                                              contourOffsets, firstLocation,
                                                               P<CubicRec>(ctx->geoRecords),
                                                               P<ContourV>(ctx->geoContours), scal + 13);
     HIP_TRY(hipGetLastError());
@@ -1142,9 +1218,13 @@ int geometry_counting(rive_hip_context* ctx, const rive_hip_flush_desc* d, const
     const uint32_t allSpans = spanTotal + 2; // + padding spans
     RIVE_TRY(ensure(ctx, ctx->geoSpans, (size_t)allSpans * sizeof(SpanV)));
     padding_span_kernel<<<1, 64, 0, ctx->stream>>>(P<SpanV>(ctx->geoSpans), scal + 8, firstLocation);
+    FrameBounds oneFlush{};
+    oneFlush.n = 1;
+    oneFlush.v[0] = 0;
+    oneFlush.v[1] = recordTotal;
     if (recordTotal > 0)
         span_write_kernel<<<(recordTotal + 255) / 256, 256, 0, ctx->stream>>>(
-            P<CubicRec>(ctx->geoRecords), recordTotal, P<uint32_t>(ctx->geoSpanOffsets), 2u,
+            P<CubicRec>(ctx->geoRecords), recordTotal, P<uint32_t>(ctx->geoSpanOffsets), oneFlush,
             P<SpanV>(ctx->geoSpans));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
@@ -1156,6 +1236,162 @@ int geometry_counting(rive_hip_context* ctx, const rive_hip_flush_desc* d, const
     ctx->lastContourCount = contourTotal;
     ctx->lastRecordCount = recordTotal;
     ctx->lastTessVertexCount = texelsUsed;
+    return RIVE_HIP_OK;
+}
+} // namespace
+
+namespace
+{
+// One logical flush's layout from a frame's counting pass.
+struct GeoLayout
+{
+    const rivehip::SpanV* spans;
+    uint32_t spanCount;
+    const rivehip::ContourV* contours;
+    uint32_t contourCount;
+    uint32_t recordCount;
+    uint32_t texelsUsed; // incl. the 8 + 1 padding texels
+    uint32_t tessRows;
+};
+
+// GEOMETRY-mode front half for several logical flushes whose draws are
+// contiguous in the DRAW buffer (descs[k].first_draw follow each other):
+// one counting launch, frame-wide scans, one emission launch and one span
+// pass, each flush laid out from location firstLocation and contour ID 1 on
+// its own, exactly as n separate flushes would be.
+int geometry_counting_frame(rive_hip_context* ctx, const rive_hip_flush_desc* descs, uint32_t nf,
+                            GeoLayout* out)
+{
+    using namespace rivehip;
+    const uint64_t firstDraw = descs[0].first_draw;
+    FrameBounds drawBounds{};
+    drawBounds.n = nf;
+    uint64_t n64 = 0;
+    for (uint32_t k = 0; k < nf; ++k)
+    {
+        drawBounds.v[k] = (uint32_t)n64;
+        n64 += descs[k].draw_count;
+    }
+    drawBounds.v[nf] = (uint32_t)n64;
+    const uint32_t n = (uint32_t)n64;
+    auto* draws = reinterpret_cast<rive_hip_draw*>(ctx->bufs[RIVE_HIP_BUFFER_DRAW].dev) + firstDraw;
+    const auto* verbs = reinterpret_cast<const uint8_t*>(ctx->bufs[RIVE_HIP_BUFFER_VERB].dev);
+    const auto* points = reinterpret_cast<const V2d*>(ctx->bufs[RIVE_HIP_BUFFER_POINT].dev);
+    if (n > 0 && (verbs == nullptr || points == nullptr))
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "GEOMETRY flush without VERB/POINT buffers");
+    const uint32_t firstLocation = descs[0].first_path_tess_location ? descs[0].first_path_tess_location : kPatchSpan;
+    RIVE_TRY(ensure(ctx, ctx->scalars, 64));
+    uint32_t* scal = P<uint32_t>(ctx->scalars);
+    HIP_TRY(hipMemsetAsync(ctx->scalars.ptr, 0, 64, ctx->stream));
+    // 5 arrays of n+1 (tess, contour, record counts; tess, contour locals) +
+    // 3 scans of n+1 + bounds values.
+    RIVE_TRY(ensure(ctx, ctx->geoCounts, ((size_t)8 * (n + 1) + 4 * (kMaxFrameFlushes + 1) + 16) * 4));
+    uint32_t* tessCounts = P<uint32_t>(ctx->geoCounts);
+    uint32_t* contourCounts = tessCounts + (n + 1);
+    uint32_t* recordCounts = contourCounts + (n + 1);
+    uint32_t* tessOffsets = recordCounts + (n + 1);
+    uint32_t* contourOffsets = tessOffsets + (n + 1);
+    uint32_t* recordOffsets = contourOffsets + (n + 1);
+    uint32_t* tessLocal = recordOffsets + (n + 1);
+    uint32_t* contourLocal = tessLocal + (n + 1);
+    uint32_t* boundVals = contourLocal + (n + 1);
+    if (n > 0)
+        count_kernel<<<(n + 31) / 32, 32, 0, ctx->stream>>>(
+            draws, n, verbs, ctx->bufs[RIVE_HIP_BUFFER_VERB].size, points,
+            ctx->bufs[RIVE_HIP_BUFFER_POINT].size / sizeof(V2d), scal + 12, tessCounts, contourCounts,
+            recordCounts);
+    HIP_TRY(hipGetLastError());
+    RIVE_TRY(device_scan(ctx, tessCounts, n, tessOffsets, scal + 8));
+    RIVE_TRY(device_scan(ctx, contourCounts, n, contourOffsets, scal + 9));
+    RIVE_TRY(device_scan(ctx, recordCounts, n, recordOffsets, scal + 10));
+    frame_bounds_kernel<<<1, kMaxFrameFlushes + 1, 0, ctx->stream>>>(drawBounds, tessOffsets, contourOffsets,
+                                                                     recordOffsets, boundVals);
+    HIP_TRY(hipGetLastError());
+    // One readback: the totals, the bounds and the range check flag.
+    const uint32_t m = nf + 1;
+    std::vector<uint32_t> hv(3 * m + 8);
+    HIP_TRY(hipMemcpyAsync(hv.data(), boundVals, (size_t)3 * m * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(hv.data() + 3 * m, scal + 8, 20, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (hv[3 * m + 4] != 0)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT,
+                    "draw record with verb/point ranges outside the VERB/POINT buffers");
+    const uint32_t* tb = hv.data();
+    const uint32_t* cb = hv.data() + m;
+    const uint32_t* rb = hv.data() + 2 * m;
+    if (hv[3 * m + 2] == kScanSaturated)
+        return fail(RIVE_HIP_ERROR_UNSUPPORTED, "flush too large: records exceed 2^32-1");
+    FrameBounds recordBounds{};
+    FrameBounds tessTotals{};
+    recordBounds.n = tessTotals.n = nf;
+    for (uint32_t k = 0; k < nf; ++k)
+    {
+        const uint32_t tess = tb[k + 1] - tb[k], contoursK = cb[k + 1] - cb[k];
+        if (contoursK > 0xffffu)
+            return fail(RIVE_HIP_ERROR_UNSUPPORTED, "more than 65535 contours in one flush");
+        if (tess > kMaxFlushTexels || tb[k + 1] < tb[k])
+            return fail(RIVE_HIP_ERROR_UNSUPPORTED, "flush too large: more than 2^31 tessellation vertices");
+        recordBounds.v[k] = rb[k];
+        tessTotals.v[k] = tess;
+        out[k].contourCount = contoursK;
+        out[k].recordCount = rb[k + 1] - rb[k];
+        out[k].texelsUsed = firstLocation + tess + 1u;
+        out[k].tessRows = std::max<uint32_t>((out[k].texelsUsed + kTessTextureWidth - 1) / kTessTextureWidth, 1);
+    }
+    recordBounds.v[nf] = rb[nf];
+    const uint32_t recordTotal = rb[nf], contourTotal = cb[nf];
+    RIVE_TRY(ensure(ctx, ctx->geoRecords, (size_t)std::max<uint32_t>(recordTotal, 1) * sizeof(CubicRec)));
+    RIVE_TRY(ensure(ctx, ctx->geoContours, (size_t)std::max<uint32_t>(contourTotal, 1) * sizeof(ContourV)));
+    if (n > 0)
+    {
+        frame_local_kernel<<<(n + 255) / 256, 256, 0, ctx->stream>>>(n, drawBounds, tessOffsets, contourOffsets,
+                                                                     tessLocal, contourLocal);
+        emit_kernel<<<(n + 31) / 32, 32, 0, ctx->stream>>>(draws, n, verbs, points, tessOffsets, contourOffsets,
+                                                              recordOffsets, tessLocal, contourLocal, firstLocation,
+                                                              P<CubicRec>(ctx->geoRecords),
+                                                              P<ContourV>(ctx->geoContours), scal + 13);
+    }
+    HIP_TRY(hipGetLastError());
+    // spans
+    RIVE_TRY(ensure(ctx, ctx->geoSpanCounts, (size_t)(recordTotal + 1) * 4));
+    RIVE_TRY(ensure(ctx, ctx->geoSpanOffsets, (size_t)(recordTotal + 1) * 4));
+    if (recordTotal > 0)
+        span_count_kernel<<<(recordTotal + 255) / 256, 256, 0, ctx->stream>>>(
+            P<CubicRec>(ctx->geoRecords), recordTotal, P<uint32_t>(ctx->geoSpanCounts));
+    RIVE_TRY(device_scan(ctx, P<uint32_t>(ctx->geoSpanCounts), recordTotal, P<uint32_t>(ctx->geoSpanOffsets),
+                         scal + 11));
+    frame_bounds_kernel<<<1, kMaxFrameFlushes + 1, 0, ctx->stream>>>(recordBounds, P<uint32_t>(ctx->geoSpanOffsets),
+                                                                     P<uint32_t>(ctx->geoSpanOffsets), nullptr,
+                                                                     boundVals);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(hv.data(), boundVals, (size_t)m * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(hv.data() + 3 * m, scal + 11, 12, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (hv[3 * m + 2] != 0) // scal[13]: emission overflow
+        return fail(RIVE_HIP_ERROR_DEVICE, "curve emission disagreed with the counting pass");
+    if (hv[3 * m] == kScanSaturated)
+        return fail(RIVE_HIP_ERROR_UNSUPPORTED, "flush too large: tessellation spans exceed 2^32-1");
+    const uint32_t spanTotal = hv[3 * m];
+    FrameBounds spanBase{};
+    spanBase.n = nf;
+    for (uint32_t k = 0; k < nf; ++k)
+    {
+        // flush k: its two padding spans, then its records' spans
+        spanBase.v[k] = hv[k] + 2u * k;
+        out[k].spanCount = hv[k + 1] - hv[k] + 2u;
+    }
+    RIVE_TRY(ensure(ctx, ctx->geoSpans, ((size_t)spanTotal + 2u * nf) * sizeof(SpanV)));
+    frame_padding_kernel<<<nf, 64, 0, ctx->stream>>>(P<SpanV>(ctx->geoSpans), spanBase, tessTotals, firstLocation);
+    if (recordTotal > 0)
+        span_write_kernel<<<(recordTotal + 255) / 256, 256, 0, ctx->stream>>>(
+            P<CubicRec>(ctx->geoRecords), recordTotal, P<uint32_t>(ctx->geoSpanOffsets), recordBounds,
+            P<SpanV>(ctx->geoSpans));
+    HIP_TRY(hipGetLastError());
+    for (uint32_t k = 0; k < nf; ++k)
+    {
+        out[k].spans = P<SpanV>(ctx->geoSpans) + spanBase.v[k];
+        out[k].contours = P<ContourV>(ctx->geoContours) + cb[k];
+    }
     return RIVE_HIP_OK;
 }
 } // namespace

@@ -1127,10 +1127,7 @@ bool RenderContextHIPImpl::flushFrame(const std::vector<gpu::FlushDescriptor>& f
         !upload(RIVE_HIP_BUFFER_POINT, points.data(), points.size() * sizeof(float)) ||
         !upload(RIVE_HIP_BUFFER_GRAD_SPAN, grads.data(), grads.size() * sizeof(rive_hip_grad_span)))
         return false;
-    for (const rive_hip_flush_desc& d : descs)
-        if (rive_hip_flush(m_ctx, &d) != RIVE_HIP_OK)
-            return false;
-    return true;
+    return descs.empty() || rive_hip_flush_frame(m_ctx, descs.data(), (uint32_t)descs.size()) == RIVE_HIP_OK;
 }
 } // namespace rive
 

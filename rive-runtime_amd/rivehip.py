@@ -101,7 +101,7 @@ EXPORTED = [
     "rive_hip_render_stream", "rive_hip_last_flush_stats", "rive_hip_image_create",
     "rive_hip_image_destroy", "rive_hip_render_streams", "rive_hip_last_flush_canvases",
     "rive_stream_generate_batch", "rive_stream_contour_bound", "rive_hip_flush_noalloc",
-    "rive_hip_count_draws", "rive_hip_render_stream_ex", "rive_hip_last_frame",
+    "rive_hip_count_draws", "rive_hip_render_stream_ex", "rive_hip_last_frame", "rive_hip_flush_frame",
 ]
 
 _lib = None
@@ -141,6 +141,7 @@ def lib():
     L.rive_hip_render_stream_ex.argtypes = [vp, ctypes.POINTER(Stream), vp, u32, ctypes.POINTER(u32)]
     L.rive_hip_count_draws.argtypes = [vp, u32, vp, vp]
     L.rive_hip_last_frame.argtypes = [vp, ctypes.POINTER(FlushDesc), u32, ctypes.POINTER(u32)]
+    L.rive_hip_flush_frame.argtypes = [vp, ctypes.POINTER(FlushDesc), u32]
     L.rive_hip_render_streams.argtypes = [vp, ctypes.POINTER(ctypes.POINTER(Stream)),
                                           ctypes.POINTER(vp), u32]
     L.rive_hip_last_flush_canvases.argtypes = [vp, ctypes.POINTER(Canvas), u32, ctypes.POINTER(u32)]
@@ -292,6 +293,14 @@ class Context:
         arr = (FlushDesc * max(n.value, 1))()
         check(self.L.rive_hip_last_frame(self.h, arr, n.value, ctypes.byref(n)))
         return [arr[k] for k in range(n.value)]
+
+    def flush_frame(self, descs, target=None):
+        """rive_hip_flush_frame over a frame's logical flush descriptors."""
+        arr = (FlushDesc * len(descs))(*descs)
+        if target is not None:
+            for k in range(len(descs)):
+                arr[k].target = target
+        check(self.L.rive_hip_flush_frame(self.h, arr, len(descs)))
 
     def reflush_frame(self, descs, target=None):
         """Re-issues a frame's logical flushes from the records resident in

@@ -156,6 +156,10 @@ def test_full_workload_modes_agree_and_repeat(hip_ctx):
         a = hip_ctx.read_pixels(t, 1920, 1080)
         hip_ctx.render_stream(s, t)
         b = hip_ctx.read_pixels(t, 1920, 1080)
+        # the same six flushes one rive_hip_flush at a time (the frame call
+        # counts them in one launch)
+        hip_ctx.reflush_frame(hip_ctx.last_frame(), t)
+        seq = hip_ctx.read_pixels(t, 1920, 1080)
         f.flush_all_spans_on(hip_ctx, t, clear_color=0, dither=1)
         c2 = hip_ctx.read_pixels(t, 1920, 1080)
     finally:
@@ -163,5 +167,6 @@ def test_full_workload_modes_agree_and_repeat(hip_ctx):
         f.close()
         rivehip.free_stream(s)
     assert same_bits(a, b)
+    assert same_bits(a, seq)
     assert same_bits(a, c2)
     assert a[..., 3].max() > 0
