@@ -248,7 +248,28 @@ typedef struct rive_hip_flush_desc
      * way.  All zero for a frame of one logical flush. */
     uint64_t first_path, first_paint, first_paint_aux, first_contour;
     uint64_t first_grad_span, first_tess_span, first_draw;
+    /* SPANS mode, as the reference hands a flush over: instead of per-draw
+     * rive_hip_draw records, its DrawBatch list (FlushDescriptor::drawList,
+     * gpu.hpp:1219-1281) -- runs of midpoint-fan patches.  When
+     * patch_batch_count > 0 the device derives each path's tessellation range
+     * from its TessVertexSpans (each names its contour, whose ContourData
+     * names the path), draws the paths in path-ID order, takes each path's
+     * image paint from the batch holding its first patch, and ignores
+     * draw_count. */
+    const struct rive_hip_patch_batch* patch_batches;
+    uint32_t patch_batch_count;
 } rive_hip_flush_desc;
+
+/* One DrawBatch of DrawType::midpointFanPatches: patches first_patch ..
+ * first_patch + patch_count - 1 of the tessellation texture (8 texels each,
+ * DrawBatch::baseElement / elementCount) and the batch's image paint. */
+typedef struct rive_hip_patch_batch
+{
+    uint32_t first_patch;
+    uint32_t patch_count;
+    uint32_t image_index;   /* 0 none, k = rive_hip_flush_desc.images[k - 1] */
+    uint32_t image_sampler; /* ImageSampler::asKey() */
+} rive_hip_patch_batch;
 
 /* One canvas of a batched flush: paths first_path .. first_path+path_count-1
  * render into `target`.  No counterpart in the reference, whose FlushDescriptor

@@ -98,6 +98,7 @@ struct rive_hip_context
     ImageDesc* hostImages = nullptr; // pinned staging of the image table
     uint32_t hostImagesCap = 0;
     DevBuf canvasTable, pathCanvas;
+    DevBuf patchTable, patchWork; // SPANS flushes described by patch batches
     CanvasDev* hostCanvases = nullptr; // pinned staging of the canvas table
     uint32_t hostCanvasesCap = 0;
     std::vector<rive_hip_canvas> lastCanvases;
@@ -145,7 +146,7 @@ struct rive_hip_context
 namespace
 {
 // The scratch buffers of a context (every DevBuf member), in one list.
-constexpr int kDevBufCount = 29;
+constexpr int kDevBufCount = 31;
 void devbufs(rive_hip_context* c, DevBuf** out)
 {
     DevBuf* all[kDevBufCount] = {&c->spanTexelCounts, &c->spanTexelOffsets, &c->scanPartials, &c->scalars,
@@ -154,7 +155,8 @@ void devbufs(rive_hip_context* c, DevBuf** out)
                                  &c->ramps, &c->drawTess, &c->geoCounts, &c->geoRecords,
                                  &c->geoContours, &c->geoSpans, &c->geoSpanCounts, &c->geoSpanOffsets,
                                  &c->verts, &c->imageTable, &c->pathImage, &c->canvasTable,
-                                 &c->pathCanvas, &c->cellPacked, &c->pathDraws, &c->pathPaint};
+                                 &c->pathCanvas, &c->cellPacked, &c->pathDraws, &c->pathPaint,
+                                 &c->patchTable, &c->patchWork};
     for (int i = 0; i < kDevBufCount; ++i)
         out[i] = all[i];
 }
@@ -557,8 +559,145 @@ int read_total(rive_hip_context* ctx, const uint32_t* devSlot, uint32_t* value, 
     return RIVE_HIP_OK;
 }
 
+// ---- Per-path ranges for a SPANS flush described by its DrawBatch list
+// (reference input).  Every TessVertexSpan names its contour, whose
+// ContourData names the path, and covers one row's run of texels forward
+// (and, for a double-sided fill, one mirrored run): a path's forward range is
+// the union of its spans' forward runs, its mirrored range the union of their
+// reflections.  Paths are drawn in path-ID (submission) order; the patch
+// batches give the image paint of the paths whose patches they cover.
+struct PatchBatchDev
+{
+    uint32_t firstPatch, endPatch;
+    uint32_t image; // image index | sampler << 16
+    uint32_t pad;
+};
+
+__global__ void span_ranges_kernel(const SpanV* __restrict__ spans, uint32_t n, const ContourV* __restrict__ contours,
+                                   uint32_t contourCount, uint32_t pathCount, uint32_t texels,
+                                   uint32_t* __restrict__ fwdMin, uint32_t* __restrict__ fwdMax,
+                                   uint32_t* __restrict__ mirMin, uint32_t* __restrict__ mirMax,
+                                   uint32_t* __restrict__ bad)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const SpanV s = spans[i];
+    const uint32_t cid = s.flags & CONTOUR_ID_MASK;
+    if (cid == 0u)
+        return; // padding
+    if (cid > contourCount)
+    {
+        atomicAdd(bad, 1u);
+        return;
+    }
+    const uint32_t path = contours[cid - 1].pathID;
+    if (path == 0u || path > pathCount)
+    {
+        atomicAdd(bad, 1u);
+        return;
+    }
+    const int32_t W = (int32_t)kTessTextureWidth;
+    for (int rect = 0; rect < 2; ++rect)
+    {
+        const float yf = rect == 0 ? s.y : s.ry;
+        if (!(yf >= 0.f) || yf >= (float)(texels / kTessTextureWidth))
+            continue;
+        const int32_t packed = rect == 0 ? s.x0x1 : s.rx0x1;
+        const int32_t x0 = (int32_t)((uint32_t)packed << 16) >> 16, x1 = packed >> 16;
+        const int32_t lo = max(min(x0, x1), 0), hi = min(max(x0, x1), W);
+        if (lo >= hi)
+            continue;
+        const uint32_t row = (uint32_t)yf * kTessTextureWidth;
+        const bool mirrored = x1 < x0;
+        atomicMin(mirrored ? &mirMin[path] : &fwdMin[path], row + (uint32_t)lo);
+        atomicMax(mirrored ? &mirMax[path] : &fwdMax[path], row + (uint32_t)hi);
+    }
+}
+
+__global__ void span_draw_flags_kernel(uint32_t pathCount, const uint32_t* __restrict__ fwdMax,
+                                       const uint32_t* __restrict__ mirMax, uint32_t* __restrict__ flag)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; // path i + 1
+    if (i < pathCount)
+        flag[i] = (fwdMax[i + 1] | mirMax[i + 1]) != 0u ? 1u : 0u;
+}
+
+// One DrawTess per drawn path (validated like draw_tess_kernel); cnt[k] = its
+// forward texels for the primitive scan.
+__global__ void span_draws_kernel(uint32_t pathCount, const uint32_t* __restrict__ flag,
+                                  const uint32_t* __restrict__ drawIdx, const uint32_t* __restrict__ fwdMin,
+                                  const uint32_t* __restrict__ fwdMax, const uint32_t* __restrict__ mirMin,
+                                  const uint32_t* __restrict__ mirMax, const PathDataV* __restrict__ paths,
+                                  const PatchBatchDev* __restrict__ batches, uint32_t nb, uint32_t imageCount,
+                                  DrawTess* __restrict__ out, uint32_t* __restrict__ cnt,
+                                  uint32_t* __restrict__ pathImage, uint32_t* __restrict__ bad)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pathCount || flag[i] == 0u)
+        return;
+    const uint32_t path = i + 1, k = drawIdx[i];
+    DrawTess t{};
+    t.pathID = path;
+    t.isStroke = paths[path].strokeRadius != 0.f ? 1u : 0u;
+    const uint32_t nf = fwdMax[path] > 0u ? fwdMax[path] - fwdMin[path] : 0u;
+    const uint32_t nm = mirMax[path] > 0u ? mirMax[path] - mirMin[path] : 0u;
+    t.forwardBase = nf != 0u ? fwdMin[path] : 0u;
+    t.forwardCount = nf;
+    t.tessVertexCount = nf + nm;
+    bool ok;
+    if (nm != 0u)
+    {
+        // a double-sided fill: reverseThenForward when the mirrored run comes
+        // first (draw.cpp:1899-1990)
+        t.dirs = (nf == 0u || mirMin[path] < fwdMin[path]) ? 3u : 2u;
+        t.tessLocation = nf != 0u ? min(fwdMin[path], mirMin[path]) : mirMin[path];
+        ok = nf == nm && !t.isStroke;
+    }
+    else
+    {
+        t.dirs = 0u;
+        t.tessLocation = t.forwardBase;
+        ok = t.isStroke != 0u; // raster-ordering fills are double-sided
+    }
+    // the batch holding the path's first patch supplies its image paint
+    const uint32_t patch = t.tessLocation / 8u;
+    for (uint32_t b = 0; b < nb; ++b)
+        if (patch >= batches[b].firstPatch && patch < batches[b].endPatch)
+        {
+            const uint32_t img = batches[b].image;
+            if ((img & 0xffffu) != 0u)
+            {
+                if ((img & 0xffffu) > imageCount || (img >> 16) >= 18u)
+                    ok = false;
+                else
+                    pathImage[path] = img;
+            }
+            break;
+        }
+    if (!ok)
+    {
+        atomicAdd(bad, 1u);
+        t.forwardCount = 0u;
+        t.pathID = 0u;
+    }
+    out[k] = t;
+    cnt[k] = t.forwardCount;
+}
+
+__global__ void draw_prim_base_kernel(uint32_t n, const uint32_t* __restrict__ base, DrawTess* __restrict__ out)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n)
+        out[k].primBase = base[k];
+}
+
 // Raster half of the pipeline, shared by both flush modes.  Inputs: the
 // tessellation texture, contours, paths, paints and per-draw ranges.
+int derive_patch_draws(rive_hip_context* ctx, const rive_hip_flush_desc* d, const SpanV* spans, uint32_t spanCount,
+                       const ContourV* contours, uint32_t contourCount, uint32_t texels, uint32_t* drawCount,
+                       uint32_t* totalPrims);
+
 int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const SpanV* spans,
                     uint32_t spanCount, const ContourV* contours, uint32_t contourCount, uint32_t tessRows,
                     DrawTess* drawTess, uint32_t drawCount, uint32_t totalPrims)
@@ -642,6 +781,14 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
         ctx->lastTexels = texels;
     }
     HIP_TRY(hipGetLastError());
+    if (d->mode == RIVE_HIP_FLUSH_SPANS && d->patch_batch_count > 0)
+    {
+        RIVE_TRY(derive_patch_draws(ctx, d, spans, spanCount, contours, contourCount, texels, &drawCount,
+                                    &totalPrims));
+        drawTess = P<DrawTess>(ctx->drawTess);
+        ctx->stats.draw_count = drawCount;
+        ctx->stats.prim_count = totalPrims;
+    }
     HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
 
     // ---- expansion + binning
@@ -729,6 +876,68 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
             canvasCount, d->dither);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
+    return RIVE_HIP_OK;
+}
+
+int derive_patch_draws(rive_hip_context* ctx, const rive_hip_flush_desc* d, const SpanV* spans, uint32_t spanCount,
+                       const ContourV* contours, uint32_t contourCount, uint32_t texels, uint32_t* drawCount,
+                       uint32_t* totalPrims)
+{
+    if (d->patch_batches == nullptr)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "patch_batch_count without patch_batches");
+    const uint32_t nb = d->patch_batch_count;
+    std::vector<PatchBatchDev> table(nb);
+    for (uint32_t b = 0; b < nb; ++b)
+    {
+        const rive_hip_patch_batch& pb = d->patch_batches[b];
+        if ((uint64_t)pb.first_patch + pb.patch_count > (uint64_t)texels / 8u)
+            return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "patch batch outside the tessellation texture");
+        table[b] = {pb.first_patch, pb.first_patch + pb.patch_count, pb.image_index | (pb.image_sampler << 16), 0u};
+    }
+    const uint32_t np = d->path_count;
+    *drawCount = 0;
+    *totalPrims = 0;
+    if (np == 0)
+        return RIVE_HIP_OK;
+    uint32_t* scal = P<uint32_t>(ctx->scalars);
+    RIVE_TRY(ensure(ctx, ctx->patchTable, sizeof(PatchBatchDev) * std::max<uint32_t>(nb, 1)));
+    if (nb > 0)
+        HIP_TRY(hipMemcpyAsync(ctx->patchTable.ptr, table.data(), sizeof(PatchBatchDev) * nb,
+                               hipMemcpyHostToDevice, ctx->stream));
+    // 8 arrays of np + 2 words
+    const size_t m = (size_t)np + 2;
+    RIVE_TRY(ensure(ctx, ctx->patchWork, 8 * m * 4));
+    uint32_t* w = P<uint32_t>(ctx->patchWork);
+    uint32_t *fwdMin = w, *fwdMax = w + m, *mirMin = w + 2 * m, *mirMax = w + 3 * m;
+    uint32_t *flag = w + 4 * m, *drawIdx = w + 5 * m, *cnt = w + 6 * m, *base = w + 7 * m;
+    HIP_TRY(hipMemsetAsync(fwdMin, 0xff, m * 4, ctx->stream));
+    HIP_TRY(hipMemsetAsync(mirMin, 0xff, m * 4, ctx->stream));
+    HIP_TRY(hipMemsetAsync(fwdMax, 0, m * 4, ctx->stream));
+    HIP_TRY(hipMemsetAsync(mirMax, 0, m * 4, ctx->stream));
+    HIP_TRY(hipMemsetAsync(cnt, 0, m * 4, ctx->stream));
+    if (spanCount > 0)
+        span_ranges_kernel<<<(spanCount + 255) / 256, 256, 0, ctx->stream>>>(
+            spans, spanCount, contours, contourCount, np, texels, fwdMin, fwdMax, mirMin, mirMax, scal + 6);
+    span_draw_flags_kernel<<<(np + 255) / 256, 256, 0, ctx->stream>>>(np, fwdMax, mirMax, flag);
+    RIVE_TRY(device_scan(ctx, flag, np, drawIdx, scal + 7));
+    RIVE_TRY(ensure(ctx, ctx->drawTess, (size_t)(np + 1) * sizeof(DrawTess)));
+    span_draws_kernel<<<(np + 255) / 256, 256, 0, ctx->stream>>>(
+        np, flag, drawIdx, fwdMin, fwdMax, mirMin, mirMax, ctx->in.paths, P<PatchBatchDev>(ctx->patchTable), nb,
+        d->image_count, P<DrawTess>(ctx->drawTess), cnt, P<uint32_t>(ctx->pathImage), scal + 6);
+    RIVE_TRY(device_scan(ctx, cnt, np, base, scal + 3));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(ctx->hostScalars, scal + 3, 20, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->hostScalars[3] != 0)
+        return fail(RIVE_HIP_ERROR_UNSUPPORTED,
+                    "spans naming no contour or path, or contour directions the raster-ordering pipeline "
+                    "does not use");
+    *totalPrims = ctx->hostScalars[0];
+    *drawCount = ctx->hostScalars[4];
+    if (*drawCount > 0)
+        draw_prim_base_kernel<<<(*drawCount + 255) / 256, 256, 0, ctx->stream>>>(*drawCount, base,
+                                                                                 P<DrawTess>(ctx->drawTess));
+    HIP_TRY(hipGetLastError());
     return RIVE_HIP_OK;
 }
 
@@ -875,6 +1084,10 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d, const GeoLay
         if (ctx->bufs[k].mapped)
             return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "flush with a mapped buffer");
     auto need = [&](int kind, size_t bytes) { return ctx->bufs[kind].size >= bytes; };
+    // A SPANS flush described by DrawBatch patch runs has its draws derived
+    // on the device; its draw_count is not read.
+    const bool patchMode = d->mode == RIVE_HIP_FLUSH_SPANS && d->patch_batch_count > 0;
+    const uint32_t drawCountIn = patchMode ? 0u : d->draw_count;
     // Record counts are bounded before any size arithmetic: path IDs index
     // per-path device arrays sized (path_count + 1) records.
     if (d->path_count > kMaxFlushPaths)
@@ -893,7 +1106,7 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d, const GeoLay
         !need(RIVE_HIP_BUFFER_PAINT, (d->first_paint + pathRecords) * sizeof(rive_hip_paint_data)) ||
         !need(RIVE_HIP_BUFFER_PAINT_AUX, (d->first_paint_aux + pathRecords) * sizeof(rive_hip_paint_aux)) ||
         !need(RIVE_HIP_BUFFER_GRAD_SPAN, (d->first_grad_span + d->grad_span_count) * sizeof(rive_hip_grad_span)) ||
-        !need(RIVE_HIP_BUFFER_DRAW, (d->first_draw + d->draw_count) * sizeof(rive_hip_draw)))
+        !need(RIVE_HIP_BUFFER_DRAW, (d->first_draw + drawCountIn) * sizeof(rive_hip_draw)))
         return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "buffers smaller than the flush descriptor");
     {
         auto dev = [&](int kind, uint64_t first, size_t recordBytes) -> unsigned char* {
@@ -917,7 +1130,7 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d, const GeoLay
             reinterpret_cast<const rive_hip_grad_span*>(host(RIVE_HIP_BUFFER_GRAD_SPAN, d->first_grad_span, 16));
     }
     RIVE_TRY(ensure(ctx, ctx->scalars, 64));
-    RIVE_TRY(ensure(ctx, ctx->drawTess, (size_t)std::max<uint32_t>(d->draw_count, 1) * sizeof(DrawTess)));
+    RIVE_TRY(ensure(ctx, ctx->drawTess, (size_t)std::max<uint32_t>(drawCountIn, 1) * sizeof(DrawTess)));
     HIP_TRY(hipMemsetAsync(ctx->scalars.ptr, 0, 64, ctx->stream));
     if (pre == nullptr) // a frame's flushes have their start and counting events set already
         HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream)); // flush start
@@ -933,7 +1146,7 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d, const GeoLay
             return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "span/contour buffers too small");
         // Draws with texels index ContourData records (every tessellated
         // vertex names its contour), so a SPANS flush needs them.
-        if (d->draw_count > 0 && d->tess_span_count > 0 && d->contour_count == 0)
+        if ((drawCountIn > 0 || patchMode) && d->tess_span_count > 0 && d->contour_count == 0)
             return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "SPANS flush with spans and draws but no contours");
         spans = ctx->in.spans;
         spanCount = d->tess_span_count;
@@ -992,25 +1205,29 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d, const GeoLay
     }
     RIVE_TRY(ensure(ctx, ctx->pathImage, (size_t)(d->path_count + 1) * 4));
     HIP_TRY(hipMemsetAsync(ctx->pathImage.ptr, 0, (size_t)(d->path_count + 1) * 4, ctx->stream));
-    draw_tess_kernel<<<1, 1024, 0, ctx->stream>>>(
-        ctx->in.draws, d->draw_count,
-        d->path_count, tessRows * kTessTextureWidth, d->image_count, P<DrawTess>(ctx->drawTess),
-        P<uint32_t>(ctx->pathImage), scal + 3, scal + 4);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(ctx->hostScalars, scal + 3, 8, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    uint32_t totalPrims = ctx->hostScalars[0];
-    if (ctx->hostScalars[1] != 0)
-        return fail(RIVE_HIP_ERROR_UNSUPPORTED,
-                    "draw record out of range, or contour directions the raster-ordering "
-                    "pipeline does not use");
+    uint32_t totalPrims = 0;
+    if (!patchMode)
+    {
+        draw_tess_kernel<<<1, 1024, 0, ctx->stream>>>(
+            ctx->in.draws, drawCountIn,
+            d->path_count, tessRows * kTessTextureWidth, d->image_count, P<DrawTess>(ctx->drawTess),
+            P<uint32_t>(ctx->pathImage), scal + 3, scal + 4);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(ctx->hostScalars, scal + 3, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        totalPrims = ctx->hostScalars[0];
+        if (ctx->hostScalars[1] != 0)
+            return fail(RIVE_HIP_ERROR_UNSUPPORTED,
+                        "draw record out of range, or contour directions the raster-ordering "
+                        "pipeline does not use");
+    }
     if (totalPrims > kMaxFlushTexels)
         return fail(RIVE_HIP_ERROR_UNSUPPORTED, "flush too large: draws cover more than 2^31 texels");
     ctx->lastCanvases = canvases;
     ctx->lastSpanCount = spanCount;
     ctx->lastSpans = spans;
     ctx->lastContours = contours;
-    ctx->lastDrawCount = d->draw_count;
+    ctx->lastDrawCount = drawCountIn;
     ctx->lastMode = d->mode;
     {
         rive_hip_flush_stats& st = ctx->stats;
@@ -1020,7 +1237,7 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d, const GeoLay
         st.load_action = canvases[0].load_action;
         st.mode = d->mode;
         st.path_count = d->path_count;
-        st.draw_count = d->draw_count;
+        st.draw_count = drawCountIn;
         st.contour_count = ctx->lastContourCount;
         st.span_count = spanCount;
         st.tess_vertex_count = d->mode == RIVE_HIP_FLUSH_GEOMETRY ? ctx->lastTessVertexCount : d->tess_vertex_count;
@@ -1046,8 +1263,7 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d, const GeoLay
             }
     }
     int rc = raster_pipeline(ctx, d, spans, spanCount, contours, ctx->lastContourCount, tessRows,
-                             P<DrawTess>(ctx->drawTess),
-                             d->draw_count, totalPrims);
+                             P<DrawTess>(ctx->drawTess), drawCountIn, totalPrims);
     if (rc)
         return rc;
     ctx->timed = true;

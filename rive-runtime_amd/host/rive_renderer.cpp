@@ -198,6 +198,54 @@ std::shared_ptr<Gradient> Gradient::MakeRadial(float cx, float cy, float radius,
 }
 
 // ---------------------------------------------------------------- RiveRenderer
+ColorInt colorModulateOpacity(ColorInt c, float opacity)
+{
+    const float o = (float)(c >> 24) / 255.f * opacity;
+    const uint32_t a = (uint32_t)std::lround(255.f * std::max(0.f, std::min(1.f, o)));
+    return (a << 24) | (c & 0x00ffffffu);
+}
+
+std::shared_ptr<Gradient> Gradient::getModulated(float opacity) const
+{
+    if (opacity == 1.f)
+        return std::const_pointer_cast<Gradient>(shared_from_this());
+    if (m_lastModulated && m_lastModulatedOpacity == opacity)
+        return m_lastModulated;
+    auto g = std::make_shared<Gradient>();
+    g->paintType = paintType;
+    g->stops = stops;
+    memcpy(g->coeffs, coeffs, sizeof(coeffs));
+    for (ColorInt c : colors)
+        g->colors.push_back(colorModulateOpacity(c, opacity));
+    m_lastModulated = g;
+    m_lastModulatedOpacity = opacity;
+    return g;
+}
+
+std::shared_ptr<RenderBuffer> Factory::makeRenderBuffer(RenderBufferType type, uint32_t, size_t bytes)
+{
+    return std::make_shared<HostRenderBuffer>(type, bytes);
+}
+std::shared_ptr<RenderShader> Factory::makeLinearGradient(float sx, float sy, float ex, float ey,
+                                                          const ColorInt colors[], const float stops[], size_t count)
+{
+    return Gradient::MakeLinear(sx, sy, ex, ey, colors, stops, count);
+}
+std::shared_ptr<RenderShader> Factory::makeRadialGradient(float cx, float cy, float radius, const ColorInt colors[],
+                                                          const float stops[], size_t count)
+{
+    return Gradient::MakeRadial(cx, cy, radius, colors, stops, count);
+}
+std::shared_ptr<RiveRenderPath> Factory::makeRenderPath(RawPath& path, FillRule rule)
+{
+    auto p = std::make_shared<RiveRenderPath>(std::move(path), rule);
+    path = RawPath();
+    return p;
+}
+std::shared_ptr<RiveRenderPath> Factory::makeEmptyRenderPath() { return std::make_shared<RiveRenderPath>(); }
+std::shared_ptr<RiveRenderPaint> Factory::makeRenderPaint() { return std::make_shared<RiveRenderPaint>(); }
+std::shared_ptr<RenderImage> Factory::decodeImage(const uint8_t*, size_t) { return nullptr; }
+
 RiveRenderer::IAABB RiveRenderer::IAABB::intersect(const IAABB& o) const
 {
     return {std::max(l, o.l), std::max(t, o.t), std::min(r, o.r), std::min(b, o.b)};
@@ -236,6 +284,20 @@ void RiveRenderer::restore()
 }
 void RiveRenderer::transform(const Mat2D& m) { m_stack.back().matrix = m_stack.back().matrix * m; }
 
+void RiveRenderer::modulateOpacity(float opacity)
+{
+    // rive_renderer.cpp:115-119
+    m_stack.back().modulatedOpacity = std::max(0.f, m_stack.back().modulatedOpacity * opacity);
+}
+
+void RiveRenderer::drawImageMesh(const RenderImage*, ImageSampler, std::shared_ptr<RenderBuffer>,
+                                 std::shared_ptr<RenderBuffer>, std::shared_ptr<RenderBuffer>, uint32_t, uint32_t,
+                                 BlendMode, float)
+{
+    ++m_unsupportedDraws;
+    fprintf(stderr, "RiveRenderer::drawImageMesh: image meshes are not supported by the HIP backend\n");
+}
+
 void RiveRenderer::drawImage(const RenderImage* renderImage, ImageSampler sampler, BlendMode blendMode,
                              float opacity)
 {
@@ -247,7 +309,10 @@ void RiveRenderer::drawImage(const RenderImage* renderImage, ImageSampler sample
     save();
     transform(Mat2D((float)image->width(), 0, 0, (float)image->height(), 0, 0));
     RiveRenderPaint paint;
-    paint.image(image, std::max(0.f, opacity));
+    // rive_renderer.cpp:400-402: the modulated opacity scales the image's;
+    // the unit rect below is then drawn at opacity 1.
+    paint.image(image, std::max(0.f, opacity * m_stack.back().modulatedOpacity));
+    m_stack.back().modulatedOpacity = 1.f;
     paint.blendMode(blendMode);
     paint.imageSampler(sampler);
     drawPath(&m_unitRectPath, &paint);
@@ -442,7 +507,8 @@ void RiveRenderer::drawPath(RenderPath* renderPath, RenderPaint* renderPaint)
     }
     m_context->pushDrawBatch(std::move(clips),
                              m_context->makeRecord(path, st.matrix, paint, 0, false, 0, path->getFillRule(),
-                                                   st.hasClipRect ? &st.clipRectInverse : nullptr));
+                                                   st.hasClipRect ? &st.clipRectInverse : nullptr,
+                                                   st.modulatedOpacity));
 }
 
 namespace gpu
@@ -489,6 +555,7 @@ void RenderContext::beginFrame(const FrameDescriptor& fd)
 {
     m_frame = fd;
     m_batches.clear();
+    m_modulatedGradients.clear();
     m_canvases.clear();
     m_provLastClipID = 0;
     m_provContentSerial = 0;
@@ -529,9 +596,10 @@ uint32_t RenderContext::getClipContentID() const
     return 0;
 }
 
+
 DrawRecord RenderContext::makeRecord(const RiveRenderPath* path, const Mat2D& m, const RiveRenderPaint* paint,
                                      uint32_t clipID, bool clipUpdate, uint32_t outerClipID, FillRule fillRule,
-                                     const Mat2D* clipRectInverse) const
+                                     const Mat2D* clipRectInverse, float opacity)
 {
     DrawRecord r{};
     r.raw = &path->getRawPath();
@@ -553,12 +621,20 @@ DrawRecord RenderContext::makeRecord(const RiveRenderPath* path, const Mat2D& m,
         if (g != nullptr)
         {
             params = g->paintType;
+            if (opacity != 1.f)
+            {
+                m_modulatedGradients.push_back(g->getModulated(opacity));
+                g = m_modulatedGradients.back().get();
+            }
             r.gradient = g;
         }
         else
         {
             params = 1; // solid color
-            r.paint.value = swizzle_rgba(paint->getColor());
+            ColorInt color = paint->getColor();
+            if (opacity != 1.f)
+                color = colorModulateOpacity(color, opacity);
+            r.paint.value = swizzle_rgba(color);
         }
         params |= clipID << 16 | shader_blend_mode(paint->getBlendMode()) << 4;
         if (paint->getImage() != nullptr)
@@ -734,6 +810,7 @@ bool RenderContext::flush(rive_hip_target* target, uint32_t maxFlushes, uint32_t
             *flushCount = 1;
         m_batches.clear();
         m_canvases.clear();
+        m_modulatedGradients.clear();
         return ok;
     }
 
@@ -852,6 +929,7 @@ bool RenderContext::flush(rive_hip_target* target, uint32_t maxFlushes, uint32_t
     }
     bool ok = m_impl->flushFrame(descs);
     m_batches.clear();
+    m_modulatedGradients.clear();
     m_provLastClipID = 0;
     m_provContentSerial = 0;
     m_provClipIDs.clear();

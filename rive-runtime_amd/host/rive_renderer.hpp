@@ -111,18 +111,31 @@ public:
 };
 
 // Linear and radial gradients (renderer/src/gradient.cpp:38-175).
-class Gradient : public RenderShader
+class Gradient : public RenderShader, public std::enable_shared_from_this<Gradient>
 {
 public:
     static std::shared_ptr<Gradient> MakeLinear(float sx, float sy, float ex, float ey, const ColorInt colors[],
                                                 const float stops[], size_t count);
     static std::shared_ptr<Gradient> MakeRadial(float cx, float cy, float radius, const ColorInt colors[],
                                                 const float stops[], size_t count);
+    // gradient.cpp:191-220: this gradient with every stop's alpha scaled by
+    // opacity (itself when opacity is 1); the last modulated copy is cached.
+    std::shared_ptr<Gradient> getModulated(float opacity) const;
+    size_t count() const { return colors.size(); }
     uint32_t paintType = 0; // 2 linear, 3 radial
     std::vector<ColorInt> colors;
     std::vector<float> stops;
     float coeffs[3] = {0, 0, 0};
+
+private:
+    mutable float m_lastModulatedOpacity = -1.f;
+    mutable std::shared_ptr<Gradient> m_lastModulated;
 };
+
+// colorModulateOpacity (src/shapes/paint/color.cpp:46-67): the color with
+// its alpha scaled by opacity, rounded back to 8 bits.
+ColorInt colorModulateOpacity(ColorInt color, float opacity);
+inline uint32_t colorAlpha(ColorInt c) { return c >> 24; }
 
 class RenderPath
 {
@@ -179,6 +192,7 @@ class RiveRenderPath : public RenderPath
 {
 public:
     explicit RiveRenderPath(FillRule rule = FillRule::nonZero) : m_fillRule(rule) {}
+    RiveRenderPath(RawPath&& raw, FillRule rule) : m_raw(std::move(raw)), m_fillRule(rule) {}
     void moveTo(float x, float y) { m_raw.move({x, y}); }
     void lineTo(float x, float y) { m_raw.line({x, y}); }
     void cubicTo(float ox, float oy, float ix, float iy, float x, float y)
@@ -211,6 +225,7 @@ public:
     void feather(float f) { m_feather = f; }
     void blendMode(BlendMode b) { m_blend = b; }
     void shader(std::shared_ptr<Gradient> g) { m_shader = std::move(g); }
+    void shader(std::shared_ptr<RenderShader> s) { m_shader = std::dynamic_pointer_cast<Gradient>(s); }
     // RiveRenderPaint::image (rive_render_paint.cpp:58-64): white at `opacity`
     // modulated by the image.
     void image(const RiveRenderImage* img, float opacity)
@@ -245,6 +260,14 @@ private:
     ImageSampler m_sampler;
 };
 
+// A GPU buffer handle (renderer.hpp RenderBuffer), for drawImageMesh.
+enum class RenderBufferType : uint8_t { index, vertex };
+class RenderBuffer
+{
+public:
+    virtual ~RenderBuffer() = default;
+};
+
 class Renderer
 {
 public:
@@ -255,10 +278,54 @@ public:
     virtual void drawPath(RenderPath* path, RenderPaint* paint) = 0;
     virtual void clipPath(RenderPath* path) = 0;
     virtual void drawImage(const RenderImage* image, ImageSampler sampler, BlendMode blendMode, float opacity) = 0;
+    virtual void drawImageMesh(const RenderImage* image, ImageSampler sampler,
+                               std::shared_ptr<RenderBuffer> vertices_f32, std::shared_ptr<RenderBuffer> uvCoords_f32,
+                               std::shared_ptr<RenderBuffer> indices_u16, uint32_t vertexCount, uint32_t indexCount,
+                               BlendMode blendMode, float opacity) = 0;
+    // renderer.hpp:231-235: scales the opacity of later draws; stacked
+    // multiplicatively, captured by save() and restored by restore().
+    virtual void modulateOpacity(float opacity) = 0;
     // renderer.cpp: conveniences over transform().
     void translate(float x, float y) { transform(Mat2D(1, 0, 0, 1, x, y)); }
     void scale(float sx, float sy) { transform(Mat2D(sx, 0, 0, sy, 0, 0)); }
     void rotate(float radians) { transform(Mat2D::fromRotation(radians)); }
+};
+
+// A render buffer in host memory (the front end reads image-mesh buffers on
+// the host; nothing here draws from them).
+class HostRenderBuffer : public RenderBuffer
+{
+public:
+    HostRenderBuffer(RenderBufferType type, size_t bytes) : m_type(type), m_data(bytes) {}
+    RenderBufferType type() const { return m_type; }
+    uint8_t* data() { return m_data.data(); }
+    size_t size() const { return m_data.size(); }
+
+private:
+    RenderBufferType m_type;
+    std::vector<uint8_t> m_data;
+};
+
+// The factory draw streams create their objects with
+// (include/rive/factory.hpp:31-69; the reference's RenderContext is one).
+class Factory
+{
+public:
+    virtual ~Factory() = default;
+    virtual std::shared_ptr<RenderBuffer> makeRenderBuffer(RenderBufferType type, uint32_t flags, size_t bytes);
+    virtual std::shared_ptr<RenderShader> makeLinearGradient(float sx, float sy, float ex, float ey,
+                                                             const ColorInt colors[], const float stops[],
+                                                             size_t count);
+    virtual std::shared_ptr<RenderShader> makeRadialGradient(float cx, float cy, float radius,
+                                                             const ColorInt colors[], const float stops[],
+                                                             size_t count);
+    // Takes the path's verbs and points (the RawPath is left empty).
+    virtual std::shared_ptr<RiveRenderPath> makeRenderPath(RawPath& path, FillRule rule);
+    virtual std::shared_ptr<RiveRenderPath> makeEmptyRenderPath();
+    virtual std::shared_ptr<RiveRenderPaint> makeRenderPaint();
+    // Encoded-image decoding (PNG, JPEG, WebP) is not part of this backend:
+    // always null.
+    virtual std::shared_ptr<RenderImage> decodeImage(const uint8_t* bytes, size_t size);
 };
 
 namespace gpu
@@ -326,7 +393,7 @@ public:
                             std::vector<uint32_t>* contours) = 0;
 };
 
-class RenderContext
+class RenderContext : public Factory
 {
 public:
     struct FrameDescriptor // render_context.hpp:102-144
@@ -358,9 +425,11 @@ public:
     uint64_t newClipSerial() { return ++m_lastClipSerial; }
     // The records of one draw (PathDraw::Make + the PathData/PaintData/
     // PaintAuxData it writes, draw.cpp:420-825, gpu.cpp:859-1063).
+    // opacity: the renderer's modulated opacity, applied to the paint's
+    // color or gradient stops (draw.cpp:727-741, gradient.cpp:191-220).
     DrawRecord makeRecord(const RiveRenderPath* path, const Mat2D& m, const RiveRenderPaint* paint, uint32_t clipID,
                           bool clipUpdate, uint32_t outerClipID, FillRule fillRule,
-                          const Mat2D* clipRectInverse = nullptr) const;
+                          const Mat2D* clipRectInverse = nullptr, float opacity = 1.f);
     // A draw and the clip stack it sits under; clip draws and clip IDs are
     // assigned per logical flush when the frame is flushed (applyClip,
     // rive_renderer.cpp:631-824).
@@ -385,6 +454,8 @@ private:
     };
     std::vector<CanvasRecord> m_canvases;
     uint64_t m_lastClipSerial = 0;
+    // Opacity-modulated copies of gradients, alive until the frame is flushed.
+    std::vector<std::shared_ptr<Gradient>> m_modulatedGradients;
     // applyClip's bookkeeping as draws arrive (for getClipContentID): clip
     // IDs handed out so far and the element whose content is current.
     uint32_t m_provLastClipID = 0;
@@ -403,6 +474,14 @@ public:
     void drawPath(RenderPath* path, RenderPaint* paint) override;
     void clipPath(RenderPath* path) override;
     void drawImage(const RenderImage* image, ImageSampler sampler, BlendMode blendMode, float opacity) override;
+    // Image meshes (the reference's ImageMeshDraw) are not drawn by this
+    // backend: the call is reported on stderr and counted.
+    void drawImageMesh(const RenderImage* image, ImageSampler sampler, std::shared_ptr<RenderBuffer> vertices_f32,
+                       std::shared_ptr<RenderBuffer> uvCoords_f32, std::shared_ptr<RenderBuffer> indices_u16,
+                       uint32_t vertexCount, uint32_t indexCount, BlendMode blendMode, float opacity) override;
+    void modulateOpacity(float opacity) override;
+    float currentModulatedOpacity() const { return m_stack.back().modulatedOpacity; }
+    uint32_t unsupportedDraws() const { return m_unsupportedDraws; }
 
     // The current clip rect (rive_renderer.hpp: hasClipRect, getClipRect,
     // getClipRectMatrix): left, top, right, bottom in the space of its matrix.
@@ -449,7 +528,9 @@ private:
         float clipRect[4] = {0, 0, 0, 0};
         Mat2D clipRectMatrix;
         Mat2D clipRectInverse;
+        float modulatedOpacity = 1.f;
     };
+    uint32_t m_unsupportedDraws = 0;
     void clipRectImpl(float rect[4], RiveRenderPath* path);
     void clipPathImpl(RiveRenderPath* path);
     static IAABB MapBounds(const Mat2D& m, const RawPath& raw, float outsetX, float outsetY);

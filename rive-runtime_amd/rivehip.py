@@ -72,7 +72,14 @@ class FlushDesc(ctypes.Structure):
                 ("first_path", ctypes.c_uint64), ("first_paint", ctypes.c_uint64),
                 ("first_paint_aux", ctypes.c_uint64), ("first_contour", ctypes.c_uint64),
                 ("first_grad_span", ctypes.c_uint64), ("first_tess_span", ctypes.c_uint64),
-                ("first_draw", ctypes.c_uint64)]
+                ("first_draw", ctypes.c_uint64), ("patch_batches", ctypes.c_void_p),
+                ("patch_batch_count", ctypes.c_uint32)]
+
+
+class PatchBatch(ctypes.Structure):
+    """rive_hip_patch_batch: one DrawBatch of midpoint-fan patches."""
+    _fields_ = [("first_patch", ctypes.c_uint32), ("patch_count", ctypes.c_uint32),
+                ("image_index", ctypes.c_uint32), ("image_sampler", ctypes.c_uint32)]
 
 
 class Canvas(ctypes.Structure):

@@ -3,7 +3,8 @@
 // (tests/unit_tests/renderer/pls_renderer_test.cpp): IsAABB (:825-932),
 // clip-rects (:553-738), nan-render-path, stroke-thickness, empty-paths,
 // empty-paths2 and invalid-gradient-stops (:740-823, :934-1045, :1113-1174), and
-// checks the same expectations.  Prints "OK" or the failures.
+// checks the same expectations; and the modulate-opacity tests
+// (modulate_opacity_test.cpp).  Prints "OK" or the failures.
 #include "rive_renderer.hpp"
 
 #include <cmath>
@@ -38,7 +39,12 @@ bool near_rect(const RiveRenderer::AABB& r, float l, float t, float rr, float b)
 class RecordingImpl : public gpu::RenderContextImpl
 {
 public:
-    bool flushFrame(const std::vector<gpu::FlushDescriptor>&) override { return true; }
+    bool flushFrame(const std::vector<gpu::FlushDescriptor>& f) override
+    {
+        last = f;
+        return true;
+    }
+    std::vector<gpu::FlushDescriptor> last;
     // Every draw counts as one small path: these scenes fit one logical flush.
     bool countDraws(const std::vector<gpu::DrawRecord>& draws, std::vector<uint32_t>* tess,
                     std::vector<uint32_t>* contours) override
@@ -304,10 +310,142 @@ void robustness_cases()
     EXPECT(grad({0, nan, 1}) == 0);
 }
 
+// The reference's modulate-opacity renderer tests
+// (tests/unit_tests/renderer/modulate_opacity_test.cpp:58-420): stacking over
+// save/restore, repeated calls, identity, zero, deep nesting, clips and
+// transforms, gradient modulation and its single-entry cache; plus the alpha
+// each draw's PaintData carries.
+void modulate_opacity_cases()
+{
+    auto impl = std::make_unique<RecordingImpl>();
+    RecordingImpl* rec = impl.get();
+    gpu::RenderContext ctx(std::move(impl));
+    gpu::RenderContext::FrameDescriptor fd;
+    fd.renderTargetWidth = fd.renderTargetHeight = 100;
+    ctx.beginFrame(fd);
+    RawPath raw;
+    add_rect(raw, 0, 0, 100, 100, true);
+    auto path = ctx.makeRenderPath(raw, FillRule::nonZero);
+    EXPECT(raw.empty());
+    auto paint = ctx.makeRenderPaint();
+    paint->color(0xffffffff);
+    RiveRenderer r(&ctx);
+    auto approx = [](float a, float b) { return std::fabs(a - b) <= 1e-5f * std::max(1.f, std::fabs(b)); };
+    EXPECT(r.currentModulatedOpacity() == 1.f);
+    r.save();
+    r.modulateOpacity(.5f);
+    EXPECT(approx(r.currentModulatedOpacity(), .5f));
+    r.save();
+    r.modulateOpacity(.5f);
+    EXPECT(approx(r.currentModulatedOpacity(), .25f));
+    r.drawPath(path.get(), paint.get()); // alpha 255 * .25 = 63.75 -> 64
+    r.restore();
+    EXPECT(approx(r.currentModulatedOpacity(), .5f));
+    r.drawPath(path.get(), paint.get()); // 128
+    r.restore();
+    EXPECT(approx(r.currentModulatedOpacity(), 1.f));
+    r.drawPath(path.get(), paint.get()); // 255
+    r.save();
+    r.modulateOpacity(.5f);
+    r.modulateOpacity(.5f);
+    r.modulateOpacity(.5f);
+    EXPECT(approx(r.currentModulatedOpacity(), .125f));
+    r.drawPath(path.get(), paint.get()); // 32
+    r.restore();
+    r.save();
+    r.modulateOpacity(0.f);
+    EXPECT(r.currentModulatedOpacity() == 0.f);
+    r.drawPath(path.get(), paint.get()); // 0
+    r.restore();
+    for (int i = 0; i < 10; ++i)
+    {
+        r.save();
+        r.modulateOpacity(.9f);
+    }
+    EXPECT(approx(r.currentModulatedOpacity(), 0.3486784401f));
+    r.drawPath(path.get(), paint.get()); // lround(255 * .34868) = 89
+    for (int i = 0; i < 10; ++i)
+        r.restore();
+    EXPECT(approx(r.currentModulatedOpacity(), 1.f));
+    // with a clip path and a transform
+    RawPath ovalRaw;
+    ovalRaw.move({90, 50});
+    ovalRaw.cubic({90, 72}, {72, 90}, {50, 90});
+    ovalRaw.cubic({28, 90}, {10, 72}, {10, 50});
+    ovalRaw.cubic({10, 28}, {28, 10}, {50, 10});
+    ovalRaw.cubic({72, 10}, {90, 28}, {90, 50});
+    ovalRaw.close();
+    auto clip = ctx.makeRenderPath(ovalRaw, FillRule::nonZero);
+    r.save();
+    r.clipPath(clip.get());
+    r.modulateOpacity(.5f);
+    r.drawPath(path.get(), paint.get()); // 128, after its clip draw
+    r.restore();
+    r.save();
+    r.translate(25, 25);
+    r.scale(.5f, .5f);
+    r.modulateOpacity(.75f);
+    EXPECT(approx(r.currentModulatedOpacity(), .75f));
+    r.drawPath(path.get(), paint.get()); // lround(191.25) = 191
+    r.restore();
+    // a gradient paint: every stop's alpha is modulated
+    const ColorInt gcolors[] = {0xffff0000, 0xff0000ff};
+    const float gstops[] = {0.f, 1.f};
+    auto gpaint = ctx.makeRenderPaint();
+    gpaint->shader(ctx.makeLinearGradient(0, 0, 100, 0, gcolors, gstops, 2));
+    r.save();
+    r.modulateOpacity(.5f);
+    r.drawPath(path.get(), gpaint.get());
+    r.restore();
+    r.drawPath(path.get(), gpaint.get());
+    EXPECT(ctx.flush(nullptr));
+    EXPECT(rec->last.size() == 1);
+    if (rec->last.size() == 1)
+    {
+        const gpu::FlushDescriptor& f = rec->last[0];
+        // PaintData values are RGBA with alpha in the top byte; record 0 is
+        // the clear color, the clip draw sits before its draw.
+        std::vector<uint32_t> alphas;
+        for (size_t i = 1; i < f.paints.size(); ++i)
+            if ((f.paints[i].params & 0xfu) == 1u)
+                alphas.push_back(f.paints[i].value >> 24);
+        const std::vector<uint32_t> want = {64, 128, 255, 32, 0, 89, 128, 191};
+        EXPECT(alphas == want);
+        // the two gradient draws: a simple ramp each, the modulated one with
+        // alpha 128 stops, the plain one with 255
+        EXPECT(f.gradSpans.size() == 2);
+        if (f.gradSpans.size() == 2)
+        {
+            EXPECT((f.gradSpans[0].color0 >> 24) == 128 && (f.gradSpans[0].color1 >> 24) == 128);
+            EXPECT((f.gradSpans[1].color0 >> 24) == 255 && (f.gradSpans[1].color1 >> 24) == 255);
+        }
+    }
+    // Gradient::getModulated: identity, colors, cache, stops, radial
+    auto g = Gradient::MakeLinear(0, 0, 100, 0, gcolors, gstops, 2);
+    EXPECT(g->getModulated(1.f).get() == g.get());
+    auto m1 = g->getModulated(.5f);
+    EXPECT(m1.get() != g.get() && colorAlpha(m1->colors[0]) == 128 && colorAlpha(m1->colors[1]) == 128);
+    EXPECT(g->getModulated(.5f).get() == m1.get());
+    auto m3 = g->getModulated(.25f);
+    EXPECT(m3.get() != m1.get() && g->getModulated(.25f).get() == m3.get());
+    auto m5 = g->getModulated(.5f);
+    EXPECT(m5.get() != m1.get() && m5.get() != m3.get());
+    const ColorInt c3[] = {0xffff0000, 0xff00ff00, 0xff0000ff};
+    const float s3[] = {0.f, .3f, 1.f};
+    auto g3 = Gradient::MakeLinear(0, 0, 100, 0, c3, s3, 3);
+    auto g3m = g3->getModulated(.5f);
+    EXPECT(g3m->count() == g3->count() && g3m->stops[0] <= g3m->stops[1] && g3m->stops[1] <= g3m->stops[2]);
+    const ColorInt rc[] = {0x80ff0000, 0x80ffffff};
+    auto rg = Gradient::MakeRadial(50, 50, 50, rc, gstops, 2);
+    auto rgm = rg->getModulated(.5f);
+    EXPECT(rgm->paintType == rg->paintType && colorAlpha(rgm->colors[0]) == 64 && colorAlpha(rgm->colors[1]) == 64);
+}
+
 int main()
 {
     is_aabb_cases();
     robustness_cases();
+    modulate_opacity_cases();
 
     gpu::RenderContext ctx(std::make_unique<RecordingImpl>());
     gpu::RenderContext::FrameDescriptor fd;

@@ -1,0 +1,59 @@
+"""The batched workload on the device: the 256 seeded canvases (sides
+64..1024, aspect 0.5..2, 10..2000 closed cubic fills each; mt19937_64 seed
+0x5249564500000004) rendered by batched flushes, as the bench renders them.
+
+Every canvas must equal its own single-canvas flush bit for bit, and a
+subset of 16 canvases spanning the size range (every 17th by area, the
+smallest and the largest included) must match the oracle: at most 2/255 per
+channel on all but 0.1% of pixels, 8/255 anywhere, and every pixel beyond
+2/255 on an anti-aliased edge.
+"""
+import numpy as np
+import pytest
+
+import oracle_binding as ob
+import rivehip
+
+pytestmark = pytest.mark.gpu
+
+
+def test_256_canvas_batch(hip_ctx):
+    print("[cfg4] 256 canvases in batched flushes", flush=True)
+    streams = rivehip.generate_batch(256)
+    sizes = [(s.contents.width, s.contents.height) for s in streams]
+    groups = rivehip.flush_groups(streams)
+    assert sum(len(g) for g in groups) == 256 and len(groups) >= 2
+    targets = [hip_ctx.target(w, h) for w, h in sizes]
+    try:
+        for grp in groups:
+            hip_ctx.render_streams([streams[i] for i in grp], [targets[i] for i in grp])
+        batched = [hip_ctx.read_pixels(t, w, h) for t, (w, h) in zip(targets, sizes)]
+        for i, (s, (w, h)) in enumerate(zip(streams, sizes)):
+            t = hip_ctx.target(w, h)
+            try:
+                hip_ctx.render_stream(s, t)
+                alone = hip_ctx.read_pixels(t, w, h)
+            finally:
+                hip_ctx.free_target(t)
+            assert np.array_equal(batched[i], alone), f"canvas {i} ({w}x{h}) differs from its own flush"
+        assert sum(int(b[..., 3].max() > 0) for b in batched) == 256
+        by_area = sorted(range(256), key=lambda i: (sizes[i][0] * sizes[i][1], i))
+        subset = [by_area[k] for k in range(0, 256, 17)] + [by_area[-1]]
+        assert len(subset) >= 16
+        for i in subset:
+            f = ob.OracleFlush(streams[i])
+            try:
+                ref, edges, _ = f.render_edges()
+            finally:
+                f.close()
+            d = np.abs(batched[i].astype(np.int32) - ref.astype(np.int32)).max(axis=2)
+            over = d > 2
+            print(f"[cfg4] canvas {i} {sizes[i][0]}x{sizes[i][1]} {streams[i].contents.draw_count} paths: "
+                  f"max {int(d.max())}/255", flush=True)
+            assert over.mean() <= 1e-3 and int(d.max()) <= 8, f"canvas {i}: max {int(d.max())}"
+            assert not np.any(over & ~edges), f"canvas {i}: pixels beyond 2/255 off AA edges"
+    finally:
+        for t in targets:
+            hip_ctx.free_target(t)
+        for s in streams:
+            rivehip.free_stream(s)
