@@ -127,3 +127,61 @@ class StreamBuilder:
         s.image_count = len(self.images)
         self._struct = s
         return ctypes.pointer(s)
+
+
+# Streams that cross one logical flush's resource limits
+# (render_context.cpp:43-53,484-492,517-541,617-654).
+
+def pixel_rects_stream(n, width=256):
+    """n opaque 1x1-px rectangles, draw i on pixel (i % width, i // width),
+    each with its own color: past 30 719 draws the path IDs run out.
+    Returns (builder, ARGB colors)."""
+    height = (n + width - 1) // width
+    b = StreamBuilder(width, height, clear_color=0xff000000)
+    colors = []
+    for i in range(n):
+        x, y = i % width, i // width
+        p = b.path([(0, x, y), (1, x + 1, y), (1, x + 1, y + 1), (1, x, y + 1), (5,)])
+        c = 0xff000000 | ((i * 2654435761) & 0xffffff)
+        b.draw(p, color=c)
+        colors.append(c)
+    return b, np.asarray(colors, np.uint32)
+
+
+def contour_limit_stream(draws=66):
+    """A 1000-contour path drawn `draws` times: 65 draws fit under the
+    65 535 contour IDs of a flush."""
+    b = StreamBuilder(128, 128)
+    cmds = []
+    for k in range(1000):
+        x, y = (k % 40) * 3, (k // 40) * 3
+        cmds += [(0, x, y), (1, x + 2, y), (1, x + 2, y + 2), (1, x, y + 2), (5,)]
+    p = b.path(cmds)
+    for i in range(draws):
+        b.draw(p, color=0x80000000 | (i * 0x030507))
+    return b
+
+
+def tess_limit_stream(draws=120):
+    """Identical wide round-joined strokes of a 300-cubic zigzag, ~38 k
+    tessellation vertices each: a flush holds about 109 of them."""
+    b = StreamBuilder(512, 512)
+    cmds = [(0, 10, 10)]
+    for k in range(300):
+        y = 10 + 1.6 * k
+        cmds.append((4, 500, y, 10, y + 0.4, 500, y + 0.8))
+    p = b.path(cmds)
+    for i in range(draws):
+        b.draw(p, color=0x20000000 | (i * 0x010203), stroke=300.0, join=ROUND, cap=ROUND_CAP)
+    return b
+
+
+def gradient_limit_stream(draws=2049):
+    """`draws` distinct three-stop gradients, one complex ramp row each: the
+    gradient texture holds 2048 rows."""
+    b = StreamBuilder(64, 64)
+    p = b.path([(0, 1, 1), (1, 9, 1), (1, 9, 9), (1, 1, 9), (5,)])
+    for k in range(draws):
+        stops = [(0.0, 0xff000000 | k), (0.5, 0xffff0000), (1.0, 0xff0000ff)]
+        b.draw(p, gradient=("linear", 1.0, 1.0, 9.0, 9.0, stops))
+    return b

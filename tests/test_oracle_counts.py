@@ -28,7 +28,8 @@ import pytest
 
 import oracle_binding as ob
 import rivehip
-from streams import BEVEL, BUTT, MITER, ROUND, ROUND_CAP, SQUARE, StreamBuilder
+from streams import (BEVEL, BUTT, MITER, ROUND, ROUND_CAP, SQUARE, StreamBuilder, contour_limit_stream,
+                     gradient_limit_stream, pixel_rects_stream, tess_limit_stream)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libref_math.so")
@@ -520,3 +521,87 @@ def test_tessellated_vertices_closed_forms(scene):
                 prev = theta
             strokes += 1
     assert fills > 0 or strokes > 0
+
+
+# Limits of one logical flush (render_context.cpp:43-53,484-492,517-541,617-654).
+MAX_PATH_ID = (0x1F << 10) - 1 - 1023 - 1          # MaxPathID(1) - 1
+MAX_CONTOUR_ID = 65535
+MAX_TESS_BEFORE_PADDING = 2048 * 2048 - (8 + 16 + 1)
+MAX_GRADIENT_ROWS = 2048
+
+
+def flush_infos(f):
+    out = []
+    for k in range(f.flush_count):
+        f.select(k)
+        out.append(dict(paths=f.path_count, contours=f.contour_count, tess=f.tess_vertex_count,
+                        simple=f.grad_simple_rows, complex=f.grad_complex_rows, draws=f.draws().copy()))
+    f.select(0)
+    return out
+
+
+def test_logical_flush_path_limit_and_preserve():
+    """30 720 one-path draws: the first logical flush takes exactly
+    MaxPathID(1) - 1 = 30 719 paths, the last draw starts a second flush,
+    and that flush loads the first one's pixels (every pixel keeps its own
+    draw's color, the last one included)."""
+    n = MAX_PATH_ID + 1
+    b, colors = pixel_rects_stream(n)
+    f = ob.OracleFlush(b.build())
+    try:
+        info = flush_infos(f)
+        assert [i["paths"] for i in info] == [MAX_PATH_ID, 1]
+        img, _ = f.render()
+    finally:
+        f.close()
+    px = img.reshape(-1, 4)[:n]
+    argb = (0xff000000 | (px[:, 0].astype(np.uint32) << 16) | (px[:, 1].astype(np.uint32) << 8)
+            | px[:, 2].astype(np.uint32))
+    np.testing.assert_array_equal(argb, colors)
+
+
+def test_logical_flush_contour_limit():
+    """66 draws of a 1000-contour path: 65 fit under the 65 535 contour IDs,
+    the 66th starts a new flush."""
+    b = contour_limit_stream()
+    f = ob.OracleFlush(b.build())
+    try:
+        info = flush_infos(f)
+    finally:
+        f.close()
+    assert [len(i["draws"]) for i in info] == [65, 1]
+    assert [i["contours"] for i in info] == [65000, 1000]
+
+
+def test_logical_flush_tessellation_limit():
+    """Identical wide round-joined strokes: a flush holds as many as fit in
+    2048^2 - 25 vertices, each draw's count taken from its own draw record."""
+    b = tess_limit_stream()
+    f = ob.OracleFlush(b.build())
+    try:
+        info = flush_infos(f)
+    finally:
+        f.close()
+    v = int(info[0]["draws"][0, 2])
+    assert v > 20000, v
+    assert all(int(i["draws"][k, 2]) == v for i in info for k in range(len(i["draws"])))
+    per_flush = MAX_TESS_BEFORE_PADDING // v
+    assert [len(i["draws"]) for i in info][:-1] == [per_flush] * (len(info) - 1)
+    assert len(info) >= 2
+    assert sum(len(i["draws"]) for i in info) == 120
+    for i in info:
+        # 8 leading padding vertices, the draws, 1 trailing padding vertex.
+        assert i["tess"] == 8 + v * len(i["draws"]) + 1
+
+
+def test_logical_flush_gradient_row_limit():
+    """2049 distinct three-stop gradients need 2049 complex ramp rows; the
+    gradient texture holds 2048, so the last draw starts a new flush."""
+    b = gradient_limit_stream()
+    f = ob.OracleFlush(b.build())
+    try:
+        info = flush_infos(f)
+    finally:
+        f.close()
+    assert [i["complex"] for i in info] == [MAX_GRADIENT_ROWS, 1]
+    assert [len(i["draws"]) for i in info] == [MAX_GRADIENT_ROWS, 1]
