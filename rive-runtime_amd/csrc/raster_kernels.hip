@@ -1613,17 +1613,19 @@ __device__ void stroke_prim_row(const Prim& p, int j, const TileGeom& g, uint32_
     }
 }
 
+// Coverage of a winding count under the path's fill rule (draw_path.frag
+// fill-rule resolution): clockwise clamps the signed count to [0, 1], nonzero
+// takes min(|count|, 1), even-odd the distance to the nearest even integer
+// (1 - |2 fract(|count| / 2) - 1|, the same value for every count this
+// raster forms: integer windings plus multiples of 2^-16 of border coverage,
+// for which both forms are exact).
 __device__ __forceinline__ float fill_rule_coverage(float count, uint32_t flags)
 {
     if (flags & 4u) // clockwise
         return fminf(fmaxf(count, 0.f), 1.f);
-    float coverage = fabsf(count);
     if (flags & 2u) // even-odd
-    {
-        const float h = coverage * .5f;
-        coverage = 1.f - fabsf((h - floorf(h)) * 2.f + -1.f);
-    }
-    return fminf(coverage, 1.f);
+        return fminf(fabsf(fmaf(-2.f, rintf(count * .5f), count)), 1.f);
+    return fminf(fabsf(count), 1.f);
 }
 
 // Image sampling with the reference's sampler state (pipeline_manager_vulkan.cpp:
@@ -1990,14 +1992,19 @@ __device__ __forceinline__ float clip_rect_coverage(const float* m, int px, int 
 // scalar load); the rest is read from PathRaster where it is used.
 struct PathPaint
 {
-    uint32_t kind;  // PathRaster flags | paintType << 16 | kPaintImage
+    uint32_t kind;  // PathRaster flags | paintType << 16 | kPaintImage | kPaintPlain | kPaintNoOp
     uint32_t clips; // clipID | outerClipID << 16
     float color255[4];
     float alpha;
-    uint32_t pad;
+    float oneMinusAlpha; // 1 - alpha, the src-over factor at full coverage
 };
 static_assert(sizeof(PathPaint) == 32, "PathPaint layout");
 constexpr uint32_t kPaintImage = 1u << 20;
+// A solid src-over paint with no clip, clip rect or advanced blend mode: at
+// full coverage it blends as dst * (1 - alpha) + color255 (kPaintPlain), or
+// leaves the pixel as it is when alpha is 0 (kPaintNoOp).
+constexpr uint32_t kPaintPlain = 1u << 21;
+constexpr uint32_t kPaintNoOp = 1u << 22;
 
 __device__ __forceinline__ void resolve_pixel(const PathPaint& pp, const PathRaster* __restrict__ prp,
                                               float coverage, int px, int py, float noise255,
@@ -2054,6 +2061,26 @@ __device__ __forceinline__ void resolve_pixel(const PathPaint& pp, const PathRas
     out[0] += n;
     out[1] += n;
     out[2] += n;
+    colPacked = pack_unorm8x4(out);
+}
+
+// resolve_pixel of a kPaintPlain paint at coverage 1: the same operations on
+// the same values (src = color255 * 1, a = alpha * 1 != 0, so the dither
+// applies).
+__device__ __forceinline__ void resolve_plain_full(const PathPaint& pp, float noise255, uint32_t& colPacked)
+{
+    const uint32_t c = colPacked;
+    // the factor in a vector register, so that each fused multiply-add takes
+    // one scalar operand (the color) and needs no copy
+    float ia = pp.oneMinusAlpha;
+    asm volatile("" : "+v"(ia));
+    float out[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        out[k] = fmaf(float((c >> (8 * k)) & 0xffu), ia, pp.color255[k]);
+    out[0] += noise255;
+    out[1] += noise255;
+    out[2] += noise255;
     colPacked = pack_unorm8x4(out);
 }
 
@@ -2345,8 +2372,19 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                 const int32_t C = (int32_t)uniform_u32(rel.z);
                 const PathPaint pr = pathPaint[path];
                 const uint32_t prFlags = pr.kind & 0xffffu;
+                if (ni == 0)
+                {
+                    // No primitives here: the listing keeps such a path only
+                    // where its corner winding gives coverage, and an integer
+                    // winding's coverage is 1 under every fill rule.
+                    if (pr.kind & kPaintPlain)
+                        resolve_plain_full(pr, noise255, col);
+                    else if (!(pr.kind & kPaintNoOp))
+                        resolve_pixel(pr, raster + path, 1.f, px, py, noise255, ramps, gradHeight, images, col,
+                                      clip);
+                    continue;
+                }
                 float cov;
-                if (ni != 0)
                 {
                     const uint32_t a = uniform_u32((rel.w - baseSA) >> 16);
                     const int32_t v = sAcc[a][t];
@@ -2373,9 +2411,7 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                         cov = fill_rule_coverage(fmaf((float)b, 1.f / 65536.f, (float)w), prFlags); // exact scaling
                     }
                 }
-                else
-                    cov = (prFlags & 1u) ? 0.f : fill_rule_coverage((float)C, prFlags);
-                if (cov != 0.f || (prFlags & 8u))
+                if ((cov != 0.f && !(pr.kind & kPaintNoOp)) || (prFlags & 8u))
                     resolve_pixel(pr, raster + path, cov, px, py, noise255, ramps, gradHeight, images, col, clip);
             }
             __syncthreads(); // every wave has read sLeft
@@ -2469,7 +2505,11 @@ __global__ void path_raster_kernel(const PathDataV* __restrict__ paths, const ui
     for (int k = 0; k < 4; ++k)
         pp.color255[k] = r.color255[k];
     pp.alpha = r.alpha;
-    pp.pad = 0;
+    pp.oneMinusAlpha = 1.f - r.alpha;
+    const bool plain = (flags & (8u | 16u | 0xf00u)) == 0u && (r.clipID & 0xffffu) == 0u &&
+                       paintType == SOLID_COLOR_PAINT_TYPE && r.image == 0u;
+    if (plain)
+        pp.kind |= r.alpha != 0.f ? kPaintPlain : kPaintNoOp;
     outPaint[i] = pp;
 }
 
