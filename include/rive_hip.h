@@ -374,6 +374,7 @@ typedef struct rive_hip_flush_stats
     uint32_t first_path_tess_location;
     uint32_t grad_simple_ramps;  /* two-texel ramps (one GradientSpan each) */
     uint32_t clear_color, dither;
+    uint32_t guard_pieces;       /* pieces of primitives clipped to the guard box */
 } rive_hip_flush_stats;
 
 int rive_hip_last_flush_stats(rive_hip_context* ctx, rive_hip_flush_stats* out);

@@ -1086,11 +1086,16 @@ struct FxP
 {
     int64_t x, y;
 };
+// Vertex positions on the 1/256-px grid.  A non-finite position has no
+// place on it: the triangles using it are dropped, as a GPU rasteriser drops
+// primitives with NaN or infinite positions.  Finite positions are kept
+// exactly up to 2^30 subpixels (4 194 304 px) from the origin and clamped
+// beyond, so that the int64 orientation products below stay exact.
+constexpr float kSnapLimit = 1073741824.f; // 2^30
+inline bool snappable(float v) { return std::isfinite(v * 256.f); }
 inline int64_t snap(float v)
 {
-    float c = std::min(std::max(v * 256.f, -8388608.f), 8388607.f);
-    if (std::isnan(c))
-        c = 0;
+    const float c = std::min(std::max(v * 256.f, -kSnapLimit), kSnapLimit);
     return (int64_t)rintf(c);
 }
 // Orientation with symbolic perturbation of q by (+e, +e^2).
@@ -1683,7 +1688,7 @@ void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* 
             {
                 const OutVertex o = draw_vertex(f, tex, patch[v], firstInstance + k);
                 ov[(size_t)k * 42 + v] = o;
-                if (!o.discard)
+                if (!o.discard && snappable(o.x) && snappable(o.y))
                 {
                     const int64_t y = snap(o.y);
                     lo = std::min(lo, y);
@@ -1710,6 +1715,9 @@ void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* 
                     const OutVertex& B = iv[idx[t * 3 + 1]];
                     const OutVertex& C = iv[idx[t * 3 + 2]];
                     if (A.discard || B.discard || C.discard)
+                        continue;
+                    if (!snappable(A.x) || !snappable(A.y) || !snappable(B.x) || !snappable(B.y) ||
+                        !snappable(C.x) || !snappable(C.y))
                         continue;
                     FxP a{snap(A.x), snap(A.y)}, b{snap(B.x), snap(B.y)}, c{snap(C.x), snap(C.y)};
                     int64_t area = edgeD(a, b, c);

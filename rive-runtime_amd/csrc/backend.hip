@@ -99,6 +99,9 @@ struct rive_hip_context
     uint32_t hostImagesCap = 0;
     DevBuf canvasTable, pathCanvas;
     DevBuf patchTable, patchWork; // SPANS flushes described by patch batches
+    // primitives leaving the guard box: their list, pieces per draw, the
+    // pieces' offsets and cursors, and the draw list with the pieces' draws
+    DevBuf specialList, farList, drawPieces, drawPieceOffset, pieceCursor, drawTessAll;
     CanvasDev* hostCanvases = nullptr; // pinned staging of the canvas table
     uint32_t hostCanvasesCap = 0;
     std::vector<rive_hip_canvas> lastCanvases;
@@ -146,7 +149,11 @@ struct rive_hip_context
 namespace
 {
 // The scratch buffers of a context (every DevBuf member), in one list.
-constexpr int kDevBufCount = 31;
+constexpr int kDevBufCount = 37;
+// The device scalars of a flush (totals, counters), 32 words: the pipeline
+// uses slots 0-7, device counting 8-13, guard-box pieces kFarSlot .. +3.
+constexpr size_t kScalarBytes = 128;
+constexpr int kFarSlot = 16;
 void devbufs(rive_hip_context* c, DevBuf** out)
 {
     DevBuf* all[kDevBufCount] = {&c->spanTexelCounts, &c->spanTexelOffsets, &c->scanPartials, &c->scalars,
@@ -156,7 +163,8 @@ void devbufs(rive_hip_context* c, DevBuf** out)
                                  &c->geoContours, &c->geoSpans, &c->geoSpanCounts, &c->geoSpanOffsets,
                                  &c->verts, &c->imageTable, &c->pathImage, &c->canvasTable,
                                  &c->pathCanvas, &c->cellPacked, &c->pathDraws, &c->pathPaint,
-                                 &c->patchTable, &c->patchWork};
+                                 &c->patchTable, &c->patchWork, &c->specialList, &c->farList, &c->drawPieces,
+                                 &c->drawPieceOffset, &c->pieceCursor, &c->drawTessAll};
     for (int i = 0; i < kDevBufCount; ++i)
         out[i] = all[i];
 }
@@ -538,6 +546,24 @@ int device_scan(rive_hip_context* ctx, const uint32_t* in, uint32_t n, uint32_t*
     return RIVE_HIP_OK;
 }
 
+// Grows a scratch buffer to at least `bytes`, keeping its first `keep` bytes.
+int grow_keep(rive_hip_context* ctx, DevBuf& b, size_t keep, size_t bytes)
+{
+    if (bytes <= b.cap && (!ctx->arena.active || b.carved))
+        return RIVE_HIP_OK;
+    DevBuf n{};
+    RIVE_TRY(ensure(ctx, n, bytes));
+    if (keep > 0)
+        HIP_TRY(hipMemcpyAsync(n.ptr, b.ptr, keep, hipMemcpyDeviceToDevice, ctx->stream));
+    if (!ctx->arena.active && b.ptr != nullptr)
+    {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipFree(b.ptr));
+    }
+    b = n;
+    return RIVE_HIP_OK;
+}
+
 int read_scalar(rive_hip_context* ctx, const uint32_t* devSlot, uint32_t* value)
 {
     HIP_TRY(hipMemcpyAsync(ctx->hostScalars, devSlot, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -804,24 +830,68 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     if (totalPrims > 0)
         vertex_kernel<<<(totalPrims + 255) / 256, 256, 0, ctx->stream>>>(
             P<TessTexel>(ctx->tess), (int)texels, paths, drawTess, drawCount, totalPrims, P<VtxPair>(ctx->verts));
+    RIVE_TRY(ensure(ctx, ctx->specialList, (size_t)std::max<uint32_t>(totalPrims, 1) * 4));
+    RIVE_TRY(ensure(ctx, ctx->farList, (size_t)std::max<uint32_t>(totalPrims, 1) * 4));
+    RIVE_TRY(ensure(ctx, ctx->drawPieces, (size_t)(drawCount + 1) * 4));
+    HIP_TRY(hipMemsetAsync(ctx->drawPieces.ptr, 0, (size_t)(drawCount + 1) * 4, ctx->stream));
     if (totalPrims > 0)
+    {
         expand_kernel<<<(totalPrims + 255) / 256, 256, 0, ctx->stream>>>(
             P<TessTexel>(ctx->tess), (int)texels, P<VtxPair>(ctx->verts), contours, contourCount, paths,
-            drawTess, drawCount, totalPrims, P<Prim>(ctx->prims), P<PathCells>(ctx->cells), scal + 5);
+            drawTess, drawCount, totalPrims, P<Prim>(ctx->prims), P<PathCells>(ctx->cells), scal + 5,
+            P<uint32_t>(ctx->specialList), scal + kFarSlot + 3);
+        // a grid-stride pass over the set-aside primitives (usually none)
+        special_prims_kernel<<<64, 256, 0, ctx->stream>>>(
+            P<TessTexel>(ctx->tess), (int)texels, contours, contourCount, paths, drawTess, drawCount,
+            P<Prim>(ctx->prims), P<PathCells>(ctx->cells), P<uint32_t>(ctx->specialList), scal + kFarSlot + 3,
+            P<uint32_t>(ctx->farList), scal + kFarSlot, P<uint32_t>(ctx->drawPieces));
+    }
     cells_layout_kernel<<<1, 1024, 0, ctx->stream>>>(P<PathCells>(ctx->cells), pathCount + 1,
                                                      P<uint32_t>(ctx->pathCanvas), P<CanvasDev>(ctx->canvasTable),
                                                      scal + 1);
     HIP_TRY(hipGetLastError());
-    // Cell total and the count of texels naming no ContourData record.
+    // Cell total, the count of texels naming no ContourData record, and the
+    // primitives leaving the guard box with their piece total.
     HIP_TRY(hipMemcpyAsync(ctx->hostScalars, scal + 1, 20, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->hostScalars + 5, scal + kFarSlot, 8, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     const uint32_t totalCells = ctx->hostScalars[0];
+    const uint32_t farPrims = ctx->hostScalars[5], pieces = ctx->hostScalars[6];
     if (ctx->hostScalars[4] != 0)
         return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT,
                     "tessellated vertices name contour IDs outside 1..contour_count");
     if (totalCells == 0xffffffffu)
         return fail(RIVE_HIP_ERROR_UNSUPPORTED, "flush too large: path cell grids exceed 2^32-1 cells");
+    if ((uint64_t)totalPrims + pieces > 0xfffffff0ull)
+        return fail(RIVE_HIP_ERROR_UNSUPPORTED, "flush too large: primitives and guard-box pieces exceed 2^32");
     int rc = RIVE_HIP_OK;
+    // Primitives leaving the guard box become their pieces, appended after
+    // the flush's primitives, each draw's in a range binned as one more draw.
+    const DrawTess* binDraws = drawTess;
+    uint32_t binDrawCount = drawCount;
+    ctx->stats.guard_pieces = pieces;
+    if (pieces > 0)
+    {
+        RIVE_TRY(grow_keep(ctx, ctx->prims, (size_t)totalPrims * sizeof(Prim),
+                           (size_t)(totalPrims + pieces) * sizeof(Prim)));
+        RIVE_TRY(ensure(ctx, ctx->drawPieceOffset, (size_t)(drawCount + 1) * 4));
+        RIVE_TRY(ensure(ctx, ctx->pieceCursor, (size_t)(drawCount + 1) * 4));
+        RIVE_TRY(ensure(ctx, ctx->drawTessAll, (size_t)drawCount * 2 * sizeof(DrawTess)));
+        HIP_TRY(hipMemsetAsync(ctx->pieceCursor.ptr, 0, (size_t)(drawCount + 1) * 4, ctx->stream));
+        rc = device_scan(ctx, P<uint32_t>(ctx->drawPieces), drawCount, P<uint32_t>(ctx->drawPieceOffset),
+                         scal + kFarSlot + 2);
+        if (rc)
+            return rc;
+        far_pieces_kernel<<<(farPrims + 255) / 256, 256, 0, ctx->stream>>>(
+            P<Prim>(ctx->prims), P<uint32_t>(ctx->farList), scal + kFarSlot, drawTess, drawCount,
+            P<uint32_t>(ctx->drawPieceOffset), P<uint32_t>(ctx->pieceCursor), totalPrims);
+        piece_draws_kernel<<<(drawCount + 255) / 256, 256, 0, ctx->stream>>>(
+            drawTess, drawCount, P<uint32_t>(ctx->drawPieces), P<uint32_t>(ctx->drawPieceOffset), totalPrims,
+            P<DrawTess>(ctx->drawTessAll));
+        HIP_TRY(hipGetLastError());
+        binDraws = P<DrawTess>(ctx->drawTessAll);
+        binDrawCount = drawCount * 2;
+    }
     RIVE_TRY(ensure(ctx, ctx->cellCount, (size_t)(totalCells + 1) * 4));
     RIVE_TRY(ensure(ctx, ctx->cellOffset, (size_t)(totalCells + 1) * 4));
     RIVE_TRY(ensure(ctx, ctx->cellCursor, (size_t)(totalCells + 1) * 4));
@@ -832,8 +902,8 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     RIVE_TRY(ensure(ctx, ctx->pathDraws, (size_t)(pathCount + 1) * 4));
     HIP_TRY(hipMemsetAsync(ctx->pathDraws.ptr, 0, (size_t)(pathCount + 1) * 4, ctx->stream));
     if (totalPrims > 0 && drawCount > 0)
-        bin_count_draw_kernel<<<drawCount, 256, 0, ctx->stream>>>(
-            P<Prim>(ctx->prims), drawTess, P<PathCells>(ctx->cells), P<uint32_t>(ctx->cellCount),
+        bin_count_draw_kernel<<<binDrawCount, 256, 0, ctx->stream>>>(
+            P<Prim>(ctx->prims), binDraws, P<PathCells>(ctx->cells), P<uint32_t>(ctx->cellCount),
             P<int32_t>(ctx->cellBackdrop), P<uint32_t>(ctx->pathDraws));
     rc = device_scan(ctx, P<uint32_t>(ctx->cellCount), totalCells, P<uint32_t>(ctx->cellOffset), scal + 2);
     if (rc)
@@ -855,8 +925,8 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     ctx->stats.tile_count = tileTotal;
     RIVE_TRY(ensure(ctx, ctx->binned, (size_t)std::max<uint32_t>(totalPairs, 1) * 4));
     if (totalPrims > 0 && drawCount > 0)
-        bin_scatter_draw_kernel<<<drawCount, 256, 0, ctx->stream>>>(
-            P<Prim>(ctx->prims), drawTess, P<PathCells>(ctx->cells), P<uint32_t>(ctx->cellOffset),
+        bin_scatter_draw_kernel<<<binDrawCount, 256, 0, ctx->stream>>>(
+            P<Prim>(ctx->prims), binDraws, P<PathCells>(ctx->cells), P<uint32_t>(ctx->cellOffset),
             P<uint32_t>(ctx->cellCursor), P<uint32_t>(ctx->pathDraws), P<uint32_t>(ctx->binned));
     RIVE_TRY(ensure(ctx, ctx->pathRaster, (size_t)(pathCount + 1) * sizeof(PathRaster)));
     RIVE_TRY(ensure(ctx, ctx->pathPaint, (size_t)(pathCount + 1) * sizeof(PathPaint)));
@@ -1129,9 +1199,9 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d, const GeoLay
         in.gradSpansHost =
             reinterpret_cast<const rive_hip_grad_span*>(host(RIVE_HIP_BUFFER_GRAD_SPAN, d->first_grad_span, 16));
     }
-    RIVE_TRY(ensure(ctx, ctx->scalars, 64));
+    RIVE_TRY(ensure(ctx, ctx->scalars, kScalarBytes));
     RIVE_TRY(ensure(ctx, ctx->drawTess, (size_t)std::max<uint32_t>(drawCountIn, 1) * sizeof(DrawTess)));
-    HIP_TRY(hipMemsetAsync(ctx->scalars.ptr, 0, 64, ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->scalars.ptr, 0, kScalarBytes, ctx->stream));
     if (pre == nullptr) // a frame's flushes have their start and counting events set already
         HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream)); // flush start
 

@@ -1280,9 +1280,9 @@ int geometry_counting_frame(rive_hip_context* ctx, const rive_hip_flush_desc* de
     if (n > 0 && (verbs == nullptr || points == nullptr))
         return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "GEOMETRY flush without VERB/POINT buffers");
     const uint32_t firstLocation = descs[0].first_path_tess_location ? descs[0].first_path_tess_location : kPatchSpan;
-    RIVE_TRY(ensure(ctx, ctx->scalars, 64));
+    RIVE_TRY(ensure(ctx, ctx->scalars, kScalarBytes));
     uint32_t* scal = P<uint32_t>(ctx->scalars);
-    HIP_TRY(hipMemsetAsync(ctx->scalars.ptr, 0, 64, ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->scalars.ptr, 0, kScalarBytes, ctx->stream));
     // 5 arrays of n+1 (tess, contour, record counts; tess, contour locals) +
     // 3 scans of n+1 + bounds values.
     RIVE_TRY(ensure(ctx, ctx->geoCounts, ((size_t)8 * (n + 1) + 4 * (kMaxFrameFlushes + 1) + 16) * 4));
@@ -1413,7 +1413,7 @@ extern "C" int rive_hip_count_draws(rive_hip_context* ctx, uint32_t n, uint32_t*
     const auto* points = reinterpret_cast<const V2d*>(ctx->bufs[RIVE_HIP_BUFFER_POINT].dev);
     if (verbs == nullptr || points == nullptr)
         return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "count without VERB/POINT buffers");
-    RIVE_TRY(ensure(ctx, ctx->scalars, 64));
+    RIVE_TRY(ensure(ctx, ctx->scalars, kScalarBytes));
     RIVE_TRY(ensure(ctx, ctx->geoCounts, (size_t)(3 * (size_t)(n + 1) + 16) * 4));
     uint32_t* tessCounts = P<uint32_t>(ctx->geoCounts);
     uint32_t* contourCounts = tessCounts + (n + 1);

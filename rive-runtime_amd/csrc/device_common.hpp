@@ -71,6 +71,19 @@ constexpr uint32_t kPrimFill = 1u;
 constexpr uint32_t kPrimStroke = 2u;
 constexpr uint32_t kPrimNegate = 4u;
 constexpr uint32_t kPrimEmpty = 8u;
+// Triangle 1 / 2 of the primitive is dropped (a vertex of it had no finite
+// position); the other and, for fills, the inner edge stay.
+constexpr uint32_t kPrimNoT1 = 16u;
+constexpr uint32_t kPrimNoT2 = 32u;
+// The primitive leaves the guard box; binning skips it and its pieces inside
+// the box are appended after the flush's primitives.
+constexpr uint32_t kPrimFar = 64u;
+// A fill primitive without its inner edge (its contour has no finite fan
+// midpoint): only its border triangles draw.
+constexpr uint32_t kPrimNoEdge = 128u;
+// Set aside by the expansion for the pass that handles non-finite vertices
+// and vertices outside the guard box.
+constexpr uint32_t kPrimSpecial = 256u;
 
 // Per-path raster parameters, unpacked from PathData/PaintData/PaintAux.
 // The fields the src-over resolve reads come first.

@@ -388,14 +388,31 @@ __device__ __forceinline__ float manhattan_pixel_width(const float M[4], float n
     float vy = M[1] * nx + M[3] * ny;
     return (fabsf(vx) + fabsf(vy)) * (1.f / (vx * vx + vy * vy));
 }
+// Vertex positions on the 1/256-px grid.  A non-finite position snaps to
+// kSnapInvalid: the triangles using it are dropped, as a GPU rasteriser drops
+// primitives with NaN or infinite positions.  Finite positions are exact up
+// to 2^30 subpixels (4 194 304 px) from the origin and clamped beyond.
+constexpr int32_t kSnapInvalid = INT32_MIN;
+constexpr float kSnapLimit = 1073741824.f; // 2^30
 __device__ __forceinline__ int32_t snap(float v)
 {
-    // NaN snaps to 0 (before the clamp: fmin/fmax would turn it into a bound).
-    if (isnan(v))
-        return 0;
-    const float c = fminf(fmaxf(v * 256.f, -8388608.f), 8388607.f);
-    return __float2int_rn(c);
+    const float s = v * 256.f;
+    if (!isfinite(s))
+        return kSnapInvalid;
+    return __float2int_rn(fminf(fmaxf(s, -kSnapLimit), kSnapLimit));
 }
+
+// The guard box (subpixels): the raster's integer predicates are exact for
+// coordinates in [-2^23, 2^23), and a canvas spans at most [0, 2^22).  A
+// primitive with a vertex outside the box is replaced by the pieces of it
+// that the clamp onto the box maps it to: for points strictly inside the
+// box, the clamp preserves every winding number (the segment from a point to
+// its clamp stays outside the open box), and pieces whose border or stroke
+// triangles lie inside the box keep them.
+constexpr int32_t kGuardLo = -(1 << 22);
+constexpr int32_t kGuardHi = (1 << 23) - 1;
+__device__ __forceinline__ bool in_guard(int32_t v) { return v >= kGuardLo && v <= kGuardHi; }
+__device__ __forceinline__ int32_t clamp_guard(int32_t v) { return min(max(v, kGuardLo), kGuardHi); }
 
 struct VsOut
 {
@@ -592,11 +609,294 @@ __global__ void vertex_kernel(const TessTexel* __restrict__ tex, int texels, con
     verts[g] = v;
 }
 
+// The pieces a primitive with vertices outside the guard box is replaced by
+// (written to out[0..n) when W; the count either way).
+//
+// Fill: the inner edge I0 -> I1 maps under the clamp onto the box to a chain
+// of straight pieces (the clamp is affine between the parameters where the
+// edge crosses the box's four lines); the points of the chain are rounded
+// once, so consecutive pieces, and the neighbouring primitives, meet exactly.
+// A piece inside the box keeps the matching part of the border
+// parallelogram (outer points at the same parameters; a fill's border
+// coverage is the affine function that is 1 on the inner and 0 on the outer
+// line, the same on any part of it); a piece on the box's boundary lies
+// 16384 px or more from every canvas and keeps the inner edge only.
+//
+// Stroke: the quad is cut at the parameters where either side crosses a box
+// line; parts inside the box are exact sub-quads with interpolated coverage,
+// parts wholly beyond one box line are dropped, and the rest (strokes wider
+// than the box margin) are clamped onto the box.
+__device__ __forceinline__ int32_t round_guard(double v)
+{
+    return clamp_guard((int32_t)fmin(fmax(rint(v), (double)kGuardLo), (double)kGuardHi));
+}
+__device__ __forceinline__ void add_crossings(double a, double b, double* ts, int* n)
+{
+    // parameters in (0, 1) where a + t (b - a) meets a box line
+    if (a == b)
+        return;
+    const double lines[2] = {(double)kGuardLo, (double)kGuardHi};
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+    {
+        const double t = (lines[k] - a) / (b - a);
+        if (t > 0. && t < 1.)
+            ts[(*n)++] = t;
+    }
+}
+__device__ __forceinline__ void sort_params(double* ts, int n)
+{
+    for (int i = 1; i < n; ++i)
+    {
+        const double v = ts[i];
+        int j = i - 1;
+        while (j >= 0 && ts[j] > v)
+        {
+            ts[j + 1] = ts[j];
+            --j;
+        }
+        ts[j + 1] = v;
+    }
+}
+template <bool W> __device__ uint32_t clip_pieces(const Prim& p, uint32_t flags, Prim* __restrict__ out)
+{
+    double ts[10];
+    int nt = 0;
+    ts[nt++] = 0.;
+    uint32_t n = 0;
+    const uint32_t keep = flags & ~(kPrimEmpty | kPrimFar);
+    if (flags & kPrimFill)
+    {
+        const double ax = p.p[0], ay = p.p[1], bx = p.p[4], by = p.p[5];
+        add_crossings(ax, bx, ts, &nt);
+        add_crossings(ay, by, ts, &nt);
+        ts[nt++] = 1.;
+        sort_params(ts, nt);
+        int32_t px = clamp_guard(p.p[0]), py = clamp_guard(p.p[1]);
+        for (int k = 0; k + 1 < nt; ++k)
+        {
+            const double t0 = ts[k], t1 = ts[k + 1];
+            if (!(t1 > t0))
+                continue;
+            const bool last = k + 2 == nt;
+            const int32_t qx = last ? clamp_guard(p.p[4]) : round_guard(ax + t1 * (bx - ax));
+            const int32_t qy = last ? clamp_guard(p.p[5]) : round_guard(ay + t1 * (by - ay));
+            const double tm = .5 * (t0 + t1);
+            const double mx = ax + tm * (bx - ax), my = ay + tm * (by - ay);
+            const bool inside = mx > (double)kGuardLo && mx < (double)kGuardHi && my > (double)kGuardLo &&
+                                my < (double)kGuardHi;
+            if (px != qx || py != qy || inside)
+            {
+                if (W)
+                {
+                    Prim q;
+                    q.p[0] = px;
+                    q.p[1] = py;
+                    q.p[4] = qx;
+                    q.p[5] = qy;
+                    if (inside)
+                    {
+                        const double ox = p.p[2], oy = p.p[3], ux = p.p[6], uy = p.p[7];
+                        q.p[2] = round_guard(ox + t0 * (ux - ox));
+                        q.p[3] = round_guard(oy + t0 * (uy - oy));
+                        q.p[6] = round_guard(ox + t1 * (ux - ox));
+                        q.p[7] = round_guard(oy + t1 * (uy - oy));
+                        q.flags = keep;
+                    }
+                    else
+                    {
+                        q.p[2] = px;
+                        q.p[3] = py;
+                        q.p[6] = qx;
+                        q.p[7] = qy;
+                        q.flags = keep | kPrimNoT1 | kPrimNoT2;
+                    }
+#pragma unroll
+                    for (int c = 0; c < 8; ++c)
+                        q.cov[c] = p.cov[c];
+                    q.path = p.path;
+                    q.pad[0] = q.pad[1] = 0;
+                    out[n] = q;
+                }
+                ++n;
+            }
+            px = qx;
+            py = qy;
+        }
+        return n;
+    }
+    // stroke: sides P0- -> P1- (vertices 0, 2) and P0+ -> P1+ (1, 3)
+    add_crossings(p.p[0], p.p[4], ts, &nt);
+    add_crossings(p.p[1], p.p[5], ts, &nt);
+    add_crossings(p.p[2], p.p[6], ts, &nt);
+    add_crossings(p.p[3], p.p[7], ts, &nt);
+    ts[nt++] = 1.;
+    sort_params(ts, nt);
+    for (int k = 0; k + 1 < nt; ++k)
+    {
+        const double t0 = ts[k], t1 = ts[k + 1];
+        if (!(t1 > t0))
+            continue;
+        double c[8]; // sub-quad corners: P0-, P0+, P1-, P1+ at t0 / t1
+#pragma unroll
+        for (int v = 0; v < 2; ++v)
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+            {
+                const double from = p.p[2 * v + a], to = p.p[4 + 2 * v + a];
+                c[2 * v + a] = from + t0 * (to - from);
+                c[4 + 2 * v + a] = from + t1 * (to - from);
+            }
+        bool allIn = true;
+        bool beyond[4] = {true, true, true, true}; // all x < lo, x > hi, y < lo, y > hi
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+        {
+            const double x = c[2 * v], y = c[2 * v + 1];
+            allIn &= x >= (double)kGuardLo && x <= (double)kGuardHi && y >= (double)kGuardLo &&
+                     y <= (double)kGuardHi;
+            beyond[0] &= x < (double)kGuardLo;
+            beyond[1] &= x > (double)kGuardHi;
+            beyond[2] &= y < (double)kGuardLo;
+            beyond[3] &= y > (double)kGuardHi;
+        }
+        if (!allIn && (beyond[0] || beyond[1] || beyond[2] || beyond[3]))
+            continue;
+        if (W)
+        {
+            Prim q;
+#pragma unroll
+            for (int v = 0; v < 8; ++v)
+                q.p[v] = round_guard(c[v]);
+            // exact ends where the cut is at the primitive's own ends
+            if (t0 == 0.)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    q.p[v] = clamp_guard(p.p[v]);
+            if (t1 == 1.)
+#pragma unroll
+                for (int v = 4; v < 8; ++v)
+                    q.p[v] = clamp_guard(p.p[v]);
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+            {
+                const float from = p.cov[a], to = p.cov[4 + a];
+                q.cov[a] = (float)((double)from + t0 * (double)(to - from));
+                q.cov[4 + a] = (float)((double)from + t1 * (double)(to - from));
+            }
+            q.flags = keep;
+            q.path = p.path;
+            q.pad[0] = q.pad[1] = 0;
+            out[n] = q;
+        }
+        ++n;
+    }
+    return n;
+}
+
+// A fill primitive's non-finite inner vertices move to the contour's
+// midpoint (the fan triangles through them are dropped).
+__device__ __forceinline__ void replace_inner_vertices(Prim& out, int32_t mx, int32_t my)
+{
+    if (out.p[0] == kSnapInvalid || out.p[1] == kSnapInvalid)
+    {
+        out.p[0] = mx;
+        out.p[1] = my;
+    }
+    if (out.p[4] == kSnapInvalid || out.p[5] == kSnapInvalid)
+    {
+        out.p[4] = mx;
+        out.p[5] = my;
+    }
+}
+
+// Drops the triangles through non-finite vertices: T1 uses vertices 0, 1, 2
+// and T2 vertices 2, 1, 3 (fills: I0, O0, I1 and I1, O0, O1; strokes: P0-,
+// P0+, P1- and P1-, P0+, P1+).  The non-finite vertices get a finite
+// vertex's place for the bounds.  True when nothing of the primitive is left.
+__device__ __forceinline__ bool drop_nonfinite(Prim& out, uint32_t* flags)
+{
+    bool bad[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        bad[k] = out.p[2 * k] == kSnapInvalid || out.p[2 * k + 1] == kSnapInvalid;
+    if (bad[0] || bad[1] || bad[2])
+        *flags |= kPrimNoT1;
+    if (bad[1] || bad[2] || bad[3])
+        *flags |= kPrimNoT2;
+    int good = -1;
+#pragma unroll
+    for (int k = 3; k >= 0; --k)
+        if (!bad[k])
+            good = k;
+    if (good < 0)
+        return true;
+    const int32_t gx = out.p[2 * good], gy = out.p[2 * good + 1];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (bad[k])
+        {
+            out.p[2 * k] = gx;
+            out.p[2 * k + 1] = gy;
+        }
+    return false;
+}
+
+// Records a primitive leaving the guard box with its piece count.
+__device__ __forceinline__ void count_far(const Prim& out, uint32_t flags, uint32_t g, uint32_t di,
+                                       uint32_t* __restrict__ farList, uint32_t* __restrict__ farCounts,
+                                       uint32_t* __restrict__ drawPieces)
+{
+    const uint32_t n = clip_pieces<false>(out, flags, nullptr);
+    if (n != 0)
+    {
+        farList[atomicAdd(&farCounts[0], 1u)] = g;
+        atomicAdd(&farCounts[1], n);
+        atomicAdd(&drawPieces[di], n);
+    }
+}
+
+// Writes the guard-box pieces of the far primitives after the flush's
+// primitives, each draw's pieces in its own range.
+__global__ void far_pieces_kernel(Prim* __restrict__ prims, const uint32_t* __restrict__ farList,
+                                  const uint32_t* __restrict__ farCounts, const DrawTess* __restrict__ draws,
+                                  uint32_t drawCount, const uint32_t* __restrict__ drawPieceOffset,
+                                  uint32_t* __restrict__ pieceCursor, uint32_t totalPrims)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= farCounts[0])
+        return;
+    const uint32_t g = farList[i];
+    const Prim p = prims[g];
+    const uint32_t di = find_draw(draws, drawCount, g);
+    const uint32_t n = clip_pieces<false>(p, p.flags, nullptr);
+    const uint32_t at = totalPrims + drawPieceOffset[di] + atomicAdd(&pieceCursor[di], n);
+    clip_pieces<true>(p, p.flags, prims + at);
+}
+
+// The draw list binning reads when a flush has far primitives: the draws,
+// then one more per draw owning its pieces.
+__global__ void piece_draws_kernel(const DrawTess* __restrict__ draws, uint32_t drawCount,
+                                   const uint32_t* __restrict__ drawPieces,
+                                   const uint32_t* __restrict__ drawPieceOffset, uint32_t totalPrims,
+                                   DrawTess* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= drawCount)
+        return;
+    DrawTess d = draws[i];
+    out[i] = d;
+    d.forwardCount = drawPieces[i];
+    d.primBase = totalPrims + drawPieceOffset[i];
+    out[drawCount + i] = d;
+}
+
 __global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels, const VtxPair* __restrict__ verts,
                               const ContourV* __restrict__ contours, uint32_t contourCount,
                               const PathDataV* __restrict__ paths, const DrawTess* __restrict__ draws,
                               uint32_t drawCount, uint32_t totalPrims, Prim* __restrict__ prims,
-                              PathCells* __restrict__ cells, uint32_t* __restrict__ badContours)
+                              PathCells* __restrict__ cells, uint32_t* __restrict__ badContours,
+                              uint32_t* __restrict__ specialList, uint32_t* __restrict__ specialCount)
 {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t di = find_draw_block(draws, drawCount, g);
@@ -660,8 +960,32 @@ __global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels, con
     uint32_t flags = stroke ? kPrimStroke : kPrimFill;
     if (!stroke && ((cfi & NEGATE_PATH_FILL_COVERAGE_FLAG) != 0u))
         flags |= kPrimNegate;
-    bool empty;
-    if (stroke)
+    bool empty = false;
+    // A fill contour without a finite midpoint has no fan: its inner edges
+    // draw nothing.
+    if (!stroke)
+    {
+        const float fx = (pd.m[0] * cd.mx + pd.m[2] * cd.my) + 0.f + pd.m[4];
+        const float fy = (pd.m[1] * cd.mx + pd.m[3] * cd.my) + 0.f + pd.m[5];
+        if (!isfinite(fx * 256.f) || !isfinite(fy * 256.f))
+            flags |= kPrimNoEdge;
+    }
+    int32_t lo = out.p[0], hi = out.p[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k)
+    {
+        lo = min(lo, out.p[k]);
+        hi = max(hi, out.p[k]);
+    }
+    if (!in_guard(lo) || !in_guard(hi))
+    {
+        // A vertex outside the guard box or without a finite position
+        // (kSnapInvalid lies below the box): special_prims_kernel settles it.
+        flags |= kPrimSpecial;
+        empty = true;
+        specialList[atomicAdd(specialCount, 1u)] = g;
+    }
+    else if (stroke)
     {
         // Triangles (P0-, P0+, P1-) and (P1-, P0+, P1+); clockwise only.
         int64_t A1 = orient64(out.p[0], out.p[1], out.p[2], out.p[3], out.p[4], out.p[5]);
@@ -672,7 +996,8 @@ __global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels, con
     {
         int64_t A1 = orient64(out.p[0], out.p[1], out.p[2], out.p[3], out.p[4], out.p[5]);
         int64_t A2 = orient64(out.p[4], out.p[5], out.p[2], out.p[3], out.p[6], out.p[7]);
-        empty = out.p[0] == out.p[4] && out.p[1] == out.p[5] && A1 == 0 && A2 == 0;
+        const bool noEdge = (flags & kPrimNoEdge) || (out.p[0] == out.p[4] && out.p[1] == out.p[5]);
+        empty = noEdge && A1 == 0 && A2 == 0;
     }
     if (empty)
         flags |= kPrimEmpty;
@@ -725,6 +1050,89 @@ __global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels, con
         atomicMin(&pc->ty0, ty0);
         atomicMax(&pc->tx1, tx1);
         atomicMax(&pc->ty1, ty1);
+    }
+}
+
+// The primitives expand_kernel set aside (a vertex outside the guard box or
+// without a finite position), in a grid-stride loop over their list: the
+// triangles through non-finite vertices are dropped (a fill's inner vertex
+// moves to the contour's midpoint, its fan triangles being dropped), and a
+// primitive still leaving the box is counted for its pieces.  Its path's
+// bounds take the primitive clamped onto the box.
+__global__ void special_prims_kernel(const TessTexel* __restrict__ tex, int texels,
+                                     const ContourV* __restrict__ contours, uint32_t contourCount,
+                                     const PathDataV* __restrict__ paths, const DrawTess* __restrict__ draws,
+                                     uint32_t drawCount, Prim* __restrict__ prims, PathCells* __restrict__ cells,
+                                     const uint32_t* __restrict__ specialList,
+                                     const uint32_t* __restrict__ specialCount, uint32_t* __restrict__ farList,
+                                     uint32_t* __restrict__ farCounts, uint32_t* __restrict__ drawPieces)
+{
+    const uint32_t n = *specialCount;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+    {
+        const uint32_t g = specialList[k];
+        Prim out = prims[g];
+        uint32_t flags = out.flags & ~(kPrimEmpty | kPrimSpecial);
+        const uint32_t di = find_draw(draws, drawCount, g);
+        const DrawTess d = draws[di];
+        const bool stroke = (flags & kPrimStroke) != 0u;
+        bool anyBad = false;
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            anyBad |= out.p[c] == kSnapInvalid;
+        if (anyBad && !stroke && !(flags & kPrimNoEdge))
+        {
+            const int i = min((int)(d.forwardBase + (g - d.primBase)), texels - 2);
+            const uint32_t cid = min(max(tex[i].w & CONTOUR_ID_MASK, 1u), max(contourCount, 1u));
+            const ContourV cd = contours[cid - 1];
+            const PathDataV pd = paths[d.pathID];
+            const float fx = (pd.m[0] * cd.mx + pd.m[2] * cd.my) + 0.f + pd.m[4];
+            const float fy = (pd.m[1] * cd.mx + pd.m[3] * cd.my) + 0.f + pd.m[5];
+            replace_inner_vertices(out, snap(fx), snap(fy));
+        }
+        bool empty = anyBad && drop_nonfinite(out, &flags);
+        if (!empty)
+        {
+            const int64_t A1 =
+                (flags & kPrimNoT1) ? 0 : orient64(out.p[0], out.p[1], out.p[2], out.p[3], out.p[4], out.p[5]);
+            const int64_t A2 =
+                (flags & kPrimNoT2) ? 0 : orient64(out.p[4], out.p[5], out.p[2], out.p[3], out.p[6], out.p[7]);
+            if (stroke)
+                empty = A1 <= 0 && A2 <= 0;
+            else
+                empty = ((flags & kPrimNoEdge) || (out.p[0] == out.p[4] && out.p[1] == out.p[5])) && A1 == 0 &&
+                        A2 == 0;
+        }
+        if (empty)
+        {
+            out.flags = flags | kPrimEmpty;
+            prims[g] = out;
+            continue;
+        }
+        int32_t lo = out.p[0], hi = out.p[0];
+#pragma unroll
+        for (int c = 1; c < 8; ++c)
+        {
+            lo = min(lo, out.p[c]);
+            hi = max(hi, out.p[c]);
+        }
+        if (!in_guard(lo) || !in_guard(hi))
+        {
+            // binned as its pieces, written once their count is known
+            flags |= kPrimEmpty | kPrimFar;
+            count_far(out, flags, g, di, farList, farCounts, drawPieces);
+        }
+        out.flags = flags;
+        prims[g] = out;
+        int32_t q[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            q[c] = clamp_guard(out.p[c]);
+        PathCells* pc = &cells[d.pathID];
+        atomicMin(&pc->tx0, min(min(q[0], q[2]), min(q[4], q[6])) >> (kSubpixelLog2 + kTileLog2));
+        atomicMin(&pc->ty0, min(min(q[1], q[3]), min(q[5], q[7])) >> (kSubpixelLog2 + kTileLog2));
+        atomicMax(&pc->tx1, (max(max(q[0], q[2]), max(q[4], q[6])) >> (kSubpixelLog2 + kTileLog2)) + 1);
+        atomicMax(&pc->ty1, (max(max(q[1], q[3]), max(q[5], q[7])) >> (kSubpixelLog2 + kTileLog2)) + 1);
     }
 }
 
@@ -927,7 +1335,7 @@ __device__ __forceinline__ void for_prim_cells(const Prim& p, const PathCells& c
 template <typename F>
 __device__ __forceinline__ void for_prim_crossings(const Prim& p, const PathCells& c, F visit)
 {
-    if (!(p.flags & kPrimFill))
+    if (!(p.flags & kPrimFill) || (p.flags & kPrimNoEdge))
         return;
     const int32_t ax = p.p[0], ay = p.p[1], bx = p.p[4], by = p.p[5];
     if (ay == by)
@@ -1380,7 +1788,7 @@ __device__ void fill_prim_left(const Prim& p, const TileGeom& g, int32_t* __rest
 {
     const int32_t X0 = g.X0, Y0 = g.Y0;
     const int32_t xmin = min(p.p[0], p.p[4]), xmax = max(p.p[0], p.p[4]);
-    if (!(xmin <= X0 && X0 < xmax))
+    if (!(xmin <= X0 && X0 < xmax) || (p.flags & kPrimNoEdge))
         return;
     const int neg = (p.flags & kPrimNegate) ? -1 : 1;
     if (p.p[1] != p.p[5])
@@ -1425,7 +1833,7 @@ __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* 
     const int32_t yq = Y0 + (j << kSubpixelLog2) + (1 << (kSubpixelLog2 - 1));
     const int neg = (p.flags & kPrimNegate) ? -1 : 1;
     const EdgeTI<I> e = make_edge_t<I>(p.p[0], p.p[1], p.p[4], p.p[5], X0, Y0);
-    if (p.p[1] != p.p[5])
+    if (p.p[1] != p.p[5] && !(p.flags & kPrimNoEdge))
     {
         const int32_t ymin = min(p.p[1], p.p[5]), ymax = max(p.p[1], p.p[5]);
         if (ymin <= yq && yq < ymax)
@@ -1455,7 +1863,7 @@ __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* 
         }
     }
     // Border triangle T1 = (I0, O0, I1), coverage 1 - lambda(O0).
-    const I A1 = orient_t<I>(p.p[0], p.p[1], p.p[2], p.p[3], p.p[4], p.p[5]);
+    const I A1 = (p.flags & kPrimNoT1) ? (I)0 : orient_t<I>(p.p[0], p.p[1], p.p[2], p.p[3], p.p[4], p.p[5]);
     if (A1 != 0)
     {
         int ilo, ihi;
@@ -1487,7 +1895,7 @@ __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* 
         }
     }
     // Border triangle T2 = (I1, O0, O1), coverage lambda(I1).
-    const I A2 = orient_t<I>(p.p[4], p.p[5], p.p[2], p.p[3], p.p[6], p.p[7]);
+    const I A2 = (p.flags & kPrimNoT2) ? (I)0 : orient_t<I>(p.p[4], p.p[5], p.p[2], p.p[3], p.p[6], p.p[7]);
     if (A2 != 0)
     {
         int ilo, ihi;
@@ -1528,8 +1936,8 @@ __device__ void stroke_prim_row(const Prim& p, int j, const TileGeom& g, uint32_
     const int32_t yq = Y0 + (j << kSubpixelLog2) + (1 << (kSubpixelLog2 - 1));
     // T1 = (P0-, P0+, P1-): vertices 0, 1, 2; T2 = (P1-, P0+, P1+): 2, 1, 3.
     // Only clockwise triangles draw.
-    const I A1 = orient_t<I>(p.p[0], p.p[1], p.p[2], p.p[3], p.p[4], p.p[5]);
-    const I A2 = orient_t<I>(p.p[4], p.p[5], p.p[2], p.p[3], p.p[6], p.p[7]);
+    const I A1 = (p.flags & kPrimNoT1) ? (I)0 : orient_t<I>(p.p[0], p.p[1], p.p[2], p.p[3], p.p[4], p.p[5]);
+    const I A2 = (p.flags & kPrimNoT2) ? (I)0 : orient_t<I>(p.p[4], p.p[5], p.p[2], p.p[3], p.p[6], p.p[7]);
     const EdgeTI<I> e12 = make_edge_t<I>(p.p[2], p.p[3], p.p[4], p.p[5], X0, Y0);
     if (A1 > 0)
     {

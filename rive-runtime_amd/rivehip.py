@@ -94,7 +94,7 @@ class FlushStats(ctypes.Structure):
         "width", "height", "load_action", "mode", "path_count", "draw_count", "contour_count",
         "span_count", "tess_vertex_count", "curve_record_count", "prim_count", "bin_entries",
         "tile_count", "grad_simple_rows", "grad_complex_rows", "aux_path_count", "grad_span_count",
-        "first_path_tess_location", "grad_simple_ramps", "clear_color", "dither")]
+        "first_path_tess_location", "grad_simple_ramps", "clear_color", "dither", "guard_pieces")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

@@ -59,6 +59,24 @@ def test_fill_edge_coverage_ramp(deg, c0):
     assert np.abs(img[..., 3].astype(float) - 255 * cov).max() <= 1.0
 
 
+@pytest.mark.parametrize("deg,c0,far", [(45, 40.3, 1e6), (30, 30.0, 3e6), (200, -20.0, 5e5)])
+def test_fill_edge_coverage_ramp_from_far_vertices(deg, c0, far):
+    """The same half-plane ramp when the edge runs between vertices up to
+    millions of pixels off the canvas (exact integer predicates on the
+    unclipped triangles, no coordinate clamp below 2^30 subpixels)."""
+    a = math.radians(deg)
+    nx, ny = math.cos(a), math.sin(a)
+    b = StreamBuilder(48, 48)
+    b.draw(b.path(halfplane_poly(nx, ny, c0, lo=-far, hi=far)), color=0xffffffff)
+    img = render(b)
+    ys, xs = np.mgrid[0:48, 0:48] + 0.5
+    d = c0 - (nx * xs + ny * ys)
+    cov = np.clip(0.5 + d / (abs(nx) + abs(ny)), 0, 1)
+    # float32 vertex positions a million pixels out carry ~0.06 px of error
+    assert np.abs(img[..., 3].astype(float) - 255 * cov).max() <= 16.0
+    assert np.mean(np.abs(img[..., 3].astype(float) - 255 * cov)) <= 1.0
+
+
 @pytest.mark.parametrize("x0,x1", [(10, 30.25), (10.5, 30.5), (9.75, 20.125)])
 def test_axis_aligned_rect_exact_and_premultiplied(x0, x1):
     b = StreamBuilder(40, 40)
