@@ -40,12 +40,16 @@ METRIC = "1080p 5k-path frames/s + HBM roofline % (plain vs batched); p50/p99 fr
 HBM_PEAK_GBS = 8000.0
 W, H, DRAWS = 1920, 1080, 5000
 SEED = rivehip.CONFIG_SEEDS[2]
-# HBM traffic of raster_kernel on this workload from rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by what the same
-# counter reads for a 1 GiB device copy (FETCH x2.0, WRITE x1.0); the runs are
-# summarised in profiles/raster_kernel_1080p_5000paths_traffic.md.
-RASTER_FETCH_KIB, RASTER_WRITE_KIB = 6639241, 40740
-RASTER_TRAFFIC_BYTES = RASTER_FETCH_KIB * 1024 * 2 + RASTER_WRITE_KIB * 1024
+# Counters of raster_kernel on this workload, summed over the frame's six
+# logical flushes (one dispatch each), from rocprofv3 --pmc runs summarised
+# in profiles/raster_kernel_1080p_5000paths_counters.md: FETCH_SIZE and
+# WRITE_SIZE (KiB), each scaled by what the same counter reads for a 1 GiB
+# device copy (FETCH x2.0, WRITE x1.0), and SQ_INSTS_VALU (wave instructions).
+RASTER_FRAME_FETCH_KIB, RASTER_FRAME_WRITE_KIB = 5056040, 244149
+RASTER_FRAME_VALU_INSTS = 8.526e9
+RASTER_FRAME_TRAFFIC_BYTES = RASTER_FRAME_FETCH_KIB * 1024 * 2 + RASTER_FRAME_WRITE_KIB * 1024
+# VALU issue: 256 CUs, one wave64 VALU instruction per CU per clock at 2.4 GHz.
+VALU_PEAK_PER_S = 256 * 2.4e9
 
 
 def percentile(xs, q):
@@ -289,9 +293,17 @@ def main():
                           "note": "host front end + PCIe upload + flush"},
         "roofline": {"kernel": "raster_kernel", "bound": "hbm", "achieved": round(achieved, 3),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
-                     "traffic": RASTER_TRAFFIC_BYTES, "algorithmic_bytes": b_raster,
-                     "kernel_ms": pass_ms["raster"], "launches": len(stats),
-                     "note": "algorithmic bytes and kernel time summed over the frame's logical flushes"},
+                     "traffic": int(RASTER_FRAME_TRAFFIC_BYTES / len(stats)),
+                     "algorithmic_bytes": int(b_raster / len(stats)),
+                     "kernel_ms": round(pass_ms["raster"] / len(stats), 4), "launches": len(stats),
+                     "note": "per launch: the frame's logical flushes' algorithmic bytes, measured HBM "
+                             "traffic and HIP-event kernel time, each averaged over its raster launches"},
+        "valu": {"kernel": "raster_kernel", "insts_per_frame": RASTER_FRAME_VALU_INSTS,
+                 "achieved": round(RASTER_FRAME_VALU_INSTS / raster_s, 1), "peak": VALU_PEAK_PER_S,
+                 "unit": "wave instructions/s",
+                 "frac": round(RASTER_FRAME_VALU_INSTS / raster_s / VALU_PEAK_PER_S, 4),
+                 "note": "SQ_INSTS_VALU per frame (rocprofv3) over the HIP-event raster time; the "
+                         "pass is bound by VALU issue, not HBM"},
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample_draws),
         "batched": batched,
     }
