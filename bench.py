@@ -176,6 +176,56 @@ def batched_leg(args):
                          "algorithmic_bytes": b_raster, "kernel_ms": pass_ms["raster"]}}
 
 
+SERVE_SEED = 0x5249564500000005
+SERVE_SIZES = [(256, 256), (512, 512), (1920, 1080)]
+SERVE_RATES = (100, 300, 1000, 3000, 5000)
+
+
+def serving_leg(args):
+    """Poisson request streams at each rate through rive_hip_serve: 8 seeded
+    animated scenes (transform trees of 50-500 nodes, 20-1500 cubic paths),
+    a uniform animation time, output size U{256^2, 512^2, 1920x1080};
+    continuous batching under a fixed HBM budget.  Latency is arrival to
+    pixels in HBM."""
+    import random
+
+    scenes = [rivehip.Scene(SERVE_SEED + k) for k in range(8)]
+    ctx = rivehip.Context(0)
+    budget = args.serve_budget_gb * (1 << 30)
+    out = []
+    try:
+        # warm-up: the pool's targets and the pipeline buffers of each size
+        ctx.serve(scenes, [(k % 8, 1.0, w, h, 0.0) for k, (w, h) in enumerate(SERVE_SIZES * 2)], budget,
+                  args.serve_max_batch)
+        for rate in SERVE_RATES:
+            rnd = random.Random(SERVE_SEED + rate)
+            n = max(1, int(rate * args.serve_seconds))
+            t, reqs = 0.0, []
+            for _ in range(n):
+                t += rnd.expovariate(rate)
+                w, h = rnd.choice(SERVE_SIZES)
+                reqs.append((rnd.randrange(len(scenes)), rnd.uniform(0.0, 4.0), w, h, t))
+            res = ctx.serve(scenes, reqs, budget, args.serve_max_batch)
+            lat = [(r.done_s - r.arrival_s) * 1e3 for r in res]
+            span = max(r.done_s for r in res) - reqs[0][4]
+            batches = {}
+            for r in res:
+                batches[r.batch] = r.batch_size
+            out.append({"rate_req_per_s": rate, "requests": n, "throughput_req_per_s": round(n / span, 1),
+                        "p50_ms": round(percentile(lat, 50), 3), "p99_ms": round(percentile(lat, 99), 3),
+                        "batches": len(batches),
+                        "mean_batch": round(statistics.mean(batches.values()), 2),
+                        "mpixels_per_s": round(sum(w * h for _, _, w, h, _ in reqs) / span / 1e6, 1)})
+            print(f"# serving {rate} req/s: {out[-1]}", file=sys.stderr, flush=True)
+    finally:
+        ctx.close()
+        for sc in scenes:
+            sc.close()
+    return {"workload": "Poisson arrivals of animated-scene requests (8 seeded scenes, output U{256^2, "
+                        "512^2, 1920x1080}), continuous batching", "hbm_budget_bytes": budget,
+            "max_batch": args.serve_max_batch, "seconds_of_arrivals": args.serve_seconds, "rates": out}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,6 +234,10 @@ def main():
     ap.add_argument("--cpu-sample-draws", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched 256-canvas leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-seconds", type=float, default=0.5, help="seconds of arrivals per rate")
+    ap.add_argument("--serve-budget-gb", type=float, default=8.0)
+    ap.add_argument("--serve-max-batch", type=int, default=64)
     ap.add_argument("--batch-canvases", type=int, default=256)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device once (the reference kernel for "
@@ -253,6 +307,7 @@ def main():
     if int(pixels[..., 3].max()) == 0:
         raise RuntimeError("the frame rendered nothing")
     batched = None if args.no_batched else batched_leg(args)
+    serving = None if args.no_serving else serving_leg(args)
 
     total_s = sum(step_ms) / 1e3
     fps = args.steps / total_s
@@ -306,6 +361,7 @@ def main():
                          "pass is bound by VALU issue, not HBM"},
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample_draws),
         "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

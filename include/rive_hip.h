@@ -493,6 +493,87 @@ uint32_t rive_stream_contour_bound(const rive_stream* stream);
  * arrays until rive_stream_free. */
 int rive_stream_generate_batch(uint64_t seed, uint32_t n, rive_stream** out);
 
+/* ------------------------------------------------------------------------ */
+/* Animated scenes and the serving loop.                                    */
+/* ------------------------------------------------------------------------ */
+
+/* A transform tree of nodes with keyframed affine animation and closed cubic
+ * paths hanging off the nodes, generated from a seed (std::mt19937_64 with
+ * the mappings above): 50-500 nodes of depth at most 6, 2-10 keys per node
+ * over 4 s, each key easing linearly or along a cubic bezier towards the
+ * next; 20-1500 paths of 8-32 cubics over [-256, 256]^2 with solid colours.
+ * Node transforms and their composition follow the reference runtime
+ * (transform_component.cpp:63-84; cubic_ease_interpolator.cpp:5-18;
+ * cubic_interpolator_solver.cpp:12-95).  Not a reference interface: the
+ * serving workload's scene source. */
+typedef struct rive_scene rive_scene;
+
+typedef struct rive_scene_info_t
+{
+    uint32_t node_count, path_count;
+    uint64_t cubic_count;
+    float duration; /* seconds */
+    float extent;   /* side of the square the paths live in */
+} rive_scene_info_t;
+
+typedef struct rive_scene_key
+{
+    float time;
+    float value[5]; /* x, y, rotation (radians), scaleX, scaleY */
+    uint32_t cubic; /* 0 linear, 1 cubic-bezier ease towards the next key */
+    float ease[4];  /* x1, y1, x2, y2 */
+} rive_scene_key;
+
+int rive_scene_generate(uint64_t seed, rive_scene** out);
+void rive_scene_free(rive_scene* scene);
+int rive_scene_info(const rive_scene* scene, rive_scene_info_t* info);
+/* Node `node`'s parent (-1 for the root) and keys (min(capacity, count)
+ * written; *key_count gets the count). */
+int rive_scene_node(const rive_scene* scene, uint32_t node, int32_t* parent, rive_scene_key* keys,
+                    uint32_t capacity, uint32_t* key_count);
+/* World transforms of every node at time t, 6 floats each ([a b c d tx ty]). */
+int rive_scene_world_transforms(const rive_scene* scene, float t, float* out);
+/* The scene at time t as a draw stream for a width x height target (the
+ * scene extent fits the shorter side, centred); free with rive_stream_free. */
+int rive_scene_stream(const rive_scene* scene, float t, uint32_t width, uint32_t height, rive_stream** out);
+/* The cubic-bezier easing of factor f (the scene keys' interpolation). */
+float rive_ease_cubic(float x1, float y1, float x2, float y2, float f);
+/* The serving loop's estimate of the HBM a request needs. */
+uint64_t rive_scene_request_bytes(const rive_scene* scene, uint32_t width, uint32_t height);
+
+typedef struct rive_hip_request
+{
+    uint32_t scene;   /* index into the scene array */
+    float time;       /* animation time, seconds */
+    uint32_t width, height;
+    double arrival_s; /* arrival, seconds after the loop starts; non-decreasing */
+} rive_hip_request;
+
+typedef struct rive_hip_serve_config
+{
+    uint64_t hbm_budget_bytes; /* admission limit per batch (estimated) */
+    uint32_t max_batch;        /* requests per batch */
+} rive_hip_serve_config;
+
+typedef struct rive_hip_request_result
+{
+    double arrival_s, admit_s, done_s; /* done: the batch's pixels are in HBM */
+    uint32_t batch, batch_size;
+    uint64_t batch_bytes;
+} rive_hip_request_result;
+
+/* Serves n timed requests in real time with continuous batching: whenever
+ * the device is free, every request that has arrived is admitted in arrival
+ * order while the batch's estimated HBM stays within the budget and its size
+ * within max_batch, and the batch renders as one batched flush (one canvas
+ * per request).  Output targets stay resident per size and are reused.
+ * pixels: NULL, or per request NULL or width * height * 4 bytes that receive
+ * the request's RGBA8 result.  Not a reference interface: the serving loop
+ * around RenderContext::flush. */
+int rive_hip_serve(rive_hip_context* ctx, const rive_scene* const* scenes, uint32_t scene_count,
+                   const rive_hip_request* requests, uint32_t n, const rive_hip_serve_config* config,
+                   rive_hip_request_result* results, uint8_t* const* pixels);
+
 #ifdef __cplusplus
 }
 #endif

@@ -100,6 +100,30 @@ class FlushStats(ctypes.Structure):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
 
 
+class SceneInfo(ctypes.Structure):
+    _fields_ = [("node_count", ctypes.c_uint32), ("path_count", ctypes.c_uint32),
+                ("cubic_count", ctypes.c_uint64), ("duration", ctypes.c_float), ("extent", ctypes.c_float)]
+
+
+class SceneKey(ctypes.Structure):
+    _fields_ = [("time", ctypes.c_float), ("value", ctypes.c_float * 5), ("cubic", ctypes.c_uint32),
+                ("ease", ctypes.c_float * 4)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_uint32), ("time", ctypes.c_float), ("width", ctypes.c_uint32),
+                ("height", ctypes.c_uint32), ("arrival_s", ctypes.c_double)]
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_batch", ctypes.c_uint32)]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("arrival_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double),
+                ("batch", ctypes.c_uint32), ("batch_size", ctypes.c_uint32), ("batch_bytes", ctypes.c_uint64)]
+
+
 EXPORTED = [
     "rive_hip_last_error", "rive_hip_create", "rive_hip_destroy", "rive_hip_buffer_resize",
     "rive_hip_buffer_map", "rive_hip_buffer_unmap", "rive_hip_target_create",
@@ -109,6 +133,9 @@ EXPORTED = [
     "rive_hip_image_destroy", "rive_hip_render_streams", "rive_hip_last_flush_canvases",
     "rive_stream_generate_batch", "rive_stream_contour_bound", "rive_hip_flush_noalloc",
     "rive_hip_count_draws", "rive_hip_render_stream_ex", "rive_hip_last_frame", "rive_hip_flush_frame",
+    "rive_scene_generate", "rive_scene_free", "rive_scene_info", "rive_scene_node",
+    "rive_scene_world_transforms", "rive_scene_stream", "rive_ease_cubic", "rive_scene_request_bytes",
+    "rive_hip_serve",
 ]
 
 _lib = None
@@ -159,6 +186,21 @@ def lib():
     L.rive_hip_image_create.argtypes = [vp, u32, u32, u32, vp, ctypes.POINTER(vp)]
     L.rive_hip_image_destroy.argtypes = [vp]
     L.rive_hip_image_destroy.restype = None
+    L.rive_scene_generate.argtypes = [ctypes.c_uint64, ctypes.POINTER(vp)]
+    L.rive_scene_free.argtypes = [vp]
+    L.rive_scene_free.restype = None
+    L.rive_scene_info.argtypes = [vp, ctypes.POINTER(SceneInfo)]
+    L.rive_scene_node.argtypes = [vp, u32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(SceneKey), u32,
+                                  ctypes.POINTER(u32)]
+    L.rive_scene_world_transforms.argtypes = [vp, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]
+    L.rive_scene_stream.argtypes = [vp, ctypes.c_float, u32, u32, ctypes.POINTER(ctypes.POINTER(Stream))]
+    L.rive_ease_cubic.argtypes = [ctypes.c_float] * 5
+    L.rive_ease_cubic.restype = ctypes.c_float
+    L.rive_scene_request_bytes.argtypes = [vp, u32, u32]
+    L.rive_scene_request_bytes.restype = ctypes.c_uint64
+    L.rive_hip_serve.argtypes = [vp, ctypes.POINTER(vp), u32, ctypes.POINTER(Request), u32,
+                                 ctypes.POINTER(ServeConfig), ctypes.POINTER(RequestResult),
+                                 ctypes.POINTER(ctypes.c_void_p)]
     _lib = L
     return L
 
@@ -211,6 +253,52 @@ def generate_batch(n=256, seed=None):
     arr = (ctypes.POINTER(Stream) * n)()
     check(L.rive_stream_generate_batch(CONFIG_SEEDS[4] if seed is None else seed, n, arr))
     return list(arr)
+
+
+class Scene:
+    """An animated scene (rive_scene_generate): a transform tree with keyframed
+    affine animation and cubic paths on its nodes."""
+
+    def __init__(self, seed):
+        self.L = lib()
+        self.h = ctypes.c_void_p()
+        check(self.L.rive_scene_generate(seed, ctypes.byref(self.h)))
+        info = SceneInfo()
+        check(self.L.rive_scene_info(self.h, ctypes.byref(info)))
+        self.node_count, self.path_count = int(info.node_count), int(info.path_count)
+        self.cubic_count, self.duration, self.extent = int(info.cubic_count), float(info.duration), float(info.extent)
+
+    def close(self):
+        if self.h:
+            self.L.rive_scene_free(self.h)
+            self.h = ctypes.c_void_p()
+
+    def node(self, i):
+        """(parent, [SceneKey...]) of node i."""
+        parent = ctypes.c_int32()
+        n = ctypes.c_uint32()
+        check(self.L.rive_scene_node(self.h, i, ctypes.byref(parent), None, 0, ctypes.byref(n)))
+        keys = (SceneKey * max(n.value, 1))()
+        check(self.L.rive_scene_node(self.h, i, ctypes.byref(parent), keys, n.value, ctypes.byref(n)))
+        return int(parent.value), [keys[k] for k in range(n.value)]
+
+    def world_transforms(self, t):
+        out = (ctypes.c_float * (6 * self.node_count))()
+        check(self.L.rive_scene_world_transforms(self.h, t, out))
+        return list(out)
+
+    def stream(self, t, width, height):
+        """The scene at time t as a draw stream (free with free_stream)."""
+        p = ctypes.POINTER(Stream)()
+        check(self.L.rive_scene_stream(self.h, t, width, height, ctypes.byref(p)))
+        return p
+
+    def request_bytes(self, width, height):
+        return int(self.L.rive_scene_request_bytes(self.h, width, height))
+
+
+def ease_cubic(x1, y1, x2, y2, f):
+    return float(lib().rive_ease_cubic(x1, y1, x2, y2, f))
 
 
 class Context:
@@ -346,6 +434,26 @@ class Context:
         desc.grad_complex_rows = stats.grad_complex_rows
         desc.first_path_tess_location = stats.first_path_tess_location
         self.flush(desc)
+
+    def serve(self, scenes, requests, hbm_budget_bytes, max_batch, pixels=None):
+        """rive_hip_serve: requests is a list of (scene index, time, width,
+        height, arrival seconds) in arrival order; pixels, when given, maps a
+        request index to a (height, width, 4) uint8 array receiving its
+        result.  Returns a list of RequestResult."""
+        n = len(requests)
+        sc = (ctypes.c_void_p * len(scenes))(*[x.h.value for x in scenes])
+        rq = (Request * max(n, 1))()
+        for i, (si, t, w, h, at) in enumerate(requests):
+            rq[i] = Request(si, t, w, h, at)
+        cfg = ServeConfig(int(hbm_budget_bytes), int(max_batch))
+        res = (RequestResult * max(n, 1))()
+        px = None
+        if pixels:
+            px = (ctypes.c_void_p * max(n, 1))()
+            for i, a in pixels.items():
+                px[i] = a.ctypes.data
+        check(self.L.rive_hip_serve(self.h, sc, len(scenes), rq, n, ctypes.byref(cfg), res, px))
+        return [res[i] for i in range(n)]
 
     def image(self, rgba_premul, mip_levels=0):
         """Uploads a premultiplied RGBA8 image (h, w, 4) and builds its mips."""
