@@ -57,11 +57,21 @@ struct Arena
     size_t used = 0, cap = 0;
 };
 
+// A mapped record buffer: one device copy and a ring of kRingSlots pinned
+// host staging slots (the reference keeps a ring of 3 per buffer,
+// gpu.hpp:77), so the host can write the next flush's records while the
+// device still reads the previous ones.  Mapping waits only for the copy out
+// of the slot it reuses.  `host` is the slot mapped last.
+constexpr int kRingSlots = 3;
 struct MappedBuffer
 {
     void* dev = nullptr;
     void* host = nullptr;
+    void* slots[kRingSlots] = {};
+    hipEvent_t copied[kRingSlots] = {}; // recorded after each slot's upload
+    int slot = 0;
     size_t size = 0; // requested size (exact, as the reference requires)
+    size_t cap = 0;  // allocated bytes (device and each slot)
     bool mapped = false;
 };
 
@@ -317,8 +327,13 @@ void rive_hip_destroy(rive_hip_context* ctx)
     {
         if (b.dev)
             (void)hipFree(b.dev);
-        if (b.host)
-            (void)hipHostFree(b.host);
+        for (int k = 0; k < kRingSlots; ++k)
+        {
+            if (b.slots[k])
+                (void)hipHostFree(b.slots[k]);
+            if (b.copied[k])
+                (void)hipEventDestroy(b.copied[k]);
+        }
     }
     DevBuf* all[kDevBufCount];
     devbufs(ctx, all);
@@ -347,18 +362,33 @@ int rive_hip_buffer_resize(rive_hip_context* ctx, int kind, size_t bytes)
     MappedBuffer& b = ctx->bufs[kind];
     if (b.mapped)
         return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "resize while mapped");
+    if (bytes <= b.cap)
+    {
+        // the allocation already holds it: only the size changes
+        b.size = bytes;
+        return RIVE_HIP_OK;
+    }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (b.dev)
         HIP_TRY(hipFree(b.dev));
-    if (b.host)
-        HIP_TRY(hipHostFree(b.host));
+    for (int k = 0; k < kRingSlots; ++k)
+        if (b.slots[k])
+        {
+            HIP_TRY(hipHostFree(b.slots[k]));
+            b.slots[k] = nullptr;
+        }
     b.dev = b.host = nullptr;
-    b.size = bytes;
-    if (bytes > 0)
+    b.size = b.cap = 0;
+    HIP_TRY(hipMalloc(&b.dev, bytes));
+    for (int k = 0; k < kRingSlots; ++k)
     {
-        HIP_TRY(hipMalloc(&b.dev, bytes));
-        HIP_TRY(hipHostMalloc(&b.host, bytes, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc(&b.slots[k], bytes, hipHostMallocDefault));
+        if (!b.copied[k])
+            HIP_TRY(hipEventCreateWithFlags(&b.copied[k], hipEventDisableTiming));
     }
+    b.slot = 0;
+    b.host = b.slots[0];
+    b.size = b.cap = bytes;
     return RIVE_HIP_OK;
 }
 
@@ -375,14 +405,16 @@ void* rive_hip_buffer_map(rive_hip_context* ctx, int kind, size_t bytes)
         fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "map exceeds buffer size");
         return nullptr;
     }
-    // The staging copy of the previous flush must have landed before the host
-    // overwrites it (the reference keeps 3 ring slots; one slot and a stream
-    // wait keep the same guarantee).
-    if (hipStreamSynchronize(ctx->stream) != hipSuccess)
+    // The next ring slot: its previous upload must have left it before the
+    // host overwrites it.
+    const int next = (b.slot + 1) % kRingSlots;
+    if (hipEventSynchronize(b.copied[next]) != hipSuccess)
     {
-        fail(RIVE_HIP_ERROR_DEVICE, "stream sync failed");
+        fail(RIVE_HIP_ERROR_DEVICE, "staging slot wait failed");
         return nullptr;
     }
+    b.slot = next;
+    b.host = b.slots[next];
     b.mapped = true;
     return b.host;
 }
@@ -399,6 +431,7 @@ int rive_hip_buffer_unmap(rive_hip_context* ctx, int kind, size_t bytes)
         return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "unmap exceeds buffer size");
     if (bytes > 0)
         HIP_TRY(hipMemcpyAsync(b.dev, b.host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipEventRecord(b.copied[b.slot], ctx->stream));
     return RIVE_HIP_OK;
 }
 
