@@ -603,6 +603,8 @@ DrawRecord RenderContext::makeRecord(const RiveRenderPath* path, const Mat2D& m,
 {
     DrawRecord r{};
     r.raw = &path->getRawPath();
+    r.residentVerb = path->residentVerb();
+    r.residentPoint = path->residentPoint();
     const bool stroke = paint != nullptr && paint->getIsStroked();
     float strokeRadius = stroke ? fmaxf(paint->getThickness() * .5f, 1.17549435e-38f) : 0.f;
     memcpy(r.path.matrix, m.values(), sizeof(r.path.matrix));
@@ -961,20 +963,34 @@ FlushDescriptor RenderContext::layoutFlush(std::vector<DrawRecord>& draws, rive_
     std::vector<std::pair<uint32_t, uint32_t>> simple;
     std::vector<const Gradient*> complex;
     std::vector<PendingGrad> pending;
+    // Draws whose paths all have resident geometry read it in place; any
+    // other flush copies every path's verbs and points.
+    bool resident = !draws.empty();
+    for (const DrawRecord& r : draws)
+        resident = resident && r.residentVerb >= 0;
+    fd.residentGeometry = resident;
     for (DrawRecord& r : draws)
     {
         const uint32_t pathID = (uint32_t)fd.paths.size();
         r.draw.path_id = pathID;
-        r.draw.first_verb = (uint32_t)fd.verbs.size();
         r.draw.verb_count = (uint32_t)r.raw->verbs().size();
-        r.draw.first_point = (uint32_t)(fd.points.size() / 2);
         r.draw.point_count = (uint32_t)r.raw->points().size();
-        for (PathVerb v : r.raw->verbs())
-            fd.verbs.push_back((uint8_t)v);
-        for (Vec2D p : r.raw->points())
+        if (resident)
         {
-            fd.points.push_back(p.x);
-            fd.points.push_back(p.y);
+            r.draw.first_verb = (uint32_t)r.residentVerb;
+            r.draw.first_point = r.residentPoint;
+        }
+        else
+        {
+            r.draw.first_verb = (uint32_t)fd.verbs.size();
+            r.draw.first_point = (uint32_t)(fd.points.size() / 2);
+            for (PathVerb v : r.raw->verbs())
+                fd.verbs.push_back((uint8_t)v);
+            for (Vec2D p : r.raw->points())
+            {
+                fd.points.push_back(p.x);
+                fd.points.push_back(p.y);
+            }
         }
         if (r.gradient != nullptr)
         {
@@ -1197,12 +1213,15 @@ bool RenderContextHIPImpl::flushFrame(const std::vector<gpu::FlushDescriptor>& f
         grads.insert(grads.end(), fd.gradSpans.begin(), fd.gradSpans.end());
         descs.push_back(d);
     }
+    bool resident = !flushes.empty();
+    for (const gpu::FlushDescriptor& fd : flushes)
+        resident = resident && fd.residentGeometry;
     if (!upload(RIVE_HIP_BUFFER_PATH, paths.data(), paths.size() * sizeof(rive_hip_path_data)) ||
         !upload(RIVE_HIP_BUFFER_PAINT, paints.data(), paints.size() * sizeof(rive_hip_paint_data)) ||
         !upload(RIVE_HIP_BUFFER_PAINT_AUX, aux.data(), aux.size() * sizeof(rive_hip_paint_aux)) ||
         !upload(RIVE_HIP_BUFFER_DRAW, draws.data(), draws.size() * sizeof(rive_hip_draw)) ||
-        !upload(RIVE_HIP_BUFFER_VERB, verbs.data(), verbs.size()) ||
-        !upload(RIVE_HIP_BUFFER_POINT, points.data(), points.size() * sizeof(float)) ||
+        (!resident && !upload(RIVE_HIP_BUFFER_VERB, verbs.data(), verbs.size())) ||
+        (!resident && !upload(RIVE_HIP_BUFFER_POINT, points.data(), points.size() * sizeof(float))) ||
         !upload(RIVE_HIP_BUFFER_GRAD_SPAN, grads.data(), grads.size() * sizeof(rive_hip_grad_span)))
         return false;
     return descs.empty() || rive_hip_flush_frame(m_ctx, descs.data(), (uint32_t)descs.size()) == RIVE_HIP_OK;
@@ -1431,3 +1450,122 @@ extern "C" int rive_hip_render_streams(rive_hip_context* ctx, const rive_stream*
     }
     return RIVE_HIP_OK;
 }
+
+namespace rive
+{
+int ResidentSession::init(rive_hip_context* ctx, const std::vector<ResidentScene>& scenes)
+{
+    m_ctx = ctx;
+    m_paths.assign(scenes.size(), {});
+    m_paints.assign(scenes.size(), {});
+    m_contours.assign(scenes.size(), 0);
+    std::vector<uint8_t> verbs;
+    std::vector<float> points;
+    for (size_t s = 0; s < scenes.size(); ++s)
+    {
+        const ResidentScene& sc = scenes[s];
+        m_paths[s].reserve(sc.pathCount);
+        m_paints[s].resize(sc.pathCount);
+        for (uint32_t i = 0; i < sc.pathCount; ++i)
+        {
+            const rive_stream_path& sp = sc.paths[i];
+            RiveRenderPath p((FillRule)sp.fill_rule);
+            const float* pts = sc.points + 2 * (size_t)sp.first_point;
+            uint32_t pc = 0;
+            for (uint32_t k = 0; k < sp.verb_count; ++k)
+            {
+                switch (sc.verbs[sp.first_verb + k])
+                {
+                    case 0:
+                        p.moveTo(pts[2 * pc], pts[2 * pc + 1]);
+                        pc += 1;
+                        ++m_contours[s];
+                        break;
+                    case 1:
+                        p.lineTo(pts[2 * pc], pts[2 * pc + 1]);
+                        pc += 1;
+                        break;
+                    case 4:
+                        p.cubicTo(pts[2 * pc], pts[2 * pc + 1], pts[2 * pc + 2], pts[2 * pc + 3], pts[2 * pc + 4],
+                                  pts[2 * pc + 5]);
+                        pc += 3;
+                        break;
+                    case 5:
+                        p.close();
+                        break;
+                    default:
+                        return RIVE_HIP_ERROR_INVALID_ARGUMENT;
+                }
+            }
+            p.setResident((uint32_t)verbs.size(), (uint32_t)(points.size() / 2));
+            for (PathVerb v : p.getRawPath().verbs())
+                verbs.push_back((uint8_t)v);
+            for (Vec2D q : p.getRawPath().points())
+            {
+                points.push_back(q.x);
+                points.push_back(q.y);
+            }
+            m_paths[s].push_back(std::move(p));
+            m_paints[s][i].color(sc.colors[i]);
+        }
+    }
+    // the resident geometry: one upload each for the whole session
+    auto put = [&](int kind, const void* data, size_t bytes) -> int {
+        const size_t need = std::max<size_t>(bytes, 16);
+        int st = rive_hip_buffer_resize(ctx, kind, need);
+        if (st != RIVE_HIP_OK)
+            return st;
+        void* p = rive_hip_buffer_map(ctx, kind, bytes);
+        if (p == nullptr && bytes > 0)
+            return RIVE_HIP_ERROR_DEVICE;
+        if (bytes > 0)
+            memcpy(p, data, bytes);
+        return rive_hip_buffer_unmap(ctx, kind, bytes);
+    };
+    int st = put(RIVE_HIP_BUFFER_VERB, verbs.data(), verbs.size());
+    if (st == RIVE_HIP_OK)
+        st = put(RIVE_HIP_BUFFER_POINT, points.data(), points.size() * sizeof(float));
+    return st;
+}
+
+int ResidentSession::render(const std::vector<SceneCanvas>& canvases)
+{
+    size_t begin = 0;
+    while (begin < canvases.size())
+    {
+        uint64_t contours = m_contours[canvases[begin].scene];
+        size_t end = begin + 1;
+        while (end < canvases.size() && contours + m_contours[canvases[end].scene] <= kMaxContoursPerFlush)
+            contours += m_contours[canvases[end++].scene];
+        gpu::RenderContext rc(std::make_unique<RenderContextHIPImpl>(m_ctx));
+        std::vector<std::unique_ptr<RiveRenderer>> renderers;
+        for (size_t c = begin; c < end; ++c)
+        {
+            const SceneCanvas& cv = canvases[c];
+            gpu::RenderContext::FrameDescriptor fd;
+            fd.renderTargetWidth = cv.width;
+            fd.renderTargetHeight = cv.height;
+            fd.clearColor = 0;
+            fd.ditherMode = 1;
+            if (c == begin)
+                rc.beginFrame(fd);
+            rc.beginCanvas(fd, cv.target);
+            renderers.push_back(std::make_unique<RiveRenderer>(&rc));
+            RiveRenderer& r = *renderers.back();
+            std::vector<RiveRenderPath>& paths = m_paths[cv.scene];
+            for (size_t i = 0; i < paths.size(); ++i)
+            {
+                const float* m = cv.matrices + 6 * i;
+                r.save();
+                r.transform(Mat2D(m[0], m[1], m[2], m[3], m[4], m[5]));
+                r.drawPath(&paths[i], &m_paints[cv.scene][i]);
+                r.restore();
+            }
+        }
+        if (!rc.flush(canvases[begin].target))
+            return RIVE_HIP_ERROR_DEVICE;
+        begin = end;
+    }
+    return RIVE_HIP_OK;
+}
+} // namespace rive

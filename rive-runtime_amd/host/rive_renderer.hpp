@@ -204,10 +204,22 @@ public:
     FillRule getFillRule() const { return m_fillRule; }
     const RawPath& getRawPath() const { return m_raw; }
     RawPath& mutableRawPath() { return m_raw; }
+    // Where this path's verbs and points already sit in the context's VERB
+    // and POINT buffers (uploaded once and kept resident); -1 when they are
+    // copied into every flush.
+    void setResident(uint32_t verbBase, uint32_t pointBase)
+    {
+        m_residentVerb = verbBase;
+        m_residentPoint = pointBase;
+    }
+    int64_t residentVerb() const { return m_residentVerb; }
+    uint32_t residentPoint() const { return m_residentPoint; }
 
 private:
     RawPath m_raw;
     FillRule m_fillRule;
+    int64_t m_residentVerb = -1;
+    uint32_t m_residentPoint = 0;
 };
 
 class RiveRenderPaint : public RenderPaint
@@ -342,6 +354,8 @@ struct DrawRecord
     const RawPath* raw;
     const Gradient* gradient; // null for solid/clip paints
     const RiveRenderImage* image;
+    int64_t residentVerb = -1; // the path's resident geometry, if any
+    uint32_t residentPoint = 0;
 };
 
 // One element of the clip stack a draw sits under (rive_renderer.hpp
@@ -373,6 +387,7 @@ struct FlushDescriptor
     uint32_t gradSimpleRows = 0, gradComplexRows = 0;
     std::vector<rive_hip_image*> images;
     std::vector<rive_hip_canvas> canvases; // batched flush; empty: renderTarget
+    bool residentGeometry = false;         // draws read resident VERB/POINT data; verbs/points empty
 };
 
 // Backend plugin interface (render_context_impl.hpp:26-243, reduced to what
@@ -557,4 +572,40 @@ private:
     rive_hip_context* m_ctx;
     size_t m_sizes[RIVE_HIP_BUFFER_KIND_COUNT] = {};
 };
+// The serving loop's scenes with their geometry resident on the device:
+// every path's verbs and points are uploaded once into the VERB and POINT
+// buffers, and each batch only records draws (matrices and paints) and
+// flushes, one canvas per request.
+struct ResidentScene
+{
+    const uint8_t* verbs;
+    const float* points;
+    const rive_stream_path* paths;
+    uint32_t pathCount;
+    const uint32_t* colors; // ColorInt per path, solid srcOver fills
+};
+
+struct SceneCanvas
+{
+    uint32_t scene;
+    const float* matrices; // 6 per path: the path's view matrix
+    rive_hip_target* target;
+    uint32_t width, height;
+};
+
+class ResidentSession
+{
+public:
+    int init(rive_hip_context* ctx, const std::vector<ResidentScene>& scenes);
+    // Renders the canvases with batched flushes (consecutive canvases share
+    // a flush while their contours fit the 16-bit contour IDs).
+    int render(const std::vector<SceneCanvas>& canvases);
+
+private:
+    rive_hip_context* m_ctx = nullptr;
+    std::vector<std::vector<RiveRenderPath>> m_paths;
+    std::vector<std::vector<RiveRenderPaint>> m_paints;
+    std::vector<uint32_t> m_contours;
+};
+
 } // namespace rive

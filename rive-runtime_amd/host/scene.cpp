@@ -15,10 +15,13 @@
 // HBM footprint stays within the budget and its size within the slot count,
 // renders the whole batch as one batched flush (one canvas per request, all
 // canvases in the same kernel launches), and records arrival, admission and
-// completion times.  Output targets of each size stay resident in a pool and
-// are reused across batches, as do the context's pipeline buffers.
+// completion times.  Every scene's path geometry is uploaded to the device
+// once and stays resident; a batch records only the paths' matrices and
+// paints.  Output targets of each size stay resident in a pool and are
+// reused across batches, as do the context's pipeline buffers.
 #include "../../include/rive_hip.h"
 #include "owned_stream.hpp"
+#include "rive_renderer.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -345,24 +348,36 @@ extern "C" float rive_ease_cubic(float x1, float y1, float x2, float y2, float f
 
 namespace
 {
-// The scene at time t drawn into width x height: the scene extent fits the
-// shorter side, centred.
+// Each path's matrix at time t for a width x height target: the scene
+// extent fits the shorter side, centred, under the path's node's world
+// transform.
+void view_matrices(const rive_scene* sc, float t, uint32_t W, uint32_t H, std::vector<float>& world,
+                   std::vector<float>& out)
+{
+    sc->world(t, world);
+    const float s = (float)std::min(W, H) / sc->extent;
+    const float view[6] = {s, 0.f, 0.f, s, W * .5f, H * .5f};
+    out.resize(6 * sc->paths.size());
+    for (uint32_t i = 0; i < sc->paths.size(); ++i)
+        mat_mul(view, &world[6 * (size_t)sc->pathNode[i]], &out[6 * (size_t)i]);
+}
+
+// The scene at time t as a draw stream for width x height.
 OwnedStream* scene_stream(const rive_scene* sc, float t, uint32_t W, uint32_t H, std::vector<float>& world)
 {
     auto* o = new OwnedStream;
     o->verbs = sc->verbs;
     o->points = sc->points;
     o->paths = sc->paths;
-    sc->world(t, world);
-    const float s = (float)std::min(W, H) / sc->extent;
-    const float view[6] = {s, 0.f, 0.f, s, W * .5f, H * .5f};
+    std::vector<float> m;
+    view_matrices(sc, t, W, H, world, m);
     o->draws.resize(sc->paths.size());
     for (uint32_t i = 0; i < sc->paths.size(); ++i)
     {
         rive_stream_draw& d = o->draws[i];
         std::memset(&d, 0, sizeof(d));
         d.path = i;
-        mat_mul(view, &world[6 * (size_t)sc->pathNode[i]], d.matrix);
+        std::memcpy(d.matrix, &m[6 * (size_t)i], sizeof(d.matrix));
         d.blend_mode = 3; // srcOver
         d.color = sc->colors[i];
         d.opacity = 1.f;
@@ -431,6 +446,22 @@ extern "C" int rive_hip_serve(rive_hip_context* ctx, const rive_scene* const* sc
             owned.push_back(*t);
         return st;
     };
+    // every scene's geometry resident on the device for the whole loop
+    rive::ResidentSession session;
+    {
+        std::vector<rive::ResidentScene> rs;
+        for (uint32_t i = 0; i < scene_count; ++i)
+        {
+            const rive_scene* sc = scenes[i];
+            if (sc == nullptr)
+                return RIVE_HIP_ERROR_INVALID_ARGUMENT;
+            rs.push_back({sc->verbs.data(), sc->points.data(), sc->paths.data(), (uint32_t)sc->paths.size(),
+                          sc->colors.data()});
+        }
+        const int st = session.init(ctx, rs);
+        if (st != RIVE_HIP_OK)
+            return st;
+    }
     int status = RIVE_HIP_OK;
     const auto t0 = std::chrono::steady_clock::now();
     std::deque<uint32_t> pending;
@@ -461,22 +492,24 @@ extern "C" int rive_hip_serve(rive_hip_context* ctx, const rive_scene* const* sc
             pending.pop_front();
         }
         const double admit = now_s(t0);
-        std::vector<std::unique_ptr<OwnedStream>> streams;
-        std::vector<const rive_stream*> sp;
+        // only the transforms change per request: the paths' view matrices
+        std::vector<std::vector<float>> matrices(batch.size());
+        std::vector<rive::SceneCanvas> canvases;
         std::vector<rive_hip_target*> targets;
-        for (uint32_t i : batch)
+        for (size_t k = 0; k < batch.size(); ++k)
         {
-            const rive_hip_request& q = requests[i];
-            streams.emplace_back(scene_stream(scenes[q.scene], q.time, q.width, q.height, world));
-            sp.push_back(&streams.back()->s);
+            const rive_hip_request& q = requests[batch[k]];
+            const rive_scene* sc = scenes[q.scene];
+            view_matrices(sc, q.time, q.width, q.height, world, matrices[k]);
             rive_hip_target* t = nullptr;
             status = take(q.width, q.height, &t);
             if (status != RIVE_HIP_OK)
                 break;
             targets.push_back(t);
+            canvases.push_back({q.scene, matrices[k].data(), t, q.width, q.height});
         }
         if (status == RIVE_HIP_OK)
-            status = rive_hip_render_streams(ctx, sp.data(), targets.data(), (uint32_t)batch.size());
+            status = session.render(canvases);
         if (status == RIVE_HIP_OK)
             status = rive_hip_sync(ctx);
         const double finish = now_s(t0);
