@@ -770,6 +770,45 @@ struct LogicalFlushState
 };
 } // namespace
 
+bool RenderContext::prepareCanvases(rive_hip_target* target, FlushDescriptor* out)
+{
+    if (m_canvases.empty())
+        return false;
+    std::vector<DrawRecord> draws;
+    std::vector<rive_hip_canvas> canvases;
+    for (size_t c = 0; c < m_canvases.size(); ++c)
+    {
+        const size_t end = c + 1 < m_canvases.size() ? m_canvases[c + 1].firstBatch : m_batches.size();
+        LogicalFlushState st;
+        const size_t first = draws.size();
+        for (size_t b = m_canvases[c].firstBatch; b < end; ++b)
+        {
+            Batch& batch = m_batches[b];
+            const size_t firstClip = st.firstClipToIssue(batch.clips);
+            uint32_t parent = firstClip > 0 ? st.clipIDOf(batch.clips[firstClip - 1].serial) : 0;
+            for (size_t i = firstClip; i < batch.clips.size(); ++i)
+            {
+                const ClipRef& ce = batch.clips[i];
+                const uint32_t id = ++st.lastClipID;
+                st.clipIDs.push_back({ce.serial, id});
+                if (ce.onFrame)
+                    draws.push_back(makeRecord(ce.path, ce.matrix, nullptr, id, true, parent, ce.fillRule));
+                parent = id;
+            }
+            if (!batch.clips.empty())
+                st.contentSerial = batch.clips.back().serial;
+            draws.push_back(batch.draw);
+            draws.back().paint.params = (draws.back().paint.params & 0xffffu) | (parent << 16);
+        }
+        canvases.push_back({m_canvases[c].target, (uint32_t)first + 1, (uint32_t)(draws.size() - first),
+                            m_canvases[c].frame.clearColor, m_canvases[c].frame.loadAction});
+    }
+    *out = layoutFlush(draws, target, m_frame.loadAction, &canvases);
+    m_batches.clear();
+    m_canvases.clear();
+    return true;
+}
+
 bool RenderContext::flush(rive_hip_target* target, uint32_t maxFlushes, uint32_t* flushCount)
 {
     if (flushCount != nullptr)
@@ -778,40 +817,11 @@ bool RenderContext::flush(rive_hip_target* target, uint32_t maxFlushes, uint32_t
     {
         // Batched: every canvas in one flush, each with its own clip IDs; no
         // logical-flush split (no counterpart in the reference).
-        std::vector<DrawRecord> draws;
-        std::vector<rive_hip_canvas> canvases;
-        for (size_t c = 0; c < m_canvases.size(); ++c)
-        {
-            const size_t end = c + 1 < m_canvases.size() ? m_canvases[c + 1].firstBatch : m_batches.size();
-            LogicalFlushState st;
-            const size_t first = draws.size();
-            for (size_t b = m_canvases[c].firstBatch; b < end; ++b)
-            {
-                Batch& batch = m_batches[b];
-                const size_t firstClip = st.firstClipToIssue(batch.clips);
-                uint32_t parent = firstClip > 0 ? st.clipIDOf(batch.clips[firstClip - 1].serial) : 0;
-                for (size_t i = firstClip; i < batch.clips.size(); ++i)
-                {
-                    const ClipRef& ce = batch.clips[i];
-                    const uint32_t id = ++st.lastClipID;
-                    st.clipIDs.push_back({ce.serial, id});
-                    if (ce.onFrame)
-                        draws.push_back(makeRecord(ce.path, ce.matrix, nullptr, id, true, parent, ce.fillRule));
-                    parent = id;
-                }
-                if (!batch.clips.empty())
-                    st.contentSerial = batch.clips.back().serial;
-                draws.push_back(batch.draw);
-                draws.back().paint.params = (draws.back().paint.params & 0xffffu) | (parent << 16);
-            }
-            canvases.push_back({m_canvases[c].target, (uint32_t)first + 1, (uint32_t)(draws.size() - first),
-                                m_canvases[c].frame.clearColor, m_canvases[c].frame.loadAction});
-        }
-        bool ok = m_impl->flushFrame({layoutFlush(draws, target, m_frame.loadAction, &canvases)});
+        FlushDescriptor fd;
+        prepareCanvases(target, &fd);
+        bool ok = m_impl->flushFrame({fd});
         if (flushCount != nullptr)
             *flushCount = 1;
-        m_batches.clear();
-        m_canvases.clear();
         m_modulatedGradients.clear();
         return ok;
     }
@@ -1530,6 +1540,23 @@ int ResidentSession::init(rive_hip_context* ctx, const std::vector<ResidentScene
 
 int ResidentSession::render(const std::vector<SceneCanvas>& canvases)
 {
+    std::vector<gpu::FlushDescriptor> flushes;
+    int st = prepare(canvases, &flushes);
+    return st != RIVE_HIP_OK ? st : submit(flushes);
+}
+
+int ResidentSession::submit(const std::vector<gpu::FlushDescriptor>& flushes)
+{
+    RenderContextHIPImpl impl(m_ctx);
+    for (const gpu::FlushDescriptor& fd : flushes)
+        if (!impl.flushFrame({fd}))
+            return RIVE_HIP_ERROR_DEVICE;
+    return RIVE_HIP_OK;
+}
+
+int ResidentSession::prepare(const std::vector<SceneCanvas>& canvases, std::vector<gpu::FlushDescriptor>* out)
+{
+    out->clear();
     size_t begin = 0;
     while (begin < canvases.size())
     {
@@ -1562,8 +1589,9 @@ int ResidentSession::render(const std::vector<SceneCanvas>& canvases)
                 r.restore();
             }
         }
-        if (!rc.flush(canvases[begin].target))
-            return RIVE_HIP_ERROR_DEVICE;
+        out->emplace_back();
+        if (!rc.prepareCanvases(canvases[begin].target, &out->back()))
+            return RIVE_HIP_ERROR_INVALID_ARGUMENT;
         begin = end;
     }
     return RIVE_HIP_OK;

@@ -428,6 +428,11 @@ public:
     // maxFlushes > 0 stops after that many (a debugging aid); flushCount, if
     // given, receives the frame's number of logical flushes.
     bool flush(rive_hip_target* target, uint32_t maxFlushes = 0, uint32_t* flushCount = nullptr);
+    // The batched-canvas flush's host half: lays every canvas begun since
+    // beginFrame out into one FlushDescriptor (no device work) and clears the
+    // frame; RenderContextImpl::flushFrame then submits it.  False when no
+    // canvas was begun.
+    bool prepareCanvases(rive_hip_target* target, FlushDescriptor* out);
     const FrameDescriptor& frameDescriptor() const { return m_frame; }
     RenderContextImpl* impl() { return m_impl.get(); }
 
@@ -600,6 +605,11 @@ public:
     // Renders the canvases with batched flushes (consecutive canvases share
     // a flush while their contours fit the 16-bit contour IDs).
     int render(const std::vector<SceneCanvas>& canvases);
+    // render() in two halves: prepare records and lays the flushes out on
+    // the host (no device work, so it can overlap earlier flushes still on
+    // the device); submit uploads them and launches their passes.
+    int prepare(const std::vector<SceneCanvas>& canvases, std::vector<gpu::FlushDescriptor>* out);
+    int submit(const std::vector<gpu::FlushDescriptor>& flushes);
 
 private:
     rive_hip_context* m_ctx = nullptr;

@@ -15,7 +15,8 @@
 // HBM footprint stays within the budget and its size within the slot count,
 // renders the whole batch as one batched flush (one canvas per request, all
 // canvases in the same kernel launches), and records arrival, admission and
-// completion times.  Every scene's path geometry is uploaded to the device
+// completion times.  While a batch is on the device the host already admits
+// and lays out the next one.  Every scene's path geometry is uploaded to the device
 // once and stays resident; a batch records only the paths' matrices and
 // paints.  Output targets of each size stay resident in a pool and are
 // reused across batches, as do the context's pipeline buffers.
@@ -467,69 +468,93 @@ extern "C" int rive_hip_serve(rive_hip_context* ctx, const rive_scene* const* sc
     std::deque<uint32_t> pending;
     uint32_t next = 0, done = 0, batchIndex = 0;
     std::vector<float> world;
-    while (done < n && status == RIVE_HIP_OK)
+    // A batch laid out on the host: its requests, targets and flushes.
+    struct Prepared
     {
-        double now = now_s(t0);
+        std::vector<uint32_t> requests;
+        std::vector<std::vector<float>> matrices;
+        std::vector<rive_hip_target*> targets;
+        std::vector<rive::gpu::FlushDescriptor> flushes;
+        double admit = 0;
+        uint64_t bytes = 0;
+    };
+    // Admits the requests that have arrived (in arrival order, under the slot
+    // count and the HBM budget; a request larger than the budget alone is
+    // admitted alone) and lays them out; an empty batch when none waits.
+    auto form = [&](Prepared& p) -> int {
+        p = Prepared{};
+        const double now = now_s(t0);
         while (next < n && requests[next].arrival_s <= now)
             pending.push_back(next++);
-        if (pending.empty())
-        {
-            std::this_thread::sleep_for(std::chrono::duration<double>(requests[next].arrival_s - now));
-            continue;
-        }
-        // admit in arrival order under the slot count and the HBM budget (a
-        // request larger than the budget alone is admitted alone)
-        std::vector<uint32_t> batch;
-        uint64_t bytes = 0;
-        while (!pending.empty() && batch.size() < config->max_batch)
+        while (!pending.empty() && p.requests.size() < config->max_batch)
         {
             const rive_hip_request& q = requests[pending.front()];
             const uint64_t b = request_bytes(scenes[q.scene], q.width, q.height);
-            if (!batch.empty() && bytes + b > config->hbm_budget_bytes)
+            if (!p.requests.empty() && p.bytes + b > config->hbm_budget_bytes)
                 break;
-            bytes += b;
-            batch.push_back(pending.front());
+            p.bytes += b;
+            p.requests.push_back(pending.front());
             pending.pop_front();
         }
-        const double admit = now_s(t0);
+        if (p.requests.empty())
+            return RIVE_HIP_OK;
+        p.admit = now_s(t0);
         // only the transforms change per request: the paths' view matrices
-        std::vector<std::vector<float>> matrices(batch.size());
+        p.matrices.resize(p.requests.size());
         std::vector<rive::SceneCanvas> canvases;
-        std::vector<rive_hip_target*> targets;
-        for (size_t k = 0; k < batch.size(); ++k)
+        for (size_t k = 0; k < p.requests.size(); ++k)
         {
-            const rive_hip_request& q = requests[batch[k]];
-            const rive_scene* sc = scenes[q.scene];
-            view_matrices(sc, q.time, q.width, q.height, world, matrices[k]);
+            const rive_hip_request& q = requests[p.requests[k]];
+            view_matrices(scenes[q.scene], q.time, q.width, q.height, world, p.matrices[k]);
             rive_hip_target* t = nullptr;
-            status = take(q.width, q.height, &t);
+            int st = take(q.width, q.height, &t);
+            if (st != RIVE_HIP_OK)
+                return st;
+            p.targets.push_back(t);
+            canvases.push_back({q.scene, p.matrices[k].data(), t, q.width, q.height});
+        }
+        return session.prepare(canvases, &p.flushes);
+    };
+    Prepared cur, following;
+    while (done < n && status == RIVE_HIP_OK)
+    {
+        if (cur.requests.empty())
+        {
+            status = form(cur);
             if (status != RIVE_HIP_OK)
                 break;
-            targets.push_back(t);
-            canvases.push_back({q.scene, matrices[k].data(), t, q.width, q.height});
+            if (cur.requests.empty())
+            {
+                std::this_thread::sleep_for(std::chrono::duration<double>(requests[next].arrival_s - now_s(t0)));
+                continue;
+            }
         }
+        status = session.submit(cur.flushes);
+        // lay the next batch out on the host while this one is on the device
         if (status == RIVE_HIP_OK)
-            status = session.render(canvases);
+            status = form(following);
         if (status == RIVE_HIP_OK)
             status = rive_hip_sync(ctx);
         const double finish = now_s(t0);
-        for (size_t k = 0; k < batch.size() && status == RIVE_HIP_OK; ++k)
+        for (size_t k = 0; k < cur.requests.size() && status == RIVE_HIP_OK; ++k)
         {
-            const uint32_t i = batch[k];
+            const uint32_t i = cur.requests[k];
             if (pixels != nullptr && pixels[i] != nullptr)
-                status = rive_hip_read_pixels(ctx, targets[k], pixels[i]);
+                status = rive_hip_read_pixels(ctx, cur.targets[k], pixels[i]);
             rive_hip_request_result& res = results[i];
             res.arrival_s = requests[i].arrival_s;
-            res.admit_s = admit;
+            res.admit_s = cur.admit;
             res.done_s = finish;
             res.batch = batchIndex;
-            res.batch_size = (uint32_t)batch.size();
-            res.batch_bytes = bytes;
+            res.batch_size = (uint32_t)cur.requests.size();
+            res.batch_bytes = cur.bytes;
             ++done;
         }
-        for (size_t k = 0; k < targets.size(); ++k)
-            pool[{requests[batch[k]].width, requests[batch[k]].height}].push_back(targets[k]);
+        for (size_t k = 0; k < cur.targets.size(); ++k)
+            pool[{requests[cur.requests[k]].width, requests[cur.requests[k]].height}].push_back(cur.targets[k]);
         ++batchIndex;
+        cur = std::move(following);
+        following = Prepared{};
     }
     for (rive_hip_target* t : owned)
         rive_hip_target_destroy(t);
