@@ -574,6 +574,34 @@ int rive_hip_serve(rive_hip_context* ctx, const rive_scene* const* scenes, uint3
                    const rive_hip_request* requests, uint32_t n, const rive_hip_serve_config* config,
                    rive_hip_request_result* results, uint8_t* const* pixels);
 
+/* ------------------------------------------------------------------------ */
+/* Deferred 2D command stream (record / replay).                             */
+/* ------------------------------------------------------------------------ */
+
+/* A recorded sequence of rive::Renderer calls as the reference's wire
+ * format (render_commands.hpp:17-350, command_stream.hpp:34-165): opcode
+ * byte + plain-data payload per command, variable-size data in an 8-byte
+ * aligned blob arena.  Recorded here from a draw stream: makePath per path,
+ * per draw makePaint and its paint settings (with makeLinearGradient /
+ * makeRadialGradient + paintShader for gradients), save, transform,
+ * clipPath per clip, drawPath, restore.  Streams with images are
+ * unsupported.  The frame (size, clear colour, dither) travels beside it. */
+typedef struct rive_cmd_buffer rive_cmd_buffer;
+
+int rive_cmd_record_stream(const rive_stream* stream, rive_cmd_buffer** out);
+void rive_cmd_free(rive_cmd_buffer* buffer);
+/* The buffer's bytes (valid until it is freed). */
+int rive_cmd_bytes(const rive_cmd_buffer* buffer, const uint8_t** commands, uint64_t* command_bytes,
+                   const uint8_t** blobs, uint64_t* blob_bytes);
+/* A buffer holding copies of the given bytes (e.g. read from a file). */
+int rive_cmd_from_bytes(const uint8_t* commands, uint64_t command_bytes, const uint8_t* blobs,
+                        uint64_t blob_bytes, uint32_t width, uint32_t height, uint32_t clear_color,
+                        uint32_t dither, rive_cmd_buffer** out);
+/* Replays the commands through RiveRenderer into `target` and flushes;
+ * RIVE_HIP_ERROR_INVALID_ARGUMENT on an unknown opcode, a truncated payload,
+ * a blob out of range or a handle naming no resource. */
+int rive_hip_render_commands(rive_hip_context* ctx, const rive_cmd_buffer* buffer, rive_hip_target* target);
+
 #ifdef __cplusplus
 }
 #endif

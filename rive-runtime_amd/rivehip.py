@@ -135,7 +135,8 @@ EXPORTED = [
     "rive_hip_count_draws", "rive_hip_render_stream_ex", "rive_hip_last_frame", "rive_hip_flush_frame",
     "rive_scene_generate", "rive_scene_free", "rive_scene_info", "rive_scene_node",
     "rive_scene_world_transforms", "rive_scene_stream", "rive_ease_cubic", "rive_scene_request_bytes",
-    "rive_hip_serve",
+    "rive_hip_serve", "rive_cmd_record_stream", "rive_cmd_free", "rive_cmd_bytes", "rive_cmd_from_bytes",
+    "rive_hip_render_commands",
 ]
 
 _lib = None
@@ -201,6 +202,14 @@ def lib():
     L.rive_hip_serve.argtypes = [vp, ctypes.POINTER(vp), u32, ctypes.POINTER(Request), u32,
                                  ctypes.POINTER(ServeConfig), ctypes.POINTER(RequestResult),
                                  ctypes.POINTER(ctypes.c_void_p)]
+    L.rive_cmd_record_stream.argtypes = [ctypes.POINTER(Stream), ctypes.POINTER(vp)]
+    L.rive_cmd_free.argtypes = [vp]
+    L.rive_cmd_free.restype = None
+    L.rive_cmd_bytes.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(vp),
+                                 ctypes.POINTER(ctypes.c_uint64)]
+    L.rive_cmd_from_bytes.argtypes = [vp, ctypes.c_uint64, vp, ctypes.c_uint64, u32, u32, u32, u32,
+                                      ctypes.POINTER(vp)]
+    L.rive_hip_render_commands.argtypes = [vp, vp, vp]
     _lib = L
     return L
 
@@ -295,6 +304,64 @@ class Scene:
 
     def request_bytes(self, width, height):
         return int(self.L.rive_scene_request_bytes(self.h, width, height))
+
+
+class CommandBuffer:
+    """A deferred 2D command stream (rive_cmd_*).  Files written by save():
+    b"RIVECMD1", then u32 width, height, clear colour, dither, u64 command
+    bytes, u64 blob bytes, then the command and blob bytes."""
+
+    MAGIC = b"RIVECMD1"
+
+    def __init__(self, handle, width, height, clear_color, dither):
+        self.L = lib()
+        self.h = handle
+        self.width, self.height, self.clear_color, self.dither = width, height, clear_color, dither
+
+    @classmethod
+    def record(cls, stream):
+        h = ctypes.c_void_p()
+        check(lib().rive_cmd_record_stream(stream, ctypes.byref(h)))
+        st = stream.contents
+        return cls(h, st.width, st.height, st.clear_color, st.dither)
+
+    @classmethod
+    def from_bytes(cls, commands, blobs, width, height, clear_color=0, dither=1):
+        h = ctypes.c_void_p()
+        cb = ctypes.create_string_buffer(bytes(commands), max(len(commands), 1))
+        bb = ctypes.create_string_buffer(bytes(blobs), max(len(blobs), 1))
+        check(lib().rive_cmd_from_bytes(cb, len(commands), bb, len(blobs), width, height, clear_color, dither,
+                                        ctypes.byref(h)))
+        return cls(h, width, height, clear_color, dither)
+
+    def bytes(self):
+        """(command bytes, blob bytes)."""
+        c, b = ctypes.c_void_p(), ctypes.c_void_p()
+        nc, nb = ctypes.c_uint64(), ctypes.c_uint64()
+        check(self.L.rive_cmd_bytes(self.h, ctypes.byref(c), ctypes.byref(nc), ctypes.byref(b), ctypes.byref(nb)))
+        return (ctypes.string_at(c, nc.value) if nc.value else b""), (ctypes.string_at(b, nb.value) if nb.value else b"")
+
+    def save(self, path):
+        import struct
+        c, b = self.bytes()
+        with open(path, "wb") as f:
+            f.write(self.MAGIC + struct.pack("<4I2Q", self.width, self.height, self.clear_color, self.dither,
+                                             len(c), len(b)) + c + b)
+
+    @classmethod
+    def load(cls, path):
+        import struct
+        data = open(path, "rb").read()
+        if data[:8] != cls.MAGIC:
+            raise ValueError("not a command-stream file")
+        w, h, clear, dither, nc, nb = struct.unpack_from("<4I2Q", data, 8)
+        off = 8 + struct.calcsize("<4I2Q")
+        return cls.from_bytes(data[off:off + nc], data[off + nc:off + nc + nb], w, h, clear, dither)
+
+    def close(self):
+        if self.h:
+            self.L.rive_cmd_free(self.h)
+            self.h = ctypes.c_void_p()
 
 
 def ease_cubic(x1, y1, x2, y2, f):
@@ -434,6 +501,10 @@ class Context:
         desc.grad_complex_rows = stats.grad_complex_rows
         desc.first_path_tess_location = stats.first_path_tess_location
         self.flush(desc)
+
+    def render_commands(self, cmd, target):
+        """Replays a CommandBuffer into target (rive_hip_render_commands)."""
+        check(self.L.rive_hip_render_commands(self.h, cmd.h, target))
 
     def serve(self, scenes, requests, hbm_budget_bytes, max_batch, pixels=None):
         """rive_hip_serve: requests is a list of (scene index, time, width,
