@@ -493,6 +493,21 @@ uint32_t rive_stream_contour_bound(const rive_stream* stream);
  * arrays until rive_stream_free. */
 int rive_stream_generate_batch(uint64_t seed, uint32_t n, rive_stream** out);
 
+/* A frame captured in a HIP graph: the frame's flushes (the descriptors of
+ * rive_hip_flush_frame, whose records must stay as they are) run once
+ * eagerly, recording every device total the host reads to size buffers and
+ * launches; the same host logic then runs again under stream capture,
+ * reading the recorded totals, so the graph holds every kernel and copy of
+ * the frame and no host readback.  Launches are asynchronous on the
+ * context's stream (rive_hip_sync waits).  No counterpart in the
+ * reference: the batched path's frame capture.  Patch-batch flushes are
+ * unsupported. */
+typedef struct rive_hip_frame_graph rive_hip_frame_graph;
+int rive_hip_frame_graph_create(rive_hip_context* ctx, const rive_hip_flush_desc* descs, uint32_t n,
+                                rive_hip_frame_graph** out);
+int rive_hip_frame_graph_launch(rive_hip_frame_graph* graph);
+void rive_hip_frame_graph_destroy(rive_hip_frame_graph* graph);
+
 /* ------------------------------------------------------------------------ */
 /* Animated scenes and the serving loop.                                    */
 /* ------------------------------------------------------------------------ */

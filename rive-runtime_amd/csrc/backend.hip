@@ -16,6 +16,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <memory>
 #include <vector>
 
 using namespace rivehip;
@@ -151,6 +152,21 @@ struct rive_hip_context
     // The descriptors of the last frame's logical flushes (a frame starts at
     // a flush whose records start at offset 0), with the image and canvas
     // tables they point at copied here.
+    // Host reads of device totals during a flush.  Live: copy and wait.
+    // Record: the same, keeping every value read on the tape.  Replay: no
+    // device access, the values come off the tape in the same order (a frame
+    // graph's capture re-runs the host logic of a recorded frame this way).
+    enum class Reads
+    {
+        live,
+        record,
+        replay
+    } reads = Reads::live;
+    std::vector<uint8_t> tape;
+    size_t tapePos = 0;
+    std::vector<std::pair<void*, size_t>> pendingReads;
+    bool capturing = false; // inside hipStreamBeginCapture: no allocation, no sync
+    std::vector<void*> captureHeld; // pinned copies of tables the captured copies read
     std::vector<rive_hip_flush_desc> frameDescs;
     std::vector<std::vector<rive_hip_image*>> frameImages;
     std::vector<std::vector<rive_hip_canvas>> frameCanvases;
@@ -203,6 +219,8 @@ int ensure(rive_hip_context* ctx, DevBuf& b, size_t bytes)
     }
     if (bytes <= b.cap)
         return RIVE_HIP_OK;
+    if (ctx->capturing)
+        return fail(RIVE_HIP_ERROR_DEVICE, "a captured frame needs a buffer its recording did not size");
     if (b.ptr != nullptr)
     {
         hipError_t e = hipFree(b.ptr);
@@ -597,10 +615,104 @@ int grow_keep(rive_hip_context* ctx, DevBuf& b, size_t keep, size_t bytes)
     return RIVE_HIP_OK;
 }
 
+// Queues a read of device bytes into host memory; settle() completes the
+// queued reads (see rive_hip_context::reads).
+int fetch(rive_hip_context* ctx, void* hostDst, const void* devSrc, size_t bytes)
+{
+    if (ctx->reads != rive_hip_context::Reads::replay)
+        HIP_TRY(hipMemcpyAsync(hostDst, devSrc, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->pendingReads.emplace_back(hostDst, bytes);
+    return RIVE_HIP_OK;
+}
+
+int settle(rive_hip_context* ctx)
+{
+    if (ctx->reads == rive_hip_context::Reads::replay)
+    {
+        for (const auto& r : ctx->pendingReads)
+        {
+            if (ctx->tapePos + r.second > ctx->tape.size())
+            {
+                ctx->pendingReads.clear();
+                return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "frame replay read past its recording");
+            }
+            memcpy(r.first, ctx->tape.data() + ctx->tapePos, r.second);
+            ctx->tapePos += r.second;
+        }
+        ctx->pendingReads.clear();
+        return RIVE_HIP_OK;
+    }
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess)
+    {
+        ctx->pendingReads.clear();
+        return fail(RIVE_HIP_ERROR_DEVICE, std::string("stream sync: ") + hipGetErrorString(e));
+    }
+    if (ctx->reads == rive_hip_context::Reads::record)
+        for (const auto& r : ctx->pendingReads)
+        {
+            const uint8_t* b = static_cast<const uint8_t*>(r.first);
+            ctx->tape.insert(ctx->tape.end(), b, b + r.second);
+        }
+    ctx->pendingReads.clear();
+    return RIVE_HIP_OK;
+}
+
+// Fills `bytes` (a multiple of 4) of device memory with a repeated byte: a
+// kernel rather than hipMemsetAsync, so that a captured frame holds only
+// kernel and copy nodes.
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t value, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+int clear(rive_hip_context* ctx, void* ptr, uint8_t byte, size_t bytes)
+{
+    if (bytes == 0)
+        return RIVE_HIP_OK;
+    if ((bytes & 3) != 0 || (reinterpret_cast<uintptr_t>(ptr) & 3) != 0)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "clear of an unaligned range");
+    const size_t n = bytes / 4;
+    const uint32_t v = 0x01010101u * byte;
+    const uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 4096);
+    fill_u32_kernel<<<blocks, 256, 0, ctx->stream>>>(static_cast<uint32_t*>(ptr), v, n);
+    HIP_TRY(hipGetLastError());
+    return RIVE_HIP_OK;
+}
+
+// Copies a host table (canvas or image descriptors) to the device.  While a
+// frame is captured, the copy reads a pinned snapshot the graph keeps, since
+// the staging table is rewritten by every flush.
+int upload_table(rive_hip_context* ctx, void* devDst, const void* hostSrc, size_t bytes)
+{
+    if (bytes == 0)
+        return RIVE_HIP_OK;
+    const void* src = hostSrc;
+    if (ctx->capturing)
+    {
+        void* held = nullptr;
+        HIP_TRY(hipHostMalloc(&held, bytes, hipHostMallocDefault));
+        memcpy(held, hostSrc, bytes);
+        ctx->captureHeld.push_back(held);
+        src = held;
+    }
+    HIP_TRY(hipMemcpyAsync(devDst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return RIVE_HIP_OK;
+}
+
+// Records a pass-timing event, except while a frame is being captured into a
+// graph (its passes are timed on eager flushes).
+int mark(rive_hip_context* ctx, hipEvent_t ev)
+{
+    if (!ctx->capturing)
+        HIP_TRY(hipEventRecord(ev, ctx->stream));
+    return RIVE_HIP_OK;
+}
+
 int read_scalar(rive_hip_context* ctx, const uint32_t* devSlot, uint32_t* value)
 {
-    HIP_TRY(hipMemcpyAsync(ctx->hostScalars, devSlot, 4, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    RIVE_TRY(fetch(ctx, ctx->hostScalars, devSlot, 4));
+    RIVE_TRY(settle(ctx));
     *value = ctx->hostScalars[0];
     return RIVE_HIP_OK;
 }
@@ -794,8 +906,7 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
         maxTilesY = std::max(maxTilesY, cd.tilesY);
     }
     RIVE_TRY(ensure(ctx, ctx->canvasTable, sizeof(CanvasDev) * std::max<uint32_t>(canvasCount, 1)));
-    HIP_TRY(hipMemcpyAsync(ctx->canvasTable.ptr, ctx->hostCanvases, sizeof(CanvasDev) * canvasCount,
-                           hipMemcpyHostToDevice, ctx->stream));
+    RIVE_TRY(upload_table(ctx, ctx->canvasTable.ptr, ctx->hostCanvases, sizeof(CanvasDev) * canvasCount));
     uint32_t* scal = P<uint32_t>(ctx->scalars);
     const uint32_t pathCount = d->path_count;
     const PathDataV* paths = ctx->in.paths;
@@ -804,22 +915,22 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     const uint32_t gradRows = std::max<uint32_t>(d->grad_simple_rows + d->grad_complex_rows, 1);
 
     // ---- color ramps
-    HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
+    RIVE_TRY(mark(ctx, ctx->ev[0]));
     RIVE_TRY(ensure(ctx, ctx->ramps, (size_t)gradRows * kGradWidth * 4));
     if (d->grad_span_count > 0)
     {
-        HIP_TRY(hipMemsetAsync(ctx->ramps.ptr, 0, (size_t)gradRows * kGradWidth * 4, ctx->stream));
+        RIVE_TRY(clear(ctx, ctx->ramps.ptr, 0, (size_t)gradRows * kGradWidth * 4));
         ramp_kernel<<<d->grad_span_count, 64, 0, ctx->stream>>>(
             ctx->in.gradSpans,
             d->grad_span_count, gradRows, P<uint32_t>(ctx->ramps));
     }
     ctx->lastGradRows = gradRows;
-    HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
+    RIVE_TRY(mark(ctx, ctx->ev[2]));
 
     // ---- tessellation: texel counts -> scan -> one thread per texel
     const uint32_t texels = tessRows * kTessTextureWidth;
     RIVE_TRY(ensure(ctx, ctx->tess, (size_t)texels * sizeof(TessTexel)));
-    HIP_TRY(hipMemsetAsync(ctx->tess.ptr, 0, (size_t)texels * sizeof(TessTexel), ctx->stream));
+    RIVE_TRY(clear(ctx, ctx->tess.ptr, 0, (size_t)texels * sizeof(TessTexel)));
     if (spanCount > 0)
     {
         RIVE_TRY(ensure(ctx, ctx->spanTexelCounts, (size_t)spanCount * 4));
@@ -848,11 +959,11 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
         ctx->stats.draw_count = drawCount;
         ctx->stats.prim_count = totalPrims;
     }
-    HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
+    RIVE_TRY(mark(ctx, ctx->ev[3]));
 
     // ---- expansion + binning
     RIVE_TRY(ensure(ctx, ctx->pathCanvas, (size_t)(pathCount + 1) * 4));
-    HIP_TRY(hipMemsetAsync(ctx->pathCanvas.ptr, 0xff, (size_t)(pathCount + 1) * 4, ctx->stream));
+    RIVE_TRY(clear(ctx, ctx->pathCanvas.ptr, 0xff, (size_t)(pathCount + 1) * 4));
     path_canvas_kernel<<<canvasCount, 256, 0, ctx->stream>>>(P<CanvasDev>(ctx->canvasTable),
                                                              P<uint32_t>(ctx->pathCanvas));
     RIVE_TRY(ensure(ctx, ctx->cells, (size_t)(pathCount + 1) * sizeof(PathCells)));
@@ -866,7 +977,7 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     RIVE_TRY(ensure(ctx, ctx->specialList, (size_t)std::max<uint32_t>(totalPrims, 1) * 4));
     RIVE_TRY(ensure(ctx, ctx->farList, (size_t)std::max<uint32_t>(totalPrims, 1) * 4));
     RIVE_TRY(ensure(ctx, ctx->drawPieces, (size_t)(drawCount + 1) * 4));
-    HIP_TRY(hipMemsetAsync(ctx->drawPieces.ptr, 0, (size_t)(drawCount + 1) * 4, ctx->stream));
+    RIVE_TRY(clear(ctx, ctx->drawPieces.ptr, 0, (size_t)(drawCount + 1) * 4));
     if (totalPrims > 0)
     {
         expand_kernel<<<(totalPrims + 255) / 256, 256, 0, ctx->stream>>>(
@@ -885,9 +996,9 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     HIP_TRY(hipGetLastError());
     // Cell total, the count of texels naming no ContourData record, and the
     // primitives leaving the guard box with their piece total.
-    HIP_TRY(hipMemcpyAsync(ctx->hostScalars, scal + 1, 20, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->hostScalars + 5, scal + kFarSlot, 8, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    RIVE_TRY(fetch(ctx, ctx->hostScalars, scal + 1, 20));
+    RIVE_TRY(fetch(ctx, ctx->hostScalars + 5, scal + kFarSlot, 8));
+    RIVE_TRY(settle(ctx));
     const uint32_t totalCells = ctx->hostScalars[0];
     const uint32_t farPrims = ctx->hostScalars[5], pieces = ctx->hostScalars[6];
     if (ctx->hostScalars[4] != 0)
@@ -910,7 +1021,7 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
         RIVE_TRY(ensure(ctx, ctx->drawPieceOffset, (size_t)(drawCount + 1) * 4));
         RIVE_TRY(ensure(ctx, ctx->pieceCursor, (size_t)(drawCount + 1) * 4));
         RIVE_TRY(ensure(ctx, ctx->drawTessAll, (size_t)drawCount * 2 * sizeof(DrawTess)));
-        HIP_TRY(hipMemsetAsync(ctx->pieceCursor.ptr, 0, (size_t)(drawCount + 1) * 4, ctx->stream));
+        RIVE_TRY(clear(ctx, ctx->pieceCursor.ptr, 0, (size_t)(drawCount + 1) * 4));
         rc = device_scan(ctx, P<uint32_t>(ctx->drawPieces), drawCount, P<uint32_t>(ctx->drawPieceOffset),
                          scal + kFarSlot + 2);
         if (rc)
@@ -929,11 +1040,11 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
     RIVE_TRY(ensure(ctx, ctx->cellOffset, (size_t)(totalCells + 1) * 4));
     RIVE_TRY(ensure(ctx, ctx->cellCursor, (size_t)(totalCells + 1) * 4));
     RIVE_TRY(ensure(ctx, ctx->cellBackdrop, (size_t)(totalCells + 1) * 4));
-    HIP_TRY(hipMemsetAsync(ctx->cellCount.ptr, 0, (size_t)(totalCells + 1) * 4, ctx->stream));
-    HIP_TRY(hipMemsetAsync(ctx->cellCursor.ptr, 0, (size_t)(totalCells + 1) * 4, ctx->stream));
-    HIP_TRY(hipMemsetAsync(ctx->cellBackdrop.ptr, 0, (size_t)(totalCells + 1) * 4, ctx->stream));
+    RIVE_TRY(clear(ctx, ctx->cellCount.ptr, 0, (size_t)(totalCells + 1) * 4));
+    RIVE_TRY(clear(ctx, ctx->cellCursor.ptr, 0, (size_t)(totalCells + 1) * 4));
+    RIVE_TRY(clear(ctx, ctx->cellBackdrop.ptr, 0, (size_t)(totalCells + 1) * 4));
     RIVE_TRY(ensure(ctx, ctx->pathDraws, (size_t)(pathCount + 1) * 4));
-    HIP_TRY(hipMemsetAsync(ctx->pathDraws.ptr, 0, (size_t)(pathCount + 1) * 4, ctx->stream));
+    RIVE_TRY(clear(ctx, ctx->pathDraws.ptr, 0, (size_t)(pathCount + 1) * 4));
     if (totalPrims > 0 && drawCount > 0)
         bin_count_draw_kernel<<<binDrawCount, 256, 0, ctx->stream>>>(
             P<Prim>(ctx->prims), binDraws, P<PathCells>(ctx->cells), P<uint32_t>(ctx->cellCount),
@@ -967,7 +1078,7 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
         paths, paints, aux, P<uint32_t>(ctx->pathImage), pathCount, gradRows, P<PathRaster>(ctx->pathRaster),
         P<PathPaint>(ctx->pathPaint));
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
+    RIVE_TRY(mark(ctx, ctx->ev[4]));
 
     // ---- raster + resolve
     if (tileTotal > 0)
@@ -978,7 +1089,7 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
             P<uint32_t>(ctx->ramps), gradRows, P<ImageDesc>(ctx->imageTable), P<CanvasDev>(ctx->canvasTable),
             canvasCount, d->dither);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
+    RIVE_TRY(mark(ctx, ctx->ev[5]));
     return RIVE_HIP_OK;
 }
 
@@ -1013,11 +1124,11 @@ int derive_patch_draws(rive_hip_context* ctx, const rive_hip_flush_desc* d, cons
     uint32_t* w = P<uint32_t>(ctx->patchWork);
     uint32_t *fwdMin = w, *fwdMax = w + m, *mirMin = w + 2 * m, *mirMax = w + 3 * m;
     uint32_t *flag = w + 4 * m, *drawIdx = w + 5 * m, *cnt = w + 6 * m, *base = w + 7 * m;
-    HIP_TRY(hipMemsetAsync(fwdMin, 0xff, m * 4, ctx->stream));
-    HIP_TRY(hipMemsetAsync(mirMin, 0xff, m * 4, ctx->stream));
-    HIP_TRY(hipMemsetAsync(fwdMax, 0, m * 4, ctx->stream));
-    HIP_TRY(hipMemsetAsync(mirMax, 0, m * 4, ctx->stream));
-    HIP_TRY(hipMemsetAsync(cnt, 0, m * 4, ctx->stream));
+    RIVE_TRY(clear(ctx, fwdMin, 0xff, m * 4));
+    RIVE_TRY(clear(ctx, mirMin, 0xff, m * 4));
+    RIVE_TRY(clear(ctx, fwdMax, 0, m * 4));
+    RIVE_TRY(clear(ctx, mirMax, 0, m * 4));
+    RIVE_TRY(clear(ctx, cnt, 0, m * 4));
     if (spanCount > 0)
         span_ranges_kernel<<<(spanCount + 255) / 256, 256, 0, ctx->stream>>>(
             spans, spanCount, contours, contourCount, np, texels, fwdMin, fwdMax, mirMin, mirMax, scal + 6);
@@ -1029,8 +1140,8 @@ int derive_patch_draws(rive_hip_context* ctx, const rive_hip_flush_desc* d, cons
         d->image_count, P<DrawTess>(ctx->drawTess), cnt, P<uint32_t>(ctx->pathImage), scal + 6);
     RIVE_TRY(device_scan(ctx, cnt, np, base, scal + 3));
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(ctx->hostScalars, scal + 3, 20, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    RIVE_TRY(fetch(ctx, ctx->hostScalars, scal + 3, 20));
+    RIVE_TRY(settle(ctx));
     if (ctx->hostScalars[3] != 0)
         return fail(RIVE_HIP_ERROR_UNSUPPORTED,
                     "spans naming no contour or path, or contour directions the raster-ordering pipeline "
@@ -1234,9 +1345,9 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d, const GeoLay
     }
     RIVE_TRY(ensure(ctx, ctx->scalars, kScalarBytes));
     RIVE_TRY(ensure(ctx, ctx->drawTess, (size_t)std::max<uint32_t>(drawCountIn, 1) * sizeof(DrawTess)));
-    HIP_TRY(hipMemsetAsync(ctx->scalars.ptr, 0, kScalarBytes, ctx->stream));
+    RIVE_TRY(clear(ctx, ctx->scalars.ptr, 0, kScalarBytes));
     if (pre == nullptr) // a frame's flushes have their start and counting events set already
-        HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream)); // flush start
+        RIVE_TRY(mark(ctx, ctx->ev[6])); // flush start
 
     const SpanV* spans;
     uint32_t spanCount;
@@ -1256,7 +1367,7 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d, const GeoLay
         contours = ctx->in.contours;
         tessRows = std::max<uint32_t>((d->tess_vertex_count + kTessTextureWidth - 1) / kTessTextureWidth, 1);
         ctx->lastContourCount = d->contour_count;
-        HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream)); // no counting pass
+        RIVE_TRY(mark(ctx, ctx->ev[1])); // no counting pass
     }
     else if (d->mode == RIVE_HIP_FLUSH_GEOMETRY && pre != nullptr)
     {
@@ -1303,11 +1414,10 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d, const GeoLay
             ctx->hostImages[k] = d->images[k]->desc;
         }
         RIVE_TRY(ensure(ctx, ctx->imageTable, sizeof(ImageDesc) * d->image_count));
-        HIP_TRY(hipMemcpyAsync(ctx->imageTable.ptr, ctx->hostImages, sizeof(ImageDesc) * d->image_count,
-                               hipMemcpyHostToDevice, ctx->stream));
+        RIVE_TRY(upload_table(ctx, ctx->imageTable.ptr, ctx->hostImages, sizeof(ImageDesc) * d->image_count));
     }
     RIVE_TRY(ensure(ctx, ctx->pathImage, (size_t)(d->path_count + 1) * 4));
-    HIP_TRY(hipMemsetAsync(ctx->pathImage.ptr, 0, (size_t)(d->path_count + 1) * 4, ctx->stream));
+    RIVE_TRY(clear(ctx, ctx->pathImage.ptr, 0, (size_t)(d->path_count + 1) * 4));
     uint32_t totalPrims = 0;
     if (!patchMode)
     {
@@ -1316,8 +1426,8 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d, const GeoLay
             d->path_count, tessRows * kTessTextureWidth, d->image_count, P<DrawTess>(ctx->drawTess),
             P<uint32_t>(ctx->pathImage), scal + 3, scal + 4);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(ctx->hostScalars, scal + 3, 8, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        RIVE_TRY(fetch(ctx, ctx->hostScalars, scal + 3, 8));
+        RIVE_TRY(settle(ctx));
         totalPrims = ctx->hostScalars[0];
         if (ctx->hostScalars[1] != 0)
             return fail(RIVE_HIP_ERROR_UNSUPPORTED,
@@ -1373,6 +1483,158 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d, const GeoLay
     return RIVE_HIP_OK;
 }
 } // namespace
+
+struct rive_hip_frame_graph
+{
+    rive_hip_context* ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<void*> held; // pinned table snapshots the graph's copies read
+    // Everything the graph's kernels read or write, except the targets and
+    // images: its own pipeline buffers and a device snapshot of each record
+    // buffer, so eager flushes between launches neither overwrite its
+    // inputs nor reallocate its buffers.
+    std::vector<DevBuf> scratch;
+    void* records[RIVE_HIP_BUFFER_KIND_COUNT] = {};
+};
+
+extern "C" void rive_hip_frame_graph_destroy(rive_hip_frame_graph* g)
+{
+    if (g == nullptr)
+        return;
+    if (g->ctx)
+        (void)hipStreamSynchronize(g->ctx->stream);
+    if (g->exec)
+        (void)hipGraphExecDestroy(g->exec);
+    if (g->graph)
+        (void)hipGraphDestroy(g->graph);
+    for (void* p : g->held)
+        (void)hipHostFree(p);
+    for (DevBuf& b : g->scratch)
+        if (b.ptr && !b.carved)
+            (void)hipFree(b.ptr);
+    for (void* p : g->records)
+        if (p)
+            (void)hipFree(p);
+    delete g;
+}
+
+extern "C" int rive_hip_frame_graph_create(rive_hip_context* ctx, const rive_hip_flush_desc* descs, uint32_t n,
+                                           rive_hip_frame_graph** out)
+{
+    if (ctx == nullptr || descs == nullptr || n == 0 || out == nullptr)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null argument or empty frame");
+    for (uint32_t k = 0; k < n; ++k)
+        if (descs[k].patch_batch_count != 0)
+            return fail(RIVE_HIP_ERROR_UNSUPPORTED, "frame graphs take no patch-batch flushes");
+    for (int k = 0; k < RIVE_HIP_BUFFER_KIND_COUNT; ++k)
+        if (ctx->bufs[k].mapped)
+            return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "a record buffer is mapped");
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto g = std::make_unique<rive_hip_frame_graph>();
+    g->ctx = ctx;
+    // The graph's own buffers: the context's pipeline buffers are set aside
+    // while the frame is recorded and captured (the flushes allocate fresh
+    // ones, which the graph keeps), and the record buffers read through
+    // device snapshots.
+    DevBuf* all[kDevBufCount];
+    devbufs(ctx, all);
+    DevBuf saved[kDevBufCount];
+    for (int i = 0; i < kDevBufCount; ++i)
+    {
+        saved[i] = *all[i];
+        *all[i] = DevBuf{};
+    }
+    void* savedDev[RIVE_HIP_BUFFER_KIND_COUNT];
+    for (int k = 0; k < RIVE_HIP_BUFFER_KIND_COUNT; ++k)
+        savedDev[k] = ctx->bufs[k].dev;
+    auto restore = [&]() {
+        for (int i = 0; i < kDevBufCount; ++i)
+        {
+            g->scratch.push_back(*all[i]);
+            *all[i] = saved[i];
+        }
+        for (int k = 0; k < RIVE_HIP_BUFFER_KIND_COUNT; ++k)
+            ctx->bufs[k].dev = savedDev[k];
+        ctx->lastSpans = ctx->lastContours = nullptr;
+        ctx->in = rive_hip_context::FlushInputs{};
+    };
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < RIVE_HIP_BUFFER_KIND_COUNT && e == hipSuccess; ++k)
+        if (ctx->bufs[k].dev != nullptr && ctx->bufs[k].size > 0)
+        {
+            e = hipMalloc(&g->records[k], ctx->bufs[k].size);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(g->records[k], ctx->bufs[k].dev, ctx->bufs[k].size, hipMemcpyDeviceToDevice,
+                                   ctx->stream);
+            if (e == hipSuccess)
+                ctx->bufs[k].dev = g->records[k];
+        }
+    if (e != hipSuccess)
+    {
+        restore();
+        rive_hip_frame_graph_destroy(g.release());
+        return fail(RIVE_HIP_ERROR_OUT_OF_MEMORY, std::string("frame graph records: ") + hipGetErrorString(e));
+    }
+    // 1. an eager frame that sizes every buffer and records the totals it reads
+    ctx->pendingReads.clear();
+    ctx->tape.clear();
+    ctx->reads = rive_hip_context::Reads::record;
+    int rc = rive_hip_flush_frame(ctx, descs, n);
+    ctx->reads = rive_hip_context::Reads::live;
+    if (rc == RIVE_HIP_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
+        rc = fail(RIVE_HIP_ERROR_DEVICE, "frame graph: recording flush failed");
+    if (rc != RIVE_HIP_OK)
+    {
+        restore();
+        rive_hip_frame_graph_destroy(g.release());
+        return rc;
+    }
+    // 2. the same host logic again, reading the recorded totals, with every
+    //    kernel and copy captured
+    ctx->reads = rive_hip_context::Reads::replay;
+    ctx->tapePos = 0;
+    ctx->capturing = true;
+    ctx->captureHeld.clear();
+    e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed);
+    if (e == hipSuccess)
+    {
+        rc = rive_hip_flush_frame(ctx, descs, n);
+        e = hipStreamEndCapture(ctx->stream, &g->graph);
+    }
+    ctx->reads = rive_hip_context::Reads::live;
+    ctx->capturing = false;
+    g->held.swap(ctx->captureHeld);
+    const bool consumed = ctx->tapePos == ctx->tape.size();
+    ctx->pendingReads.clear();
+    restore();
+    if (e != hipSuccess)
+    {
+        rive_hip_frame_graph_destroy(g.release());
+        return fail(RIVE_HIP_ERROR_DEVICE, std::string("frame capture: ") + hipGetErrorString(e));
+    }
+    if (rc != RIVE_HIP_OK || !consumed)
+    {
+        rive_hip_frame_graph_destroy(g.release());
+        return rc != RIVE_HIP_OK ? rc : fail(RIVE_HIP_ERROR_DEVICE, "frame capture diverged from its recording");
+    }
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess)
+    {
+        rive_hip_frame_graph_destroy(g.release());
+        return fail(RIVE_HIP_ERROR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    *out = g.release();
+    return RIVE_HIP_OK;
+}
+
+extern "C" int rive_hip_frame_graph_launch(rive_hip_frame_graph* g)
+{
+    if (g == nullptr || g->exec == nullptr)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null frame graph");
+    HIP_TRY(hipGraphLaunch(g->exec, g->ctx->stream));
+    return RIVE_HIP_OK;
+}
 
 namespace
 {
@@ -1439,17 +1701,17 @@ extern "C" int rive_hip_flush_frame(rive_hip_context* ctx, const rive_hip_flush_
         return RIVE_HIP_OK;
     }
     RIVE_TRY(use_event_set(ctx, 0));
-    HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream)); // frame start
+    RIVE_TRY(mark(ctx, ctx->ev[6])); // frame start
     std::vector<GeoLayout> layouts(n);
     RIVE_TRY(geometry_counting_frame(ctx, descs, n, layouts.data()));
-    HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream)); // the frame's counting done
+    RIVE_TRY(mark(ctx, ctx->ev[1])); // the frame's counting done
     for (uint32_t k = 0; k < n; ++k)
     {
         RIVE_TRY(use_event_set(ctx, k));
         if (k > 0)
         {
-            HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream));
-            HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
+            RIVE_TRY(mark(ctx, ctx->ev[6]));
+            RIVE_TRY(mark(ctx, ctx->ev[1]));
         }
         ctx->evSetsUsed = k + 1;
         RIVE_TRY(flush_impl(ctx, &descs[k], &layouts[k]));

@@ -1173,8 +1173,8 @@ int geometry_counting(rive_hip_context* ctx, const rive_hip_flush_desc* d, const
     rc = device_scan(ctx, recordCounts, n, recordOffsets, scal + 10);
     if (rc)
         return rc;
-    HIP_TRY(hipMemcpyAsync(ctx->hostScalars, scal + 8, 20, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    RIVE_TRY(fetch(ctx, ctx->hostScalars, scal + 8, 20));
+    RIVE_TRY(settle(ctx));
     if (ctx->hostScalars[4] != 0)
         return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT,
                     "draw record with verb/point ranges outside the VERB/POINT buffers");
@@ -1227,7 +1227,7 @@ int geometry_counting(rive_hip_context* ctx, const rive_hip_flush_desc* d, const
             P<CubicRec>(ctx->geoRecords), recordTotal, P<uint32_t>(ctx->geoSpanOffsets), oneFlush,
             P<SpanV>(ctx->geoSpans));
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
+    RIVE_TRY(mark(ctx, ctx->ev[1]));
     const uint32_t texelsUsed = firstLocation + tessTotal + 1u;
     *tessRows = std::max<uint32_t>((texelsUsed + kTessTextureWidth - 1) / kTessTextureWidth, 1);
     *spans = P<SpanV>(ctx->geoSpans);
@@ -1282,7 +1282,7 @@ int geometry_counting_frame(rive_hip_context* ctx, const rive_hip_flush_desc* de
     const uint32_t firstLocation = descs[0].first_path_tess_location ? descs[0].first_path_tess_location : kPatchSpan;
     RIVE_TRY(ensure(ctx, ctx->scalars, kScalarBytes));
     uint32_t* scal = P<uint32_t>(ctx->scalars);
-    HIP_TRY(hipMemsetAsync(ctx->scalars.ptr, 0, kScalarBytes, ctx->stream));
+    RIVE_TRY(clear(ctx, ctx->scalars.ptr, 0, kScalarBytes));
     // 5 arrays of n+1 (tess, contour, record counts; tess, contour locals) +
     // 3 scans of n+1 + bounds values.
     RIVE_TRY(ensure(ctx, ctx->geoCounts, ((size_t)8 * (n + 1) + 4 * (kMaxFrameFlushes + 1) + 16) * 4));
@@ -1310,9 +1310,9 @@ int geometry_counting_frame(rive_hip_context* ctx, const rive_hip_flush_desc* de
     // One readback: the totals, the bounds and the range check flag.
     const uint32_t m = nf + 1;
     std::vector<uint32_t> hv(3 * m + 8);
-    HIP_TRY(hipMemcpyAsync(hv.data(), boundVals, (size_t)3 * m * 4, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(hv.data() + 3 * m, scal + 8, 20, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    RIVE_TRY(fetch(ctx, hv.data(), boundVals, (size_t)3 * m * 4));
+    RIVE_TRY(fetch(ctx, hv.data() + 3 * m, scal + 8, 20));
+    RIVE_TRY(settle(ctx));
     if (hv[3 * m + 4] != 0)
         return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT,
                     "draw record with verb/point ranges outside the VERB/POINT buffers");
@@ -1364,9 +1364,9 @@ int geometry_counting_frame(rive_hip_context* ctx, const rive_hip_flush_desc* de
                                                                      P<uint32_t>(ctx->geoSpanOffsets), nullptr,
                                                                      boundVals);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(hv.data(), boundVals, (size_t)m * 4, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(hv.data() + 3 * m, scal + 11, 12, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    RIVE_TRY(fetch(ctx, hv.data(), boundVals, (size_t)m * 4));
+    RIVE_TRY(fetch(ctx, hv.data() + 3 * m, scal + 11, 12));
+    RIVE_TRY(settle(ctx));
     if (hv[3 * m + 2] != 0) // scal[13]: emission overflow
         return fail(RIVE_HIP_ERROR_DEVICE, "curve emission disagreed with the counting pass");
     if (hv[3 * m] == kScanSaturated)
@@ -1419,16 +1419,16 @@ extern "C" int rive_hip_count_draws(rive_hip_context* ctx, uint32_t n, uint32_t*
     uint32_t* contourCounts = tessCounts + (n + 1);
     uint32_t* recordCounts = contourCounts + (n + 1);
     uint32_t* scal = P<uint32_t>(ctx->scalars);
-    HIP_TRY(hipMemsetAsync(scal + 12, 0, 4, ctx->stream));
+    RIVE_TRY(clear(ctx, scal + 12, 0, 4));
     count_kernel<<<(n + 31) / 32, 32, 0, ctx->stream>>>(
         reinterpret_cast<const rive_hip_draw*>(ctx->bufs[RIVE_HIP_BUFFER_DRAW].dev), n, verbs,
         ctx->bufs[RIVE_HIP_BUFFER_VERB].size, points, ctx->bufs[RIVE_HIP_BUFFER_POINT].size / sizeof(V2d),
         scal + 12, tessCounts, contourCounts, recordCounts);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(ctx->hostScalars, scal + 12, 4, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(tessOut, tessCounts, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(contourOut, contourCounts, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    RIVE_TRY(fetch(ctx, ctx->hostScalars, scal + 12, 4));
+    RIVE_TRY(fetch(ctx, tessOut, tessCounts, (size_t)n * 4));
+    RIVE_TRY(fetch(ctx, contourOut, contourCounts, (size_t)n * 4));
+    RIVE_TRY(settle(ctx));
     if (ctx->hostScalars[0] != 0)
         return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT,
                     "draw record with verb/point ranges outside the VERB/POINT buffers");

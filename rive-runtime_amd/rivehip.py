@@ -136,7 +136,8 @@ EXPORTED = [
     "rive_scene_generate", "rive_scene_free", "rive_scene_info", "rive_scene_node",
     "rive_scene_world_transforms", "rive_scene_stream", "rive_ease_cubic", "rive_scene_request_bytes",
     "rive_hip_serve", "rive_cmd_record_stream", "rive_cmd_free", "rive_cmd_bytes", "rive_cmd_from_bytes",
-    "rive_hip_render_commands",
+    "rive_hip_render_commands", "rive_hip_frame_graph_create", "rive_hip_frame_graph_launch",
+    "rive_hip_frame_graph_destroy",
 ]
 
 _lib = None
@@ -210,6 +211,10 @@ def lib():
     L.rive_cmd_from_bytes.argtypes = [vp, ctypes.c_uint64, vp, ctypes.c_uint64, u32, u32, u32, u32,
                                       ctypes.POINTER(vp)]
     L.rive_hip_render_commands.argtypes = [vp, vp, vp]
+    L.rive_hip_frame_graph_create.argtypes = [vp, ctypes.POINTER(FlushDesc), u32, ctypes.POINTER(vp)]
+    L.rive_hip_frame_graph_launch.argtypes = [vp]
+    L.rive_hip_frame_graph_destroy.argtypes = [vp]
+    L.rive_hip_frame_graph_destroy.restype = None
     _lib = L
     return L
 
@@ -501,6 +506,23 @@ class Context:
         desc.grad_complex_rows = stats.grad_complex_rows
         desc.first_path_tess_location = stats.first_path_tess_location
         self.flush(desc)
+
+    def frame_graph(self, descs, target=None):
+        """rive_hip_frame_graph_create over a frame's descriptors (optionally
+        retargeted); returns a handle for frame_graph_launch / _destroy."""
+        arr = (FlushDesc * len(descs))(*descs)
+        if target is not None:
+            for d in arr:
+                d.target = target
+        g = ctypes.c_void_p()
+        check(self.L.rive_hip_frame_graph_create(self.h, arr, len(descs), ctypes.byref(g)))
+        return g
+
+    def frame_graph_launch(self, g):
+        check(self.L.rive_hip_frame_graph_launch(g))
+
+    def frame_graph_destroy(self, g):
+        self.L.rive_hip_frame_graph_destroy(g)
 
     def render_commands(self, cmd, target):
         """Replays a CommandBuffer into target (rive_hip_render_commands)."""
