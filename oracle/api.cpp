@@ -200,4 +200,51 @@ void oracle_blend_coeffs(const float* src3, const float* dstPremul4, uint32_t mo
     advanced_blend_coeffs(src3, dstPremul4, mode, out3);
 }
 uint32_t oracle_shader_blend_mode(uint32_t riveBlendMode) { return shader_blend_mode(riveBlendMode); }
+
+// ---- feathering ----
+void oracle_feather_tables(uint16_t* gauss512, uint16_t* inverse512)
+{
+    memcpy(gauss512, feather_table_f16(), GAUSSIAN_TABLE_SIZE * 2);
+    memcpy(inverse512, inverse_feather_table_f16(), GAUSSIAN_TABLE_SIZE * 2);
+}
+float oracle_table_lookup(const uint16_t* table512, float x) { return table_lookup(table512, x); }
+float oracle_feather(float x) { return feather(x); }
+float oracle_inverse_feather(float x) { return inverse_feather(x); }
+float oracle_eval_feathered_fill(const float* c4) { return eval_feathered_fill(c4); }
+float oracle_eval_feathered_stroke(const float* c4) { return eval_feathered_stroke(c4); }
+void oracle_pack_feathered_fill(float cornerTheta, float spokeX, float spokeY, float outset, float* out4)
+{
+    pack_feathered_fill_coverages(cornerTheta, V2{spokeX, spokeY}, outset, out4);
+}
+float oracle_cubic_max_height(const float* pts8, float* outT)
+{
+    V2 p[4];
+    load_pts(pts8, p);
+    return find_cubic_max_height(p[0], p[1], p[2], p[3], outT);
+}
+float oracle_cubic_local_curvature(const float* pts8, float T, float spread)
+{
+    V2 p[4];
+    load_pts(pts8, p);
+    return measure_cubic_local_curvature(p[0], p[1], p[2], p[3], T, spread);
+}
+// Softens one path for feathering; returns the number of output points and
+// the number of verbs in *verbCountOut, or the required sizes when the
+// output buffers are too small.
+uint32_t oracle_soften_path(const uint8_t* verbs, uint32_t verbCount, const float* pts, uint32_t ptCount,
+                            float feather, float matrixMaxScale, uint8_t* outVerbs, uint32_t verbCap,
+                            float* outPts, uint32_t ptCap, uint32_t* verbCountOut)
+{
+    std::vector<uint8_t> v;
+    std::vector<V2> p;
+    soften_path_for_feathering(verbs, verbCount, reinterpret_cast<const V2*>(pts), ptCount, feather,
+                               matrixMaxScale, &v, &p);
+    *verbCountOut = (uint32_t)v.size();
+    if (v.size() <= verbCap && p.size() <= ptCap)
+    {
+        memcpy(outVerbs, v.data(), v.size());
+        memcpy(outPts, p.data(), p.size() * sizeof(V2));
+    }
+    return (uint32_t)p.size();
+}
 }

@@ -455,6 +455,14 @@ uint32_t parametric_count_from_pow4(float n4)
     n = sclamp(n, 1, (float)kMaxParametricSegments);
     return (uint32_t)n;
 }
+// feather_join_segment_count (draw.cpp:330-340): half a turn of polar
+// segments plus the helper vertices.
+uint32_t feather_join_segment_count(float polarSegmentsPerRadian)
+{
+    uint32_t n = (uint32_t)ceilf(polarSegmentsPerRadian * kPI) + (FEATHER_JOIN_HELPER_VERTEX_COUNT + 1);
+    return std::max(n, FEATHER_JOIN_MIN_SEGMENT_COUNT);
+}
+
 uint32_t polar_count(V2 t0, V2 t1, float polarSegmentsPerRadian)
 {
     // draw.cpp:1219-1238
@@ -542,7 +550,9 @@ PathTess tessellate_path_records(const PathDrawDesc& d, uint32_t firstContourID,
 {
     PathTess result;
     const bool isStroke = d.strokeRadius > 0;
-    const bool isStrokeOrFeather = isStroke; // feathers are not restated yet
+    // Feathered fills are tessellated like strokes, with feather joins
+    // (draw.cpp:776-806).
+    const bool isStrokeOrFeather = isStroke || d.featherRadius > 0;
     const uint8_t* verbs = d.verbs;
     const V2* P = d.pts;
     const uint32_t nVerbs = d.verbCount;
@@ -554,10 +564,26 @@ PathTess tessellate_path_records(const PathDrawDesc& d, uint32_t firstContourID,
     {
         strokeMatrixMaxScale = find_max_scale(d.matrix);
         float r_ = 0;
-        r_ += d.strokeRadius * strokeMatrixMaxScale;
+        if (d.featherRadius != 0)
+        {
+            r_ = d.featherRadius * strokeMatrixMaxScale;
+            // The polar density needed at FEATHER_POLAR_SEGMENT_MIN_ANGLE
+            // (draw.cpp:784-802).
+            const float FEATHER_MAX_SCREEN_SPACE_RADIUS =
+                1 / (kPolarPrecision * (1 - COS_FEATHER_POLAR_SEGMENT_MIN_ANGLE_OVER_2));
+            r_ = std::min(r_, FEATHER_MAX_SCREEN_SPACE_RADIUS);
+        }
+        if (isStroke)
+            r_ += d.strokeRadius * strokeMatrixMaxScale;
         polarSegmentsPerRadian = calc_polar_segments_per_radian(r_);
     }
-    const float parametricPrecision = kParametricPrecision;
+    float parametricPrecision = kParametricPrecision;
+    if (d.featherRadius > 1)
+    {
+        // Wide feathers tessellate to a coarser tolerance (draw.cpp:879-889).
+        parametricPrecision = std::min(parametricPrecision * 100.f / (d.featherRadius * strokeMatrixMaxScale),
+                                       parametricPrecision);
+    }
     const bool roundJoinStroked = isStroke && d.join == Join::round;
 
     std::vector<ContourInfo> contours;
@@ -694,7 +720,8 @@ PathTess tessellate_path_records(const PathDrawDesc& d, uint32_t firstContourID,
                 endpointsSum = endpointsSum + p[3];
                 float t[2] = {0, 0};
                 bool areCusps = false;
-                uint8_t numChops = (uint8_t)find_cubic_convex_180_chops(p, t, &areCusps);
+                // Feathered fills were already chopped (draw.cpp:1062-1067).
+                uint8_t numChops = isStroke ? (uint8_t)find_cubic_convex_180_chops(p, t, &areCusps) : 0;
                 uint8_t chopKey = (uint8_t)((numChops << 1) | (areCusps ? 1 : 0));
                 chopKeys.push_back(chopKey);
                 chopTs.push_back({t[0], t[1]});
@@ -725,7 +752,12 @@ PathTess tessellate_path_records(const PathDrawDesc& d, uint32_t firstContourID,
                 {
                     n4s.push_back(wang_cubic_pow4(q, parametricPrecision, d.matrix));
                     ++curveIdx;
-                    tangentPairs.push_back({find_cubic_tan0(q), find_cubic_tan1(q)});
+                    // Feathered fills have no polar segments: a zero
+                    // rotation (draw.cpp:1111-1128).
+                    if (isStroke)
+                        tangentPairs.push_back({find_cubic_tan0(q), find_cubic_tan1(q)});
+                    else
+                        tangentPairs.push_back({V2{0, 1}, V2{0, 1}});
                     ++rotationIdx;
                 }
                 break;
@@ -761,7 +793,9 @@ PathTess tessellate_path_records(const PathDrawDesc& d, uint32_t firstContourID,
         {
             for (size_t j = c.firstRotationIdx; j < c.endRotationIdx; ++j)
                 v += polarCounts[j];
-            if (d.join == Join::round)
+            if (!isStroke)
+                v += (uint32_t)c.strokeJoinCount * (feather_join_segment_count(polarSegmentsPerRadian) - 1);
+            else if (d.join == Join::round)
                 v -= (uint32_t)c.strokeJoinCount;
             else
                 v += (uint32_t)c.strokeJoinCount * (NUM_SEGMENTS_IN_MITER_OR_BEVEL_JOIN - 1);
@@ -853,7 +887,7 @@ PathTess tessellate_path_records(const PathDrawDesc& d, uint32_t firstContourID,
         uint32_t emulatedCapAsJoinFlags = 0;
         if (isStrokeOrFeather)
         {
-            joinTypeFlags = join_type_flags(d.join);
+            joinTypeFlags = isStroke ? join_type_flags(d.join) : FEATHER_JOIN_CONTOUR_FLAG;
             if (c.strokeCapSegmentCount != 0)
             {
                 Cap cap = !c.closed ? d.cap : empty_stroke_cap(true, d.join, d.cap);
@@ -884,7 +918,8 @@ PathTess tessellate_path_records(const PathDrawDesc& d, uint32_t firstContourID,
         uint32_t contourIDWithFlags = d.contourFlags | contourID;
         ++contourID;
 
-        const uint32_t numSegmentsInNotRoundJoin = NUM_SEGMENTS_IN_MITER_OR_BEVEL_JOIN;
+        const uint32_t numSegmentsInNotRoundJoin = isStroke ? NUM_SEGMENTS_IN_MITER_OR_BEVEL_JOIN
+                                                            : feather_join_segment_count(polarSegmentsPerRadian);
         V2 joinTangent{0, 1};
         uint32_t joinSegmentCount = 1;
         V2 implicitClose[2];
@@ -909,7 +944,7 @@ PathTess tessellate_path_records(const PathDrawDesc& d, uint32_t firstContourID,
                 case kLine:
                 {
                     const V2* lp = P + ptc - 1;
-                    if (isStroke)
+                    if (isStrokeOrFeather)
                     {
                         if (c.closed || !isFinalVerb)
                         {

@@ -182,6 +182,7 @@ struct PathDrawDesc
     uint32_t ptCount = 0;
     Mat matrix;
     float strokeRadius = 0; // 0 => fill
+    float featherRadius = 0; // find_feather_radius(paint feather), 0 => none
     Join join = Join::miter;
     Cap cap = Cap::butt;
     FillRule fillRule = FillRule::nonZero;
@@ -219,6 +220,44 @@ void write_spans(const CubicRecord& r, Dirs dirs, std::vector<TessVertexSpan>* o
 // Padding spans (render_context.cpp:3404-3424).
 void write_padding_span(uint32_t count, uint32_t location,
                         std::vector<TessVertexSpan>* out);
+
+// ---- feathering (feather.cpp) ----
+constexpr uint32_t GAUSSIAN_TABLE_SIZE = 512;                  // gpu.hpp:2088
+constexpr float kPI_OVER_2 = 1.57079632679f;                   // common.glsl:9
+constexpr float FEATHER_COVERAGE_BIAS = -2.f;                  // draw_path_common.glsl:9
+constexpr float FEATHER_COVERAGE_THRESHOLD = -1.5f;            // draw_path_common.glsl:13
+constexpr float FEATHER_X_COORD_BIAS = .25f;                   // draw_path_common.glsl:17
+constexpr float HORIZONTAL_COTANGENT_THRESHOLD = 1e3f;         // draw_path_common.glsl:21
+constexpr float HORIZONTAL_COTANGENT_VALUE = 1e6f;             // draw_path_common.glsl:24-25
+constexpr uint32_t FEATHER_JOIN_HELPER_VERTEX_COUNT = 3;       // constants.glsl:31
+constexpr uint32_t FEATHER_JOIN_MIN_SEGMENT_COUNT = 2 + 4;     // constants.glsl:35-40
+constexpr float FEATHER_POLAR_SEGMENT_MIN_ANGLE = 3.14159265f / 16; // gpu.hpp:69
+constexpr float COS_FEATHER_POLAR_SEGMENT_MIN_ANGLE_OVER_2 = 0.99518472667f; // gpu.hpp:72-73
+
+uint16_t cast_f32_to_f16(float x);                            // gpu.cpp:1959-1980
+float cast_f16_to_f32(uint16_t h);                            // gpu.cpp:1944-1957
+void gaussian_integral_table(float table[GAUSSIAN_TABLE_SIZE]);         // gpu.cpp:1991-2063
+void inverse_gaussian_integral_table(float table[GAUSSIAN_TABLE_SIZE]); // gpu.cpp:2126-2181
+const uint16_t* feather_table_f16();         // the two tables above, as fp16 texels
+const uint16_t* inverse_feather_table_f16();
+float table_lookup(const uint16_t* table, float x); // linear filter, clamp to edge
+float feather(float x);                      // FEATHER (common.glsl:53-60)
+float inverse_feather(float x);              // INVERSE_FEATHER (common.glsl:61-68)
+void pack_feathered_fill_coverages(float cornerTheta, V2 spokeNorm, float outset, float out[4]);
+float eval_feathered_fill(const float coverages[4]);   // draw_path_common.glsl:153-240
+float eval_feathered_stroke(const float coverages[4]); // draw_path_common.glsl:242-258
+float clamped_divide(float a, float b);                // bezier_utils.glsl:158-169
+float find_cubic_max_height(V2 p0, V2 p1, V2 p2, V2 p3, float* outT); // bezier_utils.glsl:171-246
+float measure_cubic_local_curvature(V2 p0, V2 p1, V2 p2, V2 p3, float T,
+                                    float desiredSpread); // bezier_utils.glsl:72-156
+void soften_feathered_cubic(V2* p0, V2* p1, V2* p2, V2* p3, float featherRadius); // tessellate.glsl:146-193
+// RiveRenderPath::makeSoftenedCopyForFeathering (rive_render_path.cpp:233-449).
+void soften_path_for_feathering(const uint8_t* verbs, uint32_t verbCount, const V2* pts, uint32_t ptCount,
+                                float feather, float matrixMaxScale, std::vector<uint8_t>* outVerbs,
+                                std::vector<V2>* outPts);
+// find_feather_radius (draw.cpp:313-318): the paint's feather is two
+// standard deviations wide, the radius covers three.
+inline float find_feather_radius(float paintFeather) { return paintFeather * (3.f / 2); }
 
 // ---- tessellation shader restatement (tessellate.glsl:61-567) ----
 struct TessVertex

@@ -17,6 +17,7 @@ struct DrawInfo
     uint32_t tessVertexCount; // both directions
     Dirs dirs;
     bool isStroke;
+    bool featheredFill = false;
 };
 
 // An image with its mip chain (2x2 box filter with rounding), premultiplied

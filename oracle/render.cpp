@@ -407,7 +407,7 @@ struct Builder
                           const Mat& matrix, float strokeRadius, Join join, Cap cap, FillRule fillRule,
                           uint32_t paintType, uint32_t paintValue, uint32_t clipID, uint32_t outerClipID,
                           const Gradient* grad, uint32_t blendMode, uint32_t image, uint32_t sampler,
-                          const Mat* clipRectInverse = nullptr)
+                          const Mat* clipRectInverse = nullptr, float featherRadius = 0)
     {
         PathDrawDesc d;
         d.verbs = verbs;
@@ -419,6 +419,7 @@ struct Builder
         d.join = join;
         d.cap = cap;
         d.fillRule = fillRule;
+        d.featherRadius = featherRadius;
         derive_directions(&d);
 
         uint32_t pathID = (uint32_t)cur().pathData.size(); // record 0 is reserved
@@ -429,7 +430,7 @@ struct Builder
         rive_hip_path_data pd{};
         memcpy(pd.matrix, matrix.m, sizeof(pd.matrix));
         pd.stroke_radius = strokeRadius;
-        pd.feather_radius = 0;
+        pd.feather_radius = featherRadius;
         pd.z_index = 0;
         cur().pathData.push_back(pd);
 
@@ -529,6 +530,7 @@ struct Builder
         di.tessVertexCount = pt.midpointFanTessVertexCount;
         di.dirs = d.dirs;
         di.isStroke = strokeRadius > 0;
+        di.featheredFill = strokeRadius == 0 && featherRadius > 0;
         cur().draws.push_back(di);
         for (size_t c = 0; c < pt.contours.size(); ++c)
             cur().contours.push_back(pt.contours[c]);
@@ -708,7 +710,7 @@ struct Builder
         }
     };
     Needs measure(const uint8_t* verbs, uint32_t verbCount, const V2* pts, uint32_t ptCount, const Mat& matrix,
-                  float strokeRadius, Join join, Cap cap, FillRule fillRule) const
+                  float strokeRadius, Join join, Cap cap, FillRule fillRule, float featherRadius = 0) const
     {
         PathDrawDesc d;
         d.verbs = verbs;
@@ -720,6 +722,7 @@ struct Builder
         d.join = join;
         d.cap = cap;
         d.fillRule = fillRule;
+        d.featherRadius = featherRadius;
         derive_directions(&d);
         d.pathID = 1;
         PathTess pt = tessellate_path_records(d, 1, 0);
@@ -956,17 +959,45 @@ struct Builder
                 continue;
             if (!(d.feather >= 0))
                 continue;
-            if (d.feather != 0)
-                continue; // feathering is not restated yet
+            // Feathers (rive_renderer.cpp:164-184): only clockwise fills are
+            // feathered, and a fill whose feather spans more than a pixel on
+            // screen is drawn from a copy of its path softened for it.
+            const uint8_t* dverbs = s.verbs + sp.first_verb;
+            uint32_t dverbCount = sp.verb_count;
+            const V2* dpts = pathPts(d.path);
+            uint32_t dptCount = sp.point_count;
+            std::vector<uint8_t> softVerbs;
+            std::vector<V2> softPts;
+            if (d.feather != 0 && !isStroke)
+            {
+                if ((FillRule)sp.fill_rule != FillRule::clockwise)
+                    continue;
+                const float matrixMaxScale = find_max_scale(m);
+                if (d.feather * matrixMaxScale > 1)
+                {
+                    soften_path_for_feathering(dverbs, dverbCount, dpts, dptCount, d.feather, matrixMaxScale,
+                                               &softVerbs, &softPts);
+                    dverbs = softVerbs.data();
+                    dverbCount = (uint32_t)softVerbs.size();
+                    dpts = softPts.data();
+                    dptCount = (uint32_t)softPts.size();
+                }
+            }
+            const float featherRadius = d.feather != 0 ? find_feather_radius(d.feather) : 0.f;
             // PathDraw::Make bounds (draw.cpp:452-494)
             float ox = 0, oy = 0;
-            if (isStroke)
+            if (isStroke || featherRadius != 0)
             {
-                float outset = d.thickness * .5f;
-                if ((Join)d.join == Join::miter)
-                    outset *= 4.f;
-                else if ((Cap)d.cap == Cap::square)
-                    outset *= 1.41421356f;
+                float outset = 0;
+                if (isStroke)
+                {
+                    outset = d.thickness * .5f;
+                    if ((Join)d.join == Join::miter)
+                        outset *= 4.f;
+                    else if ((Cap)d.cap == Cap::square)
+                        outset *= 1.41421356f;
+                }
+                outset += featherRadius;
                 // strokePixelOutset = paintMatrix.mapBoundingBox({0,0,outset,outset})
                 V2 corners[4] = {{0, 0}, {outset, 0}, {0, outset}, {outset, outset}};
                 Mat lin = m;
@@ -981,7 +1012,7 @@ struct Builder
                 ox = (r - l) + 1;
                 oy = (b - t) + 1;
             }
-            IAABB pixelBounds = map_bounds_round_out(m, pathPts(d.path), sp.point_count, ox, oy);
+            IAABB pixelBounds = map_bounds_round_out(m, dpts, dptCount, ox, oy);
             if (pixelBounds.intersect(frame).empty())
                 continue;
             // applyClip (rive_renderer.cpp:631-824)
@@ -1008,11 +1039,12 @@ struct Builder
                 strokeRadius = fmaxf(d.thickness * .5f, 1.17549435e-38f);
             Mat crInv;
             const bool hasCR = cr.inverse(&crInv);
-            const Needs needs = measurePath(d.path, m, strokeRadius, (Join)d.join, (Cap)d.cap, (FillRule)sp.fill_rule);
+            const Needs needs = measure(dverbs, dverbCount, dpts, dptCount, m, strokeRadius, (Join)d.join,
+                                        (Cap)d.cap, (FillRule)sp.fill_rule, featherRadius);
             clipAndPush(height, frame, needs, gp, [&](uint32_t drawClipID) {
-                pushPathDraw(d.path, m, strokeRadius, (Join)d.join, (Cap)d.cap, (FillRule)sp.fill_rule, paintType,
-                             paintValue, drawClipID, 0, gp, shader_blend_mode(d.blend_mode),
-                             hasCR ? &crInv : nullptr);
+                pushGeometryDraw(dverbs, dverbCount, dpts, dptCount, m, strokeRadius, (Join)d.join, (Cap)d.cap,
+                                 (FillRule)sp.fill_rule, paintType, paintValue, drawClipID, 0, gp,
+                                 shader_blend_mode(d.blend_mode), 0, 0, hasCR ? &crInv : nullptr, featherRadius);
             });
         }
         endFlush();
@@ -1157,11 +1189,62 @@ std::array<PatchVertexDef, 42> make_midpoint_fan_patch(std::array<uint16_t, 72>*
     return v;
 }
 
+// PatchType::midpointFanCenterAA (gpu.cpp:446-687), the patch of feathered
+// fills: an AA border on both sides of the edge (coverage 1/2 on the edge, 0
+// at outset +-1), the curve fan and the triangle to the contour midpoint.
+std::array<PatchVertexDef, 74> make_midpoint_fan_center_aa_patch(std::array<uint16_t, 120>* idx)
+{
+    std::array<PatchVertexDef, 74> v{};
+    int n = 0;
+    for (int i = 0; i < 8; ++i)
+    {
+        float l = (float)i, r = l + 1;
+        v[n + 0] = {l, 0, .5f, r - 1, 0, .5f, 0};
+        v[n + 1] = {l, 1, 0, l - 1, 0, .5f, 0};
+        v[n + 2] = {r, 0, .5f, r - 1, 1, 0, 0};
+        v[n + 3] = {r, 1, 0, l - 1, 1, 0, 0};
+        n += 4;
+    }
+    for (int i = 0; i < 8; ++i)
+    {
+        float l = (float)i, r = l + 1;
+        v[n + 0] = {l, -0.f, .5f, r - 1, -0.f, .5f, 0};
+        v[n + 1] = {r, -0.f, .5f, r - 1, -1, 0, 0};
+        v[n + 2] = {l, -1, 0, l - 1, -0.f, .5f, 0};
+        v[n + 3] = {r, -1, 0, l - 1, -1, 0, 0};
+        n += 4;
+    }
+    for (int i = 0; i <= 8; ++i)
+        v[n++] = {(float)i, 0, 1, (float)i - 1, 0, 1, 1};
+    v[n++] = {0, 0, 1, 0, 0, 1, 2};
+    int k = 0;
+    const uint16_t pat[6] = {0, 1, 2, 2, 1, 3};
+    const uint16_t neg[6] = {0, 2, 1, 1, 2, 3};
+    for (int s = 0; s < 8; ++s)
+        for (int j = 0; j < 6; ++j)
+            (*idx)[k++] = (uint16_t)(s * 4 + pat[j]);
+    for (int s = 0; s < 8; ++s)
+        for (int j = 0; j < 6; ++j)
+            (*idx)[k++] = (uint16_t)(32 + s * 4 + neg[j]);
+    const int fan = 64;
+    for (int step = 1; step < 8; step <<= 1)
+        for (int i = 0; i < 8; i += step * 2)
+        {
+            (*idx)[k++] = (uint16_t)(fan + i);
+            (*idx)[k++] = (uint16_t)(fan + i + step);
+            (*idx)[k++] = (uint16_t)(fan + i + step * 2);
+        }
+    (*idx)[k++] = (uint16_t)(fan);
+    (*idx)[k++] = (uint16_t)(fan + 8);
+    (*idx)[k++] = (uint16_t)(73);
+    return v;
+}
+
 struct OutVertex
 {
     bool discard;
     float x, y;   // pixel coordinates
-    float cx, cy; // coverages.xy
+    float c[4];   // coverages
     uint32_t pathID;
 };
 
@@ -1175,7 +1258,7 @@ inline float manhattan_pixel_width(const float M[4], float nx, float ny)
 }
 
 // unpack_tessellated_path_vertex (draw_path_common.glsl:275-789), midpoint-fan
-// patches, no feather, no MSAA.
+// and midpoint-fan-center-AA patches, no MSAA.
 OutVertex draw_vertex(const Flush& f, const std::vector<TessVertex>& tex,
                       const PatchVertexDef& pv, int instanceID)
 {
@@ -1197,6 +1280,7 @@ OutVertex draw_vertex(const Flush& f, const std::vector<TessVertex>& tex,
     const float M[4] = {pd.matrix[0], pd.matrix[1], pd.matrix[2], pd.matrix[3]};
     float tx = pd.matrix[4], ty = pd.matrix[5];
     float strokeRadius = pd.stroke_radius;
+    float featherRadius = pd.feather_radius;
     uint32_t mirroredContourFlag = contourIDWithFlags & MIRRORED_CONTOUR_CONTOUR_FLAG;
     if (mirroredContourFlag != 0u)
     {
@@ -1226,11 +1310,75 @@ OutVertex draw_vertex(const Flush& f, const std::vector<TessVertex>& tex,
         contourIDWithFlags = (tv.contourIDWithFlags & ~MIRRORED_CONTOUR_CONTOUR_FLAG) |
                              mirroredContourFlag;
     }
-    float theta = ubits(tv.z);
+    float theta;
+    float featherJoinEdge0Theta = 0, featherJoinCornerTheta = 0;
+    const bool featherJoin =
+        (contourIDWithFlags & JOIN_TYPE_MASK) == FEATHER_JOIN_CONTOUR_FLAG && pv.type == 0;
+    if (featherJoin)
+    {
+        // draw_path_common.glsl:373-468: the join's stepping is worked out
+        // here from the tessellation vertices on either side of it.
+        uint32_t joinDataPacked = tv.z;
+        float joinVertexID = float(joinDataPacked & 0xffffu);
+        float joinSegmentCount = float(joinDataPacked >> 16);
+        int ex = (int)(-joinVertexID - 1.f), ey = (int)(joinSegmentCount - joinVertexID + 1.f);
+        if ((contourIDWithFlags & MIRRORED_CONTOUR_CONTOUR_FLAG) != 0u)
+        {
+            ex = -ex;
+            ey = -ey;
+        }
+        TessVertex before = tex[(size_t)(tessVertexIdx + ex)];
+        TessVertex after = tex[(size_t)(tessVertexIdx + ey)];
+        if ((after.contourIDWithFlags & (MIRRORED_CONTOUR_CONTOUR_FLAG | 0xffffu)) !=
+            (before.contourIDWithFlags & (MIRRORED_CONTOUR_CONTOUR_FLAG | 0xffffu)))
+            after = tex[(size_t)vertexIndex0];
+        featherJoinEdge0Theta = ubits(before.z);
+        float featherJoinEdge1Theta = ubits(after.z);
+        featherJoinCornerTheta = featherJoinEdge1Theta - featherJoinEdge0Theta;
+        if (fabsf(featherJoinCornerTheta) > PI_)
+            featherJoinCornerTheta -= _2PI_ * signf_(featherJoinCornerTheta);
+        float nonHelperSegmentCount = joinSegmentCount + 1.f - float(FEATHER_JOIN_HELPER_VERTEX_COUNT);
+        float forwardSegmentCount = std::min(
+            std::max(rintf(fabsf(featherJoinCornerTheta) / PI_ * nonHelperSegmentCount), 1.f),
+            nonHelperSegmentCount - 1.f);
+        float backwardSegmentCount = nonHelperSegmentCount - forwardSegmentCount;
+        if (joinVertexID <= backwardSegmentCount)
+        {
+            featherJoinCornerTheta = -(PI_ * signf_(featherJoinCornerTheta) - featherJoinCornerTheta);
+            joinSegmentCount = backwardSegmentCount;
+            if (joinVertexID == backwardSegmentCount)
+                outset = -outset;
+        }
+        else if (joinVertexID == backwardSegmentCount + 1.f)
+        {
+            joinVertexID = 0;
+            joinSegmentCount = 0;
+            outset = 0;
+        }
+        else
+        {
+            joinVertexID -= backwardSegmentCount + 2.f;
+            joinSegmentCount = forwardSegmentCount;
+        }
+        if (joinVertexID == joinSegmentCount)
+            theta = featherJoinEdge1Theta;
+        else
+            theta = featherJoinEdge0Theta + featherJoinCornerTheta * (joinVertexID / joinSegmentCount);
+    }
+    else
+    {
+        theta = ubits(tv.z);
+    }
     float nx = sinf(theta), ny = -cosf(theta);
     float ox = tv.x, oy = tv.y;
     float offx = 0, offy = 0;
-    float covx, covy;
+    float covx, covy, covz = 0, covw = 0;
+    if (featherRadius != 0.f)
+    {
+        // Never softer than 1.5 standard deviations across 1/2 px (:478-485).
+        float mx = M[0] * nx + M[2] * ny, my = M[1] * nx + M[3] * ny;
+        featherRadius = std::max(featherRadius, 1.f / sqrtf(mx * mx + my * my));
+    }
     if (strokeRadius != 0.f)
     {
         float det = M[0] * M[3] - M[2] * M[1];
@@ -1239,9 +1387,9 @@ OutVertex draw_vertex(const Flush& f, const std::vector<TessVertex>& tex,
             outset = std::min(outset, 0.f);
         if (contourIDWithFlags & RIGHT_JOIN_CONTOUR_FLAG)
             outset = std::max(outset, 0.f);
-        float aaRadius = manhattan_pixel_width(M, nx, ny) * AA_RADIUS;
+        float aaRadius = featherRadius != 0.f ? featherRadius : manhattan_pixel_width(M, nx, ny) * AA_RADIUS;
         float globalCoverage = 1.f;
-        if (aaRadius > strokeRadius)
+        if (aaRadius > strokeRadius && featherRadius == 0.f)
         {
             globalCoverage = strokeRadius / aaRadius;
             strokeRadius = aaRadius;
@@ -1316,6 +1464,8 @@ OutVertex draw_vertex(const Flush& f, const std::vector<TessVertex>& tex,
         covx *= globalCoverage;
         covy *= globalCoverage;
         covy = std::max(covy, 1e-4f);
+        if (featherRadius != 0.f)
+            covx = FEATHER_COVERAGE_BIAS - covx; // a feathered stroke
         float px_ = outset * vox, py_ = outset * voy;
         offx = M[0] * px_ + M[2] * py_;
         offy = M[1] * px_ + M[3] * py_;
@@ -1329,15 +1479,51 @@ OutVertex draw_vertex(const Flush& f, const std::vector<TessVertex>& tex,
     {
         covx = fillCoverage;
         covy = -1.f;
-        // sign(MUL(outset * norm, inverse(M))) * AA_RADIUS
-        float det = M[0] * M[3] - M[2] * M[1];
-        float idet = 1.f / det;
-        float vx = outset * nx, vy = outset * ny;
-        // row vector * inverse(M); inverse(M) columns: (M3, -M1)/det, (-M2, M0)/det
-        float rx = (vx * M[3] + vy * -M[1]) * idet;
-        float ry = (vx * -M[2] + vy * M[0]) * idet;
-        offx = signf_(rx) * AA_RADIUS;
-        offy = signf_(ry) * AA_RADIUS;
+        if (featherRadius != 0.f)
+        {
+            // draw_path_common.glsl:661-744: a feathered edge starts out as a
+            // flat edge feathering y0 = fillCoverage; the spokes of a feather
+            // join carry the corner's coordinates instead.
+            covy = FEATHER_COVERAGE_BIAS;
+            covz = HORIZONTAL_COTANGENT_VALUE;
+            covw = fillCoverage;
+            if (featherJoin)
+            {
+                if (featherJoinCornerTheta < 0.f)
+                {
+                    featherJoinEdge0Theta += featherJoinCornerTheta;
+                    featherJoinCornerTheta = -featherJoinCornerTheta;
+                }
+                float spokeTheta = theta - featherJoinEdge0Theta;
+                float sm = spokeTheta + kPI_OVER_2;
+                spokeTheta = (sm - _2PI_ * floorf(sm / _2PI_)) - kPI_OVER_2; // GLSL mod
+                spokeTheta = std::min(std::max(spokeTheta, 0.f), featherJoinCornerTheta);
+                if (spokeTheta > featherJoinCornerTheta * .5f)
+                    spokeTheta = featherJoinCornerTheta - spokeTheta;
+                float c4[4];
+                pack_feathered_fill_coverages(featherJoinCornerTheta, V2{sinf(spokeTheta), cosf(spokeTheta)},
+                                              outset, c4);
+                covx = c4[0];
+                covy = c4[1];
+                covz = c4[2];
+                covw = c4[3];
+            }
+            float px_ = (outset * featherRadius) * nx, py_ = (outset * featherRadius) * ny;
+            offx = M[0] * px_ + M[2] * py_;
+            offy = M[1] * px_ + M[3] * py_;
+        }
+        else
+        {
+            // sign(MUL(outset * norm, inverse(M))) * AA_RADIUS
+            float det = M[0] * M[3] - M[2] * M[1];
+            float idet = 1.f / det;
+            float vx = outset * nx, vy = outset * ny;
+            // row vector * inverse(M); inverse(M) columns: (M3, -M1)/det, (-M2, M0)/det
+            float rx = (vx * M[3] + vy * -M[1]) * idet;
+            float ry = (vx * -M[2] + vy * M[0]) * idet;
+            offx = signf_(rx) * AA_RADIUS;
+            offy = signf_(ry) * AA_RADIUS;
+        }
         if (bool(contourIDWithFlags & MIRRORED_CONTOUR_CONTOUR_FLAG) !=
             bool(contourIDWithFlags & NEGATE_PATH_FILL_COVERAGE_FLAG))
             covx = -covx;
@@ -1349,8 +1535,10 @@ OutVertex draw_vertex(const Flush& f, const std::vector<TessVertex>& tex,
     }
     o.x = (M[0] * ox + M[2] * oy) + offx + tx;
     o.y = (M[1] * ox + M[3] * oy) + offy + ty;
-    o.cx = covx;
-    o.cy = covy;
+    o.c[0] = covx;
+    o.c[1] = covy;
+    o.c[2] = covz;
+    o.c[3] = covw;
     o.pathID = pathID;
     o.discard = false;
     return o;
@@ -1658,8 +1846,10 @@ void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* 
     std::vector<float> cov((size_t)W * H, 0.f);
     std::vector<uint32_t> stamp((size_t)W * H, 0u);
 
-    std::array<uint16_t, 72> idx;
-    auto patch = make_midpoint_fan_patch(&idx);
+    std::array<uint16_t, 72> fanIdx;
+    const auto fanPatch = make_midpoint_fan_patch(&fanIdx);
+    std::array<uint16_t, 120> caaIdx;
+    const auto caaPatch = make_midpoint_fan_center_aa_patch(&caaIdx);
 
     // Rows are split into bands, one thread per band at a time: a pixel's
     // triangles still accumulate in submission order, so the result does not
@@ -1678,16 +1868,20 @@ void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* 
         const int firstInstance = (int)(di.tessLocation / 8);
         const int endInstance = (int)((di.tessLocation + di.tessVertexCount) / 8);
         const int nInst = endInstance - firstInstance;
-        ov.resize((size_t)nInst * 42);
+        // Feathered fills draw midpoint-fan-center-AA patches (draw.cpp:1866-1870).
+        const PatchVertexDef* patch = di.featheredFill ? caaPatch.data() : fanPatch.data();
+        const uint16_t* idx = di.featheredFill ? caaIdx.data() : fanIdx.data();
+        const int NV = di.featheredFill ? 74 : 42, NT = di.featheredFill ? 40 : 24;
+        ov.resize((size_t)nInst * NV);
         instRows.resize((size_t)nInst * 2);
 #pragma omp parallel for schedule(static)
         for (int k = 0; k < nInst; ++k)
         {
             int64_t lo = INT64_MAX, hi = INT64_MIN;
-            for (int v = 0; v < 42; ++v)
+            for (int v = 0; v < NV; ++v)
             {
                 const OutVertex o = draw_vertex(f, tex, patch[v], firstInstance + k);
-                ov[(size_t)k * 42 + v] = o;
+                ov[(size_t)k * NV + v] = o;
                 if (!o.discard && snappable(o.x) && snappable(o.y))
                 {
                     const int64_t y = snap(o.y);
@@ -1708,8 +1902,8 @@ void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* 
             {
                 if (instRows[2 * k] > yb1 || instRows[2 * k + 1] < yb0)
                     continue;
-                const OutVertex* iv = &ov[(size_t)k * 42];
-                for (int t = 0; t < 24; ++t)
+                const OutVertex* iv = &ov[(size_t)k * NV];
+                for (int t = 0; t < NT; ++t)
                 {
                     const OutVertex& A = iv[idx[t * 3 + 0]];
                     const OutVertex& B = iv[idx[t * 3 + 1]];
@@ -1770,21 +1964,26 @@ void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* 
                             double lb = (double)edgeD(c, a, q) * inv;
                             double lc = (double)edgeD(a, b, q) * inv;
                             size_t p = (size_t)y * W + x;
-                            float fcx = (float)(la * A.cx + lb * B.cx + lc * C.cx);
+                            float c4[4];
+                            for (int q = 0; q < 4; ++q)
+                                c4[q] = (float)(la * A.c[q] + lb * B.c[q] + lc * C.c[q]);
                             float count = stamp[p] == pathID ? cov[p] : 0.f;
                             if (stamp[p] != pathID)
                             {
                                 stamp[p] = pathID;
                                 touched.push_back((uint32_t)p);
                             }
+                            // apply_frag_coverage (draw_path.vert:500-545)
                             if (isStroke)
                             {
-                                float fcy = (float)(la * A.cy + lb * B.cy + lc * C.cy);
-                                count = std::max(std::min(fcx, fcy), count);
+                                float fc = c4[0] < FEATHER_COVERAGE_THRESHOLD ? eval_feathered_stroke(c4)
+                                                                              : std::min(c4[0], c4[1]);
+                                count = std::max(fc, count);
                             }
                             else
                             {
-                                count = count + fcx;
+                                float fc = c4[1] < FEATHER_COVERAGE_THRESHOLD ? eval_feathered_fill(c4) : c4[0];
+                                count = count + fc;
                             }
                             cov[p] = round_to_f16(count);
                         }
@@ -1802,6 +2001,7 @@ void tessellate_flush(const Flush& f, std::vector<TessVertex>* tex)
     rows = std::max(rows, 1u);
     tex->assign((size_t)rows * kTessTextureWidth, TessVertex{0, 0, 0, 0});
     for (const TessVertexSpan& s : f.spans)
-        tessellate_span(s, f.contours.data(), nullptr, tex->data(), rows);
+        tessellate_span(s, f.contours.data(), reinterpret_cast<const float*>(f.pathData.data()), tex->data(),
+                        rows);
 }
 } // namespace oracle

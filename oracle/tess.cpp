@@ -67,10 +67,26 @@ inline uint32_t fbits(float f)
 void tessellate_span(const TessVertexSpan& s, const ContourData* contours,
                      const float* pathRecords, TessVertex* texture, uint32_t textureRows)
 {
-    (void)contours;
-    (void)pathRecords; // only needed by feathered fills / outer curves
     F2 P0{s.pts[0].x, s.pts[0].y}, P1{s.pts[1].x, s.pts[1].y};
     F2 P2{s.pts[2].x, s.pts[2].y}, P3{s.pts[3].x, s.pts[3].y};
+    // The span's path (tessellate.glsl:125-144): a feathered fill's cubic is
+    // flattened in proportion to its curvature first (:146-193).
+    {
+        const uint32_t contourID = s.contourIDWithFlags & CONTOUR_ID_MASK;
+        const uint32_t pathID = contourID > 0u && contours != nullptr ? contours[contourID - 1].pathID : 0u;
+        if (pathID != 0u && pathRecords != nullptr)
+        {
+            const float strokeRadius = pathRecords[pathID * 16 + 6];
+            const float featherRadius = pathRecords[pathID * 16 + 7];
+            if (featherRadius != 0.f && strokeRadius == 0.f)
+            {
+                V2 q0{P0.x, P0.y}, q1{P1.x, P1.y}, q2{P2.x, P2.y}, q3{P3.x, P3.y};
+                soften_feathered_cubic(&q0, &q1, &q2, &q3, featherRadius);
+                P1 = F2{q1.x, q1.y};
+                P2 = F2{q2.x, q2.y};
+            }
+        }
+    }
     const uint32_t segmentCounts = s.segmentCounts;
     const uint32_t parametricSegmentCount = segmentCounts & 0x3ffu;
     const uint32_t polarSegmentCount = (segmentCounts >> 10) & 0x3ffu;

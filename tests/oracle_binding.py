@@ -55,6 +55,27 @@ def lib():
         L.oracle_chop_cubic_at2.restype = None
         L.oracle_convex_180_chops.argtypes = [fp, fp, ctypes.POINTER(ctypes.c_int)]
         L.oracle_convex_180_chops.restype = ctypes.c_int
+        u16p = ctypes.POINTER(ctypes.c_uint16)
+        L.oracle_feather_tables.argtypes = [u16p, u16p]
+        L.oracle_feather_tables.restype = None
+        L.oracle_table_lookup.argtypes = [u16p, f]
+        L.oracle_table_lookup.restype = f
+        for name in ["oracle_feather", "oracle_inverse_feather"]:
+            getattr(L, name).argtypes = [f]
+            getattr(L, name).restype = f
+        for name in ["oracle_eval_feathered_fill", "oracle_eval_feathered_stroke"]:
+            getattr(L, name).argtypes = [fp]
+            getattr(L, name).restype = f
+        L.oracle_pack_feathered_fill.argtypes = [f, f, f, f, fp]
+        L.oracle_pack_feathered_fill.restype = None
+        L.oracle_cubic_max_height.argtypes = [fp, fp]
+        L.oracle_cubic_max_height.restype = f
+        L.oracle_cubic_local_curvature.argtypes = [fp, f, f]
+        L.oracle_cubic_local_curvature.restype = f
+        L.oracle_soften_path.argtypes = [ctypes.POINTER(ctypes.c_uint8), ctypes.c_uint32, fp, ctypes.c_uint32, f, f,
+                                         ctypes.POINTER(ctypes.c_uint8), ctypes.c_uint32, fp, ctypes.c_uint32,
+                                         ctypes.POINTER(ctypes.c_uint32)]
+        L.oracle_soften_path.restype = ctypes.c_uint32
         _lib = L
     return _lib
 
