@@ -170,6 +170,9 @@ struct rive_hip_context
     std::vector<rive_hip_flush_desc> frameDescs;
     std::vector<std::vector<rive_hip_image*>> frameImages;
     std::vector<std::vector<rive_hip_canvas>> frameCanvases;
+    // The feather function tables (FEATHER, then INVERSE_FEATHER), uploaded
+    // once at creation.
+    float* featherTables = nullptr;
 };
 
 namespace
@@ -285,6 +288,96 @@ hipError_t scratch_address(int device, unsigned char** out)
 }
 } // namespace
 
+namespace
+{
+// The Gaussian-integral function tables of feathering, generated the way the
+// reference generates the fp16 tables it ships (gpu.cpp:1984-2181), so the
+// texel values are the same: a running integral of the normal distribution
+// (mu = 256, sigma = 512/6) sampled 7 times per entry, centered on 1/2,
+// monotonic, clamped and eased into 0 and 1 over 8 entries; its inverse from
+// 32 samples per entry; both rounded to fp16 by adding half of the dropped
+// mantissa and truncating (gpu.cpp:1959-1980).
+float normal_density(float x, float mu, float inverseSigma)
+{
+    const float y = (x - mu) * inverseSigma;
+    return expf((float)(-.5 * (double)y * (double)y)) * inverseSigma * 0.398942280401433f;
+}
+float to_f16_and_back(float x)
+{
+    uint32_t b;
+    memcpy(&b, &x, 4);
+    b += 0x1000u;
+    const uint32_t e = (b & 0x7F800000u) >> 23, m = b & 0x007FFFFFu;
+    uint32_t h = (b & 0x80000000u) >> 16;
+    if (e > 112u)
+        h |= (((e - 112u) << 10) & 0x7C00u) | m >> 13;
+    if (e < 113u && e > 101u)
+        h |= (((0x007FF000u + m) >> (125u - e)) + 1u) >> 1;
+    if (e > 143u)
+        h |= 0x7FFFu;
+    const uint32_t he = (h >> 10) & 0x1fu, hm = h & 0x3ffu;
+    float v = he == 0 ? ldexpf((float)hm, -24) : ldexpf((float)(hm | 0x400u), (int)he - 25);
+    return (h & 0x8000u) ? -v : v;
+}
+void feather_tables_host(float* out) // 2 x kFeatherTableSize
+{
+    const uint32_t N = kFeatherTableSize;
+    const float sigma = N / (3.f * 2), inverseSigma = 1 / sigma, mu = N * .5f;
+    std::vector<float> t(N);
+    float integral = 0;
+    for (uint32_t i = 0; i < N; ++i)
+    {
+        for (int k = 0; k < 7; ++k)
+            integral += normal_density((float)i + (float)(k - 3) / 7, mu, inverseSigma) / 7;
+        t[i] = integral;
+    }
+    const float shift = (float)(.5 - (t[N / 2 - 1] + t[N / 2]) / 2);
+    t[0] = fminf(fmaxf(0, t[0] + shift), 1);
+    for (uint32_t i = 1; i < N; ++i)
+        t[i] = fminf(fmaxf(t[i - 1], t[i] + shift), 1);
+    for (uint32_t i = 0; i < N; ++i)
+    {
+        if (i < 8)
+            t[i] *= (float)i / 8.f;
+        if (i > N - 8 - 1)
+        {
+            const float diffToOne = (float)(1.0 - t[i]);
+            t[i] = t[i] + diffToOne * ((float)(i - (N - 8) + 1) / 8.f);
+        }
+    }
+    for (uint32_t i = 0; i < N; ++i)
+        out[i] = to_f16_and_back(t[i]);
+    // the inverse
+    const uint32_t samples = N * 32;
+    integral = 0;
+    for (uint32_t i = 0; i < (samples + 1) / 2; ++i)
+        integral += normal_density((float)i / 32, mu, inverseSigma) / 32;
+    integral = (float)(.5 - integral);
+    std::fill(t.begin(), t.end(), 0.f);
+    t[N - 1] = 1;
+    float lastX = NAN, lastY = 0;
+    for (uint32_t i = 0; i < samples; ++i)
+    {
+        integral += normal_density((float)i / 32, mu, inverseSigma) / 32;
+        const float x = fminf(fmaxf(0, integral), 1) * N;
+        const float y = (i + .5f) / samples;
+        const uint32_t cell = (uint32_t)x;
+        const float center = cell + .5f;
+        if (center == mu)
+            t[cell] = .5f;
+        else if (lastX <= center && x >= center)
+        {
+            const float w = (center - lastX) / (x - lastX);
+            t[cell] = lastY * (1 - w) + y * w;
+        }
+        lastX = x;
+        lastY = y;
+    }
+    for (uint32_t i = 0; i < N; ++i)
+        out[N + i] = to_f16_and_back(t[i]);
+}
+} // namespace
+
 extern "C" {
 
 const char* rive_hip_last_error(void) { return g_lastError.c_str(); }
@@ -319,6 +412,14 @@ int rive_hip_create(int device, rive_hip_context** out)
         ctx->flushCanvases.reserve(1);
         ctx->lastCanvases.reserve(1);
         e = scratch_address(device, &ctx->scratchPtr);
+    }
+    if (e == hipSuccess)
+        e = hipMalloc((void**)&ctx->featherTables, sizeof(float) * 2 * kFeatherTableSize);
+    if (e == hipSuccess)
+    {
+        std::vector<float> tables(2 * kFeatherTableSize);
+        feather_tables_host(tables.data());
+        e = hipMemcpy(ctx->featherTables, tables.data(), sizeof(float) * tables.size(), hipMemcpyHostToDevice);
     }
     ctx->evSets.reserve(kMaxFrameEventSets);
     ctx->evSets.emplace_back();
@@ -360,6 +461,8 @@ void rive_hip_destroy(rive_hip_context* ctx)
             (void)hipFree(b->ptr);
     if (ctx->hostScalars)
         (void)hipHostFree(ctx->hostScalars);
+    if (ctx->featherTables)
+        (void)hipFree(ctx->featherTables);
     if (ctx->hostImages)
         (void)hipHostFree(ctx->hostImages);
     if (ctx->hostCanvases)
@@ -947,7 +1050,9 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
             return rc;
         if (used > 0)
             tessellate_kernel<<<(used + 255) / 256, 256, 0, ctx->stream>>>(
-                spans, spanCount, P<uint32_t>(ctx->spanTexelOffsets), used, P<TessTexel>(ctx->tess));
+                spans, spanCount, P<uint32_t>(ctx->spanTexelOffsets), used, P<TessTexel>(ctx->tess),
+                reinterpret_cast<const uint4*>(contours), contourCount, reinterpret_cast<const float*>(paths),
+                pathCount, ctx->featherTables);
         ctx->lastTexels = texels;
     }
     HIP_TRY(hipGetLastError());
@@ -1087,7 +1192,7 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
             P<uint2>(ctx->cellPacked),
             P<uint32_t>(ctx->binned), P<Prim>(ctx->prims),
             P<uint32_t>(ctx->ramps), gradRows, P<ImageDesc>(ctx->imageTable), P<CanvasDev>(ctx->canvasTable),
-            canvasCount, d->dither);
+            canvasCount, d->dither, ctx->featherTables);
     HIP_TRY(hipGetLastError());
     RIVE_TRY(mark(ctx, ctx->ev[5]));
     return RIVE_HIP_OK;

@@ -85,6 +85,38 @@ constexpr uint32_t kPrimNoEdge = 128u;
 // and vertices outside the guard box.
 constexpr uint32_t kPrimSpecial = 256u;
 
+// A feathered fill's segment (draw_path_common.glsl:107-258): p = [E0, T0,
+// E1, T1], the edge points (outset 0) and the outer points (outset +1) of both
+// ends; the inner points (outset -1) are B = 2E - T.  Its four border
+// triangles (E0,T0,E1), (E1,T0,T1), (E0,B0,E1), (E1,B0,B1) add the feathered
+// coverage of their interpolated coverage vectors; the edge E0 -> E1 carries
+// the fan's winding like a fill's inner edge.  An end that is a feather-join
+// spoke stores its coverage vectors (kPrimJoin0/1): cov[0..3] the outer
+// vector of end 0 and pad[0] the y0 of its edge vector, cov[4..7] and pad[1]
+// those of end 1 (an edge vector of a spoke is (.75, -2.5, cotTheta, y0), its
+// cotTheta that of the outer vector).  Other ends take the flat-edge vectors
+// (.5, -2, 1e6, .5) on the edge and (0, -2, 1e6, 0) outside.
+constexpr uint32_t kPrimFeather = 512u;
+constexpr uint32_t kPrimJoin0 = 1024u;
+constexpr uint32_t kPrimJoin1 = 2048u;
+
+// The Gaussian-integral function tables of feathering (gpu.cpp:1991-2181):
+// 512 fp16 texels each, held as floats, FEATHER first, INVERSE_FEATHER
+// second.
+constexpr uint32_t kFeatherTableSize = 512;
+
+// A table sampled like a 1D texture with linear filtering and clamp-to-edge
+// at normalized coordinate x (the FEATHER macro, common.glsl:53-68).
+__device__ __forceinline__ float feather_table(const float* __restrict__ t, float x)
+{
+    x = fminf(fmaxf(x, 0.f), 1.f); // NaN -> 0
+    const float u = x * (float)kFeatherTableSize - .5f;
+    const int r = (int)fminf(u + 1.f, (float)(kFeatherTableSize - 1));
+    const int l = max(r - 1, 0);
+    const float w = fminf(fmaxf(u - (float)l, 0.f), 1.f);
+    return t[l] * (1.f - w) + t[r] * w;
+}
+
 // Per-path raster parameters, unpacked from PathData/PaintData/PaintAux.
 // The fields the src-over resolve reads come first.
 struct PathRaster

@@ -178,9 +178,117 @@ __device__ __forceinline__ float atan2v(F2 v) // common.glsl:197-202
     return v.y >= 0.f ? th : -th;
 }
 
+// ---- feathered-fill softening (tessellate.glsl:146-193) ----
+__device__ __forceinline__ float clamped_divide(float a, float b) // bezier_utils.glsl:158-169
+{
+    a = b < 0.f ? -a : a;
+    b = fabsf(b);
+    return a > 0.f ? (a < b ? a / b : 1.f) : 0.f;
+}
+// find_cubic_max_height (bezier_utils.glsl:171-246): three Newton steps on
+// the height above the baseline p0 -> p3 of a convex cubic.
+__device__ float cubic_max_height(F2 p0, F2 p1, F2 p2, F2 p3, float* outT)
+{
+    const F2 base = f2sub(p3, p0);
+    const float lengthBase = sqrtf(f2dot(base, base));
+    if (lengthBase == 0.f)
+    {
+        *outT = .5f;
+        return 0.f;
+    }
+    const F2 norm{-base.y / lengthBase, base.x / lengthBase};
+    const float h2 = f2dot(norm, f2sub(p2, p0));
+    const float h1 = f2dot(norm, f2sub(p1, p0));
+    const float dh = h1 - h2;
+    const float _3A = 3.f * dh, B = -h1 - dh, C = h1;
+    float t = .5f;
+    for (int i = 0; i < 3; ++i)
+    {
+        const float _3At = _3A * t;
+        t = clamped_divide(_3At * t - C, 2.f * (_3At + B));
+    }
+    *outT = t;
+    return fabsf(t * (t * (t * _3A + 3.f * B) + 3.f * C));
+}
+// measure_cubic_local_curvature (bezier_utils.glsl:72-156): the tangent's
+// rotation across a spread along the tangent, centered at T.
+__device__ float cubic_local_curvature(F2 p0, F2 p1, F2 p2, F2 p3, float T, float desiredSpread)
+{
+    constexpr float PI_ = 3.14159265359f;
+    const F2 C = f2sub(p1, p0), D = f2sub(p2, p1), E = f2sub(p3, p0);
+    const F2 B = f2sub(D, C);
+    const F2 A{-3.f * D.x + E.x, -3.f * D.y + E.y};
+    F2 tangent{((A.x * T) + 2.f * B.x) * T + C.x, ((A.y * T) + 2.f * B.y) * T + C.y};
+    tangent = F2{3.f * tangent.x, 3.f * tangent.y};
+    const float lengthTan = sqrtf(f2dot(tangent, tangent));
+    if (lengthTan == 0.f)
+        return 0.f;
+    tangent = F2{tangent.x * (1.f / lengthTan), tangent.y * (1.f / lengthTan)};
+    const float A_ = 2.f * f2dot(A, tangent);
+    const float C_ = 3.f * (A_ * T + 4.f * f2dot(B, tangent)) * T + 6.f * f2dot(C, tangent);
+    const float maxDT = fminf(T, 1.f - T);
+    const float maxSpread = (A_ * maxDT * maxDT + C_) * maxDT;
+    const float targetSpread = fminf(desiredSpread, maxSpread * .9999f);
+    float dt;
+    if (A_ == 0.f)
+        dt = targetSpread / C_;
+    else
+    {
+        const float r = 1.f / A_;
+        const float b = C_ * r, c = -targetSpread * r;
+        const float Q = (-1.f / 3.f) * b, R = .5f * c;
+        const float discr = R * R - Q * Q * Q;
+        if (discr < 0.f)
+        {
+            const float sqrtQ = sqrtf(Q);
+            const float theta = acosf(R / (sqrtQ * sqrtQ * sqrtQ));
+            dt = -2.f * sqrtQ * cosf(theta * (1.f / 3.f) + (-PI_ * 2.f / 3.f));
+        }
+        else
+        {
+            float Ar = powf(fabsf(R) + sqrtf(discr), 1.f / 3.f);
+            if (R < 0.f)
+                Ar = -Ar;
+            dt = Ar != 0.f ? Ar + Q / Ar : 0.f;
+        }
+    }
+    dt = fabsf(dt);
+    const float t0 = T - dt, t1 = T + dt;
+    const F2 tan0d{(A.x * t0 + 2.f * B.x) * t0 + C.x, (A.y * t0 + 2.f * B.y) * t0 + C.y};
+    const F2 tan1d{(A.x * t1 + 2.f * B.x) * t1 + C.x, (A.y * t1 + 2.f * B.y) * t1 + C.y};
+    const F2 e0 = f2sub(f2neq(p0, p1) ? p1 : f2neq(p1, p2) ? p2 : p3, p0);
+    const F2 e1 = f2sub(p3, f2neq(p3, p2) ? p2 : f2neq(p2, p1) ? p1 : p0);
+    return acosf(cosine_between(t0 < 1e-3f ? e0 : tan0d, t1 > 1.f - 1e-3f ? e1 : tan1d));
+}
+// Flattens a feathered fill's cubic in proportion to its curvature across one
+// standard deviation, so the feather's center dims the way a convolution of a
+// curved edge does.
+__device__ void soften_feathered_cubic(F2 p0, F2* p1, F2* p2, F2 p3, float featherRadius,
+                                       const float* __restrict__ inverseFeather)
+{
+    constexpr float PI_ = 3.14159265359f;
+    float maxHeightT;
+    const float height = cubic_max_height(p0, *p1, *p2, p3, &maxHeightT);
+    const float oneStddev = featherRadius * (1.f / 3.f);
+    const float curvature = cubic_local_curvature(p0, *p1, *p2, p3, maxHeightT, oneStddev);
+    float dimming = 1.f - curvature * (1.f / PI_);
+    const F2 d = f2sub(p3, p0);
+    const float stddevsPow2 = f2dot(d, d) / (oneStddev * oneStddev);
+    dimming = fminf(dimming, (stddevsPow2 - 1.f) * .5f);
+    dimming = fminf(dimming, .99f);
+    const float x = feather_table(inverseFeather, .5f * dimming) * -2.f + 1.f;
+    const float softness = clamped_divide(x * featherRadius, height);
+    const F2 f1{(p3.x - p0.x) * (1.f / 3.f) + p0.x, (p3.y - p0.y) * (1.f / 3.f) + p0.y};
+    const F2 f2{(p3.x - p0.x) * (2.f / 3.f) + p0.x, (p3.y - p0.y) * (2.f / 3.f) + p0.y};
+    *p1 = F2{(f1.x - p1->x) * softness + p1->x, (f1.y - p1->y) * softness + p1->y};
+    *p2 = F2{(f2.x - p2->x) * softness + p2->x, (f2.y - p2->y) * softness + p2->y};
+}
+
 __global__ void tessellate_kernel(const SpanV* __restrict__ spans, uint32_t spanCount,
                                   const uint32_t* __restrict__ texelOffsets, uint32_t totalTexels,
-                                  TessTexel* __restrict__ tex)
+                                  TessTexel* __restrict__ tex, const uint4* __restrict__ contours,
+                                  uint32_t contourCount, const float* __restrict__ pathData, uint32_t pathCount,
+                                  const float* __restrict__ featherTables)
 {
     constexpr float PI_ = 3.14159265359f;
     constexpr float TWO_PI = 6.28318530718f;
@@ -204,6 +312,19 @@ __global__ void tessellate_kernel(const SpanV* __restrict__ spans, uint32_t span
     const uint32_t row = (uint32_t)s.y;
 
     F2 P0{s.p[0], s.p[1]}, P1{s.p[2], s.p[3]}, P2{s.p[4], s.p[5]}, P3{s.p[6], s.p[7]};
+    {
+        // the span's path (tessellate.glsl:125-144): feathered fills soften
+        const uint32_t contourID = s.flags & CONTOUR_ID_MASK;
+        if (contourID != 0u && contourID <= contourCount)
+        {
+            const uint32_t pathID = contours[contourID - 1].z;
+            const uint32_t pid = pathID <= pathCount ? pathID : 0u;
+            const float strokeRadius = pathData[(size_t)pid * 16 + 6];
+            const float featherRadius = pathData[(size_t)pid * 16 + 7];
+            if (pid != 0u && featherRadius != 0.f && strokeRadius == 0.f)
+                soften_feathered_cubic(P0, &P1, &P2, P3, featherRadius, featherTables + kFeatherTableSize);
+        }
+    }
     const uint32_t parametricSegmentCount = s.counts & 0x3ffu;
     const uint32_t polarSegmentCount = (s.counts >> 10) & 0x3ffu;
     const uint32_t joinSegmentCount = s.counts >> 20;
@@ -537,10 +658,176 @@ __device__ VsOut draw_vertex(const TessTexel* __restrict__ tex, int texels, int 
     return o;
 }
 
+// One end of a feathered fill's segment: the vertex shader's feather branches
+// (draw_path_common.glsl:373-485, 661-744) for the end's edge vertex (outset
+// 0) and outer vertex (outset +1) of a midpoint-fan-center-AA patch.  A
+// feather-join texel steps its own angle from the texels on either side of
+// the join; its spokes carry the corner's local coordinates.
+struct FeatherEnd
+{
+    float ex, ey, tx, ty; // edge and outer positions (pixels)
+    float c[4];           // the outer vertex's coverage vector
+    float y0E;            // y0 of the edge vertex's vector (cotTheta shared)
+    bool join;
+};
+
+__device__ __forceinline__ void pack_feathered_coverages(float cornerTheta, float spokeX, float spokeY,
+                                                         float outset, float* out)
+{
+    // draw_path_common.glsl:107-150
+    constexpr float PI_OVER_2 = 1.57079632679f;
+    const float lx = (1.f - spokeX * fabsf(outset)) * .5f, ly = (1.f - spokeY * fabsf(outset)) * .5f;
+    float cotTheta, y0;
+    if (fabsf(cornerTheta - PI_OVER_2) < 1.f / 1e3f)
+    {
+        cotTheta = 0.f;
+        y0 = 0.f;
+    }
+    else
+    {
+        const float tanTheta = tanf(cornerTheta);
+        cotTheta = signf_(PI_OVER_2 - cornerTheta) / fmaxf(fabsf(tanTheta), 1.f / 1e6f);
+        y0 = cotTheta >= 0.f ? ly - (1.f - lx) * tanTheta : ly + lx * tanTheta;
+    }
+    out[0] = fmaxf(lx, 0.f) + .25f;
+    out[1] = -ly - 2.f;
+    out[2] = cotTheta;
+    out[3] = y0;
+}
+
+__device__ FeatherEnd feather_end(const TessTexel* __restrict__ tex, int texels, int k, uint32_t cf,
+                                  const PathDataV& pd, uint32_t vertexIndex0)
+{
+    constexpr float PI_ = 3.14159265359f;
+    constexpr float TWO_PI = 6.28318530718f;
+    constexpr float PI_OVER_2 = 1.57079632679f;
+    const TessTexel tv = tex[k];
+    const float M[4] = {pd.m[0], pd.m[1], pd.m[2], pd.m[3]};
+    FeatherEnd r;
+    r.join = (cf & JOIN_TYPE_MASK) == FEATHER_JOIN_CONTOUR_FLAG;
+    float theta, edge0 = 0.f, corner = 0.f, outset = 1.f, outsetE = 0.f;
+    if (r.join)
+    {
+        float joinVertexID = (float)(tv.z & 0xffffu);
+        float joinSegmentCount = (float)(tv.z >> 16);
+        int ex = (int)(-joinVertexID - 1.f), ey = (int)(joinSegmentCount - joinVertexID + 1.f);
+        if (cf & MIRRORED_CONTOUR_CONTOUR_FLAG)
+        {
+            ex = -ex;
+            ey = -ey;
+        }
+        const TessTexel before = tex[min(max(k + ex, 0), texels - 1)];
+        TessTexel after = tex[min(max(k + ey, 0), texels - 1)];
+        if ((after.w & (MIRRORED_CONTOUR_CONTOUR_FLAG | 0xffffu)) !=
+            (before.w & (MIRRORED_CONTOUR_CONTOUR_FLAG | 0xffffu)))
+            after = tex[min((int)vertexIndex0, texels - 1)];
+        edge0 = u2f(before.z);
+        const float edge1 = u2f(after.z);
+        corner = edge1 - edge0;
+        if (fabsf(corner) > PI_)
+            corner -= TWO_PI * signf_(corner);
+        const float nonHelper = joinSegmentCount + 1.f - 3.f; // FEATHER_JOIN_HELPER_VERTEX_COUNT
+        const float fwd = fminf(fmaxf(rintf(fabsf(corner) / PI_ * nonHelper), 1.f), nonHelper - 1.f);
+        const float bwd = nonHelper - fwd;
+        if (joinVertexID <= bwd)
+        {
+            // a backward segment: the angle outside the join
+            corner = -(PI_ * signf_(corner) - corner);
+            joinSegmentCount = bwd;
+            if (joinVertexID == bwd)
+            {
+                outset = -outset;
+                outsetE = -outsetE;
+            }
+        }
+        else if (joinVertexID == bwd + 1.f)
+        {
+            // the throwaway vertex between the backward and forward segments
+            joinVertexID = 0.f;
+            joinSegmentCount = 0.f;
+            outset = 0.f;
+            outsetE = 0.f;
+        }
+        else
+        {
+            joinVertexID -= bwd + 2.f;
+            joinSegmentCount = fwd;
+        }
+        theta = joinVertexID == joinSegmentCount ? edge1 : edge0 + corner * (joinVertexID / joinSegmentCount);
+    }
+    else
+    {
+        theta = u2f(tv.z);
+    }
+    const float nx = sinf(theta), ny = -cosf(theta);
+    float featherRadius = pd.featherRadius;
+    {
+        const float mx = M[0] * nx + M[2] * ny, my = M[1] * nx + M[3] * ny;
+        featherRadius = fmaxf(featherRadius, 1.f / sqrtf(mx * mx + my * my));
+    }
+    if (r.join)
+    {
+        if (corner < 0.f)
+        {
+            edge0 += corner;
+            corner = -corner;
+        }
+        float spoke = theta - edge0;
+        const float sm = spoke + PI_OVER_2;
+        spoke = (sm - TWO_PI * floorf(sm / TWO_PI)) - PI_OVER_2; // GLSL mod
+        spoke = fminf(fmaxf(spoke, 0.f), corner);
+        if (spoke > corner * .5f)
+            spoke = corner - spoke;
+        const float sx = sinf(spoke), sy = cosf(spoke);
+        float cE[4];
+        pack_feathered_coverages(corner, sx, sy, outsetE, cE);
+        pack_feathered_coverages(corner, sx, sy, outset, r.c);
+        r.y0E = cE[3];
+    }
+    else
+    {
+        r.c[0] = 0.f;
+        r.c[1] = -2.f;
+        r.c[2] = 1e6f;
+        r.c[3] = 0.f;
+        r.y0E = .5f;
+    }
+    const float bx = M[0] * tv.x + M[2] * tv.y, by = M[1] * tv.x + M[3] * tv.y;
+    const float oE = outsetE * featherRadius, oT = outset * featherRadius;
+    const float px0 = oE * nx, py0 = oE * ny, px1 = oT * nx, py1 = oT * ny;
+    r.ex = bx + (M[0] * px0 + M[2] * py0) + pd.m[4];
+    r.ey = by + (M[1] * px0 + M[3] * py0) + pd.m[5];
+    r.tx = bx + (M[0] * px1 + M[2] * py1) + pd.m[4];
+    r.ty = by + (M[1] * px1 + M[3] * py1) + pd.m[5];
+    return r;
+}
+
 __device__ __forceinline__ int64_t orient64(int32_t ax, int32_t ay, int32_t bx, int32_t by, int32_t qx,
                                             int32_t qy)
 {
     return (int64_t)(bx - ax) * (int64_t)(qy - ay) - (int64_t)(by - ay) * (int64_t)(qx - ax);
+}
+
+// The bounding box of a primitive's points (a feathered segment's inner
+// points 2E - T included).
+__device__ __forceinline__ void prim_extent(const Prim& p, int32_t* minx, int32_t* miny, int32_t* maxx,
+                                            int32_t* maxy)
+{
+    *minx = min(min(p.p[0], p.p[2]), min(p.p[4], p.p[6]));
+    *maxx = max(max(p.p[0], p.p[2]), max(p.p[4], p.p[6]));
+    *miny = min(min(p.p[1], p.p[3]), min(p.p[5], p.p[7]));
+    *maxy = max(max(p.p[1], p.p[3]), max(p.p[5], p.p[7]));
+    if (p.flags & kPrimFeather)
+    {
+        // 2E - T for both ends: E is the midpoint of T and B, so the box of
+        // {E, T, B} is the box of {T, B} widened to 2E - T.
+        const int32_t b0x = 2 * p.p[0] - p.p[2], b0y = 2 * p.p[1] - p.p[3];
+        const int32_t b1x = 2 * p.p[4] - p.p[6], b1y = 2 * p.p[5] - p.p[7];
+        *minx = min(*minx, min(b0x, b1x));
+        *maxx = max(*maxx, max(b0x, b1x));
+        *miny = min(*miny, min(b0y, b1y));
+        *maxy = max(*maxy, max(b0y, b1y));
+    }
 }
 
 // A forward texel's two outset vertices, snapped, with stroke coverage.
@@ -594,6 +881,8 @@ __global__ void vertex_kernel(const TessTexel* __restrict__ tex, int texels, con
     const DrawTess d = draws[di];
     const int i = min((int)(d.forwardBase + (g - d.primBase)), texels - 2);
     const PathDataV pd = paths[d.pathID];
+    if (pd.featherRadius != 0.f && pd.strokeRadius == 0.f)
+        return; // feathered segments evaluate their ends in expand_kernel
     const uint32_t cfi = tex[i].w;
     const VsOut a0 = draw_vertex(tex, texels, i, cfi, -1.f, pd);
     const VsOut a1 = draw_vertex(tex, texels, i, cfi, +1.f, pd);
@@ -929,33 +1218,59 @@ __global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels, con
         cfj = (tex[j].w & ~MIRRORED_CONTOUR_CONTOUR_FLAG) | (cfi & MIRRORED_CONTOUR_CONTOUR_FLAG);
     }
     const bool stroke = pd.strokeRadius != 0.f;
-    // fill: [I0, O0, I1, O1]; stroke: [P0-, P0+, P1-, P1+]
-    const VtxPair va = verts[g];
-    VtxPair vb;
-    const int64_t rel = (int64_t)j - (int64_t)d.forwardBase;
-    if (rel >= 0 && rel < (int64_t)d.forwardCount && cfj == tex[j].w)
-        vb = verts[d.primBase + (uint32_t)rel];
+    const bool feathered = !stroke && pd.featherRadius != 0.f;
+    if (feathered)
+    {
+        // [E0, T0, E1, T1]; the inner points are 2E - T
+        const FeatherEnd a = feather_end(tex, texels, i, cfi, pd, cd.vertexIndex0);
+        const FeatherEnd b = feather_end(tex, texels, j, cfj, pd, cd.vertexIndex0);
+        out.p[0] = snap(a.ex);
+        out.p[1] = snap(a.ey);
+        out.p[2] = snap(a.tx);
+        out.p[3] = snap(a.ty);
+        out.p[4] = snap(b.ex);
+        out.p[5] = snap(b.ey);
+        out.p[6] = snap(b.tx);
+        out.p[7] = snap(b.ty);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+        {
+            out.cov[k] = a.c[k];
+            out.cov[4 + k] = b.c[k];
+        }
+        out.pad[0] = f2u(a.y0E);
+        out.pad[1] = f2u(b.y0E);
+    }
     else
     {
-        // next(i) outside this draw's forward texels, or with other flags
-        const VsOut b0 = draw_vertex(tex, texels, j, cfj, -1.f, pd);
-        const VsOut b1 = draw_vertex(tex, texels, j, cfj, +1.f, pd);
-        vb.p[0] = snap(b0.x);
-        vb.p[1] = snap(b0.y);
-        vb.p[2] = snap(b1.x);
-        vb.p[3] = snap(b1.y);
-        vb.cov[0] = b0.cx;
-        vb.cov[1] = b0.cy;
-        vb.cov[2] = b1.cx;
-        vb.cov[3] = b1.cy;
-    }
+        // fill: [I0, O0, I1, O1]; stroke: [P0-, P0+, P1-, P1+]
+        const VtxPair va = verts[g];
+        VtxPair vb;
+        const int64_t rel = (int64_t)j - (int64_t)d.forwardBase;
+        if (rel >= 0 && rel < (int64_t)d.forwardCount && cfj == tex[j].w)
+            vb = verts[d.primBase + (uint32_t)rel];
+        else
+        {
+            // next(i) outside this draw's forward texels, or with other flags
+            const VsOut b0 = draw_vertex(tex, texels, j, cfj, -1.f, pd);
+            const VsOut b1 = draw_vertex(tex, texels, j, cfj, +1.f, pd);
+            vb.p[0] = snap(b0.x);
+            vb.p[1] = snap(b0.y);
+            vb.p[2] = snap(b1.x);
+            vb.p[3] = snap(b1.y);
+            vb.cov[0] = b0.cx;
+            vb.cov[1] = b0.cy;
+            vb.cov[2] = b1.cx;
+            vb.cov[3] = b1.cy;
+        }
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-    {
-        out.p[k] = va.p[k];
-        out.p[4 + k] = vb.p[k];
-        out.cov[k] = va.cov[k];
-        out.cov[4 + k] = vb.cov[k];
+        for (int k = 0; k < 4; ++k)
+        {
+            out.p[k] = va.p[k];
+            out.p[4 + k] = vb.p[k];
+            out.cov[k] = va.cov[k];
+            out.cov[4 + k] = vb.cov[k];
+        }
     }
     uint32_t flags = stroke ? kPrimStroke : kPrimFill;
     if (!stroke && ((cfi & NEGATE_PATH_FILL_COVERAGE_FLAG) != 0u))
@@ -977,10 +1292,30 @@ __global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels, con
         lo = min(lo, out.p[k]);
         hi = max(hi, out.p[k]);
     }
+    if (feathered && in_guard(lo) && in_guard(hi))
+    {
+        // the inner points 2E - T must lie in the box as well
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+        {
+            const int32_t bxk = 2 * out.p[4 * k] - out.p[4 * k + 2], byk = 2 * out.p[4 * k + 1] - out.p[4 * k + 3];
+            lo = min(lo, min(bxk, byk));
+            hi = max(hi, max(bxk, byk));
+        }
+    }
+    if (feathered)
+    {
+        const bool j0 = (cfi & JOIN_TYPE_MASK) == FEATHER_JOIN_CONTOUR_FLAG;
+        const bool j1 = (cfj & JOIN_TYPE_MASK) == FEATHER_JOIN_CONTOUR_FLAG;
+        flags |= kPrimFeather | (j0 ? kPrimJoin0 : 0u) | (j1 ? kPrimJoin1 : 0u);
+    }
     if (!in_guard(lo) || !in_guard(hi))
     {
         // A vertex outside the guard box or without a finite position
         // (kSnapInvalid lies below the box): special_prims_kernel settles it.
+        // A feathered segment there is settled as a fill's border
+        // [E0, T0, E1, T1] without its feather.
+        flags &= ~(kPrimFeather | kPrimJoin0 | kPrimJoin1);
         flags |= kPrimSpecial;
         empty = true;
         specialList[atomicAdd(specialCount, 1u)] = g;
@@ -994,6 +1329,7 @@ __global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels, con
     }
     else
     {
+        // (a feathered segment's inner triangles mirror its outer ones)
         int64_t A1 = orient64(out.p[0], out.p[1], out.p[2], out.p[3], out.p[4], out.p[5]);
         int64_t A2 = orient64(out.p[4], out.p[5], out.p[2], out.p[3], out.p[6], out.p[7]);
         const bool noEdge = (flags & kPrimNoEdge) || (out.p[0] == out.p[4] && out.p[1] == out.p[5]);
@@ -1010,10 +1346,8 @@ __global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels, con
     int32_t tx0 = 0x7fffffff, ty0 = 0x7fffffff, tx1 = -0x7fffffff, ty1 = -0x7fffffff;
     if (!empty)
     {
-        const int32_t minx = min(min(out.p[0], out.p[2]), min(out.p[4], out.p[6]));
-        const int32_t maxx = max(max(out.p[0], out.p[2]), max(out.p[4], out.p[6]));
-        const int32_t miny = min(min(out.p[1], out.p[3]), min(out.p[5], out.p[7]));
-        const int32_t maxy = max(max(out.p[1], out.p[3]), max(out.p[5], out.p[7]));
+        int32_t minx, maxx, miny, maxy;
+        prim_extent(out, &minx, &miny, &maxx, &maxy);
         tx0 = minx >> (kSubpixelLog2 + kTileLog2);
         ty0 = miny >> (kSubpixelLog2 + kTileLog2);
         tx1 = (maxx >> (kSubpixelLog2 + kTileLog2)) + 1;
@@ -1225,10 +1559,8 @@ __global__ void cells_layout_kernel(PathCells* __restrict__ cells, uint32_t n,
 __device__ __forceinline__ void prim_tile_range(const Prim& p, const PathCells& c, int* tx0, int* ty0,
                                                 int* tx1, int* ty1)
 {
-    int32_t minx = min(min(p.p[0], p.p[2]), min(p.p[4], p.p[6]));
-    int32_t maxx = max(max(p.p[0], p.p[2]), max(p.p[4], p.p[6]));
-    int32_t miny = min(min(p.p[1], p.p[3]), min(p.p[5], p.p[7]));
-    int32_t maxy = max(max(p.p[1], p.p[3]), max(p.p[5], p.p[7]));
+    int32_t minx, miny, maxx, maxy;
+    prim_extent(p, &minx, &miny, &maxx, &maxy);
     *tx0 = max(minx >> (kSubpixelLog2 + kTileLog2), c.tx0);
     *ty0 = max(miny >> (kSubpixelLog2 + kTileLog2), c.ty0);
     *tx1 = min((maxx >> (kSubpixelLog2 + kTileLog2)) + 1, c.tx1);
@@ -1311,7 +1643,8 @@ __device__ __forceinline__ void for_prim_cells(const Prim& p, const PathCells& c
         vx[k] = (float)p.p[2 * k];
         vy[k] = (float)p.p[2 * k + 1];
     }
-    const bool trim = ty1 - ty0 > 1 && tx1 - tx0 > 1;
+    // (a feathered segment's six points are binned by their box)
+    const bool trim = ty1 - ty0 > 1 && tx1 - tx0 > 1 && !(p.flags & kPrimFeather);
     for (int ty = ty0; ty < ty1; ++ty)
     {
         int a = tx0, b = tx1 - 1;
@@ -1752,12 +2085,19 @@ template <typename I> __device__ __forceinline__ EdgeK edge_k(const EdgeTI<I>& e
 // the clipped edges bound it.
 __device__ __forceinline__ void prim_rows(const Prim& p, const TileGeom& g, int* r0, int* r1)
 {
-    const int32_t miny = min(min(p.p[1], p.p[3]), min(p.p[5], p.p[7]));
-    const int32_t maxy = max(max(p.p[1], p.p[3]), max(p.p[5], p.p[7]));
+    int32_t minx, miny, maxx, maxy;
+    prim_extent(p, &minx, &miny, &maxx, &maxy);
     const int32_t lo = miny - g.Y0 - (1 << (kSubpixelLog2 - 1));
     const int32_t hi = maxy - g.Y0 - (1 << (kSubpixelLog2 - 1));
     int a0 = max(-((-lo) >> kSubpixelLog2), 0);   // ceil(lo / 256)
     int a1 = min(hi >> kSubpixelLog2, kTile - 1); // floor(hi / 256)
+    if (p.flags & kPrimFeather)
+    {
+        // rows of the box of its six points
+        *r0 = a0;
+        *r1 = a1;
+        return;
+    }
 
     // tile-relative vertices: |coordinate| < 2^24, exact in float
     float vx[4], vy[4];
@@ -1926,6 +2266,135 @@ __device__ void fill_prim_row(const Prim& p, int j, const TileGeom& g, int32_t* 
             }
         }
     }
+}
+
+// eval_feathered_fill (draw_path_common.glsl:153-240): the feather above y0
+// plus a 4-sample midpoint integral of FEATHER(u) FEATHER'(t) across a
+// corner, signed by coverages.x.
+__device__ __forceinline__ float eval_feathered_fill(const float* c, const float* __restrict__ table)
+{
+    const float cotTheta = c[2];
+    const float y0 = fmaxf(c[3], 0.f);
+    float cov = cotTheta >= 0.f ? feather_table(table, y0) : 0.f;
+    if (fabsf(cotTheta) < 1e3f)
+    {
+        const float x = fabsf(c[0]) - .25f;
+        const float y = -c[1] - 2.f;
+        const float dt = (y - y0) * 0.5984134206f;
+        const float k[4] = {0.20888568955f, 0.62665706865f, 1.04442844776f, 1.46219982687f};
+        float sum = 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+        {
+            const float t = y0 + dt * k[i];
+            const float u = t * -cotTheta + (y * cotTheta + x);
+            const float t_ = t * 5.09593080173f + -2.54796540086f;
+            sum += feather_table(table, u) * exp2f(-t_ * t_);
+        }
+        cov += sum * dt;
+    }
+    return cov * signf_(c[0]);
+}
+
+// One sample row of one triangle of a feathered segment: the coverage
+// vectors interpolated at each pixel inside it, evaluated, and added with the
+// triangle's orientation (the fixed-point border units of the fill rows).
+__device__ __forceinline__ void feather_tri_row(int32_t ax, int32_t ay, int32_t bx, int32_t by, int32_t cx,
+                                                int32_t cy, const float* ca, const float* cb, const float* cc,
+                                                int neg, int j, const TileGeom& g, int32_t* __restrict__ sBorder,
+                                                const float* __restrict__ table)
+{
+    typedef double I;
+    const I A = orient_t<I>(ax, ay, bx, by, cx, cy);
+    if (A == 0)
+        return;
+    const int32_t X0 = g.X0, Y0 = g.Y0;
+    const int32_t j2 = 2 * j + 1;
+    const int32_t yq = Y0 + (j << kSubpixelLog2) + (1 << (kSubpixelLog2 - 1));
+    int ilo, ihi;
+    tri_row_candidates(ax, ay, bx, by, cx, cy, yq, X0, &ilo, &ihi);
+    if (ilo > ihi)
+        return;
+    const int o = A > 0 ? 1 : -1;
+    const EdgeTI<I> eab = make_edge_t<I>(ax, ay, bx, by, X0, Y0);
+    const EdgeTI<I> ebc = make_edge_t<I>(bx, by, cx, cy, X0, Y0);
+    const EdgeTI<I> eca = make_edge_t<I>(cx, cy, ax, ay, X0, Y0);
+    const EdgeK kab = edge_k<I>(eab, o), kbc = edge_k<I>(ebc, o), kca = edge_k<I>(eca, o);
+    const float inv = 1.f / (float)A;
+    const int32_t i2 = 2 * ilo + 1;
+    int32_t vab = kab.A * i2 + kab.B * j2, vbc = kbc.A * i2 + kbc.B * j2, vca = kca.A * i2 + kca.B * j2;
+    int32_t wab = eab.A * i2 + eab.B * j2, wbc = ebc.A * i2 + ebc.B * j2, wca = eca.A * i2 + eca.B * j2;
+    const float dab = (float)eab.D0, dbc = (float)ebc.D0, dca = (float)eca.D0;
+    for (int i = ilo; i <= ihi; ++i)
+    {
+        if (vab >= kab.thr && vbc >= kbc.thr && vca >= kca.thr)
+        {
+            // barycentrics: the edge opposite each vertex over the area
+            const float la = (dbc + 128.f * (float)wbc) * inv;
+            const float lb = (dca + 128.f * (float)wca) * inv;
+            const float lc = (dab + 128.f * (float)wab) * inv;
+            float c4[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                c4[q] = la * ca[q] + lb * cb[q] + lc * cc[q];
+            const float v = fminf(fmaxf(eval_feathered_fill(c4, table), -1.f), 1.f);
+            atomicAdd(&sBorder[j * kTile + i], neg * o * __float2int_rn(v * 65536.f));
+        }
+        vab += 2 * kab.A;
+        vbc += 2 * kbc.A;
+        vca += 2 * kca.A;
+        wab += 2 * eab.A;
+        wbc += 2 * ebc.A;
+        wca += 2 * eca.A;
+    }
+}
+
+// One sample row of a feathered segment [E0, T0, E1, T1]: the edge's crossing
+// column and its four border triangles (the inner points are 2E - T).  Kept
+// out of line, reading the primitive itself, so the fill and stroke rows keep
+// their registers.
+__device__ __noinline__ void feather_prim_row(const Prim* __restrict__ prims, uint32_t idx, int j, int32_t X0,
+                                              int32_t Y0, int32_t* __restrict__ sDelta,
+                                              int32_t* __restrict__ sBorder, int32_t deltaUnit,
+                                              const float* __restrict__ table)
+{
+    Prim p = prims[idx];
+    const TileGeom g{X0, Y0};
+    {
+        Prim edge = p;
+        edge.flags |= kPrimNoT1 | kPrimNoT2;
+        fill_prim_row<double>(edge, j, g, sDelta, sBorder, deltaUnit);
+    }
+    const int neg = (p.flags & kPrimNegate) ? -1 : 1;
+    float cE0[4] = {.5f, -2.f, 1e6f, .5f}, cT0[4] = {0.f, -2.f, 1e6f, 0.f};
+    float cE1[4] = {.5f, -2.f, 1e6f, .5f}, cT1[4] = {0.f, -2.f, 1e6f, 0.f};
+    if (p.flags & kPrimJoin0)
+    {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            cT0[q] = p.cov[q];
+        cE0[0] = .75f;
+        cE0[1] = -2.5f;
+        cE0[2] = cT0[2];
+        cE0[3] = u2f(p.pad[0]);
+    }
+    if (p.flags & kPrimJoin1)
+    {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            cT1[q] = p.cov[4 + q];
+        cE1[0] = .75f;
+        cE1[1] = -2.5f;
+        cE1[2] = cT1[2];
+        cE1[3] = u2f(p.pad[1]);
+    }
+    const int32_t e0x = p.p[0], e0y = p.p[1], t0x = p.p[2], t0y = p.p[3];
+    const int32_t e1x = p.p[4], e1y = p.p[5], t1x = p.p[6], t1y = p.p[7];
+    const int32_t b0x = 2 * e0x - t0x, b0y = 2 * e0y - t0y, b1x = 2 * e1x - t1x, b1y = 2 * e1y - t1y;
+    feather_tri_row(e0x, e0y, t0x, t0y, e1x, e1y, cE0, cT0, cE1, neg, j, g, sBorder, table);
+    feather_tri_row(e1x, e1y, t0x, t0y, t1x, t1y, cE1, cT0, cT1, neg, j, g, sBorder, table);
+    feather_tri_row(e0x, e0y, b0x, b0y, e1x, e1y, cE0, cT0, cE1, neg, j, g, sBorder, table);
+    feather_tri_row(e1x, e1y, b0x, b0y, b1x, b1y, cE1, cT0, cT1, neg, j, g, sBorder, table);
 }
 
 template <typename I>
@@ -2519,7 +2988,8 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
     const PathCells* __restrict__ cells, const PathRaster* __restrict__ raster,
     const PathPaint* __restrict__ pathPaint, const uint2* __restrict__ cellPacked, const uint32_t* __restrict__ binned, const Prim* __restrict__ prims,
     const uint32_t* __restrict__ ramps, uint32_t gradHeight, const ImageDesc* __restrict__ images,
-    const CanvasDev* __restrict__ canvases, uint32_t canvasCount, uint32_t dither)
+    const CanvasDev* __restrict__ canvases, uint32_t canvasCount, uint32_t dither,
+    const float* __restrict__ featherTable)
 {
     __shared__ __align__(16) int32_t sAcc[kAccArrays][kTile * kTile];
     __shared__ int32_t sLeft[kAccArrays][kTile];
@@ -2748,9 +3218,13 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                         const uint32_t inf = sItemInfo[owner];
                         const int a = (int)(inf & 15u);
                         const int row = (int)((inf >> 8) & 15u) + (int)(it - (inf >> 16));
-                        const Prim pm = prims[sItemPrim[owner]];
+                        const uint32_t pIdx = sItemPrim[owner];
+                        const Prim pm = prims[pIdx];
                         if (inf & 32u)
                             stroke_prim_row<double>(pm, row, g, reinterpret_cast<uint32_t*>(sAcc[a]));
+                        else if (pm.flags & kPrimFeather)
+                            feather_prim_row(prims, pIdx, row, g.X0, g.Y0, sAcc[a], sAcc[a + ((inf >> 4) & 1u)],
+                                             (inf & 16u) ? 1 : (1 << 24), featherTable);
                         else if (inf & 16u)
                             fill_prim_row<double>(pm, row, g, sAcc[a], sAcc[a + 1], 1);
                         else

@@ -464,19 +464,37 @@ void RiveRenderer::drawPath(RenderPath* renderPath, RenderPaint* renderPaint)
         return;
     if (paint->getIsStroked() && !(paint->getThickness() > 0))
         return;
-    if (!(paint->getFeather() >= 0) || paint->getFeather() != 0)
-        return; // feathered paints are not supported by this backend yet
+    if (!(paint->getFeather() >= 0))
+        return;
     if (st.overallClipPixelBounds.empty())
         return;
-    float ox = 0, oy = 0;
-    if (paint->getIsStroked())
+    // Feathers (rive_renderer.cpp:164-184): only clockwise fills are
+    // feathered; a fill whose feather spans more than a pixel on screen draws
+    // a copy of its path softened for the feather.
+    const float feather = paint->getFeather();
+    if (feather != 0 && !paint->getIsStroked())
     {
-        // PathDraw::Make stroke outset (draw.cpp:452-494)
-        float outset = paint->getThickness() * .5f;
-        if (paint->getJoin() == StrokeJoin::miter)
-            outset *= 4.f;
-        else if (paint->getCap() == StrokeCap::square)
-            outset *= 1.41421356f;
+        if (path->getFillRule() != FillRule::clockwise)
+            return;
+        const float matrixMaxScale = st.matrix.findMaxScale();
+        if (feather * matrixMaxScale > 1)
+            path = m_context->keepForFrame(path->makeSoftenedCopyForFeathering(feather, matrixMaxScale));
+    }
+    float ox = 0, oy = 0;
+    if (paint->getIsStroked() || feather != 0)
+    {
+        // PathDraw::Make stroke and feather outset (draw.cpp:452-494)
+        float outset = 0;
+        if (paint->getIsStroked())
+        {
+            outset = paint->getThickness() * .5f;
+            if (paint->getJoin() == StrokeJoin::miter)
+                outset *= 4.f;
+            else if (paint->getCap() == StrokeCap::square)
+                outset *= 1.41421356f;
+        }
+        if (feather != 0)
+            outset += feather * (3.f / 2); // find_feather_radius
         Mat2D lin(st.matrix[0], st.matrix[1], st.matrix[2], st.matrix[3], 0, 0);
         Vec2D c[4] = {{0, 0}, {outset, 0}, {0, outset}, {outset, outset}};
         float l = INFINITY, t = INFINITY, r = -INFINITY, b = -INFINITY;
@@ -556,6 +574,7 @@ void RenderContext::beginFrame(const FrameDescriptor& fd)
     m_frame = fd;
     m_batches.clear();
     m_modulatedGradients.clear();
+    m_framePaths.clear();
     m_canvases.clear();
     m_provLastClipID = 0;
     m_provContentSerial = 0;
@@ -607,8 +626,13 @@ DrawRecord RenderContext::makeRecord(const RiveRenderPath* path, const Mat2D& m,
     r.residentPoint = path->residentPoint();
     const bool stroke = paint != nullptr && paint->getIsStroked();
     float strokeRadius = stroke ? fmaxf(paint->getThickness() * .5f, 1.17549435e-38f) : 0.f;
+    // find_feather_radius (draw.cpp:313-318): two standard deviations wide,
+    // the radius covers three.
+    const float featherRadius =
+        !clipUpdate && paint != nullptr && paint->getFeather() != 0 ? paint->getFeather() * (3.f / 2) : 0.f;
     memcpy(r.path.matrix, m.values(), sizeof(r.path.matrix));
     r.path.stroke_radius = strokeRadius;
+    r.path.feather_radius = featherRadius;
     // PaintData (gpu.cpp:879-939)
     uint32_t params;
     if (clipUpdate)
@@ -682,16 +706,29 @@ DrawRecord RenderContext::makeRecord(const RiveRenderPath* path, const Mat2D& m,
     r.paint.params = params;
     // PathDraw's per-draw constants (draw.cpp:561-825)
     rive_hip_draw& d = r.draw;
-    d.is_stroke = stroke ? 1 : 0;
+    d.is_stroke = stroke ? 1 : (featherRadius != 0 ? 2 : 0);
     d.join = stroke ? (uint32_t)paint->getJoin() : 0;
     d.cap = stroke ? (uint32_t)paint->getCap() : 0;
     memcpy(d.matrix, m.values(), sizeof(d.matrix));
     d.stroke_radius = strokeRadius;
     d.parametric_precision = 4.f;
+    if (stroke || featherRadius != 0)
+    {
+        // initForMidpointFan (draw.cpp:776-889): strokes and feathers take
+        // polar segments for their radius on screen, a feather's capped where
+        // its segments reach pi/16; wide feathers tessellate coarser.
+        d.matrix_max_scale = m.findMaxScale();
+        float r_ = 0;
+        if (featherRadius != 0)
+            r_ = std::min(featherRadius * d.matrix_max_scale, 1 / (8 * (1 - 0.99518472667f)));
+        if (stroke)
+            r_ += strokeRadius * d.matrix_max_scale;
+        d.polar_segments_per_radian = polar_segments_per_radian(r_);
+        if (featherRadius > 1)
+            d.parametric_precision = std::min(4.f * 100.f / (featherRadius * d.matrix_max_scale), 4.f);
+    }
     if (stroke)
     {
-        d.matrix_max_scale = m.findMaxScale();
-        d.polar_segments_per_radian = polar_segments_per_radian(strokeRadius * d.matrix_max_scale);
         d.contour_directions = 0; // forward
     }
     else if (fillRule == FillRule::clockwise)
@@ -823,6 +860,7 @@ bool RenderContext::flush(rive_hip_target* target, uint32_t maxFlushes, uint32_t
         if (flushCount != nullptr)
             *flushCount = 1;
         m_modulatedGradients.clear();
+    m_framePaths.clear();
         return ok;
     }
 
@@ -942,6 +980,7 @@ bool RenderContext::flush(rive_hip_target* target, uint32_t maxFlushes, uint32_t
     bool ok = m_impl->flushFrame(descs);
     m_batches.clear();
     m_modulatedGradients.clear();
+    m_framePaths.clear();
     m_provLastClipID = 0;
     m_provContentSerial = 0;
     m_provClipIDs.clear();

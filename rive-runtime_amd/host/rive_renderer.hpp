@@ -214,6 +214,9 @@ public:
     }
     int64_t residentVerb() const { return m_residentVerb; }
     uint32_t residentPoint() const { return m_residentPoint; }
+    // A copy chopped wherever its curves turn by an angle set by the feather's
+    // size on screen (rive_render_path.cpp:323-449).
+    std::shared_ptr<RiveRenderPath> makeSoftenedCopyForFeathering(float feather, float matrixMaxScale) const;
 
 private:
     RawPath m_raw;
@@ -450,6 +453,13 @@ public:
     DrawRecord makeRecord(const RiveRenderPath* path, const Mat2D& m, const RiveRenderPaint* paint, uint32_t clipID,
                           bool clipUpdate, uint32_t outerClipID, FillRule fillRule,
                           const Mat2D* clipRectInverse = nullptr, float opacity = 1.f);
+    // Keeps a path the frame's draws point at (a softened copy) alive until
+    // the frame is flushed.
+    RiveRenderPath* keepForFrame(std::shared_ptr<RiveRenderPath> path)
+    {
+        m_framePaths.push_back(std::move(path));
+        return m_framePaths.back().get();
+    }
     // A draw and the clip stack it sits under; clip draws and clip IDs are
     // assigned per logical flush when the frame is flushed (applyClip,
     // rive_renderer.cpp:631-824).
@@ -474,8 +484,10 @@ private:
     };
     std::vector<CanvasRecord> m_canvases;
     uint64_t m_lastClipSerial = 0;
-    // Opacity-modulated copies of gradients, alive until the frame is flushed.
+    // Opacity-modulated copies of gradients and softened copies of feathered
+    // paths, alive until the frame is flushed.
     std::vector<std::shared_ptr<Gradient>> m_modulatedGradients;
+    std::vector<std::shared_ptr<RiveRenderPath>> m_framePaths;
     // applyClip's bookkeeping as draws arrive (for getClipContentID): clip
     // IDs handed out so far and the element whose content is current.
     uint32_t m_provLastClipID = 0;

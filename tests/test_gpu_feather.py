@@ -1,0 +1,232 @@
+"""Feathered fills on the device against the oracle (row a12).
+
+* Layout: a feathered fill is counted like a stroke with feather joins, from
+  a path the front end softened for the feather; the device's draw ranges,
+  contours and spans equal the oracle's bit for bit.
+* Pixels (GEOMETRY mode, the whole pipeline) and SPANS mode (the oracle's own
+  records through the device's tessellation, expansion and raster) agree
+  with the oracle within the tolerance below.
+* The device's feathered edge and corners follow the Gaussian closed forms
+  the oracle is pinned to.
+
+Tolerance: the feather integrand is nonlinear, so a one-subpixel difference
+in where a vertex lands, or a cotangent interpolated on the other side of the
+corner/flat threshold, moves a pixel by a few code values: within 6/255 on
+all but 0.5% of pixels and within 24/255 everywhere.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.special import ndtr
+
+import oracle_binding as ob
+import rivehip
+from streams import CLOCKWISE, NONZERO, StreamBuilder, rect
+
+pytestmark = pytest.mark.gpu
+
+TOL, BAD_FRAC, MAX_DIFF = 6, 5e-3, 24
+
+
+def announce(tag):
+    print(f"[feather] {tag}", flush=True)
+
+
+def pixel_check(out, ref, tol=TOL, bad_frac=BAD_FRAC, max_diff=MAX_DIFF):
+    d = np.abs(out.astype(np.int32) - ref.astype(np.int32)).max(axis=2)
+    bad = float((d > tol).mean())
+    assert bad <= bad_frac and int(d.max()) <= max_diff, f"max diff {int(d.max())}, fraction > {tol}: {bad:.5f}"
+
+
+def ellipse(cx, cy, rx, ry):
+    k = 0.5522847498
+    return [(0, cx + rx, cy), (4, cx + rx, cy + k * ry, cx + k * rx, cy + ry, cx, cy + ry),
+            (4, cx - k * rx, cy + ry, cx - rx, cy + k * ry, cx - rx, cy),
+            (4, cx - rx, cy - k * ry, cx - k * rx, cy - ry, cx, cy - ry),
+            (4, cx + k * rx, cy - ry, cx + rx, cy - k * ry, cx + rx, cy), (5,)]
+
+
+def star(cx, cy, r0, r1, n):
+    pts = []
+    for i in range(2 * n):
+        r = r0 if i % 2 == 0 else r1
+        a = math.pi * i / n
+        pts.append((cx + r * math.cos(a), cy + r * math.sin(a)))
+    return [(0,) + pts[0]] + [(1,) + p for p in pts[1:]] + [(5,)]
+
+
+def blob(rng, cx, cy, r):
+    # a closed loop of cubics with random control points (may self-intersect)
+    n = int(rng.integers(3, 7))
+    ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+    pts = [(cx + r * rng.uniform(.5, 1) * math.cos(a), cy + r * rng.uniform(.5, 1) * math.sin(a)) for a in ang]
+    cmds = [(0,) + pts[0]]
+    for i in range(n):
+        p0, p3 = pts[i], pts[(i + 1) % n]
+        c1 = (p0[0] + rng.uniform(-r, r) * .5, p0[1] + rng.uniform(-r, r) * .5)
+        c2 = (p3[0] + rng.uniform(-r, r) * .5, p3[1] + rng.uniform(-r, r) * .5)
+        cmds.append((4,) + c1 + c2 + p3)
+    cmds.append((5,))
+    return cmds
+
+
+def matrix(rng, cx, cy, mirror):
+    a = rng.uniform(0, 2 * np.pi)
+    sx, sy = rng.uniform(.6, 1.6, 2)
+    if mirror:
+        sx = -sx
+    c, s = math.cos(a), math.sin(a)
+    # about (cx, cy)
+    m = np.array([[c * sx, -s * sy], [s * sx, c * sy]])
+    t = np.array([cx, cy]) - m @ np.array([cx, cy])
+    return (m[0, 0], m[1, 0], m[0, 1], m[1, 1], t[0], t[1])
+
+
+def random_scene(seed, w=384, h=320, n=24):
+    rng = np.random.default_rng(seed)
+    b = StreamBuilder(w, h, clear_color=0xff202020)
+    for i in range(n):
+        cx, cy = rng.uniform(0, w), rng.uniform(0, h)
+        r = rng.uniform(12, 80)
+        kind = int(rng.integers(0, 4))
+        if kind == 0:
+            cmds = rect(cx - r, cy - r * .6, cx + r, cy + r * .6)
+        elif kind == 1:
+            cmds = ellipse(cx, cy, r, r * rng.uniform(.3, 1))
+        elif kind == 2:
+            cmds = star(cx, cy, r, r * .45, int(rng.integers(3, 9)))
+        else:
+            cmds = blob(rng, cx, cy, r)
+        feather = float(rng.choice([0.0, .4, 1.5, 4.0, 9.0, 16.0])) if i % 5 else 0.0
+        rule = CLOCKWISE if feather or rng.random() < .5 else NONZERO
+        p = b.path(cmds, fill_rule=rule)
+        color = (int(rng.integers(120, 256)) << 24) | int(rng.integers(0, 1 << 24))
+        b.draw(p, color=color, matrix=matrix(rng, cx, cy, mirror=bool(rng.random() < .3)), feather=feather)
+    return b
+
+
+def _oracle(b):
+    s = b.build()
+    f = ob.OracleFlush(s)
+    return s, f
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_feathered_layout_bit_exact(hip_ctx, seed):
+    announce(f"layout seed {seed}")
+    b = random_scene(seed)
+    s, f = _oracle(b)
+    t = hip_ctx.target(b.width, b.height)
+    try:
+        assert f.flush_count == 1
+        hip_ctx.render_stream(s, t)
+        hip_ctx.sync()
+        spans = hip_ctx.readback(0).view(rivehip.TESS_SPAN)
+        contours = hip_ctx.readback(1).view(rivehip.CONTOUR)
+        draws = hip_ctx.readback(3).view(rivehip.DRAW)
+        od = f.draws()
+        assert len(draws) == len(od)
+        assert np.array_equal(draws["path_id"], od[:, 0])
+        assert np.array_equal(draws["tess_location"], od[:, 1])
+        assert np.array_equal(draws["tess_vertex_count"], od[:, 2])
+        ocont = f.get(1)
+        assert len(contours) == len(ocont) and contours.tobytes() == ocont.tobytes()
+        ospans = f.get(0)
+        assert len(spans) == len(ospans)
+        mism = np.nonzero(spans.view(np.uint32).reshape(-1, 16) != ospans.view(np.uint32).reshape(-1, 16))
+        assert mism[0].size == 0, f"first mismatching span {mism[0][:4]}, fields {mism[1][:4]}"
+        # feather joins were laid out
+        assert np.any((spans["contour_id_with_flags"] & (7 << 26)) == (1 << 26))
+    finally:
+        hip_ctx.free_target(t)
+        f.close()
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_feathered_scene_pixels(hip_ctx, seed):
+    announce(f"pixels seed {seed}")
+    b = random_scene(seed)
+    s, f = _oracle(b)
+    ref, _ = f.render()
+    t = hip_ctx.target(b.width, b.height)
+    try:
+        hip_ctx.render_stream(s, t)
+        out = hip_ctx.read_pixels(t, b.width, b.height)
+        f.flush_spans_on(hip_ctx, t, clear_color=0xff202020, dither=0)
+        spans_out = hip_ctx.read_pixels(t, b.width, b.height)
+    finally:
+        hip_ctx.free_target(t)
+        f.close()
+    pixel_check(out, ref)
+    pixel_check(spans_out, ref)
+
+
+def test_feathered_edge_and_corner_closed_forms(hip_ctx):
+    announce("closed forms")
+    feather, sig = 8.0, 4.0
+    b = StreamBuilder(256, 256, clear_color=0xff000000)
+    b.draw(b.path(rect(64, 64, 600, 600), fill_rule=CLOCKWISE), color=0xffffffff, feather=feather)
+    s = b.build()
+    t = hip_ctx.target(256, 256)
+    try:
+        hip_ctx.render_stream(s, t)
+        out = hip_ctx.read_pixels(t, 256, 256)[..., 0].astype(np.float64) / 255
+    finally:
+        hip_ctx.free_target(t)
+    xs = np.arange(30, 110)
+    np.testing.assert_allclose(out[200, xs], ndtr((xs + .5 - 64) / sig), atol=6e-3)
+    yy, xx = np.mgrid[52:80, 52:80]
+    want = ndtr((xx + .5 - 64) / sig) * ndtr((yy + .5 - 64) / sig)
+    assert np.abs(out[52:80, 52:80] - want).max() < 1e-2
+    # a concave corner, mirrored through a left-handed matrix (the fill is
+    # then drawn forward-then-reverse with its coverage negated)
+    b = StreamBuilder(256, 256, clear_color=0xff000000)
+    m = (-1.0, 0.0, 0.0, 1.0, 400.0, 0.0)  # x -> 400 - x: the L's corner lands at (200, 200)
+    # clockwise in the path's own coordinates (the fill rule's sense: a
+    # left-handed matrix reverses the drawing so it stays filled)
+    ell = [(0, 150, 150), (1, 250, 150), (1, 250, 200), (1, 200, 200), (1, 200, 250), (1, 150, 250), (5,)]
+    b.draw(b.path(ell, fill_rule=CLOCKWISE), color=0xffffffff, feather=feather, matrix=m)
+    s, f = _oracle(b)
+    ref, _ = f.render()
+    t = hip_ctx.target(256, 256)
+    try:
+        hip_ctx.render_stream(s, t)
+        out2 = hip_ctx.read_pixels(t, 256, 256)
+    finally:
+        hip_ctx.free_target(t)
+        f.close()
+    pixel_check(out2, ref)
+    cov = ref[..., 0].astype(np.float64) / 255
+    assert cov[175, 175] > .99 and cov[225, 225] > .99 and cov[225, 175] < .01
+    # the L covers x in [150, 250] x y in [150, 200] and [200, 250] x [200, 250]
+    # after the mirror: its concave corner at (200, 200) opens toward -x, +y
+    yy, xx = np.mgrid[188:212, 188:212]
+    want = 1 - ndtr((200 - (xx + .5)) / sig) * ndtr((yy + .5 - 200) / sig)
+    assert np.abs(cov[188:212, 188:212] - want).max() < 1e-2
+
+
+def test_feathercusp_and_mixed_draws(hip_ctx):
+    # feather_test.cpp:13-47 (a zero-area double cusp, feather 32), between
+    # plain fills, a stroke and a feathered fill under a clip
+    announce("feathercusp")
+    b = StreamBuilder(400, 400, clear_color=0xff000000)
+    cusp = b.path([(0, 95.164070, 299.999939),
+                   (4, -125.766922, 299.999939, 274.233032, 299.999939, -95.164070, 299.999939)],
+                  fill_rule=CLOCKWISE)
+    b.draw(b.path(rect(20, 20, 200, 120)), color=0xff3366cc)
+    b.draw(cusp, color=0xffffffff, feather=32.0 / 2)
+    b.draw(b.path(ellipse(250, 250, 90, 60), fill_rule=CLOCKWISE), color=0xccff8800, feather=6.0)
+    b.draw(b.path(star(120, 280, 70, 30, 5)), color=0xff00ff00, stroke=6.0)
+    clip = b.path(rect(200, 40, 380, 200))
+    b.draw(b.path(ellipse(290, 120, 100, 70), fill_rule=CLOCKWISE), color=0xffff00ff, feather=10.0, clips=(clip,))
+    s, f = _oracle(b)
+    ref, _ = f.render()
+    t = hip_ctx.target(400, 400)
+    try:
+        hip_ctx.render_stream(s, t)
+        out = hip_ctx.read_pixels(t, 400, 400)
+    finally:
+        hip_ctx.free_target(t)
+        f.close()
+    pixel_check(out, ref)
