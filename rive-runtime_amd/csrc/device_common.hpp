@@ -99,6 +99,9 @@ constexpr uint32_t kPrimSpecial = 256u;
 constexpr uint32_t kPrimFeather = 512u;
 constexpr uint32_t kPrimJoin0 = 1024u;
 constexpr uint32_t kPrimJoin1 = 2048u;
+// A feathered stroke's primitive: its coverage x is biased by -2 (x < -1.5
+// marks it) and evaluates to 1 - FEATHER(1 - x) - FEATHER(1 - y).
+constexpr uint32_t kPrimFeatherStroke = 4096u;
 
 // The Gaussian-integral function tables of feathering (gpu.cpp:1991-2181):
 // 512 fp16 texels each, held as floats, FEATHER first, INVERSE_FEATHER

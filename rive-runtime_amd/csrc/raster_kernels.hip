@@ -554,6 +554,13 @@ __device__ VsOut draw_vertex(const TessTexel* __restrict__ tex, int texels, int 
     float nx = sinf(theta), ny = -cosf(theta);
     float offx, offy, covx = 0.f, covy = 0.f;
     float strokeRadius = pd.strokeRadius;
+    float featherRadius = pd.featherRadius;
+    if (featherRadius != 0.f)
+    {
+        // never harder than 1.5 standard deviations across 1/2 px (:478-485)
+        const float mx = M[0] * nx + M[2] * ny, my = M[1] * nx + M[3] * ny;
+        featherRadius = fmaxf(featherRadius, 1.f / sqrtf(mx * mx + my * my));
+    }
     if (strokeRadius != 0.f)
     {
         float det = M[0] * M[3] - M[2] * M[1];
@@ -562,9 +569,10 @@ __device__ VsOut draw_vertex(const TessTexel* __restrict__ tex, int texels, int 
             outset = fminf(outset, 0.f);
         if (cf & RIGHT_JOIN_CONTOUR_FLAG)
             outset = fmaxf(outset, 0.f);
-        float aaRadius = manhattan_pixel_width(M, nx, ny) * AA_RADIUS;
+        // a feathered stroke's AA ramp is its feather (draw_path_common.glsl:499-509)
+        float aaRadius = featherRadius != 0.f ? featherRadius : manhattan_pixel_width(M, nx, ny) * AA_RADIUS;
         float globalCoverage = 1.f;
-        if (aaRadius > strokeRadius)
+        if (aaRadius > strokeRadius && featherRadius == 0.f)
         {
             globalCoverage = strokeRadius / aaRadius;
             strokeRadius = aaRadius;
@@ -636,6 +644,8 @@ __device__ VsOut draw_vertex(const TessTexel* __restrict__ tex, int texels, int 
         covx *= globalCoverage;
         covy *= globalCoverage;
         covy = fmaxf(covy, 1e-4f);
+        if (featherRadius != 0.f)
+            covx = -2.f - covx; // FEATHER_COVERAGE_BIAS marks a feathered stroke
         float px_ = outset * vox, py_ = outset * voy;
         offx = M[0] * px_ + M[2] * py_;
         offy = M[1] * px_ + M[3] * py_;
@@ -1303,6 +1313,8 @@ __global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels, con
             hi = max(hi, max(bxk, byk));
         }
     }
+    if (stroke && pd.featherRadius != 0.f)
+        flags |= kPrimFeatherStroke;
     if (feathered)
     {
         const bool j0 = (cfi & JOIN_TYPE_MASK) == FEATHER_JOIN_CONTOUR_FLAG;
@@ -2397,8 +2409,20 @@ __device__ __noinline__ void feather_prim_row(const Prim* __restrict__ prims, ui
     feather_tri_row(e1x, e1y, b0x, b0y, b1x, b1y, cE1, cT0, cT1, neg, j, g, sBorder, table);
 }
 
-template <typename I>
-__device__ void stroke_prim_row(const Prim& p, int j, const TileGeom& g, uint32_t* __restrict__ sMax)
+// find_stroke_coverage (draw_path.vert:500-508): min(x, y), or for a
+// feathered stroke (x biased below -1.5) 1 - FEATHER(1 - x) - FEATHER(1 - y)
+// (eval_feathered_stroke, draw_path_common.glsl:242-258).
+template <bool FEATHER>
+__device__ __forceinline__ float stroke_coverage(float cx, float cy, const float* __restrict__ table)
+{
+    if (FEATHER && cx < -1.5f)
+        return 1.f - feather_table(table, 3.f + cx) - feather_table(table, 1.f - cy);
+    return fminf(cx, cy);
+}
+
+template <typename I, bool FEATHER = false>
+__device__ void stroke_prim_row(const Prim& p, int j, const TileGeom& g, uint32_t* __restrict__ sMax,
+                                const float* __restrict__ table = nullptr)
 {
     const int32_t X0 = g.X0, Y0 = g.Y0;
     const int32_t j2 = 2 * j + 1;
@@ -2437,7 +2461,7 @@ __device__ void stroke_prim_row(const Prim& p, int j, const TileGeom& g, uint32_
             {
                 if (k01 >= t01 && k12 >= t12 && k20 >= t20)
                 {
-                    const float v = fminf(cx, cy);
+                    const float v = stroke_coverage<FEATHER>(cx, cy, table);
                     if (v > 0.f)
                         atomicMax(&sMax[j * kTile + i], __float_as_uint(v));
                 }
@@ -2476,7 +2500,7 @@ __device__ void stroke_prim_row(const Prim& p, int j, const TileGeom& g, uint32_
             {
                 if (-k12 >= t21 && k13 >= t13 && k32 >= t32)
                 {
-                    const float v = fminf(cx, cy);
+                    const float v = stroke_coverage<FEATHER>(cx, cy, table);
                     if (v > 0.f)
                         atomicMax(&sMax[j * kTile + i], __float_as_uint(v));
                 }
@@ -2488,6 +2512,15 @@ __device__ void stroke_prim_row(const Prim& p, int j, const TileGeom& g, uint32_
             }
         }
     }
+}
+
+// A feathered stroke's sample row, out of line like feather_prim_row.
+__device__ __noinline__ void feather_stroke_prim_row(const Prim* __restrict__ prims, uint32_t idx, int j,
+                                                     int32_t X0, int32_t Y0, uint32_t* __restrict__ sMax,
+                                                     const float* __restrict__ table)
+{
+    const Prim p = prims[idx];
+    stroke_prim_row<double, true>(p, j, TileGeom{X0, Y0}, sMax, table);
 }
 
 // Coverage of a winding count under the path's fill rule (draw_path.frag
@@ -3221,7 +3254,13 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                         const uint32_t pIdx = sItemPrim[owner];
                         const Prim pm = prims[pIdx];
                         if (inf & 32u)
-                            stroke_prim_row<double>(pm, row, g, reinterpret_cast<uint32_t*>(sAcc[a]));
+                        {
+                            if (pm.flags & kPrimFeatherStroke)
+                                feather_stroke_prim_row(prims, pIdx, row, g.X0, g.Y0,
+                                                        reinterpret_cast<uint32_t*>(sAcc[a]), featherTable);
+                            else
+                                stroke_prim_row<double>(pm, row, g, reinterpret_cast<uint32_t*>(sAcc[a]));
+                        }
                         else if (pm.flags & kPrimFeather)
                             feather_prim_row(prims, pIdx, row, g.X0, g.Y0, sAcc[a], sAcc[a + ((inf >> 4) & 1u)],
                                              (inf & 16u) ? 1 : (1 << 24), featherTable);

@@ -230,3 +230,39 @@ def test_feathercusp_and_mixed_draws(hip_ctx):
         hip_ctx.free_target(t)
         f.close()
     pixel_check(out, ref)
+
+
+def test_feathered_strokes(hip_ctx):
+    # Feathered strokes with every join and cap against the oracle, and a
+    # straight one against the convolved band Phi((d+h)/s) - Phi((d-h)/s).
+    announce("strokes")
+    b = StreamBuilder(384, 320, clear_color=0xff101010)
+    b.draw(b.path([(0, -50.0, 40.0), (1, 450.0, 40.0)]), color=0xffffffff, stroke=8.0, feather=6.0)
+    rng = np.random.default_rng(7)
+    joins, caps = [0, 1, 2], [0, 1, 2]
+    for i in range(12):
+        cx, cy = rng.uniform(40, 344), rng.uniform(90, 300)
+        cmds = star(cx, cy, rng.uniform(20, 60), rng.uniform(8, 20), int(rng.integers(3, 7)))
+        if i % 3 == 0:
+            cmds = cmds[:-1]  # open: caps
+        elif i % 3 == 1:
+            cmds = blob(rng, cx, cy, rng.uniform(20, 60))
+        b.draw(b.path(cmds), color=(0xd0 << 24) | int(rng.integers(0, 1 << 24)), stroke=float(rng.uniform(1, 14)),
+               join=joins[i % 3], cap=caps[(i // 3) % 3], feather=float(rng.choice([.5, 2.0, 5.0, 12.0])),
+               matrix=matrix(rng, cx, cy, mirror=bool(i % 4 == 0)))
+    s, f = _oracle(b)
+    ref, _ = f.render()
+    t = hip_ctx.target(b.width, b.height)
+    try:
+        hip_ctx.render_stream(s, t)
+        out = hip_ctx.read_pixels(t, b.width, b.height)
+    finally:
+        hip_ctx.free_target(t)
+        f.close()
+    pixel_check(out, ref)
+    cov = out[..., 0].astype(np.float64) / 255
+    ys = np.arange(20, 61)
+    d = ys + .5 - 40
+    want = ndtr((d + 4) / 3) - ndtr((d - 4) / 3)
+    # the background is 0x10 (0.063); the white stroke blends over it
+    np.testing.assert_allclose(cov[ys, 200], want + (1 - want) * 16 / 255, atol=8e-3)

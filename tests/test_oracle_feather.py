@@ -301,3 +301,19 @@ def test_feathercusp_renders():
     b.draw(p, color=0xffffffff, feather=32.0)
     cov = _render(b)
     assert np.isfinite(cov).all()
+
+
+def test_feathered_stroke_profile_is_a_convolved_band():
+    # A straight stroke of half-width h feathered by f is the band |d| <= h
+    # convolved with a Gaussian of standard deviation f/2 across it:
+    # Phi((d + h)/sigma) - Phi((d - h)/sigma) (draw_path_common.glsl:242-258
+    # on the stroke's linear coverages, aaRadius = the feather radius).
+    for width, feather in [(6.0, 4.0), (20.0, 8.0), (2.0, 6.0)]:
+        h, sig = width / 2, feather / 2
+        b = StreamBuilder(256, 128, clear_color=0xff000000)
+        b.draw(b.path([(0, -100.0, 64.0), (1, 400.0, 64.0)]), color=0xffffffff, stroke=width, feather=feather)
+        cov = _render(b)
+        ys = np.arange(30, 98)
+        d = ys + .5 - 64
+        want = ndtr((d + h) / sig) - ndtr((d - h) / sig)
+        np.testing.assert_allclose(cov[ys, 128], want, atol=5e-3)
