@@ -18,6 +18,14 @@ struct DrawInfo
     Dirs dirs;
     bool isStroke;
     bool featheredFill = false;
+    // A feather drawn through the atlas (draw.cpp:387-418): its coverage is
+    // rendered into its own atlas region (aw x ah texels, atlas coordinate =
+    // pixel * atlasScale + atlasTranslate) and blitted over the pixel bounds
+    // [blitL, blitR) x [blitT, blitB) with bilinear filtering.
+    bool atlas = false;
+    float atlasScale = 0, atlasTx = 0, atlasTy = 0;
+    uint32_t aw = 0, ah = 0;
+    int blitL = 0, blitT = 0, blitR = 0, blitB = 0;
 };
 
 // An image with its mip chain (2x2 box filter with rounding), premultiplied

@@ -382,6 +382,17 @@ struct Builder
         uint32_t row, col;
     };
     std::vector<PendingPaint> pendingGradPaints;
+    // Feather-atlas draws of the current logical flush: their tessellation
+    // is written after the main draws' (render_context.cpp:2255-2345).
+    struct PendingAtlas
+    {
+        PathDrawDesc d;
+        std::vector<uint8_t> verbs;
+        std::vector<V2> pts;
+        size_t drawIndex;
+        bool stroke, scissored;
+    };
+    std::vector<PendingAtlas> pendingAtlas;
 
     Builder(const rive_stream& s_, std::vector<Flush>& out) : s(s_), flushes(out) {}
     Flush& cur() { return *fp; }
@@ -407,7 +418,8 @@ struct Builder
                           const Mat& matrix, float strokeRadius, Join join, Cap cap, FillRule fillRule,
                           uint32_t paintType, uint32_t paintValue, uint32_t clipID, uint32_t outerClipID,
                           const Gradient* grad, uint32_t blendMode, uint32_t image, uint32_t sampler,
-                          const Mat* clipRectInverse = nullptr, float featherRadius = 0)
+                          const Mat* clipRectInverse = nullptr, float featherRadius = 0,
+                          const IAABB* atlasBounds = nullptr)
     {
         PathDrawDesc d;
         d.verbs = verbs;
@@ -421,6 +433,14 @@ struct Builder
         d.fillRule = fillRule;
         d.featherRadius = featherRadius;
         derive_directions(&d);
+        const bool atlas = atlasBounds != nullptr;
+        if (atlas && strokeRadius == 0)
+        {
+            // Atlas fills are drawn one way (draw.cpp:659-680): forward, or
+            // reverse under a left-handed matrix.
+            const float det = matrix.m[0] * matrix.m[3] - matrix.m[1] * matrix.m[2];
+            d.dirs = det < 0 ? Dirs::reverse : Dirs::forward;
+        }
 
         uint32_t pathID = (uint32_t)cur().pathData.size(); // record 0 is reserved
         d.pathID = pathID;
@@ -431,6 +451,27 @@ struct Builder
         memcpy(pd.matrix, matrix.m, sizeof(pd.matrix));
         pd.stroke_radius = strokeRadius;
         pd.feather_radius = featherRadius;
+        float atlasScale = 0, atlasTx = 0, atlasTy = 0;
+        uint32_t aw = 0, ah = 0;
+        IAABB visible{0, 0, 0, 0};
+        if (atlas)
+        {
+            // The atlas region (draw.cpp:1441-1486): the visible part of the
+            // pixel bounds at scale 16 / max(r * maxScale, 16), placed with 2
+            // texels of padding on every side (where in the atlas it lands
+            // does not change the result: the placement is integral).
+            visible = atlasBounds->intersect(IAABB{0, 0, (int)s.width, (int)s.height});
+            atlasScale = 16.f / std::max(featherRadius * find_max_scale(matrix), 16.f);
+            const uint32_t w = (uint32_t)ceilf((float)(visible.r - visible.l) * atlasScale);
+            const uint32_t h = (uint32_t)ceilf((float)(visible.b - visible.t) * atlasScale);
+            aw = w + 4;
+            ah = h + 4;
+            atlasTx = 2.f - (float)visible.l * atlasScale;
+            atlasTy = 2.f - (float)visible.t * atlasScale;
+            pd.atlas_transform[0] = atlasScale;
+            pd.atlas_transform[1] = atlasTx;
+            pd.atlas_transform[2] = atlasTy;
+        }
         pd.z_index = 0;
         cur().pathData.push_back(pd);
 
@@ -531,6 +572,38 @@ struct Builder
         di.dirs = d.dirs;
         di.isStroke = strokeRadius > 0;
         di.featheredFill = strokeRadius == 0 && featherRadius > 0;
+        if (atlas)
+        {
+            di.atlas = true;
+            di.atlasScale = atlasScale;
+            di.atlasTx = atlasTx;
+            di.atlasTy = atlasTy;
+            di.aw = aw;
+            di.ah = ah;
+            di.blitL = atlasBounds->l;
+            di.blitT = atlasBounds->t;
+            di.blitR = atlasBounds->r;
+            di.blitB = atlasBounds->b;
+            cur().draws.push_back(di);
+            cur().pathTessVertexCounts.push_back(pt.midpointFanTessVertexCount);
+            if (pt.midpointFanTessVertexCount > 0)
+            {
+                resPaths += 1;
+                resContours += pt.contours.size();
+                resTess += pt.midpointFanTessVertexCount;
+            }
+            // Its tessellation comes after the main draws'.
+            PendingAtlas pa;
+            pa.d = d;
+            pa.verbs.assign(verbs, verbs + verbCount);
+            pa.pts.assign(pts, pts + ptCount);
+            pa.drawIndex = cur().draws.size() - 1;
+            pa.stroke = strokeRadius > 0;
+            pa.scissored = !(visible.l == atlasBounds->l && visible.t == atlasBounds->t &&
+                             visible.r == atlasBounds->r && visible.b == atlasBounds->b);
+            pendingAtlas.push_back(std::move(pa));
+            return;
+        }
         cur().draws.push_back(di);
         for (size_t c = 0; c < pt.contours.size(); ++c)
             cur().contours.push_back(pt.contours[c]);
@@ -815,9 +888,40 @@ struct Builder
         pendingGradPaints.clear();
     }
 
+    // pushFeatherAtlasTessellation for every atlas draw of the flush, fills
+    // then strokes, unscissored then scissored (render_context.cpp:2255-2345).
+    void writeAtlasTessellation()
+    {
+        Flush& f = cur();
+        for (bool stroked : {false, true})
+            for (bool scissored : {false, true})
+                for (PendingAtlas& pa : pendingAtlas)
+                {
+                    if (pa.stroke != stroked || pa.scissored != scissored)
+                        continue;
+                    pa.d.verbs = pa.verbs.data();
+                    pa.d.pts = pa.pts.data();
+                    PathTess pt = tessellate_path_records(pa.d, nextContourID, tessLocation);
+                    DrawInfo& di = f.draws[pa.drawIndex];
+                    di.tessLocation = tessLocation;
+                    di.tessVertexCount = pt.midpointFanTessVertexCount;
+                    for (const ContourData& c : pt.contours)
+                        f.contours.push_back(c);
+                    nextContourID += (uint32_t)pt.contours.size();
+                    for (const CubicRecord& r : pt.records)
+                    {
+                        f.records.push_back(r);
+                        write_spans(r, pa.d.dirs, &f.spans);
+                    }
+                    tessLocation += pt.midpointFanTessVertexCount;
+                }
+        pendingAtlas.clear();
+    }
+
     void endFlush()
     {
         Flush& f = cur();
+        writeAtlasTessellation();
         finishGradients();
         f.pathCount = (uint32_t)f.pathData.size() - 1;
         f.pathImage.resize(f.pathCount + 1, 0);
@@ -1039,12 +1143,34 @@ struct Builder
                 strokeRadius = fmaxf(d.thickness * .5f, 1.17549435e-38f);
             Mat crInv;
             const bool hasCR = cr.inverse(&crInv);
-            const Needs needs = measure(dverbs, dverbCount, dpts, dptCount, m, strokeRadius, (Join)d.join,
-                                        (Cap)d.cap, (FillRule)sp.fill_rule, featherRadius);
+            // SelectCoverageType (draw.cpp:387-418): a feather is drawn
+            // through the atlas once it can render at half resolution or less.
+            const bool atlas =
+                featherRadius != 0 && 16.f / std::max(featherRadius * find_max_scale(m), 16.f) <= .5f;
+            Needs needs = measure(dverbs, dverbCount, dpts, dptCount, m, strokeRadius, (Join)d.join, (Cap)d.cap,
+                                  (FillRule)sp.fill_rule, featherRadius);
+            if (atlas && !isStroke)
+            {
+                // counted one way (forward or reverse) rather than both
+                PathDrawDesc one;
+                one.verbs = dverbs;
+                one.verbCount = dverbCount;
+                one.pts = dpts;
+                one.ptCount = dptCount;
+                one.matrix = m;
+                one.fillRule = (FillRule)sp.fill_rule;
+                one.featherRadius = featherRadius;
+                derive_directions(&one);
+                one.dirs = Dirs::forward;
+                one.pathID = 1;
+                PathTess pt = tessellate_path_records(one, 1, 0);
+                needs.tess = pt.midpointFanTessVertexCount;
+            }
             clipAndPush(height, frame, needs, gp, [&](uint32_t drawClipID) {
                 pushGeometryDraw(dverbs, dverbCount, dpts, dptCount, m, strokeRadius, (Join)d.join, (Cap)d.cap,
                                  (FillRule)sp.fill_rule, paintType, paintValue, drawClipID, 0, gp,
-                                 shader_blend_mode(d.blend_mode), 0, 0, hasCR ? &crInv : nullptr, featherRadius);
+                                 shader_blend_mode(d.blend_mode), 0, 0, hasCR ? &crInv : nullptr, featherRadius,
+                                 atlas ? &pixelBounds : nullptr);
             });
         }
         endFlush();
@@ -1851,6 +1977,95 @@ void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* 
     std::array<uint16_t, 120> caaIdx;
     const auto caaPatch = make_midpoint_fan_center_aa_patch(&caaIdx);
 
+    // The feather atlas (render_atlas.glsl): every atlas draw's triangles,
+    // placed by its atlas transform, rendered into its own region of an R16F
+    // texture -- fills add signed coverage with no culling (additive blend,
+    // the sign from the triangle's facing), strokes keep the maximum of
+    // clockwise triangles (max blend) -- each blend rounding to fp16
+    // (gpu.hpp:2076-2080, pipeline_manager_vulkan.cpp:69).
+    std::vector<std::vector<float>> atlases(f.draws.size());
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < (int)f.draws.size(); ++k)
+    {
+        const DrawInfo& di = f.draws[(size_t)k];
+        if (!di.atlas || di.tessVertexCount == 0)
+            continue;
+        std::vector<float>& A = atlases[(size_t)k];
+        A.assign((size_t)di.aw * di.ah, 0.f);
+        const PatchVertexDef* patch = di.featheredFill ? caaPatch.data() : fanPatch.data();
+        const uint16_t* idx = di.featheredFill ? caaIdx.data() : fanIdx.data();
+        const int NV = di.featheredFill ? 74 : 42, NT = di.featheredFill ? 40 : 24;
+        const int firstInstance = (int)(di.tessLocation / 8);
+        const int nInst = (int)((di.tessLocation + di.tessVertexCount) / 8) - firstInstance;
+        std::vector<OutVertex> iv((size_t)NV);
+        for (int inst = 0; inst < nInst; ++inst)
+        {
+            for (int v = 0; v < NV; ++v)
+            {
+                OutVertex o = draw_vertex(f, tex, patch[v], firstInstance + inst);
+                o.x = o.x * di.atlasScale + di.atlasTx; // render_atlas.glsl:35-37
+                o.y = o.y * di.atlasScale + di.atlasTy;
+                iv[(size_t)v] = o;
+            }
+            for (int t = 0; t < NT; ++t)
+            {
+                const OutVertex* V[3] = {&iv[idx[t * 3 + 0]], &iv[idx[t * 3 + 1]], &iv[idx[t * 3 + 2]]};
+                if (V[0]->discard || V[1]->discard || V[2]->discard)
+                    continue;
+                bool ok = true;
+                for (int q = 0; q < 3; ++q)
+                    ok = ok && snappable(V[q]->x) && snappable(V[q]->y);
+                if (!ok)
+                    continue;
+                FxP a{snap(V[0]->x), snap(V[0]->y)}, b{snap(V[1]->x), snap(V[1]->y)}, c{snap(V[2]->x), snap(V[2]->y)};
+                int64_t area = edgeD(a, b, c);
+                if (area == 0 || (di.isStroke && area < 0))
+                    continue;
+                float sign = 1.f;
+                if (area < 0)
+                {
+                    // a back-facing fill triangle subtracts; test it clockwise
+                    std::swap(b, c);
+                    std::swap(V[1], V[2]);
+                    area = -area;
+                    sign = -1.f;
+                }
+                int64_t minx = std::min({a.x, b.x, c.x}), maxx = std::max({a.x, b.x, c.x});
+                int64_t miny = std::min({a.y, b.y, c.y}), maxy = std::max({a.y, b.y, c.y});
+                int x0 = (int)std::max<int64_t>(0, (minx - 128) >> 8);
+                int x1 = (int)std::min<int64_t>((int64_t)di.aw - 1, (maxx - 128 + 255) >> 8);
+                int y0 = (int)std::max<int64_t>(0, (miny - 128) >> 8);
+                int y1 = (int)std::min<int64_t>((int64_t)di.ah - 1, (maxy - 128 + 255) >> 8);
+                const double inv = 1.0 / (double)area;
+                for (int y = y0; y <= y1; ++y)
+                    for (int x = x0; x <= x1; ++x)
+                    {
+                        FxP q{(int64_t)x * 256 + 128, (int64_t)y * 256 + 128};
+                        if (sos_sign(a, b, q) <= 0 || sos_sign(b, c, q) <= 0 || sos_sign(c, a, q) <= 0)
+                            continue;
+                        double la = (double)edgeD(b, c, q) * inv;
+                        double lb = (double)edgeD(c, a, q) * inv;
+                        double lc = (double)edgeD(a, b, q) * inv;
+                        float c4[4];
+                        for (int r = 0; r < 4; ++r)
+                            c4[r] = (float)(la * V[0]->c[r] + lb * V[1]->c[r] + lc * V[2]->c[r]);
+                        float& texel = A[(size_t)y * di.aw + x];
+                        if (di.isStroke)
+                        {
+                            float fc = c4[0] < FEATHER_COVERAGE_THRESHOLD ? eval_feathered_stroke(c4)
+                                                                          : std::min(c4[0], c4[1]);
+                            texel = std::max(texel, round_to_f16(fc));
+                        }
+                        else
+                        {
+                            float fc = c4[1] < FEATHER_COVERAGE_THRESHOLD ? eval_feathered_fill(c4) : c4[0];
+                            texel = round_to_f16(texel + sign * fc);
+                        }
+                    }
+            }
+        }
+    }
+
     // Rows are split into bands, one thread per band at a time: a pixel's
     // triangles still accumulate in submission order, so the result does not
     // depend on the thread count.
@@ -1859,10 +2074,50 @@ void render_flush(const Flush& f, const std::vector<TessVertex>& tex, uint32_t* 
     std::vector<OutVertex> ov;
     std::vector<int> instRows;
 
-    for (const DrawInfo& di : f.draws)
+    for (size_t drawIdx = 0; drawIdx < f.draws.size(); ++drawIdx)
     {
+        const DrawInfo& di = f.draws[drawIdx];
         if (di.tessVertexCount == 0)
             continue;
+        if (di.atlas)
+        {
+            // The blit (draw_mesh.frag:88-99): over the pixel bounds, the
+            // atlas sampled with linear filtering at pixel * scale +
+            // translate, clamped to [0, 1].
+            const std::vector<float>& A = atlases[drawIdx];
+            const uint32_t pathID = di.pathID;
+#pragma omp parallel for schedule(dynamic, 1)
+            for (int band = 0; band < bands; ++band)
+            {
+                const int yb0 = std::max((int)((int64_t)band * H / bands), di.blitT);
+                const int yb1 = std::min((int)((int64_t)(band + 1) * H / bands), di.blitB) - 1;
+                std::vector<uint32_t>& touched = touchedBands[band];
+                touched.clear();
+                for (int y = yb0; y <= yb1; ++y)
+                    for (int x = std::max(di.blitL, 0); x < std::min(di.blitR, (int)W); ++x)
+                    {
+                        const float u = ((float)x + .5f) * di.atlasScale + di.atlasTx - .5f;
+                        const float v = ((float)y + .5f) * di.atlasScale + di.atlasTy - .5f;
+                        const float fu = floorf(u), fv = floorf(v);
+                        const float wu = u - fu, wv = v - fv;
+                        auto texel = [&](int i, int j) {
+                            i = std::min(std::max(i, 0), (int)di.aw - 1);
+                            j = std::min(std::max(j, 0), (int)di.ah - 1);
+                            return A[(size_t)j * di.aw + i];
+                        };
+                        const int i0 = (int)fu, j0 = (int)fv;
+                        const float top = texel(i0, j0) * (1.f - wu) + texel(i0 + 1, j0) * wu;
+                        const float bot = texel(i0, j0 + 1) * (1.f - wu) + texel(i0 + 1, j0 + 1) * wu;
+                        const float sample = top * (1.f - wv) + bot * wv;
+                        const size_t p = (size_t)y * W + x;
+                        stamp[p] = pathID;
+                        touched.push_back((uint32_t)p);
+                        cov[p] = std::min(std::max(sample, 0.f), 1.f);
+                    }
+                resolve_touched(f, ramps, pathID, di.isStroke, touched, cov, clipCov, clipIDs, rgba, edgeMask);
+            }
+            continue;
+        }
         const uint32_t pathID = di.pathID;
         const bool isStroke = di.isStroke;
         const int firstInstance = (int)(di.tessLocation / 8);

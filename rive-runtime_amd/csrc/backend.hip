@@ -914,13 +914,24 @@ __global__ void span_draws_kernel(uint32_t pathCount, const uint32_t* __restrict
     DrawTess t{};
     t.pathID = path;
     t.isStroke = paths[path].strokeRadius != 0.f ? 1u : 0u;
+    const bool featheredFill = !t.isStroke && paths[path].featherRadius != 0.f;
     const uint32_t nf = fwdMax[path] > 0u ? fwdMax[path] - fwdMin[path] : 0u;
     const uint32_t nm = mirMax[path] > 0u ? mirMax[path] - mirMin[path] : 0u;
     t.forwardBase = nf != 0u ? fwdMin[path] : 0u;
     t.forwardCount = nf;
     t.tessVertexCount = nf + nm;
     bool ok;
-    if (nm != 0u)
+    if (featheredFill && nf == 0u && nm != 0u)
+    {
+        // a feathered fill through the atlas under a left-handed matrix:
+        // reverse only, read back to front
+        t.dirs = 1u;
+        t.tessLocation = mirMin[path];
+        t.forwardBase = mirMin[path];
+        t.forwardCount = nm;
+        ok = true;
+    }
+    else if (nm != 0u)
     {
         // a double-sided fill: reverseThenForward when the mirrored run comes
         // first (draw.cpp:1899-1990)
@@ -932,7 +943,8 @@ __global__ void span_draws_kernel(uint32_t pathCount, const uint32_t* __restrict
     {
         t.dirs = 0u;
         t.tessLocation = t.forwardBase;
-        ok = t.isStroke != 0u; // raster-ordering fills are double-sided
+        // raster-ordering fills are double-sided, atlas feathers one way
+        ok = t.isStroke != 0u || featheredFill;
     }
     // the batch holding the path's first patch supplies its image paint
     const uint32_t patch = t.tessLocation / 8u;
@@ -1298,16 +1310,17 @@ __global__ void draw_tess_kernel(const rive_hip_draw* __restrict__ draws, uint32
                     t.forwardBase = d.tess_location + d.tess_vertex_count / 2;
                     t.forwardCount = d.tess_vertex_count / 2;
                     break;
-                default: // reverse-only draws exist only in MSAA/atlas modes
+                default: // reverse only: a feathered fill through the atlas
                     t.forwardBase = d.tess_location;
-                    t.forwardCount = 0;
-                    if (d.tess_vertex_count != 0)
+                    t.forwardCount = d.is_stroke == 2u ? d.tess_vertex_count : 0u;
+                    if (d.is_stroke != 2u && d.tess_vertex_count != 0)
                         atomicAdd(unsupported, 1u);
                     break;
             }
-            // A fill drawn forward only would need positive-winding-only
-            // coverage; rasterOrdering fills are always double sided.
-            if (!d.is_stroke && d.contour_directions == 0 && d.tess_vertex_count != 0)
+            // A plain fill drawn forward only would need positive-winding-only
+            // coverage; rasterOrdering fills are always double sided (a
+            // feathered fill through the atlas is drawn one way).
+            if (d.is_stroke == 0u && d.contour_directions == 0 && d.tess_vertex_count != 0)
                 atomicAdd(unsupported, 1u);
             // Reject records that would index outside the path or texel arrays.
             if (d.path_id == 0 || d.path_id > pathCount ||

@@ -1202,7 +1202,12 @@ __global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels, con
     if (g >= totalPrims)
         return;
     const DrawTess d = draws[di];
-    const int i = min((int)(d.forwardBase + (g - d.primBase)), texels - 2);
+    // A reverse-only draw (a feathered fill through the atlas under a
+    // left-handed matrix) holds its contours back to front: read it from the
+    // end, so each segment is its forward one.
+    const bool reversed = d.dirs == 1u;
+    const int i = reversed ? max((int)(d.forwardBase + d.forwardCount - 1u - (g - d.primBase)), 1)
+                           : min((int)(d.forwardBase + (g - d.primBase)), texels - 2);
     Prim out;
     out.path = d.pathID;
     out.pad[0] = out.pad[1] = 0;
@@ -1218,7 +1223,7 @@ __global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels, con
     const PathDataV pd = paths[d.pathID];
     // next(i): the next vertex of the contour, wrapping closed contours to
     // vertexIndex0 (draw_path_common.glsl:336-388).
-    int j = i + 1;
+    int j = reversed ? i - 1 : i + 1;
     uint32_t cfj = tex[j].w;
     if ((cfj & (MIRRORED_CONTOUR_CONTOUR_FLAG | 0xffffu)) !=
         (cfi & (MIRRORED_CONTOUR_CONTOUR_FLAG | 0xffffu)))
@@ -1428,7 +1433,8 @@ __global__ void special_prims_kernel(const TessTexel* __restrict__ tex, int texe
             anyBad |= out.p[c] == kSnapInvalid;
         if (anyBad && !stroke && !(flags & kPrimNoEdge))
         {
-            const int i = min((int)(d.forwardBase + (g - d.primBase)), texels - 2);
+            const int i = d.dirs == 1u ? max((int)(d.forwardBase + d.forwardCount - 1u - (g - d.primBase)), 1)
+                                       : min((int)(d.forwardBase + (g - d.primBase)), texels - 2);
             const uint32_t cid = min(max(tex[i].w & CONTOUR_ID_MASK, 1u), max(contourCount, 1u));
             const ContourV cd = contours[cid - 1];
             const PathDataV pd = paths[d.pathID];

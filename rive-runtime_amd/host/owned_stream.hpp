@@ -59,6 +59,8 @@ struct OwnedStream
     std::vector<rive_stream_draw> draws;
     std::vector<uint32_t> stopColors;
     std::vector<float> stopOffsets;
+    std::vector<std::vector<uint8_t>> imagePixels; // premultiplied RGBA8
+    std::vector<rive_stream_image> images;
 
     uint32_t addPath(Rand& r, uint32_t cubics, bool close, uint32_t fillRule, float W, float H)
     {
@@ -110,6 +112,10 @@ struct OwnedStream
         s.stop_colors = stopColors.data();
         s.stop_offsets = stopOffsets.data();
         s.stop_count = (uint32_t)stopColors.size();
+        for (size_t i = 0; i < images.size(); ++i)
+            images[i].rgba_premul = imagePixels[i].data();
+        s.images = images.empty() ? nullptr : images.data();
+        s.image_count = (uint32_t)images.size();
     }
 };
 

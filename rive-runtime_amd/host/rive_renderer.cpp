@@ -523,10 +523,31 @@ void RiveRenderer::drawPath(RenderPath* renderPath, RenderPaint* renderPaint)
         const ClipElement& ce = m_clipStack[i];
         clips.push_back({ce.serial, ce.path, ce.matrix, ce.fillRule, !ce.pixelBounds.intersect(frame).empty()});
     }
-    m_context->pushDrawBatch(std::move(clips),
-                             m_context->makeRecord(path, st.matrix, paint, 0, false, 0, path->getFillRule(),
-                                                   st.hasClipRect ? &st.clipRectInverse : nullptr,
-                                                   st.modulatedOpacity));
+    gpu::DrawRecord rec = m_context->makeRecord(path, st.matrix, paint, 0, false, 0, path->getFillRule(),
+                                           st.hasClipRect ? &st.clipRectInverse : nullptr, st.modulatedOpacity);
+    const float featherRadius = feather * (3.f / 2);
+    if (feather != 0 && 16.f / std::max(featherRadius * st.matrix.findMaxScale(), 16.f) <= .5f)
+    {
+        // SelectCoverageType (draw.cpp:387-418): once the feather can render
+        // at half resolution or less the reference draws it through an
+        // offscreen atlas.  Its PathData carries the atlas transform (the
+        // visible pixel bounds at scale 16 / (r * maxScale), 2 texels of
+        // padding) and its fill is tessellated one way (draw.cpp:659-680).
+        const IAABB visible = pixelBounds.intersect(frame);
+        const float scale = 16.f / std::max(featherRadius * st.matrix.findMaxScale(), 16.f);
+        rec.path.atlas_transform[0] = scale;
+        rec.path.atlas_transform[1] = 2.f - (float)visible.l * scale;
+        rec.path.atlas_transform[2] = 2.f - (float)visible.t * scale;
+        rec.atlas = true;
+        rec.atlasScissored = !(visible.l == pixelBounds.l && visible.t == pixelBounds.t &&
+                               visible.r == pixelBounds.r && visible.b == pixelBounds.b);
+        if (!paint->getIsStroked())
+        {
+            const Mat2D& m = st.matrix;
+            rec.draw.contour_directions = m[0] * m[3] - m[2] * m[1] < 0 ? 1u : 0u; // reverse : forward
+        }
+    }
+    m_context->pushDrawBatch(std::move(clips), rec);
 }
 
 namespace gpu
@@ -1107,6 +1128,25 @@ FlushDescriptor RenderContext::layoutFlush(std::vector<DrawRecord>& draws, rive_
         fd.paints.push_back(r.paint);
         fd.paintAux.push_back(r.aux);
         fd.draws.push_back(r.draw);
+    }
+    if (canvases == nullptr)
+    {
+        // Atlas feathers are tessellated after the flush's other draws:
+        // fills then strokes, unscissored then scissored
+        // (render_context.cpp:2255-2345).  Their path IDs stay in submission
+        // order; only the tessellation order moves.
+        std::vector<rive_hip_draw> ordered;
+        ordered.reserve(fd.draws.size());
+        for (size_t i = 0; i < draws.size(); ++i)
+            if (!draws[i].atlas)
+                ordered.push_back(fd.draws[i]);
+        for (int stroked = 0; stroked < 2; ++stroked)
+            for (int scissored = 0; scissored < 2; ++scissored)
+                for (size_t i = 0; i < draws.size(); ++i)
+                    if (draws[i].atlas && (int)(draws[i].draw.is_stroke == 1) == stroked &&
+                        (int)draws[i].atlasScissored == scissored)
+                        ordered.push_back(fd.draws[i]);
+        fd.draws.swap(ordered);
     }
     // Gradient spans (render_context.cpp:1462-1533).
     const uint32_t simpleRows = (uint32_t)((simple.size() * 2 + 511) / 512);

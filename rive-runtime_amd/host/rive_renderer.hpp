@@ -359,6 +359,10 @@ struct DrawRecord
     const RiveRenderImage* image;
     int64_t residentVerb = -1; // the path's resident geometry, if any
     uint32_t residentPoint = 0;
+    // A feather the reference draws through its offscreen atlas
+    // (draw.cpp:387-418): tessellated one way, after the flush's other draws;
+    // scissored when its pixel bounds leave the render target.
+    bool atlas = false, atlasScissored = false;
 };
 
 // One element of the clip stack a draw sits under (rive_renderer.hpp

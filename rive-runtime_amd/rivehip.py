@@ -224,7 +224,7 @@ def check(rc):
         raise RuntimeError(f"rive_hip error {rc}: {lib().rive_hip_last_error().decode()}")
 
 
-CONFIG_SEEDS = {1: 0x5249564500000001, 2: 0x5249564500000002, 4: 0x5249564500000004}
+CONFIG_SEEDS = {1: 0x5249564500000001, 2: 0x5249564500000002, 3: 0x5249564500000003, 4: 0x5249564500000004}
 
 
 def generate_stream(config, seed=None, width=0, height=0, draw_count=0):

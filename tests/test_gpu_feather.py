@@ -83,7 +83,7 @@ def matrix(rng, cx, cy, mirror):
     return (m[0, 0], m[1, 0], m[0, 1], m[1, 1], t[0], t[1])
 
 
-def random_scene(seed, w=384, h=320, n=24):
+def random_scene(seed, w=384, h=320, n=24, feathers=(0.0, .4, 1.5, 4.0, 9.0, 16.0)):
     rng = np.random.default_rng(seed)
     b = StreamBuilder(w, h, clear_color=0xff202020)
     for i in range(n):
@@ -98,7 +98,7 @@ def random_scene(seed, w=384, h=320, n=24):
             cmds = star(cx, cy, r, r * .45, int(rng.integers(3, 9)))
         else:
             cmds = blob(rng, cx, cy, r)
-        feather = float(rng.choice([0.0, .4, 1.5, 4.0, 9.0, 16.0])) if i % 5 else 0.0
+        feather = float(rng.choice(feathers)) if i % 5 else 0.0
         rule = CLOCKWISE if feather or rng.random() < .5 else NONZERO
         p = b.path(cmds, fill_rule=rule)
         color = (int(rng.integers(120, 256)) << 24) | int(rng.integers(0, 1 << 24))
@@ -112,6 +112,28 @@ def _oracle(b):
     return s, f
 
 
+def check_layout(hip_ctx, f):
+    """The device's draws (in its tessellation order: atlas feathers last),
+    contours and spans against the oracle's (draws in path order)."""
+    spans = hip_ctx.readback(0).view(rivehip.TESS_SPAN)
+    contours = hip_ctx.readback(1).view(rivehip.CONTOUR)
+    draws = hip_ctx.readback(3).view(rivehip.DRAW)
+    od = f.draws()
+    assert len(draws) == len(od)
+    dv = draws[np.argsort(draws["path_id"], kind="stable")]
+    oo = od[np.argsort(od[:, 0], kind="stable")]
+    assert np.array_equal(dv["path_id"], oo[:, 0])
+    assert np.array_equal(dv["tess_location"], oo[:, 1])
+    assert np.array_equal(dv["tess_vertex_count"], oo[:, 2])
+    ocont = f.get(1)
+    assert len(contours) == len(ocont) and contours.tobytes() == ocont.tobytes()
+    ospans = f.get(0)
+    assert len(spans) == len(ospans)
+    mism = np.nonzero(spans.view(np.uint32).reshape(-1, 16) != ospans.view(np.uint32).reshape(-1, 16))
+    assert mism[0].size == 0, f"first mismatching span {mism[0][:4]}, fields {mism[1][:4]}"
+    return spans
+
+
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_feathered_layout_bit_exact(hip_ctx, seed):
     announce(f"layout seed {seed}")
@@ -122,25 +144,56 @@ def test_feathered_layout_bit_exact(hip_ctx, seed):
         assert f.flush_count == 1
         hip_ctx.render_stream(s, t)
         hip_ctx.sync()
-        spans = hip_ctx.readback(0).view(rivehip.TESS_SPAN)
-        contours = hip_ctx.readback(1).view(rivehip.CONTOUR)
-        draws = hip_ctx.readback(3).view(rivehip.DRAW)
-        od = f.draws()
-        assert len(draws) == len(od)
-        assert np.array_equal(draws["path_id"], od[:, 0])
-        assert np.array_equal(draws["tess_location"], od[:, 1])
-        assert np.array_equal(draws["tess_vertex_count"], od[:, 2])
-        ocont = f.get(1)
-        assert len(contours) == len(ocont) and contours.tobytes() == ocont.tobytes()
-        ospans = f.get(0)
-        assert len(spans) == len(ospans)
-        mism = np.nonzero(spans.view(np.uint32).reshape(-1, 16) != ospans.view(np.uint32).reshape(-1, 16))
-        assert mism[0].size == 0, f"first mismatching span {mism[0][:4]}, fields {mism[1][:4]}"
+        spans = check_layout(hip_ctx, f)
         # feather joins were laid out
         assert np.any((spans["contour_id_with_flags"] & (7 << 26)) == (1 << 26))
     finally:
         hip_ctx.free_target(t)
         f.close()
+
+
+# Feathers wide enough on screen (radius * max scale >= 32 px) that the
+# reference draws them through its quarter-resolution atlas: laid out one way
+# after the flush's other draws, the oracle renders the atlas and its bilinear
+# blit, the device evaluates the same coverage at every pixel.
+ATLAS_FEATHERS = (0.0, 6.0, 22.0, 30.0, 40.0)
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_atlas_feather_layout_bit_exact(hip_ctx, seed):
+    announce(f"atlas layout seed {seed}")
+    b = random_scene(seed, feathers=ATLAS_FEATHERS)
+    s, f = _oracle(b)
+    t = hip_ctx.target(b.width, b.height)
+    try:
+        hip_ctx.render_stream(s, t)
+        hip_ctx.sync()
+        check_layout(hip_ctx, f)
+        # atlas draws are tessellated after the others
+        od = f.draws()
+        assert np.any(np.diff(od[:, 1].astype(np.int64)) < 0)
+    finally:
+        hip_ctx.free_target(t)
+        f.close()
+
+
+@pytest.mark.parametrize("seed", [11, 12, 13])
+def test_atlas_feather_scene_pixels(hip_ctx, seed):
+    announce(f"atlas pixels seed {seed}")
+    b = random_scene(seed, feathers=ATLAS_FEATHERS)
+    s, f = _oracle(b)
+    ref, _ = f.render()
+    t = hip_ctx.target(b.width, b.height)
+    try:
+        hip_ctx.render_stream(s, t)
+        out = hip_ctx.read_pixels(t, b.width, b.height)
+    finally:
+        hip_ctx.free_target(t)
+        f.close()
+    d = np.abs(out.astype(np.int32) - ref.astype(np.int32)).max(axis=2)
+    print(f"[feather] atlas seed {seed}: max {int(d.max())}, mean {d.mean():.4f}, "
+          f"> 2: {(d > 2).mean():.5f}", flush=True)
+    pixel_check(out, ref)
 
 
 @pytest.mark.parametrize("seed", [1, 2, 3, 4])
