@@ -1661,8 +1661,18 @@ __device__ __forceinline__ void for_prim_cells(const Prim& p, const PathCells& c
         vx[k] = (float)p.p[2 * k];
         vy[k] = (float)p.p[2 * k + 1];
     }
-    // (a feathered segment's six points are binned by their box)
-    const bool trim = ty1 - ty0 > 1 && tx1 - tx0 > 1 && !(p.flags & kPrimFeather);
+    // A feathered segment is two quads, [E0, T0, E1, T1] and its mirror
+    // [E0, B0, E1, B1] (B = 2E - T): its part in a slab is bounded by both.
+    const bool feather = (p.flags & kPrimFeather) != 0u;
+    float bx[4], by[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        const int e = k & 2; // the edge point of this corner's end
+        bx[k] = (k & 1) ? (float)(2 * p.p[2 * e] - p.p[2 * k]) : vx[k];
+        by[k] = (k & 1) ? (float)(2 * p.p[2 * e + 1] - p.p[2 * k + 1]) : vy[k];
+    }
+    const bool trim = ty1 - ty0 > 1 && tx1 - tx0 > 1;
     for (int ty = ty0; ty < ty1; ++ty)
     {
         int a = tx0, b = tx1 - 1;
@@ -1670,7 +1680,18 @@ __device__ __forceinline__ void for_prim_cells(const Prim& p, const PathCells& c
         {
             const float Y0 = (float)(ty * kTileSub);
             float xlo, xhi;
-            if (!slab_extent(vy, vx, Y0 - kNear, Y0 + kFar, &xlo, &xhi))
+            bool any = slab_extent(vy, vx, Y0 - kNear, Y0 + kFar, &xlo, &xhi);
+            if (feather)
+            {
+                float xlo2, xhi2;
+                if (slab_extent(by, bx, Y0 - kNear, Y0 + kFar, &xlo2, &xhi2))
+                {
+                    xlo = any ? fminf(xlo, xlo2) : xlo2;
+                    xhi = any ? fmaxf(xhi, xhi2) : xhi2;
+                    any = true;
+                }
+            }
+            if (!any)
                 continue;
             // tile tx's widened region spans [tx*4096 - 64, tx*4096 + 3968 + 64]
             a = max(a, (int)ceilf(fmaxf((xlo - kFar) * (1.f / kTileSub), -1.f)));
@@ -2109,13 +2130,7 @@ __device__ __forceinline__ void prim_rows(const Prim& p, const TileGeom& g, int*
     const int32_t hi = maxy - g.Y0 - (1 << (kSubpixelLog2 - 1));
     int a0 = max(-((-lo) >> kSubpixelLog2), 0);   // ceil(lo / 256)
     int a1 = min(hi >> kSubpixelLog2, kTile - 1); // floor(hi / 256)
-    if (p.flags & kPrimFeather)
-    {
-        // rows of the box of its six points
-        *r0 = a0;
-        *r1 = a1;
-        return;
-    }
+    const bool feather = (p.flags & kPrimFeather) != 0u;
 
     // tile-relative vertices: |coordinate| < 2^24, exact in float
     float vx[4], vy[4];
@@ -2126,7 +2141,26 @@ __device__ __forceinline__ void prim_rows(const Prim& p, const TileGeom& g, int*
         vy[k] = (float)(p.p[2 * k + 1] - g.Y0);
     }
     float ylo, yhi;
-    if (!slab_extent(vx, vy, -64.f, (float)(kTile << kSubpixelLog2) + 64.f, &ylo, &yhi))
+    bool any = slab_extent(vx, vy, -64.f, (float)(kTile << kSubpixelLog2) + 64.f, &ylo, &yhi);
+    if (feather)
+    {
+        // and the mirrored quad [E0, B0, E1, B1]
+        float bx[4], by[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+        {
+            bx[k] = (k & 1) ? 2.f * vx[k & 2] - vx[k] : vx[k];
+            by[k] = (k & 1) ? 2.f * vy[k & 2] - vy[k] : vy[k];
+        }
+        float ylo2, yhi2;
+        if (slab_extent(bx, by, -64.f, (float)(kTile << kSubpixelLog2) + 64.f, &ylo2, &yhi2))
+        {
+            ylo = any ? fminf(ylo, ylo2) : ylo2;
+            yhi = any ? fmaxf(yhi, yhi2) : yhi2;
+            any = true;
+        }
+    }
+    if (!any)
     {
         *r0 = 1;
         *r1 = 0;
