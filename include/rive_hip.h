@@ -121,7 +121,8 @@ typedef struct rive_hip_draw
     uint32_t contour_flags;    /* e.g. NEGATE_PATH_FILL_COVERAGE_FLAG */
     uint32_t join;             /* 0 miter, 1 round, 2 bevel */
     uint32_t cap;              /* 0 butt, 1 round, 2 square */
-    uint32_t is_stroke;
+    uint32_t is_stroke;        /* 0 fill, 1 stroke, 2 feathered fill (counted like a
+                                  stroke whose joins are feather joins) */
     float matrix[6];
     float stroke_radius;
     float matrix_max_scale;
@@ -449,7 +450,10 @@ typedef struct rive_stream
 
 /* Seeded synthetic streams (std::mt19937_64 with the float/int mappings of
  * tests/common/rand.hpp:13-54).  config: 1 = 1024x1024, 200 closed cubic
- * fills; 2 = 1920x1080, 5000 paths of fills/strokes/gradients/clips.
+ * fills; 2 = 1920x1080, 5000 paths of fills/strokes/gradients/clips;
+ * 3 = 3840x2160, 20000 paths of config 2's mix with 25% feathered clockwise
+ * fills (feather 1..40 px), 20% advanced blends and 64 transformed 256x256
+ * images.
  * path_scale < 1 keeps only that fraction of the draws (parity-test sizes).
  * The stream owns its arrays until rive_stream_free. */
 int rive_stream_generate(int config, uint64_t seed, uint32_t width, uint32_t height,
