@@ -226,6 +226,47 @@ def serving_leg(args):
             "max_batch": args.serve_max_batch, "seconds_of_arrivals": args.serve_seconds, "rates": out}
 
 
+def cfg3_leg(args):
+    """The 4K feathered workload: 3840x2160, 20 000 seeded draws of the
+    1080p mix with 25% feathered clockwise fills (feather 1..40 px), 20%
+    advanced blends and 64 transformed 256x256 images, flushed from
+    HBM-resident records like the headline frame."""
+    ctx = rivehip.Context(0)
+    s = rivehip.generate_stream(3)
+    W3, H3 = s.contents.width, s.contents.height
+    t = ctx.target(W3, H3)
+    try:
+        # (images are created per stream render, so the frame is rendered
+        # from the stream each time: front end, upload and device passes)
+        flushes = ctx.render_stream(s, t)
+        ctx.sync()
+        ms, device_ms = [], []
+        for _ in range(args.cfg3_frames):
+            t0 = time.perf_counter()
+            ctx.render_stream(s, t)
+            ctx.sync()
+            ms.append((time.perf_counter() - t0) * 1e3)
+            device_ms.append(ctx.pass_times()[5])
+        passes = ctx.pass_times()
+    finally:
+        ctx.free_target(t)
+        rivehip.free_stream(s)
+        ctx.close()
+    mean = statistics.mean(ms)
+    print(f"# cfg3: {mean:.1f} ms per frame over {len(ms)} frames", file=sys.stderr, flush=True)
+    names = ["ramps", "count_spans", "tessellate", "bin", "raster", "total"]
+    return {"workload": "3840x2160 RGBA8 frame, 20000 seeded draws: the 1080p mix plus 25% feathered "
+                        "clockwise fills (feather U[1,40] px), 20% multiply/screen/overlay/color-dodge "
+                        "blends, 64 transformed 256x256 images",
+            "frames_per_s": round(1e3 / mean, 3), "ms_per_frame": round(mean, 2),
+            "device_ms_per_frame": round(statistics.mean(device_ms), 2),
+            "note": "end to end: host front end, upload and the device passes of every logical flush; "
+                    "device_ms_per_frame is the HIP-event time of the device passes",
+            "p50_ms": round(percentile(ms, 50), 2), "frames": len(ms), "logical_flushes": flushes,
+            "mpixels_per_s": round(W3 * H3 / mean / 1e3, 1),
+            "passes_ms_last_frame": {n: round(passes[i], 3) for i, n in enumerate(names)}}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -235,6 +276,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched 256-canvas leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-cfg3", action="store_true", help="skip the 4K feathered leg")
+    ap.add_argument("--cfg3-frames", type=int, default=3)
     ap.add_argument("--serve-seconds", type=float, default=0.5, help="seconds of arrivals per rate")
     ap.add_argument("--serve-budget-gb", type=float, default=8.0)
     ap.add_argument("--serve-max-batch", type=int, default=64)
@@ -329,6 +372,7 @@ def main():
         raise RuntimeError("the frame rendered nothing")
     batched = None if args.no_batched else batched_leg(args)
     serving = None if args.no_serving else serving_leg(args)
+    cfg3 = None if args.no_cfg3 else cfg3_leg(args)
 
     total_s = sum(step_ms) / 1e3
     fps = args.steps / total_s
@@ -384,6 +428,7 @@ def main():
         "frame_graph": graph,
         "batched": batched,
         "serving": serving,
+        "cfg3": cfg3,
     }
     print(json.dumps(out), flush=True)
 

@@ -147,6 +147,7 @@ struct rive_hip_context
         const rivehip::SpanV* spans = nullptr;
         const rivehip::ContourV* contours = nullptr;
         const rive_hip_paint_data* paintsHost = nullptr;
+        const rive_hip_path_data* pathsHost = nullptr;
         const rive_hip_grad_span* gradSpansHost = nullptr;
     } in;
     // The descriptors of the last frame's logical flushes (a frame starts at
@@ -1199,12 +1200,20 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
 
     // ---- raster + resolve
     if (tileTotal > 0)
-        raster_kernel<<<tileTotal, kWgThreads, 0, ctx->stream>>>(
+    {
+        // The feather variant only when a path of the flush is feathered (or
+        // the host copy of its records is not at hand).
+        bool feathered = ctx->in.pathsHost == nullptr;
+        for (uint32_t i = 1; !feathered && i <= d->path_count; ++i)
+            feathered = ctx->in.pathsHost[i].feather_radius != 0.f;
+        auto kernel = feathered ? raster_kernel<true> : raster_kernel<false>;
+        kernel<<<tileTotal, kWgThreads, 0, ctx->stream>>>(
             P<PathCells>(ctx->cells), P<PathRaster>(ctx->pathRaster), P<PathPaint>(ctx->pathPaint),
             P<uint2>(ctx->cellPacked),
             P<uint32_t>(ctx->binned), P<Prim>(ctx->prims),
             P<uint32_t>(ctx->ramps), gradRows, P<ImageDesc>(ctx->imageTable), P<CanvasDev>(ctx->canvasTable),
             canvasCount, d->dither, ctx->featherTables);
+    }
     HIP_TRY(hipGetLastError());
     RIVE_TRY(mark(ctx, ctx->ev[5]));
     return RIVE_HIP_OK;
@@ -1458,6 +1467,7 @@ int flush_impl(rive_hip_context* ctx, const rive_hip_flush_desc* d, const GeoLay
         in.spans = reinterpret_cast<const SpanV*>(dev(RIVE_HIP_BUFFER_TESS_SPAN, d->first_tess_span, 64));
         in.contours = reinterpret_cast<const ContourV*>(dev(RIVE_HIP_BUFFER_CONTOUR, d->first_contour, 16));
         in.paintsHost = reinterpret_cast<const rive_hip_paint_data*>(host(RIVE_HIP_BUFFER_PAINT, d->first_paint, 8));
+        in.pathsHost = reinterpret_cast<const rive_hip_path_data*>(host(RIVE_HIP_BUFFER_PATH, d->first_path, 64));
         in.gradSpansHost =
             reinterpret_cast<const rive_hip_grad_span*>(host(RIVE_HIP_BUFFER_GRAD_SPAN, d->first_grad_span, 16));
     }

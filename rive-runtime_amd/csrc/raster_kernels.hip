@@ -3057,6 +3057,9 @@ __device__ __forceinline__ uint32_t slot_arrays(uint32_t n)
     return n == 0 ? 0u : (1u | ((n > kPackedPrims ? 2u : 1u) << 16));
 }
 
+// FEATHER: the flush has feathered paths (their rows are out-of-line calls,
+// whose call frames cost the other variant registers and scratch).
+template <bool FEATHER>
 __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
     const PathCells* __restrict__ cells, const PathRaster* __restrict__ raster,
     const PathPaint* __restrict__ pathPaint, const uint2* __restrict__ cellPacked, const uint32_t* __restrict__ binned, const Prim* __restrict__ prims,
@@ -3295,13 +3298,13 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
                         const Prim pm = prims[pIdx];
                         if (inf & 32u)
                         {
-                            if (pm.flags & kPrimFeatherStroke)
+                            if (FEATHER && (pm.flags & kPrimFeatherStroke))
                                 feather_stroke_prim_row(prims, pIdx, row, g.X0, g.Y0,
                                                         reinterpret_cast<uint32_t*>(sAcc[a]), featherTable);
                             else
                                 stroke_prim_row<double>(pm, row, g, reinterpret_cast<uint32_t*>(sAcc[a]));
                         }
-                        else if (pm.flags & kPrimFeather)
+                        else if (FEATHER && (pm.flags & kPrimFeather))
                             feather_prim_row(prims, pIdx, row, g.X0, g.Y0, sAcc[a], sAcc[a + ((inf >> 4) & 1u)],
                                              (inf & 16u) ? 1 : (1 << 24), featherTable);
                         else if (inf & 16u)
