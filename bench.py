@@ -90,6 +90,12 @@ def cpu_baseline(sample_draws):
                       f"by draw count"}
 
 
+def tune_report(chosen, times):
+    """The autotuner's choice and each candidate's median raster ms."""
+    return {"chosen": chosen,
+            "candidates_ms": {f"order{o}_persistent{p}_waves{w}": round(v, 3) for (o, p, w), v in times.items()}}
+
+
 def batched_leg(args):
     """The batched workload: 256 seeded canvases (sides 64..1024, 10..2000
     closed cubic fills each) rendered by batched flushes from records resident
@@ -120,6 +126,10 @@ def batched_leg(args):
             render_all()
             e2e.append((time.perf_counter() - t0) * 1e3)
         resident = [(ctx, ctx.last_flush_stats(), ctx.last_flush_canvases()) for ctx in ctxs]
+        # Each group's raster config from the autotuner, on its own flush.
+        tuned = [ctx.autotune_frame(ctx.last_frame(), trials=3) for ctx in ctxs]
+        tune_ms = {k: sum(t[1][k] for t in tuned) for k in tuned[0][1]}
+        print(f"# batched raster autotune: {[t[0] for t in tuned]}", file=sys.stderr, flush=True)
         # Per-pass times (and the raster roofline) from flushes run one group
         # at a time, so no pass shares the GPU with another group's.
         passes = []
@@ -171,6 +181,9 @@ def batched_leg(args):
             "ms_per_batch": round(step_s * 1e3, 4), "p50_ms": round(percentile(step_ms, 50), 4),
             "p99_ms": round(percentile(step_ms, 99), 4), "passes_ms": pass_ms,
             "end_to_end_ms": round(statistics.mean(e2e), 4),
+            "raster_autotune": {"chosen_per_flush": [t[0] for t in tuned],
+                                "candidates_ms": tune_report(tuned[0][0], tune_ms)["candidates_ms"],
+                                "note": "candidates_ms sums the flushes' median raster times"},
             "roofline": {"kernel": "raster_kernel", "bound": "hbm", "achieved": round(achieved, 3),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
                          "algorithmic_bytes": b_raster, "kernel_ms": pass_ms["raster"]}}
@@ -194,6 +207,22 @@ def serving_leg(args):
     budget = args.serve_budget_gb * (1 << 30)
     out = []
     try:
+        # The raster config from the autotuner, on a batched flush of one
+        # request per scene and output size.
+        mix = [(sc, w, h) for sc in scenes for (w, h) in SERVE_SIZES]
+        streams = [sc.stream(1.0, w, h) for sc, w, h in mix]
+        first = rivehip.flush_groups(streams)[0]  # what fits one flush's contour IDs
+        targets = [ctx.target(mix[i][1], mix[i][2]) for i in first]
+        try:
+            ctx.render_streams([streams[i] for i in first], targets)
+            ctx.sync()
+            tuned, tune_ms = ctx.autotune_frame(ctx.last_frame(), trials=3)
+        finally:
+            for t in targets:
+                ctx.free_target(t)
+            for st in streams:
+                rivehip.free_stream(st)
+        print(f"# serving raster autotune: {tuned}", file=sys.stderr, flush=True)
         # warm-up: the pool's targets and the pipeline buffers of each size
         ctx.serve(scenes, [(k % 8, 1.0, w, h, 0.0) for k, (w, h) in enumerate(SERVE_SIZES * 2)], budget,
                   args.serve_max_batch)
@@ -223,7 +252,8 @@ def serving_leg(args):
             sc.close()
     return {"workload": "Poisson arrivals of animated-scene requests (8 seeded scenes, output U{256^2, "
                         "512^2, 1920x1080}), continuous batching", "hbm_budget_bytes": budget,
-            "max_batch": args.serve_max_batch, "seconds_of_arrivals": args.serve_seconds, "rates": out}
+            "max_batch": args.serve_max_batch, "seconds_of_arrivals": args.serve_seconds,
+            "raster_autotune": tune_report(tuned, tune_ms), "rates": out}
 
 
 def cfg3_leg(args):
@@ -320,6 +350,10 @@ def main():
         frame = ctx.last_frame()
         print(f"# end-to-end frame mean {statistics.mean(e2e):.3f} ms; {len(frame)} logical flushes",
               file=sys.stderr, flush=True)
+        # The raster autotuner: every raster config timed on this frame, the
+        # fastest kept for the timed steps (untimed set-up, like the warm-up).
+        tuned, tune_ms = ctx.autotune_frame(frame, trials=5)
+        print(f"# raster autotune: {tuned}", file=sys.stderr, flush=True)
 
         # Resident frames: the frame's logical flushes alone, from records
         # already in HBM.
@@ -425,6 +459,7 @@ def main():
                  "note": "SQ_INSTS_VALU per frame (rocprofv3) over the HIP-event raster time; the "
                          "pass is bound by VALU issue, not HBM"},
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample_draws),
+        "raster_autotune": tune_report(tuned, tune_ms),
         "frame_graph": graph,
         "batched": batched,
         "serving": serving,

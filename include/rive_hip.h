@@ -317,8 +317,10 @@ int rive_hip_count_draws(rive_hip_context* ctx, uint32_t draw_count, uint32_t* t
 
 /* The descriptors of the last frame's logical flushes, in order: the flushes
  * since the last one whose records start at offset 0 (first_* all zero).
- * Their image and canvas tables are copies the context keeps until the next
- * frame.  Re-issuing them with rive_hip_flush re-renders the frame from the
+ * Their image and canvas tables are copies the context keeps: they stay
+ * valid while the same frame is re-issued, and until a flush records other
+ * tables in their place (the images and targets they name must still be
+ * alive).  Re-issuing them with rive_hip_flush re-renders the frame from the
  * records still resident in HBM (what a serving loop or a benchmark of the
  * flush alone does).  Writes min(capacity, count) descriptors; *count gets
  * the full count. */
@@ -379,6 +381,43 @@ typedef struct rive_hip_flush_stats
 } rive_hip_flush_stats;
 
 int rive_hip_last_flush_stats(rive_hip_context* ctx, rive_hip_flush_stats* out);
+
+/* How the raster pass runs on this context.  No counterpart in the reference
+ * (its raster is the GPU's fixed-function pipeline); these are the launch
+ * choices of the tile raster, set by hand or by rive_hip_autotune_frame.
+ * Every choice renders the same pixels.
+ *  tile_order     0: tiles in canvas/row/column order; 1: heaviest first (a
+ *                 per-tile cost estimate from the binning, two small kernels).
+ *  persistent     0: one workgroup per tile; 1: a resident grid (waves_per_simd
+ *                 workgroups per CU) taking tiles from a device queue.
+ *  waves_per_simd 5 or 6: the occupancy the raster is compiled for (its
+ *                 per-workgroup LDS allows at most 6). */
+typedef struct rive_hip_raster_config
+{
+    uint32_t tile_order;
+    uint32_t persistent;
+    uint32_t waves_per_simd;
+    uint32_t reserved; /* 0 */
+} rive_hip_raster_config;
+
+int rive_hip_set_raster_config(rive_hip_context* ctx, const rive_hip_raster_config* config);
+int rive_hip_get_raster_config(rive_hip_context* ctx, rive_hip_raster_config* out);
+
+/* The raster autotuner (no counterpart in the reference).  Runs a frame's
+ * logical flushes (descs, as for rive_hip_flush_frame) once untimed and then
+ * `trials` (1..64) times under each of RIVE_HIP_RASTER_CANDIDATES raster
+ * configs -- tile_order x persistent x waves_per_simd, in the order
+ * {0,0,6} {0,0,5} {0,1,6} {0,1,5} {1,0,6} {1,0,5} {1,1,6} {1,1,5} -- times
+ * each flush's raster pass with the context's HIP events (a trial runs the
+ * flushes one by one), and keeps the config of the
+ * smallest median on the context.  The frame's targets are saved before and
+ * restored after, so their pixels are as they were.  out (may be null) gets
+ * the chosen config, times_ms (may be null, RIVE_HIP_RASTER_CANDIDATES
+ * floats) each candidate's median raster milliseconds.  Allocates a copy of
+ * the targets; synchronizes the stream. */
+#define RIVE_HIP_RASTER_CANDIDATES 8
+int rive_hip_autotune_frame(rive_hip_context* ctx, const rive_hip_flush_desc* descs, uint32_t n,
+                            uint32_t trials, rive_hip_raster_config* out, float* times_ms);
 
 /* The canvas table of the last flush (one entry for a single-target flush).
  * Writes min(capacity, count) entries; *count gets the full count. */

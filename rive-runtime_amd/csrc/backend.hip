@@ -77,6 +77,7 @@ struct MappedBuffer
 };
 
 constexpr int kEvents = 7;
+constexpr int kRasterCandidateCount = RIVE_HIP_RASTER_CANDIDATES;
 // Bounds on one flush's records (checked before any size arithmetic).
 constexpr uint32_t kMaxFlushPaths = 1u << 24;
 constexpr uint32_t kMaxFlushTexels = 1u << 31;
@@ -113,6 +114,11 @@ struct rive_hip_context
     // primitives leaving the guard box: their list, pieces per draw, the
     // pieces' offsets and cursors, and the draw list with the pieces' draws
     DevBuf specialList, farList, drawPieces, drawPieceOffset, pieceCursor, drawTessAll;
+    // the raster's tile order: cost class per tile, the order, and the
+    // class histogram, class cursors and tile queue head (2 * 256 + 1 words)
+    DevBuf tileClass, tileOrder, tileQueue;
+    rive_hip_raster_config rasterConfig = {0u, 0u, 6u, 0u};
+    int cuCount = 256;
     CanvasDev* hostCanvases = nullptr; // pinned staging of the canvas table
     uint32_t hostCanvasesCap = 0;
     std::vector<rive_hip_canvas> lastCanvases;
@@ -171,6 +177,7 @@ struct rive_hip_context
     std::vector<rive_hip_flush_desc> frameDescs;
     std::vector<std::vector<rive_hip_image*>> frameImages;
     std::vector<std::vector<rive_hip_canvas>> frameCanvases;
+    size_t frameLen = 0; // flushes of the recorded frame (slots past it keep their storage)
     // The feather function tables (FEATHER, then INVERSE_FEATHER), uploaded
     // once at creation.
     float* featherTables = nullptr;
@@ -179,7 +186,7 @@ struct rive_hip_context
 namespace
 {
 // The scratch buffers of a context (every DevBuf member), in one list.
-constexpr int kDevBufCount = 37;
+constexpr int kDevBufCount = 40;
 // The device scalars of a flush (totals, counters), 32 words: the pipeline
 // uses slots 0-7, device counting 8-13, guard-box pieces kFarSlot .. +3.
 constexpr size_t kScalarBytes = 128;
@@ -194,7 +201,8 @@ void devbufs(rive_hip_context* c, DevBuf** out)
                                  &c->verts, &c->imageTable, &c->pathImage, &c->canvasTable,
                                  &c->pathCanvas, &c->cellPacked, &c->pathDraws, &c->pathPaint,
                                  &c->patchTable, &c->patchWork, &c->specialList, &c->farList, &c->drawPieces,
-                                 &c->drawPieceOffset, &c->pieceCursor, &c->drawTessAll};
+                                 &c->drawPieceOffset, &c->pieceCursor, &c->drawTessAll, &c->tileClass,
+                                 &c->tileOrder, &c->tileQueue};
     for (int i = 0; i < kDevBufCount; ++i)
         out[i] = all[i];
 }
@@ -414,6 +422,8 @@ int rive_hip_create(int device, rive_hip_context** out)
         ctx->lastCanvases.reserve(1);
         e = scratch_address(device, &ctx->scratchPtr);
     }
+    if (e == hipSuccess)
+        e = hipDeviceGetAttribute(&ctx->cuCount, hipDeviceAttributeMultiprocessorCount, device);
     if (e == hipSuccess)
         e = hipMalloc((void**)&ctx->featherTables, sizeof(float) * 2 * kFeatherTableSize);
     if (e == hipSuccess)
@@ -985,6 +995,19 @@ int derive_patch_draws(rive_hip_context* ctx, const rive_hip_flush_desc* d, cons
                        const ContourV* contours, uint32_t contourCount, uint32_t texels, uint32_t* drawCount,
                        uint32_t* totalPrims);
 
+// The raster variant for a flush: feathered rows or not, persistent or one
+// workgroup per tile, 5 or 6 waves per SIMD.
+using RasterFn = decltype(&raster_kernel<false, false, 6>);
+RasterFn raster_fn(bool feathered, bool persistent, int waves)
+{
+    static const RasterFn table[2][2][2] = {
+        {{raster_kernel<false, false, 5>, raster_kernel<false, false, 6>},
+         {raster_kernel<false, true, 5>, raster_kernel<false, true, 6>}},
+        {{raster_kernel<true, false, 5>, raster_kernel<true, false, 6>},
+         {raster_kernel<true, true, 5>, raster_kernel<true, true, 6>}}};
+    return table[feathered ? 1 : 0][persistent ? 1 : 0][waves == 5 ? 0 : 1];
+}
+
 int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const SpanV* spans,
                     uint32_t spanCount, const ContourV* contours, uint32_t contourCount, uint32_t tessRows,
                     DrawTess* drawTess, uint32_t drawCount, uint32_t totalPrims)
@@ -1206,13 +1229,35 @@ int raster_pipeline(rive_hip_context* ctx, const rive_hip_flush_desc* d, const S
         bool feathered = ctx->in.pathsHost == nullptr;
         for (uint32_t i = 1; !feathered && i <= d->path_count; ++i)
             feathered = ctx->in.pathsHost[i].feather_radius != 0.f;
-        auto kernel = feathered ? raster_kernel<true> : raster_kernel<false>;
-        kernel<<<tileTotal, kWgThreads, 0, ctx->stream>>>(
+        const rive_hip_raster_config& rc = ctx->rasterConfig;
+        const bool persistent = rc.persistent != 0;
+        const size_t queueBytes = (2 * kCostClasses + 1) * 4;
+        RIVE_TRY(ensure(ctx, ctx->tileQueue, queueBytes));
+        RIVE_TRY(clear(ctx, ctx->tileQueue.ptr, 0, queueBytes));
+        uint32_t* queue = P<uint32_t>(ctx->tileQueue);
+        const uint32_t* order = nullptr;
+        if (rc.tile_order != 0)
+        {
+            RIVE_TRY(ensure(ctx, ctx->tileClass, (size_t)tileTotal * 4));
+            RIVE_TRY(ensure(ctx, ctx->tileOrder, (size_t)tileTotal * 4));
+            tile_cost_kernel<<<(tileTotal + 3) / 4, 256, 0, ctx->stream>>>(
+                P<PathCells>(ctx->cells), P<uint2>(ctx->cellPacked), P<CanvasDev>(ctx->canvasTable), canvasCount,
+                tileTotal, P<uint32_t>(ctx->tileClass), queue);
+            tile_order_kernel<<<(tileTotal + 255) / 256, 256, 0, ctx->stream>>>(
+                P<uint32_t>(ctx->tileClass), queue, queue + kCostClasses, tileTotal, P<uint32_t>(ctx->tileOrder));
+            order = P<uint32_t>(ctx->tileOrder);
+        }
+        const int waves = rc.waves_per_simd == 5 ? 5 : 6;
+        // a resident grid: one workgroup of four waves per SIMD's share
+        const uint32_t grid =
+            persistent ? std::min<uint32_t>(tileTotal, (uint32_t)std::max(ctx->cuCount, 1) * (uint32_t)waves)
+                       : tileTotal;
+        raster_fn(feathered, persistent, waves)<<<grid, kWgThreads, 0, ctx->stream>>>(
             P<PathCells>(ctx->cells), P<PathRaster>(ctx->pathRaster), P<PathPaint>(ctx->pathPaint),
             P<uint2>(ctx->cellPacked),
             P<uint32_t>(ctx->binned), P<Prim>(ctx->prims),
             P<uint32_t>(ctx->ramps), gradRows, P<ImageDesc>(ctx->imageTable), P<CanvasDev>(ctx->canvasTable),
-            canvasCount, d->dither, ctx->featherTables);
+            canvasCount, d->dither, ctx->featherTables, order, tileTotal, queue + 2 * kCostClasses);
     }
     HIP_TRY(hipGetLastError());
     RIVE_TRY(mark(ctx, ctx->ev[5]));
@@ -1850,29 +1895,45 @@ extern "C" int rive_hip_flush_frame(rive_hip_context* ctx, const rive_hip_flush_
 
 namespace
 {
+bool same_canvases(const std::vector<rive_hip_canvas>& a, const std::vector<rive_hip_canvas>& b)
+{
+    if (a.size() != b.size())
+        return false;
+    for (size_t i = 0; i < a.size(); ++i)
+        if (a[i].target != b[i].target || a[i].first_path != b[i].first_path ||
+            a[i].path_count != b[i].path_count || a[i].clear_color != b[i].clear_color ||
+            a[i].load_action != b[i].load_action)
+            return false;
+    return true;
+}
+
 void record_frame_desc(rive_hip_context* ctx, const rive_hip_flush_desc* d)
 {
+    // Copy the tables first: d may point into the recorded frame itself (a
+    // frame re-issued from rive_hip_last_frame).
+    const rive_hip_flush_desc desc = *d;
+    std::vector<rive_hip_image*> images(d->images, d->images + (d->images ? d->image_count : 0));
+    std::vector<rive_hip_canvas> canvases(d->canvases, d->canvases + (d->canvases ? d->canvas_count : 0));
+    if (d->first_path == 0 && d->first_draw == 0 && d->first_tess_span == 0 && d->first_contour == 0)
+        ctx->frameLen = 0;
+    const size_t k = ctx->frameLen++;
+    if (ctx->frameImages.size() <= k)
     {
-        // Copy the tables first: d may point into the recorded frame itself
-        // (a frame re-issued from rive_hip_last_frame).
-        const rive_hip_flush_desc desc = *d;
-        std::vector<rive_hip_image*> images(d->images, d->images + (d->images ? d->image_count : 0));
-        std::vector<rive_hip_canvas> canvases(d->canvases, d->canvases + (d->canvases ? d->canvas_count : 0));
-        if (d->first_path == 0 && d->first_draw == 0 && d->first_tess_span == 0 && d->first_contour == 0)
-        {
-            ctx->frameDescs.clear();
-            ctx->frameImages.clear();
-            ctx->frameCanvases.clear();
-        }
-        ctx->frameDescs.push_back(desc);
-        ctx->frameImages.push_back(std::move(images));
-        ctx->frameCanvases.push_back(std::move(canvases));
-        for (size_t k = 0; k < ctx->frameDescs.size(); ++k)
-        {
-            ctx->frameDescs[k].images = ctx->frameImages[k].empty() ? nullptr : ctx->frameImages[k].data();
-            ctx->frameDescs[k].canvases = ctx->frameCanvases[k].empty() ? nullptr : ctx->frameCanvases[k].data();
-        }
+        ctx->frameImages.emplace_back();
+        ctx->frameCanvases.emplace_back();
     }
+    // A slot keeps its storage while its tables stay the same, so the table
+    // pointers rive_hip_last_frame handed out stay valid while that frame is
+    // re-issued flush by flush (an inner vector keeps its buffer when the
+    // outer one grows).
+    if (ctx->frameImages[k] != images)
+        ctx->frameImages[k] = std::move(images);
+    if (!same_canvases(ctx->frameCanvases[k], canvases))
+        ctx->frameCanvases[k] = std::move(canvases);
+    ctx->frameDescs.resize(k + 1);
+    ctx->frameDescs[k] = desc;
+    ctx->frameDescs[k].images = ctx->frameImages[k].empty() ? nullptr : ctx->frameImages[k].data();
+    ctx->frameDescs[k].canvases = ctx->frameCanvases[k].empty() ? nullptr : ctx->frameCanvases[k].data();
 }
 } // namespace
 
@@ -1917,6 +1978,130 @@ extern "C" int rive_hip_flush_noalloc(rive_hip_context* ctx, const rive_hip_flus
         *all[i] = saved[i];
     ctx->lastFlushNoAlloc = true;
     return rc;
+}
+
+// The raster configurations rive_hip_autotune_frame tries, in the order of
+// its times_ms output.
+constexpr rive_hip_raster_config kRasterCandidates[kRasterCandidateCount] = {
+    {0u, 0u, 6u, 0u}, {0u, 0u, 5u, 0u}, {0u, 1u, 6u, 0u}, {0u, 1u, 5u, 0u},
+    {1u, 0u, 6u, 0u}, {1u, 0u, 5u, 0u}, {1u, 1u, 6u, 0u}, {1u, 1u, 5u, 0u}};
+
+extern "C" int rive_hip_autotune_frame(rive_hip_context* ctx, const rive_hip_flush_desc* descs, uint32_t n,
+                                       uint32_t trials, rive_hip_raster_config* out, float* timesMs)
+{
+    if (ctx == nullptr || descs == nullptr || n == 0)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "autotune: null context or empty frame");
+    if (trials == 0 || trials > 64)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "autotune: trials must be 1..64");
+    if (ctx->capturing)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "autotune inside a stream capture");
+    HIP_TRY(hipSetDevice(ctx->device));
+    // The frame and its tables copied first: descs may point into the frame
+    // the context records, which the trials re-record.
+    std::vector<rive_hip_flush_desc> frame(descs, descs + n);
+    std::vector<std::vector<rive_hip_image*>> images(n);
+    std::vector<std::vector<rive_hip_canvas>> canvases(n);
+    for (uint32_t k = 0; k < n; ++k)
+    {
+        if (frame[k].images != nullptr)
+            images[k].assign(frame[k].images, frame[k].images + frame[k].image_count);
+        if (frame[k].canvases != nullptr)
+            canvases[k].assign(frame[k].canvases, frame[k].canvases + frame[k].canvas_count);
+        frame[k].images = images[k].empty() ? nullptr : images[k].data();
+        frame[k].canvases = canvases[k].empty() ? nullptr : canvases[k].data();
+    }
+    descs = frame.data();
+    // The frame's targets, saved so that the trials leave their pixels as the
+    // frame's first rendering will find them.
+    std::vector<rive_hip_target*> targets;
+    for (uint32_t k = 0; k < n; ++k)
+    {
+        const rive_hip_flush_desc& d = descs[k];
+        if (d.canvas_count > 0 && d.canvases == nullptr)
+            return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "autotune: canvas_count without canvases");
+        if (d.canvas_count > 0)
+            for (uint32_t c = 0; c < d.canvas_count; ++c)
+                targets.push_back(d.canvases[c].target);
+        else
+            targets.push_back(d.target);
+    }
+    std::sort(targets.begin(), targets.end());
+    targets.erase(std::unique(targets.begin(), targets.end()), targets.end());
+    size_t bytes = 0;
+    for (rive_hip_target* t : targets)
+    {
+        if (t == nullptr || t->ctx != ctx)
+            return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "autotune: a flush names a target of another context");
+        bytes += (size_t)t->width * t->height * 4;
+    }
+    unsigned char* saved = nullptr;
+    HIP_TRY(hipMalloc((void**)&saved, std::max<size_t>(bytes, 4)));
+    size_t at = 0;
+    hipError_t e = hipSuccess;
+    for (rive_hip_target* t : targets)
+    {
+        const size_t b = (size_t)t->width * t->height * 4;
+        if (e == hipSuccess && b > 0)
+            e = hipMemcpyAsync(saved + at, t->pixels, b, hipMemcpyDeviceToDevice, ctx->stream);
+        at += b;
+    }
+    const rive_hip_raster_config before = ctx->rasterConfig;
+    int rc = e == hipSuccess ? RIVE_HIP_OK : fail(RIVE_HIP_ERROR_DEVICE, hipGetErrorString(e));
+    float best = INFINITY;
+    int bestK = 0;
+    std::vector<float> t(trials);
+    for (int k = 0; k < kRasterCandidateCount && rc == RIVE_HIP_OK; ++k)
+    {
+        ctx->rasterConfig = kRasterCandidates[k];
+        rc = rive_hip_flush_frame(ctx, descs, n); // untimed: first use, buffer growth
+        // a trial: the frame flush by flush, each flush's raster pass timed
+        for (uint32_t r = 0; r < trials && rc == RIVE_HIP_OK; ++r)
+        {
+            t[r] = 0.f;
+            for (uint32_t f = 0; f < n && rc == RIVE_HIP_OK; ++f)
+            {
+                float ms[6];
+                rc = rive_hip_flush(ctx, &descs[f]);
+                if (rc == RIVE_HIP_OK)
+                    rc = rive_hip_pass_times(ctx, ms, 6);
+                if (rc == RIVE_HIP_OK)
+                    t[r] += ms[4];
+            }
+        }
+        if (rc != RIVE_HIP_OK)
+            break;
+        std::sort(t.begin(), t.end());
+        const float median = t[trials / 2];
+        if (timesMs != nullptr)
+            timesMs[k] = median;
+        if (median < best)
+        {
+            best = median;
+            bestK = k;
+        }
+    }
+    at = 0;
+    for (rive_hip_target* tg : targets)
+    {
+        const size_t b = (size_t)tg->width * tg->height * 4;
+        if (e == hipSuccess && b > 0)
+            e = hipMemcpyAsync(tg->pixels, saved + at, b, hipMemcpyDeviceToDevice, ctx->stream);
+        at += b;
+    }
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(saved);
+    if (rc != RIVE_HIP_OK)
+    {
+        ctx->rasterConfig = before;
+        return rc;
+    }
+    if (e != hipSuccess)
+        return fail(RIVE_HIP_ERROR_DEVICE, std::string("autotune: ") + hipGetErrorString(e));
+    ctx->rasterConfig = kRasterCandidates[bestK];
+    if (out != nullptr)
+        *out = ctx->rasterConfig;
+    return RIVE_HIP_OK;
 }
 
 extern "C" int rive_hip_pass_times(rive_hip_context* ctx, float* ms, int n)
@@ -1996,6 +2181,26 @@ extern "C" int rive_hip_debug_readback(rive_hip_context* ctx, int what, void* ds
         HIP_TRY(hipMemcpyAsync(dst, src, std::min(bytes, capacity), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
+    return RIVE_HIP_OK;
+}
+
+extern "C" int rive_hip_set_raster_config(rive_hip_context* ctx, const rive_hip_raster_config* config)
+{
+    if (ctx == nullptr || config == nullptr)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null context or config");
+    if (config->tile_order > 1 || config->persistent > 1 ||
+        (config->waves_per_simd != 5 && config->waves_per_simd != 6) || config->reserved != 0)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT,
+                    "raster config: tile_order and persistent are 0 or 1, waves_per_simd 5 or 6, reserved 0");
+    ctx->rasterConfig = *config;
+    return RIVE_HIP_OK;
+}
+
+extern "C" int rive_hip_get_raster_config(rive_hip_context* ctx, rive_hip_raster_config* out)
+{
+    if (ctx == nullptr || out == nullptr)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null context or output");
+    *out = ctx->rasterConfig;
     return RIVE_HIP_OK;
 }
 

@@ -3057,15 +3057,121 @@ __device__ __forceinline__ uint32_t slot_arrays(uint32_t n)
     return n == 0 ? 0u : (1u | ((n > kPackedPrims ? 2u : 1u) << 16));
 }
 
+// ---- tile order: the raster's tiles heaviest first
+//
+// A tile's work is dominated by the paths it lists and the primitives binned
+// to it, so its estimated cost is 4 units per listed path (a path with
+// primitives here or a nonzero corner winding) plus one per primitive.  Tiles
+// are grouped into kCostClasses classes, eight per octave of cost, and the
+// order lists the classes from the heaviest down (the order within a class is
+// whatever the atomics give: every tile is rendered independently, so the
+// image does not depend on it).  Dispatching in this order starts the long
+// tiles first, so they do not trail at the end of the launch.
+constexpr int kCostClasses = 256;
+
+__device__ __forceinline__ uint32_t cost_class(uint32_t cost)
+{
+    const uint32_t v = cost + 1u;
+    const int e = 31 - __clz(v);
+    const uint32_t f = e >= 3 ? (v >> (e - 3)) & 7u : (v << (3 - e)) & 7u;
+    const uint32_t k = min((uint32_t)e * 8u + f, (uint32_t)kCostClasses - 1u);
+    return (uint32_t)kCostClasses - 1u - k; // class 0 is the heaviest
+}
+
+// One wave per tile: its cost class into tileClass, counted in hist.
+__global__ __launch_bounds__(256) void tile_cost_kernel(const PathCells* __restrict__ cells,
+                                                        const uint2* __restrict__ cellPacked,
+                                                        const CanvasDev* __restrict__ canvases, uint32_t canvasCount,
+                                                        uint32_t tileTotal, uint32_t* __restrict__ tileClass,
+                                                        uint32_t* __restrict__ hist)
+{
+    const uint32_t item = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (item >= tileTotal)
+        return;
+    uint32_t lo = 0, hi = canvasCount - 1;
+    while (lo < hi)
+    {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (canvases[mid].tileBase <= item)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const CanvasDev cv = canvases[lo];
+    const int tile = (int)(item - cv.tileBase);
+    const int tx = tile % cv.tilesX, ty = tile / cv.tilesX;
+    uint32_t cost = 0;
+    for (uint32_t p = cv.firstPath + (uint32_t)lane; p < cv.pathEnd; p += 64u)
+    {
+        const PathCells c = cells[p];
+        if (tx >= c.tx0 && tx < c.tx1 && ty >= c.ty0 && ty < c.ty1)
+        {
+            const uint32_t cell = c.base + (uint32_t)(ty - c.ty0) * c.width + (uint32_t)(tx - c.tx0);
+            const uint2 here = cellPacked[cell];
+            const uint32_t n = cellPacked[cell + 1].x - here.x;
+            cost += n != 0 ? 4u + n : (here.y != 0 ? 4u : 0u);
+        }
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1)
+        cost += (uint32_t)__shfl_xor((int)cost, d);
+    if (lane == 0)
+    {
+        const uint32_t cls = cost_class(cost);
+        tileClass[item] = cls;
+        atomicAdd(&hist[cls], 1u);
+    }
+}
+
+// Places every tile in the order: its class's first position (the exclusive
+// prefix of the histogram) plus its place among the tiles of that class.
+// cursor[] starts at zero.
+__global__ __launch_bounds__(256) void tile_order_kernel(const uint32_t* __restrict__ tileClass,
+                                                         const uint32_t* __restrict__ hist,
+                                                         uint32_t* __restrict__ cursor, uint32_t tileTotal,
+                                                         uint32_t* __restrict__ order)
+{
+    static_assert(kCostClasses == 256, "one class per thread");
+    __shared__ uint32_t sPre[kCostClasses];
+    const int t = threadIdx.x;
+    uint32_t v = hist[t];
+    sPre[t] = v;
+    __syncthreads();
+    for (int d = 1; d < kCostClasses; d <<= 1)
+    {
+        const uint32_t a = t >= d ? sPre[t - d] : 0u;
+        __syncthreads();
+        sPre[t] += a;
+        __syncthreads();
+    }
+    const uint32_t first = sPre[t] - v; // exclusive
+    __syncthreads();
+    sPre[t] = first;
+    __syncthreads();
+    const uint32_t item = blockIdx.x * 256u + (uint32_t)t;
+    if (item < tileTotal)
+    {
+        const uint32_t cls = tileClass[item];
+        order[sPre[cls] + atomicAdd(&cursor[cls], 1u)] = item;
+    }
+}
+
 // FEATHER: the flush has feathered paths (their rows are out-of-line calls,
 // whose call frames cost the other variant registers and scratch).
-template <bool FEATHER>
-__global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
+// PERSISTENT: a resident grid whose workgroups take tiles from a device
+// queue (the counter at queueHead, zero at launch) until it runs dry; else one
+// workgroup per tile.  Either way the k-th tile taken is tileOrder[k] (the
+// flush's tiles heaviest first) or, without an order, tile k.  WAVES is the
+// occupancy the kernel is compiled for (waves per SIMD).
+template <bool FEATHER, bool PERSISTENT, int WAVES>
+__global__ __launch_bounds__(kWgThreads, WAVES) void raster_kernel(
     const PathCells* __restrict__ cells, const PathRaster* __restrict__ raster,
     const PathPaint* __restrict__ pathPaint, const uint2* __restrict__ cellPacked, const uint32_t* __restrict__ binned, const Prim* __restrict__ prims,
     const uint32_t* __restrict__ ramps, uint32_t gradHeight, const ImageDesc* __restrict__ images,
     const CanvasDev* __restrict__ canvases, uint32_t canvasCount, uint32_t dither,
-    const float* __restrict__ featherTable)
+    const float* __restrict__ featherTable, const uint32_t* __restrict__ tileOrder, uint32_t tileTotal,
+    uint32_t* __restrict__ queueHead)
 {
     __shared__ __align__(16) int32_t sAcc[kAccArrays][kTile * kTile];
     __shared__ int32_t sLeft[kAccArrays][kTile];
@@ -3081,311 +3187,330 @@ __global__ __launch_bounds__(kWgThreads, 6) void raster_kernel(
     __shared__ uint32_t sItemPrim[kWgThreads];
     __shared__ uint32_t sItemInfo[kWgThreads]; // array | wide << 4 | stroke << 5 | row0 << 8 | item0 << 16
 
+    __shared__ uint32_t sTake;
+
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
-    // this workgroup's canvas: the last one whose first tile is at or before it
-    uint32_t lo = 0, hi = canvasCount - 1;
-    while (lo < hi)
+    for (;;)
     {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (canvases[mid].tileBase <= blockIdx.x)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    const CanvasDev cv = canvases[lo];
-    const int width = cv.width, height = cv.height, tilesX = cv.tilesX;
-    uint32_t* __restrict__ target = cv.pixels;
-    const uint32_t loadPreserve = cv.loadPreserve, clearPremul = cv.clearPremul;
-    const uint32_t pathBegin = cv.firstPath, pathEnd = cv.pathEnd;
-    const int tile = (int)(blockIdx.x - cv.tileBase);
-    const int tx = tile % tilesX, ty = tile / tilesX;
-    const int px = tx * kTile + (t & (kTile - 1)), py = ty * kTile + (t >> kTileLog2);
-    TileGeom g;
-    g.X0 = tx << (kSubpixelLog2 + kTileLog2);
-    g.Y0 = ty << (kSubpixelLog2 + kTileLog2);
-
-    // The pixel's RGBA8 color (quantized after every path, as the reference's
-    // color buffer is), and its clip buffer entry as clip ID (low 16 bits)
-    // with fp16 coverage (high 16 bits), like the reference's clipBuffer.
-    const bool inImage = px < width && py < height;
-    uint32_t col = clearPremul;
-    if (loadPreserve && inImage)
-        col = target[(size_t)py * width + px];
-    uint32_t clip = 0u;
-    const float noise255 = dither ? dither_noise(px, py) * 255.f : 0.f;
-
-    // clear accumulators for the first batch (the listing's barriers order
-    // this before their use)
-    static_assert(kAccArrays * kTile == kWgThreads, "one sLeft entry per thread");
-    for (int q = t; q < kAccArrays * kTile * kTile / 4; q += kWgThreads)
-        reinterpret_cast<int4*>(&sAcc[0][0])[q] = make_int4(0, 0, 0, 0);
-    (&sLeft[0][0])[t] = 0;
-
-    for (uint32_t base = pathBegin; base < pathEnd; base += kWindow)
-    {
-        // ---- list the window's paths that touch the tile
-        const uint32_t p = base + (uint32_t)t;
-        uint32_t off = 0, n = 0;
-        int32_t C = 0;
-        bool listed = false;
-        if (t < kWindow && p < pathEnd)
+        uint32_t take = blockIdx.x;
+        if (PERSISTENT)
         {
-            const PathCells c = cells[p];
-            if (tx >= c.tx0 && tx < c.tx1 && ty >= c.ty0 && ty < c.ty1)
+            __syncthreads(); // the previous tile is done with sTake and the arrays
+            if (t == 0)
+                sTake = atomicAdd(queueHead, 1u);
+            __syncthreads();
+            take = sTake;
+            if (take >= tileTotal)
+                break; // the same value in every wave: the whole workgroup leaves
+        }
+        const uint32_t item = tileOrder != nullptr ? tileOrder[take] : take;
+        // this tile's canvas: the last one whose first tile is at or before it
+        uint32_t lo = 0, hi = canvasCount - 1;
+        while (lo < hi)
+        {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (canvases[mid].tileBase <= item)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const CanvasDev cv = canvases[lo];
+        const int width = cv.width, height = cv.height, tilesX = cv.tilesX;
+        uint32_t* __restrict__ target = cv.pixels;
+        const uint32_t loadPreserve = cv.loadPreserve, clearPremul = cv.clearPremul;
+        const uint32_t pathBegin = cv.firstPath, pathEnd = cv.pathEnd;
+        const int tile = (int)(item - cv.tileBase);
+        const int tx = tile % tilesX, ty = tile / tilesX;
+        const int px = tx * kTile + (t & (kTile - 1)), py = ty * kTile + (t >> kTileLog2);
+        TileGeom g;
+        g.X0 = tx << (kSubpixelLog2 + kTileLog2);
+        g.Y0 = ty << (kSubpixelLog2 + kTileLog2);
+
+        // The pixel's RGBA8 color (quantized after every path, as the reference's
+        // color buffer is), and its clip buffer entry as clip ID (low 16 bits)
+        // with fp16 coverage (high 16 bits), like the reference's clipBuffer.
+        const bool inImage = px < width && py < height;
+        uint32_t col = clearPremul;
+        if (loadPreserve && inImage)
+            col = target[(size_t)py * width + px];
+        uint32_t clip = 0u;
+        const float noise255 = dither ? dither_noise(px, py) * 255.f : 0.f;
+
+        // clear accumulators for the first batch (the listing's barriers order
+        // this before their use)
+        static_assert(kAccArrays * kTile == kWgThreads, "one sLeft entry per thread");
+        for (int q = t; q < kAccArrays * kTile * kTile / 4; q += kWgThreads)
+            reinterpret_cast<int4*>(&sAcc[0][0])[q] = make_int4(0, 0, 0, 0);
+        (&sLeft[0][0])[t] = 0;
+
+        for (uint32_t base = pathBegin; base < pathEnd; base += kWindow)
+        {
+            // ---- list the window's paths that touch the tile
+            const uint32_t p = base + (uint32_t)t;
+            uint32_t off = 0, n = 0;
+            int32_t C = 0;
+            bool listed = false;
+            if (t < kWindow && p < pathEnd)
             {
-                const uint32_t cell = c.base + (uint32_t)(ty - c.ty0) * c.width + (uint32_t)(tx - c.tx0);
-                const uint2 here = cellPacked[cell];
-                off = here.x;
-                n = cellPacked[cell + 1].x - off;
-                C = (int32_t)here.y;
-                // A path with no primitives here covers the tile with its corner
-                // winding's coverage; when that is zero it draws nothing, and a
-                // clip update's zero-coverage write reads the same as no write
-                // (every clip draw has a fresh ID), so it is not listed.
-                listed = n != 0;
-                if (n == 0 && C != 0)
+                const PathCells c = cells[p];
+                if (tx >= c.tx0 && tx < c.tx1 && ty >= c.ty0 && ty < c.ty1)
                 {
-                    const uint32_t flags = pathPaint[p].kind & 0xffffu;
-                    listed = !(flags & 1u) && fill_rule_coverage((float)C, flags) != 0.f;
+                    const uint32_t cell = c.base + (uint32_t)(ty - c.ty0) * c.width + (uint32_t)(tx - c.tx0);
+                    const uint2 here = cellPacked[cell];
+                    off = here.x;
+                    n = cellPacked[cell + 1].x - off;
+                    C = (int32_t)here.y;
+                    // A path with no primitives here covers the tile with its corner
+                    // winding's coverage; when that is zero it draws nothing, and a
+                    // clip update's zero-coverage write reads the same as no write
+                    // (every clip draw has a fresh ID), so it is not listed.
+                    listed = n != 0;
+                    if (n == 0 && C != 0)
+                    {
+                        const uint32_t flags = pathPaint[p].kind & 0xffffu;
+                        listed = !(flags & 1u) && fill_rule_coverage((float)C, flags) != 0.f;
+                    }
                 }
             }
-        }
-        const uint32_t vSA = slot_arrays(n);
-        uint32_t iSA = vSA, iP = n;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1)
-        {
-            const uint32_t a0 = (uint32_t)__shfl_up((int)iSA, d), a1 = (uint32_t)__shfl_up((int)iP, d);
-            if (lane >= d)
+            const uint32_t vSA = slot_arrays(n);
+            uint32_t iSA = vSA, iP = n;
+    #pragma unroll
+            for (int d = 1; d < 64; d <<= 1)
             {
-                iSA += a0;
-                iP += a1;
-            }
-        }
-        const uint64_t bal = __ballot(listed);
-        const uint32_t posInWave = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-        if (lane == 63)
-        {
-            sWave[0][wave] = iSA;
-            sWave[1][wave] = iP;
-            sWave[2][wave] = (uint32_t)__popcll(bal);
-        }
-        __syncthreads();
-        uint32_t preSA = 0, preP = 0, preL = 0, nRel = 0;
-#pragma unroll
-        for (int w = 0; w < kWgThreads / 64; ++w)
-        {
-            if (w < wave)
-            {
-                preSA += sWave[0][w];
-                preP += sWave[1][w];
-                preL += sWave[2][w];
-            }
-            nRel += sWave[2][w];
-        }
-        nRel = uniform_u32(nRel);
-        if (listed)
-        {
-            const uint32_t q = preL + posInWave;
-            sRel[q] = make_uint4(p, n, (uint32_t)C, preSA + iSA - vSA);
-            sRelB[q] = make_uint2(off, preP + iP - n);
-        }
-        __syncthreads();
-
-        for (uint32_t s = 0; s < nRel;)
-        {
-            // ---- the batch: entries s .. s+len-1, cut where a limit breaks
-            const uint32_t baseP = uniform_u32(sRelB[s].y), baseSA = uniform_u32(sRel[s].w);
-            uint32_t len;
-            {
-                const uint32_t i = s + (uint32_t)lane;
-                bool stop = true;
-                if (i < nRel)
+                const uint32_t a0 = (uint32_t)__shfl_up((int)iSA, d), a1 = (uint32_t)__shfl_up((int)iP, d);
+                if (lane >= d)
                 {
-                    const uint32_t ni = sRel[i].y;
-                    const uint32_t sa = sRel[i].w + slot_arrays(ni) - baseSA;
-                    const uint32_t primsIncl = sRelB[i].y + ni - baseP;
-                    stop = (sa >> 16) > (uint32_t)kAccArrays || (primsIncl > kBatchPrims && (sa & 0xffffu) > 1u);
+                    iSA += a0;
+                    iP += a1;
                 }
-                const uint64_t m = __ballot(stop);
-                len = uniform_u32(m ? (uint32_t)__builtin_ctzll(m) : (uint32_t)kBatchEntries);
             }
-            const uint32_t last = s + len - 1;
-            const uint32_t nPrim = uniform_u32(sRelB[last].y + sRel[last].y - baseP);
-            const uint32_t arraysUsed = uniform_u32((sRel[last].w + slot_arrays(sRel[last].y) - baseSA) >> 16);
-
-            if (arraysUsed != 0)
+            const uint64_t bal = __ballot(listed);
+            const uint32_t posInWave = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+            if (lane == 63)
             {
-                // (the arrays and sLeft are clear: set at the start and
-                // cleared again after the previous batch's resolve)
-                for (uint32_t c0 = 0; c0 < nPrim; c0 += kBatchPrims)
+                sWave[0][wave] = iSA;
+                sWave[1][wave] = iP;
+                sWave[2][wave] = (uint32_t)__popcll(bal);
+            }
+            __syncthreads();
+            uint32_t preSA = 0, preP = 0, preL = 0, nRel = 0;
+    #pragma unroll
+            for (int w = 0; w < kWgThreads / 64; ++w)
+            {
+                if (w < wave)
                 {
-                    // ---- one primitive per thread
-                    const uint32_t f = c0 + (uint32_t)t;
-                    uint32_t rowsF = 0, rowsS = 0, info = 0, primIdx = 0;
-                    if (f < nPrim)
-                    {
-                        // owning entry: the last one whose first primitive is at or
-                        // before f (a batch has at most 64 entries: six fixed steps,
-                        // selects rather than branches)
-                        uint32_t lo = s;
-#pragma unroll
-                        for (uint32_t step = kBatchEntries / 2; step > 0; step >>= 1)
-                        {
-                            const uint32_t cand = lo + step;
-                            const uint32_t probe = sRelB[min(cand, last)].y - baseP;
-                            lo = (cand <= last && probe <= f) ? cand : lo;
-                        }
-                        const uint4 r = sRel[lo];
-                        const uint2 rb = sRelB[lo];
-                        const uint32_t a = (r.w - baseSA) >> 16;
-                        const bool wide = r.y > kPackedPrims;
-                        const bool stroke = pathPaint[r.x].kind & 1u;
-                        primIdx = binned[rb.x + (f - (rb.y - baseP))];
-                        const Prim pm = prims[primIdx];
-                        int r0, r1;
-                        prim_rows(pm, g, &r0, &r1);
-                        const uint32_t rows = (uint32_t)max(r1 - r0 + 1, 0);
-                        if (stroke)
-                            rowsS = rows;
-                        else
-                        {
-                            rowsF = rows;
-                            fill_prim_left(pm, g, sLeft[a]);
-                        }
-                        info = a | (wide ? 16u : 0u) | (stroke ? 32u : 0u) | ((uint32_t)min(r0, kTile - 1) << 8);
-                    }
-                    // exclusive prefix of the row counts, fills then strokes
-                    const uint32_t v = rowsF | (rowsS << 16);
-                    uint32_t inc = v;
-#pragma unroll
-                    for (int d = 1; d < 64; d <<= 1)
-                    {
-                        const uint32_t a0 = (uint32_t)__shfl_up((int)inc, d);
-                        if (lane >= d)
-                            inc += a0;
-                    }
-                    if (lane == 63)
-                        sWave[0][wave] = inc;
-                    __syncthreads();
-                    uint32_t pre = 0, tot = 0;
-#pragma unroll
-                    for (int w = 0; w < kWgThreads / 64; ++w)
-                    {
-                        const uint32_t x = sWave[0][w];
-                        if (w < wave)
-                            pre += x;
-                        tot += x;
-                    }
-                    const uint32_t ex = pre + inc - v;
-                    const uint32_t totF = tot & 0xffffu, total = uniform_u32(totF + (tot >> 16));
-                    const uint32_t item0 = rowsS != 0 ? totF + (ex >> 16) : (ex & 0xffffu);
-                    for (uint32_t k = 0; k < rowsF + rowsS; ++k)
-                        sOwner[item0 + k] = (uint8_t)t;
-                    sItemPrim[t] = primIdx;
-                    sItemInfo[t] = info | (item0 << 16);
-                    __syncthreads();
+                    preSA += sWave[0][w];
+                    preP += sWave[1][w];
+                    preL += sWave[2][w];
+                }
+                nRel += sWave[2][w];
+            }
+            nRel = uniform_u32(nRel);
+            if (listed)
+            {
+                const uint32_t q = preL + posInWave;
+                sRel[q] = make_uint4(p, n, (uint32_t)C, preSA + iSA - vSA);
+                sRelB[q] = make_uint2(off, preP + iP - n);
+            }
+            __syncthreads();
 
-                    // ---- (primitive, row) items
-                    for (uint32_t it = (uint32_t)t; it < total; it += kWgThreads)
+            for (uint32_t s = 0; s < nRel;)
+            {
+                // ---- the batch: entries s .. s+len-1, cut where a limit breaks
+                const uint32_t baseP = uniform_u32(sRelB[s].y), baseSA = uniform_u32(sRel[s].w);
+                uint32_t len;
+                {
+                    const uint32_t i = s + (uint32_t)lane;
+                    bool stop = true;
+                    if (i < nRel)
                     {
-                        const uint32_t owner = sOwner[it];
-                        const uint32_t inf = sItemInfo[owner];
-                        const int a = (int)(inf & 15u);
-                        const int row = (int)((inf >> 8) & 15u) + (int)(it - (inf >> 16));
-                        const uint32_t pIdx = sItemPrim[owner];
-                        const Prim pm = prims[pIdx];
-                        if (inf & 32u)
+                        const uint32_t ni = sRel[i].y;
+                        const uint32_t sa = sRel[i].w + slot_arrays(ni) - baseSA;
+                        const uint32_t primsIncl = sRelB[i].y + ni - baseP;
+                        stop = (sa >> 16) > (uint32_t)kAccArrays || (primsIncl > kBatchPrims && (sa & 0xffffu) > 1u);
+                    }
+                    const uint64_t m = __ballot(stop);
+                    len = uniform_u32(m ? (uint32_t)__builtin_ctzll(m) : (uint32_t)kBatchEntries);
+                }
+                const uint32_t last = s + len - 1;
+                const uint32_t nPrim = uniform_u32(sRelB[last].y + sRel[last].y - baseP);
+                const uint32_t arraysUsed = uniform_u32((sRel[last].w + slot_arrays(sRel[last].y) - baseSA) >> 16);
+
+                if (arraysUsed != 0)
+                {
+                    // (the arrays and sLeft are clear: set at the start and
+                    // cleared again after the previous batch's resolve)
+                    for (uint32_t c0 = 0; c0 < nPrim; c0 += kBatchPrims)
+                    {
+                        // ---- one primitive per thread
+                        const uint32_t f = c0 + (uint32_t)t;
+                        uint32_t rowsF = 0, rowsS = 0, info = 0, primIdx = 0;
+                        if (f < nPrim)
                         {
-                            if (FEATHER && (pm.flags & kPrimFeatherStroke))
-                                feather_stroke_prim_row(prims, pIdx, row, g.X0, g.Y0,
-                                                        reinterpret_cast<uint32_t*>(sAcc[a]), featherTable);
+                            // owning entry: the last one whose first primitive is at or
+                            // before f (a batch has at most 64 entries: six fixed steps,
+                            // selects rather than branches)
+                            uint32_t lo = s;
+    #pragma unroll
+                            for (uint32_t step = kBatchEntries / 2; step > 0; step >>= 1)
+                            {
+                                const uint32_t cand = lo + step;
+                                const uint32_t probe = sRelB[min(cand, last)].y - baseP;
+                                lo = (cand <= last && probe <= f) ? cand : lo;
+                            }
+                            const uint4 r = sRel[lo];
+                            const uint2 rb = sRelB[lo];
+                            const uint32_t a = (r.w - baseSA) >> 16;
+                            const bool wide = r.y > kPackedPrims;
+                            const bool stroke = pathPaint[r.x].kind & 1u;
+                            primIdx = binned[rb.x + (f - (rb.y - baseP))];
+                            const Prim pm = prims[primIdx];
+                            int r0, r1;
+                            prim_rows(pm, g, &r0, &r1);
+                            const uint32_t rows = (uint32_t)max(r1 - r0 + 1, 0);
+                            if (stroke)
+                                rowsS = rows;
                             else
-                                stroke_prim_row<double>(pm, row, g, reinterpret_cast<uint32_t*>(sAcc[a]));
+                            {
+                                rowsF = rows;
+                                fill_prim_left(pm, g, sLeft[a]);
+                            }
+                            info = a | (wide ? 16u : 0u) | (stroke ? 32u : 0u) | ((uint32_t)min(r0, kTile - 1) << 8);
                         }
-                        else if (FEATHER && (pm.flags & kPrimFeather))
-                            feather_prim_row(prims, pIdx, row, g.X0, g.Y0, sAcc[a], sAcc[a + ((inf >> 4) & 1u)],
-                                             (inf & 16u) ? 1 : (1 << 24), featherTable);
-                        else if (inf & 16u)
-                            fill_prim_row<double>(pm, row, g, sAcc[a], sAcc[a + 1], 1);
-                        else
-                            fill_prim_row<double>(pm, row, g, sAcc[a], sAcc[a], 1 << 24);
+                        // exclusive prefix of the row counts, fills then strokes
+                        const uint32_t v = rowsF | (rowsS << 16);
+                        uint32_t inc = v;
+    #pragma unroll
+                        for (int d = 1; d < 64; d <<= 1)
+                        {
+                            const uint32_t a0 = (uint32_t)__shfl_up((int)inc, d);
+                            if (lane >= d)
+                                inc += a0;
+                        }
+                        if (lane == 63)
+                            sWave[0][wave] = inc;
+                        __syncthreads();
+                        uint32_t pre = 0, tot = 0;
+    #pragma unroll
+                        for (int w = 0; w < kWgThreads / 64; ++w)
+                        {
+                            const uint32_t x = sWave[0][w];
+                            if (w < wave)
+                                pre += x;
+                            tot += x;
+                        }
+                        const uint32_t ex = pre + inc - v;
+                        const uint32_t totF = tot & 0xffffu, total = uniform_u32(totF + (tot >> 16));
+                        const uint32_t item0 = rowsS != 0 ? totF + (ex >> 16) : (ex & 0xffffu);
+                        for (uint32_t k = 0; k < rowsF + rowsS; ++k)
+                            sOwner[item0 + k] = (uint8_t)t;
+                        sItemPrim[t] = primIdx;
+                        sItemInfo[t] = info | (item0 << 16);
+                        __syncthreads();
+
+                        // ---- (primitive, row) items
+                        for (uint32_t it = (uint32_t)t; it < total; it += kWgThreads)
+                        {
+                            const uint32_t owner = sOwner[it];
+                            const uint32_t inf = sItemInfo[owner];
+                            const int a = (int)(inf & 15u);
+                            const int row = (int)((inf >> 8) & 15u) + (int)(it - (inf >> 16));
+                            const uint32_t pIdx = sItemPrim[owner];
+                            const Prim pm = prims[pIdx];
+                            if (inf & 32u)
+                            {
+                                if (FEATHER && (pm.flags & kPrimFeatherStroke))
+                                    feather_stroke_prim_row(prims, pIdx, row, g.X0, g.Y0,
+                                                            reinterpret_cast<uint32_t*>(sAcc[a]), featherTable);
+                                else
+                                    stroke_prim_row<double>(pm, row, g, reinterpret_cast<uint32_t*>(sAcc[a]));
+                            }
+                            else if (FEATHER && (pm.flags & kPrimFeather))
+                                feather_prim_row(prims, pIdx, row, g.X0, g.Y0, sAcc[a], sAcc[a + ((inf >> 4) & 1u)],
+                                                 (inf & 16u) ? 1 : (1 << 24), featherTable);
+                            else if (inf & 16u)
+                                fill_prim_row<double>(pm, row, g, sAcc[a], sAcc[a + 1], 1);
+                            else
+                                fill_prim_row<double>(pm, row, g, sAcc[a], sAcc[a], 1 << 24);
+                        }
+                        __syncthreads();
                     }
+                    // left-edge deltas -> prefix over rows, in place: thread t
+                    // holds array t / 16, row t % 16, and a DPP row is one array
+                    (&sLeft[0][0])[t] = row_scan16((&sLeft[0][0])[t]);
                     __syncthreads();
                 }
-                // left-edge deltas -> prefix over rows, in place: thread t
-                // holds array t / 16, row t % 16, and a DPP row is one array
-                (&sLeft[0][0])[t] = row_scan16((&sLeft[0][0])[t]);
-                __syncthreads();
-            }
 
-            // ---- resolve the batch in submission order.  Every thread takes
-            // the same entry, so a path's coverage comes from its accumulators
-            // here: the winding at a pixel is the corner winding, plus the
-            // left-edge prefix at its row, plus the crossings of its row up to
-            // its column -- a 16-lane scan, a DPP row being a row of the tile.
-            // Each pixel's cells are cleared as they are read (a two-array
-            // path's border array too).
-            for (uint32_t e = 0; e < len; ++e)
-            {
-                const uint32_t i = s + e;
-                const uint4 rel = sRel[i];
-                const uint32_t path = uniform_u32(rel.x);
-                const uint32_t ni = uniform_u32(rel.y);
-                const int32_t C = (int32_t)uniform_u32(rel.z);
-                const PathPaint pr = pathPaint[path];
-                const uint32_t prFlags = pr.kind & 0xffffu;
-                if (ni == 0)
+                // ---- resolve the batch in submission order.  Every thread takes
+                // the same entry, so a path's coverage comes from its accumulators
+                // here: the winding at a pixel is the corner winding, plus the
+                // left-edge prefix at its row, plus the crossings of its row up to
+                // its column -- a 16-lane scan, a DPP row being a row of the tile.
+                // Each pixel's cells are cleared as they are read (a two-array
+                // path's border array too).
+                for (uint32_t e = 0; e < len; ++e)
                 {
-                    // No primitives here: the listing keeps such a path only
-                    // where its corner winding gives coverage, and an integer
-                    // winding's coverage is 1 under every fill rule.
-                    if (pr.kind & kPaintPlain)
-                        resolve_plain_full(pr, noise255, col);
-                    else if (!(pr.kind & kPaintNoOp))
-                        resolve_pixel(pr, raster + path, 1.f, px, py, noise255, ramps, gradHeight, images, col,
-                                      clip);
-                    continue;
-                }
-                float cov;
-                {
-                    const uint32_t a = uniform_u32((rel.w - baseSA) >> 16);
-                    const int32_t v = sAcc[a][t];
-                    sAcc[a][t] = 0;
-                    if (prFlags & 1u)
-                        cov = fminf(__int_as_float(v), 1.f);
-                    else
+                    const uint32_t i = s + e;
+                    const uint4 rel = sRel[i];
+                    const uint32_t path = uniform_u32(rel.x);
+                    const uint32_t ni = uniform_u32(rel.y);
+                    const int32_t C = (int32_t)uniform_u32(rel.z);
+                    const PathPaint pr = pathPaint[path];
+                    const uint32_t prFlags = pr.kind & 0xffffu;
+                    if (ni == 0)
                     {
-                        int32_t d, b;
-                        if (ni > kPackedPrims)
-                        {
-                            d = v;
-                            b = sAcc[a + 1][t];
-                            sAcc[a + 1][t] = 0;
-                        }
+                        // No primitives here: the listing keeps such a path only
+                        // where its corner winding gives coverage, and an integer
+                        // winding's coverage is 1 under every fill rule.
+                        if (pr.kind & kPaintPlain)
+                            resolve_plain_full(pr, noise255, col);
+                        else if (!(pr.kind & kPaintNoOp))
+                            resolve_pixel(pr, raster + path, 1.f, px, py, noise255, ramps, gradHeight, images, col,
+                                          clip);
+                        continue;
+                    }
+                    float cov;
+                    {
+                        const uint32_t a = uniform_u32((rel.w - baseSA) >> 16);
+                        const int32_t v = sAcc[a][t];
+                        sAcc[a][t] = 0;
+                        if (prFlags & 1u)
+                            cov = fminf(__int_as_float(v), 1.f);
                         else
                         {
-                            // |border| < 2^23: it is the low 24 bits, sign-extended
-                            d = (v + (1 << 23)) >> 24;
-                            b = (int32_t)((uint32_t)v << 8) >> 8;
+                            int32_t d, b;
+                            if (ni > kPackedPrims)
+                            {
+                                d = v;
+                                b = sAcc[a + 1][t];
+                                sAcc[a + 1][t] = 0;
+                            }
+                            else
+                            {
+                                // |border| < 2^23: it is the low 24 bits, sign-extended
+                                d = (v + (1 << 23)) >> 24;
+                                b = (int32_t)((uint32_t)v << 8) >> 8;
+                            }
+                            const int32_t left = sLeft[a][t >> kTileLog2];
+                            const int32_t w = C + left + row_scan16(d);
+                            cov = fill_rule_coverage(fmaf((float)b, 1.f / 65536.f, (float)w), prFlags); // exact scaling
                         }
-                        const int32_t left = sLeft[a][t >> kTileLog2];
-                        const int32_t w = C + left + row_scan16(d);
-                        cov = fill_rule_coverage(fmaf((float)b, 1.f / 65536.f, (float)w), prFlags); // exact scaling
                     }
+                    if ((cov != 0.f && !(pr.kind & kPaintNoOp)) || (prFlags & 8u))
+                        resolve_pixel(pr, raster + path, cov, px, py, noise255, ramps, gradHeight, images, col, clip);
                 }
-                if ((cov != 0.f && !(pr.kind & kPaintNoOp)) || (prFlags & 8u))
-                    resolve_pixel(pr, raster + path, cov, px, py, noise255, ramps, gradHeight, images, col, clip);
+                __syncthreads(); // every wave has read sLeft
+                (&sLeft[0][0])[t] = 0;
+                __syncthreads(); // the next batch accumulates into the arrays and sLeft
+                s += len;
             }
-            __syncthreads(); // every wave has read sLeft
-            (&sLeft[0][0])[t] = 0;
-            __syncthreads(); // the next batch accumulates into the arrays and sLeft
-            s += len;
         }
+        if (inImage)
+            target[(size_t)py * width + px] = col;
+        if (!PERSISTENT)
+            break;
     }
-    if (inImage)
-        target[(size_t)py * width + px] = col;
 }
 
 // Unpacks PathData/PaintData/PaintAux into PathRaster (gpu.cpp:859-1063,

@@ -124,6 +124,14 @@ class RequestResult(ctypes.Structure):
                 ("batch", ctypes.c_uint32), ("batch_size", ctypes.c_uint32), ("batch_bytes", ctypes.c_uint64)]
 
 
+class RasterConfig(ctypes.Structure):
+    _fields_ = [("tile_order", ctypes.c_uint32), ("persistent", ctypes.c_uint32),
+                ("waves_per_simd", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_ if n != "reserved"}
+
+
 EXPORTED = [
     "rive_hip_last_error", "rive_hip_create", "rive_hip_destroy", "rive_hip_buffer_resize",
     "rive_hip_buffer_map", "rive_hip_buffer_unmap", "rive_hip_target_create",
@@ -137,7 +145,8 @@ EXPORTED = [
     "rive_scene_world_transforms", "rive_scene_stream", "rive_ease_cubic", "rive_scene_request_bytes",
     "rive_hip_serve", "rive_cmd_record_stream", "rive_cmd_free", "rive_cmd_bytes", "rive_cmd_from_bytes",
     "rive_hip_render_commands", "rive_hip_frame_graph_create", "rive_hip_frame_graph_launch",
-    "rive_hip_frame_graph_destroy",
+    "rive_hip_frame_graph_destroy", "rive_hip_set_raster_config", "rive_hip_get_raster_config",
+    "rive_hip_autotune_frame",
 ]
 
 _lib = None
@@ -185,6 +194,10 @@ def lib():
     L.rive_stream_contour_bound.argtypes = [ctypes.POINTER(Stream)]
     L.rive_stream_contour_bound.restype = u32
     L.rive_hip_last_flush_stats.argtypes = [vp, ctypes.POINTER(FlushStats)]
+    L.rive_hip_set_raster_config.argtypes = [vp, ctypes.POINTER(RasterConfig)]
+    L.rive_hip_get_raster_config.argtypes = [vp, ctypes.POINTER(RasterConfig)]
+    L.rive_hip_autotune_frame.argtypes = [vp, ctypes.POINTER(FlushDesc), u32, u32, ctypes.POINTER(RasterConfig),
+                                          ctypes.POINTER(ctypes.c_float)]
     L.rive_hip_image_create.argtypes = [vp, u32, u32, u32, vp, ctypes.POINTER(vp)]
     L.rive_hip_image_destroy.argtypes = [vp]
     L.rive_hip_image_destroy.restype = None
@@ -479,6 +492,26 @@ class Context:
             self.flush(d)
             out.append(self.last_flush_stats())
         return out
+
+    def set_raster_config(self, tile_order=0, persistent=0, waves_per_simd=6):
+        """How the raster pass runs (every choice renders the same pixels)."""
+        cfg = RasterConfig(tile_order, persistent, waves_per_simd, 0)
+        check(self.L.rive_hip_set_raster_config(self.h, ctypes.byref(cfg)))
+
+    def autotune_frame(self, descs, trials=5):
+        """Times every raster config on a frame's logical flushes and keeps the
+        fastest; returns (chosen config dict, {config tuple: median ms})."""
+        arr = (FlushDesc * len(descs))(*descs)
+        cfg = RasterConfig()
+        times = (ctypes.c_float * 8)()
+        check(self.L.rive_hip_autotune_frame(self.h, arr, len(descs), trials, ctypes.byref(cfg), times))
+        keys = [(o, p, w) for o in (0, 1) for p in (0, 1) for w in (6, 5)]
+        return cfg.as_dict(), {k: float(times[i]) for i, k in enumerate(keys)}
+
+    def raster_config(self):
+        cfg = RasterConfig()
+        check(self.L.rive_hip_get_raster_config(self.h, ctypes.byref(cfg)))
+        return cfg.as_dict()
 
     def last_flush_stats(self):
         st = FlushStats()
