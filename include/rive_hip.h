@@ -208,6 +208,71 @@ int rive_hip_image_create(rive_hip_context* ctx, uint32_t width, uint32_t height
                           rive_hip_image** out);
 void rive_hip_image_destroy(rive_hip_image* image);
 
+/* ---- Encoded images (RenderContext::decodeImage, render_context.cpp:237-262,
+ * and the decoders behind Bitmap::decode, decoders/src).  PNG and sequential
+ * (baseline or extended, Huffman-coded, 8-bit) JPEG are decoded; WebP is
+ * recognised and refused with RIVE_HIP_ERROR_UNSUPPORTED, as the reference
+ * built with no_rive_webp refuses it; progressive JPEG is refused too. */
+enum rive_image_format
+{
+    RIVE_IMAGE_UNKNOWN = 0,
+    RIVE_IMAGE_PNG = 1,
+    RIVE_IMAGE_JPEG = 2,
+    RIVE_IMAGE_WEBP = 3
+};
+
+/* The format (by signature, bitmap_decoder_thirdparty.cpp:36-70) and size
+ * from the header alone.  *format is set even when the call fails. */
+int rive_image_probe(const uint8_t* bytes, size_t n, uint32_t* format, uint32_t* width, uint32_t* height);
+
+/* Decodes a PNG on the host into unpremultiplied RGBA8 (capacity >= width *
+ * height * 4 from rive_image_probe), after libpng's transformations in
+ * decode_png.cpp:87-116: palette and low-bit-depth gray expanded, tRNS to
+ * alpha, 16-bit samples stripped to their high byte, gray to RGB, alpha 255
+ * when absent.  Damaged data (CRC of a critical chunk, zlib stream, filter
+ * type) and data too short for the image fail, as libpng's errors make the
+ * reference's DecodePng return null. */
+int rive_png_decode(const uint8_t* bytes, size_t n, uint8_t* rgba, size_t capacity);
+
+/* A JPEG after entropy decoding (host side): the frame's components and
+ * quantization tables, and with coef != NULL (capacity >= coef_count) the
+ * quantized coefficients of every 8x8 block in natural order, component by
+ * component over each component's MCU-padded block grid. */
+typedef struct rive_jpeg_component_info
+{
+    uint32_t id, h, v, quant_table;
+    uint32_t scale_h, scale_v; /* inverse-DCT output per block side / 8 */
+    uint32_t rep_h, rep_v;     /* further replication to the image grid */
+    uint32_t blocks_w, blocks_h;
+    uint64_t coef_offset;
+} rive_jpeg_component_info;
+
+typedef struct rive_jpeg_info
+{
+    uint32_t width, height, component_count;
+    uint32_t ycc; /* 1: YCbCr, 0: RGB (3 components) or gray (1) */
+    uint32_t hmax, vmax;
+    uint64_t coef_count;
+    rive_jpeg_component_info components[4];
+    uint16_t quant[4][64]; /* natural order */
+} rive_jpeg_info;
+
+int rive_jpeg_decode_info(const uint8_t* bytes, size_t n, rive_jpeg_info* info, int16_t* coef, size_t capacity);
+
+/* Decodes an encoded image into a premultiplied RGBA8 image in HBM with its
+ * full mip chain: what RenderContext::decodeImage does with Bitmap::decode,
+ * pixelFormat(RGBAPremul) and makeImageTexture(..., generateRemainingMips).
+ * The container and entropy decoding run on the host; JPEG's inverse DCT,
+ * chroma upsampling and colour conversion, the premultiplication and the
+ * mips run on the device.  Images beyond 16384 texels a side fail with
+ * RIVE_HIP_ERROR_UNSUPPORTED before anything is decoded. */
+int rive_hip_image_decode(rive_hip_context* ctx, const uint8_t* bytes, size_t n, rive_hip_image** out);
+
+/* Reads one mip level back (premultiplied RGBA8); with rgba NULL only the
+ * level's width and height are returned. */
+int rive_hip_image_read(rive_hip_image* image, uint32_t level, uint8_t* rgba, size_t capacity, uint32_t* width,
+                        uint32_t* height);
+
 enum rive_hip_flush_mode
 {
     RIVE_HIP_FLUSH_SPANS = 0,

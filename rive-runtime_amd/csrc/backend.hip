@@ -7,6 +7,8 @@
 #include "../../include/rive_hip.h"
 #include "device_common.hpp"
 #include "raster_kernels.hip"
+#include "image_kernels.hip"
+#include "../host/image_decode.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -602,13 +604,15 @@ struct rive_hip_image
     rivehip::ImageDesc desc;
 };
 
-int rive_hip_image_create(rive_hip_context* ctx, uint32_t width, uint32_t height, uint32_t mipLevels,
-                          const uint8_t* rgba, rive_hip_image** out)
+} // extern "C"
+
+namespace
+{
+// An image of width x height with its mip levels allocated (full chain when
+// mipLevels is 0), texels unset.
+int image_alloc(rive_hip_context* ctx, uint32_t width, uint32_t height, uint32_t mipLevels, rive_hip_image** out)
 {
     using namespace rivehip;
-    if (ctx == nullptr || out == nullptr || rgba == nullptr || width == 0 || height == 0 || width > 16384 ||
-        height > 16384)
-        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "bad image arguments");
     uint32_t full = 1;
     while ((std::max(width, height) >> full) != 0)
         ++full;
@@ -630,25 +634,257 @@ int rive_hip_image_create(rive_hip_context* ctx, uint32_t width, uint32_t height
     if (hipMalloc(&texels, (size_t)total * 4) != hipSuccess)
         return fail(RIVE_HIP_ERROR_OUT_OF_MEMORY, "image allocation failed");
     desc.texels = texels;
-    auto* img = new rive_hip_image{ctx, desc};
-    hipError_t e = hipMemcpyAsync(texels, rgba, (size_t)width * height * 4, hipMemcpyHostToDevice, ctx->stream);
-    for (uint32_t l = 0; l + 1 < levels && e == hipSuccess; ++l)
+    *out = new rive_hip_image{ctx, desc};
+    return RIVE_HIP_OK;
+}
+
+// Enqueues levels 1.. from level 0 (2x2 box filter).
+hipError_t image_mips(rive_hip_context* ctx, rive_hip_image* img)
+{
+    const rivehip::ImageDesc& d = img->desc;
+    uint32_t* texels = const_cast<uint32_t*>(d.texels);
+    hipError_t e = hipSuccess;
+    for (uint32_t l = 0; l + 1 < d.levels && e == hipSuccess; ++l)
     {
-        const int sw = (int)std::max(width >> l, 1u), sh = (int)std::max(height >> l, 1u);
-        const int dw = (int)std::max(width >> (l + 1), 1u), dh = (int)std::max(height >> (l + 1), 1u);
-        mip_kernel<<<dim3((dw + 63) / 64, dh), 64, 0, ctx->stream>>>(texels, desc.levelOffset[l], sw, sh,
-                                                                      desc.levelOffset[l + 1], dw, dh);
+        const int sw = (int)std::max(d.width >> l, 1u), sh = (int)std::max(d.height >> l, 1u);
+        const int dw = (int)std::max(d.width >> (l + 1), 1u), dh = (int)std::max(d.height >> (l + 1), 1u);
+        mip_kernel<<<dim3((dw + 63) / 64, dh), 64, 0, ctx->stream>>>(texels, d.levelOffset[l], sw, sh,
+                                                                      d.levelOffset[l + 1], dw, dh);
         e = hipGetLastError();
     }
+    return e;
+}
+
+void image_free(rive_hip_image* img)
+{
+    (void)hipFree(const_cast<uint32_t*>(img->desc.texels));
+    delete img;
+}
+
+// Device scratch of a decode, freed on every exit.
+struct DecodeTemps
+{
+    std::vector<void*> ptrs;
+    ~DecodeTemps()
+    {
+        for (void* p : ptrs)
+            (void)hipFree(p);
+    }
+    template <typename T> T* alloc(size_t bytes)
+    {
+        void* p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(bytes, 4)) != hipSuccess)
+            return nullptr;
+        ptrs.push_back(p);
+        return static_cast<T*>(p);
+    }
+};
+
+// Level 0 of `img` from a decoded PNG (unpremultiplied RGBA8 on the host).
+int decode_png_level0(rive_hip_context* ctx, const std::vector<uint8_t>& rgba, rive_hip_image* img, DecodeTemps& t)
+{
+    const size_t n = (size_t)img->desc.width * img->desc.height;
+    uint32_t* staged = t.alloc<uint32_t>(n * 4);
+    if (staged == nullptr)
+        return fail(RIVE_HIP_ERROR_OUT_OF_MEMORY, "image decode staging allocation failed");
+    HIP_TRY(hipMemcpyAsync(staged, rgba.data(), n * 4, hipMemcpyHostToDevice, ctx->stream));
+    premultiply_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, ctx->stream>>>(
+        staged, const_cast<uint32_t*>(img->desc.texels), n);
+    HIP_TRY(hipGetLastError());
+    return RIVE_HIP_OK;
+}
+
+// Level 0 of `img` from an entropy-decoded JPEG: per component the inverse
+// DCT into a sample plane, then colour conversion into the texels.
+int decode_jpeg_level0(rive_hip_context* ctx, const rivehip::image::Jpeg& j, rive_hip_image* img, DecodeTemps& t)
+{
+    int16_t* coef = t.alloc<int16_t>(j.coef.size() * 2);
+    uint16_t* qt = t.alloc<uint16_t>(sizeof(j.qt));
+    if (coef == nullptr || qt == nullptr)
+        return fail(RIVE_HIP_ERROR_OUT_OF_MEMORY, "JPEG coefficient allocation failed");
+    HIP_TRY(hipMemcpyAsync(coef, j.coef.data(), j.coef.size() * 2, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(qt, j.qt, sizeof(j.qt), hipMemcpyHostToDevice, ctx->stream));
+    JpegColor color{};
+    color.components = (uint32_t)j.comps.size();
+    color.ycc = j.ycc ? 1u : 0u;
+    color.width = j.width;
+    color.height = j.height;
+    color.out = const_cast<uint32_t*>(img->desc.texels);
+    for (size_t k = 0; k < j.comps.size(); ++k)
+    {
+        const rivehip::image::JpegComponent& c = j.comps[k];
+        JpegPlane p{};
+        p.coef = coef + c.coefOffset;
+        p.qt = qt + 64 * c.tq;
+        p.blocksW = c.blocksW;
+        p.blocksH = c.blocksH;
+        p.scaleH = c.scaleH;
+        p.scaleV = c.scaleV;
+        p.stride = c.blocksW * 8u * c.scaleH;
+        p.samples = t.alloc<uint8_t>((size_t)p.stride * c.blocksH * 8u * c.scaleV);
+        if (p.samples == nullptr)
+            return fail(RIVE_HIP_ERROR_OUT_OF_MEMORY, "JPEG sample plane allocation failed");
+        const uint32_t blocks = c.blocksW * c.blocksH;
+        jpeg_idct_kernel<<<(blocks + 63) / 64, 64, 0, ctx->stream>>>(p);
+        HIP_TRY(hipGetLastError());
+        color.plane[k] = p.samples;
+        color.stride[k] = p.stride;
+        color.repH[k] = c.repH;
+        color.repV[k] = c.repV;
+    }
+    jpeg_color_kernel<<<dim3((j.width + 255) / 256, j.height), 256, 0, ctx->stream>>>(color);
+    HIP_TRY(hipGetLastError());
+    return RIVE_HIP_OK;
+}
+
+int decode_error(const std::string& what, rivehip::image::Format f)
+{
+    return fail(f == rivehip::image::kWebp ? RIVE_HIP_ERROR_UNSUPPORTED : RIVE_HIP_ERROR_INVALID_ARGUMENT,
+                "image decode: " + what);
+}
+} // namespace
+
+extern "C" {
+
+int rive_hip_image_create(rive_hip_context* ctx, uint32_t width, uint32_t height, uint32_t mipLevels,
+                          const uint8_t* rgba, rive_hip_image** out)
+{
+    if (ctx == nullptr || out == nullptr || rgba == nullptr || width == 0 || height == 0 || width > 16384 ||
+        height > 16384)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "bad image arguments");
+    rive_hip_image* img = nullptr;
+    RIVE_TRY(image_alloc(ctx, width, height, mipLevels, &img));
+    hipError_t e = hipMemcpyAsync(const_cast<uint32_t*>(img->desc.texels), rgba, (size_t)width * height * 4,
+                                  hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = image_mips(ctx, img);
     if (e == hipSuccess)
         e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess)
     {
-        (void)hipFree(texels);
-        delete img;
+        image_free(img);
         return fail(RIVE_HIP_ERROR_DEVICE, hipGetErrorString(e));
     }
     *out = img;
+    return RIVE_HIP_OK;
+}
+
+int rive_image_probe(const uint8_t* bytes, size_t n, uint32_t* format, uint32_t* width, uint32_t* height)
+{
+    if (bytes == nullptr || format == nullptr || width == nullptr || height == nullptr)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+    rivehip::image::Format f;
+    std::string err;
+    const bool ok = rivehip::image::probe(bytes, n, &f, width, height, &err);
+    *format = f;
+    return ok ? RIVE_HIP_OK : decode_error(err, f);
+}
+
+int rive_png_decode(const uint8_t* bytes, size_t n, uint8_t* rgba, size_t capacity)
+{
+    if (bytes == nullptr || rgba == nullptr)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+    uint32_t w = 0, h = 0;
+    std::vector<uint8_t> px;
+    std::string err;
+    if (!rivehip::image::decode_png(bytes, n, &w, &h, &px, &err))
+        return decode_error(err, rivehip::image::kPng);
+    if (capacity < px.size())
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "rgba capacity is below width * height * 4");
+    std::memcpy(rgba, px.data(), px.size());
+    return RIVE_HIP_OK;
+}
+
+int rive_jpeg_decode_info(const uint8_t* bytes, size_t n, rive_jpeg_info* info, int16_t* coef, size_t capacity)
+{
+    if (bytes == nullptr || info == nullptr)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+    rivehip::image::Jpeg j;
+    std::string err;
+    const bool full = coef != nullptr;
+    if (!rivehip::image::decode_jpeg(bytes, n, &j, false, &err))
+        return decode_error(err, rivehip::image::kJpeg);
+    if (full && capacity < j.coefCount)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "coefficient capacity is below coef_count");
+    if (full && !rivehip::image::decode_jpeg(bytes, n, &j, true, &err))
+        return decode_error(err, rivehip::image::kJpeg);
+    *info = rive_jpeg_info{};
+    info->width = j.width;
+    info->height = j.height;
+    info->component_count = (uint32_t)j.comps.size();
+    info->ycc = j.ycc ? 1u : 0u;
+    info->hmax = j.hmax;
+    info->vmax = j.vmax;
+    info->coef_count = j.coefCount;
+    for (size_t k = 0; k < j.comps.size(); ++k)
+    {
+        const rivehip::image::JpegComponent& c = j.comps[k];
+        info->components[k] = {c.id,      c.h,     c.v,       c.tq,      c.scaleH,        c.scaleV,
+                               c.repH,    c.repV,  c.blocksW, c.blocksH, (uint64_t)c.coefOffset};
+    }
+    std::memcpy(info->quant, j.qt, sizeof(j.qt));
+    if (full)
+        std::memcpy(coef, j.coef.data(), j.coef.size() * 2);
+    return RIVE_HIP_OK;
+}
+
+int rive_hip_image_decode(rive_hip_context* ctx, const uint8_t* bytes, size_t n, rive_hip_image** out)
+{
+    if (ctx == nullptr || bytes == nullptr || out == nullptr)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    rivehip::image::Format f;
+    uint32_t w = 0, h = 0;
+    std::string err;
+    if (!rivehip::image::probe(bytes, n, &f, &w, &h, &err))
+        return decode_error(err, f);
+    if (w > 16384 || h > 16384)
+        return fail(RIVE_HIP_ERROR_UNSUPPORTED, "image decode: " + std::to_string(w) + "x" + std::to_string(h) +
+                                                    " exceeds the 16384-texel image limit");
+    std::vector<uint8_t> rgba;
+    rivehip::image::Jpeg j;
+    if (f == rivehip::image::kPng ? !rivehip::image::decode_png(bytes, n, &w, &h, &rgba, &err)
+                                  : !rivehip::image::decode_jpeg(bytes, n, &j, true, &err))
+        return decode_error(err, f);
+    rive_hip_image* img = nullptr;
+    RIVE_TRY(image_alloc(ctx, w, h, 0, &img));
+    int rc;
+    {
+        DecodeTemps temps;
+        rc = f == rivehip::image::kPng ? decode_png_level0(ctx, rgba, img, temps)
+                                       : decode_jpeg_level0(ctx, j, img, temps);
+        hipError_t e = rc == RIVE_HIP_OK ? image_mips(ctx, img) : hipSuccess;
+        // the temporaries are freed only once the stream is done with them
+        const hipError_t es = hipStreamSynchronize(ctx->stream);
+        if (rc == RIVE_HIP_OK && (e != hipSuccess || es != hipSuccess))
+            rc = fail(RIVE_HIP_ERROR_DEVICE, hipGetErrorString(e != hipSuccess ? e : es));
+    }
+    if (rc != RIVE_HIP_OK)
+    {
+        image_free(img);
+        return rc;
+    }
+    *out = img;
+    return RIVE_HIP_OK;
+}
+
+int rive_hip_image_read(rive_hip_image* img, uint32_t level, uint8_t* rgba, size_t capacity, uint32_t* width,
+                        uint32_t* height)
+{
+    if (img == nullptr || width == nullptr || height == nullptr)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null argument");
+    if (level >= img->desc.levels)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "level beyond the image's mip chain");
+    *width = std::max(img->desc.width >> level, 1u);
+    *height = std::max(img->desc.height >> level, 1u);
+    const size_t bytes = (size_t)*width * *height * 4;
+    if (rgba == nullptr)
+        return RIVE_HIP_OK;
+    if (capacity < bytes)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "rgba capacity is below the level's size");
+    HIP_TRY(hipSetDevice(img->ctx->device));
+    HIP_TRY(hipStreamSynchronize(img->ctx->stream));
+    HIP_TRY(hipMemcpy(rgba, img->desc.texels + img->desc.levelOffset[level], bytes, hipMemcpyDeviceToHost));
     return RIVE_HIP_OK;
 }
 

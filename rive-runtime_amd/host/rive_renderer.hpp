@@ -173,7 +173,22 @@ public:
 class RiveRenderImage : public RenderImage
 {
 public:
-    RiveRenderImage(rive_hip_image* image, uint32_t w, uint32_t h) : m_image(image), m_w(w), m_h(h) {}
+    // `owned`: the image is destroyed with this object (a decoded image);
+    // otherwise the caller keeps it.
+    RiveRenderImage(rive_hip_image* image, uint32_t w, uint32_t h, bool owned = false) :
+        m_image(image), m_w(w), m_h(h), m_owned(owned)
+    {}
+    ~RiveRenderImage() override
+    {
+        if (m_owned)
+            rive_hip_image_destroy(m_image);
+    }
+    RiveRenderImage(RiveRenderImage&& o) noexcept : m_image(o.m_image), m_w(o.m_w), m_h(o.m_h), m_owned(o.m_owned)
+    {
+        o.m_owned = false;
+    }
+    RiveRenderImage(const RiveRenderImage&) = delete;
+    RiveRenderImage& operator=(const RiveRenderImage&) = delete;
     rive_hip_image* handle() const { return m_image; }
     uint32_t width() const { return m_w; }
     uint32_t height() const { return m_h; }
@@ -181,6 +196,7 @@ public:
 private:
     rive_hip_image* m_image;
     uint32_t m_w, m_h;
+    bool m_owned;
 };
 class RenderPaint
 {
@@ -338,8 +354,8 @@ public:
     virtual std::shared_ptr<RiveRenderPath> makeRenderPath(RawPath& path, FillRule rule);
     virtual std::shared_ptr<RiveRenderPath> makeEmptyRenderPath();
     virtual std::shared_ptr<RiveRenderPaint> makeRenderPaint();
-    // Encoded-image decoding (PNG, JPEG, WebP) is not part of this backend:
-    // always null.
+    // An encoded image (PNG or JPEG) as a RenderImage; null when it cannot be
+    // decoded.  The base factory has no device: always null.
     virtual std::shared_ptr<RenderImage> decodeImage(const uint8_t* bytes, size_t size);
 };
 
@@ -413,6 +429,10 @@ public:
     // counts on the device answers here; false if it cannot.
     virtual bool countDraws(const std::vector<DrawRecord>& draws, std::vector<uint32_t>* tess,
                             std::vector<uint32_t>* contours) = 0;
+    // An encoded image decoded into a texture with its mip chain
+    // (RenderContext::decodeImage's Bitmap::decode + makeImageTexture,
+    // render_context.cpp:237-262); null when the backend cannot.
+    virtual std::shared_ptr<RiveRenderImage> decodeImageTexture(const uint8_t*, size_t) { return nullptr; }
 };
 
 class RenderContext : public Factory
@@ -442,6 +462,11 @@ public:
     bool prepareCanvases(rive_hip_target* target, FlushDescriptor* out);
     const FrameDescriptor& frameDescriptor() const { return m_frame; }
     RenderContextImpl* impl() { return m_impl.get(); }
+    // render_context.cpp:237-262: the backend decodes the image.
+    std::shared_ptr<RenderImage> decodeImage(const uint8_t* bytes, size_t size) override
+    {
+        return m_impl->decodeImageTexture(bytes, size);
+    }
 
     // The clip ID whose content the clip buffer holds after the draws pushed
     // so far (RenderContext::getClipContentID, render_context.hpp), as laid
@@ -587,6 +612,17 @@ public:
     // device (rive_hip_count_draws).
     bool countDraws(const std::vector<gpu::DrawRecord>& draws, std::vector<uint32_t>* tess,
                     std::vector<uint32_t>* contours) override;
+    // rive_hip_image_decode: host entropy decoding, device reconstruction,
+    // premultiplication and mips.
+    std::shared_ptr<RiveRenderImage> decodeImageTexture(const uint8_t* bytes, size_t size) override
+    {
+        rive_hip_image* h = nullptr;
+        uint32_t format = 0, w = 0, hgt = 0;
+        if (rive_image_probe(bytes, size, &format, &w, &hgt) != RIVE_HIP_OK ||
+            rive_hip_image_decode(m_ctx, bytes, size, &h) != RIVE_HIP_OK)
+            return nullptr;
+        return std::make_shared<RiveRenderImage>(h, w, hgt, /*owned=*/true);
+    }
 
 private:
     bool upload(int kind, const void* data, size_t bytes);

@@ -124,6 +124,20 @@ class RequestResult(ctypes.Structure):
                 ("batch", ctypes.c_uint32), ("batch_size", ctypes.c_uint32), ("batch_bytes", ctypes.c_uint64)]
 
 
+class JpegComponentInfo(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint32), ("h", ctypes.c_uint32), ("v", ctypes.c_uint32),
+                ("quant_table", ctypes.c_uint32), ("scale_h", ctypes.c_uint32), ("scale_v", ctypes.c_uint32),
+                ("rep_h", ctypes.c_uint32), ("rep_v", ctypes.c_uint32), ("blocks_w", ctypes.c_uint32),
+                ("blocks_h", ctypes.c_uint32), ("coef_offset", ctypes.c_uint64)]
+
+
+class JpegInfo(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("component_count", ctypes.c_uint32),
+                ("ycc", ctypes.c_uint32), ("hmax", ctypes.c_uint32), ("vmax", ctypes.c_uint32),
+                ("coef_count", ctypes.c_uint64), ("components", JpegComponentInfo * 4),
+                ("quant", (ctypes.c_uint16 * 64) * 4)]
+
+
 class RasterConfig(ctypes.Structure):
     _fields_ = [("tile_order", ctypes.c_uint32), ("persistent", ctypes.c_uint32),
                 ("waves_per_simd", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
@@ -146,7 +160,8 @@ EXPORTED = [
     "rive_hip_serve", "rive_cmd_record_stream", "rive_cmd_free", "rive_cmd_bytes", "rive_cmd_from_bytes",
     "rive_hip_render_commands", "rive_hip_frame_graph_create", "rive_hip_frame_graph_launch",
     "rive_hip_frame_graph_destroy", "rive_hip_set_raster_config", "rive_hip_get_raster_config",
-    "rive_hip_autotune_frame",
+    "rive_hip_autotune_frame", "rive_image_probe", "rive_png_decode", "rive_jpeg_decode_info",
+    "rive_hip_image_decode", "rive_hip_image_read",
 ]
 
 _lib = None
@@ -194,6 +209,12 @@ def lib():
     L.rive_stream_contour_bound.argtypes = [ctypes.POINTER(Stream)]
     L.rive_stream_contour_bound.restype = u32
     L.rive_hip_last_flush_stats.argtypes = [vp, ctypes.POINTER(FlushStats)]
+    L.rive_image_probe.argtypes = [vp, ctypes.c_size_t, ctypes.POINTER(u32), ctypes.POINTER(u32),
+                                   ctypes.POINTER(u32)]
+    L.rive_png_decode.argtypes = [vp, ctypes.c_size_t, vp, ctypes.c_size_t]
+    L.rive_jpeg_decode_info.argtypes = [vp, ctypes.c_size_t, ctypes.POINTER(JpegInfo), vp, ctypes.c_size_t]
+    L.rive_hip_image_decode.argtypes = [vp, vp, ctypes.c_size_t, ctypes.POINTER(vp)]
+    L.rive_hip_image_read.argtypes = [vp, u32, vp, ctypes.c_size_t, ctypes.POINTER(u32), ctypes.POINTER(u32)]
     L.rive_hip_set_raster_config.argtypes = [vp, ctypes.POINTER(RasterConfig)]
     L.rive_hip_get_raster_config.argtypes = [vp, ctypes.POINTER(RasterConfig)]
     L.rive_hip_autotune_frame.argtypes = [vp, ctypes.POINTER(FlushDesc), u32, u32, ctypes.POINTER(RasterConfig),
@@ -256,6 +277,38 @@ def free_stream(p):
 def contour_bound(stream):
     """Upper bound of the contours a stream produces (rive_stream_contour_bound)."""
     return int(lib().rive_stream_contour_bound(stream))
+
+
+IMAGE_FORMATS = {0: "unknown", 1: "png", 2: "jpeg", 3: "webp"}
+
+
+def image_probe(data):
+    """(format name, width, height) of an encoded image (rive_image_probe)."""
+    buf = bytes(data)
+    f, w, h = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+    check(lib().rive_image_probe(buf, len(buf), ctypes.byref(f), ctypes.byref(w), ctypes.byref(h)))
+    return IMAGE_FORMATS[f.value], w.value, h.value
+
+
+def png_decode(data):
+    """A PNG decoded on the host: unpremultiplied RGBA8, shape (h, w, 4)."""
+    buf = bytes(data)
+    _, w, h = image_probe(buf)
+    out = np.empty((h, w, 4), np.uint8)
+    check(lib().rive_png_decode(buf, len(buf), out.ctypes.data, out.nbytes))
+    return out
+
+
+def jpeg_decode_info(data, coefficients=True):
+    """(JpegInfo, int16 coefficients or None) of a JPEG after entropy decoding."""
+    buf = bytes(data)
+    info = JpegInfo()
+    check(lib().rive_jpeg_decode_info(buf, len(buf), ctypes.byref(info), None, 0))
+    if not coefficients:
+        return info, None
+    coef = np.empty(max(int(info.coef_count), 1), np.int16)
+    check(lib().rive_jpeg_decode_info(buf, len(buf), ctypes.byref(info), coef.ctypes.data, coef.size))
+    return info, coef[:int(info.coef_count)]
 
 
 def flush_groups(streams, limit=65535):
@@ -491,6 +544,25 @@ class Context:
                 d.target = target
             self.flush(d)
             out.append(self.last_flush_stats())
+        return out
+
+    def image_decode(self, data):
+        """An encoded image decoded into a device image with its mip chain."""
+        buf = bytes(data)
+        h = ctypes.c_void_p()
+        check(self.L.rive_hip_image_decode(self.h, buf, len(buf), ctypes.byref(h)))
+        return h
+
+    def free_image(self, image):
+        self.L.rive_hip_image_destroy(image)
+
+    def image_read(self, image, level=0):
+        """One mip level of a device image, premultiplied RGBA8 (h, w, 4)."""
+        w, h = ctypes.c_uint32(), ctypes.c_uint32()
+        check(self.L.rive_hip_image_read(image, level, None, 0, ctypes.byref(w), ctypes.byref(h)))
+        out = np.empty((h.value, w.value, 4), np.uint8)
+        check(self.L.rive_hip_image_read(image, level, out.ctypes.data, out.nbytes, ctypes.byref(w),
+                                         ctypes.byref(h)))
         return out
 
     def set_raster_config(self, tile_order=0, persistent=0, waves_per_simd=6):

@@ -24,7 +24,7 @@ HEADER = os.path.join(ROOT, "include", "rive_hip.h")
 def header_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(rive_(?:hip|stream|scene|ease|cmd)_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(rive_(?:hip|stream|scene|ease|cmd|image|png|jpeg)_\w+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_function():
