@@ -209,10 +209,11 @@ int rive_hip_image_create(rive_hip_context* ctx, uint32_t width, uint32_t height
 void rive_hip_image_destroy(rive_hip_image* image);
 
 /* ---- Encoded images (RenderContext::decodeImage, render_context.cpp:237-262,
- * and the decoders behind Bitmap::decode, decoders/src).  PNG and sequential
- * (baseline or extended, Huffman-coded, 8-bit) JPEG are decoded; WebP is
- * recognised and refused with RIVE_HIP_ERROR_UNSUPPORTED, as the reference
- * built with no_rive_webp refuses it; progressive JPEG is refused too. */
+ * and the decoders behind Bitmap::decode, decoders/src).  PNG and
+ * Huffman-coded 8-bit JPEG (baseline, extended and progressive) are decoded;
+ * WebP is recognised and refused with RIVE_HIP_ERROR_UNSUPPORTED, as the
+ * reference built with no_rive_webp refuses it; lossless, hierarchical and
+ * arithmetic-coded JPEG are refused too. */
 enum rive_image_format
 {
     RIVE_IMAGE_UNKNOWN = 0,
@@ -234,10 +235,11 @@ int rive_image_probe(const uint8_t* bytes, size_t n, uint32_t* format, uint32_t*
  * reference's DecodePng return null. */
 int rive_png_decode(const uint8_t* bytes, size_t n, uint8_t* rgba, size_t capacity);
 
-/* A JPEG after entropy decoding (host side): the frame's components and
- * quantization tables, and with coef != NULL (capacity >= coef_count) the
- * quantized coefficients of every 8x8 block in natural order, component by
- * component over each component's MCU-padded block grid. */
+/* A JPEG after entropy decoding (host side; every scan of a progressive
+ * file applied): the frame's components and quantization tables, and with
+ * coef != NULL (capacity >= coef_count) the quantized coefficients of every
+ * 8x8 block in natural order, component by component over each component's
+ * MCU-padded block grid. */
 typedef struct rive_jpeg_component_info
 {
     uint32_t id, h, v, quant_table;

@@ -1,4 +1,5 @@
-// PNG and sequential-JPEG decoding, host side (see image_decode.hpp).
+// PNG and JPEG (sequential and progressive) decoding, host side (see
+// image_decode.hpp).
 #include "image_decode.hpp"
 
 #include <algorithm>
@@ -762,6 +763,7 @@ bool decode_jpeg(const uint8_t* b, size_t n, Jpeg* j, bool coefficients, std::st
     *j = Jpeg{};
     JpegHuffman dc[4], ac[4];
     bool qtDefined[4] = {false, false, false, false};
+    bool progressive = false;
     bool sawFrame = false, sawJfif = false, sawAdobe = false;
     int adobeTransform = -1;
     uint32_t restartInterval = 0;
@@ -793,8 +795,9 @@ bool decode_jpeg(const uint8_t* b, size_t n, Jpeg* j, bool coefficients, std::st
         const uint8_t* s = b + pos + 2;
         const size_t sl = len - 2;
         pos += len;
-        if (m == 0xc0 || m == 0xc1)
+        if (m == 0xc0 || m == 0xc1 || m == 0xc2)
         {
+            progressive = m == 0xc2;
             if (sawFrame || sl < 6)
                 return fail(err, "bad or repeated JPEG frame header");
             if (s[0] != 8)
@@ -819,8 +822,8 @@ bool decode_jpeg(const uint8_t* b, size_t n, Jpeg* j, bool coefficients, std::st
             }
             sawFrame = true;
         }
-        else if ((m >= 0xc2 && m <= 0xcb && m != 0xc4 && m != 0xc8) || (m >= 0xcd && m <= 0xcf))
-            return fail(err, "progressive, lossless, hierarchical and arithmetic-coded JPEGs are not supported");
+        else if ((m >= 0xc3 && m <= 0xcb && m != 0xc4 && m != 0xc8) || (m >= 0xcd && m <= 0xcf))
+            return fail(err, "lossless, hierarchical and arithmetic-coded JPEGs are not supported");
         else if (m == 0xc4)
         {
             size_t q = 0;
@@ -954,12 +957,21 @@ bool decode_jpeg(const uint8_t* b, size_t n, Jpeg* j, bool coefficients, std::st
                     return fail(err, "JPEG scan names an unknown component");
                 td[i] = s[2 + 2 * i] >> 4;
                 ta[i] = s[2 + 2 * i] & 15;
-                if (td[i] > 3 || ta[i] > 3 || !dc[td[i]].defined || !ac[ta[i]].defined)
-                    return fail(err, "JPEG scan uses an undefined Huffman table");
             }
-            const uint8_t ss = s[1 + 2 * ns], se = s[2 + 2 * ns], ahal = s[3 + 2 * ns];
-            if (ss != 0 || se != 63 || ahal != 0)
+            const int ss = s[1 + 2 * ns], se = s[2 + 2 * ns], ah = s[3 + 2 * ns] >> 4, al = s[3 + 2 * ns] & 15;
+            if (!progressive && (ss != 0 || se != 63 || ah != 0 || al != 0))
                 return fail(err, "bad sequential JPEG scan parameters");
+            // progressive scans (T.81 G.1): a DC scan (Ss = Se = 0) or one
+            // component's band of AC coefficients, each a first pass or a
+            // refinement of one more bit
+            if (progressive && (se < ss || se > 63 || (ss == 0 && se != 0) || (ss != 0 && ns != 1) || al > 13 ||
+                                ah > 13 || (ah != 0 && ah != al + 1)))
+                return fail(err, "bad progressive JPEG scan parameters");
+            const bool needDc = ss == 0, needAc = se > 0;
+            for (int i = 0; i < ns; ++i)
+                if (td[i] > 3 || ta[i] > 3 || (needDc && ah == 0 && !dc[td[i]].defined) ||
+                    (needAc && !ac[ta[i]].defined))
+                    return fail(err, "JPEG scan uses an undefined Huffman table");
             // MCU geometry: interleaved scans cover the MCU grid, a
             // single-component scan the component's own block grid
             uint32_t mcusX, mcusY;
@@ -978,6 +990,115 @@ bool decode_jpeg(const uint8_t* b, size_t n, Jpeg* j, bool coefficients, std::st
             }
             JpegBits in{b, n, pos};
             int pred[4] = {0, 0, 0, 0};
+            uint32_t eobrun = 0; // progressive AC: blocks left in the current end-of-band run
+            const int p1 = 1 << al, m1 = -(1 << al);
+            // one block of a scan, by scan type (libjpeg jdhuff.c / jdphuff.c)
+            auto refine = [&](int16_t* coefp) {
+                if (*coefp != 0 && in.bits(1) != 0 && (*coefp & p1) == 0)
+                    *coefp = (int16_t)(*coefp + (*coefp >= 0 ? p1 : m1));
+            };
+            auto block = [&](int16_t* blk, int i) {
+                if (!progressive)
+                {
+                    const int t = in.decode(dc[td[i]]);
+                    pred[i] += extend((int)in.bits(t & 15), t & 15);
+                    blk[0] = (int16_t)pred[i];
+                    for (int k = 1; k < 64;)
+                    {
+                        const int rs = in.decode(ac[ta[i]]);
+                        const int r = rs >> 4, sz = rs & 15;
+                        if (sz != 0)
+                        {
+                            k += r;
+                            blk[kNatural[k]] = (int16_t)extend((int)in.bits(sz), sz);
+                            ++k;
+                        }
+                        else if (r == 15)
+                            k += 16;
+                        else
+                            break;
+                    }
+                }
+                else if (ss == 0 && ah == 0) // DC first pass
+                {
+                    const int t = in.decode(dc[td[i]]);
+                    pred[i] += extend((int)in.bits(t & 15), t & 15);
+                    blk[0] = (int16_t)(pred[i] * (1 << al));
+                }
+                else if (ss == 0) // DC refinement: one more bit
+                {
+                    if (in.bits(1) != 0)
+                        blk[0] = (int16_t)(blk[0] | p1);
+                }
+                else if (ah == 0) // AC first pass
+                {
+                    if (eobrun > 0)
+                    {
+                        --eobrun;
+                        return;
+                    }
+                    for (int k = ss; k <= se; ++k)
+                    {
+                        const int rs = in.decode(ac[ta[i]]);
+                        const int r = rs >> 4, sz = rs & 15;
+                        if (sz != 0)
+                        {
+                            k += r;
+                            blk[kNatural[k]] = (int16_t)(extend((int)in.bits(sz), sz) * (1 << al));
+                        }
+                        else if (r == 15)
+                            k += 15;
+                        else
+                        {
+                            eobrun = (1u << r) - 1;
+                            if (r != 0)
+                                eobrun += in.bits(r);
+                            break;
+                        }
+                    }
+                }
+                else // AC refinement (T.81 G.1.2.3)
+                {
+                    int k = ss;
+                    if (eobrun == 0)
+                    {
+                        for (; k <= se; ++k)
+                        {
+                            const int rs = in.decode(ac[ta[i]]);
+                            int r = rs >> 4;
+                            int value = 0;
+                            if ((rs & 15) != 0)
+                                value = in.bits(1) != 0 ? p1 : m1; // a newly nonzero coefficient
+                            else if (r != 15)
+                            {
+                                eobrun = 1u << r;
+                                if (r != 0)
+                                    eobrun += in.bits(r);
+                                break;
+                            }
+                            // pass r zero-history coefficients, refining the
+                            // nonzero ones on the way
+                            do
+                            {
+                                int16_t* c = blk + kNatural[k];
+                                if (*c != 0)
+                                    refine(c);
+                                else if (--r < 0)
+                                    break;
+                                ++k;
+                            } while (k <= se);
+                            if (value != 0)
+                                blk[kNatural[k]] = (int16_t)value;
+                        }
+                    }
+                    if (eobrun > 0)
+                    {
+                        for (; k <= se; ++k)
+                            refine(blk + kNatural[k]);
+                        --eobrun;
+                    }
+                }
+            };
             uint32_t todo = restartInterval;
             bool skip = false; // data ran short: the rest of the interval stays zero
             for (uint32_t my = 0; my < mcusY; ++my)
@@ -989,6 +1110,7 @@ bool decode_jpeg(const uint8_t* b, size_t n, Jpeg* j, bool coefficients, std::st
                         {
                             in.restart();
                             pred[0] = pred[1] = pred[2] = pred[3] = 0;
+                            eobrun = 0;
                             todo = restartInterval;
                             skip = in.ranShort;
                         }
@@ -1004,26 +1126,7 @@ bool decode_jpeg(const uint8_t* b, size_t n, Jpeg* j, bool coefficients, std::st
                             for (int h = 0; h < bh; ++h)
                             {
                                 const uint32_t bx = mx * bh + h, by = my * bv + v;
-                                int16_t* blk = j->coef.data() + c.coefOffset + ((size_t)by * c.blocksW + bx) * 64;
-                                const int t = in.decode(dc[td[i]]);
-                                const int diff = extend((int)in.bits(t & 15), t & 15);
-                                pred[i] += diff;
-                                blk[0] = (int16_t)pred[i];
-                                for (int k = 1; k < 64;)
-                                {
-                                    const int rs = in.decode(ac[ta[i]]);
-                                    const int r = rs >> 4, sz = rs & 15;
-                                    if (sz != 0)
-                                    {
-                                        k += r;
-                                        blk[kNatural[k]] = (int16_t)extend((int)in.bits(sz), sz);
-                                        ++k;
-                                    }
-                                    else if (r == 15)
-                                        k += 16;
-                                    else
-                                        break;
-                                }
+                                block(j->coef.data() + c.coefOffset + ((size_t)by * c.blocksW + bx) * 64, i);
                             }
                     }
                     if (in.ranShort)

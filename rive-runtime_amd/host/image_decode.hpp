@@ -1,5 +1,5 @@
 // Encoded-image decoding on the host side: the container and entropy-coded
-// parts of PNG and baseline JPEG, which are sequential by nature.  The
+// parts of PNG and JPEG, which are sequential by nature.  The
 // per-pixel work that follows (JPEG dequantization, inverse DCT, chroma
 // upsampling and colour conversion; premultiplication; mip generation) runs
 // on the device (image kernels in csrc/).
@@ -40,8 +40,8 @@ bool probe(const uint8_t* bytes, size_t n, Format* format, uint32_t* width, uint
 bool decode_png(const uint8_t* bytes, size_t n, uint32_t* width, uint32_t* height, std::vector<uint8_t>* rgba,
                 std::string* err);
 
-// A sequential (baseline or extended, Huffman-coded, 8-bit) JPEG after
-// entropy decoding: the quantized coefficients of every 8x8 block, in
+// A Huffman-coded 8-bit JPEG (baseline, extended or progressive) after
+// entropy decoding of all its scans: the quantized coefficients of every 8x8 block, in
 // natural (row-major) order, component by component, each component's blocks
 // row-major over its MCU-padded grid.
 struct JpegComponent

@@ -136,10 +136,150 @@ def _segment(marker, payload):
     return bytes([0xFF, marker]) + (len(payload) + 2).to_bytes(2, "big") + bytes(payload)
 
 
+# libjpeg's jpeg_simple_progression script for three components (jcparam.c):
+# (components, Ss, Se, Ah, Al).
+SIMPLE_PROGRESSION = [([0, 1, 2], 0, 0, 0, 1), ([0], 1, 5, 0, 2), ([2], 1, 63, 0, 1), ([1], 1, 63, 0, 1),
+                      ([0], 6, 63, 0, 2), ([0], 1, 63, 2, 1), ([0, 1, 2], 0, 0, 1, 0), ([2], 1, 63, 1, 0),
+                      ([1], 1, 63, 1, 0), ([0], 1, 63, 1, 0)]
+GRAY_PROGRESSION = [([0], 0, 0, 0, 1), ([0], 1, 5, 0, 2), ([0], 6, 63, 0, 2), ([0], 1, 63, 2, 1),
+                    ([0], 0, 0, 1, 0), ([0], 1, 63, 1, 0)]
+
+
+def _scan_units(comps, scan, mcus_x, mcus_y):
+    """A scan's MCUs as lists of (component, block row, block column)."""
+    units = []
+    if len(scan) == 1:
+        c = comps[scan[0]]
+        for by in range(-(-c["ch"] // 8)):
+            for bx in range(-(-c["cw"] // 8)):
+                units.append([(scan[0], by, bx)])
+    else:
+        for my in range(mcus_y):
+            for mx in range(mcus_x):
+                u = []
+                for k in scan:
+                    c = comps[k]
+                    for v in range(c["v"]):
+                        for hh in range(c["h"]):
+                            u.append((k, my * c["v"] + v, mx * c["h"] + hh))
+                units.append(u)
+    return units
+
+
+def _progressive_events(comps, units, ss, se, ah, al, restart):
+    """The scan's output as events: ("s", table key, symbol), ("b", value,
+    bits) and ("rst",), following libjpeg's jcphuff.c encoders."""
+    ev = []
+    state = {"eobrun": 0, "be": []}
+    key_ac = None
+
+    def emit_eobrun():
+        if state["eobrun"] > 0:
+            n = state["eobrun"].bit_length() - 1
+            ev.append(("s", key_ac, n << 4))
+            if n:
+                ev.append(("b", state["eobrun"] & ((1 << n) - 1), n))
+            state["eobrun"] = 0
+            ev.extend(("b", bit, 1) for bit in state["be"])
+            state["be"] = []
+
+    pred = {}
+    for i, u in enumerate(units):
+        if restart and i > 0 and i % restart == 0:
+            emit_eobrun()
+            ev.append(("rst",))
+        if i == 0 or (restart and i % restart == 0):
+            pred = {}
+        for (k, by, bx) in u:
+            c = comps[k]
+            z = [int(v) for v in c["coef"][by, bx][ZIGZAG]]
+            if ss == 0 and ah == 0:
+                v = z[0] >> al
+                diff = v - pred.get(k, 0)
+                pred[k] = v
+                s = category(diff)
+                ev.append(("s", ("dc", c["tq"]), s))
+                if s:
+                    ev.append(("b", magnitude_bits(diff, s), s))
+            elif ss == 0:
+                ev.append(("b", (z[0] >> al) & 1, 1))
+            elif ah == 0:
+                key_ac = ("ac", c["tq"])
+                r = 0
+                for kk in range(ss, se + 1):
+                    t = abs(z[kk]) >> al
+                    if t == 0:
+                        r += 1
+                        continue
+                    emit_eobrun()
+                    while r > 15:
+                        ev.append(("s", key_ac, 0xF0))
+                        r -= 16
+                    n = t.bit_length()
+                    ev.append(("s", key_ac, (r << 4) | n))
+                    ev.append(("b", t if z[kk] > 0 else (~t) & ((1 << n) - 1), n))
+                    r = 0
+                if r > 0:
+                    state["eobrun"] += 1
+                    if state["eobrun"] == 0x7FFF:
+                        emit_eobrun()
+            else:
+                key_ac = ("ac", c["tq"])
+                absv = {kk: abs(z[kk]) >> al for kk in range(ss, se + 1)}
+                eob = max([kk for kk in absv if absv[kk] == 1], default=0)
+                r, br = 0, []
+                for kk in range(ss, se + 1):
+                    t = absv[kk]
+                    if t == 0:
+                        r += 1
+                        continue
+                    while r > 15 and kk <= eob:
+                        emit_eobrun()
+                        ev.append(("s", key_ac, 0xF0))
+                        r -= 16
+                        ev.extend(("b", bit, 1) for bit in br)
+                        br = []
+                    if t > 1:
+                        br.append(t & 1)
+                        continue
+                    emit_eobrun()
+                    ev.append(("s", key_ac, (r << 4) | 1))
+                    ev.append(("b", 0 if z[kk] < 0 else 1, 1))
+                    ev.extend(("b", bit, 1) for bit in br)
+                    br = []
+                    r = 0
+                if r > 0 or br:
+                    state["eobrun"] += 1
+                    state["be"].extend(br)
+                    if state["eobrun"] == 0x7FFF or len(state["be"]) > 1000 - 64 + 1:
+                        emit_eobrun()
+    emit_eobrun()
+    return ev
+
+
+def _write_events(ev, codes):
+    out = BitWriter()
+    rst = 0
+    for e in ev:
+        if e[0] == "s":
+            code, length = codes[e[1]][e[2]]
+            out.put(code, length)
+        elif e[0] == "b":
+            out.put(e[1], e[2])
+        else:
+            out.flush()
+            out.out += bytes([0xFF, 0xD0 + rst])
+            rst = (rst + 1) & 7
+    out.flush()
+    return out.out
+
+
 def encode(image, quality=90, sampling=(2, 2), restart=0, ids=None, jfif=True, adobe=None,
-           separate_scans=False):
+           separate_scans=False, progressive=None):
     """Encodes an (h, w, 3) RGB or (h, w) gray uint8 image; returns (bytes,
-    [per-component int16 coefficient arrays of shape (blocks_h, blocks_w, 64)])."""
+    [per-component int16 coefficient arrays of shape (blocks_h, blocks_w, 64)]).
+    progressive: True for libjpeg's simple progression, or a list of
+    (components, Ss, Se, Ah, Al) scans."""
     img = np.asarray(image)
     gray = img.ndim == 2
     h, w = img.shape[:2]
@@ -194,9 +334,41 @@ def encode(image, quality=90, sampling=(2, 2), restart=0, ids=None, jfif=True, a
     sof = bytearray([8]) + h.to_bytes(2, "big") + w.to_bytes(2, "big") + bytes([len(comps)])
     for c in comps:
         sof += bytes([c["id"], (c["h"] << 4) | c["v"], c["tq"]])
-    header += _segment(0xC0, sof)
+    header += _segment(0xC2 if progressive else 0xC0, sof)
     if restart:
         header += _segment(0xDD, restart.to_bytes(2, "big"))
+    if progressive:
+        script = progressive if progressive is not True else (GRAY_PROGRESSION if gray else SIMPLE_PROGRESSION)
+        written = [np.zeros_like(c["coef"]) for c in comps]
+        body = bytearray()
+        for scan, ss, se, ah, al in script:
+            units = _scan_units(comps, scan, mcus_x, mcus_y)
+            for u in units:
+                for (k, by, bx) in u:
+                    lo, hi = (0, 1) if ss == 0 else (ss, se + 1)
+                    zz = ZIGZAG[lo:hi]
+                    written[k][by, bx, zz] = comps[k]["coef"][by, bx, zz]
+            ev = _progressive_events(comps, units, ss, se, ah, al, restart)
+            stats = {}
+            for e in ev:
+                if e[0] == "s":
+                    stats.setdefault(e[1], {}).setdefault(e[2], 0)
+                    stats[e[1]][e[2]] += 1
+            codes = {}
+            dht = bytearray()
+            for (kind, t), freq in sorted(stats.items()):
+                bits, values = huffman_table(freq)
+                codes[(kind, t)] = canonical_codes(bits, values)
+                dht += bytes([(0 if kind == "dc" else 0x10) | t]) + bytes(bits) + bytes(values)
+            if dht:
+                body += _segment(0xC4, dht)
+            sos = bytearray([len(scan)])
+            for k in scan:
+                sos += bytes([comps[k]["id"], (comps[k]["tq"] << 4) | comps[k]["tq"]])
+            sos += bytes([ss, se, (ah << 4) | al])
+            body += _segment(0xDA, sos)
+            body += _write_events(ev, codes)
+        return bytes(header + body + b"\xFF\xD9"), written
 
     written = [np.zeros_like(c["coef"]) for c in comps]
     body = bytearray()

@@ -12,7 +12,7 @@ of channel values): the device computes full-resolution components with
 libjpeg's 13-bit fixed-point transform, which may differ from the exact value
 by one step before colour conversion spreads it.  Synthetic images decode to
 within the quantization error of their source (PSNR >= 30 dB at quality 92),
-for 4:2:0, 4:2:2, 4:4:0, 4:4:4, gray and RGB-coded files.  Images beyond the
+for 4:2:0, 4:2:2, 4:4:0, 4:4:4, gray, RGB-coded and progressive files.  Images beyond the
 16384-texel limit (the reference's bad.jpg) and WebP are refused with
 RIVE_HIP_ERROR_UNSUPPORTED; the reference's bad.png is refused as its
 DecodePng refuses it.
@@ -96,7 +96,7 @@ def check_jpeg(levels, data):
 
 
 JPEG_CASES = [dict(sampling=(2, 2)), dict(sampling=(2, 1)), dict(sampling=(1, 2)), dict(sampling=(1, 1)),
-              dict(sampling=(2, 2), restart=4), dict(gray=True),
+              dict(sampling=(2, 2), restart=4), dict(gray=True), dict(sampling=(2, 2), progressive=True),
               dict(sampling=(1, 1), ids=[ord("R"), ord("G"), ord("B")], jfif=False)]
 
 

@@ -12,7 +12,9 @@ DecodePng does (image_decoders_test.cpp:49-70).
 
 JPEG: images written by a test encoder (tests/jpeg_test_encoder.py) give back
 exactly the quantized coefficients the encoder wrote, for 4:2:0, 4:2:2, 4:4:4
-and gray, interleaved or one component per scan, with restart intervals; a
+and gray, interleaved or one component per scan, with restart intervals, and
+for progressive files (libjpeg's simple progression: DC and AC successive
+approximation, spectral bands, end-of-band runs); a
 file cut short leaves the rest of its blocks zero (libjpeg's behaviour for
 insufficient data); the reference's JPEGs report the dimensions its test
 asserts (image_decoders_test.cpp:21-47).
@@ -243,6 +245,28 @@ def test_jpeg_coefficients_round_trip(case):
         rgb = case.get("ids") == [ord("R"), ord("G"), ord("B")] or case.get("adobe") == 0
         assert gray or info.ycc == (0 if rgb else 1)
         assert np.array_equal(coef, _flat(written)), f"{case} {w}x{h}"
+
+
+@pytest.mark.parametrize("sampling,restart", [((2, 2), 0), ((2, 1), 3), ((1, 1), 0), (None, 0), (None, 5)])
+def test_jpeg_progressive_round_trip(sampling, restart):
+    # libjpeg's simple progression: DC first/refinement, spectral bands and
+    # successive-approximation AC refinement (T.81 G.1.2), on noise (short
+    # end-of-band runs) and on a smooth image (runs of many blocks)
+    rng = np.random.default_rng(17)
+    gray = sampling is None
+    for w, h, smooth in [(37, 21, False), (64, 48, False), (160, 120, True)]:
+        if smooth:
+            yy, xx = np.mgrid[0:h, 0:w]
+            base = np.stack([128 + 60 * np.sin(xx / 23.0), 128 + 50 * np.cos(yy / 17.0), 90 + 0 * xx], -1)
+        else:
+            base = rng.integers(0, 256, (h, w, 3))
+        img = np.clip(base, 0, 255).astype(np.uint8)
+        if gray:
+            img = img[..., 0]
+        kw = {} if gray else {"sampling": sampling}
+        data, written = enc.encode(img, quality=50 if smooth else 85, progressive=True, restart=restart, **kw)
+        info, coef = rivehip.jpeg_decode_info(data)
+        assert np.array_equal(coef, _flat(written)), f"{sampling} {w}x{h} restart {restart}"
 
 
 def test_jpeg_truncated_leaves_zero_blocks():
