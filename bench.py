@@ -256,6 +256,62 @@ def serving_leg(args):
             "raster_autotune": tune_report(tuned, tune_ms), "rates": out}
 
 
+def image_decode_leg(args):
+    """Encoded-image decoding (RenderContext::decodeImage): a 2048x2048 RGBA
+    PNG and a 1024x768 4:2:0 JPEG, synthetic (a smooth gradient with noise),
+    each decoded into a premultiplied device image with its mip chain.  The
+    host share (inflate and row filters; JPEG entropy decoding) is timed
+    alone through the host-only entry points; the rest is upload and device
+    work (JPEG inverse DCT and colour conversion, premultiplication, mips)."""
+    import zlib
+
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import jpeg_test_encoder  # the test suite's JPEG writer makes the input
+
+    rng = np.random.default_rng(0x1A4E)
+    yy, xx = np.mgrid[0:2048, 0:2048]
+    img = np.stack([xx // 8, yy // 8, (xx + yy) // 16, 255 - xx // 16], -1) + rng.integers(0, 8, (2048, 2048, 4))
+    img = np.clip(img, 0, 255).astype(np.uint8)
+    rows = img.reshape(2048, -1).astype(np.int16)
+    sub = np.concatenate([rows[:, :4], rows[:, 4:] - rows[:, :-4]], 1).astype(np.uint8)  # filter 1 (Sub)
+    raw = np.concatenate([np.ones((2048, 1), np.uint8), sub], 1).tobytes()
+    import struct
+
+    def chunk(kind, body):
+        return struct.pack(">I", len(body)) + kind + body + struct.pack(">I", zlib.crc32(kind + body) & 0xFFFFFFFF)
+
+    png = (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", 2048, 2048, 8, 6, 0, 0, 0)) +
+           chunk(b"IDAT", zlib.compress(raw, 6)) + chunk(b"IEND", b""))
+    jimg = np.ascontiguousarray(img[:768, :1024, :3])
+    jpeg, _ = jpeg_test_encoder.encode(jimg, quality=90, sampling=(2, 2))
+    ctx = rivehip.Context(0)
+    out = {}
+    try:
+        for name, data, host in [("png_2048x2048_rgba", png, lambda d: rivehip.png_decode(d)),
+                                 ("jpeg_1024x768_420", jpeg, lambda d: rivehip.jpeg_decode_info(d))]:
+            _, w, h = rivehip.image_probe(data)
+            ctx.free_image(ctx.image_decode(data))  # warm-up
+            total, host_ms = [], []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                ctx.free_image(ctx.image_decode(data))
+                total.append((time.perf_counter() - t0) * 1e3)
+                t0 = time.perf_counter()
+                host(data)
+                host_ms.append((time.perf_counter() - t0) * 1e3)
+            t = statistics.median(total)
+            out[name] = {"bytes": len(data), "ms": round(t, 3), "host_ms": round(statistics.median(host_ms), 3),
+                         "mpixels_per_s": round(w * h / t / 1e3, 1)}
+            print(f"# image decode {name}: {out[name]}", file=sys.stderr, flush=True)
+    finally:
+        ctx.close()
+    out["note"] = ("ms: rive_hip_image_decode end to end (host decode, upload, device passes, mips, sync); "
+                   "host_ms: the host-only part alone")
+    return out
+
+
 def cfg3_leg(args):
     """The 4K feathered workload: 3840x2160, 20 000 seeded draws of the
     1080p mix with 25% feathered clockwise fills (feather 1..40 px), 20%
@@ -308,6 +364,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-cfg3", action="store_true", help="skip the 4K feathered leg")
     ap.add_argument("--cfg3-frames", type=int, default=3)
+    ap.add_argument("--no-image-decode", action="store_true", help="skip the image decoding leg")
     ap.add_argument("--serve-seconds", type=float, default=0.5, help="seconds of arrivals per rate")
     ap.add_argument("--serve-budget-gb", type=float, default=8.0)
     ap.add_argument("--serve-max-batch", type=int, default=64)
@@ -407,6 +464,7 @@ def main():
     batched = None if args.no_batched else batched_leg(args)
     serving = None if args.no_serving else serving_leg(args)
     cfg3 = None if args.no_cfg3 else cfg3_leg(args)
+    images = None if args.no_image_decode else image_decode_leg(args)
 
     total_s = sum(step_ms) / 1e3
     fps = args.steps / total_s
@@ -464,6 +522,7 @@ def main():
         "batched": batched,
         "serving": serving,
         "cfg3": cfg3,
+        "image_decode": images,
     }
     print(json.dumps(out), flush=True)
 

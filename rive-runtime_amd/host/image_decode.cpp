@@ -153,8 +153,9 @@ bool zlib_inflate(const uint8_t* src, size_t n, size_t expected, std::vector<uin
         return fail(err, "bad zlib header");
     if (flg & 0x20)
         return fail(err, "zlib preset dictionary is not allowed in PNG");
-    out->clear();
-    out->reserve(expected);
+    out->assign(expected, 0);
+    uint8_t* o = out->data();
+    size_t at = 0;
     BitReader in{src + 2, n - 2};
     Huffman lit, dist;
     bool last = false;
@@ -168,8 +169,10 @@ bool zlib_inflate(const uint8_t* src, size_t n, size_t expected, std::vector<uin
             const uint32_t len = in.bits(16), nlen = in.bits(16);
             if ((len ^ 0xffffu) != nlen)
                 return fail(err, "bad stored block length");
+            if (at + len > expected)
+                return fail(err, "too much image data");
             for (uint32_t i = 0; i < len; ++i)
-                out->push_back((uint8_t)in.bits(8));
+                o[at++] = (uint8_t)in.bits(8);
         }
         else if (type == 1 || type == 2)
         {
@@ -236,7 +239,11 @@ bool zlib_inflate(const uint8_t* src, size_t n, size_t expected, std::vector<uin
                 if (sym < 0 || in.overrun)
                     return fail(err, "bad literal/length code or data ends early");
                 if (sym < 256)
-                    out->push_back((uint8_t)sym);
+                {
+                    if (at >= expected)
+                        return fail(err, "too much image data");
+                    o[at++] = (uint8_t)sym;
+                }
                 else if (sym == 256)
                     break;
                 else
@@ -249,24 +256,25 @@ bool zlib_inflate(const uint8_t* src, size_t n, size_t expected, std::vector<uin
                     if (di < 0 || di >= 30)
                         return fail(err, "bad distance code");
                     const uint32_t d = kDistBase[di] + in.bits(kDistExtra[di]);
-                    if (d > out->size())
+                    if (d > at)
                         return fail(err, "distance before the start of the data");
-                    const size_t from = out->size() - d;
-                    for (uint32_t k = 0; k < len; ++k)
-                        out->push_back((*out)[from + k]);
+                    if (at + len > expected)
+                        return fail(err, "too much image data");
+                    if (d >= len)
+                        std::memcpy(o + at, o + at - d, len);
+                    else
+                        for (uint32_t k = 0; k < len; ++k)
+                            o[at + k] = o[at + k - d]; // overlapping: repeats the last d bytes
+                    at += len;
                 }
-                if (out->size() > expected)
-                    return fail(err, "too much image data");
             }
         }
         else
             return fail(err, "bad deflate block type");
         if (in.overrun)
             return fail(err, "compressed data ends early");
-        if (out->size() > expected)
-            return fail(err, "too much image data");
     }
-    if (out->size() != expected)
+    if (at != expected)
         return fail(err, "not enough image data");
     in.align();
     uint32_t adler = 0;
@@ -274,11 +282,18 @@ bool zlib_inflate(const uint8_t* src, size_t n, size_t expected, std::vector<uin
         adler = adler << 8 | in.bits(8);
     if (in.overrun)
         return fail(err, "missing Adler-32");
+    // Adler-32 in runs of 5552 bytes, the longest whose sums cannot overflow
     uint32_t a = 1, b = 0;
-    for (size_t i = 0; i < out->size(); ++i)
+    for (size_t i = 0; i < expected;)
     {
-        a = (a + (*out)[i]) % 65521u;
-        b = (b + a) % 65521u;
+        const size_t end = std::min(expected, i + 5552);
+        for (; i < end; ++i)
+        {
+            a += o[i];
+            b += a;
+        }
+        a %= 65521u;
+        b %= 65521u;
     }
     if ((b << 16 | a) != adler)
         return fail(err, "Adler-32 mismatch");
