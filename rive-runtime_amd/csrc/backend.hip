@@ -698,6 +698,20 @@ int decode_png_level0(rive_hip_context* ctx, const std::vector<uint8_t>& rgba, r
 // DCT into a sample plane, then colour conversion into the texels.
 int decode_jpeg_level0(rive_hip_context* ctx, const rivehip::image::Jpeg& j, rive_hip_image* img, DecodeTemps& t)
 {
+    static std::once_flag basisOnce[16];
+    static hipError_t basisErr[16];
+    const int dev = std::min(std::max(ctx->device, 0), 15);
+    std::call_once(basisOnce[dev], [&] {
+        double basis[24][8];
+        for (int r = 0; r < 24; ++r)
+        {
+            const int n = r < 8 ? 8 : 16, x = r < 8 ? r : r - 8;
+            for (int u = 0; u < 8; ++u)
+                basis[r][u] = (u == 0 ? 0.70710678118654752440 : 1.0) * std::cos((2 * x + 1) * u * M_PI / (2 * n));
+        }
+        basisErr[dev] = hipMemcpyToSymbol(HIP_SYMBOL(kIdctBasis), basis, sizeof(basis));
+    });
+    HIP_TRY(basisErr[dev]);
     int16_t* coef = t.alloc<int16_t>(j.coef.size() * 2);
     uint16_t* qt = t.alloc<uint16_t>(sizeof(j.qt));
     if (coef == nullptr || qt == nullptr)

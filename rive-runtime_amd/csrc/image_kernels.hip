@@ -153,26 +153,32 @@ __device__ void idct_islow(const int32_t* in, uint8_t* out, uint32_t stride)
     }
 }
 
+// C(u) cos((2x+1) u pi / 2n) for n = 8 (rows 0..7) and n = 16 (rows 8..23),
+// x = row index within its block, u = column; uploaded once per device.
+__constant__ double kIdctBasis[24][8];
+
 // The block's cosine series at (8*scaleH) x (8*scaleV) sample positions.
 __device__ void idct_scaled(const int32_t* in, uint8_t* out, uint32_t stride, int nh, int nv)
 {
+    const double(*bh)[8] = nh == 16 ? kIdctBasis + 8 : kIdctBasis;
+    const double(*bv)[8] = nv == 16 ? kIdctBasis + 8 : kIdctBasis;
     double tmp[8][16];
     for (int v = 0; v < 8; ++v)
         for (int x = 0; x < nh; ++x)
         {
             double s = 0.0;
+#pragma unroll
             for (int u = 0; u < 8; ++u)
-                s += (u == 0 ? 0.70710678118654752440 : 1.0) * (double)in[v * 8 + u] *
-                     cospi((double)((2 * x + 1) * u) / (double)(2 * nh));
+                s += bh[x][u] * (double)in[v * 8 + u];
             tmp[v][x] = s;
         }
     for (int y = 0; y < nv; ++y)
         for (int x = 0; x < nh; ++x)
         {
             double s = 0.0;
+#pragma unroll
             for (int v = 0; v < 8; ++v)
-                s += (v == 0 ? 0.70710678118654752440 : 1.0) * tmp[v][x] *
-                     cospi((double)((2 * y + 1) * v) / (double)(2 * nv));
+                s += bv[y][v] * tmp[v][x];
             out[(size_t)y * stride + x] = (uint8_t)fmin(fmax(floor(0.25 * s + 128.5), 0.0), 255.0);
         }
 }

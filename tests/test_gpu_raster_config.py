@@ -107,3 +107,33 @@ def test_raster_config_rejects_bad_values(ctx):
         with pytest.raises(RuntimeError):
             ctx.set_raster_config(*bad)
     assert ctx.raster_config() == {"tile_order": 0, "persistent": 0, "waves_per_simd": 6}
+
+
+def test_graph_replays_with_a_persistent_ordered_raster(ctx):
+    # The heaviest-first order and the persistent queue inside a captured
+    # frame: the queue head and class counters are cleared by fill kernels in
+    # the graph, so every replay takes every tile once and equals the eager
+    # frame bit for bit.
+    print("[raster-config] frame graph with tile order + persistent raster", flush=True)
+    s = rivehip.generate_stream(2, width=960, height=540, draw_count=1500)
+    t = ctx.target(960, 540)
+    g = None
+    try:
+        ctx.set_raster_config(1, 1, 6)
+        ctx.render_stream(s, t)
+        frame = ctx.last_frame()
+        ctx.flush_frame(frame, t)
+        ctx.sync()
+        eager = ctx.read_pixels(t, 960, 540)
+        g = ctx.frame_graph(frame, t)
+        for _ in range(3):
+            ctx.render_stream(s, t)  # eager work between replays
+            ctx.sync()
+            ctx.frame_graph_launch(g)
+            ctx.sync()
+            assert np.array_equal(ctx.read_pixels(t, 960, 540), eager)
+    finally:
+        if g is not None:
+            ctx.frame_graph_destroy(g)
+        ctx.free_target(t)
+        rivehip.free_stream(s)
