@@ -54,6 +54,10 @@ int oracle_select(oracle_flush* of, uint32_t index)
 
 void oracle_free(oracle_flush* of) { delete of; }
 
+// Interior triangulation on or off for the frames built from here on
+// (off by default, the reference's frameBudgetMs = 0).
+void oracle_set_interior_triangulation(int enabled) { set_interior_triangulation(enabled != 0); }
+
 // what: 0 spans, 1 contours, 2 PathData, 3 PaintData, 4 PaintAux, 5 grad spans,
 // 6 records (CubicRecord), 7 draws (DrawInfo as 5 u32), 8 info (u32 x 10)
 size_t oracle_get(oracle_flush* of, int what, void* dst, size_t cap)

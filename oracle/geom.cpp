@@ -545,9 +545,228 @@ struct Writer
 };
 } // namespace
 
+namespace
+{
+bool g_interiorTriangulation = false;
+
+// gpu::find_transformed_area (gpu.cpp:1160-1173): the area of the path's
+// local bounds mapped to the screen.
+float transformed_area(const PathDrawDesc& d)
+{
+    if (d.ptCount == 0)
+        return 0;
+    float l = d.pts[0].x, t = d.pts[0].y, r = l, b = t;
+    for (uint32_t i = 1; i < d.ptCount; ++i)
+    {
+        l = std::min(l, d.pts[i].x), r = std::max(r, d.pts[i].x);
+        t = std::min(t, d.pts[i].y), b = std::max(b, d.pts[i].y);
+    }
+    const float* m = d.matrix.m;
+    auto map = [m](float x, float y) { return V2{m[0] * x + m[2] * y + m[4], m[1] * x + m[3] * y + m[5]}; };
+    const V2 q[4] = {map(l, t), map(r, t), map(r, b), map(l, b)};
+    const V2 v[3] = {q[1] - q[0], q[2] - q[0], q[3] - q[0]};
+    return (fabsf(cross(v[0], v[1])) + fabsf(cross(v[1], v[2]))) * .5f;
+}
+
+// iterateOuterCubics (draw.cpp:2401-2650) for an interior-triangulated fill:
+// every line and every cubic (chopped uniformly in T into pieces of at most
+// 16 parametric segments) becomes one outer-cubic patch of 16 segments plus a
+// bowtie join, flagged to cull the segments it does not need; the gaps
+// between a chopped cubic's chord and its pieces' chords are filled by
+// triangle strips retrofitted into patches.  The triangulation of the
+// interior itself (GrInnerFanTriangulator) is not restated: it adds no
+// tessellation besides its grout patches, which are left out (none here).
+PathTess tessellate_outer_cubic_records(const PathDrawDesc& d, uint32_t firstContourID, uint32_t tessLocation)
+{
+    constexpr uint32_t kSpan = 16, kSpanPlusJoin = 17;
+    constexpr uint32_t kMaxSubdivisions = (1023 + kSpan - 1) / kSpan;
+    PathTess result;
+    auto subdivisions = [&](const V2* p) {
+        float n = ceilf(wang_cubic(p, kParametricPrecision, d.matrix) * (1.f / kSpan));
+        return (uint32_t)sclamp(n, 1, (float)kMaxSubdivisions);
+    };
+    // counting pass
+    uint32_t patchCount = 0, contourCount = 0;
+    {
+        V2 p0{0, 0};
+        const V2* pts = d.pts;
+        for (uint32_t v = 0; v < d.verbCount; ++v)
+        {
+            switch (d.verbs[v])
+            {
+                case 0: // move
+                    if (contourCount != 0 && !(pts[-1] == p0))
+                        ++patchCount;
+                    p0 = pts[0];
+                    ++contourCount;
+                    break;
+                case 1:
+                    ++patchCount;
+                    break;
+                case 4:
+                {
+                    const V2 c[4] = {pts[-1], pts[0], pts[1], pts[2]};
+                    const uint32_t n = subdivisions(c);
+                    patchCount += n + (n - 1 + 2) / 3;
+                    break;
+                }
+                default:
+                    break;
+            }
+            pts += pts_advance(d.verbs[v]);
+        }
+        if (contourCount != 0 && !(d.pts[d.ptCount - 1] == p0))
+            ++patchCount;
+    }
+    if (patchCount == 0)
+        return result;
+    result.tessVertexCount = patchCount * kSpanPlusJoin;
+    result.midpointFanTessVertexCount = double_sided(d.dirs) ? result.tessVertexCount * 2 : result.tessVertexCount;
+    result.maxTessellatedSegmentCount = patchCount;
+
+    Writer w;
+    w.dirs = d.dirs;
+    w.out = &result.records;
+    const uint32_t count = result.midpointFanTessVertexCount;
+    switch (d.dirs)
+    {
+        case Dirs::forward:
+            w.pathTessLocation = tessLocation;
+            break;
+        case Dirs::reverse:
+            w.pathMirroredTessLocation = tessLocation + count;
+            break;
+        case Dirs::reverseThenForward:
+            w.pathTessLocation = w.pathMirroredTessLocation = tessLocation + count / 2;
+            break;
+        case Dirs::forwardThenReverse:
+            w.pathTessLocation = tessLocation;
+            w.pathMirroredTessLocation = tessLocation + count;
+            break;
+    }
+    uint32_t contourID = firstContourID;
+    uint32_t cf = 0;
+    auto pushContour = [&]() {
+        ContourData cd;
+        cd.midpoint = V2{0, 0};
+        cd.pathID = d.pathID;
+        cd.vertexIndex0 = w.nextVertexIndex();
+        result.contours.push_back(cd);
+        result.contourFirstRecord.push_back((uint32_t)result.records.size());
+        result.contourPadding.push_back(0);
+        cf = d.contourFlags | contourID++;
+    };
+    auto pushPatch = [&](const V2* p) {
+        w.pushCubic(p, V2{0, 0}, kSpan, 1, 1, cf | CULL_EXCESS_TESSELLATION_SEGMENTS_CONTOUR_FLAG);
+    };
+    // pushRetrofitCubicTriStrip (render_context.cpp:3231-3257)
+    auto pushStrip = [&](const V2* p, size_t n) {
+        const V2 strip[4] = {p[0], p[1], p[std::min<size_t>(3, n - 1)], p[2]};
+        w.pushCubic(strip, p[std::min<size_t>(4, n - 1)], kSpan, 1, 1, cf | RETROFIT_TRI_STRIP_CONTOUR_FLAG);
+    };
+    V2 p0{0, 0};
+    uint32_t contours = 0;
+    const V2* pts = d.pts;
+    for (uint32_t v = 0; v < d.verbCount; ++v)
+    {
+        switch (d.verbs[v])
+        {
+            case 0:
+                if (contours != 0 && !(pts[-1] == p0))
+                {
+                    const auto c = line_to_cubic(pts[-1], p0);
+                    pushPatch(c.data());
+                }
+                pushContour();
+                p0 = pts[0];
+                ++contours;
+                break;
+            case 1:
+            {
+                const auto c = line_to_cubic(pts[-1], pts[0]);
+                pushPatch(c.data());
+                break;
+            }
+            case 4:
+            {
+                const V2 c[4] = {pts[-1], pts[0], pts[1], pts[2]};
+                const uint32_t n = subdivisions(c);
+                if (n == 1)
+                {
+                    pushPatch(c);
+                    break;
+                }
+                // chop uniformly in T (bezier_utils.cpp:104-158, null tValues)
+                std::vector<V2> chops(3 * n + 1);
+                {
+                    const V2* src = c;
+                    V2* dst = chops.data();
+                    int i = 0;
+                    const int tCount = (int)n - 1;
+                    for (; i < tCount - 1; i += 2)
+                    {
+                        const float k = (float)(tCount + 1 - i);
+                        V2 tmp[10];
+                        chop_cubic_at2(src, tmp, 1.f / k, 2.f / k);
+                        memcpy(dst, tmp, sizeof(tmp));
+                        src = dst = dst + 6;
+                    }
+                    if (i < tCount)
+                    {
+                        V2 tmp[7];
+                        chop_cubic_at(src, tmp, .5f);
+                        memcpy(dst, tmp, sizeof(tmp));
+                    }
+                }
+                for (uint32_t i = 0; i < n; ++i)
+                    pushPatch(chops.data() + 3 * i);
+                for (uint32_t a = 1; a < n; a += 3)
+                {
+                    const uint32_t remaining = n - a;
+                    const uint32_t b = remaining == 1 ? a : a + 1;
+                    V2 strip[5];
+                    size_t m = 0;
+                    strip[m++] = chops[b * 3];
+                    strip[m++] = chops[(b + 1) * 3];
+                    if (remaining > 1)
+                        strip[m++] = chops[(b - 1) * 3];
+                    if (remaining > 2)
+                        strip[m++] = chops[(b + 2) * 3];
+                    strip[m++] = chops[0];
+                    pushStrip(strip, m);
+                }
+                break;
+            }
+            default:
+                break;
+        }
+        pts += pts_advance(d.verbs[v]);
+    }
+    if (contours != 0 && !(d.pts[d.ptCount - 1] == p0))
+    {
+        const auto c = line_to_cubic(d.pts[d.ptCount - 1], p0);
+        pushPatch(c.data());
+    }
+    return result;
+}
+} // namespace
+
+void set_interior_triangulation(bool enabled) { g_interiorTriangulation = enabled; }
+
+bool uses_interior_triangulation(const PathDrawDesc& d)
+{
+    // PathDraw::Make (draw.cpp:510-538) with TriangulationController's
+    // guards alone (frameBudgetMs = infinity): a plain fill of at least
+    // 512 x 512 px on screen and at most 256 verbs.
+    return g_interiorTriangulation && d.strokeRadius == 0 && d.featherRadius == 0 && d.verbCount <= 256 &&
+           transformed_area(d) >= 512.f * 512.f;
+}
+
 PathTess tessellate_path_records(const PathDrawDesc& d, uint32_t firstContourID,
                                  uint32_t tessLocation)
 {
+    if (uses_interior_triangulation(d))
+        return tessellate_outer_cubic_records(d, firstContourID, tessLocation);
     PathTess result;
     const bool isStroke = d.strokeRadius > 0;
     // Feathered fills are tessellated like strokes, with feather joins

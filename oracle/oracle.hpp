@@ -204,6 +204,12 @@ struct PathTess
     uint32_t maxTessellatedSegmentCount = 0;
 };
 
+// Interior triangulation (draw.cpp:510-538): off by default (the reference's
+// frameBudgetMs = 0); when on, eligible fills are laid out as outer-cubic
+// patches (iterateOuterCubics) by tessellate_path_records.
+void set_interior_triangulation(bool enabled);
+bool uses_interior_triangulation(const PathDrawDesc& d);
+
 // Derive contourDirections / contourFlags the way PathDraw's constructor does
 // for the rasterOrdering interlock mode (draw.cpp:651-719).
 void derive_directions(PathDrawDesc* d);

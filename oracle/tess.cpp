@@ -88,7 +88,26 @@ void tessellate_span(const TessVertexSpan& s, const ContourData* contours,
         }
     }
     const uint32_t segmentCounts = s.segmentCounts;
-    const uint32_t parametricSegmentCount = segmentCounts & 0x3ffu;
+    uint32_t parametricSegmentCount = segmentCounts & 0x3ffu;
+    if ((s.contourIDWithFlags & CULL_EXCESS_TESSELLATION_SEGMENTS_CONTOUR_FLAG) != 0u)
+    {
+        // An outer-cubic patch has a fixed span: Wang's formula again on the
+        // path's matrix, the excess segments collapsed to T = 0
+        // (tessellate.glsl:195-211).
+        const uint32_t contourID = s.contourIDWithFlags & CONTOUR_ID_MASK;
+        const uint32_t pathID = contourID > 0u && contours != nullptr ? contours[contourID - 1].pathID : 0u;
+        const float* m = pathRecords != nullptr ? pathRecords + pathID * 16 : nullptr;
+        if (m != nullptr)
+        {
+            const F2 a{(-2.f * P1.x + P2.x) + P0.x, (-2.f * P1.y + P2.y) + P0.y};
+            const F2 b{(-2.f * P2.x + P3.x) + P1.x, (-2.f * P2.y + P3.y) + P1.y};
+            const F2 d0{m[0] * a.x + m[2] * a.y, m[1] * a.x + m[3] * a.y};
+            const F2 d1{m[0] * b.x + m[2] * b.y, m[1] * b.x + m[3] * b.y};
+            const float mm = std::max(d0.x * d0.x + d0.y * d0.y, d1.x * d1.x + d1.y * d1.y);
+            const float n = std::max(ceilf(sqrtf(.75f * 4.f * sqrtf(mm))), 1.f);
+            parametricSegmentCount = std::min((uint32_t)n, parametricSegmentCount);
+        }
+    }
     const uint32_t polarSegmentCount = (segmentCounts >> 10) & 0x3ffu;
     const uint32_t joinSegmentCount = segmentCounts >> 20;
     const uint32_t totalVertexCount =

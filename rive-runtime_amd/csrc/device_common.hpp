@@ -15,6 +15,7 @@ constexpr uint32_t kGradWidth = 512;
 
 constexpr uint32_t CONTOUR_ID_MASK = 0xffffu;
 constexpr uint32_t RETROFIT_TRI_STRIP_CONTOUR_FLAG = 1u << 31;
+constexpr uint32_t CULL_EXCESS_TESSELLATION_SEGMENTS_CONTOUR_FLAG = 1u << 29;
 constexpr uint32_t JOIN_TYPE_MASK = 7u << 26;
 constexpr uint32_t MITER_CLIP_JOIN_CONTOUR_FLAG = 5u << 26;
 constexpr uint32_t MITER_REVERT_JOIN_CONTOUR_FLAG = 4u << 26;

@@ -325,7 +325,23 @@ __global__ void tessellate_kernel(const SpanV* __restrict__ spans, uint32_t span
                 soften_feathered_cubic(P0, &P1, &P2, P3, featherRadius, featherTables + kFeatherTableSize);
         }
     }
-    const uint32_t parametricSegmentCount = s.counts & 0x3ffu;
+    uint32_t parametricSegmentCount = s.counts & 0x3ffu;
+    if ((s.flags & CULL_EXCESS_TESSELLATION_SEGMENTS_CONTOUR_FLAG) != 0u)
+    {
+        // An outer-cubic patch has a fixed span: Wang's formula again on the
+        // path's matrix, the excess segments collapsed to T = 0
+        // (tessellate.glsl:195-211).
+        const uint32_t contourID = s.flags & CONTOUR_ID_MASK;
+        const uint32_t pathID = contourID != 0u && contourID <= contourCount ? contours[contourID - 1].z : 0u;
+        const float* m = pathData + (size_t)(pathID <= pathCount ? pathID : 0u) * 16;
+        const F2 a{(-2.f * P1.x + P2.x) + P0.x, (-2.f * P1.y + P2.y) + P0.y};
+        const F2 b{(-2.f * P2.x + P3.x) + P1.x, (-2.f * P2.y + P3.y) + P1.y};
+        const F2 d0{m[0] * a.x + m[2] * a.y, m[1] * a.x + m[3] * a.y};
+        const F2 d1{m[0] * b.x + m[2] * b.y, m[1] * b.x + m[3] * b.y};
+        const float mm = fmaxf(d0.x * d0.x + d0.y * d0.y, d1.x * d1.x + d1.y * d1.y);
+        const float n = fmaxf(ceilf(sqrtf(.75f * 4.f * sqrtf(mm))), 1.f);
+        parametricSegmentCount = min((uint32_t)n, parametricSegmentCount);
+    }
     const uint32_t polarSegmentCount = (s.counts >> 10) & 0x3ffu;
     const uint32_t joinSegmentCount = s.counts >> 20;
     const uint32_t totalVertexCount = parametricSegmentCount + polarSegmentCount + joinSegmentCount - 1u;
@@ -1223,10 +1239,38 @@ __global__ void expand_kernel(const TessTexel* __restrict__ tex, int texels, con
     const PathDataV pd = paths[d.pathID];
     // next(i): the next vertex of the contour, wrapping closed contours to
     // vertexIndex0 (draw_path_common.glsl:336-388).
-    int j = reversed ? i - 1 : i + 1;
+    const int step = reversed ? -1 : 1;
+    int j = i + step;
     uint32_t cfj = tex[j].w;
-    if ((cfj & (MIRRORED_CONTOUR_CONTOUR_FLAG | 0xffffu)) !=
-        (cfi & (MIRRORED_CONTOUR_CONTOUR_FLAG | 0xffffu)))
+    auto sameContour = [cfi](uint32_t w) {
+        return (w & (MIRRORED_CONTOUR_CONTOUR_FLAG | 0xffffu)) == (cfi & (MIRRORED_CONTOUR_CONTOUR_FLAG | 0xffffu));
+    };
+    // An interior-triangulated fill's outline is its outer-cubic patches; the
+    // triangle strips retrofitted into its patches (the glue between a
+    // chopped cubic's chord and its pieces', draw.cpp:2510-2545) lie inside
+    // and are not part of it: their texels give no segment, and a segment
+    // reaching one continues to the next outline texel of the contour.
+    bool wrap;
+    if ((cfi & RETROFIT_TRI_STRIP_CONTOUR_FLAG) != 0u)
+    {
+        j = i;
+        cfj = cfi;
+        wrap = false;
+    }
+    else
+    {
+        if ((cfj & RETROFIT_TRI_STRIP_CONTOUR_FLAG) != 0u && sameContour(cfj))
+        {
+            int k = j;
+            while (k + step >= 0 && k + step < texels && sameContour(tex[k].w) &&
+                   (tex[k].w & RETROFIT_TRI_STRIP_CONTOUR_FLAG) != 0u)
+                k += step;
+            j = k;
+            cfj = tex[j].w;
+        }
+        wrap = !sameContour(cfj) || (cfj & RETROFIT_TRI_STRIP_CONTOUR_FLAG) != 0u;
+    }
+    if (wrap)
     {
         bool isClosed = pd.strokeRadius == 0.f || cd.mx != 0.f;
         j = isClosed ? min((int)cd.vertexIndex0, texels - 1) : i;

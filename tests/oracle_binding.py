@@ -24,6 +24,8 @@ def lib():
         L.oracle_build.argtypes = [ctypes.POINTER(rivehip.Stream)]
         L.oracle_free.argtypes = [vp]
         L.oracle_free.restype = None
+        L.oracle_set_interior_triangulation.argtypes = [ctypes.c_int]
+        L.oracle_set_interior_triangulation.restype = None
         L.oracle_get.restype = ctypes.c_size_t
         L.oracle_get.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t]
         L.oracle_tessellate.restype = ctypes.c_size_t
@@ -82,6 +84,12 @@ def lib():
 
 RECORD_DTYPES = {0: rivehip.TESS_SPAN, 1: rivehip.CONTOUR, 2: rivehip.PATH_DATA,
                  3: rivehip.PAINT_DATA, 4: rivehip.PAINT_AUX, 5: rivehip.GRAD_SPAN}
+
+
+def set_interior_triangulation(enabled):
+    """Lay eligible fills out as outer-cubic patches (draw.cpp:510-538, the
+    triangulation controller's guards alone) in frames built from now on."""
+    lib().oracle_set_interior_triangulation(1 if enabled else 0)
 
 
 class OracleFlush:
