@@ -122,7 +122,9 @@ typedef struct rive_hip_draw
     uint32_t join;             /* 0 miter, 1 round, 2 bevel */
     uint32_t cap;              /* 0 butt, 1 round, 2 square */
     uint32_t is_stroke;        /* 0 fill, 1 stroke, 2 feathered fill (counted like a
-                                  stroke whose joins are feather joins) */
+                                  stroke whose joins are feather joins), 3 fill laid
+                                  out for interior triangulation (outer-cubic
+                                  patches, draw.cpp:2401-2650) */
     float matrix[6];
     float stroke_radius;
     float matrix_max_scale;
@@ -469,6 +471,16 @@ typedef struct rive_hip_raster_config
 
 int rive_hip_set_raster_config(rive_hip_context* ctx, const rive_hip_raster_config* config);
 int rive_hip_get_raster_config(rive_hip_context* ctx, rive_hip_raster_config* out);
+
+/* Interior triangulation for frames the context's front end lays out
+ * (RenderContext::FrameDescriptor::triangulationThresholds,
+ * render_context.hpp:114, and PathDraw::Make, draw.cpp:510-538): off (0,
+ * the reference's frameBudgetMs = 0) by default; on, a plain fill of at
+ * least 512 x 512 px on screen and at most 256 verbs becomes outer-cubic
+ * patches (rive_hip_draw.is_stroke = 3), decided by those guards alone.
+ * The getter returns the setting (0 for a null context). */
+int rive_hip_set_interior_triangulation(rive_hip_context* ctx, uint32_t enabled);
+uint32_t rive_hip_get_interior_triangulation(const rive_hip_context* ctx);
 
 /* The raster autotuner (no counterpart in the reference).  Runs a frame's
  * logical flushes (descs, as for rive_hip_flush_frame) once untimed and then

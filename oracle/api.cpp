@@ -81,8 +81,9 @@ size_t oracle_get(oracle_flush* of, int what, void* dst, size_t cap)
                 v.push_back(d.tessLocation);
                 v.push_back(d.tessVertexCount);
                 v.push_back((uint32_t)d.dirs);
-                // rive_hip_draw.is_stroke: 0 fill, 1 stroke, 2 feathered fill
-                v.push_back(d.isStroke ? 1u : (d.featheredFill ? 2u : 0u));
+                // rive_hip_draw.is_stroke: 0 fill, 1 stroke, 2 feathered fill,
+                // 3 fill laid out as outer-cubic patches
+                v.push_back(d.isStroke ? 1u : (d.featheredFill ? 2u : (d.outerCubics ? 3u : 0u)));
             }
             return copy_out(v, dst, cap);
         }

@@ -18,6 +18,7 @@ struct DrawInfo
     Dirs dirs;
     bool isStroke;
     bool featheredFill = false;
+    bool outerCubics = false; // laid out as outer-cubic patches (interior triangulation)
     // A feather drawn through the atlas (draw.cpp:387-418): its coverage is
     // rendered into its own atlas region (aw x ah texels, atlas coordinate =
     // pixel * atlasScale + atlasTranslate) and blitted over the pixel bounds

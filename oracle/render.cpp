@@ -572,6 +572,7 @@ struct Builder
         di.dirs = d.dirs;
         di.isStroke = strokeRadius > 0;
         di.featheredFill = strokeRadius == 0 && featherRadius > 0;
+        di.outerCubics = uses_interior_triangulation(d);
         if (atlas)
         {
             di.atlas = true;

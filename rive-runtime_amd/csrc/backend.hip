@@ -120,6 +120,7 @@ struct rive_hip_context
     // class histogram, class cursors and tile queue head (2 * 256 + 1 words)
     DevBuf tileClass, tileOrder, tileQueue;
     rive_hip_raster_config rasterConfig = {0u, 0u, 6u, 0u};
+    bool interiorTriangulation = false; // the front end's layout of large fills
     int cuCount = 256;
     CanvasDev* hostCanvases = nullptr; // pinned staging of the canvas table
     uint32_t hostCanvasesCap = 0;
@@ -2444,6 +2445,19 @@ extern "C" int rive_hip_set_raster_config(rive_hip_context* ctx, const rive_hip_
                     "raster config: tile_order and persistent are 0 or 1, waves_per_simd 5 or 6, reserved 0");
     ctx->rasterConfig = *config;
     return RIVE_HIP_OK;
+}
+
+extern "C" int rive_hip_set_interior_triangulation(rive_hip_context* ctx, uint32_t enabled)
+{
+    if (ctx == nullptr)
+        return fail(RIVE_HIP_ERROR_INVALID_ARGUMENT, "null context");
+    ctx->interiorTriangulation = enabled != 0;
+    return RIVE_HIP_OK;
+}
+
+extern "C" uint32_t rive_hip_get_interior_triangulation(const rive_hip_context* ctx)
+{
+    return ctx != nullptr && ctx->interiorTriangulation ? 1u : 0u;
 }
 
 extern "C" int rive_hip_get_raster_config(rive_hip_context* ctx, rive_hip_raster_config* out)

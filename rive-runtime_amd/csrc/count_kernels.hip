@@ -96,6 +96,141 @@ __device__ __forceinline__ void push_cubic(Emit* e, const V2d* pts, V2d jt, uint
     ++e->rec;
 }
 
+// An interior-triangulated fill (is_stroke 3; iterateOuterCubics,
+// draw.cpp:2401-2650): per contour, every line and the closing line as one
+// outer-cubic patch (16 parametric segments, one polar, one bowtie join,
+// flagged to cull the segments it does not need), every cubic cut uniformly
+// in T into clamp(ceil(Wang / 16), 1, 64) pieces, one patch each, followed by
+// ceil((n - 1) / 3) triangle strips retrofitted into patches between the
+// cubic's chord and its pieces' chords.  Contours have no midpoint and no
+// padding.  The interior triangles are not part of the tessellation.
+template <bool EMIT>
+__device__ DrawCounts walk_outer_draw(const rive_hip_draw& d, const uint8_t* __restrict__ verbs,
+                                      const V2d* __restrict__ P, Emit* e)
+{
+    constexpr uint32_t kSpan = 16, kMaxSubdivisions = (1023 + kSpan - 1) / kSpan;
+    constexpr uint32_t kCull = CULL_EXCESS_TESSELLATION_SEGMENTS_CONTOUR_FLAG;
+    DrawCounts out{0, 0, 0, 0};
+    uint32_t cid = 0;
+    auto patch = [&](const V2d* p) {
+        ++out.records;
+        if (EMIT)
+            push_cubic(e, p, V2d{0.f, 0.f}, kSpan, 1, 1, cid | kCull);
+    };
+    auto line = [&](V2d a, V2d b) {
+        // convert_line_to_cubic (draw.cpp:110-121)
+        const float third = 1.f / 3.f;
+        const V2d c[4] = {a, (b - a) * third + a, (a - b) * third + b, b};
+        patch(c);
+    };
+    auto same = [](V2d a, V2d b) { return a.x == b.x && a.y == b.y; };
+    V2d p0{0.f, 0.f};
+    uint32_t contours = 0, pi = 0;
+    for (uint32_t k = 0; k < d.verb_count; ++k)
+    {
+        const uint8_t v = verbs[k];
+        if (v == 0)
+        {
+            if (contours != 0 && !same(P[pi - 1], p0))
+                line(P[pi - 1], p0);
+            ++out.contours;
+            if (EMIT)
+            {
+                ContourV cdat;
+                cdat.mx = 0.f;
+                cdat.my = 0.f;
+                cdat.pathID = d.path_id;
+                cdat.vertexIndex0 = e->dirs != 1u ? e->fwd : e->mir - 1u;
+                if (e->contourIndex < e->contourCap)
+                    e->contours[e->contourIndex] = cdat;
+                else
+                    atomicOr(e->overflow, 1u);
+                ++e->contourIndex;
+                cid = d.contour_flags | e->contourID;
+                ++e->contourID;
+            }
+            p0 = P[pi];
+            ++contours;
+            pi += 1;
+        }
+        else if (v == 1)
+        {
+            line(P[pi - 1], P[pi]);
+            pi += 1;
+        }
+        else if (v == 4)
+        {
+            const V2d* c = P + pi - 1;
+            const float w = sqrtf(sqrtf(wang_pow4(c, 4.f, d.matrix)));
+            const uint32_t n = (uint32_t)sclamp(ceilf(w * (1.f / kSpan)), 1.f, (float)kMaxSubdivisions);
+            if (n == 1)
+                patch(c);
+            else
+            {
+                out.records += n - 1 + (n + 1) / 3;
+                if (EMIT)
+                {
+                    // chop uniformly in T, two chops at a time
+                    // (bezier_utils.cpp:104-158 with null tValues)
+                    V2d piece[4] = {c[0], c[1], c[2], c[3]}, start = c[0];
+                    V2d ends[kMaxSubdivisions + 1];
+                    ends[0] = c[0];
+                    const int tCount = (int)n - 1;
+                    int i = 0, made = 0;
+                    for (; i < tCount - 1; i += 2)
+                    {
+                        V2d ch[10];
+                        const float kk = (float)(tCount + 1 - i);
+                        chop_at2(piece, ch, 1.f / kk, 2.f / kk);
+                        push_cubic(e, ch, V2d{0.f, 0.f}, kSpan, 1, 1, cid | kCull);
+                        push_cubic(e, ch + 3, V2d{0.f, 0.f}, kSpan, 1, 1, cid | kCull);
+                        ends[++made] = ch[3];
+                        ends[++made] = ch[6];
+                        piece[0] = ch[6], piece[1] = ch[7], piece[2] = ch[8], piece[3] = ch[9];
+                    }
+                    if (i < tCount)
+                    {
+                        V2d ch[7];
+                        chop_at(piece, ch, .5f);
+                        push_cubic(e, ch, V2d{0.f, 0.f}, kSpan, 1, 1, cid | kCull);
+                        ends[++made] = ch[3];
+                        piece[0] = ch[3], piece[1] = ch[4], piece[2] = ch[5], piece[3] = ch[6];
+                    }
+                    push_cubic(e, piece, V2d{0.f, 0.f}, kSpan, 1, 1, cid | kCull);
+                    ends[++made] = piece[3];
+                    // the fan of glue strips anchored at the cubic's start
+                    for (uint32_t a = 1; a < n; a += 3)
+                    {
+                        const uint32_t remaining = n - a;
+                        const uint32_t b = remaining == 1 ? a : a + 1;
+                        V2d strip[5];
+                        uint32_t m = 0;
+                        strip[m++] = ends[b];
+                        strip[m++] = ends[b + 1];
+                        if (remaining > 1)
+                            strip[m++] = ends[b - 1];
+                        if (remaining > 2)
+                            strip[m++] = ends[b + 2];
+                        strip[m++] = start;
+                        // pushRetrofitCubicTriStrip (render_context.cpp:3231-3257)
+                        const V2d q[4] = {strip[0], strip[1], strip[m < 4 ? m - 1 : 3], strip[2]};
+                        push_cubic(e, q, strip[m < 5 ? m - 1 : 4], kSpan, 1, 1,
+                                   cid | RETROFIT_TRI_STRIP_CONTOUR_FLAG);
+                    }
+                }
+                ++out.records; // the last piece (the others are counted above)
+            }
+            pi += 3;
+        }
+        else if (v == 2)
+            pi += 2;
+    }
+    if (contours != 0 && d.point_count != 0 && !same(P[d.point_count - 1], p0))
+        line(P[d.point_count - 1], p0);
+    out.tessVertexCount = out.records * (kSpan + 1);
+    return out;
+}
+
 // Walks one draw.  EMIT = false counts; EMIT = true writes records.
 template <bool EMIT>
 __device__ DrawCounts walk_draw(const rive_hip_draw& d, const uint8_t* __restrict__ verbs,
@@ -546,7 +681,8 @@ __global__ void count_kernel(const rive_hip_draw* __restrict__ draws, uint32_t n
         recordCounts[i] = 0;
         return;
     }
-    DrawCounts c = walk_draw<false>(d, verbs + d.first_verb, points + d.first_point, nullptr);
+    DrawCounts c = d.is_stroke == 3u ? walk_outer_draw<false>(d, verbs + d.first_verb, points + d.first_point, nullptr)
+                                     : walk_draw<false>(d, verbs + d.first_verb, points + d.first_point, nullptr);
     // midpointFanTessVertexCount: doubled for double-sided draws (draw.cpp:1384-1390)
     const bool doubleSided = d.contour_directions == 2u || d.contour_directions == 3u;
     tessCounts[i] = doubleSided ? c.tessVertexCount * 2u : c.tessVertexCount;
@@ -602,7 +738,10 @@ __global__ void emit_kernel(rive_hip_draw* __restrict__ draws, uint32_t n, const
             e.mir = loc + count;
             break;
     }
-    walk_draw<true>(d, verbs + d.first_verb, points + d.first_point, &e);
+    if (d.is_stroke == 3u)
+        walk_outer_draw<true>(d, verbs + d.first_verb, points + d.first_point, &e);
+    else
+        walk_draw<true>(d, verbs + d.first_verb, points + d.first_point, &e);
     if (e.rec != e.recCap || e.contourIndex != e.contourCap)
         atomicOr(overflow, 1u);
     draws[i].tess_location = loc;

@@ -728,6 +728,28 @@ DrawRecord RenderContext::makeRecord(const RiveRenderPath* path, const Mat2D& m,
     // PathDraw's per-draw constants (draw.cpp:561-825)
     rive_hip_draw& d = r.draw;
     d.is_stroke = stroke ? 1 : (featherRadius != 0 ? 2 : 0);
+    // PathDraw::Make (draw.cpp:510-538): with interior triangulation on, a
+    // plain fill of at least 512 x 512 px on screen (find_transformed_area,
+    // gpu.cpp:1160-1173) and at most 256 verbs is laid out as outer-cubic
+    // patches (the triangulation controller's guards alone decide).
+    if (d.is_stroke == 0 && m_impl->interiorTriangulation() && path->getRawPath().verbs().size() <= 256 &&
+        !path->getRawPath().points().empty())
+    {
+        const auto& pts = path->getRawPath().points();
+        float l = pts[0].x, t = pts[0].y, rr = l, b = t;
+        for (const Vec2D& q : pts)
+        {
+            l = std::min(l, q.x), rr = std::max(rr, q.x);
+            t = std::min(t, q.y), b = std::max(b, q.y);
+        }
+        const float* mv = m.values();
+        auto map = [mv](float x, float y) { return Vec2D{mv[0] * x + mv[2] * y + mv[4], mv[1] * x + mv[3] * y + mv[5]}; };
+        const Vec2D q0 = map(l, t), q1 = map(rr, t), q2 = map(rr, b), q3 = map(l, b);
+        const Vec2D v0{q1.x - q0.x, q1.y - q0.y}, v1{q2.x - q0.x, q2.y - q0.y}, v2{q3.x - q0.x, q3.y - q0.y};
+        const float area = (fabsf(v0.x * v1.y - v0.y * v1.x) + fabsf(v1.x * v2.y - v1.y * v2.x)) * .5f;
+        if (area >= 512.f * 512.f)
+            d.is_stroke = 3;
+    }
     d.join = stroke ? (uint32_t)paint->getJoin() : 0;
     d.cap = stroke ? (uint32_t)paint->getCap() : 0;
     memcpy(d.matrix, m.values(), sizeof(d.matrix));

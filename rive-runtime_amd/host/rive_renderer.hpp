@@ -433,6 +433,10 @@ public:
     // (RenderContext::decodeImage's Bitmap::decode + makeImageTexture,
     // render_context.cpp:237-262); null when the backend cannot.
     virtual std::shared_ptr<RiveRenderImage> decodeImageTexture(const uint8_t*, size_t) { return nullptr; }
+    // Whether large plain fills are laid out for interior triangulation
+    // (FrameDescriptor::triangulationThresholds with a budget above zero);
+    // off by default (frameBudgetMs = 0).
+    virtual bool interiorTriangulation() const { return false; }
 };
 
 class RenderContext : public Factory
@@ -623,6 +627,7 @@ public:
             return nullptr;
         return std::make_shared<RiveRenderImage>(h, w, hgt, /*owned=*/true);
     }
+    bool interiorTriangulation() const override { return rive_hip_get_interior_triangulation(m_ctx) != 0; }
 
 private:
     bool upload(int kind, const void* data, size_t bytes);

@@ -147,6 +147,7 @@ class RasterConfig(ctypes.Structure):
 
 
 EXPORTED = [
+    "rive_hip_set_interior_triangulation", "rive_hip_get_interior_triangulation",
     "rive_hip_last_error", "rive_hip_create", "rive_hip_destroy", "rive_hip_buffer_resize",
     "rive_hip_buffer_map", "rive_hip_buffer_unmap", "rive_hip_target_create",
     "rive_hip_target_destroy", "rive_hip_flush", "rive_hip_sync", "rive_hip_read_pixels",
@@ -217,6 +218,9 @@ def lib():
     L.rive_hip_image_read.argtypes = [vp, u32, vp, ctypes.c_size_t, ctypes.POINTER(u32), ctypes.POINTER(u32)]
     L.rive_hip_set_raster_config.argtypes = [vp, ctypes.POINTER(RasterConfig)]
     L.rive_hip_get_raster_config.argtypes = [vp, ctypes.POINTER(RasterConfig)]
+    L.rive_hip_set_interior_triangulation.argtypes = [vp, ctypes.c_uint32]
+    L.rive_hip_get_interior_triangulation.argtypes = [vp]
+    L.rive_hip_get_interior_triangulation.restype = ctypes.c_uint32
     L.rive_hip_autotune_frame.argtypes = [vp, ctypes.POINTER(FlushDesc), u32, u32, ctypes.POINTER(RasterConfig),
                                           ctypes.POINTER(ctypes.c_float)]
     L.rive_hip_image_create.argtypes = [vp, u32, u32, u32, vp, ctypes.POINTER(vp)]
@@ -564,6 +568,14 @@ class Context:
         check(self.L.rive_hip_image_read(image, level, out.ctypes.data, out.nbytes, ctypes.byref(w),
                                          ctypes.byref(h)))
         return out
+
+    def set_interior_triangulation(self, enabled):
+        """Whether the front end lays large plain fills out as outer-cubic
+        patches (interior triangulation); off by default."""
+        check(self.L.rive_hip_set_interior_triangulation(self.h, 1 if enabled else 0))
+
+    def interior_triangulation(self):
+        return bool(self.L.rive_hip_get_interior_triangulation(self.h))
 
     def set_raster_config(self, tile_order=0, persistent=0, waves_per_simd=6):
         """How the raster pass runs (every choice renders the same pixels)."""
