@@ -114,3 +114,20 @@ def test_cubic_pieces_and_glue_strips():
         assert len(rec) == n + strips + 1 and (rec[-1]["cfi"] & CULL) != 0
     finally:
         f.close()
+
+
+@pytest.mark.parametrize("kw,code", [(dict(), 3), (dict(size=400.0), 0), (dict(stroke=4.0), 1),
+                                     (dict(feather=3.0), 2)])
+def test_draw_export_marks_outer_cubic_draws(kw, code):
+    """The draw table's is_stroke code (the front end's mark that the device
+    counts by) is 3 exactly for a draw laid out as outer-cubic patches."""
+    f, _, _ = layout(lambda sb: square(sb, **kw))
+    try:
+        assert f.draws()[:, 4].tolist() == [code]
+    finally:
+        f.close()
+    f, _, _ = layout(lambda sb: square(sb, **kw), triangulate=False)
+    try:
+        assert f.draws()[:, 4].tolist() == [0 if code == 3 else code]
+    finally:
+        f.close()
