@@ -256,6 +256,57 @@ def serving_leg(args):
             "raster_autotune": tune_report(tuned, tune_ms), "rates": out}
 
 
+def triangulation_leg(args):
+    """The 1080p 5000-path frame through the front end with interior
+    triangulation off (the default: midpoint fans) and on (large fills as
+    outer-cubic patches, laid out on the device): end-to-end frame time, the
+    device passes, the draws laid out as patches and the largest pixel
+    difference between the two frames."""
+    import numpy as np
+
+    s = rivehip.generate_stream(2, seed=SEED)
+    ctx = rivehip.Context(0)
+    t = ctx.target(W, H)
+    res = {}
+    frames = {}
+    try:
+        for on in (False, True):
+            ctx.set_interior_triangulation(on)
+            for _ in range(2):
+                ctx.render_stream(s, t)
+                ctx.sync()
+            ms = []
+            for _ in range(max(3, args.steps // 3)):
+                t0 = time.perf_counter()
+                ctx.render_stream(s, t)
+                ctx.sync()
+                ms.append((time.perf_counter() - t0) * 1e3)
+            passes = ctx.pass_times()
+            frames[on] = ctx.read_pixels(t, W, H).astype(np.int32)
+            leg = {"end_to_end_ms": round(statistics.median(ms), 3), "raster_ms": round(passes[4], 3),
+                   "device_ms": round(passes[5], 3)}
+            if on:
+                marked = 0
+                for k in range(ctx.render_stream(s, t)):
+                    ctx.render_stream(s, t, max_flushes=k + 1)
+                    ctx.sync()
+                    marked += int((ctx.readback(3).view(rivehip.DRAW)["is_stroke"] == 3).sum())
+                leg["outer_cubic_draws"] = marked
+            res["on" if on else "off"] = leg
+        ctx.set_interior_triangulation(False)
+    finally:
+        ctx.free_target(t)
+        rivehip.free_stream(s)
+        ctx.close()
+    res["max_diff_on_vs_off"] = int(np.abs(frames[True] - frames[False]).max())
+    res["note"] = ("end to end: front end, upload and flush of every logical flush; raster_ms and device_ms: "
+                   "HIP-event pass times summed over the frame's flushes; the two frames differ only by the two "
+                   "tessellations of the large fills' curves (each within 1/4 px)")
+    print(f"# triangulation: off {res['off']['end_to_end_ms']} ms, on {res['on']['end_to_end_ms']} ms",
+          file=sys.stderr, flush=True)
+    return res
+
+
 def batched_1080p_leg(args, plain_fps):
     """The 1080p 5000-path frame batched: 8 seeded frames of that workload
     (seeds k = 0..7 above the headline's) as 8 canvases of one flush, from
@@ -519,6 +570,7 @@ def main():
     total_s = sum(step_ms) / 1e3
     fps = args.steps / total_s
     batched1080 = None if args.no_batched else batched_1080p_leg(args, fps)
+    triangulation = None if args.no_batched else triangulation_leg(args)
     names = ["ramps", "count_spans", "tessellate", "bin", "raster", "total"]
     pass_ms = {n: round(statistics.mean(p[i] for p in passes), 4) for i, n in enumerate(names)}
     raster_s = pass_ms["raster"] / 1e3
@@ -572,6 +624,7 @@ def main():
         "frame_graph": graph,
         "batched": batched,
         "batched_1080p": batched1080,
+        "interior_triangulation_1080p": triangulation,
         "serving": serving,
         "cfg3": cfg3,
         "image_decode": images,
